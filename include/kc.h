@@ -1,0 +1,141 @@
+/*
+ * kc.h — C ABI of the MI355X (gfx950) k-mer spectrum engine.
+ *
+ * The engine replaces the first-stage k-mer path of spades-rs:
+ *   2-bit canonical k-mer extraction from packed reads   (dna.rs, packed_reads.rs)
+ *   two-tier Bloom filter insert / query                  (bloom.rs)
+ *   Hammer single-base correction against the filter      (hammer.rs)
+ *   exact count table, cutoff and solid set               (assemble.rs:122-298, 692-735)
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Host buffers are borrowed for the call.
+ *   - Every function returns KC_OK (0) or a negative KC_E* status; nothing
+ *     throws or aborts across the ABI.  kc_last_error() / kc_filter_last_error()
+ *     give a message for the last failure on that object.
+ *   - A k-mer is passed as the reference's Kmer256(lo: u128, hi: u128)
+ *     (dna.rs:137-138) in four little-endian 64-bit limbs, the limb order of
+ *     bloom.rs:62-65:  w[0] = lo bits 0..63, w[1] = lo bits 64..127,
+ *     w[2] = hi bits 0..63, w[3] = hi bits 64..127.  For k <= 64 the k-mer
+ *     occupies the low 2k bits of lo, first base most significant, hi = 0.
+ *   - One object owns one HIP stream on one device.  Calls on one object must be
+ *     serialised by the caller; separate objects may be used concurrently.
+ *   - Results are deterministic: identical inputs give identical outputs.  The
+ *     solid Bloom tier is the one the reference produces in its sequential
+ *     order (read 0..n, window 0..L-k, hashes h1, h2, h3), i.e. at `-t 1`.
+ */
+#ifndef KC_H
+#define KC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KC_OK 0
+#define KC_EINVAL (-1)       /* bad argument (k out of range, NULL pointer, size mismatch) */
+#define KC_ENOMEM (-2)       /* device allocation failed */
+#define KC_EHIP (-3)         /* HIP runtime error */
+#define KC_ESTATE (-4)       /* call out of order (e.g. export before run) */
+#define KC_EUNSUPPORTED (-5) /* valid in the reference but not supported by this build */
+
+#define KC_MAX_K 63u         /* this build: 1 <= k <= 63 (reference Kmer256 allows up to 128) */
+
+typedef struct kc_ctx kc_ctx;
+typedef struct kc_filter kc_filter;
+
+/* Options of one spectrum run; mirrors the AssemblerConfig fields the path reads
+ * (assemble.rs:35-103): k, bloom_bits, error_correct, min_coverage. */
+typedef struct kc_opts {
+    int32_t device;        /* HIP device ordinal */
+    uint32_t k;            /* k-mer size, 1..KC_MAX_K (CLI: odd, 11..127; main.rs:11-28) */
+    uint64_t bloom_bits;   /* capacity bits per tier (TwoTierFilter::new, bloom.rs:26-43) */
+    double min_coverage;   /* determine_solid_kmer_cutoff input (assemble.rs:692) */
+    int32_t error_correct; /* run Hammer before counting (assemble.rs:188-192) */
+    float table_load;      /* device hash-table target load factor, 0 = default (0.5) */
+    uint32_t flags;        /* KC_FLAG_* */
+} kc_opts;
+
+#define KC_FLAG_TIMING 1u  /* record per-kernel HIP events (kc_kernel_times) */
+
+/* Summary of the last run. */
+typedef struct kc_info {
+    uint64_t n_reads;          /* reads uploaded */
+    uint64_t n_occ;            /* k-mer occurrences (windows) = sum max(len-k+1, 0) */
+    uint64_t n_distinct;       /* distinct canonical k-mers over all windows */
+    uint64_t n_counts;         /* entries of the count table (Bloom-solid distinct k-mers) */
+    uint64_t n_solid;          /* entries with count >= cutoff (the solid set) */
+    uint64_t corrected_reads;  /* reads Hammer modified (error_correct only) */
+    uint64_t num_words;        /* 64-bit words per Bloom tier */
+    uint64_t table_capacity;   /* device hash-table slots used */
+    uint32_t cutoff;           /* determine_solid_kmer_cutoff result */
+    uint32_t top_cov;          /* 95th-percentile count used by the cutoff, capped at 1023 */
+    double ms_stage2;          /* device time of the Bloom build */
+    double ms_stage3;          /* device time of correction + counting + sort + cutoff */
+} kc_info;
+
+/* ---- Standalone two-tier filter (bloom.rs) ------------------------------- */
+
+/* TwoTierFilter::new(capacity_bits) (bloom.rs:26-43): max(ceil(bits/64), 1) words per tier. */
+int kc_filter_create(kc_filter** out, int32_t device, uint64_t capacity_bits);
+void kc_filter_destroy(kc_filter* f);
+const char* kc_filter_last_error(const kc_filter* f);
+/* TwoTierFilter::memory_usage_bytes (bloom.rs:129-131). */
+uint64_t kc_filter_memory_usage_bytes(const kc_filter* f);
+uint64_t kc_filter_num_words(const kc_filter* f);
+/* TwoTierFilter::insert_kmer256 (bloom.rs:83-107) applied to keys[0], keys[1], ... in that
+ * order, after every earlier call.  promoted (nullable) receives each call's return value. */
+int kc_filter_insert_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* promoted);
+/* TwoTierFilter::is_solid_kmer256 (bloom.rs:111-126) for each key. */
+int kc_filter_is_solid_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* out);
+/* Copy both tiers (num_words u64 each). */
+int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words);
+/* ErrorCorrector::new(k).correct_read_in_place (hammer.rs:28-120) for every read of a
+ * PackedReads-layout store against this filter; `out` receives the corrected store (same
+ * offsets/lengths), n_corrected the number of reads modified (hammer.rs:123-143).
+ * Reads of at most 1024 bases. */
+int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_t data_len, const uint64_t* offsets,
+                            const uint32_t* lengths, size_t n_reads, uint8_t* out, uint64_t* n_corrected);
+
+/* ---- Spectrum context (assemble.rs Stage 2 + Stage 3) -------------------- */
+
+int kc_create(kc_ctx** out, const kc_opts* opts);
+void kc_destroy(kc_ctx* ctx);
+const char* kc_last_error(const kc_ctx* ctx);
+
+/* Upload a PackedReads store (packed_reads.rs:42-59): data = 2-bit bases, 4 per byte,
+ * first base in bits 7..6, each read byte-aligned at offsets[i]; lengths[i] bases. */
+int kc_reads_upload(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets,
+                    const uint32_t* lengths, size_t n_reads);
+
+/* Stage 2: stream every window of every read into the two-tier filter (assemble.rs:129-161). */
+int kc_bloom_build(kc_ctx* ctx);
+/* Stage 3: optional Hammer (error_correct), exact counts of Bloom-solid k-mers, sort,
+ * cutoff and solid set (assemble.rs:166-292).  Requires kc_bloom_build. */
+int kc_count(kc_ctx* ctx, kc_info* info);
+/* Stage 2 + Stage 3. */
+int kc_run(kc_ctx* ctx, kc_info* info);
+int kc_info_get(const kc_ctx* ctx, kc_info* info);
+
+/* Both Bloom tiers after Stage 2 (num_words must equal kc_info.num_words). */
+int kc_bloom_export(kc_ctx* ctx, uint64_t* seen, uint64_t* solid, size_t num_words);
+/* is_solid_kmer256 (bloom.rs:111-126) against the context's filter. */
+int kc_bloom_query(kc_ctx* ctx, const uint64_t* keys4, size_t n, uint8_t* out);
+/* The count table sorted ascending by Kmer256 order: keys4 (n x 4 limbs), counts, and
+ * solid_mask[i] = counts[i] >= cutoff.  Any of the three may be NULL.  n must equal
+ * kc_info.n_counts. */
+int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask, size_t n);
+/* Packed reads as counted (after Hammer when error_correct), same offsets and lengths
+ * as uploaded; data_len must equal the uploaded data_len. */
+int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len);
+
+/* Per-kernel device time of the last run (KC_FLAG_TIMING): fills up to cap entries of
+ * names (static strings), total ms and launch counts; returns the number of kernels. */
+int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KC_H */

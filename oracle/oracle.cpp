@@ -1,0 +1,698 @@
+// oracle.cpp — CPU restatement of the spades-rs k-mer spectrum path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X engine:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// the library built from it.  Nothing in the product links it.
+//
+// It restates, in plain C++17, the reference Rust code listed below.  Each
+// function cites the reference file:line it follows.  Run with threads == 1
+// it is the sequential order of the reference at `-t 1` (read index 0..n,
+// window 0..L-k, hashes h1 -> h2 -> h3), which is the only well-defined
+// bit-exact target because the reference's solid Bloom tier depends on
+// insertion order under rayon (bloom.rs:95-103).  Run with threads > 1 it
+// follows the reference's rayon chunking and mutex-sharded count maps
+// (assemble.rs:136-242) and serves as the timed CPU baseline.
+//
+// Pinning: tests/test_oracle_*.py check every function here against the
+// reference's own known-answer tests (dna.rs:329-432, bloom.rs:138-156,
+// hammer.rs:150-192, tests/test_dna_kmer.rs, tests/test_hammer_and_bloom.rs,
+// tests/test_packed_reads.rs), against an independent NumPy transcription of
+// bloom.rs:61-126 (oracle/hash_np.py), and against closed forms / brute force
+// (seen tier = OR over all occurrences; counts = brute-force multiplicities;
+// cutoff on hand-built histograms).
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------
+// dna.rs
+// ---------------------------------------------------------------------------
+
+// dna.rs:11-22 — ASCII -> 2-bit code, 0xFF for anything that is not ACGTacgt.
+static uint8_t ascii_to_2bit(uint8_t b) {
+    switch (b) {
+        case 'A': case 'a': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return 0xFF;
+    }
+}
+
+// dna.rs:25,41-43
+static uint8_t bit2_to_base(uint8_t v) { return "ACGT"[v & 3]; }
+
+static uint64_t reverse_bits64(uint64_t x) {
+    uint64_t r = 0;
+    for (int i = 0; i < 64; ++i) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+static u128 reverse_bits128(u128 x) {
+    uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    return ((u128)reverse_bits64(lo) << 64) | (u128)reverse_bits64(hi);
+}
+
+// dna.rs:65-78 revcomp_kmer_u64
+static uint64_t revcomp_kmer_u64(uint64_t kmer, unsigned k) {
+    if (k == 0 || k > 32) return 0;
+    kmer = reverse_bits64(~kmer);
+    kmer = ((kmer >> 1) & 0x5555555555555555ULL) | ((kmer & 0x5555555555555555ULL) << 1);
+    unsigned sh = 2 * (32 - k);
+    return sh >= 64 ? 0 : (kmer >> sh);
+}
+
+// dna.rs:82-89 canonical_kmer_u64 (ties go to the forward strand)
+static uint64_t canonical_kmer_u64(uint64_t kmer, unsigned k, int* is_rc) {
+    uint64_t rc = revcomp_kmer_u64(kmer, k);
+    if (kmer <= rc) { *is_rc = 0; return kmer; }
+    *is_rc = 1;
+    return rc;
+}
+
+// dna.rs:118-132 revcomp_kmer_u128
+static u128 revcomp_kmer_u128(u128 kmer, unsigned k) {
+    if (k == 0 || k > 64) return 0;
+    const u128 m55 = ((u128)0x5555555555555555ULL << 64) | (u128)0x5555555555555555ULL;
+    kmer = reverse_bits128(~kmer);
+    kmer = ((kmer >> 1) & m55) | ((kmer & m55) << 1);
+    if (k == 64) return kmer;
+    return kmer >> (2 * (64 - k));
+}
+
+// dna.rs:137-138 — Kmer256(lo, hi); derived Ord is (lo, hi) lexicographic.
+struct Kmer256 {
+    u128 lo, hi;
+    bool operator==(const Kmer256& o) const { return lo == o.lo && hi == o.hi; }
+    bool operator<(const Kmer256& o) const { return lo < o.lo || (lo == o.lo && hi < o.hi); }
+    bool operator<=(const Kmer256& o) const { return !(o < *this); }
+};
+
+struct Kmer256Hash {
+    size_t operator()(const Kmer256& x) const {
+        uint64_t a = (uint64_t)x.lo, b = (uint64_t)(x.lo >> 64);
+        uint64_t c = (uint64_t)x.hi, d = (uint64_t)(x.hi >> 64);
+        uint64_t h = a * 0x9E3779B97F4A7C15ULL;
+        h ^= (b + 0x632BE59BD9B4E019ULL) * 0xC2B2AE3D27D4EB4FULL;
+        h ^= (c ^ (d << 1)) * 0x165667B19E3779F9ULL;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ULL;
+        h ^= h >> 29;
+        return (size_t)h;
+    }
+};
+
+// dna.rs:152-180 Kmer256::revcomp
+static Kmer256 kmer_revcomp(const Kmer256& x, unsigned k) {
+    if (k == 0 || k > 128) return Kmer256{0, 0};
+    if (k <= 64) return Kmer256{revcomp_kmer_u128(x.lo, k), 0};
+    unsigned k2 = k - 64;
+    u128 lo_rc = revcomp_kmer_u128(x.lo, 64);
+    u128 hi_rc = revcomp_kmer_u128(x.hi, k2);
+    unsigned shift_lo = 2 * (64 - k2);
+    u128 rc_lo = (shift_lo >= 128 ? (u128)0 : (hi_rc << shift_lo)) |
+                 (k2 >= 64 ? (u128)0 : (lo_rc >> (2 * k2)));
+    u128 mask_hi = k2 >= 64 ? ~(u128)0 : (((u128)1 << (2 * k2)) - 1);
+    u128 rc_hi = lo_rc & mask_hi;
+    return Kmer256{rc_lo, rc_hi};
+}
+
+// dna.rs:185-192 Kmer256::canonical
+static Kmer256 kmer_canonical(const Kmer256& x, unsigned k, int* is_rc) {
+    Kmer256 rc = kmer_revcomp(x, k);
+    if (x <= rc) { if (is_rc) *is_rc = 0; return x; }
+    if (is_rc) *is_rc = 1;
+    return rc;
+}
+
+// dna.rs:196-221 Kmer256::extend_right
+static Kmer256 kmer_extend_right(const Kmer256& x, uint8_t b, unsigned k) {
+    if (k == 0 || k > 128) return Kmer256{0, 0};
+    u128 b2 = (u128)(b & 3);
+    if (k <= 64) {
+        u128 mask = k == 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1);
+        return Kmer256{((x.lo << 2) | b2) & mask, 0};
+    }
+    unsigned k2 = k - 64;
+    u128 mask_hi = k2 == 64 ? ~(u128)0 : (((u128)1 << (2 * k2)) - 1);
+    u128 s64 = (x.hi >> (2 * (k2 - 1))) & 3;
+    return Kmer256{(x.lo << 2) | s64, ((x.hi << 2) | b2) & mask_hi};
+}
+
+// dna.rs:225-245 Kmer256::prepend_left
+static Kmer256 kmer_prepend_left(const Kmer256& x, uint8_t a, unsigned k) {
+    if (k == 0 || k > 128) return Kmer256{0, 0};
+    u128 a2 = (u128)(a & 3);
+    if (k <= 64) return Kmer256{(x.lo >> 2) | (a2 << (2 * (k - 1))), 0};
+    unsigned k2 = k - 64;
+    u128 mask_hi = k2 == 64 ? ~(u128)0 : (((u128)1 << (2 * k2)) - 1);
+    u128 s63 = x.lo & 3;
+    return Kmer256{(x.lo >> 2) | (a2 << 126), ((x.hi >> 2) | (s63 << (2 * (k2 - 1)))) & mask_hi};
+}
+
+// dna.rs:299-321 Kmer256::from_bytes — O(k) re-encode of an ASCII window.
+static bool kmer_from_bytes(const uint8_t* s, size_t len, unsigned k, Kmer256* out) {
+    if (k == 0 || len < k || k > 128) return false;
+    u128 lo = 0, hi = 0;
+    if (k <= 64) {
+        for (unsigned i = 0; i < k; ++i) {
+            uint8_t c = ascii_to_2bit(s[i]);
+            if (c == 0xFF) return false;
+            lo = (lo << 2) | (u128)c;
+        }
+    } else {
+        for (unsigned i = 0; i < 64; ++i) {
+            uint8_t c = ascii_to_2bit(s[i]);
+            if (c == 0xFF) return false;
+            lo = (lo << 2) | (u128)c;
+        }
+        for (unsigned i = 64; i < k; ++i) {
+            uint8_t c = ascii_to_2bit(s[i]);
+            if (c == 0xFF) return false;
+            hi = (hi << 2) | (u128)c;
+        }
+    }
+    *out = Kmer256{lo, hi};
+    return true;
+}
+
+// dna.rs:274-295 Kmer256::to_string
+static void kmer_to_string(const Kmer256& x, unsigned k, uint8_t* out) {
+    if (k <= 64) {
+        u128 v = x.lo;
+        for (int i = (int)k - 1; i >= 0; --i) { out[i] = bit2_to_base((uint8_t)(v & 3)); v >>= 2; }
+    } else {
+        u128 v = x.lo;
+        for (int i = 63; i >= 0; --i) { out[i] = bit2_to_base((uint8_t)(v & 3)); v >>= 2; }
+        u128 w = x.hi;
+        for (int i = (int)k - 1; i >= 64; --i) { out[i] = bit2_to_base((uint8_t)(w & 3)); w >>= 2; }
+    }
+}
+
+static void kmer_to_words(const Kmer256& x, uint64_t* w) {
+    w[0] = (uint64_t)x.lo; w[1] = (uint64_t)(x.lo >> 64);
+    w[2] = (uint64_t)x.hi; w[3] = (uint64_t)(x.hi >> 64);
+}
+
+static Kmer256 kmer_from_words(const uint64_t* w) {
+    return Kmer256{((u128)w[1] << 64) | w[0], ((u128)w[3] << 64) | w[2]};
+}
+
+// ---------------------------------------------------------------------------
+// bloom.rs
+// ---------------------------------------------------------------------------
+
+static inline uint64_t rotl64(uint64_t x, unsigned r) { return (x << r) | (x >> (64 - r)); }
+
+// bloom.rs:61-78 hash_kmer256 — three rounds over the four 64-bit limbs.
+static uint64_t hash_kmer256(const Kmer256& km, uint64_t seed) {
+    uint64_t w0 = (uint64_t)km.lo, w1 = (uint64_t)(km.lo >> 64);
+    uint64_t w2 = (uint64_t)km.hi, w3 = (uint64_t)(km.hi >> 64);
+    uint64_t x = w0 ^ seed;
+    x = (x ^ (rotl64(w1, 21) * 0x9e3779b97f4a7c15ULL)) * 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x = (x ^ (rotl64(w2, 43) * 0xbf58476d1ce4e5b9ULL)) * 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    x = (x ^ (rotl64(w3, 11) * 0x94d049bb133111ebULL)) * 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    return x;
+}
+
+// bloom.rs:84-86 seeds, in h1, h2, h3 order.
+static const uint64_t SEEDS[3] = {0x517cc1b727220a95ULL, 0x9e3779b97f4a7c15ULL, 0x6c62272e07bb0142ULL};
+
+// bloom.rs:13-43 TwoTierFilter
+struct TwoTierFilter {
+    uint64_t size_bits = 0, num_words = 0;
+    std::vector<std::atomic<uint64_t>> seen, solid;
+    explicit TwoTierFilter(uint64_t capacity_bits) {
+        num_words = std::max<uint64_t>((capacity_bits + 63) / 64, 1);  // div_ceil(64).max(1)
+        size_bits = num_words * 64;
+        seen = std::vector<std::atomic<uint64_t>>(num_words);
+        solid = std::vector<std::atomic<uint64_t>>(num_words);
+        for (uint64_t i = 0; i < num_words; ++i) { seen[i].store(0); solid[i].store(0); }
+    }
+    // bloom.rs:83-107 insert_kmer256: seen fetch_or h1,h2,h3; promote iff all three were set.
+    bool insert(const Kmer256& km) {
+        uint64_t h[3];
+        for (int j = 0; j < 3; ++j) h[j] = hash_kmer256(km, SEEDS[j]) % size_bits;
+        uint64_t prev[3], bit[3];
+        for (int j = 0; j < 3; ++j) {
+            bit[j] = 1ULL << (h[j] % 64);
+            prev[j] = seen[h[j] / 64].fetch_or(bit[j], std::memory_order_relaxed);
+        }
+        if ((prev[0] & bit[0]) && (prev[1] & bit[1]) && (prev[2] & bit[2])) {
+            for (int j = 0; j < 3; ++j) solid[h[j] / 64].fetch_or(bit[j], std::memory_order_relaxed);
+            return true;
+        }
+        return false;
+    }
+    // bloom.rs:111-126 is_solid_kmer256
+    bool is_solid(const Kmer256& km) const {
+        for (int j = 0; j < 3; ++j) {
+            uint64_t h = hash_kmer256(km, SEEDS[j]) % size_bits;
+            if (!(solid[h / 64].load(std::memory_order_relaxed) & (1ULL << (h % 64)))) return false;
+        }
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// hammer.rs
+// ---------------------------------------------------------------------------
+
+// hammer.rs:13-18
+static uint32_t hamming_distance_2bit(uint64_t a, uint64_t b) {
+    uint64_t d = a ^ b;
+    return (uint32_t)__builtin_popcountll((d | (d >> 1)) & 0x5555555555555555ULL);
+}
+
+static bool window_solid(const TwoTierFilter& f, const uint8_t* s, unsigned k) {
+    Kmer256 km;
+    if (!kmer_from_bytes(s, k, k, &km)) return false;
+    return f.is_solid(kmer_canonical(km, k, nullptr));
+}
+
+static uint8_t to_ascii_upper(uint8_t c) { return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; }
+
+// hammer.rs:34-120 ErrorCorrector::correct_read_in_place.
+static bool correct_read_in_place(const TwoTierFilter& f, unsigned k, uint8_t* seq, size_t len,
+                                  std::vector<uint8_t>& scratch) {
+    if (k == 0 || k > 128 || len < k) return false;
+    scratch.assign(seq, seq + len);
+    bool modified = false;
+    for (size_t i = 0; i + k <= len; ++i) {
+        Kmer256 km;
+        if (!kmer_from_bytes(&scratch[i], k, k, &km)) continue;
+        if (f.is_solid(kmer_canonical(km, k, nullptr))) continue;
+        bool have_best = false;
+        size_t best_off = 0;
+        uint8_t best_base = 0;
+        size_t max_gain = 0;
+        for (size_t offset = 0; offset < k; ++offset) {
+            size_t pos = i + offset;
+            uint8_t orig = scratch[pos];
+            uint8_t orig_upper = to_ascii_upper(orig);
+            size_t start_scan = pos >= (k - 1) ? pos - (k - 1) : 0;  // saturating_sub
+            size_t end_scan = std::min(pos, len - k);
+            size_t base_solid = 0;
+            for (size_t s = start_scan; s <= end_scan; ++s) {
+                if (s == i) continue;
+                if (window_solid(f, &scratch[s], k)) ++base_solid;
+            }
+            static const uint8_t CAND[4] = {'A', 'C', 'G', 'T'};
+            for (int c = 0; c < 4; ++c) {
+                uint8_t cand = CAND[c];
+                if (cand == orig_upper) continue;
+                scratch[pos] = cand;
+                if (window_solid(f, &scratch[i], k)) {
+                    size_t solid_count = 1;  // window i is solid with this candidate
+                    for (size_t s = start_scan; s <= end_scan; ++s) {
+                        if (s == i) continue;
+                        if (window_solid(f, &scratch[s], k)) ++solid_count;
+                    }
+                    size_t gain = solid_count > base_solid ? solid_count - base_solid : 0;  // saturating_sub
+                    if (gain > max_gain) {
+                        max_gain = gain;
+                        have_best = true;
+                        best_off = offset;
+                        best_base = cand;
+                    }
+                }
+                scratch[pos] = orig;
+            }
+        }
+        if (have_best) {
+            seq[i + best_off] = best_base;
+            scratch[i + best_off] = best_base;
+            modified = true;
+        }
+    }
+    return modified;
+}
+
+// ---------------------------------------------------------------------------
+// packed_reads.rs
+// ---------------------------------------------------------------------------
+
+// packed_reads.rs:547-578 pack_clean: MSB-first, 4 bases per byte, each read byte-aligned;
+// non-ACGT bytes are skipped (release behaviour).
+static void pack_clean(const uint8_t* read, size_t len, std::vector<uint8_t>& data,
+                       std::vector<uint64_t>& offsets, std::vector<uint32_t>& lengths) {
+    offsets.push_back(data.size());
+    uint32_t clean_len = 0;
+    uint8_t byte = 0;
+    int shift = 6;
+    for (size_t i = 0; i < len; ++i) {
+        uint8_t b2 = ascii_to_2bit(read[i]);
+        if (b2 == 0xFF) continue;
+        ++clean_len;
+        byte |= (uint8_t)(b2 << shift);
+        if (shift == 0) { data.push_back(byte); byte = 0; shift = 6; }
+        else shift -= 2;
+    }
+    if (shift != 6) data.push_back(byte);
+    lengths.push_back(clean_len);
+}
+
+// packed_reads.rs:592-613 get_read
+static void get_read(const uint8_t* data, uint64_t off, uint32_t len, std::vector<uint8_t>& buf) {
+    buf.clear();
+    if (len == 0) return;
+    size_t nbytes = (len + 3) / 4;
+    size_t rem = len;
+    for (size_t j = 0; j < nbytes; ++j) {
+        uint8_t byte = data[off + j];
+        size_t take = std::min<size_t>(rem, 4);
+        for (int s = 3; s >= (int)(4 - take); --s) buf.push_back(bit2_to_base((byte >> (s * 2)) & 3));
+        rem -= take;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// assemble.rs:692-735 determine_solid_kmer_cutoff
+// ---------------------------------------------------------------------------
+static uint32_t determine_solid_kmer_cutoff(std::vector<uint32_t> covs, double min_coverage, uint32_t* top_out) {
+    if (top_out) *top_out = 0;
+    if (min_coverage > 5.0) {
+        double v = std::max(min_coverage * 0.35, 2.0);
+        return (uint32_t)v;  // `as u32` saturates; values here are small and finite
+    }
+    if (covs.size() <= 100) return 2;
+    std::sort(covs.begin(), covs.end());
+    uint32_t top = covs[covs.size() * 95 / 100];
+    if (top_out) *top_out = top;
+    if (top < 30) return 2;
+    size_t mvs = std::min<uint32_t>(std::max<uint32_t>(top / 4, 10), 35);
+    std::vector<size_t> hist(mvs + 1, 0);
+    for (uint32_t c : covs)
+        if ((size_t)c <= mvs) hist[c] += 1;
+    uint32_t valley = 2;
+    size_t min_val = SIZE_MAX;
+    bool found = false;
+    for (size_t c = 2; c <= mvs; ++c) {
+        size_t h = hist[c];
+        if (h <= min_val) { min_val = h; valley = (uint32_t)c; }
+        else if (h > min_val * 2 && c > (size_t)valley + 2) { found = true; break; }
+    }
+    return found ? valley : 2;
+}
+
+// ---------------------------------------------------------------------------
+// assemble.rs:122-298 Stage 2 (Bloom build) + Stage 3 (optional Hammer, count, cutoff, solid set)
+// ---------------------------------------------------------------------------
+
+struct Spectrum {
+    unsigned k = 0;
+    uint64_t num_words = 0;
+    std::vector<uint64_t> seen, solid;
+    std::vector<Kmer256> keys;      // sorted ascending (Kmer256 Ord)
+    std::vector<uint32_t> counts;
+    std::vector<uint8_t> solid_mask;
+    std::vector<uint8_t> corrected_data;  // packed reads after Hammer (same offsets/lengths)
+    uint32_t cutoff = 0, top_cov = 0;
+    uint64_t corrected_reads = 0, n_solid = 0, n_occ = 0;
+    double stage2_s = 0, stage3_s = 0;
+};
+
+template <class F>
+static void run_chunks(size_t num_chunks, int threads, F&& fn) {
+    if (threads <= 1) {
+        for (size_t c = 0; c < num_chunks; ++c) fn(c);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&] {
+            for (;;) {
+                size_t c = next.fetch_add(1);
+                if (c >= num_chunks) break;
+                fn(c);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+static size_t next_pow2(size_t x) {
+    size_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+static Spectrum* spectrum_run(const uint8_t* data, const uint64_t* offsets, const uint32_t* lengths,
+                              uint64_t n_reads, unsigned k, uint64_t bloom_bits, int error_correct,
+                              double min_coverage, int threads) {
+    auto* out = new Spectrum();
+    out->k = k;
+    auto t0 = std::chrono::steady_clock::now();
+    TwoTierFilter filter(bloom_bits);  // assemble.rs:130
+    out->num_words = filter.num_words;
+
+    // assemble.rs:136-139 rayon chunking
+    size_t num_threads = (size_t)std::max(threads, 1);
+    size_t chunk_size = std::max<size_t>(n_reads / num_threads, 1000);
+    size_t num_chunks = n_reads == 0 ? 0 : (n_reads + chunk_size - 1) / chunk_size;
+
+    std::atomic<uint64_t> n_occ{0};
+    // assemble.rs:141-157 Stage 2
+    run_chunks(num_chunks, threads, [&](size_t c) {
+        size_t start = c * chunk_size, end = std::min<size_t>(start + chunk_size, n_reads);
+        std::vector<uint8_t> buf;
+        buf.reserve(512);
+        uint64_t local = 0;
+        for (size_t idx = start; idx < end; ++idx) {
+            get_read(data, offsets[idx], lengths[idx], buf);
+            if (buf.size() < k) continue;
+            for (size_t i = 0; i + k <= buf.size(); ++i) {
+                Kmer256 km;
+                if (kmer_from_bytes(&buf[i], k, k, &km)) {
+                    filter.insert(kmer_canonical(km, k, nullptr));
+                    ++local;
+                }
+            }
+        }
+        n_occ.fetch_add(local);
+    });
+    out->n_occ = n_occ.load();
+    auto t1 = std::chrono::steady_clock::now();
+
+    // assemble.rs:167-219 Stage 3
+    size_t num_shards = std::min<size_t>(std::max<size_t>(next_pow2(num_threads * 4), 8), 64);
+    size_t shard_mask = num_shards - 1;
+    std::vector<std::mutex> locks(num_shards);
+    std::vector<std::unordered_map<Kmer256, uint32_t, Kmer256Hash>> shards(num_shards);
+    for (auto& m : shards) m.reserve(75000);
+    std::atomic<uint64_t> corrected{0};
+    std::vector<uint8_t> corrected_data;
+    if (error_correct) {
+        uint64_t total = 0;
+        for (uint64_t r = 0; r < n_reads; ++r) total = std::max<uint64_t>(total, offsets[r] + (lengths[r] + 3) / 4);
+        corrected_data.assign(data, data + total);
+    }
+
+    run_chunks(num_chunks, threads, [&](size_t c) {
+        size_t start = c * chunk_size, end = std::min<size_t>(start + chunk_size, n_reads);
+        std::vector<std::vector<Kmer256>> tb(num_shards);
+        for (auto& v : tb) v.reserve(256);
+        std::vector<uint8_t> buf, scratch;
+        buf.reserve(512);
+        scratch.reserve(512);
+        for (size_t idx = start; idx < end; ++idx) {
+            get_read(data, offsets[idx], lengths[idx], buf);
+            if (buf.size() < k) continue;
+            if (error_correct && correct_read_in_place(filter, k, buf.data(), buf.size(), scratch)) {
+                corrected.fetch_add(1);
+                // record the corrected read in packed form at its own offset (pack_clean layout)
+                uint64_t off = offsets[idx];
+                for (size_t j = 0; j < buf.size(); j += 4) {
+                    uint8_t byte = 0;
+                    for (size_t q = 0; q < 4 && j + q < buf.size(); ++q)
+                        byte |= (uint8_t)(ascii_to_2bit(buf[j + q]) << (6 - 2 * q));
+                    corrected_data[off + j / 4] = byte;
+                }
+            }
+            for (size_t i = 0; i + k <= buf.size(); ++i) {
+                Kmer256 km;
+                if (!kmer_from_bytes(&buf[i], k, k, &km)) continue;
+                Kmer256 can = kmer_canonical(km, k, nullptr);
+                if (filter.is_solid(can)) {
+                    size_t si = (size_t)(uint64_t)can.lo & shard_mask;
+                    tb[si].push_back(can);
+                    if (tb[si].size() >= 256) {
+                        std::lock_guard<std::mutex> g(locks[si]);
+                        for (auto& x : tb[si]) shards[si][x] += 1;
+                        tb[si].clear();
+                    }
+                }
+            }
+        }
+        for (size_t si = 0; si < num_shards; ++si) {
+            if (tb[si].empty()) continue;
+            std::lock_guard<std::mutex> g(locks[si]);
+            for (auto& x : tb[si]) shards[si][x] += 1;
+            tb[si].clear();
+        }
+    });
+    out->corrected_reads = corrected.load();
+
+    // Filter state is exported (the reference drops it here, assemble.rs:229).
+    out->seen.resize(filter.num_words);
+    out->solid.resize(filter.num_words);
+    for (uint64_t w = 0; w < filter.num_words; ++w) {
+        out->seen[w] = filter.seen[w].load();
+        out->solid[w] = filter.solid[w].load();
+    }
+
+    // assemble.rs:238-242 merge; outputs are sorted here so they can be compared.
+    std::vector<std::pair<Kmer256, uint32_t>> all;
+    size_t total = 0;
+    for (auto& m : shards) total += m.size();
+    all.reserve(total);
+    for (auto& m : shards)
+        for (auto& kv : m) all.push_back(kv);
+    std::sort(all.begin(), all.end(), [](const std::pair<Kmer256, uint32_t>& a, const std::pair<Kmer256, uint32_t>& b) {
+        return a.first < b.first;
+    });
+    out->keys.resize(all.size());
+    out->counts.resize(all.size());
+    for (size_t i = 0; i < all.size(); ++i) { out->keys[i] = all[i].first; out->counts[i] = all[i].second; }
+
+    // assemble.rs:286-292 cutoff + solid set
+    out->cutoff = determine_solid_kmer_cutoff(out->counts, min_coverage, &out->top_cov);
+    out->solid_mask.resize(all.size());
+    uint64_t ns = 0;
+    for (size_t i = 0; i < all.size(); ++i) {
+        out->solid_mask[i] = out->counts[i] >= out->cutoff;
+        ns += out->solid_mask[i];
+    }
+    out->n_solid = ns;
+    if (error_correct) out->corrected_data.swap(corrected_data);
+    auto t2 = std::chrono::steady_clock::now();
+    out->stage2_s = std::chrono::duration<double>(t1 - t0).count();
+    out->stage3_s = std::chrono::duration<double>(t2 - t1).count();
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI for the ctypes test wrapper (oracle/oracle.py)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+uint8_t or_ascii_to_2bit(uint8_t b) { return ascii_to_2bit(b); }
+uint8_t or_bit2_to_base(uint8_t v) { return bit2_to_base(v); }
+uint64_t or_revcomp_u64(uint64_t x, uint32_t k) { return revcomp_kmer_u64(x, k); }
+uint64_t or_canonical_u64(uint64_t x, uint32_t k, int* is_rc) { return canonical_kmer_u64(x, k, is_rc); }
+uint32_t or_hamming_2bit(uint64_t a, uint64_t b) { return hamming_distance_2bit(a, b); }
+
+int or_kmer_from_bytes(const uint8_t* s, uint64_t len, uint32_t k, uint64_t* w4) {
+    Kmer256 km;
+    if (!kmer_from_bytes(s, len, k, &km)) return 0;
+    kmer_to_words(km, w4);
+    return 1;
+}
+void or_kmer_to_string(const uint64_t* w4, uint32_t k, uint8_t* out) { kmer_to_string(kmer_from_words(w4), k, out); }
+void or_kmer_revcomp(const uint64_t* w4, uint32_t k, uint64_t* out) { kmer_to_words(kmer_revcomp(kmer_from_words(w4), k), out); }
+int or_kmer_canonical(const uint64_t* w4, uint32_t k, uint64_t* out) {
+    int rc = 0;
+    kmer_to_words(kmer_canonical(kmer_from_words(w4), k, &rc), out);
+    return rc;
+}
+void or_kmer_extend_right(const uint64_t* w4, uint8_t b, uint32_t k, uint64_t* out) {
+    kmer_to_words(kmer_extend_right(kmer_from_words(w4), b, k), out);
+}
+void or_kmer_prepend_left(const uint64_t* w4, uint8_t a, uint32_t k, uint64_t* out) {
+    kmer_to_words(kmer_prepend_left(kmer_from_words(w4), a, k), out);
+}
+uint64_t or_hash_kmer256(const uint64_t* w4, uint64_t seed) { return hash_kmer256(kmer_from_words(w4), seed); }
+void or_hash_many(const uint64_t* w4, uint64_t n, uint64_t seed, uint64_t* out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = hash_kmer256(kmer_from_words(w4 + 4 * i), seed);
+}
+
+void* or_filter_new(uint64_t capacity_bits) { return new TwoTierFilter(capacity_bits); }
+void or_filter_free(void* f) { delete (TwoTierFilter*)f; }
+uint64_t or_filter_num_words(void* f) { return ((TwoTierFilter*)f)->num_words; }
+int or_filter_insert(void* f, const uint64_t* w4) { return ((TwoTierFilter*)f)->insert(kmer_from_words(w4)); }
+int or_filter_is_solid(void* f, const uint64_t* w4) { return ((TwoTierFilter*)f)->is_solid(kmer_from_words(w4)); }
+void or_filter_insert_many(void* f, const uint64_t* w4, uint64_t n, uint8_t* ret) {
+    for (uint64_t i = 0; i < n; ++i) {
+        int r = ((TwoTierFilter*)f)->insert(kmer_from_words(w4 + 4 * i));
+        if (ret) ret[i] = (uint8_t)r;
+    }
+}
+void or_filter_is_solid_many(void* f, const uint64_t* w4, uint64_t n, uint8_t* out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = (uint8_t)((TwoTierFilter*)f)->is_solid(kmer_from_words(w4 + 4 * i));
+}
+void or_filter_export(void* f, uint64_t* seen, uint64_t* solid) {
+    auto* F = (TwoTierFilter*)f;
+    for (uint64_t w = 0; w < F->num_words; ++w) { seen[w] = F->seen[w].load(); solid[w] = F->solid[w].load(); }
+}
+int or_correct_read(void* f, uint32_t k, uint8_t* seq, uint64_t len) {
+    std::vector<uint8_t> scratch;
+    return correct_read_in_place(*(TwoTierFilter*)f, k, seq, len, scratch);
+}
+
+uint32_t or_cutoff(const uint32_t* counts, uint64_t n, double min_cov, uint32_t* top) {
+    return determine_solid_kmer_cutoff(std::vector<uint32_t>(counts, counts + n), min_cov, top);
+}
+
+// Packs reads given as a concatenated ASCII buffer with per-read starts/lengths.
+// Returns the packed byte length; data_out must hold sum(ceil(len/4)) bytes.
+uint64_t or_pack(const uint8_t* ascii, const uint64_t* starts, const uint64_t* lens, uint64_t n,
+                 uint8_t* data_out, uint64_t* offsets_out, uint32_t* lengths_out) {
+    std::vector<uint8_t> data;
+    std::vector<uint64_t> offs;
+    std::vector<uint32_t> lns;
+    for (uint64_t i = 0; i < n; ++i) pack_clean(ascii + starts[i], lens[i], data, offs, lns);
+    std::memcpy(data_out, data.data(), data.size());
+    std::memcpy(offsets_out, offs.data(), n * 8);
+    std::memcpy(lengths_out, lns.data(), n * 4);
+    return data.size();
+}
+void or_get_read(const uint8_t* data, uint64_t off, uint32_t len, uint8_t* out) {
+    std::vector<uint8_t> buf;
+    get_read(data, off, len, buf);
+    if (!buf.empty()) std::memcpy(out, buf.data(), buf.size());
+}
+
+void* or_spectrum_run(const uint8_t* data, const uint64_t* offsets, const uint32_t* lengths, uint64_t n_reads,
+                      uint32_t k, uint64_t bloom_bits, int error_correct, double min_cov, int threads) {
+    return spectrum_run(data, offsets, lengths, n_reads, k, bloom_bits, error_correct, min_cov, threads);
+}
+void or_spectrum_free(void* h) { delete (Spectrum*)h; }
+uint64_t or_spectrum_num_words(void* h) { return ((Spectrum*)h)->num_words; }
+uint64_t or_spectrum_num_counts(void* h) { return ((Spectrum*)h)->keys.size(); }
+uint64_t or_spectrum_n_occ(void* h) { return ((Spectrum*)h)->n_occ; }
+uint32_t or_spectrum_cutoff(void* h) { return ((Spectrum*)h)->cutoff; }
+uint32_t or_spectrum_top_cov(void* h) { return ((Spectrum*)h)->top_cov; }
+uint64_t or_spectrum_n_solid(void* h) { return ((Spectrum*)h)->n_solid; }
+uint64_t or_spectrum_corrected_reads(void* h) { return ((Spectrum*)h)->corrected_reads; }
+double or_spectrum_stage2_secs(void* h) { return ((Spectrum*)h)->stage2_s; }
+double or_spectrum_stage3_secs(void* h) { return ((Spectrum*)h)->stage3_s; }
+void or_spectrum_export_bloom(void* h, uint64_t* seen, uint64_t* solid) {
+    auto* S = (Spectrum*)h;
+    std::memcpy(seen, S->seen.data(), S->num_words * 8);
+    std::memcpy(solid, S->solid.data(), S->num_words * 8);
+}
+void or_spectrum_export_counts(void* h, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask) {
+    auto* S = (Spectrum*)h;
+    for (size_t i = 0; i < S->keys.size(); ++i) kmer_to_words(S->keys[i], keys4 + 4 * i);
+    std::memcpy(counts, S->counts.data(), S->counts.size() * 4);
+    std::memcpy(solid_mask, S->solid_mask.data(), S->solid_mask.size());
+}
+uint64_t or_spectrum_corrected_bytes(void* h) { return ((Spectrum*)h)->corrected_data.size(); }
+void or_spectrum_export_corrected(void* h, uint8_t* out) {
+    auto* S = (Spectrum*)h;
+    std::memcpy(out, S->corrected_data.data(), S->corrected_data.size());
+}
+
+}  // extern "C"

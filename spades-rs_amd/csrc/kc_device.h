@@ -1,0 +1,230 @@
+// kc_device.h — device primitives of the gfx950 k-mer spectrum engine.
+//
+// A k-mer (k <= 63) is held as its 2k-bit value V, first base most significant
+// (the value Kmer256::from_bytes gives for k <= 64, dna.rs:299-321), in two
+// 64-bit words: lo = bits 0..63, hi = bits 64..127.  WIDE = false is the
+// k <= 32 case (hi == 0), WIDE = true covers 33 <= k <= 63.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kc {
+
+struct Kv {
+    uint64_t lo, hi;
+};
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ bool kv_le(const Kv& a, const Kv& b) {
+    return a.hi < b.hi || (a.hi == b.hi && a.lo <= b.lo);
+}
+
+// Reverse the order of the 32 2-bit groups of x, keeping the two bits of a group in order.
+__device__ __forceinline__ uint64_t rev_groups(uint64_t x) {
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    return __builtin_bswap64(x);
+}
+
+// Reverse complement of a k-mer value (complement = code ^ 3, dna.rs:1-8): the same
+// result as Kmer256::revcomp (dna.rs:152-180) for k <= 63.
+template <bool WIDE>
+__device__ __forceinline__ Kv revcomp(const Kv& v, uint32_t k) {
+    if (!WIDE) return Kv{rev_groups(~v.lo) >> (64 - 2 * k), 0};
+    const uint64_t rhi = rev_groups(~v.lo), rlo = rev_groups(~v.hi);
+    const uint32_t s2 = 128 - 2 * k;  // 2..62
+    return Kv{(rlo >> s2) | (rhi << (64 - s2)), rhi >> s2};
+}
+
+// Kmer256::canonical (dna.rs:185-192): min(fwd, rc), ties to the forward strand.
+template <bool WIDE>
+__device__ __forceinline__ Kv canonical(const Kv& v, uint32_t k) {
+    const Kv rc = revcomp<WIDE>(v, k);
+    return kv_le(v, rc) ? v : rc;
+}
+
+// The 2k bits starting at bit `sh` (0..31) of a big-endian bit stream held in the
+// 32-bit words p[0..4] (byte 0 = bits 31..24 of the first word after bswap).  p must be
+// 4-byte aligned with 3 (k <= 32) or 5 (k <= 63) readable words.
+template <bool WIDE>
+__device__ __forceinline__ Kv extract_be(const uint32_t* p, uint32_t sh, uint32_t k) {
+    if (!WIDE) {
+        const uint64_t w0 = __builtin_bswap32(p[0]), w1 = __builtin_bswap32(p[1]);
+        const uint32_t w2 = __builtin_bswap32(p[2]);
+        uint64_t top = (w0 << 32) | w1;
+        top = sh ? ((top << sh) | (uint64_t)(w2 >> (32 - sh))) : top;
+        return Kv{top >> (64 - 2 * k), 0};
+    }
+    const uint64_t a = ((uint64_t)__builtin_bswap32(p[0]) << 32) | __builtin_bswap32(p[1]);
+    const uint64_t b = ((uint64_t)__builtin_bswap32(p[2]) << 32) | __builtin_bswap32(p[3]);
+    const uint64_t c = (uint64_t)__builtin_bswap32(p[4]) << 32;
+    const uint64_t hi = sh ? ((a << sh) | (b >> (64 - sh))) : a;
+    const uint64_t lo = sh ? ((b << sh) | (c >> (64 - sh))) : b;
+    const uint32_t s2 = 128 - 2 * k;  // 2..62 for 33 <= k <= 63
+    return Kv{(lo >> s2) | (hi << (64 - s2)), hi >> s2};
+}
+
+// Window starting at base s of a read whose packed bytes start at byte `off` of `data`
+// (PackedReads layout, packed_reads.rs:547-578: base j in byte j/4, bits 6-2*(j%4)).
+// `data` must be readable 20 bytes past the last read byte.
+template <bool WIDE>
+__device__ __forceinline__ Kv load_window(const uint8_t* __restrict__ data, uint64_t off, uint32_t s, uint32_t k) {
+    const uint64_t b0 = off + (s >> 2);
+    const uint64_t a = b0 & ~3ull;
+    const uint32_t sh = (uint32_t)(b0 - a) * 8 + 2 * (s & 3);
+    return extract_be<WIDE>(reinterpret_cast<const uint32_t*>(data + a), sh, k);
+}
+
+// ---- bloom.rs:61-78 hash_kmer256 -------------------------------------------------
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+// h % d for a runtime d = 64 * num_words (bloom.rs:84-86).  Power-of-two sizes use a mask;
+// otherwise magic = floor(2^64 / d) gives q' = mulhi(h, magic) in {q-1, q}, so one
+// conditional subtraction yields the exact remainder.
+struct Modulus {
+    uint64_t d, magic, mask;
+    uint32_t pow2;
+    __device__ __forceinline__ uint64_t mod(uint64_t h) const {
+        if (pow2) return h & mask;
+        const uint64_t q = __umul64hi(h, magic);
+        const uint64_t r = h - q * d;
+        return r >= d ? r - d : r;
+    }
+};
+
+constexpr uint64_t SEED1 = 0x517cc1b727220a95ull;
+constexpr uint64_t SEED2 = 0x9e3779b97f4a7c15ull;
+constexpr uint64_t SEED3 = 0x6c62272e07bb0142ull;
+
+// The three Bloom bit positions h_j % size_bits of a Kmer256 given as limbs w0..w3.
+__device__ __forceinline__ void bloom_positions(uint64_t w0, uint64_t w1, uint64_t w2, uint64_t w3,
+                                                const Modulus& M, uint64_t pos[3]) {
+    const uint64_t m1 = rotl64(w1, 21) * 0x9e3779b97f4a7c15ull;
+    const uint64_t m2 = rotl64(w2, 43) * 0xbf58476d1ce4e5b9ull;
+    const uint64_t m3 = rotl64(w3, 11) * 0x94d049bb133111ebull;
+    const uint64_t seeds[3] = {SEED1, SEED2, SEED3};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        uint64_t x = w0 ^ seeds[j];
+        x = (x ^ m1) * 0xff51afd7ed558ccdull;
+        x ^= x >> 33;
+        x = (x ^ m2) * 0xc4ceb9fe1a85ec53ull;
+        x ^= x >> 33;
+        x = (x ^ m3) * 0xff51afd7ed558ccdull;
+        x ^= x >> 33;
+        pos[j] = M.mod(x);
+    }
+}
+
+// bloom.rs:111-126 is_solid_kmer256 for a k <= 64 k-mer value (hi limbs zero).
+__device__ __forceinline__ bool bloom_is_solid(const Kv& v, const uint64_t* __restrict__ solid, const Modulus& M) {
+    uint64_t p[3];
+    bloom_positions(v.lo, v.hi, 0, 0, M, p);
+    const uint64_t a = solid[p[0] >> 6], b = solid[p[1] >> 6], c = solid[p[2] >> 6];
+    return ((a >> (p[0] & 63)) & (b >> (p[1] & 63)) & (c >> (p[2] & 63)) & 1ull) != 0;
+}
+
+// ---- device hash table -----------------------------------------------------------
+//
+// Open addressing, linear probing, one slot = key words + first-occurrence event + count.
+// NW = 1: the key is V itself (k <= 32; a canonical 32-mer is never all ones because
+// revcomp(T^32) = A^32 < T^32, so ~0 can mark an empty word).  NW = 2 (33 <= k <= 63):
+// V (< 2^126) is split into two 63-bit words A = V >> 63, B = V & (2^63 - 1), neither of
+// which can be ~0.  A slot is claimed word by word with 64-bit CAS (A first): every word
+// goes from EMPTY to its final value exactly once, so all inserters of one key take the
+// same decision at every slot and the key lands in exactly one slot without locks.
+
+template <int NW, typename Ev>
+struct Slot {
+    uint64_t k[NW];
+    Ev t0;         // smallest event index (global window number) of this key
+    uint32_t cnt;  // occurrences
+};
+
+template <int NW>
+__device__ __forceinline__ void pack_key(const Kv& v, uint64_t key[NW]) {
+    if (NW == 1) {
+        key[0] = v.lo;
+    } else {
+        key[0] = (v.hi << 1) | (v.lo >> 63);
+        key[NW - 1] = v.lo & 0x7FFFFFFFFFFFFFFFull;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ Kv unpack_key(const uint64_t* key) {
+    if (NW == 1) return Kv{key[0], 0};
+    return Kv{key[NW - 1] | (key[0] << 63), key[0] >> 1};
+}
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+template <int NW>
+__device__ __forceinline__ uint64_t table_hash(const uint64_t key[NW]) {
+    uint64_t h = key[0] * 0x9E3779B97F4A7C15ull;
+    if (NW == 2) h ^= key[NW - 1] + 0x632BE59BD9B4E019ull;
+    return fmix64(h);
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_relaxed(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Insert one occurrence (event t) of key.  Returns false when max_probe slots were
+// inspected without finding or claiming the key (the caller flags table overflow).
+template <int NW, typename Ev, bool TRACK_T0>
+__device__ __forceinline__ bool table_insert(Slot<NW, Ev>* __restrict__ tab, uint64_t cap, const uint64_t key[NW],
+                                             Ev t, uint32_t max_probe) {
+    uint64_t i = __umul64hi(table_hash<NW>(key), cap);
+    for (uint32_t probe = 0; probe < max_probe; ++probe) {
+        Slot<NW, Ev>* s = tab + i;
+        bool match = true;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            uint64_t cur = ld_relaxed(&s->k[w]);
+            if (cur == EMPTY_KEY) {
+                cur = atomicCAS(reinterpret_cast<unsigned long long*>(&s->k[w]), (unsigned long long)EMPTY_KEY,
+                                (unsigned long long)key[w]);
+                if (cur == EMPTY_KEY) cur = key[w];
+            }
+            if (cur != key[w]) {
+                match = false;
+                break;
+            }
+        }
+        if (match) {
+            atomicAdd(&s->cnt, 1u);
+            if (TRACK_T0) {
+                if (t < ld_relaxed(&s->t0)) atomicMin(&s->t0, t);
+            }
+            return true;
+        }
+        i = (i + 1 == cap) ? 0 : i + 1;
+    }
+    return false;
+}
+
+// Wave-aggregated append: returns a unique index for every lane with pred set.
+__device__ __forceinline__ uint64_t wave_append(bool pred, unsigned long long* counter) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (mask != 0 && (int)lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    const uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    return base + __popcll(below);
+}
+
+}  // namespace kc

@@ -1,0 +1,114 @@
+"""ctypes binding of libkc.so (the C ABI declared in include/kc.h).
+
+The library is the only compute path: if it is missing this module raises at
+import time instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libkc.so")
+
+KC_OK = 0
+KC_EINVAL = -1
+KC_ENOMEM = -2
+KC_EHIP = -3
+KC_ESTATE = -4
+KC_EUNSUPPORTED = -5
+KC_MAX_K = 63
+KC_FLAG_TIMING = 1
+
+STATUS_NAMES = {KC_EINVAL: "KC_EINVAL", KC_ENOMEM: "KC_ENOMEM", KC_EHIP: "KC_EHIP",
+                KC_ESTATE: "KC_ESTATE", KC_EUNSUPPORTED: "KC_EUNSUPPORTED"}
+
+# Every entry point include/kc.h declares.
+EXPORTS = (
+    "kc_filter_create", "kc_filter_destroy", "kc_filter_last_error", "kc_filter_memory_usage_bytes",
+    "kc_filter_num_words", "kc_filter_insert_kmer256", "kc_filter_is_solid_kmer256", "kc_filter_export",
+    "kc_filter_correct_reads",
+    "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
+    "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
+)
+
+
+class KcError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class kc_opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("k", ctypes.c_uint32), ("bloom_bits", ctypes.c_uint64),
+                ("min_coverage", ctypes.c_double), ("error_correct", ctypes.c_int32),
+                ("table_load", ctypes.c_float), ("flags", ctypes.c_uint32)]
+
+
+class kc_info(ctypes.Structure):
+    _fields_ = [("n_reads", ctypes.c_uint64), ("n_occ", ctypes.c_uint64), ("n_distinct", ctypes.c_uint64),
+                ("n_counts", ctypes.c_uint64), ("n_solid", ctypes.c_uint64), ("corrected_reads", ctypes.c_uint64),
+                ("num_words", ctypes.c_uint64), ("table_capacity", ctypes.c_uint64), ("cutoff", ctypes.c_uint32),
+                ("top_cov", ctypes.c_uint32), ("ms_stage2", ctypes.c_double), ("ms_stage3", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_dblp = ctypes.POINTER(ctypes.c_double)
+_sz = ctypes.c_size_t
+
+_SIGS = {
+    "kc_filter_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_uint64]),
+    "kc_filter_destroy": (None, [_vp]),
+    "kc_filter_last_error": (ctypes.c_char_p, [_vp]),
+    "kc_filter_memory_usage_bytes": (ctypes.c_uint64, [_vp]),
+    "kc_filter_num_words": (ctypes.c_uint64, [_vp]),
+    "kc_filter_insert_kmer256": (ctypes.c_int, [_vp, _u64p, _sz, _u8p]),
+    "kc_filter_is_solid_kmer256": (ctypes.c_int, [_vp, _u64p, _sz, _u8p]),
+    "kc_filter_export": (ctypes.c_int, [_vp, _u64p, _u64p, _sz]),
+    "kc_filter_correct_reads": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz, _u64p, _u32p, _sz, _u8p, _u64p]),
+    "kc_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(kc_opts)]),
+    "kc_destroy": (None, [_vp]),
+    "kc_last_error": (ctypes.c_char_p, [_vp]),
+    "kc_reads_upload": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u32p, _sz]),
+    "kc_bloom_build": (ctypes.c_int, [_vp]),
+    "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
+    "kc_run": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
+    "kc_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
+    "kc_bloom_export": (ctypes.c_int, [_vp, _u64p, _u64p, _sz]),
+    "kc_bloom_query": (ctypes.c_int, [_vp, _u64p, _sz, _u8p]),
+    "kc_counts_export": (ctypes.c_int, [_vp, _u64p, _u32p, _u8p, _sz]),
+    "kc_reads_export": (ctypes.c_int, [_vp, _u8p, _sz]),
+    "kc_kernel_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), _dblp, _u32p, ctypes.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libkc.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C spades-rs_amd` "
+                              f"(or __graft_entry__.build()); there is no fallback path")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a, t):
+    """Pointer to a numpy array's data (None for an empty array)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+u8p, u32p, u64p = _u8p, _u32p, _u64p

@@ -1,0 +1,261 @@
+"""Host interface of the gfx950 engine, mirroring the reference's Rust API.
+
+  TwoTierFilter        bloom.rs:13-131      (device-resident two-tier filter)
+  ErrorCorrector       hammer.rs:20-144     (Hammer against a TwoTierFilter)
+  AssemblerConfig      assemble.rs:35-103   (the fields the k-mer path reads)
+  count_kmers          assemble.rs:122-298  (Stage 2 + Stage 3 of
+                                             run_assembly_with_packed_reads)
+
+Everything runs through libkc.so (include/kc.h); there is no host fallback.
+A k-mer crosses the boundary as the four little-endian limbs of the
+reference's Kmer256(lo: u128, hi: u128) (dna.rs:137-138).
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import KcError, lib, ptr, u8p, u32p, u64p
+
+
+def _check(status, obj, errfn):
+    if status != _abi.KC_OK:
+        msg = errfn(obj).decode() if obj else ""
+        raise KcError(status, msg)
+
+
+def _keys4(keys):
+    """Accept u64 k-mers (k <= 32, as the reference's insert(u64)) or (n, 4) limbs."""
+    a = np.asarray(keys, dtype=np.uint64)
+    if a.ndim == 1:
+        out = np.zeros((a.shape[0], 4), dtype=np.uint64)
+        out[:, 0] = a
+        return out
+    return np.ascontiguousarray(a.reshape(-1, 4))
+
+
+class TwoTierFilter:
+    """bloom.rs:13-131 on one GPU.
+
+    insert_kmer256 keeps the reference's sequential semantics: a batch is applied as if
+    its keys were inserted one after the other, after all earlier batches."""
+
+    def __init__(self, capacity_bits, device=0):
+        h = ctypes.c_void_p()
+        st = lib().kc_filter_create(ctypes.byref(h), device, int(capacity_bits))
+        if st != _abi.KC_OK:
+            raise KcError(st, "kc_filter_create failed")
+        self._h = h
+        self.num_words = lib().kc_filter_num_words(h)
+        self.size_bits = self.num_words * 64
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kc_filter_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def memory_usage_bytes(self):
+        """bloom.rs:129-131."""
+        return lib().kc_filter_memory_usage_bytes(self._h)
+
+    def insert_kmer256_many(self, keys4):
+        k4 = _keys4(keys4)
+        prom = np.zeros(k4.shape[0], dtype=np.uint8)
+        _check(lib().kc_filter_insert_kmer256(self._h, ptr(k4, u64p), k4.shape[0], ptr(prom, u8p)),
+               self._h, lib().kc_filter_last_error)
+        return prom.astype(bool)
+
+    def is_solid_kmer256_many(self, keys4):
+        k4 = _keys4(keys4)
+        out = np.zeros(k4.shape[0], dtype=np.uint8)
+        _check(lib().kc_filter_is_solid_kmer256(self._h, ptr(k4, u64p), k4.shape[0], ptr(out, u8p)),
+               self._h, lib().kc_filter_last_error)
+        return out.astype(bool)
+
+    def insert_kmer256(self, limbs):
+        """bloom.rs:83-107."""
+        return bool(self.insert_kmer256_many(np.array([limbs], dtype=np.uint64))[0])
+
+    def is_solid_kmer256(self, limbs):
+        """bloom.rs:111-126."""
+        return bool(self.is_solid_kmer256_many(np.array([limbs], dtype=np.uint64))[0])
+
+    def insert(self, kmer):
+        """bloom.rs:48-50: Kmer256(kmer as u128, 0)."""
+        return self.insert_kmer256((int(kmer), 0, 0, 0))
+
+    def is_solid(self, kmer):
+        """bloom.rs:54-56."""
+        return self.is_solid_kmer256((int(kmer), 0, 0, 0))
+
+    def export(self):
+        seen = np.zeros(self.num_words, dtype=np.uint64)
+        solid = np.zeros(self.num_words, dtype=np.uint64)
+        _check(lib().kc_filter_export(self._h, ptr(seen, u64p), ptr(solid, u64p), self.num_words),
+               self._h, lib().kc_filter_last_error)
+        return seen, solid
+
+
+class ErrorCorrector:
+    """hammer.rs:20-144: single-base correction of reads against a TwoTierFilter."""
+
+    def __init__(self, k):
+        self.k = int(k)
+
+    def correct_packed(self, packed, filt):
+        """Correct every read of a PackedReads store; returns (corrected data, n_corrected)."""
+        data, offs, lens = packed.arrays()
+        out = np.zeros(max(data.shape[0], 1), dtype=np.uint8)
+        n_corr = ctypes.c_uint64(0)
+        d = data if data.shape[0] else np.zeros(1, np.uint8)
+        _check(lib().kc_filter_correct_reads(filt._h, self.k, ptr(d, u8p), data.shape[0], ptr(offs, u64p),
+                                             ptr(lens, u32p), lens.shape[0], ptr(out, u8p), ctypes.byref(n_corr)),
+               filt._h, lib().kc_filter_last_error)
+        return out[: data.shape[0]], int(n_corr.value)
+
+    def correct_reads(self, reads, filt):
+        """hammer.rs:123-143: (corrected ASCII reads, number of reads corrected)."""
+        from .packed_reads import PackedReads
+        store = PackedReads()
+        for r in reads:
+            store.pack_clean(r)
+        data, n = self.correct_packed(store, filt)
+        fixed = PackedReads.from_arrays(data, store.offsets, store.lengths)
+        return [fixed.get_read(i) for i in range(len(fixed))], n
+
+    def correct_read_in_place(self, seq, filt):
+        """hammer.rs:34-120 for one ASCII read: returns (modified, corrected read)."""
+        out, n = self.correct_reads([seq], filt)
+        return n > 0, out[0]
+
+
+@dataclass
+class AssemblerConfig:
+    """The AssemblerConfig fields read by Stage 2-3 (assemble.rs:35-103 defaults)."""
+    k: int = 31
+    min_coverage: float = 5.0
+    bloom_bits: int = 512 * 1024 * 1024
+    error_correct: bool = False
+
+
+@dataclass
+class KmerSpectrum:
+    """Products of Stage 2-3: both Bloom tiers, the sorted count table (keys as Kmer256
+    limbs), the cutoff and the solid set (assemble.rs:286-292)."""
+    k: int
+    seen: np.ndarray
+    solid_bits: np.ndarray
+    keys: np.ndarray
+    counts: np.ndarray
+    solid_mask: np.ndarray
+    cutoff: int
+    info: dict
+
+    @property
+    def solid_kmers(self):
+        return self.keys[self.solid_mask.astype(bool)]
+
+
+class SpectrumEngine:
+    """One engine context: reads stay resident in HBM across runs."""
+
+    def __init__(self, config, device=0, table_load=0.0, timing=False):
+        cfg = config
+        if not 1 <= cfg.k <= _abi.KC_MAX_K:
+            raise KcError(_abi.KC_EUNSUPPORTED if cfg.k > _abi.KC_MAX_K else _abi.KC_EINVAL,
+                          f"k={cfg.k} outside 1..{_abi.KC_MAX_K}")
+        o = _abi.kc_opts(device=device, k=cfg.k, bloom_bits=int(cfg.bloom_bits), min_coverage=float(cfg.min_coverage),
+                         error_correct=int(bool(cfg.error_correct)), table_load=float(table_load),
+                         flags=_abi.KC_FLAG_TIMING if timing else 0)
+        h = ctypes.c_void_p()
+        st = lib().kc_create(ctypes.byref(h), ctypes.byref(o))
+        if st != _abi.KC_OK:
+            raise KcError(st, "kc_create failed")
+        self._h = h
+        self.config = cfg
+        self._data_len = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _chk(self, st):
+        _check(st, self._h, lib().kc_last_error)
+
+    def upload(self, packed):
+        data, offs, lens = packed.arrays()
+        d = data if data.shape[0] else np.zeros(1, np.uint8)
+        self._chk(lib().kc_reads_upload(self._h, ptr(d, u8p), data.shape[0], ptr(offs, u64p), ptr(lens, u32p),
+                                        lens.shape[0]))
+        self._data_len = data.shape[0]
+
+    def bloom_build(self):
+        self._chk(lib().kc_bloom_build(self._h))
+
+    def count(self):
+        info = _abi.kc_info()
+        self._chk(lib().kc_count(self._h, ctypes.byref(info)))
+        return info.as_dict()
+
+    def run(self):
+        info = _abi.kc_info()
+        self._chk(lib().kc_run(self._h, ctypes.byref(info)))
+        return info.as_dict()
+
+    def info(self):
+        info = _abi.kc_info()
+        self._chk(lib().kc_info_get(self._h, ctypes.byref(info)))
+        return info.as_dict()
+
+    def bloom(self):
+        nw = self.info()["num_words"]
+        seen = np.zeros(nw, dtype=np.uint64)
+        solid = np.zeros(nw, dtype=np.uint64)
+        self._chk(lib().kc_bloom_export(self._h, ptr(seen, u64p), ptr(solid, u64p), nw))
+        return seen, solid
+
+    def query(self, keys4):
+        k4 = _keys4(keys4)
+        out = np.zeros(k4.shape[0], dtype=np.uint8)
+        self._chk(lib().kc_bloom_query(self._h, ptr(k4, u64p), k4.shape[0], ptr(out, u8p)))
+        return out.astype(bool)
+
+    def counts(self):
+        n = self.info()["n_counts"]
+        keys = np.zeros((max(n, 1), 4), dtype=np.uint64)
+        cnt = np.zeros(max(n, 1), dtype=np.uint32)
+        mask = np.zeros(max(n, 1), dtype=np.uint8)
+        self._chk(lib().kc_counts_export(self._h, ptr(keys, u64p), ptr(cnt, u32p), ptr(mask, u8p), n))
+        return keys[:n], cnt[:n], mask[:n]
+
+    def reads(self):
+        out = np.zeros(max(self._data_len, 1), dtype=np.uint8)
+        self._chk(lib().kc_reads_export(self._h, ptr(out, u8p), self._data_len))
+        return out[: self._data_len]
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_double * 32)()
+        launches = (ctypes.c_uint32 * 32)()
+        n = lib().kc_kernel_times(self._h, names, ms, launches, 32)
+        return {names[i].decode(): (ms[i], launches[i]) for i in range(max(n, 0))}
+
+
+def count_kmers(packed, config, device=0):
+    """Stage 2 + Stage 3 of run_assembly_with_packed_reads (assemble.rs:129-298) on the GPU."""
+    eng = SpectrumEngine(config, device=device)
+    try:
+        eng.upload(packed)
+        info = eng.run()
+        seen, solid = eng.bloom()
+        keys, cnt, mask = eng.counts()
+        return KmerSpectrum(k=config.k, seen=seen, solid_bits=solid, keys=keys, counts=cnt, solid_mask=mask,
+                            cutoff=info["cutoff"], info=info)
+    finally:
+        eng.close()

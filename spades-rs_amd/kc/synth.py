@@ -1,0 +1,94 @@
+"""Seeded synthetic read sets in the PackedReads layout.
+
+Workloads (all seeded with numpy's PCG64 so a seed fixes every base):
+  uniform_genome_pairs  — a random genome (independent bases, given GC fraction),
+                          paired reads from fragments of length ~ N(insert_mean,
+                          insert_sd), R2 the reverse complement of the fragment's
+                          last read_len bases, uniform or position-dependent
+                          substitution errors (the substituted base is uniform over
+                          the other three), Phred qualities.
+The store is laid out as the reference builds it from paired files: all R1 mates,
+then all R2 mates (packed_reads.rs:481-489).
+"""
+import numpy as np
+
+from .packed_reads import PackedReads, pack_fixed_length
+
+
+def random_genome(rng, size, gc=0.5):
+    """Independent bases with P(G)=P(C)=gc/2, P(A)=P(T)=(1-gc)/2; codes A0 C1 G2 T3."""
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    if abs(gc - 0.5) < 1e-12:
+        return rng.integers(0, 4, size, dtype=np.uint8)
+    return rng.choice(4, size=size, p=p).astype(np.uint8)
+
+
+def error_profile(read_len, p_start, p_end=None):
+    """Per-position substitution probability: constant, or linear from p_start to p_end."""
+    if p_end is None:
+        return np.full(read_len, p_start, dtype=np.float64)
+    return np.linspace(p_start, p_end, read_len)
+
+
+def _mutate(rng, codes, perr):
+    mask = rng.random(codes.shape) < perr[None, :]
+    sub = rng.integers(1, 4, codes.shape, dtype=np.uint8)
+    return np.where(mask, (codes + sub) & 3, codes).astype(np.uint8)
+
+
+def uniform_genome_pairs(seed, genome_size, n_pairs, read_len=150, insert_mean=300.0, insert_sd=30.0,
+                         p_start=0.005, p_end=None, gc=0.5, qual=(30, 40), chunk=1 << 17, genome=None,
+                         with_quals=False):
+    """Returns (PackedReads, meta).  meta holds the genome and, with_quals, the Phred
+    quality sum of every read (R1 block then R2 block)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if genome is None:
+        genome = random_genome(rng, genome_size, gc)
+    G = genome.shape[0]
+    perr = error_profile(read_len, p_start, p_end)
+    r1_parts, r2_parts, q1, q2 = [], [], [], []
+    ar = np.arange(read_len, dtype=np.int64)
+    for c0 in range(0, n_pairs, chunk):
+        m = min(chunk, n_pairs - c0)
+        frag = np.rint(rng.normal(insert_mean, insert_sd, m)).astype(np.int64)
+        frag = np.clip(frag, read_len, G)
+        start = (rng.random(m) * (G - frag + 1)).astype(np.int64)
+        r1 = genome[start[:, None] + ar[None, :]]
+        tail = start + frag - read_len
+        r2 = 3 - genome[tail[:, None] + ar[None, ::-1]]
+        r1 = _mutate(rng, r1, perr)
+        r2 = _mutate(rng, r2, perr)
+        r1_parts.append(pack_fixed_length(r1)[0])
+        r2_parts.append(pack_fixed_length(r2)[0])
+        if with_quals:
+            if p_end is None:
+                lo, hi = qual
+                q1.append(rng.integers(lo, hi + 1, (m, read_len)).sum(axis=1))
+                q2.append(rng.integers(lo, hi + 1, (m, read_len)).sum(axis=1))
+            else:
+                qpos = np.rint(-10.0 * np.log10(perr)).astype(np.int64)
+                q1.append(np.full(m, qpos.sum()))
+                q2.append(np.full(m, qpos.sum()))
+    nb = (read_len + 3) // 4
+    data = np.concatenate(r1_parts + r2_parts) if n_pairs else np.zeros(0, np.uint8)
+    offsets = np.arange(2 * n_pairs, dtype=np.uint64) * np.uint64(nb)
+    lengths = np.full(2 * n_pairs, read_len, dtype=np.uint32)
+    store = PackedReads.from_arrays(data, offsets, lengths, pe_count=n_pairs)
+    meta = {"genome": genome, "seed": seed, "n_pairs": n_pairs, "read_len": read_len}
+    if with_quals:
+        meta["qual_sum"] = np.concatenate(q1 + q2) if n_pairs else np.zeros(0, np.int64)
+    return store, meta
+
+
+def workload_1mb_30x(seed=1, scale=1.0):
+    """1 Mb uniform genome, 2x150 bp pairs at 30x (100,000 pairs), insert N(300, 30),
+    0.5% uniform substitutions, Phred 30-40; `scale` shrinks genome and pairs together."""
+    g = max(int(1_000_000 * scale), 2000)
+    return uniform_genome_pairs(seed, g, max(int(100_000 * scale), 1), p_start=0.005)
+
+
+def workload_5mb_100x(seed=2, scale=1.0):
+    """5 Mb genome at 50% GC, 2x150 bp pairs at 100x (1,666,667 pairs), substitution rate
+    rising linearly 0.1% -> 1.0% along each read with matching Phred qualities."""
+    g = max(int(5_000_000 * scale), 2000)
+    return uniform_genome_pairs(seed, g, max(int(1_666_667 * scale), 1), p_start=0.001, p_end=0.010)

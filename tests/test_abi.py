@@ -1,0 +1,45 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry point
+include/kc.h declares; the ctypes binding covers exactly that set."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "kc.h")) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    from kc import _abi
+    lib = _abi.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_abi.EXPORTS) == names
+
+
+def test_struct_layouts_match_header():
+    from kc import _abi
+    # kc_opts: int32 device, uint32 k, uint64 bloom_bits, double, int32, float, uint32
+    assert ctypes.sizeof(_abi.kc_opts) == 40
+    assert _abi.kc_opts.bloom_bits.offset == 8 and _abi.kc_opts.min_coverage.offset == 16
+    assert ctypes.sizeof(_abi.kc_info) == 8 * 8 + 4 * 2 + 8 * 2
+
+
+def test_no_device_is_an_error_not_a_crash():
+    """Without a GPU the constructors return a status (no fallback path exists)."""
+    import torch
+    from kc import _abi
+    if torch.cuda.is_available():
+        return
+    h = ctypes.c_void_p()
+    o = _abi.kc_opts(device=0, k=21, bloom_bits=1 << 20, min_coverage=5.0)
+    assert _abi.lib().kc_create(ctypes.byref(h), ctypes.byref(o)) != 0
+    assert _abi.lib().kc_filter_create(ctypes.byref(h), 0, 1 << 20) != 0
+    assert _abi.lib().kc_create(ctypes.byref(h), None) == _abi.KC_EINVAL

@@ -137,12 +137,20 @@ __device__ __forceinline__ bool bloom_is_solid(const Kv& v, const uint64_t* __re
 // goes from EMPTY to its final value exactly once, so all inserters of one key take the
 // same decision at every slot and the key lands in exactly one slot without locks.
 
+// A slot is 16 bytes (one key word, 32-bit events) or 32 bytes, so a 64-byte line holds
+// a bucket of SPB = 4 or 2 whole slots.  Probing scans a bucket in slot order, then
+// moves to the next bucket.
 template <int NW, typename Ev>
-struct Slot {
+struct alignas((NW == 1 && sizeof(Ev) == 4) ? 16 : 32) Slot {
     uint64_t k[NW];
     Ev t0;         // smallest event index (global window number) of this key
     uint32_t cnt;  // occurrences
 };
+
+template <int NW, typename Ev>
+constexpr uint32_t slots_per_bucket() {
+    return 64 / sizeof(Slot<NW, Ev>);
+}
 
 template <int NW>
 __device__ __forceinline__ void pack_key(const Kv& v, uint64_t key[NW]) {
@@ -181,20 +189,32 @@ __device__ __forceinline__ T ld_relaxed(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Insert one occurrence (event t) of key.  Returns false when max_probe slots were
-// inspected without finding or claiming the key (the caller flags table overflow).
-template <int NW, typename Ev, bool TRACK_T0>
-__device__ __forceinline__ bool table_insert(Slot<NW, Ev>* __restrict__ tab, uint64_t cap, const uint64_t key[NW],
-                                             Ev t, uint32_t max_probe) {
-    uint64_t i = __umul64hi(table_hash<NW>(key), cap);
-    for (uint32_t probe = 0; probe < max_probe; ++probe) {
-        Slot<NW, Ev>* s = tab + i;
+// First key word of every slot of bucket b (one 64-byte line).
+template <int NW, typename Ev>
+__device__ __forceinline__ void bucket_load(const Slot<NW, Ev>* __restrict__ tab, uint64_t b,
+                                            uint64_t w0[slots_per_bucket<NW, Ev>()]) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+#pragma unroll
+    for (uint32_t s = 0; s < SPB; ++s) w0[s] = ld_relaxed(&tab[b * SPB + s].k[0]);
+}
+
+// Try to find or claim key in bucket b given the bucket's first key words w0 (possibly
+// stale: a stale word is EMPTY, and the CAS then returns the current value).
+// Returns the slot index, or ~0 when the bucket holds SPB other keys.
+template <int NW, typename Ev>
+__device__ __forceinline__ uint64_t bucket_find(Slot<NW, Ev>* __restrict__ tab, uint64_t b,
+                                                const uint64_t w0[slots_per_bucket<NW, Ev>()],
+                                                const uint64_t key[NW]) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+#pragma unroll
+    for (uint32_t s = 0; s < SPB; ++s) {
+        Slot<NW, Ev>* sl = tab + b * SPB + s;
         bool match = true;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            uint64_t cur = ld_relaxed(&s->k[w]);
+            uint64_t cur = w == 0 ? w0[s] : ld_relaxed(&sl->k[w]);
             if (cur == EMPTY_KEY) {
-                cur = atomicCAS(reinterpret_cast<unsigned long long*>(&s->k[w]), (unsigned long long)EMPTY_KEY,
+                cur = atomicCAS(reinterpret_cast<unsigned long long*>(&sl->k[w]), (unsigned long long)EMPTY_KEY,
                                 (unsigned long long)key[w]);
                 if (cur == EMPTY_KEY) cur = key[w];
             }
@@ -203,16 +223,41 @@ __device__ __forceinline__ bool table_insert(Slot<NW, Ev>* __restrict__ tab, uin
                 break;
             }
         }
-        if (match) {
-            atomicAdd(&s->cnt, 1u);
-            if (TRACK_T0) {
-                if (t < ld_relaxed(&s->t0)) atomicMin(&s->t0, t);
-            }
+        if (match) return b * SPB + s;
+    }
+    return ~0ull;
+}
+
+// Record one occurrence (event t) in slot i.
+template <int NW, typename Ev, bool TRACK_T0>
+__device__ __forceinline__ void slot_hit(Slot<NW, Ev>* __restrict__ tab, uint64_t i, Ev t) {
+    atomicAdd(&tab[i].cnt, 1u);
+    if (TRACK_T0) {
+        if (t < ld_relaxed(&tab[i].t0)) atomicMin(&tab[i].t0, t);
+    }
+}
+
+// Insert one occurrence of key, starting at bucket b whose first key words are w0.
+// Returns false when max_probe buckets were inspected without finding or claiming it.
+template <int NW, typename Ev, bool TRACK_T0>
+__device__ __forceinline__ bool table_insert_from(Slot<NW, Ev>* __restrict__ tab, uint64_t nbuckets, uint64_t b,
+                                                  uint64_t w0[slots_per_bucket<NW, Ev>()], const uint64_t key[NW],
+                                                  Ev t, uint32_t max_probe) {
+    for (uint32_t probe = 0; probe < max_probe; ++probe) {
+        const uint64_t i = bucket_find<NW, Ev>(tab, b, w0, key);
+        if (i != ~0ull) {
+            slot_hit<NW, Ev, TRACK_T0>(tab, i, t);
             return true;
         }
-        i = (i + 1 == cap) ? 0 : i + 1;
+        b = (b + 1 == nbuckets) ? 0 : b + 1;
+        bucket_load<NW, Ev>(tab, b, w0);
     }
     return false;
+}
+
+template <int NW>
+__device__ __forceinline__ uint64_t home_bucket(const uint64_t key[NW], uint64_t nbuckets) {
+    return __umul64hi(table_hash<NW>(key), nbuckets);
 }
 
 // Wave-aggregated append: returns a unique index for every lane with pred set.

@@ -141,6 +141,19 @@ __global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data,
         if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
 }
 
+// Sum of a per-thread value over the block, added to *total with one atomic per block.
+__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* total) {
+    __shared__ unsigned long long s_part[BLOCK / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += s_part[w];
+        if (t) atomicAdd(total, t);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // table passes
 // ---------------------------------------------------------------------------
@@ -158,24 +171,59 @@ __global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t cap) {
 }
 
 // Stage 2 insert (TRACK_T0, every window) or Stage 3 recount after Hammer (SOLID_ONLY:
-// only windows whose canonical k-mer is Bloom-solid, assemble.rs:196).
+// only windows whose canonical k-mer is Bloom-solid, assemble.rs:196).  Each thread
+// takes WPT windows of the tile and issues their first bucket loads together, so a wave
+// keeps WPT independent random table accesses in flight.
 template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY>
 __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                   const uint64_t* __restrict__ act_off,
                                                   const uint32_t* __restrict__ tile_first, uint64_t n_active,
                                                   uint64_t n_occ, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
-                                                  uint64_t cap, uint32_t max_probe, const uint64_t* __restrict__ solid,
-                                                  Modulus M, uint32_t* __restrict__ overflow) {
-    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
+                                                  uint64_t nbuckets, uint32_t max_probe,
+                                                  const uint64_t* __restrict__ solid, Modulus M,
+                                                  uint32_t* __restrict__ overflow) {
     constexpr int NW = WIDE ? 2 : 1;
-    tile_windows<WIDE>(blockIdx.x, data, act_ev, act_off, tile_first, n_active, n_occ, k, s_ev,
-                       [&](const Kv& v, uint64_t t) {
-                           const Kv c = canonical<WIDE>(v, k);
-                           if (SOLID_ONLY && !bloom_is_solid(c, solid, M)) return;
-                           uint64_t key[NW];
-                           pack_key<NW>(c, key);
-                           if (!table_insert<NW, Ev, TRACK_T0>(tab, cap, key, (Ev)t, max_probe)) atomicOr(overflow, 1u);
-                       });
+    constexpr uint32_t WPT = TILE_WINDOWS / BLOCK;
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
+    const uint64_t tile = blockIdx.x;
+    const uint64_t a0 = tile_first[tile];
+    const uint64_t w0 = tile * TILE_WINDOWS;
+    const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+    const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+    __syncthreads();
+    uint64_t key[WPT][NW], bkt[WPT], pre[WPT][SPB];
+    bool valid[WPT];
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j) {
+        const uint64_t w = w0 + j * BLOCK + threadIdx.x;
+        valid[j] = w < w1;
+        if (!valid[j]) continue;
+        uint32_t lo = 0, hi = na - 1;  // largest i with s_ev[i] <= w
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_ev[mid] <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const Kv c = canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
+        if (SOLID_ONLY && !bloom_is_solid(c, solid, M)) {
+            valid[j] = false;
+            continue;
+        }
+        pack_key<NW>(c, key[j]);
+        bkt[j] = home_bucket<NW>(key[j], nbuckets);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j)
+        if (valid[j]) bucket_load<NW, Ev>(tab, bkt[j], pre[j]);
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j) {
+        if (!valid[j]) continue;
+        const Ev t = (Ev)(w0 + j * BLOCK + threadIdx.x);
+        if (!table_insert_from<NW, Ev, TRACK_T0>(tab, nbuckets, bkt[j], pre[j], key[j], t, max_probe))
+            atomicOr(overflow, 1u);
+    }
 }
 
 // fs[p] = min(4*t0 + j) over the distinct k-mers' bits; also counts occupied slots.
@@ -194,9 +242,7 @@ __global__ void k_first_setter(const Slot<NW, Ev>* __restrict__ tab, uint64_t ca
 #pragma unroll
         for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
     }
-    // one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if (__lane_id() == 0 && local) atomicAdd(n_distinct, (unsigned long long)local);
+    block_add(local, n_distinct);
 }
 
 // seen[w] bit b = (fs[64w + b] is finite); clears the solid tier.  One wave per word.
@@ -240,27 +286,54 @@ __global__ void k_promote(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap, co
 }
 
 // Gather the count table: occupied slots whose k-mer is Bloom-solid (or every occupied
-// slot when the table was built from Bloom-solid windows only).
+// slot when the table was built from Bloom-solid windows only).  Each block takes chunks
+// of BLOCK * COMPACT_ITEMS slots and reserves its output range with one atomic.
+constexpr uint32_t COMPACT_ITEMS = 16;
+
 template <int NW, typename Ev, bool CHECK_SOLID>
-__global__ void k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap, const uint64_t* __restrict__ solid,
-                          Modulus M, uint64_t* __restrict__ out_a, uint64_t* __restrict__ out_b,
-                          uint32_t* __restrict__ out_cnt, unsigned long long* __restrict__ counter) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t start = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    const uint64_t span = (cap + 63) / 64 * 64;
-    for (uint64_t i = start; i < span; i += stride) {  // span is a multiple of 64: whole waves agree
-        bool take = false;
-        Slot<NW, Ev> s;
-        if (i < cap) {
-            s = tab[i];
-            if (s.k[0] != EMPTY_KEY) take = CHECK_SOLID ? bloom_is_solid(unpack_key<NW>(s.k), solid, M) : true;
+__global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap,
+                                                   const uint64_t* __restrict__ solid, Modulus M,
+                                                   uint64_t* __restrict__ out_a, uint64_t* __restrict__ out_b,
+                                                   uint32_t* __restrict__ out_cnt,
+                                                   unsigned long long* __restrict__ counter) {
+    __shared__ uint32_t s_scan[BLOCK];
+    __shared__ unsigned long long s_base;
+    const uint64_t chunk = (uint64_t)BLOCK * COMPACT_ITEMS;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t c0 = blockIdx.x * chunk; c0 < cap; c0 += (uint64_t)gridDim.x * chunk) {
+        uint32_t bits = 0, cnt = 0;
+#pragma unroll 4
+        for (uint32_t j = 0; j < COMPACT_ITEMS; ++j) {
+            const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
+            if (i >= cap) break;
+            const Slot<NW, Ev> sl = tab[i];
+            bool take = sl.k[0] != EMPTY_KEY;
+            if (CHECK_SOLID && take) take = bloom_is_solid(unpack_key<NW>(sl.k), solid, M);
+            bits |= (uint32_t)take << j;
+            cnt += take;
         }
-        const uint64_t idx = wave_append(take, counter);
-        if (take) {
-            out_a[idx] = s.k[0];
-            if (NW == 2) out_b[idx] = s.k[NW - 1];
-            out_cnt[idx] = s.cnt;
+        s_scan[tid] = cnt;
+        __syncthreads();
+        for (uint32_t off = 1; off < BLOCK; off <<= 1) {
+            const uint32_t v = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
         }
+        const uint32_t total = s_scan[BLOCK - 1];
+        if (tid == 0) s_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+        __syncthreads();
+        uint64_t o = s_base + s_scan[tid] - cnt;
+        while (bits) {
+            const uint32_t j = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const Slot<NW, Ev> sl = tab[c0 + (uint64_t)j * BLOCK + tid];
+            out_a[o] = sl.k[0];
+            if (NW == 2) out_b[o] = sl.k[NW - 1];
+            out_cnt[o] = sl.cnt;
+            ++o;
+        }
+        __syncthreads();
     }
 }
 
@@ -302,8 +375,7 @@ __global__ void k_solid_flags(const uint32_t* __restrict__ cnt, uint64_t n, uint
         f[i] = s;
         local += s;
     }
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if (__lane_id() == 0 && local) atomicAdd(n_solid, (unsigned long long)local);
+    block_add(local, n_solid);
 }
 
 // ---------------------------------------------------------------------------
@@ -690,6 +762,8 @@ struct Run {
         const double load = (c->o.table_load > 0.05f && c->o.table_load < 0.95f) ? c->o.table_load : 0.5;
         uint64_t cap = (uint64_t)std::ceil((double)distinct_est * 1.10 / load) + 1024;
         cap = std::min<uint64_t>(cap, c->n_occ * 2 + 1024);
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        cap = (cap + SPB - 1) / SPB * SPB;
         c->cap = cap;
         KC_TRY(c->d_tab.ensure(cap * sizeof(Slot<NW, Ev>)));
         return KC_OK;
@@ -712,7 +786,8 @@ struct Run {
             TIMED(KID_INSERT, (k_insert<WIDE, Ev, true, false><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
                                   c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
                                   c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ, c->o.k,
-                                  c->d_tab.as<Slot<NW, Ev>>(), c->cap, (uint32_t)std::min<uint64_t>(c->cap, 1u << 20),
+                                  c->d_tab.as<Slot<NW, Ev>>(), c->cap / slots_per_bucket<NW, Ev>(),
+                                  (uint32_t)std::min<uint64_t>(c->cap / slots_per_bucket<NW, Ev>(), 1u << 20),
                                   nullptr, c->M, c->d_counters.as<uint32_t>())));
             KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 8, hipMemcpyDeviceToHost, st));
             KC_TRY(hipStreamSynchronize(st));
@@ -819,7 +894,7 @@ struct Run {
             // recount Bloom-solid windows of the corrected reads into a cleared table
             for (int attempt = 0;; ++attempt) {
                 if (attempt) {
-                    const uint64_t cap2 = c->cap * 2;
+                    const uint64_t cap2 = c->cap * 2;  // a multiple of every bucket size
                     KC_TRY(c->d_tab.ensure(cap2 * sizeof(Slot<NW, uint32_t>)));
                     c->cap = cap2;
                 }
@@ -831,8 +906,10 @@ struct Run {
                     TIMED(KID_RECOUNT, (k_insert<WIDE, uint32_t, false, true><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
                                            c->d_corr.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
                                            c->d_act_off.as<uint64_t>(), c->d_tile_first.as<uint32_t>(), c->n_active,
-                                           c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(), c->cap,
-                                           (uint32_t)std::min<uint64_t>(c->cap, 1u << 20), c->d_solid.as<uint64_t>(),
+                                           c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(),
+                                           c->cap / slots_per_bucket<NW, uint32_t>(),
+                                           (uint32_t)std::min<uint64_t>(c->cap / slots_per_bucket<NW, uint32_t>(), 1u << 20),
+                                           c->d_solid.as<uint64_t>(),
                                            c->M, c->d_counters.as<uint32_t>())));
                 }
                 KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 32, hipMemcpyDeviceToHost, st));

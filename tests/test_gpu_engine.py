@@ -202,15 +202,23 @@ def test_spectrum_unaligned_offsets(oracle):
 
 
 def test_ecoli_1k_fixture(oracle):
-    # tests/test_pipeline.rs:33-91 configuration: k=31, 2^24 bits, Hammer on
+    # tests/test_pipeline.rs:33-91 configuration: k=31, 2^24 bits, Hammer on.  The reads
+    # carry no sequencing errors (the oracle corrects none either); every 31-mer of the
+    # 1 kb reference must be in the solid set.
+    import gzip
     import os
     from conftest import GOLDEN
+    from oracle import hash_np as H
     r1 = read_fastq_sequences(os.path.join(GOLDEN, "ecoli_1K_1.fq.gz"))
     r2 = read_fastq_sequences(os.path.join(GOLDEN, "ecoli_1K_2.fq.gz"))
     store = PackedReads.from_paired_slices(r1, r2)
-    info = assert_parity(oracle, store, 31, 16 * 1024 * 1024, ec=True)
-    assert info["corrected_reads"] > 0
-    assert_parity(oracle, store, 31, 16 * 1024 * 1024, ec=False)
+    assert assert_parity(oracle, store, 31, 16 * 1024 * 1024, ec=True)["corrected_reads"] == 0
+    info, _, _, keys, cnt, mask, _ = gpu_spectrum(store, 31, 16 * 1024 * 1024)
+    with gzip.open(os.path.join(GOLDEN, "reference_1K.fa.gz"), "rb") as fh:
+        ref = b"".join(ln.strip() for ln in fh if not ln.startswith(b">"))
+    solid = {tuple(int(x) for x in w) for w in keys[mask.astype(bool)]}
+    genome = {H.canonical_words_of_string(ref[i:i + 31]) for i in range(len(ref) - 30)}
+    assert len(ref) == 1000 and genome <= solid
 
 
 def test_spectrum_1mb_30x_full_size(oracle):

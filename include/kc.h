@@ -130,6 +130,26 @@ int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* so
  * as uploaded; data_len must equal the uploaded data_len. */
 int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len);
 
+/* ---- Several libraries in one context ------------------------------------
+ * A context may hold many independent libraries (read sets).  Each is counted exactly as
+ * if it were alone in its own context (own filter, event order, table, cutoff and solid
+ * set), but every stage of kc_run processes all of them with the same kernel launches.
+ * kc_reads_upload replaces all libraries by one (library 0, with the context's
+ * bloom_bits and min_coverage); the functions without a library argument act on library 0. */
+
+/* Append a library; *lib receives its index.  The reads are copied (borrowed only during
+ * the call).  bloom_bits / min_coverage as in kc_opts, for this library. */
+int kc_library_add(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets,
+                   const uint32_t* lengths, size_t n_reads, uint64_t bloom_bits, double min_coverage,
+                   uint32_t* lib);
+int kc_library_count(const kc_ctx* ctx, uint32_t* n);
+int kc_library_info(const kc_ctx* ctx, uint32_t lib, kc_info* info);
+int kc_library_bloom_export(kc_ctx* ctx, uint32_t lib, uint64_t* seen, uint64_t* solid, size_t num_words);
+int kc_library_bloom_query(kc_ctx* ctx, uint32_t lib, const uint64_t* keys4, size_t n, uint8_t* out);
+int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask,
+                             size_t n);
+int kc_library_reads_export(kc_ctx* ctx, uint32_t lib, uint8_t* data, size_t data_len);
+
 /* Per-kernel device time of the last run (KC_FLAG_TIMING): fills up to cap entries of
  * names (static strings), total ms and launch counts; returns the number of kernels. */
 int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap);

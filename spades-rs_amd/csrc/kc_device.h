@@ -239,17 +239,19 @@ __device__ __forceinline__ void slot_hit(Slot<NW, Ev>* __restrict__ tab, uint64_
 
 // Insert one occurrence of key, starting at bucket b whose first key words are w0.
 // Returns false when max_probe buckets were inspected without finding or claiming it.
+// The table region is buckets [b0, b0 + nbuckets); probing wraps inside it.
 template <int NW, typename Ev, bool TRACK_T0>
-__device__ __forceinline__ bool table_insert_from(Slot<NW, Ev>* __restrict__ tab, uint64_t nbuckets, uint64_t b,
-                                                  uint64_t w0[slots_per_bucket<NW, Ev>()], const uint64_t key[NW],
-                                                  Ev t, uint32_t max_probe) {
-    for (uint32_t probe = 0; probe < max_probe; ++probe) {
+__device__ __forceinline__ bool table_insert_from(Slot<NW, Ev>* __restrict__ tab, uint64_t b0, uint64_t nbuckets,
+                                                  uint64_t b, uint64_t w0[slots_per_bucket<NW, Ev>()],
+                                                  const uint64_t key[NW], Ev t, uint32_t max_probe) {
+    const uint64_t probes = nbuckets < max_probe ? nbuckets : max_probe;
+    for (uint64_t probe = 0; probe < probes; ++probe) {
         const uint64_t i = bucket_find<NW, Ev>(tab, b, w0, key);
         if (i != ~0ull) {
             slot_hit<NW, Ev, TRACK_T0>(tab, i, t);
             return true;
         }
-        b = (b + 1 == nbuckets) ? 0 : b + 1;
+        b = (b + 1 == b0 + nbuckets) ? b0 : b + 1;
         bucket_load<NW, Ev>(tab, b, w0);
     }
     return false;

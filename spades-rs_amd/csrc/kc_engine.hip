@@ -1,11 +1,13 @@
 // kc_engine.hip — gfx950 k-mer spectrum engine behind include/kc.h.
 //
 // Stage 2 (assemble.rs:129-161) and Stage 3 (assemble.rs:166-292) of spades-rs on one
-// MI355X.  The reference inserts every window into the two-tier filter in read order;
+// MI355X, for one or many independent libraries (read sets) processed by the same
+// launches.  The reference inserts every window into the two-tier filter in read order;
 // this engine computes the same filter state from the distinct canonical k-mers:
 //
 //   * every occurrence is inserted into a device hash table that keeps, per canonical
-//     k-mer, its occurrence count and its first event index t0 (global window number);
+//     k-mer, its occurrence count and its first event index t0 (window number within
+//     its library, read order then window order);
 //   * fs[p] = min over (k-mer x, hash j) with p_j(x) = p of 4*t0(x) + j is the first
 //     setter of Bloom bit p in the reference's sequential order (h1 -> h2 -> h3 within
 //     an insert, bloom.rs:95-97);
@@ -22,6 +24,10 @@
 // With error_correct, reads are first corrected against the frozen filter
 // (hammer.rs:34-120, one read per lane) and the Bloom-solid windows of the corrected
 // reads are counted in a second table pass.
+//
+// Libraries: every library has its own Bloom tiers, its own region of the table pool and
+// its own event numbering; windows of all libraries form one global window space cut
+// into tiles, and table scans find a slot's library from the region starts.
 
 #include <hip/hip_runtime.h>
 
@@ -38,19 +44,117 @@
 
 #include "../../include/kc.h"
 #include "kc_device.h"
+#include "kc_hammer.h"
+#include "kc_host.h"
 
 using namespace kc;
 
 namespace {
 
 constexpr uint32_t TILE_WINDOWS = 1024;  // windows per work tile
-constexpr uint32_t BLOCK = 256;
 constexpr uint32_t HLL_P = 12;
 constexpr uint32_t HLL_M = 1u << HLL_P;
-constexpr uint32_t HIST_BINS = 1024;     // count histogram bins; the last bin holds >= 1023
-constexpr uint32_t HAMMER_MAX_LEN = 1024;
-constexpr uint32_t HAMMER_LANE_WORDS = HAMMER_MAX_LEN / 16 + 6;  // packed read + over-read pad
-constexpr uint32_t DATA_PAD = 64;
+constexpr uint32_t HIST_BINS = 1024;  // count histogram bins; the last bin holds >= 1023
+constexpr uint32_t SCAN_ITEMS = 16;   // table slots per thread per chunk in table scans
+constexpr uint32_t LIB_LOCAL = 8;     // libraries a block aggregates in LDS per chunk
+
+// Per-library state the kernels read.
+struct LibDesc {
+    uint64_t ev0;       // first global window number of the library
+    uint64_t bucket0;   // first 64-byte bucket of its table region
+    uint64_t nbuckets;  // buckets in its table region
+    uint64_t word0;     // first word of its Bloom tiers in the seen/solid pools
+    uint64_t* seen;
+    uint64_t* solid;
+    void* fs;           // num_words * 64 first-setter entries (valid while its group runs)
+    Modulus M;
+    uint64_t num_words;
+};
+
+// Per-library results written on the device.
+struct LibOut {
+    unsigned long long n_distinct, n_counts, n_solid, corrected;
+    uint32_t overflow, cutoff, top, pad;
+};
+
+// How a count-table entry is laid out for the sort: a library index and the 2k-bit
+// k-mer value V as two words (hi, lo) that sort as (library, V), or one word when
+// everything fits in 64 bits.
+struct SortLayout {
+    uint32_t k, lib_bits, lo_bits, hi_bits;  // hi_bits == 0: one word
+    uint32_t single;                         // 1: (lib, V) packed in the lo word
+    uint32_t single_bits;
+};
+
+SortLayout make_layout(uint32_t k, uint32_t n_libs) {
+    SortLayout s{};
+    s.k = k;
+    s.lib_bits = 0;
+    while ((1u << s.lib_bits) < n_libs) ++s.lib_bits;
+    if (2 * k <= 63) {
+        s.lo_bits = 2 * k;
+        s.hi_bits = s.lib_bits;
+    } else {
+        s.lo_bits = 63;
+        s.hi_bits = 2 * k - 63 + s.lib_bits;
+    }
+    s.single = s.lo_bits + s.hi_bits <= 64;
+    s.single_bits = s.lo_bits + s.hi_bits;
+    return s;
+}
+
+__device__ __forceinline__ void encode_entry(const SortLayout& L, uint32_t lib, const Kv& v, uint64_t& hi,
+                                             uint64_t& lo) {
+    if (2 * L.k <= 63) {
+        lo = v.lo;
+        hi = lib;
+    } else {
+        lo = v.lo & 0x7FFFFFFFFFFFFFFFull;
+        hi = ((uint64_t)lib << (2 * L.k - 63)) | (v.hi << 1) | (v.lo >> 63);
+    }
+    if (L.single) {
+        lo = L.hi_bits ? ((hi << L.lo_bits) | lo) : lo;
+        hi = 0;
+    }
+}
+
+__device__ __host__ __forceinline__ uint32_t entry_lib(const SortLayout& L, uint64_t hi, uint64_t lo) {
+    if (L.lib_bits == 0) return 0;
+    if (L.single) return (uint32_t)(lo >> (L.lo_bits));
+    return (uint32_t)(hi >> (L.hi_bits - L.lib_bits));
+}
+
+// Largest l with starts[l] <= x (starts ascending, starts[0] == 0).
+__device__ __forceinline__ uint32_t find_region(const uint64_t* __restrict__ starts, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Block-level per-library accumulation for a chunk whose first library is lib0: counts
+// for lib0..lib0+LIB_LOCAL-1 are summed in LDS and flushed with one atomic each.
+struct LibAccum {
+    unsigned long long* s;  // LDS [LIB_LOCAL]
+    __device__ void reset() {
+        if (threadIdx.x < LIB_LOCAL) s[threadIdx.x] = 0;
+    }
+    __device__ void add(uint32_t lib, uint32_t lib0, unsigned long long v, unsigned long long* global_field,
+                        size_t stride_ull) {
+        if (lib - lib0 < LIB_LOCAL) atomicAdd(&s[lib - lib0], v);
+        else atomicAdd(global_field + (size_t)lib * stride_ull, v);
+    }
+    __device__ void flush(uint32_t lib0, uint32_t n_libs, unsigned long long* global_field, size_t stride_ull) {
+        if (threadIdx.x < LIB_LOCAL && lib0 + threadIdx.x < n_libs && s[threadIdx.x])
+            atomicAdd(global_field + (size_t)(lib0 + threadIdx.x) * stride_ull, s[threadIdx.x]);
+    }
+};
+
+constexpr size_t LIBOUT_STRIDE = sizeof(LibOut) / sizeof(unsigned long long);
+static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
 // read bookkeeping
@@ -67,91 +171,96 @@ __global__ void k_nwin(const uint32_t* __restrict__ lengths, uint64_t n, uint32_
     }
 }
 
-// For every active read: its first event index and byte offset, and the first active read
-// of every tile whose first window it holds.
+// For every active read: its first global window, byte offset and library, and the first
+// active read of every tile whose first window it holds.
 __global__ void k_active(const uint32_t* __restrict__ active, uint64_t n_active, const uint64_t* __restrict__ prefix,
                          const uint64_t* __restrict__ nwin, const uint64_t* __restrict__ offsets,
-                         uint64_t* __restrict__ act_ev, uint64_t* __restrict__ act_off, uint32_t* __restrict__ tile_first) {
+                         const uint32_t* __restrict__ read_lib, uint64_t* __restrict__ act_ev,
+                         uint64_t* __restrict__ act_off, uint32_t* __restrict__ act_lib,
+                         uint32_t* __restrict__ tile_first) {
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n_active;
          a += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t r = active[a];
         const uint64_t s = prefix[r], e = s + nwin[r];
         act_ev[a] = s;
         act_off[a] = offsets[r];
+        act_lib[a] = read_lib[r];
         for (uint64_t b = (s + TILE_WINDOWS - 1) / TILE_WINDOWS; b * TILE_WINDOWS < e; ++b) tile_first[b] = (uint32_t)a;
     }
 }
 
-// Iterates the windows of tile `tile`: calls f(v, event) with the forward k-mer value of
-// each window and its global event index.  All threads of the block must call it.
-template <bool WIDE, class F>
-__device__ __forceinline__ void tile_windows(uint64_t tile, const uint8_t* __restrict__ data,
-                                             const uint64_t* __restrict__ act_ev, const uint64_t* __restrict__ act_off,
-                                             const uint32_t* __restrict__ tile_first, uint64_t n_active,
-                                             uint64_t n_occ, uint32_t k, uint64_t* s_ev, F&& f) {
-    const uint64_t a0 = tile_first[tile];
-    const uint64_t w0 = tile * TILE_WINDOWS;
-    const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
-    const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
-    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
-    __syncthreads();
-    for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
-        uint32_t lo = 0, hi = na - 1;  // largest i with s_ev[i] <= w
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_ev[mid] <= w) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t s = (uint32_t)(w - s_ev[lo]);
-        const Kv v = load_window<WIDE>(data, act_off[a0 + lo], s, k);
-        f(v, w);
+// First global window of each library: prefix at its first read (or the total).
+__global__ void k_lib_ev0(const uint64_t* __restrict__ prefix, const uint64_t* __restrict__ nwin, uint64_t n_reads,
+                          const uint64_t* __restrict__ lib_read0, uint32_t n_libs, uint64_t* __restrict__ ev0) {
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l <= n_libs; l += gridDim.x * blockDim.x) {
+        const uint64_t r = lib_read0[l];
+        ev0[l] = r < n_reads ? prefix[r] : (n_reads ? prefix[n_reads - 1] + nwin[n_reads - 1] : 0);
     }
-    __syncthreads();
+}
+
+// Window w of tile: index of its active read within s_ev (largest i with s_ev[i] <= w).
+__device__ __forceinline__ uint32_t tile_read(const uint64_t* s_ev, uint32_t na, uint64_t w) {
+    uint32_t lo = 0, hi = na - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_ev[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // ---------------------------------------------------------------------------
-// distinct-count estimate (HyperLogLog, 2^12 registers) to size the hash table
+// distinct-count estimate per library (HyperLogLog, 2^12 registers) to size its table
 // ---------------------------------------------------------------------------
 
 template <bool WIDE>
 __global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                const uint64_t* __restrict__ act_off,
+                                               const uint32_t* __restrict__ act_lib,
                                                const uint32_t* __restrict__ tile_first, uint64_t n_active,
                                                uint64_t n_occ, uint64_t n_tiles, uint32_t k,
                                                uint32_t* __restrict__ regs) {
     __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
     __shared__ uint32_t s_reg[HLL_M];
-    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
-    __syncthreads();
     constexpr int NW = WIDE ? 2 : 1;
+    uint32_t cur = ~0u;  // library whose registers s_reg holds (block-uniform)
+    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        tile_windows<WIDE>(tile, data, act_ev, act_off, tile_first, n_active, n_occ, k, s_ev,
-                           [&](const Kv& v, uint64_t) {
-                               uint64_t key[NW];
-                               pack_key<NW>(canonical<WIDE>(v, k), key);
-                               const uint64_t h = table_hash<NW>(key);
-                               const uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
-                               const uint64_t w = h << HLL_P;
-                               const uint32_t rho = w ? (uint32_t)__clzll((long long)w) + 1 : 64 - HLL_P + 1;
-                               atomicMax(&s_reg[idx], rho);
-                           });
+        const uint64_t a0 = tile_first[tile];
+        const uint32_t lib0 = act_lib[a0];
+        if (lib0 != cur) {
+            __syncthreads();
+            if (cur != ~0u)
+                for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
+                    if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
+            cur = lib0;
+        }
+        const uint64_t w0 = tile * TILE_WINDOWS;
+        const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+        const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+        __syncthreads();
+        for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
+            const uint32_t lo = tile_read(s_ev, na, w);
+            const Kv v = load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k);
+            uint64_t key[NW];
+            pack_key<NW>(canonical<WIDE>(v, k), key);
+            const uint64_t h = table_hash<NW>(key);
+            const uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
+            const uint64_t hw = h << HLL_P;
+            const uint32_t rho = hw ? (uint32_t)__clzll((long long)hw) + 1 : 64 - HLL_P + 1;
+            const uint32_t lib = act_lib[a0 + lo];
+            if (lib == cur) atomicMax(&s_reg[idx], rho);
+            else atomicMax(&regs[(size_t)lib * HLL_M + idx], rho);
+        }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
-        if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
-}
-
-// Sum of a per-thread value over the block, added to *total with one atomic per block.
-__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* total) {
-    __shared__ unsigned long long s_part[BLOCK / 64];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += s_part[w];
-        if (t) atomicAdd(total, t);
-    }
+    if (cur != ~0u)
+        for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
+            if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -159,8 +268,9 @@ __device__ __forceinline__ void block_add(uint64_t v, unsigned long long* total)
 // ---------------------------------------------------------------------------
 
 template <int NW, typename Ev>
-__global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t cap) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+__global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t n_slots) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_slots;
+         i += (uint64_t)gridDim.x * blockDim.x) {
         Slot<NW, Ev> s;
 #pragma unroll
         for (int w = 0; w < NW; ++w) s.k[w] = EMPTY_KEY;
@@ -177,11 +287,11 @@ __global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t cap) {
 template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY>
 __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                   const uint64_t* __restrict__ act_off,
+                                                  const uint32_t* __restrict__ act_lib,
                                                   const uint32_t* __restrict__ tile_first, uint64_t n_active,
                                                   uint64_t n_occ, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
-                                                  uint64_t nbuckets, uint32_t max_probe,
-                                                  const uint64_t* __restrict__ solid, Modulus M,
-                                                  uint32_t* __restrict__ overflow) {
+                                                  const LibDesc* __restrict__ libs, uint32_t max_probe,
+                                                  LibOut* __restrict__ out) {
     constexpr int NW = WIDE ? 2 : 1;
     constexpr uint32_t WPT = TILE_WINDOWS / BLOCK;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
@@ -194,25 +304,22 @@ __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ da
     for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
     __syncthreads();
     uint64_t key[WPT][NW], bkt[WPT], pre[WPT][SPB];
+    uint32_t lib[WPT];
     bool valid[WPT];
 #pragma unroll
     for (uint32_t j = 0; j < WPT; ++j) {
         const uint64_t w = w0 + j * BLOCK + threadIdx.x;
         valid[j] = w < w1;
         if (!valid[j]) continue;
-        uint32_t lo = 0, hi = na - 1;  // largest i with s_ev[i] <= w
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_ev[mid] <= w) lo = mid;
-            else hi = mid - 1;
-        }
+        const uint32_t lo = tile_read(s_ev, na, w);
+        lib[j] = act_lib[a0 + lo];
         const Kv c = canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
-        if (SOLID_ONLY && !bloom_is_solid(c, solid, M)) {
+        if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
             valid[j] = false;
             continue;
         }
         pack_key<NW>(c, key[j]);
-        bkt[j] = home_bucket<NW>(key[j], nbuckets);
+        bkt[j] = libs[lib[j]].bucket0 + home_bucket<NW>(key[j], libs[lib[j]].nbuckets);
     }
 #pragma unroll
     for (uint32_t j = 0; j < WPT; ++j)
@@ -220,95 +327,147 @@ __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ da
 #pragma unroll
     for (uint32_t j = 0; j < WPT; ++j) {
         if (!valid[j]) continue;
-        const Ev t = (Ev)(w0 + j * BLOCK + threadIdx.x);
-        if (!table_insert_from<NW, Ev, TRACK_T0>(tab, nbuckets, bkt[j], pre[j], key[j], t, max_probe))
-            atomicOr(overflow, 1u);
+        const LibDesc& L = libs[lib[j]];
+        const Ev t = (Ev)(w0 + j * BLOCK + threadIdx.x - L.ev0);
+        if (!table_insert_from<NW, Ev, TRACK_T0>(tab, L.bucket0, L.nbuckets, bkt[j], pre[j], key[j], t, max_probe))
+            atomicOr(&out[lib[j]].overflow, 1u);
     }
 }
 
-// fs[p] = min(4*t0 + j) over the distinct k-mers' bits; also counts occupied slots.
+// Iterates the slots [s_begin, s_end) of the table pool in chunks of BLOCK * SCAN_ITEMS;
+// f(slot index, library, chunk's first library) for every occupied slot.
+template <int NW, typename Ev, class F, class G>
+__device__ __forceinline__ void scan_slots(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin, uint64_t s_end,
+                                           const uint64_t* __restrict__ lib_b0, uint32_t n_libs, F&& f,
+                                           G&& chunk_end) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
+    for (uint64_t c0 = s_begin + blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
+        const uint32_t lib0 = find_region(lib_b0, n_libs, c0 / SPB);
+        uint32_t lib = lib0;
+        uint64_t lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+        for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+            const uint64_t i = c0 + (uint64_t)j * BLOCK + threadIdx.x;
+            if (i >= s_end) break;
+            if (i >= lib_end) {
+                lib = find_region(lib_b0, n_libs, i / SPB);
+                lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+            }
+            f(i, lib, lib0);
+        }
+        chunk_end(lib0);
+    }
+}
+
+// fs[p] = min(4*t0 + j) over the distinct k-mers' bits; counts distinct k-mers per library.
 template <int NW, typename Ev>
-__global__ void k_first_setter(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap, Ev* __restrict__ fs, Modulus M,
-                               unsigned long long* __restrict__ n_distinct) {
-    uint64_t local = 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
-        const Slot<NW, Ev> s = tab[i];
-        if (s.k[0] == EMPTY_KEY) continue;
-        ++local;
-        const Kv v = unpack_key<NW>(s.k);
-        uint64_t p[3];
-        bloom_positions(v.lo, v.hi, 0, 0, M, p);
-        const Ev base = (Ev)(s.t0 * 4);
+__global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
+                                                        uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                        uint32_t n_libs, const LibDesc* __restrict__ libs,
+                                                        LibOut* __restrict__ out) {
+    __shared__ unsigned long long s_acc[LIB_LOCAL];
+    LibAccum acc{s_acc};
+    acc.reset();
+    __syncthreads();
+    scan_slots<NW, Ev>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t lib0) {
+            const Slot<NW, Ev> s = tab[i];
+            if (s.k[0] == EMPTY_KEY) return;
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            Ev* fs = reinterpret_cast<Ev*>(L.fs);
+            const Ev base = (Ev)(s.t0 * 4);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
-    }
-    block_add(local, n_distinct);
+            for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
+            acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
+        },
+        [&](uint32_t lib0) {
+            __syncthreads();
+            acc.flush(lib0, n_libs, &out[0].n_distinct, LIBOUT_STRIDE);
+            __syncthreads();
+            acc.reset();
+            __syncthreads();
+        });
 }
 
-// seen[w] bit b = (fs[64w + b] is finite); clears the solid tier.  One wave per word.
+// seen word w (pool index) = ballot(fs != inf) over its 64 bits; clears the solid word.
 template <typename Ev>
-__global__ void k_seen_from_fs(const Ev* __restrict__ fs, uint64_t num_words, uint64_t* __restrict__ seen,
-                               uint64_t* __restrict__ solid) {
+__global__ void k_seen_from_fs(uint64_t w_begin, uint64_t w_end, const uint64_t* __restrict__ lib_w0, uint32_t n_libs,
+                               const LibDesc* __restrict__ libs) {
     const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint32_t lane = __lane_id();
-    for (uint64_t w = wave; w < num_words; w += nwaves) {
-        const Ev v = fs[w * 64 + lane];
+    for (uint64_t w = w_begin + wave; w < w_end; w += nwaves) {
+        const uint32_t lib = find_region(lib_w0, n_libs, w);
+        const LibDesc& L = libs[lib];
+        const uint64_t lw = w - L.word0;
+        const Ev v = reinterpret_cast<const Ev*>(L.fs)[lw * 64 + lane];
         const uint64_t m = __ballot(v != (Ev)~(Ev)0);
         if (lane == 0) {
-            seen[w] = m;
-            solid[w] = 0;
+            L.seen[lw] = m;
+            L.solid[lw] = 0;
         }
     }
 }
 
 // Solid tier: OR of the bits of every promoted distinct k-mer (bloom.rs:99-103).
 template <int NW, typename Ev>
-__global__ void k_promote(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap, const Ev* __restrict__ fs,
-                          uint64_t* __restrict__ solid, Modulus M) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
-        const Slot<NW, Ev> s = tab[i];
-        if (s.k[0] == EMPTY_KEY) continue;
-        const Kv v = unpack_key<NW>(s.k);
-        uint64_t p[3];
-        bloom_positions(v.lo, v.hi, 0, 0, M, p);
-        bool promote = s.cnt >= 2;
-        if (!promote) {
-            const Ev base = (Ev)(s.t0 * 4);
-            promote = fs[p[0]] != base && fs[p[1]] != (Ev)(base + 1) && fs[p[2]] != (Ev)(base + 2);
-        }
-        if (promote) {
+__global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
+                                                   uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                   uint32_t n_libs, const LibDesc* __restrict__ libs) {
+    scan_slots<NW, Ev>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t) {
+            const Slot<NW, Ev> s = tab[i];
+            if (s.k[0] == EMPTY_KEY) return;
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            bool promote = s.cnt >= 2;
+            if (!promote) {
+                const Ev* fs = reinterpret_cast<const Ev*>(L.fs);
+                const Ev base = (Ev)(s.t0 * 4);
+                promote = fs[p[0]] != base && fs[p[1]] != (Ev)(base + 1) && fs[p[2]] != (Ev)(base + 2);
+            }
+            if (promote) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                atomicOr(reinterpret_cast<unsigned long long*>(&solid[p[j] >> 6]), 1ull << (p[j] & 63));
-        }
-    }
+                for (int j = 0; j < 3; ++j)
+                    atomicOr(reinterpret_cast<unsigned long long*>(&L.solid[p[j] >> 6]), 1ull << (p[j] & 63));
+            }
+        },
+        [](uint32_t) {});
 }
 
 // Gather the count table: occupied slots whose k-mer is Bloom-solid (or every occupied
-// slot when the table was built from Bloom-solid windows only).  Each block takes chunks
-// of BLOCK * COMPACT_ITEMS slots and reserves its output range with one atomic.
-constexpr uint32_t COMPACT_ITEMS = 16;
-
+// slot when the table was built from Bloom-solid windows only), as (hi, lo, count) sort
+// entries.  Each block reserves its output range with one atomic per chunk.
 template <int NW, typename Ev, bool CHECK_SOLID>
-__global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t cap,
-                                                   const uint64_t* __restrict__ solid, Modulus M,
-                                                   uint64_t* __restrict__ out_a, uint64_t* __restrict__ out_b,
+__global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_end,
+                                                   const uint64_t* __restrict__ lib_b0, uint32_t n_libs,
+                                                   const LibDesc* __restrict__ libs, SortLayout lay,
+                                                   uint64_t* __restrict__ out_hi, uint64_t* __restrict__ out_lo,
                                                    uint32_t* __restrict__ out_cnt,
                                                    unsigned long long* __restrict__ counter) {
     __shared__ uint32_t s_scan[BLOCK];
     __shared__ unsigned long long s_base;
-    const uint64_t chunk = (uint64_t)BLOCK * COMPACT_ITEMS;
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
     const uint32_t tid = threadIdx.x;
-    for (uint64_t c0 = blockIdx.x * chunk; c0 < cap; c0 += (uint64_t)gridDim.x * chunk) {
+    for (uint64_t c0 = blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
         uint32_t bits = 0, cnt = 0;
-#pragma unroll 4
-        for (uint32_t j = 0; j < COMPACT_ITEMS; ++j) {
+        for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
-            if (i >= cap) break;
+            if (i >= s_end) break;
             const Slot<NW, Ev> sl = tab[i];
             bool take = sl.k[0] != EMPTY_KEY;
-            if (CHECK_SOLID && take) take = bloom_is_solid(unpack_key<NW>(sl.k), solid, M);
+            if (CHECK_SOLID && take) {
+                const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
+                take = bloom_is_solid(unpack_key<NW>(sl.k), L.solid, L.M);
+            }
             bits |= (uint32_t)take << j;
             cnt += take;
         }
@@ -327,9 +486,12 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
         while (bits) {
             const uint32_t j = __ffs(bits) - 1;
             bits &= bits - 1;
-            const Slot<NW, Ev> sl = tab[c0 + (uint64_t)j * BLOCK + tid];
-            out_a[o] = sl.k[0];
-            if (NW == 2) out_b[o] = sl.k[NW - 1];
+            const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
+            const Slot<NW, Ev> sl = tab[i];
+            uint64_t hi, lo;
+            encode_entry(lay, find_region(lib_b0, n_libs, i / SPB), unpack_key<NW>(sl.k), hi, lo);
+            out_lo[o] = lo;
+            if (!lay.single) out_hi[o] = hi;
             out_cnt[o] = sl.cnt;
             ++o;
         }
@@ -348,219 +510,101 @@ __global__ void k_gather(const T* __restrict__ src, const uint32_t* __restrict__
         dst[i] = src[idx[i]];
 }
 
-template <typename T>
-__global__ void k_fill(T* __restrict__ x, uint64_t n, T v) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        x[i] = v;
-}
-
-// Histogram of min(count, HIST_BINS - 1) for determine_solid_kmer_cutoff (assemble.rs:692-735).
-__global__ void __launch_bounds__(BLOCK) k_count_hist(const uint32_t* __restrict__ cnt, uint64_t n,
+// Per-library histogram of min(count, HIST_BINS - 1) over the sorted entries, for
+// determine_solid_kmer_cutoff (assemble.rs:692-735).  Entries are grouped by library, so
+// a block keeps the histogram of its first entry's library in LDS.
+__global__ void __launch_bounds__(BLOCK) k_count_hist(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo,
+                                                      const uint32_t* __restrict__ cnt, uint64_t n, SortLayout lay,
                                                       uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[HIST_BINS];
+    const uint64_t per_block = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = blockIdx.x * per_block, e = min(n, b + per_block);
+    if (b >= e) return;
+    const uint32_t lib0 = entry_lib(lay, lay.single ? 0 : hi[b], lo[b]);
     for (uint32_t i = threadIdx.x; i < HIST_BINS; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[min(cnt[i], HIST_BINS - 1)], 1u);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t lib = entry_lib(lay, lay.single ? 0 : hi[i], lo[i]);
+        const uint32_t bin = min(cnt[i], HIST_BINS - 1);
+        if (lib == lib0) atomicAdd(&h[bin], 1u);
+        else atomicAdd(&hist[(size_t)lib * HIST_BINS + bin], 1u);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < HIST_BINS; i += blockDim.x)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
+        if (h[i]) atomicAdd(&hist[(size_t)lib0 * HIST_BINS + i], h[i]);
 }
 
-__global__ void k_solid_flags(const uint32_t* __restrict__ cnt, uint64_t n, uint32_t cutoff, uint8_t* __restrict__ f,
-                              unsigned long long* __restrict__ n_solid) {
-    uint64_t local = 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const bool s = cnt[i] >= cutoff;
+// determine_solid_kmer_cutoff (assemble.rs:692-735) per library from its histogram of
+// min(count, HIST_BINS-1).  The 95th-percentile count `top` only enters through top >= 30
+// and clamp(top/4, 10, 35), which the capped histogram decides exactly.
+__device__ __host__ uint32_t cutoff_rule(const uint32_t* hist, uint64_t n, double min_cov, uint32_t* top_out) {
+    *top_out = 0;
+    if (min_cov > 5.0) {
+        const double v = min_cov * 0.35 > 2.0 ? min_cov * 0.35 : 2.0;
+        return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+    }
+    if (n <= 100) return 2;
+    const uint64_t rank = n * 95 / 100;
+    uint64_t cum = 0;
+    uint32_t top = HIST_BINS - 1;
+    for (uint32_t b = 0; b < HIST_BINS; ++b) {
+        cum += hist[b];
+        if (cum > rank) {
+            top = b;
+            break;
+        }
+    }
+    *top_out = top;
+    if (top < 30) return 2;
+    uint32_t mvs = top / 4;
+    mvs = mvs < 10 ? 10 : (mvs > 35 ? 35 : mvs);
+    uint32_t valley = 2;
+    uint64_t min_val = ~0ull;
+    for (uint32_t c = 2; c <= mvs; ++c) {
+        const uint64_t h = hist[c];
+        if (h <= min_val) {
+            min_val = h;
+            valley = c;
+        } else if (h > min_val * 2 && c > valley + 2) {
+            return valley;
+        }
+    }
+    return 2;
+}
+
+__global__ void k_cutoff(const uint32_t* __restrict__ hist, uint32_t n_libs, const double* __restrict__ min_cov,
+                         LibOut* __restrict__ out) {
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < n_libs; l += gridDim.x * blockDim.x) {
+        const uint32_t* h = hist + (size_t)l * HIST_BINS;
+        uint64_t n = 0;
+        for (uint32_t b = 0; b < HIST_BINS; ++b) n += h[b];
+        uint32_t top = 0;
+        out[l].cutoff = cutoff_rule(h, n, min_cov[l], &top);
+        out[l].top = top;
+        out[l].n_counts = n;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo,
+                                                       const uint32_t* __restrict__ cnt, uint64_t n, SortLayout lay,
+                                                       uint32_t n_libs, const LibOut* __restrict__ cut,
+                                                       uint8_t* __restrict__ f, LibOut* __restrict__ out) {
+    __shared__ unsigned long long s_acc[LIB_LOCAL];
+    LibAccum acc{s_acc};
+    const uint64_t per_block = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = blockIdx.x * per_block, e = min(n, b + per_block);
+    if (b >= e) return;
+    acc.reset();
+    __syncthreads();
+    const uint32_t lib0 = entry_lib(lay, lay.single ? 0 : hi[b], lo[b]);
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t lib = entry_lib(lay, lay.single ? 0 : hi[i], lo[i]);
+        const bool s = cnt[i] >= cut[lib].cutoff;
         f[i] = s;
-        local += s;
+        if (s) acc.add(lib, lib0, 1, &out[0].n_solid, LIBOUT_STRIDE);
     }
-    block_add(local, n_solid);
-}
-
-// ---------------------------------------------------------------------------
-// Hammer (hammer.rs:34-120): one read per lane, the read held packed in LDS.
-// ---------------------------------------------------------------------------
-
-template <bool WIDE>
-__device__ __forceinline__ bool lds_window_solid(const uint32_t* rd, uint32_t s, uint32_t k,
-                                                 const uint64_t* __restrict__ solid, const Modulus& M) {
-    const Kv v = extract_be<WIDE>(rd + (s >> 4), (2 * s) & 31, k);
-    return bloom_is_solid(canonical<WIDE>(v, k), solid, M);
-}
-
-__device__ __forceinline__ uint32_t lds_get_base(const uint8_t* rb, uint32_t pos) {
-    return (rb[pos >> 2] >> (6 - 2 * (pos & 3))) & 3;
-}
-
-__device__ __forceinline__ void lds_set_base(uint8_t* rb, uint32_t pos, uint32_t c) {
-    const uint32_t sh = 6 - 2 * (pos & 3);
-    rb[pos >> 2] = (uint8_t)((rb[pos >> 2] & ~(3u << sh)) | (c << sh));
-}
-
-template <bool WIDE>
-__global__ void __launch_bounds__(64) k_hammer(const uint8_t* __restrict__ data, uint8_t* __restrict__ corr,
-                                               const uint32_t* __restrict__ active, uint64_t n_active,
-                                               const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ lengths,
-                                               uint32_t k, const uint64_t* __restrict__ solid, Modulus M,
-                                               unsigned long long* __restrict__ n_corrected, uint32_t* __restrict__ too_long) {
-    __shared__ uint32_t lds[64 * HAMMER_LANE_WORDS];
-    uint32_t* rd = lds + threadIdx.x * HAMMER_LANE_WORDS;
-    uint8_t* rb = reinterpret_cast<uint8_t*>(rd);
-    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n_active;
-         a += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t r = active[a];
-        const uint32_t L = lengths[r];
-        if (L > HAMMER_MAX_LEN) {
-            atomicOr(too_long, 1u);
-            continue;
-        }
-        const uint64_t off = offsets[r];
-        const uint32_t nb = (L + 3) / 4;
-        for (uint32_t j = 0; j < HAMMER_LANE_WORDS; ++j) rd[j] = 0;
-        for (uint32_t j = 0; j < nb; ++j) rb[j] = data[off + j];
-        bool modified = false;
-        for (uint32_t i = 0; i + k <= L; ++i) {
-            if (lds_window_solid<WIDE>(rd, i, k, solid, M)) continue;
-            bool have = false;
-            uint32_t best_off = 0, best_c = 0, max_gain = 0;
-            for (uint32_t o = 0; o < k; ++o) {
-                const uint32_t pos = i + o;
-                const uint32_t orig = lds_get_base(rb, pos);
-                const uint32_t start = pos >= k - 1 ? pos - (k - 1) : 0;
-                const uint32_t end = min(pos, L - k);
-                uint32_t base_solid = 0;
-                for (uint32_t s = start; s <= end; ++s)
-                    if (s != i && lds_window_solid<WIDE>(rd, s, k, solid, M)) ++base_solid;
-                for (uint32_t c = 0; c < 4; ++c) {
-                    if (c == orig) continue;
-                    lds_set_base(rb, pos, c);
-                    if (lds_window_solid<WIDE>(rd, i, k, solid, M)) {
-                        uint32_t sc = 1;
-                        for (uint32_t s = start; s <= end; ++s)
-                            if (s != i && lds_window_solid<WIDE>(rd, s, k, solid, M)) ++sc;
-                        const uint32_t gain = sc > base_solid ? sc - base_solid : 0;
-                        if (gain > max_gain) {
-                            max_gain = gain;
-                            have = true;
-                            best_off = o;
-                            best_c = c;
-                        }
-                    }
-                    lds_set_base(rb, pos, orig);
-                }
-            }
-            if (have) {
-                lds_set_base(rb, i + best_off, best_c);
-                modified = true;
-            }
-        }
-        if (modified) {
-            for (uint32_t j = 0; j < nb; ++j) corr[off + j] = rb[j];
-            atomicAdd(n_corrected, 1ull);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// standalone filter: sequential insert semantics over a batch of events
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void key_positions(const uint64_t* __restrict__ keys4, uint64_t t, const Modulus& M,
-                                              uint64_t p[3]) {
-    const uint64_t* w = keys4 + 4 * t;
-    bloom_positions(w[0], w[1], w[2], w[3], M, p);
-}
-
-__global__ void k_ev_first_setter(const uint64_t* __restrict__ keys4, uint64_t n, Modulus M, uint32_t* __restrict__ fs) {
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t p[3];
-        key_positions(keys4, t, M, p);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (uint32_t)(4 * t + j));
-    }
-}
-
-// prev_j(t) = bit set before this batch, or first set by an earlier (event, hash) of it.
-__global__ void k_ev_promote(const uint64_t* __restrict__ keys4, uint64_t n, Modulus M, const uint32_t* __restrict__ fs,
-                             const uint64_t* __restrict__ seen, uint8_t* __restrict__ prom) {
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t p[3];
-        key_positions(keys4, t, M, p);
-        bool all = true;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const bool before = (seen[p[j] >> 6] >> (p[j] & 63)) & 1;
-            all = all && (before || fs[p[j]] < (uint32_t)(4 * t + j));
-        }
-        prom[t] = all;
-    }
-}
-
-__global__ void k_ev_apply(const uint64_t* __restrict__ keys4, uint64_t n, Modulus M, const uint8_t* __restrict__ prom,
-                           uint64_t* __restrict__ seen, uint64_t* __restrict__ solid, uint32_t* __restrict__ fs) {
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t p[3];
-        key_positions(keys4, t, M, p);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            atomicOr(reinterpret_cast<unsigned long long*>(&seen[p[j] >> 6]), 1ull << (p[j] & 63));
-            if (prom[t]) atomicOr(reinterpret_cast<unsigned long long*>(&solid[p[j] >> 6]), 1ull << (p[j] & 63));
-            fs[p[j]] = ~0u;  // restore for the next batch
-        }
-    }
-}
-
-__global__ void k_query(const uint64_t* __restrict__ keys4, uint64_t n, Modulus M, const uint64_t* __restrict__ solid,
-                        uint8_t* __restrict__ out) {
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t p[3];
-        key_positions(keys4, t, M, p);
-        bool all = true;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) all = all && ((solid[p[j] >> 6] >> (p[j] & 63)) & 1);
-        out[t] = all;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// host helpers
-// ---------------------------------------------------------------------------
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && p) return hipSuccess;
-        release();
-        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
-        if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
-        else p = nullptr;
-        return e;
-    }
-    template <typename T>
-    T* as() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
-
-Modulus make_modulus(uint64_t size_bits) {
-    Modulus M{};
-    M.d = size_bits;
-    M.pow2 = (size_bits & (size_bits - 1)) == 0;
-    M.mask = size_bits - 1;
-    M.magic = M.pow2 ? 0 : (uint64_t)(((unsigned __int128)1 << 64) / size_bits);
-    return M;
-}
-
-uint32_t grid_for(uint64_t work, uint32_t block = BLOCK, uint32_t cap = 16384) {
-    uint64_t g = (work + block - 1) / block;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+    __syncthreads();
+    acc.flush(lib0, n_libs, &out[0].n_solid, LIBOUT_STRIDE);
 }
 
 enum KernelId {
@@ -571,6 +615,20 @@ const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
     "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_compact", "radix_sort", "k_count_hist",
     "k_solid_flags"};
+
+// A library as uploaded: its reads (host copy, PackedReads layout) and options.
+struct HostLib {
+    std::vector<uint8_t> data;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> len;
+    uint64_t bloom_bits = 0;
+    double min_cov = 5.0;
+    // placement in the device pools (set by the planner)
+    uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
+    uint32_t group = 0;
+    kc_info info{};
+    uint64_t entry0 = 0;  // first sorted entry
+};
 
 }  // namespace
 
@@ -589,30 +647,31 @@ struct kc_ctx {
     kc_opts o{};
     hipStream_t st = nullptr;
     std::string err;
-    // reads
-    DevBuf d_data, d_off, d_len;
-    size_t data_len = 0, n_reads = 0;
+    std::vector<HostLib> libs;
+    bool uploaded = false;  // device read buffers match `libs`
+    // reads of all libraries, concatenated
+    DevBuf d_data, d_off, d_len, d_rlib;
+    uint64_t data_len = 0, n_reads = 0;
     // read bookkeeping
-    DevBuf d_nwin, d_prefix, d_flags, d_iota, d_active, d_act_ev, d_act_off, d_tile_first;
+    DevBuf d_nwin, d_prefix, d_flags, d_iota, d_active, d_act_ev, d_act_off, d_act_lib, d_tile_first;
+    DevBuf d_lib_read0, d_lib_ev0;
     uint64_t n_occ = 0, n_active = 0, n_tiles = 0;
-    // filter
-    DevBuf d_seen, d_solid, d_fs;
-    uint64_t num_words = 0;
-    Modulus M{};
-    // table
-    DevBuf d_tab;
-    uint64_t cap = 0;
-    // count table outputs (sorted) + scratch
-    DevBuf d_a, d_b, d_cnt, d_a2, d_b2, d_cnt2, d_idx, d_idx2, d_sflags;
+    // filter pools, first setters, table pool
+    DevBuf d_seen, d_solid, d_fs, d_tab, d_libs, d_lib_b0, d_lib_w0, d_blooms, d_out, d_mincov;
+    uint64_t total_words = 0, total_buckets = 0, fs_bytes = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> groups;  // [lib begin, lib end) sharing d_fs
+    // count entries (sorted) + scratch
+    DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
-    uint64_t n_counts = 0;
+    uint64_t n_entries = 0;
+    SortLayout lay{};
     bool wide = false, ev64 = false, built = false, counted = false;
-    kc_info info{};
-    // small pinned staging
-    uint64_t* h_stage = nullptr;
+    uint64_t* h_stage = nullptr;  // small pinned staging
+    std::vector<LibOut> h_out;
     // timing
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    size_t mark_idx = 0;
     double k_ms[KID_COUNT] = {0};
     uint32_t k_launches[KID_COUNT] = {0};
 
@@ -627,17 +686,12 @@ struct kc_ctx {
         ev_pool.push_back(e);
         return e;
     }
-    size_t mark_idx = 0;
-    void mark_begin(int kid, hipEvent_t* b) {
+    void mark_begin(int, hipEvent_t* b) {
         *b = nullptr;
         if (!timing()) return;
-        if (mark_idx * 2 + 1 >= ev_pool.size()) {
-            new_event();
-            new_event();
-        }
+        while (mark_idx * 2 + 1 >= ev_pool.size()) new_event();
         *b = ev_pool[mark_idx * 2];
         (void)hipEventRecord(*b, st);
-        (void)kid;
     }
     void mark_end(int kid, hipEvent_t b) {
         if (!timing() || !b) return;
@@ -682,6 +736,51 @@ struct Run {
     int fail(int code, const std::string& m) { return c->fail(code, m); }
     void mark_begin(int kid, hipEvent_t* b) { c->mark_begin(kid, b); }
     void mark_end(int kid, hipEvent_t b) { c->mark_end(kid, b); }
+    uint32_t L() const { return (uint32_t)c->libs.size(); }
+
+    // Concatenate every library's reads into the device read buffers.
+    int upload() {
+        hipStream_t st = c->st;
+        uint64_t data_len = 0, n = 0;
+        for (auto& lib : c->libs) {
+            lib.data0 = data_len;
+            lib.read0 = n;
+            data_len += lib.data.size();
+            n += lib.len.size();
+        }
+        if (n > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "more than 2^32-1 reads in one context");
+        KC_TRY(c->d_data.ensure(data_len + DATA_PAD));
+        KC_TRY(c->d_off.ensure(n * 8 + 8));
+        KC_TRY(c->d_len.ensure(n * 4 + 4));
+        KC_TRY(c->d_rlib.ensure(n * 4 + 4));
+        k_fill<uint8_t><<<grid_for(data_len + DATA_PAD), BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), data_len + DATA_PAD,
+                                                                         (uint8_t)0);
+        KC_TRY(hipGetLastError());
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> rl;
+        for (uint32_t l = 0; l < L(); ++l) {
+            auto& lib = c->libs[l];
+            const uint64_t m = lib.len.size();
+            if (!lib.data.empty())
+                KC_TRY(hipMemcpyAsync(c->d_data.as<uint8_t>() + lib.data0, lib.data.data(), lib.data.size(),
+                                      hipMemcpyHostToDevice, st));
+            if (m) {
+                off.resize(m);
+                for (uint64_t i = 0; i < m; ++i) off[i] = lib.off[i] + lib.data0;
+                rl.assign(m, l);
+                KC_TRY(hipMemcpyAsync(c->d_off.as<uint64_t>() + lib.read0, off.data(), m * 8, hipMemcpyHostToDevice, st));
+                KC_TRY(hipMemcpyAsync(c->d_len.as<uint32_t>() + lib.read0, lib.len.data(), m * 4, hipMemcpyHostToDevice,
+                                      st));
+                KC_TRY(hipMemcpyAsync(c->d_rlib.as<uint32_t>() + lib.read0, rl.data(), m * 4, hipMemcpyHostToDevice, st));
+                KC_TRY(hipStreamSynchronize(st));  // host vectors are reused
+            }
+        }
+        KC_TRY(hipStreamSynchronize(st));
+        c->data_len = data_len;
+        c->n_reads = n;
+        c->uploaded = true;
+        return KC_OK;
+    }
 
     int prep() {
         hipStream_t st = c->st;
@@ -693,8 +792,19 @@ struct Run {
         KC_TRY(c->d_iota.ensure(n * 4 + 4));
         KC_TRY(c->d_active.ensure(n * 4 + 4));
         KC_TRY(c->d_counters.ensure(64 * 8));
+        KC_TRY(c->d_lib_read0.ensure((L() + 1) * 8));
+        KC_TRY(c->d_lib_ev0.ensure((L() + 1) * 8));
         c->n_occ = c->n_active = c->n_tiles = 0;
-        if (n == 0) return KC_OK;
+        std::vector<uint64_t> r0(L() + 1);
+        for (uint32_t l = 0; l < L(); ++l) r0[l] = c->libs[l].read0;
+        r0[L()] = n;
+        KC_TRY(hipMemcpyAsync(c->d_lib_read0.p, r0.data(), r0.size() * 8, hipMemcpyHostToDevice, st));
+        if (n == 0) {
+            for (auto& lib : c->libs) lib.info.n_occ = 0;
+            lib_ev0_.assign(L() + 1, 0);
+            KC_TRY(hipStreamSynchronize(st));
+            return KC_OK;
+        }
         TIMED(KID_NWIN, (k_nwin<<<grid_for(n), BLOCK, 0, st>>>(c->d_len.as<uint32_t>(), n, k, c->d_nwin.as<uint64_t>(),
                                                                 c->d_flags.as<uint8_t>(), c->d_iota.as<uint32_t>())));
         size_t tb = 0, tb2 = 0;
@@ -711,162 +821,270 @@ struct Run {
         TIMED(KID_SELECT, KC_TRY(rocprim::select(c->d_temp.p, tb2, c->d_iota.as<uint32_t>(), c->d_flags.as<uint8_t>(),
                                                   c->d_active.as<uint32_t>(), c->d_counters.as<uint64_t>() + 8,
                                                   (size_t)n, st)));
-        // n_occ = prefix[n-1] + nwin[n-1]; n_active = selected count
-        KC_TRY(hipMemcpyAsync(c->h_stage, c->d_prefix.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_nwin.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, st));
+        k_lib_ev0<<<1, 256, 0, st>>>(c->d_prefix.as<uint64_t>(), c->d_nwin.as<uint64_t>(), n,
+                                     c->d_lib_read0.as<uint64_t>(), L(), c->d_lib_ev0.as<uint64_t>());
+        KC_TRY(hipGetLastError());
+        std::vector<uint64_t> ev0(L() + 1);
+        KC_TRY(hipMemcpyAsync(ev0.data(), c->d_lib_ev0.p, (L() + 1) * 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipMemcpyAsync(c->h_stage + 2, c->d_counters.as<uint64_t>() + 8, 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
-        c->n_occ = c->h_stage[0] + c->h_stage[1];
+        c->n_occ = ev0[L()];
         c->n_active = c->h_stage[2];
         c->n_tiles = (c->n_occ + TILE_WINDOWS - 1) / TILE_WINDOWS;
+        for (uint32_t l = 0; l < L(); ++l) c->libs[l].info.n_occ = ev0[l + 1] - ev0[l];
+        lib_ev0_ = ev0;
         if (c->n_occ == 0) return KC_OK;
         if (c->n_tiles > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "too many windows for one launch");
         KC_TRY(c->d_act_ev.ensure(c->n_active * 8 + 8));
         KC_TRY(c->d_act_off.ensure(c->n_active * 8 + 8));
+        KC_TRY(c->d_act_lib.ensure(c->n_active * 4 + 4));
         KC_TRY(c->d_tile_first.ensure(c->n_tiles * 4 + 4));
         TIMED(KID_ACTIVE, (k_active<<<grid_for(c->n_active), BLOCK, 0, st>>>(
                               c->d_active.as<uint32_t>(), c->n_active, c->d_prefix.as<uint64_t>(),
-                              c->d_nwin.as<uint64_t>(), c->d_off.as<uint64_t>(), c->d_act_ev.as<uint64_t>(),
-                              c->d_act_off.as<uint64_t>(), c->d_tile_first.as<uint32_t>())));
+                              c->d_nwin.as<uint64_t>(), c->d_off.as<uint64_t>(), c->d_rlib.as<uint32_t>(),
+                              c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
+                              c->d_tile_first.as<uint32_t>())));
         return KC_OK;
     }
+    std::vector<uint64_t> lib_ev0_;
 
     template <bool WIDE>
-    int estimate_distinct(const uint8_t* data, uint64_t* est) {
+    int estimate_distinct() {
         hipStream_t st = c->st;
-        KC_TRY(c->d_hll.ensure(HLL_M * 4));
-        k_fill<uint32_t><<<grid_for(HLL_M), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), HLL_M, 0u);
+        const size_t nreg = (size_t)L() * HLL_M;
+        KC_TRY(c->d_hll.ensure(nreg * 4));
+        k_fill<uint32_t><<<grid_for(nreg), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), nreg, 0u);
         KC_TRY(hipGetLastError());
         const uint32_t g = (uint32_t)std::min<uint64_t>(c->n_tiles, 2048);
-        TIMED(KID_HLL, (k_hll<WIDE><<<g, BLOCK, 0, st>>>(data, c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
+        TIMED(KID_HLL, (k_hll<WIDE><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
+                                                         c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
                                                          c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
                                                          c->n_tiles, c->o.k, c->d_hll.as<uint32_t>())));
-        std::vector<uint32_t> regs(HLL_M);
-        KC_TRY(hipMemcpyAsync(regs.data(), c->d_hll.p, HLL_M * 4, hipMemcpyDeviceToHost, st));
+        std::vector<uint32_t> regs(nreg);
+        KC_TRY(hipMemcpyAsync(regs.data(), c->d_hll.p, nreg * 4, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
-        double sum = 0;
-        uint32_t zeros = 0;
-        for (uint32_t r : regs) {
-            sum += std::ldexp(1.0, -(int)r);
-            zeros += r == 0;
+        for (uint32_t l = 0; l < L(); ++l) {
+            double sum = 0;
+            uint32_t zeros = 0;
+            for (uint32_t i = 0; i < HLL_M; ++i) {
+                const uint32_t r = regs[(size_t)l * HLL_M + i];
+                sum += std::ldexp(1.0, -(int)r);
+                zeros += r == 0;
+            }
+            const double m = HLL_M;
+            double E = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
+            if (E <= 2.5 * m && zeros) E = m * std::log(m / zeros);
+            c->libs[l].est = (uint64_t)std::ceil(E);
         }
-        const double m = HLL_M;
-        double E = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
-        if (E <= 2.5 * m && zeros) E = m * std::log(m / zeros);
-        *est = (uint64_t)std::ceil(E);
         return KC_OK;
     }
 
+    // Lay out the table pool, the Bloom pools and the first-setter groups.
     template <int NW, typename Ev>
-    int alloc_table(uint64_t distinct_est) {
+    int plan(const std::vector<uint32_t>& grow) {
         const double load = (c->o.table_load > 0.05f && c->o.table_load < 0.95f) ? c->o.table_load : 0.5;
-        uint64_t cap = (uint64_t)std::ceil((double)distinct_est * 1.10 / load) + 1024;
-        cap = std::min<uint64_t>(cap, c->n_occ * 2 + 1024);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
-        cap = (cap + SPB - 1) / SPB * SPB;
-        c->cap = cap;
-        KC_TRY(c->d_tab.ensure(cap * sizeof(Slot<NW, Ev>)));
+        uint64_t b0 = 0, w0 = 0;
+        for (uint32_t l = 0; l < L(); ++l) {
+            auto& lib = c->libs[l];
+            const uint64_t occ = lib.info.n_occ;
+            uint64_t cap = (uint64_t)std::ceil((double)lib.est * 1.10 / load) + 64;
+            cap = std::min<uint64_t>(cap, occ * 2 + 64) << grow[l];
+            lib.nbuckets = occ ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
+            lib.bucket0 = b0;
+            b0 += lib.nbuckets;
+            lib.num_words = std::max<uint64_t>((lib.bloom_bits + 63) / 64, 1);
+            lib.word0 = w0;
+            w0 += lib.num_words;
+        }
+        c->total_buckets = b0;
+        c->total_words = w0;
+        KC_TRY(c->d_tab.ensure(std::max<uint64_t>(b0, 1) * 64));
+        KC_TRY(c->d_seen.ensure(w0 * 8));
+        KC_TRY(c->d_solid.ensure(w0 * 8));
+        // first-setter groups: consecutive libraries whose fs arrays fit the budget together
+        const uint64_t budget = 64ull << 30;
+        c->groups.clear();
+        uint64_t used = 0, maxg = 0;
+        uint32_t gb = 0;
+        for (uint32_t l = 0; l < L(); ++l) {
+            const uint64_t need = c->libs[l].num_words * 64 * sizeof(Ev);
+            if (l > gb && used + need > budget) {
+                c->groups.push_back({gb, l});
+                maxg = std::max(maxg, used);
+                gb = l;
+                used = 0;
+            }
+            c->libs[l].fs_off = used;
+            c->libs[l].group = (uint32_t)c->groups.size();
+            used += need;
+        }
+        c->groups.push_back({gb, L()});
+        maxg = std::max(maxg, used);
+        KC_TRY(c->d_fs.ensure(maxg));
+        c->fs_bytes = maxg;
+        // descriptors
+        std::vector<LibDesc> desc(L());
+        std::vector<LibBloom> blooms(L());
+        std::vector<uint64_t> lb0(L()), lw0(L());
+        std::vector<double> mc(L());
+        for (uint32_t l = 0; l < L(); ++l) {
+            auto& lib = c->libs[l];
+            LibDesc& d = desc[l];
+            d.ev0 = lib_ev0_[l];
+            d.bucket0 = lib.bucket0;
+            d.nbuckets = lib.nbuckets;
+            d.word0 = lib.word0;
+            d.seen = c->d_seen.as<uint64_t>() + lib.word0;
+            d.solid = c->d_solid.as<uint64_t>() + lib.word0;
+            d.fs = c->d_fs.as<uint8_t>() + lib.fs_off;
+            d.M = make_modulus(lib.num_words * 64);
+            d.num_words = lib.num_words;
+            blooms[l] = LibBloom{d.solid, d.M};
+            lb0[l] = lib.bucket0;
+            lw0[l] = lib.word0;
+            mc[l] = lib.min_cov;
+        }
+        KC_TRY(c->d_libs.ensure(L() * sizeof(LibDesc)));
+        KC_TRY(c->d_blooms.ensure(L() * sizeof(LibBloom)));
+        KC_TRY(c->d_lib_b0.ensure(L() * 8));
+        KC_TRY(c->d_lib_w0.ensure(L() * 8));
+        KC_TRY(c->d_mincov.ensure(L() * 8));
+        KC_TRY(c->d_out.ensure(L() * sizeof(LibOut)));
+        hipStream_t st = c->st;
+        KC_TRY(hipMemcpyAsync(c->d_libs.p, desc.data(), L() * sizeof(LibDesc), hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_blooms.p, blooms.data(), L() * sizeof(LibBloom), hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_lib_b0.p, lb0.data(), L() * 8, hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_lib_w0.p, lw0.data(), L() * 8, hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_mincov.p, mc.data(), L() * 8, hipMemcpyHostToDevice, st));
+        KC_TRY(hipStreamSynchronize(st));
+        return KC_OK;
+    }
+
+    int clear_out() {
+        k_fill<uint8_t><<<grid_for(L() * sizeof(LibOut)), BLOCK, 0, c->st>>>(c->d_out.as<uint8_t>(),
+                                                                           L() * sizeof(LibOut), (uint8_t)0);
+        KC_TRY(hipGetLastError());
+        return KC_OK;
+    }
+
+    int read_out() {
+        c->h_out.resize(L());
+        KC_TRY(hipMemcpyAsync(c->h_out.data(), c->d_out.p, L() * sizeof(LibOut), hipMemcpyDeviceToHost, c->st));
+        KC_TRY(hipStreamSynchronize(c->st));
         return KC_OK;
     }
 
     template <bool WIDE, typename Ev>
     int bloom_build() {
         constexpr int NW = WIDE ? 2 : 1;
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
-        uint64_t est = 0;
-        int rc = estimate_distinct<WIDE>(c->d_data.as<uint8_t>(), &est);
+        int rc = estimate_distinct<WIDE>();
         if (rc) return rc;
+        std::vector<uint32_t> grow(L(), 0);
         for (int attempt = 0;; ++attempt) {
-            rc = alloc_table<NW, Ev>(est << attempt);
+            rc = plan<NW, Ev>(grow);
             if (rc) return rc;
-            TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(c->cap), BLOCK, 0, st>>>(
-                                 c->d_tab.as<Slot<NW, Ev>>(), c->cap)));
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(c->d_counters.as<unsigned long long>(), 16, 0ull);
-            KC_TRY(hipGetLastError());
+            rc = clear_out();
+            if (rc) return rc;
+            const uint64_t n_slots = c->total_buckets * SPB;
+            TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(
+                                 c->d_tab.as<Slot<NW, Ev>>(), n_slots)));
             TIMED(KID_INSERT, (k_insert<WIDE, Ev, true, false><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
                                   c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
-                                  c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ, c->o.k,
-                                  c->d_tab.as<Slot<NW, Ev>>(), c->cap / slots_per_bucket<NW, Ev>(),
-                                  (uint32_t)std::min<uint64_t>(c->cap / slots_per_bucket<NW, Ev>(), 1u << 20),
-                                  nullptr, c->M, c->d_counters.as<uint32_t>())));
-            KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            if ((c->h_stage[0] & 0xFFFFFFFFull) == 0) break;
+                                  c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
+                                  c->o.k, c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), 1u << 20,
+                                  c->d_out.as<LibOut>())));
+            rc = read_out();
+            if (rc) return rc;
+            bool any = false;
+            for (uint32_t l = 0; l < L(); ++l)
+                if (c->h_out[l].overflow) {
+                    any = true;
+                    ++grow[l];
+                }
+            if (!any) break;
             if (attempt >= 4) return fail(KC_ENOMEM, "device hash table overflow");
         }
-        // first setters -> seen tier -> solid tier
-        const uint64_t size_bits = c->num_words * 64;
-        KC_TRY(c->d_fs.ensure(size_bits * sizeof(Ev)));
-        TIMED(KID_FS_CLEAR, (k_fill<Ev><<<grid_for(size_bits), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), size_bits, (Ev)~(Ev)0)));
-        TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev><<<grid_for(c->cap), BLOCK, 0, st>>>(
-                                    c->d_tab.as<Slot<NW, Ev>>(), c->cap, c->d_fs.as<Ev>(), c->M,
-                                    c->d_counters.as<unsigned long long>() + 1)));
-        TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for(c->num_words * 64), BLOCK, 0, st>>>(
-                            c->d_fs.as<Ev>(), c->num_words, c->d_seen.as<uint64_t>(), c->d_solid.as<uint64_t>())));
-        TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<grid_for(c->cap), BLOCK, 0, st>>>(
-                               c->d_tab.as<Slot<NW, Ev>>(), c->cap, c->d_fs.as<Ev>(), c->d_solid.as<uint64_t>(), c->M)));
-        KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipStreamSynchronize(st));
-        c->info.n_distinct = c->h_stage[0];
-        c->info.table_capacity = c->cap;
-        return KC_OK;
+        // first setters -> seen tier -> solid tier, one group of libraries at a time
+        for (auto& g : c->groups) {
+            const auto& lb = c->libs[g.first];
+            const auto& le = c->libs[g.second - 1];
+            const uint64_t s_begin = lb.bucket0 * SPB, s_end = (le.bucket0 + le.nbuckets) * SPB;
+            const uint64_t w_begin = lb.word0, w_end = le.word0 + le.num_words;
+            const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
+            TIMED(KID_FS_CLEAR, (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
+            if (s_end > s_begin)
+                TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0,
+                                                                  st>>>(
+                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                                            c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>())));
+            TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for((w_end - w_begin) * 64), BLOCK, 0, st>>>(
+                                w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>())));
+            if (s_end > s_begin)
+                TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                       c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                                       c->d_libs.as<LibDesc>())));
+        }
+        return read_out();
     }
 
-    // Sort (a[, b], cnt) ascending by key; a/b are the table key words.
-    template <int NW>
-    int sort_counts(uint64_t n) {
+    // Sort entries ascending by (library, k-mer).
+    int sort_entries(uint64_t n) {
         hipStream_t st = c->st;
         if (n == 0) return KC_OK;
-        const uint32_t k = c->o.k;
+        const SortLayout& lay = c->lay;
         size_t tb = 0;
-        if (NW == 1) {
-            KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_a.as<uint64_t>(), c->d_a2.as<uint64_t>(),
-                                             c->d_cnt.as<uint32_t>(), c->d_cnt2.as<uint32_t>(), (size_t)n, 0u, 2 * k, st));
+        if (lay.single) {
+            const uint32_t bits = lay.single_bits;
+            KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_lo.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
+                                             c->d_cnt.as<uint32_t>(), c->d_cnt2.as<uint32_t>(), (size_t)n, 0u, bits, st));
             KC_TRY(c->d_temp.ensure(tb));
             tb = c->d_temp.bytes;
-            TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_a.as<uint64_t>(),
-                                                              c->d_a2.as<uint64_t>(), c->d_cnt.as<uint32_t>(),
-                                                              c->d_cnt2.as<uint32_t>(), (size_t)n, 0u, 2 * k, st)));
-            std::swap(c->d_a, c->d_a2);
+            TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_lo.as<uint64_t>(),
+                                                              c->d_lo2.as<uint64_t>(), c->d_cnt.as<uint32_t>(),
+                                                              c->d_cnt2.as<uint32_t>(), (size_t)n, 0u, bits, st)));
+            std::swap(c->d_lo, c->d_lo2);
             std::swap(c->d_cnt, c->d_cnt2);
             return KC_OK;
         }
-        // LSD over the two words.  a, b, cnt are the unsorted columns and stay intact
-        // until the final gathers.
-        //   1. (b, iota) sorted by b        -> perm p1 in d_idx2 (sorted b discarded)
-        //   2. a1 = a[p1]                    -> d_a2
-        //   3. (a1, p1) stably sorted by a1  -> sorted a in d_b2, perm p2 in d_idx
-        //   4. b[p2] -> d_a2, cnt[p2] -> d_cnt2
+        // LSD over the two words; hi, lo, cnt stay intact until the final gathers.
+        //   1. (lo, iota) sorted by lo        -> perm p1 in d_idx2 (sorted lo discarded)
+        //   2. h1 = hi[p1]                    -> d_hi2
+        //   3. (h1, p1) stably sorted by h1   -> sorted hi in d_lo2, perm p2 in d_idx
+        //   4. lo[p2] -> d_hi2, cnt[p2] -> d_cnt2
         KC_TRY(c->d_idx.ensure(n * 4));
         KC_TRY(c->d_idx2.ensure(n * 4));
         k_iota<<<grid_for(n), BLOCK, 0, st>>>(c->d_idx.as<uint32_t>(), n);
         KC_TRY(hipGetLastError());
-        const uint32_t abits = 2 * k - 63;
         size_t need1 = 0, need2 = 0;
-        KC_TRY(rocprim::radix_sort_pairs(nullptr, need1, c->d_b.as<uint64_t>(), c->d_b2.as<uint64_t>(),
-                                         c->d_idx.as<uint32_t>(), c->d_idx2.as<uint32_t>(), (size_t)n, 0u, 63u, st));
-        KC_TRY(rocprim::radix_sort_pairs(nullptr, need2, c->d_a2.as<uint64_t>(), c->d_b2.as<uint64_t>(),
-                                         c->d_idx2.as<uint32_t>(), c->d_idx.as<uint32_t>(), (size_t)n, 0u, abits, st));
+        KC_TRY(rocprim::radix_sort_pairs(nullptr, need1, c->d_lo.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
+                                         c->d_idx.as<uint32_t>(), c->d_idx2.as<uint32_t>(), (size_t)n, 0u, lay.lo_bits,
+                                         st));
+        KC_TRY(rocprim::radix_sort_pairs(nullptr, need2, c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
+                                         c->d_idx2.as<uint32_t>(), c->d_idx.as<uint32_t>(), (size_t)n, 0u, lay.hi_bits,
+                                         st));
         KC_TRY(c->d_temp.ensure(std::max(need1, need2)));
         tb = c->d_temp.bytes;
-        TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_b.as<uint64_t>(),
-                                                          c->d_b2.as<uint64_t>(), c->d_idx.as<uint32_t>(),
-                                                          c->d_idx2.as<uint32_t>(), (size_t)n, 0u, 63u, st)));
-        TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(c->d_a.as<uint64_t>(),
+        TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_lo.as<uint64_t>(),
+                                                          c->d_lo2.as<uint64_t>(), c->d_idx.as<uint32_t>(),
+                                                          c->d_idx2.as<uint32_t>(), (size_t)n, 0u, lay.lo_bits, st)));
+        TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(c->d_hi.as<uint64_t>(),
                                                                            c->d_idx2.as<uint32_t>(),
-                                                                           c->d_a2.as<uint64_t>(), n)));
+                                                                           c->d_hi2.as<uint64_t>(), n)));
         tb = c->d_temp.bytes;
-        TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_a2.as<uint64_t>(),
-                                                          c->d_b2.as<uint64_t>(), c->d_idx2.as<uint32_t>(),
-                                                          c->d_idx.as<uint32_t>(), (size_t)n, 0u, abits, st)));
-        TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(c->d_b.as<uint64_t>(),
+        TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_hi2.as<uint64_t>(),
+                                                          c->d_lo2.as<uint64_t>(), c->d_idx2.as<uint32_t>(),
+                                                          c->d_idx.as<uint32_t>(), (size_t)n, 0u, lay.hi_bits, st)));
+        TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(c->d_lo.as<uint64_t>(),
                                                                            c->d_idx.as<uint32_t>(),
-                                                                           c->d_a2.as<uint64_t>(), n)));
+                                                                           c->d_hi2.as<uint64_t>(), n)));
         TIMED(KID_SORT, (k_gather<uint32_t><<<grid_for(n), BLOCK, 0, st>>>(c->d_cnt.as<uint32_t>(),
                                                                            c->d_idx.as<uint32_t>(),
                                                                            c->d_cnt2.as<uint32_t>(), n)));
-        std::swap(c->d_a, c->d_b2);
-        std::swap(c->d_b, c->d_a2);
+        std::swap(c->d_hi, c->d_lo2);
+        std::swap(c->d_lo, c->d_hi2);
         std::swap(c->d_cnt, c->d_cnt2);
         return KC_OK;
     }
@@ -879,8 +1097,12 @@ struct Run {
         unsigned long long* ctr = c->d_counters.as<unsigned long long>();
         k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
         KC_TRY(hipGetLastError());
-        c->info.corrected_reads = 0;
+        int rc = clear_out();
+        if (rc) return rc;
         const bool ec = c->o.error_correct != 0;
+        c->lay = make_layout(c->o.k, L());
+        if (c->lay.hi_bits > 64)
+            return fail(KC_EUNSUPPORTED, "too many libraries for this k in one context (need 2k - 63 + log2(libraries) <= 64)");
         if (ec) {
             // corr = data; Hammer writes the reads it modifies
             KC_TRY(c->d_corr.ensure(c->d_data.bytes));
@@ -888,120 +1110,73 @@ struct Run {
             if (c->n_active) {
                 TIMED(KID_HAMMER, (k_hammer<WIDE><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
                                       c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(),
-                                      c->n_active, c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(), c->o.k,
-                                      c->d_solid.as<uint64_t>(), c->M, ctr + 2, c->d_counters.as<uint32_t>() + 6)));
+                                      c->n_active, c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(),
+                                      c->d_rlib.as<uint32_t>(), c->d_blooms.as<LibBloom>(), c->o.k,
+                                      &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
+                                      c->d_counters.as<uint32_t>() + 6)));
             }
-            // recount Bloom-solid windows of the corrected reads into a cleared table
-            for (int attempt = 0;; ++attempt) {
-                if (attempt) {
-                    const uint64_t cap2 = c->cap * 2;  // a multiple of every bucket size
-                    KC_TRY(c->d_tab.ensure(cap2 * sizeof(Slot<NW, uint32_t>)));
-                    c->cap = cap2;
-                }
-                TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(c->cap), BLOCK, 0, st>>>(
-                                     c->d_tab.as<Slot<NW, uint32_t>>(), c->cap)));
-                k_fill<uint32_t><<<1, 64, 0, st>>>(c->d_counters.as<uint32_t>(), 1, 0u);
-                KC_TRY(hipGetLastError());
-                if (c->n_tiles) {
-                    TIMED(KID_RECOUNT, (k_insert<WIDE, uint32_t, false, true><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
-                                           c->d_corr.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
-                                           c->d_act_off.as<uint64_t>(), c->d_tile_first.as<uint32_t>(), c->n_active,
-                                           c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(),
-                                           c->cap / slots_per_bucket<NW, uint32_t>(),
-                                           (uint32_t)std::min<uint64_t>(c->cap / slots_per_bucket<NW, uint32_t>(), 1u << 20),
-                                           c->d_solid.as<uint64_t>(),
-                                           c->M, c->d_counters.as<uint32_t>())));
-                }
-                KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 32, hipMemcpyDeviceToHost, st));
-                KC_TRY(hipStreamSynchronize(st));
-                if ((c->h_stage[0] & 0xFFFFFFFFull) == 0) break;
-                if (attempt >= 4) return fail(KC_ENOMEM, "device hash table overflow (recount)");
+            const uint64_t SPB = slots_per_bucket<NW, uint32_t>();
+            const uint64_t n_slots = c->total_buckets * SPB;
+            TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
+                                 c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
+            if (c->n_tiles) {
+                TIMED(KID_RECOUNT, (k_insert<WIDE, uint32_t, false, true><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
+                                       c->d_corr.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
+                                       c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active,
+                                       c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(), c->d_libs.as<LibDesc>(),
+                                       1u << 20, c->d_out.as<LibOut>())));
             }
+            rc = read_out();
+            if (rc) return rc;
+            KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 32, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipStreamSynchronize(st));
             if (c->h_stage[3] & 0xFFFFFFFFull)
                 return fail(KC_EUNSUPPORTED, "error_correct supports reads of at most 1024 bases");
-            c->info.corrected_reads = c->h_stage[2];
+            for (uint32_t l = 0; l < L(); ++l)
+                if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
+                    return fail(KC_ENOMEM, "device hash table overflow (recount)");
         }
-        const uint64_t cap = c->cap;
-        KC_TRY(c->d_a.ensure(cap * 8));
-        KC_TRY(c->d_b.ensure(cap * 8));
+        const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
+        const uint64_t cap = std::max<uint64_t>(slots, 1);
+        KC_TRY(c->d_hi.ensure(cap * 8));
+        KC_TRY(c->d_lo.ensure(cap * 8));
         KC_TRY(c->d_cnt.ensure(cap * 4));
-        KC_TRY(c->d_a2.ensure(cap * 8));
-        KC_TRY(c->d_b2.ensure(cap * 8));
+        KC_TRY(c->d_hi2.ensure(cap * 8));
+        KC_TRY(c->d_lo2.ensure(cap * 8));
         KC_TRY(c->d_cnt2.ensure(cap * 4));
         if (ec) {
-            TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(cap), BLOCK, 0, st>>>(
-                                   c->d_tab.as<Slot<NW, uint32_t>>(), cap, c->d_solid.as<uint64_t>(), c->M,
-                                   c->d_a.as<uint64_t>(), c->d_b.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4)));
+            TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                   c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
+                                   c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
         } else {
-            TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(cap), BLOCK, 0, st>>>(
-                                   c->d_tab.as<Slot<NW, Ev>>(), cap, c->d_solid.as<uint64_t>(), c->M,
-                                   c->d_a.as<uint64_t>(), c->d_b.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4)));
+            TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                   c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
+                                   c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
         }
         KC_TRY(hipMemcpyAsync(c->h_stage, ctr + 4, 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
         const uint64_t n = c->h_stage[0];
-        c->n_counts = n;
-        int rc = sort_counts<NW>(n);
+        c->n_entries = n;
+        rc = sort_entries(n);
         if (rc) return rc;
-        // cutoff (assemble.rs:692-735) from the count histogram
-        KC_TRY(c->d_hist.ensure(HIST_BINS * 4));
-        k_fill<uint32_t><<<grid_for(HIST_BINS), BLOCK, 0, st>>>(c->d_hist.as<uint32_t>(), HIST_BINS, 0u);
+        // cutoff (assemble.rs:692-735) per library from its count histogram
+        const size_t nh = (size_t)L() * HIST_BINS;
+        KC_TRY(c->d_hist.ensure(nh * 4));
+        k_fill<uint32_t><<<grid_for(nh), BLOCK, 0, st>>>(c->d_hist.as<uint32_t>(), nh, 0u);
         KC_TRY(hipGetLastError());
-        if (n) TIMED(KID_HIST, (k_count_hist<<<grid_for(n, BLOCK, 1024), BLOCK, 0, st>>>(c->d_cnt.as<uint32_t>(), n,
-                                                                                        c->d_hist.as<uint32_t>())));
-        std::vector<uint32_t> hist(HIST_BINS);
-        KC_TRY(hipMemcpyAsync(hist.data(), c->d_hist.p, HIST_BINS * 4, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipStreamSynchronize(st));
-        uint32_t top = 0;
-        const uint32_t cutoff = cutoff_from_hist(hist, n, c->o.min_coverage, &top);
-        c->info.cutoff = cutoff;
-        c->info.top_cov = top;
+        if (n) TIMED(KID_HIST, (k_count_hist<<<grid_for(n, BLOCK * 16, 4096), BLOCK, 0, st>>>(
+                                   c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), n, c->lay,
+                                   c->d_hist.as<uint32_t>())));
+        k_cutoff<<<grid_for(L(), 64), 64, 0, st>>>(c->d_hist.as<uint32_t>(), L(), c->d_mincov.as<double>(),
+                                                   c->d_out.as<LibOut>());
+        KC_TRY(hipGetLastError());
         KC_TRY(c->d_sflags.ensure(n + 1));
-        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr + 5, 1, 0ull);
-        KC_TRY(hipGetLastError());
-        if (n) TIMED(KID_FLAGS, (k_solid_flags<<<grid_for(n), BLOCK, 0, st>>>(c->d_cnt.as<uint32_t>(), n, cutoff,
-                                                                            c->d_sflags.as<uint8_t>(), ctr + 5)));
-        KC_TRY(hipMemcpyAsync(c->h_stage, ctr + 5, 8, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipStreamSynchronize(st));
-        c->info.n_solid = c->h_stage[0];
-        c->info.n_counts = n;
-        return KC_OK;
-    }
-
-    // determine_solid_kmer_cutoff (assemble.rs:692-735) evaluated on a histogram of
-    // min(count, HIST_BINS-1).  The 95th-percentile count `top` only enters through
-    // top >= 30 and clamp(top/4, 10, 35), which the capped histogram decides exactly.
-    static uint32_t cutoff_from_hist(const std::vector<uint32_t>& hist, uint64_t n, double min_cov, uint32_t* top_out) {
-        *top_out = 0;
-        if (min_cov > 5.0) return (uint32_t)std::max(min_cov * 0.35, 2.0);
-        if (n <= 100) return 2;
-        const uint64_t rank = n * 95 / 100;  // index into the ascending counts
-        uint64_t cum = 0;
-        uint32_t top = HIST_BINS - 1;
-        for (uint32_t b = 0; b < HIST_BINS; ++b) {
-            cum += hist[b];
-            if (cum > rank) {
-                top = b;
-                break;
-            }
-        }
-        *top_out = top;
-        if (top < 30) return 2;
-        const uint32_t mvs = std::min<uint32_t>(std::max<uint32_t>(top / 4, 10), 35);
-        uint32_t valley = 2;
-        uint64_t min_val = ~0ull;
-        bool found = false;
-        for (uint32_t cc = 2; cc <= mvs; ++cc) {
-            const uint64_t h = hist[cc];
-            if (h <= min_val) {
-                min_val = h;
-                valley = cc;
-            } else if (h > min_val * 2 && cc > valley + 2) {
-                found = true;
-                break;
-            }
-        }
-        return found ? valley : 2;
+        if (n) TIMED(KID_FLAGS, (k_solid_flags<<<grid_for(n, BLOCK * 16, 4096), BLOCK, 0, st>>>(
+                                    c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), n, c->lay,
+                                    L(), c->d_out.as<LibOut>(), c->d_sflags.as<uint8_t>(), c->d_out.as<LibOut>())));
+        return read_out();
     }
 };
 
@@ -1017,6 +1192,26 @@ struct Run {
         if (_e != hipSuccess) return ctx->fail(_e == hipErrorOutOfMemory ? KC_ENOMEM : KC_EHIP,     \
                                                std::string(#expr) + ": " + hipGetErrorString(_e));  \
     } while (0)
+
+static int add_library(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets,
+                       const uint32_t* lengths, size_t n_reads, uint64_t bloom_bits, double min_cov) {
+    if (n_reads && (!offsets || !lengths)) return ctx->fail(KC_EINVAL, "null offsets/lengths");
+    if (data_len && !data) return ctx->fail(KC_EINVAL, "null data");
+    for (size_t i = 0; i < n_reads; ++i) {
+        const uint64_t nb = ((uint64_t)lengths[i] + 3) / 4;
+        if (offsets[i] > data_len || nb > data_len - offsets[i])
+            return ctx->fail(KC_EINVAL, "read " + std::to_string(i) + " extends past data_len");
+    }
+    HostLib lib;
+    lib.data.assign(data, data + data_len);
+    lib.off.assign(offsets, offsets + n_reads);
+    lib.len.assign(lengths, lengths + n_reads);
+    lib.bloom_bits = bloom_bits;
+    lib.min_cov = min_cov;
+    ctx->libs.push_back(std::move(lib));
+    ctx->uploaded = ctx->built = ctx->counted = false;
+    return KC_OK;
+}
 
 extern "C" {
 
@@ -1035,8 +1230,6 @@ int kc_create(kc_ctx** out, const kc_opts* opts) {
         delete ctx;
         return KC_EHIP;
     }
-    ctx->num_words = std::max<uint64_t>((opts->bloom_bits + 63) / 64, 1);
-    ctx->M = make_modulus(ctx->num_words * 64);
     ctx->wide = opts->k > 32;
     *out = ctx;
     return KC_OK;
@@ -1046,11 +1239,14 @@ void kc_destroy(kc_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->o.device);
     (void)hipStreamSynchronize(ctx->st);
-    DevBuf* bufs[] = {&ctx->d_data, &ctx->d_off, &ctx->d_len, &ctx->d_nwin, &ctx->d_prefix, &ctx->d_flags,
-                      &ctx->d_iota, &ctx->d_active, &ctx->d_act_ev, &ctx->d_act_off, &ctx->d_tile_first,
-                      &ctx->d_seen, &ctx->d_solid, &ctx->d_fs, &ctx->d_tab, &ctx->d_a, &ctx->d_b, &ctx->d_cnt,
-                      &ctx->d_a2, &ctx->d_b2, &ctx->d_cnt2, &ctx->d_idx, &ctx->d_idx2, &ctx->d_sflags,
-                      &ctx->d_corr, &ctx->d_hll, &ctx->d_hist, &ctx->d_counters, &ctx->d_temp, &ctx->d_keys,
+    DevBuf* bufs[] = {&ctx->d_data,   &ctx->d_off,     &ctx->d_len,     &ctx->d_rlib,     &ctx->d_nwin,
+                      &ctx->d_prefix, &ctx->d_flags,   &ctx->d_iota,    &ctx->d_active,   &ctx->d_act_ev,
+                      &ctx->d_act_off, &ctx->d_act_lib, &ctx->d_tile_first, &ctx->d_lib_read0, &ctx->d_lib_ev0,
+                      &ctx->d_seen,   &ctx->d_solid,   &ctx->d_fs,      &ctx->d_tab,      &ctx->d_libs,
+                      &ctx->d_lib_b0, &ctx->d_lib_w0,  &ctx->d_blooms,  &ctx->d_out,      &ctx->d_mincov,
+                      &ctx->d_hi,     &ctx->d_lo,      &ctx->d_cnt,     &ctx->d_hi2,      &ctx->d_lo2,
+                      &ctx->d_cnt2,   &ctx->d_idx,     &ctx->d_idx2,    &ctx->d_sflags,   &ctx->d_corr,
+                      &ctx->d_hll,    &ctx->d_hist,    &ctx->d_counters, &ctx->d_temp,    &ctx->d_keys,
                       &ctx->d_out8};
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1064,63 +1260,79 @@ const char* kc_last_error(const kc_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 int kc_reads_upload(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets,
                     const uint32_t* lengths, size_t n_reads) {
     if (!ctx) return KC_EINVAL;
-    if (n_reads && (!offsets || !lengths)) return ctx->fail(KC_EINVAL, "null offsets/lengths");
-    if (data_len && !data) return ctx->fail(KC_EINVAL, "null data");
-    if (n_reads > 0xFFFFFFFFull) return ctx->fail(KC_EUNSUPPORTED, "more than 2^32-1 reads in one context");
-    for (size_t i = 0; i < n_reads; ++i) {
-        const uint64_t nb = ((uint64_t)lengths[i] + 3) / 4;
-        if (offsets[i] > data_len || nb > data_len - offsets[i])
-            return ctx->fail(KC_EINVAL, "read " + std::to_string(i) + " extends past data_len");
-    }
-    CTX_TRY(hipSetDevice(ctx->o.device));
-    CTX_TRY(ctx->d_data.ensure(data_len + DATA_PAD));
-    CTX_TRY(ctx->d_off.ensure(n_reads * 8 + 8));
-    CTX_TRY(ctx->d_len.ensure(n_reads * 4 + 4));
-    CTX_TRY(hipMemsetAsync(ctx->d_data.p, 0, ctx->d_data.bytes, ctx->st));
-    if (data_len) CTX_TRY(hipMemcpyAsync(ctx->d_data.p, data, data_len, hipMemcpyHostToDevice, ctx->st));
-    if (n_reads) {
-        CTX_TRY(hipMemcpyAsync(ctx->d_off.p, offsets, n_reads * 8, hipMemcpyHostToDevice, ctx->st));
-        CTX_TRY(hipMemcpyAsync(ctx->d_len.p, lengths, n_reads * 4, hipMemcpyHostToDevice, ctx->st));
-    }
-    CTX_TRY(hipStreamSynchronize(ctx->st));
-    ctx->data_len = data_len;
-    ctx->n_reads = n_reads;
-    ctx->built = ctx->counted = false;
+    ctx->libs.clear();
+    return add_library(ctx, data, data_len, offsets, lengths, n_reads, ctx->o.bloom_bits, ctx->o.min_coverage);
+}
+
+int kc_library_add(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets, const uint32_t* lengths,
+                   size_t n_reads, uint64_t bloom_bits, double min_coverage, uint32_t* lib) {
+    if (!ctx) return KC_EINVAL;
+    if (ctx->libs.size() >= 65535) return ctx->fail(KC_EUNSUPPORTED, "at most 65535 libraries per context");
+    const int rc = add_library(ctx, data, data_len, offsets, lengths, n_reads, bloom_bits, min_coverage);
+    if (rc == KC_OK && lib) *lib = (uint32_t)ctx->libs.size() - 1;
+    return rc;
+}
+
+int kc_library_count(const kc_ctx* ctx, uint32_t* n) {
+    if (!ctx || !n) return KC_EINVAL;
+    *n = (uint32_t)ctx->libs.size();
     return KC_OK;
 }
 
 int kc_bloom_build(kc_ctx* ctx) {
     if (!ctx) return KC_EINVAL;
+    if (ctx->libs.empty()) return ctx->fail(KC_ESTATE, "no reads uploaded");
     CTX_TRY(hipSetDevice(ctx->o.device));
     ctx->reset_marks();
     ctx->built = ctx->counted = false;
-    ctx->info = kc_info{};
-    hipEvent_t e0 = ctx->new_event(), e1 = ctx->new_event();
-    CTX_TRY(hipEventRecord(e0, ctx->st));
-    CTX_TRY(ctx->d_seen.ensure(ctx->num_words * 8));
-    CTX_TRY(ctx->d_solid.ensure(ctx->num_words * 8));
     Run run(ctx);
-    int rc = run.prep();
-    if (rc) return rc;
-    if (ctx->n_occ == 0) {
-        k_fill<uint64_t><<<grid_for(ctx->num_words), BLOCK, 0, ctx->st>>>(ctx->d_seen.as<uint64_t>(), ctx->num_words, 0ull);
-        k_fill<uint64_t><<<grid_for(ctx->num_words), BLOCK, 0, ctx->st>>>(ctx->d_solid.as<uint64_t>(), ctx->num_words, 0ull);
-        CTX_TRY(hipGetLastError());
-        ctx->cap = 0;
-    } else {
-        ctx->ev64 = ctx->n_occ >= (1ull << 30) - 1;
-        if (!ctx->wide) rc = ctx->ev64 ? run.bloom_build<false, uint64_t>() : run.bloom_build<false, uint32_t>();
-        else rc = ctx->ev64 ? run.bloom_build<true, uint64_t>() : run.bloom_build<true, uint32_t>();
+    int rc = KC_OK;
+    if (!ctx->uploaded) {
+        rc = run.upload();
         if (rc) return rc;
     }
+    hipEvent_t e0 = ctx->new_event(), e1 = ctx->new_event();
+    CTX_TRY(hipEventRecord(e0, ctx->st));
+    rc = run.prep();
+    if (rc) return rc;
+    for (auto& lib : ctx->libs) {
+        const uint64_t occ = lib.info.n_occ;
+        lib.info = kc_info{};
+        lib.info.n_occ = occ;
+        lib.info.n_reads = lib.len.size();
+    }
+    ctx->ev64 = false;
+    for (auto& lib : ctx->libs) ctx->ev64 = ctx->ev64 || lib.info.n_occ >= (1ull << 30) - 1;
+    if (ctx->n_occ == 0) {
+        std::vector<uint32_t> grow(ctx->libs.size(), 0);
+        for (auto& lib : ctx->libs) lib.est = 0;
+        rc = ctx->wide ? run.plan<2, uint32_t>(grow) : run.plan<1, uint32_t>(grow);
+        if (rc) return rc;
+        k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_seen.as<uint64_t>(),
+                                                                           ctx->total_words, 0ull);
+        k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_solid.as<uint64_t>(),
+                                                                           ctx->total_words, 0ull);
+        CTX_TRY(hipGetLastError());
+        rc = run.clear_out();
+        if (rc) return rc;
+        rc = run.read_out();
+    } else if (!ctx->wide) {
+        rc = ctx->ev64 ? run.bloom_build<false, uint64_t>() : run.bloom_build<false, uint32_t>();
+    } else {
+        rc = ctx->ev64 ? run.bloom_build<true, uint64_t>() : run.bloom_build<true, uint32_t>();
+    }
+    if (rc) return rc;
     CTX_TRY(hipEventRecord(e1, ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
     float ms = 0;
     CTX_TRY(hipEventElapsedTime(&ms, e0, e1));
-    ctx->info.ms_stage2 = ms;
-    ctx->info.n_reads = ctx->n_reads;
-    ctx->info.n_occ = ctx->n_occ;
-    ctx->info.num_words = ctx->num_words;
+    for (uint32_t l = 0; l < ctx->libs.size(); ++l) {
+        auto& lib = ctx->libs[l];
+        lib.info.ms_stage2 = ms;
+        lib.info.num_words = lib.num_words;
+        lib.info.n_distinct = ctx->h_out[l].n_distinct;
+        lib.info.table_capacity = lib.nbuckets * (64 / (ctx->wide || ctx->ev64 ? 32 : 16));
+    }
     ctx->built = true;
     return KC_OK;
 }
@@ -1134,12 +1346,14 @@ int kc_count(kc_ctx* ctx, kc_info* info) {
     Run run(ctx);
     int rc = KC_OK;
     if (ctx->n_occ == 0) {
-        ctx->n_counts = 0;
-        std::vector<uint32_t> hist(HIST_BINS, 0);
-        uint32_t top = 0;
-        ctx->info.cutoff = Run::cutoff_from_hist(hist, 0, ctx->o.min_coverage, &top);
-        ctx->info.top_cov = top;
-        ctx->info.n_counts = ctx->info.n_solid = 0;
+        ctx->n_entries = 0;
+        ctx->lay = make_layout(ctx->o.k, (uint32_t)ctx->libs.size());
+        ctx->h_out.assign(ctx->libs.size(), LibOut{});
+        for (uint32_t l = 0; l < ctx->libs.size(); ++l) {
+            std::vector<uint32_t> hist(HIST_BINS, 0);
+            uint32_t top = 0;
+            ctx->h_out[l].cutoff = cutoff_rule(hist.data(), 0, ctx->libs[l].min_cov, &top);
+        }
         if (ctx->o.error_correct) {
             CTX_TRY(ctx->d_corr.ensure(ctx->d_data.bytes));
             CTX_TRY(hipMemcpyAsync(ctx->d_corr.p, ctx->d_data.p, ctx->d_data.bytes, hipMemcpyDeviceToDevice, ctx->st));
@@ -1154,10 +1368,23 @@ int kc_count(kc_ctx* ctx, kc_info* info) {
     CTX_TRY(hipStreamSynchronize(ctx->st));
     float ms = 0;
     CTX_TRY(hipEventElapsedTime(&ms, e0, e1));
-    ctx->info.ms_stage3 = ms;
+    uint64_t e = 0;
+    for (uint32_t l = 0; l < ctx->libs.size(); ++l) {
+        auto& lib = ctx->libs[l];
+        const LibOut& o = ctx->h_out[l];
+        lib.info.ms_stage3 = ms;
+        lib.info.n_counts = o.n_counts;
+        lib.info.n_solid = o.n_solid;
+        lib.info.cutoff = o.cutoff;
+        lib.info.top_cov = o.top;
+        lib.info.corrected_reads = o.corrected;
+        lib.entry0 = e;
+        e += o.n_counts;
+    }
+    if (e != ctx->n_entries) return ctx->fail(KC_EHIP, "per-library counts do not add up to the entries");
     ctx->counted = true;
     if (ctx->timing()) ctx->collect_marks();
-    if (info) *info = ctx->info;
+    if (info) *info = ctx->libs[0].info;
     return KC_OK;
 }
 
@@ -1167,84 +1394,135 @@ int kc_run(kc_ctx* ctx, kc_info* info) {
     return kc_count(ctx, info);
 }
 
-int kc_info_get(const kc_ctx* ctx, kc_info* info) {
-    if (!ctx || !info) return KC_EINVAL;
-    *info = ctx->info;
+int kc_library_info(const kc_ctx* ctx, uint32_t lib, kc_info* info) {
+    if (!ctx || !info || lib >= ctx->libs.size()) return KC_EINVAL;
+    *info = ctx->libs[lib].info;
     return KC_OK;
 }
 
-int kc_bloom_export(kc_ctx* ctx, uint64_t* seen, uint64_t* solid, size_t num_words) {
+int kc_info_get(const kc_ctx* ctx, kc_info* info) {
+    if (!ctx || !info) return KC_EINVAL;
+    if (ctx->libs.empty()) {
+        *info = kc_info{};
+        return KC_OK;
+    }
+    return kc_library_info(ctx, 0, info);
+}
+
+int kc_library_bloom_export(kc_ctx* ctx, uint32_t lib, uint64_t* seen, uint64_t* solid, size_t num_words) {
     if (!ctx) return KC_EINVAL;
-    if (!ctx->built) return ctx->fail(KC_ESTATE, "kc_bloom_export before kc_bloom_build");
-    if (num_words != ctx->num_words) return ctx->fail(KC_EINVAL, "num_words mismatch");
+    if (!ctx->built) return ctx->fail(KC_ESTATE, "bloom export before kc_bloom_build");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    const HostLib& L = ctx->libs[lib];
+    if (num_words != L.num_words) return ctx->fail(KC_EINVAL, "num_words mismatch");
     CTX_TRY(hipSetDevice(ctx->o.device));
-    if (seen) CTX_TRY(hipMemcpyAsync(seen, ctx->d_seen.p, num_words * 8, hipMemcpyDeviceToHost, ctx->st));
-    if (solid) CTX_TRY(hipMemcpyAsync(solid, ctx->d_solid.p, num_words * 8, hipMemcpyDeviceToHost, ctx->st));
+    if (seen)
+        CTX_TRY(hipMemcpyAsync(seen, ctx->d_seen.as<uint64_t>() + L.word0, num_words * 8, hipMemcpyDeviceToHost, ctx->st));
+    if (solid)
+        CTX_TRY(hipMemcpyAsync(solid, ctx->d_solid.as<uint64_t>() + L.word0, num_words * 8, hipMemcpyDeviceToHost,
+                               ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
     return KC_OK;
 }
 
-int kc_bloom_query(kc_ctx* ctx, const uint64_t* keys4, size_t n, uint8_t* out) {
+int kc_bloom_export(kc_ctx* ctx, uint64_t* seen, uint64_t* solid, size_t num_words) {
+    return kc_library_bloom_export(ctx, 0, seen, solid, num_words);
+}
+
+int kc_library_bloom_query(kc_ctx* ctx, uint32_t lib, const uint64_t* keys4, size_t n, uint8_t* out) {
     if (!ctx) return KC_EINVAL;
-    if (!ctx->built) return ctx->fail(KC_ESTATE, "kc_bloom_query before kc_bloom_build");
+    if (!ctx->built) return ctx->fail(KC_ESTATE, "bloom query before kc_bloom_build");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
     if (n == 0) return KC_OK;
     if (!keys4 || !out) return ctx->fail(KC_EINVAL, "null keys/out");
     CTX_TRY(hipSetDevice(ctx->o.device));
     CTX_TRY(ctx->d_keys.ensure(n * 32));
     CTX_TRY(ctx->d_out8.ensure(n));
     CTX_TRY(hipMemcpyAsync(ctx->d_keys.p, keys4, n * 32, hipMemcpyHostToDevice, ctx->st));
-    k_query<<<grid_for(n), BLOCK, 0, ctx->st>>>(ctx->d_keys.as<uint64_t>(), n, ctx->M, ctx->d_solid.as<uint64_t>(),
-                                                ctx->d_out8.as<uint8_t>());
-    CTX_TRY(hipGetLastError());
+    const HostLib& L = ctx->libs[lib];
+    CTX_TRY(query_solid(ctx->d_keys.as<uint64_t>(), n, make_modulus(L.num_words * 64),
+                        ctx->d_solid.as<uint64_t>() + L.word0, ctx->d_out8.as<uint8_t>(), ctx->st));
     CTX_TRY(hipMemcpyAsync(out, ctx->d_out8.p, n, hipMemcpyDeviceToHost, ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
     return KC_OK;
 }
 
-int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask, size_t n) {
+int kc_bloom_query(kc_ctx* ctx, const uint64_t* keys4, size_t n, uint8_t* out) {
+    return kc_library_bloom_query(ctx, 0, keys4, n, out);
+}
+
+int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask,
+                             size_t n) {
     if (!ctx) return KC_EINVAL;
-    if (!ctx->counted) return ctx->fail(KC_ESTATE, "kc_counts_export before kc_count");
-    if (n != ctx->n_counts) return ctx->fail(KC_EINVAL, "n mismatch");
+    if (!ctx->counted) return ctx->fail(KC_ESTATE, "counts export before kc_count");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    const HostLib& L = ctx->libs[lib];
+    if (n != L.info.n_counts) return ctx->fail(KC_EINVAL, "n mismatch");
     if (n == 0) return KC_OK;
     CTX_TRY(hipSetDevice(ctx->o.device));
-    if (counts) CTX_TRY(hipMemcpyAsync(counts, ctx->d_cnt.p, n * 4, hipMemcpyDeviceToHost, ctx->st));
-    if (solid_mask) CTX_TRY(hipMemcpyAsync(solid_mask, ctx->d_sflags.p, n, hipMemcpyDeviceToHost, ctx->st));
-    std::vector<uint64_t> a, b;
+    const uint64_t e0 = L.entry0;
+    if (counts) CTX_TRY(hipMemcpyAsync(counts, ctx->d_cnt.as<uint32_t>() + e0, n * 4, hipMemcpyDeviceToHost, ctx->st));
+    if (solid_mask)
+        CTX_TRY(hipMemcpyAsync(solid_mask, ctx->d_sflags.as<uint8_t>() + e0, n, hipMemcpyDeviceToHost, ctx->st));
+    std::vector<uint64_t> hi, lo;
+    const SortLayout& lay = ctx->lay;
     if (keys4) {
-        a.resize(n);
-        CTX_TRY(hipMemcpyAsync(a.data(), ctx->d_a.p, n * 8, hipMemcpyDeviceToHost, ctx->st));
-        if (ctx->wide) {
-            b.resize(n);
-            CTX_TRY(hipMemcpyAsync(b.data(), ctx->d_b.p, n * 8, hipMemcpyDeviceToHost, ctx->st));
+        lo.resize(n);
+        CTX_TRY(hipMemcpyAsync(lo.data(), ctx->d_lo.as<uint64_t>() + e0, n * 8, hipMemcpyDeviceToHost, ctx->st));
+        if (!lay.single) {
+            hi.resize(n);
+            CTX_TRY(hipMemcpyAsync(hi.data(), ctx->d_hi.as<uint64_t>() + e0, n * 8, hipMemcpyDeviceToHost, ctx->st));
         }
     }
     CTX_TRY(hipStreamSynchronize(ctx->st));
     if (keys4) {
+        const uint32_t k = lay.k;
         for (size_t i = 0; i < n; ++i) {
             uint64_t* w = keys4 + 4 * i;
-            if (!ctx->wide) {
-                w[0] = a[i];
-                w[1] = 0;
-            } else {  // V = A * 2^63 + B
-                w[0] = b[i] | (a[i] << 63);
-                w[1] = a[i] >> 1;
+            uint64_t vlo, vhi;
+            if (lay.single) {  // (lib << 2k) | V, or V itself when 2k == 64
+                vlo = 2 * k >= 64 ? lo[i] : (lo[i] & ((1ull << (2 * k)) - 1));
+                vhi = 0;
+            } else if (2 * k <= 63) {
+                vlo = lo[i];
+                vhi = 0;
+            } else {  // hi = (lib << (2k-63)) | (V >> 63), lo = V mod 2^63
+                const uint32_t vb = 2 * k - 63;
+                const uint64_t a = hi[i] & ((1ull << vb) - 1);
+                vlo = lo[i] | (a << 63);
+                vhi = a >> 1;
             }
+            w[0] = vlo;
+            w[1] = vhi;
             w[2] = w[3] = 0;
         }
     }
     return KC_OK;
 }
 
-int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len) {
+int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask, size_t n) {
+    return kc_library_counts_export(ctx, 0, keys4, counts, solid_mask, n);
+}
+
+int kc_library_reads_export(kc_ctx* ctx, uint32_t lib, uint8_t* data, size_t data_len) {
     if (!ctx) return KC_EINVAL;
-    if (data_len != ctx->data_len) return ctx->fail(KC_EINVAL, "data_len mismatch");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    const HostLib& L = ctx->libs[lib];
+    if (data_len != L.data.size()) return ctx->fail(KC_EINVAL, "data_len mismatch");
     if (data_len == 0) return KC_OK;
     if (!data) return ctx->fail(KC_EINVAL, "null data");
+    if (!(ctx->o.error_correct && ctx->counted) || !ctx->uploaded) {
+        std::memcpy(data, L.data.data(), data_len);
+        return KC_OK;
+    }
     CTX_TRY(hipSetDevice(ctx->o.device));
-    const void* src = (ctx->o.error_correct && ctx->counted) ? ctx->d_corr.p : ctx->d_data.p;
-    CTX_TRY(hipMemcpyAsync(data, src, data_len, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipMemcpyAsync(data, ctx->d_corr.as<uint8_t>() + L.data0, data_len, hipMemcpyDeviceToHost, ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
     return KC_OK;
+}
+
+int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len) {
+    return kc_library_reads_export(ctx, 0, data, data_len);
 }
 
 int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap) {
@@ -1258,183 +1536,6 @@ int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t*
         ++n;
     }
     return n;
-}
-
-}  // extern "C"
-
-// ---------------------------------------------------------------------------
-// C ABI: standalone filter
-// ---------------------------------------------------------------------------
-
-struct kc_filter {
-    int device = 0;
-    hipStream_t st = nullptr;
-    uint64_t num_words = 0;
-    Modulus M{};
-    DevBuf d_seen, d_solid, d_fs, d_keys, d_prom;
-    std::string err;
-    int fail(int code, const std::string& m) {
-        err = m;
-        return code;
-    }
-};
-
-#define F_TRY(expr)                                                                              \
-    do {                                                                                         \
-        hipError_t _e = (expr);                                                                  \
-        if (_e != hipSuccess) return f->fail(_e == hipErrorOutOfMemory ? KC_ENOMEM : KC_EHIP,    \
-                                             std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
-
-extern "C" {
-
-int kc_filter_create(kc_filter** out, int32_t device, uint64_t capacity_bits) {
-    if (!out) return KC_EINVAL;
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_EHIP;
-    if (device < 0 || device >= ndev) return KC_EINVAL;
-    if (hipSetDevice(device) != hipSuccess) return KC_EHIP;
-    kc_filter* f = new kc_filter();
-    f->device = device;
-    f->num_words = std::max<uint64_t>((capacity_bits + 63) / 64, 1);
-    f->M = make_modulus(f->num_words * 64);
-    const uint64_t nw = f->num_words;
-    if (hipStreamCreateWithFlags(&f->st, hipStreamNonBlocking) != hipSuccess || f->d_seen.ensure(nw * 8) ||
-        f->d_solid.ensure(nw * 8) || f->d_fs.ensure(nw * 64 * 4)) {
-        kc_filter_destroy(f);
-        return KC_ENOMEM;
-    }
-    k_fill<uint64_t><<<grid_for(nw), BLOCK, 0, f->st>>>(f->d_seen.as<uint64_t>(), nw, 0ull);
-    k_fill<uint64_t><<<grid_for(nw), BLOCK, 0, f->st>>>(f->d_solid.as<uint64_t>(), nw, 0ull);
-    k_fill<uint32_t><<<grid_for(nw * 64), BLOCK, 0, f->st>>>(f->d_fs.as<uint32_t>(), nw * 64, ~0u);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(f->st) != hipSuccess) {
-        kc_filter_destroy(f);
-        return KC_EHIP;
-    }
-    *out = f;
-    return KC_OK;
-}
-
-void kc_filter_destroy(kc_filter* f) {
-    if (!f) return;
-    (void)hipSetDevice(f->device);
-    if (f->st) (void)hipStreamSynchronize(f->st);
-    f->d_seen.release();
-    f->d_solid.release();
-    f->d_fs.release();
-    f->d_keys.release();
-    f->d_prom.release();
-    if (f->st) (void)hipStreamDestroy(f->st);
-    delete f;
-}
-
-const char* kc_filter_last_error(const kc_filter* f) { return f ? f->err.c_str() : "null filter"; }
-uint64_t kc_filter_memory_usage_bytes(const kc_filter* f) { return f ? f->num_words * 8 * 2 : 0; }
-uint64_t kc_filter_num_words(const kc_filter* f) { return f ? f->num_words : 0; }
-
-int kc_filter_insert_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* promoted) {
-    if (!f) return KC_EINVAL;
-    if (n == 0) return KC_OK;
-    if (!keys4) return f->fail(KC_EINVAL, "null keys");
-    F_TRY(hipSetDevice(f->device));
-    const size_t batch_max = (size_t)1 << 28;  // 4*t + 3 fits the 32-bit first-setter words
-    for (size_t base = 0; base < n; base += batch_max) {
-        const size_t m = std::min(batch_max, n - base);
-        F_TRY(f->d_keys.ensure(m * 32));
-        F_TRY(f->d_prom.ensure(m));
-        F_TRY(hipMemcpyAsync(f->d_keys.p, keys4 + 4 * base, m * 32, hipMemcpyHostToDevice, f->st));
-        const uint32_t g = grid_for(m);
-        k_ev_first_setter<<<g, BLOCK, 0, f->st>>>(f->d_keys.as<uint64_t>(), m, f->M, f->d_fs.as<uint32_t>());
-        k_ev_promote<<<g, BLOCK, 0, f->st>>>(f->d_keys.as<uint64_t>(), m, f->M, f->d_fs.as<uint32_t>(),
-                                              f->d_seen.as<uint64_t>(), f->d_prom.as<uint8_t>());
-        k_ev_apply<<<g, BLOCK, 0, f->st>>>(f->d_keys.as<uint64_t>(), m, f->M, f->d_prom.as<uint8_t>(),
-                                            f->d_seen.as<uint64_t>(), f->d_solid.as<uint64_t>(), f->d_fs.as<uint32_t>());
-        F_TRY(hipGetLastError());
-        if (promoted) F_TRY(hipMemcpyAsync(promoted + base, f->d_prom.p, m, hipMemcpyDeviceToHost, f->st));
-        F_TRY(hipStreamSynchronize(f->st));
-    }
-    return KC_OK;
-}
-
-int kc_filter_is_solid_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* out) {
-    if (!f) return KC_EINVAL;
-    if (n == 0) return KC_OK;
-    if (!keys4 || !out) return f->fail(KC_EINVAL, "null keys/out");
-    F_TRY(hipSetDevice(f->device));
-    F_TRY(f->d_keys.ensure(n * 32));
-    F_TRY(f->d_prom.ensure(n));
-    F_TRY(hipMemcpyAsync(f->d_keys.p, keys4, n * 32, hipMemcpyHostToDevice, f->st));
-    k_query<<<grid_for(n), BLOCK, 0, f->st>>>(f->d_keys.as<uint64_t>(), n, f->M, f->d_solid.as<uint64_t>(),
-                                              f->d_prom.as<uint8_t>());
-    F_TRY(hipGetLastError());
-    F_TRY(hipMemcpyAsync(out, f->d_prom.p, n, hipMemcpyDeviceToHost, f->st));
-    F_TRY(hipStreamSynchronize(f->st));
-    return KC_OK;
-}
-
-int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_t data_len, const uint64_t* offsets,
-                            const uint32_t* lengths, size_t n_reads, uint8_t* out, uint64_t* n_corrected) {
-    if (!f) return KC_EINVAL;
-    if (k < 1 || k > KC_MAX_K) return f->fail(KC_EINVAL, "k out of range");
-    if (n_reads && (!offsets || !lengths)) return f->fail(KC_EINVAL, "null offsets/lengths");
-    if (data_len && (!data || !out)) return f->fail(KC_EINVAL, "null data/out");
-    if (n_reads > 0xFFFFFFFFull) return f->fail(KC_EUNSUPPORTED, "more than 2^32-1 reads");
-    std::vector<uint32_t> active;
-    for (size_t i = 0; i < n_reads; ++i) {
-        const uint64_t nb = ((uint64_t)lengths[i] + 3) / 4;
-        if (offsets[i] > data_len || nb > data_len - offsets[i]) return f->fail(KC_EINVAL, "read extends past data_len");
-        if (lengths[i] > HAMMER_MAX_LEN) return f->fail(KC_EUNSUPPORTED, "reads of at most 1024 bases");
-        if (lengths[i] >= k) active.push_back((uint32_t)i);
-    }
-    F_TRY(hipSetDevice(f->device));
-    DevBuf d_data, d_corr, d_off, d_len, d_act, d_ctr;
-    F_TRY(d_data.ensure(data_len + DATA_PAD));
-    F_TRY(d_corr.ensure(data_len + DATA_PAD));
-    F_TRY(d_off.ensure(n_reads * 8 + 8));
-    F_TRY(d_len.ensure(n_reads * 4 + 4));
-    F_TRY(d_act.ensure(active.size() * 4 + 4));
-    F_TRY(d_ctr.ensure(64));
-    F_TRY(hipMemsetAsync(d_data.p, 0, d_data.bytes, f->st));
-    F_TRY(hipMemsetAsync(d_ctr.p, 0, 64, f->st));
-    if (data_len) F_TRY(hipMemcpyAsync(d_data.p, data, data_len, hipMemcpyHostToDevice, f->st));
-    if (n_reads) {
-        F_TRY(hipMemcpyAsync(d_off.p, offsets, n_reads * 8, hipMemcpyHostToDevice, f->st));
-        F_TRY(hipMemcpyAsync(d_len.p, lengths, n_reads * 4, hipMemcpyHostToDevice, f->st));
-    }
-    if (!active.empty()) F_TRY(hipMemcpyAsync(d_act.p, active.data(), active.size() * 4, hipMemcpyHostToDevice, f->st));
-    F_TRY(hipMemcpyAsync(d_corr.p, d_data.p, d_data.bytes, hipMemcpyDeviceToDevice, f->st));
-    if (!active.empty()) {
-        const uint32_t g = grid_for(active.size(), 64, 65536);
-        if (k > 32)
-            k_hammer<true><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),
-                                                 active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), k,
-                                                 f->d_solid.as<uint64_t>(), f->M, d_ctr.as<unsigned long long>(),
-                                                 d_ctr.as<uint32_t>() + 4);
-        else
-            k_hammer<false><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),
-                                                  active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), k,
-                                                  f->d_solid.as<uint64_t>(), f->M, d_ctr.as<unsigned long long>(),
-                                                  d_ctr.as<uint32_t>() + 4);
-        F_TRY(hipGetLastError());
-    }
-    uint64_t ctr[2] = {0, 0};
-    F_TRY(hipMemcpyAsync(ctr, d_ctr.p, 16, hipMemcpyDeviceToHost, f->st));
-    if (data_len) F_TRY(hipMemcpyAsync(out, d_corr.p, data_len, hipMemcpyDeviceToHost, f->st));
-    F_TRY(hipStreamSynchronize(f->st));
-    for (DevBuf* b : {&d_data, &d_corr, &d_off, &d_len, &d_act, &d_ctr}) b->release();
-    if (n_corrected) *n_corrected = ctr[0];
-    return KC_OK;
-}
-
-int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words) {
-    if (!f) return KC_EINVAL;
-    if (num_words != f->num_words) return f->fail(KC_EINVAL, "num_words mismatch");
-    F_TRY(hipSetDevice(f->device));
-    if (seen) F_TRY(hipMemcpyAsync(seen, f->d_seen.p, num_words * 8, hipMemcpyDeviceToHost, f->st));
-    if (solid) F_TRY(hipMemcpyAsync(solid, f->d_solid.p, num_words * 8, hipMemcpyDeviceToHost, f->st));
-    F_TRY(hipStreamSynchronize(f->st));
-    return KC_OK;
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ EXPORTS = (
     "kc_filter_correct_reads",
     "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
+    "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
+    "kc_library_counts_export", "kc_library_reads_export",
 )
 
 
@@ -83,6 +85,13 @@ _SIGS = {
     "kc_counts_export": (ctypes.c_int, [_vp, _u64p, _u32p, _u8p, _sz]),
     "kc_reads_export": (ctypes.c_int, [_vp, _u8p, _sz]),
     "kc_kernel_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_char_p), _dblp, _u32p, ctypes.c_int]),
+    "kc_library_add": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u32p, _sz, ctypes.c_uint64, ctypes.c_double, _u32p]),
+    "kc_library_count": (ctypes.c_int, [_vp, _u32p]),
+    "kc_library_info": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_info)]),
+    "kc_library_bloom_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u64p, _sz]),
+    "kc_library_bloom_query": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _sz, _u8p]),
+    "kc_library_counts_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u32p, _u8p, _sz]),
+    "kc_library_reads_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz]),
 }
 
 _lib = None

@@ -176,7 +176,7 @@ class SpectrumEngine:
             raise KcError(st, "kc_create failed")
         self._h = h
         self.config = cfg
-        self._data_len = 0
+        self._data_len = []
 
     def close(self):
         if getattr(self, "_h", None):
@@ -189,11 +189,31 @@ class SpectrumEngine:
         _check(st, self._h, lib().kc_last_error)
 
     def upload(self, packed):
+        """Replace every library with `packed` (library 0)."""
         data, offs, lens = packed.arrays()
         d = data if data.shape[0] else np.zeros(1, np.uint8)
         self._chk(lib().kc_reads_upload(self._h, ptr(d, u8p), data.shape[0], ptr(offs, u64p), ptr(lens, u32p),
                                         lens.shape[0]))
-        self._data_len = data.shape[0]
+        self._data_len = [data.shape[0]]
+
+    def add_library(self, packed, bloom_bits=None, min_coverage=None):
+        """Append an independent library; returns its index."""
+        data, offs, lens = packed.arrays()
+        d = data if data.shape[0] else np.zeros(1, np.uint8)
+        idx = ctypes.c_uint32(0)
+        bb = self.config.bloom_bits if bloom_bits is None else bloom_bits
+        mc = self.config.min_coverage if min_coverage is None else min_coverage
+        self._chk(lib().kc_library_add(self._h, ptr(d, u8p), data.shape[0], ptr(offs, u64p), ptr(lens, u32p),
+                                       lens.shape[0], int(bb), float(mc), ctypes.byref(idx)))
+        if idx.value == 0:
+            self._data_len = []
+        self._data_len.append(data.shape[0])
+        return idx.value
+
+    def n_libraries(self):
+        n = ctypes.c_uint32(0)
+        self._chk(lib().kc_library_count(self._h, ctypes.byref(n)))
+        return n.value
 
     def bloom_build(self):
         self._chk(lib().kc_bloom_build(self._h))
@@ -208,36 +228,38 @@ class SpectrumEngine:
         self._chk(lib().kc_run(self._h, ctypes.byref(info)))
         return info.as_dict()
 
-    def info(self):
+    def info(self, library=0):
         info = _abi.kc_info()
-        self._chk(lib().kc_info_get(self._h, ctypes.byref(info)))
+        self._chk(lib().kc_library_info(self._h, library, ctypes.byref(info)))
         return info.as_dict()
 
-    def bloom(self):
-        nw = self.info()["num_words"]
+    def bloom(self, library=0):
+        nw = self.info(library)["num_words"]
         seen = np.zeros(nw, dtype=np.uint64)
         solid = np.zeros(nw, dtype=np.uint64)
-        self._chk(lib().kc_bloom_export(self._h, ptr(seen, u64p), ptr(solid, u64p), nw))
+        self._chk(lib().kc_library_bloom_export(self._h, library, ptr(seen, u64p), ptr(solid, u64p), nw))
         return seen, solid
 
-    def query(self, keys4):
+    def query(self, keys4, library=0):
         k4 = _keys4(keys4)
         out = np.zeros(k4.shape[0], dtype=np.uint8)
-        self._chk(lib().kc_bloom_query(self._h, ptr(k4, u64p), k4.shape[0], ptr(out, u8p)))
+        self._chk(lib().kc_library_bloom_query(self._h, library, ptr(k4, u64p), k4.shape[0], ptr(out, u8p)))
         return out.astype(bool)
 
-    def counts(self):
-        n = self.info()["n_counts"]
+    def counts(self, library=0):
+        n = self.info(library)["n_counts"]
         keys = np.zeros((max(n, 1), 4), dtype=np.uint64)
         cnt = np.zeros(max(n, 1), dtype=np.uint32)
         mask = np.zeros(max(n, 1), dtype=np.uint8)
-        self._chk(lib().kc_counts_export(self._h, ptr(keys, u64p), ptr(cnt, u32p), ptr(mask, u8p), n))
+        self._chk(lib().kc_library_counts_export(self._h, library, ptr(keys, u64p), ptr(cnt, u32p), ptr(mask, u8p),
+                                                 n))
         return keys[:n], cnt[:n], mask[:n]
 
-    def reads(self):
-        out = np.zeros(max(self._data_len, 1), dtype=np.uint8)
-        self._chk(lib().kc_reads_export(self._h, ptr(out, u8p), self._data_len))
-        return out[: self._data_len]
+    def reads(self, library=0):
+        dl = self._data_len[library]
+        out = np.zeros(max(dl, 1), dtype=np.uint8)
+        self._chk(lib().kc_library_reads_export(self._h, library, ptr(out, u8p), dl))
+        return out[:dl]
 
     def kernel_times(self):
         names = (ctypes.c_char_p * 32)()

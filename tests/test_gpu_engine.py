@@ -239,3 +239,48 @@ def test_repeated_runs_are_identical():
         for a, b in zip(o[0] + o[1], outs[0][0] + outs[0][1]):
             assert np.array_equal(a, b)
     eng.close()
+
+
+# ---- several libraries in one context ----------------------------------------------
+
+def _uneven_library(seed, genome, pairs, min_len=50, max_len=250):
+    rng = np.random.default_rng(seed)
+    st, _ = synth.uniform_genome_pairs(seed=seed, genome_size=genome, n_pairs=pairs, read_len=max_len,
+                                       insert_mean=max_len * 2, insert_sd=20, p_start=0.01)
+    out = PackedReads()
+    for i in range(len(st)):
+        r = st.get_read(i)
+        out.pack_clean(r[: int(rng.integers(min_len, max_len + 1))])
+    return out
+
+
+@pytest.mark.parametrize("k,ec", [(21, False), (33, True), (55, False), (31, True), (63, False)])
+def test_many_libraries_match_each_alone(oracle, k, ec):
+    libs = [_uneven_library(1000 + i, genome=3000 + 1500 * i, pairs=150 + 40 * i) for i in range(5)]
+    libs.insert(2, PackedReads())  # an empty library
+    short = PackedReads()
+    short.pack_clean(b"ACGT" * 4)
+    libs.append(short)  # reads shorter than k only
+    bits = [1 << 16, (1 << 18) + 64, 4096, 1 << 20, 1 << 17, 1 << 15, 1 << 12]
+    n_libs = 2 if k == 63 else len(libs)  # 2k - 63 + log2(libraries) <= 64
+    eng = SpectrumEngine(AssemblerConfig(k=k, error_correct=ec))
+    for i in range(n_libs):
+        assert eng.add_library(libs[i], bloom_bits=bits[i], min_coverage=5.0 if i != 3 else 12.0) == i
+    eng.run()
+    for i in range(n_libs):
+        say(f"library {i} k={k} ec={ec}")
+        d, o, l = libs[i].arrays()
+        S = oracle.Spectrum(d, o, l, k, bits[i], ec, 5.0 if i != 3 else 12.0, 1)
+        info = eng.info(i)
+        seen, solid = eng.bloom(i)
+        keys, cnt, mask = eng.counts(i)
+        oseen, osolid = S.bloom()
+        okeys, ocnt, omask = S.counts()
+        assert info["n_occ"] == S.n_occ
+        assert np.array_equal(seen, oseen) and np.array_equal(solid, osolid)
+        assert np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt)
+        assert np.array_equal(mask, omask) and info["cutoff"] == S.cutoff and info["n_solid"] == S.n_solid
+        if ec:
+            assert info["corrected_reads"] == S.corrected_reads
+            assert np.array_equal(eng.reads(i), S.corrected_data())
+    eng.close()

@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-load", type=float, default=0.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
@@ -58,6 +60,12 @@ def main():
         print(json.dumps({"error": "no GPU visible"}), flush=True)
         sys.exit(1)
     ks = [int(x) for x in args.ks.split(",")]
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     if args.workload == "5mb_100x":
         store, meta = synth.workload_5mb_100x(seed=2)

@@ -231,10 +231,9 @@ __device__ __forceinline__ uint64_t bucket_find(Slot<NW, Ev>* __restrict__ tab, 
 // Record one occurrence (event t) in slot i.
 template <int NW, typename Ev, bool TRACK_T0>
 __device__ __forceinline__ void slot_hit(Slot<NW, Ev>* __restrict__ tab, uint64_t i, Ev t) {
-    atomicAdd(&tab[i].cnt, 1u);
-    if (TRACK_T0) {
-        if (t < ld_relaxed(&tab[i].t0)) atomicMin(&tab[i].t0, t);
-    }
+    // Neither result is needed, so both are issued without return and the wave does not wait.
+    (void)__hip_atomic_fetch_add(&tab[i].cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (TRACK_T0) (void)__hip_atomic_fetch_min(&tab[i].t0, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Insert one occurrence of key, starting at bucket b whose first key words are w0.

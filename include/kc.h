@@ -68,7 +68,7 @@ typedef struct kc_info {
     uint64_t n_solid;          /* entries with count >= cutoff (the solid set) */
     uint64_t corrected_reads;  /* reads Hammer modified (error_correct only) */
     uint64_t num_words;        /* 64-bit words per Bloom tier */
-    uint64_t table_capacity;   /* distinct-count estimate (HyperLogLog) that sized the partition */
+    uint64_t table_capacity;   /* device hash-table slots used */
     uint32_t cutoff;           /* determine_solid_kmer_cutoff result */
     uint32_t top_cov;          /* 95th-percentile count used by the cutoff, capped at 1023 */
     double ms_stage2;          /* device time of the Bloom build */

@@ -25,12 +25,9 @@
 // (hammer.rs:34-120, one read per lane) and the Bloom-solid windows of the corrected
 // reads are counted in a second table pass.
 //
-// Counting is a two-level radix partition of the windows by a hash of (library, k-mer)
-// followed by LDS hash tables, one workgroup per final bucket (see k_part1).
-//
-// Libraries: every library has its own Bloom tiers and its own event numbering; windows
-// of all libraries form one global window space cut into tiles, and every record and
-// distinct k-mer carries its library.
+// Libraries: every library has its own Bloom tiers, its own region of the table pool and
+// its own event numbering; windows of all libraries form one global window space cut
+// into tiles, and table scans find a slot's library from the region starts.
 
 #include <hip/hip_runtime.h>
 
@@ -64,6 +61,8 @@ constexpr uint32_t LIB_LOCAL = 8;     // libraries a block aggregates in LDS per
 // Per-library state the kernels read.
 struct LibDesc {
     uint64_t ev0;       // first global window number of the library
+    uint64_t bucket0;   // first 64-byte bucket of its table region
+    uint64_t nbuckets;  // buckets in its table region
     uint64_t word0;     // first word of its Bloom tiers in the seen/solid pools
     uint64_t* seen;
     uint64_t* solid;
@@ -211,414 +210,187 @@ __device__ __forceinline__ uint32_t tile_read(const uint64_t* s_ev, uint32_t na,
 }
 
 // ---------------------------------------------------------------------------
-// counting by radix partition
+// distinct-count estimate per library (HyperLogLog, 2^12 registers) to size its table
 // ---------------------------------------------------------------------------
-//
-// Every window is routed by a hash of (library, canonical k-mer) in two radix levels:
-// 256 level-1 buckets, then S level-2 buckets inside each, so that a final bucket holds
-// all occurrences of a few hundred distinct k-mers.  Both levels write contiguous bucket
-// ranges reserved with one atomic per (block, bucket), so the records stream through L2
-// instead of hitting random lines.  Each final bucket is then counted by one workgroup
-// in an LDS hash table (count and first event of every distinct k-mer) and written out
-// densely.  A bucket whose distinct k-mers do not fit the LDS table is split by further
-// hash bits and re-read, so skewed inputs stay exact.
 
-constexpr uint32_t P1_BINS = 256;
-constexpr uint32_t P2_MAX_BITS = 9;
-constexpr uint32_t P2_MAX = 1u << P2_MAX_BITS;
-constexpr uint32_t P_CHUNK = 2048;               // records per block in the level-2 passes
-constexpr uint32_t P_ITEMS = P_CHUNK / BLOCK;    // records per thread
-constexpr uint32_t LDS_CAP = 2048;               // LDS hash-table slots per counting workgroup
-constexpr uint32_t LDS_FILL = LDS_CAP * 3 / 4;   // a hash range with more distinct keys is split
-constexpr uint32_t EMPTY32 = 0xFFFFFFFFu;
-
-// Structure-of-arrays record buffer (one record per window).
-template <int NW, typename Ev>
-struct Recs {
-    uint64_t* k0;
-    uint64_t* k1;   // NW == 2
-    Ev* t;          // event index within the library (when first events are tracked)
-    uint16_t* lib;  // when the context holds more than one library
-};
-
-// Dense distinct k-mers: key words, library, count, first event.
-template <typename Ev>
-struct Distinct {
-    uint64_t* k0;
-    uint64_t* k1;
-    uint16_t* lib;
-    uint32_t* cnt;
-    Ev* t0;
-};
-
-__device__ __forceinline__ uint64_t part_hash(uint64_t th, uint32_t lib) {
-    return th ^ ((uint64_t)lib * 0xD6E8FEB86659FD93ull);
-}
-
-// Level 1: windows of every tile -> 256 buckets of capacity cap1 each.  Also keeps the
-// per-library HyperLogLog registers (regs, when not null) over the canonical k-mers.
-template <bool WIDE, typename Ev, bool SOLID_ONLY>
-__global__ void __launch_bounds__(BLOCK) k_part1(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
-                                                 const uint64_t* __restrict__ act_off,
-                                                 const uint32_t* __restrict__ act_lib,
-                                                 const uint32_t* __restrict__ tile_first, uint64_t n_active,
-                                                 uint64_t n_occ, uint64_t n_tiles, uint32_t k,
-                                                 const LibDesc* __restrict__ libs, Recs<WIDE ? 2 : 1, Ev> r1,
-                                                 uint64_t cap1, unsigned long long* __restrict__ cursor1,
-                                                 uint32_t* __restrict__ regs, uint32_t multi,
-                                                 uint32_t* __restrict__ overflow) {
-    constexpr int NW = WIDE ? 2 : 1;
-    constexpr uint32_t WPT = TILE_WINDOWS / BLOCK;
-    constexpr bool HLL = !SOLID_ONLY;
+template <bool WIDE>
+__global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
+                                               const uint64_t* __restrict__ act_off,
+                                               const uint32_t* __restrict__ act_lib,
+                                               const uint32_t* __restrict__ tile_first, uint64_t n_active,
+                                               uint64_t n_occ, uint64_t n_tiles, uint32_t k,
+                                               uint32_t* __restrict__ regs) {
     __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
-    __shared__ uint32_t s_cnt[P1_BINS];
-    __shared__ unsigned long long s_base[P1_BINS];
-    __shared__ uint32_t s_reg[HLL ? HLL_M : 1];
-    uint32_t cur = ~0u;  // library whose HLL registers s_reg holds (block-uniform)
-    if (HLL)
-        for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
+    __shared__ uint32_t s_reg[HLL_M];
+    constexpr int NW = WIDE ? 2 : 1;
+    uint32_t cur = ~0u;  // library whose registers s_reg holds (block-uniform)
+    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint64_t a0 = tile_first[tile];
-        if (HLL) {
-            const uint32_t lib0 = act_lib[a0];
-            if (lib0 != cur) {
-                __syncthreads();
-                if (cur != ~0u)
-                    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
-                        if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
-                __syncthreads();
-                for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
-                cur = lib0;
-            }
+        const uint32_t lib0 = act_lib[a0];
+        if (lib0 != cur) {
+            __syncthreads();
+            if (cur != ~0u)
+                for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
+                    if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
+            cur = lib0;
         }
         const uint64_t w0 = tile * TILE_WINDOWS;
         const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
         const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
-        for (uint32_t i = threadIdx.x; i < P1_BINS; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
-        uint64_t key[WPT][NW];
-        uint32_t lib[WPT], bin[WPT], rank[WPT];
-        Ev t[WPT];
-        bool valid[WPT];
-#pragma unroll
-        for (uint32_t j = 0; j < WPT; ++j) {
-            const uint64_t w = w0 + j * BLOCK + threadIdx.x;
-            valid[j] = w < w1;
-            if (!valid[j]) continue;
+        for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
             const uint32_t lo = tile_read(s_ev, na, w);
-            lib[j] = act_lib[a0 + lo];
-            const Kv c = canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
-            if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
-                valid[j] = false;
-                continue;
-            }
-            pack_key<NW>(c, key[j]);
-            const uint64_t th = table_hash<NW>(key[j]);
-            if (HLL) {
-                const uint32_t idx = (uint32_t)(th >> (64 - HLL_P));
-                const uint64_t hw = th << HLL_P;
-                const uint32_t rho = hw ? (uint32_t)__clzll((long long)hw) + 1 : 64 - HLL_P + 1;
-                if (lib[j] == cur) atomicMax(&s_reg[idx], rho);
-                else atomicMax(&regs[(size_t)lib[j] * HLL_M + idx], rho);
-            }
-            bin[j] = (uint32_t)(part_hash(th, lib[j]) >> 56);
-            rank[j] = atomicAdd(&s_cnt[bin[j]], 1u);
-            t[j] = (Ev)(w - libs[lib[j]].ev0);
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < P1_BINS; i += blockDim.x) {
-            const uint32_t n = s_cnt[i];
-            s_base[i] = n ? atomicAdd(&cursor1[i], (unsigned long long)n) : 0ull;
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < WPT; ++j) {
-            if (!valid[j]) continue;
-            const uint64_t dst = s_base[bin[j]] + rank[j];
-            if (dst >= cap1) {
-                atomicOr(overflow, 1u);
-                continue;
-            }
-            const uint64_t idx = (uint64_t)bin[j] * cap1 + dst;
-            r1.k0[idx] = key[j][0];
-            if (NW == 2) r1.k1[idx] = key[j][NW - 1];
-            if (!SOLID_ONLY) r1.t[idx] = t[j];
-            if (multi) r1.lib[idx] = (uint16_t)lib[j];
+            const Kv v = load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k);
+            uint64_t key[NW];
+            pack_key<NW>(canonical<WIDE>(v, k), key);
+            const uint64_t h = table_hash<NW>(key);
+            const uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
+            const uint64_t hw = h << HLL_P;
+            const uint32_t rho = hw ? (uint32_t)__clzll((long long)hw) + 1 : 64 - HLL_P + 1;
+            const uint32_t lib = act_lib[a0 + lo];
+            if (lib == cur) atomicMax(&s_reg[idx], rho);
+            else atomicMax(&regs[(size_t)lib * HLL_M + idx], rho);
         }
     }
-    if (HLL) {
-        __syncthreads();
-        if (cur != ~0u)
-            for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
-                if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
-    }
-}
-
-template <int NW, typename Ev>
-__device__ __forceinline__ uint32_t rec_bin2(const Recs<NW, Ev>& r, uint64_t idx, uint32_t multi, uint32_t s_bits,
-                                             uint64_t key[NW], uint32_t& lib) {
-    key[0] = r.k0[idx];
-    if (NW == 2) key[NW - 1] = r.k1[idx];
-    lib = multi ? r.lib[idx] : 0;
-    const uint64_t hp = part_hash(table_hash<NW>(key), lib);
-    return s_bits ? (uint32_t)(hp >> (56 - s_bits)) & ((1u << s_bits) - 1) : 0u;
-}
-
-// Level-2 histogram: per level-1 bucket, records per level-2 bucket.
-template <int NW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_part2_hist(Recs<NW, Ev> r1, uint64_t cap1,
-                                                      const uint32_t* __restrict__ chunk_bkt,
-                                                      const uint64_t* __restrict__ chunk_start,
-                                                      const unsigned long long* __restrict__ n1, uint32_t s_bits,
-                                                      uint32_t multi, uint32_t* __restrict__ hist2) {
-    __shared__ uint32_t h[P2_MAX];
-    const uint32_t S = 1u << s_bits;
-    const uint32_t b = chunk_bkt[blockIdx.x];
-    const uint64_t st = chunk_start[blockIdx.x];
-    const uint64_t en = min<uint64_t>(st + P_CHUNK, n1[b]);
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    for (uint64_t i = st + threadIdx.x; i < en; i += blockDim.x) {
-        uint64_t key[NW];
-        uint32_t lib;
-        atomicAdd(&h[rec_bin2<NW, Ev>(r1, (uint64_t)b * cap1 + i, multi, s_bits, key, lib)], 1u);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x)
-        if (h[i]) atomicAdd(&hist2[(size_t)b * S + i], h[i]);
-}
-
-// Level-2 scatter into the final buckets (cursor2 starts at the bucket offsets).
-template <int NW, typename Ev, bool WITH_T>
-__global__ void __launch_bounds__(BLOCK) k_part2(Recs<NW, Ev> r1, uint64_t cap1, Recs<NW, Ev> r2,
-                                                 const uint32_t* __restrict__ chunk_bkt,
-                                                 const uint64_t* __restrict__ chunk_start,
-                                                 const unsigned long long* __restrict__ n1, uint32_t s_bits,
-                                                 uint32_t multi, unsigned long long* __restrict__ cursor2) {
-    __shared__ uint32_t s_cnt[P2_MAX];
-    __shared__ unsigned long long s_base[P2_MAX];
-    const uint32_t S = 1u << s_bits;
-    const uint32_t b = chunk_bkt[blockIdx.x];
-    const uint64_t st = chunk_start[blockIdx.x];
-    const uint64_t en = min<uint64_t>(st + P_CHUNK, n1[b]);
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) s_cnt[i] = 0;
-    __syncthreads();
-    uint64_t key[P_ITEMS][NW];
-    uint32_t lib[P_ITEMS], bin[P_ITEMS], rank[P_ITEMS];
-    Ev t[P_ITEMS];
-#pragma unroll
-    for (uint32_t j = 0; j < P_ITEMS; ++j) {
-        const uint64_t i = st + j * BLOCK + threadIdx.x;
-        if (i >= en) continue;
-        const uint64_t idx = (uint64_t)b * cap1 + i;
-        bin[j] = rec_bin2<NW, Ev>(r1, idx, multi, s_bits, key[j], lib[j]);
-        if (WITH_T) t[j] = r1.t[idx];
-        rank[j] = atomicAdd(&s_cnt[bin[j]], 1u);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-        const uint32_t n = s_cnt[i];
-        s_base[i] = n ? atomicAdd(&cursor2[(size_t)b * S + i], (unsigned long long)n) : 0ull;
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < P_ITEMS; ++j) {
-        const uint64_t i = st + j * BLOCK + threadIdx.x;
-        if (i >= en) continue;
-        const uint64_t o = s_base[bin[j]] + rank[j];
-        r2.k0[o] = key[j][0];
-        if (NW == 2) r2.k1[o] = key[j][NW - 1];
-        if (WITH_T) r2.t[o] = t[j];
-        if (multi) r2.lib[o] = (uint16_t)lib[j];
-    }
-}
-
-// Claim-or-match of one 64-bit key word in LDS (EMPTY -> value exactly once).
-__device__ __forceinline__ bool lds_word(uint64_t* w, uint64_t v, bool& claimed) {
-    uint64_t cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == EMPTY_KEY) {
-        cur = atomicCAS(reinterpret_cast<unsigned long long*>(w), (unsigned long long)EMPTY_KEY,
-                        (unsigned long long)v);
-        if (cur == EMPTY_KEY) {
-            claimed = true;
-            cur = v;
-        }
-    }
-    return cur == v;
-}
-
-__device__ __forceinline__ bool lds_word32(uint32_t* w, uint32_t v) {
-    uint32_t cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == EMPTY32) {
-        cur = atomicCAS(w, EMPTY32, v);
-        if (cur == EMPTY32) cur = v;
-    }
-    return cur == v;
-}
-
-// Count every final bucket in LDS; emit (key, library, count, first event) densely.
-template <int NW, typename Ev, bool TRACK_T0>
-__global__ void __launch_bounds__(BLOCK) k_count_lds(Recs<NW, Ev> r2, const unsigned long long* __restrict__ off2,
-                                                     uint32_t nq, uint32_t multi, Distinct<Ev> out, uint64_t capD,
-                                                     unsigned long long* __restrict__ d_counter,
-                                                     uint32_t* __restrict__ overflow) {
-    __shared__ uint64_t s_k0[LDS_CAP];
-    __shared__ uint64_t s_k1[NW == 2 ? LDS_CAP : 1];
-    __shared__ uint32_t s_lib[LDS_CAP];
-    __shared__ uint32_t s_cnt[LDS_CAP];
-    __shared__ Ev s_t0[TRACK_T0 ? LDS_CAP : 1];
-    __shared__ uint32_t s_fill, s_over;
-    __shared__ uint32_t s_scan[BLOCK];
-    __shared__ unsigned long long s_base;
-    constexpr uint32_t PER = LDS_CAP / BLOCK;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-        const uint64_t beg = off2[q], end = off2[q + 1];
-        if (beg == end) continue;
-        // hash ranges [lo, lo + width) of 16 bits; the stack is identical in every thread
-        uint32_t st_lo[18], st_w[18];
-        int sp = 1;
-        st_lo[0] = 0;
-        st_w[0] = 65536;
-        while (sp > 0) {
-            --sp;
-            const uint32_t rlo = st_lo[sp], rw = st_w[sp];
-            for (uint32_t i = tid; i < LDS_CAP; i += BLOCK) {
-                s_k0[i] = EMPTY_KEY;
-                if (NW == 2) s_k1[i] = EMPTY_KEY;
-                s_lib[i] = EMPTY32;
-                s_cnt[i] = 0;
-                if (TRACK_T0) s_t0[i] = (Ev)~(Ev)0;
-            }
-            if (tid == 0) {
-                s_fill = 0;
-                s_over = 0;
-            }
-            __syncthreads();
-            for (uint64_t i = beg + tid; i < end; i += BLOCK) {
-                uint64_t key[NW];
-                key[0] = r2.k0[i];
-                if (NW == 2) key[NW - 1] = r2.k1[i];
-                const uint32_t lib = multi ? r2.lib[i] : 0;
-                const uint64_t hp = part_hash(table_hash<NW>(key), lib);
-                if ((uint32_t)((hp >> 32) & 0xFFFF) - rlo >= rw) continue;
-                if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
-                uint32_t slot = (uint32_t)hp & (LDS_CAP - 1);
-                bool done = false;
-                for (uint32_t probe = 0; probe < LDS_CAP; ++probe) {
-                    bool claimed = false;
-                    bool m = lds_word(&s_k0[slot], key[0], claimed);
-                    if (claimed && atomicAdd(&s_fill, 1u) + 1 > LDS_FILL) atomicOr(&s_over, 1u);
-                    if (m && NW == 2) {
-                        bool c2 = false;
-                        m = lds_word(&s_k1[slot], key[NW - 1], c2);
-                    }
-                    if (m) m = lds_word32(&s_lib[slot], lib);
-                    if (m) {
-                        atomicAdd(&s_cnt[slot], 1u);
-                        if (TRACK_T0) atomicMin(&s_t0[slot], r2.t[i]);
-                        done = true;
-                        break;
-                    }
-                    slot = (slot + 1) & (LDS_CAP - 1);
-                }
-                if (!done) atomicOr(&s_over, 1u);
-            }
-            __syncthreads();
-            if (s_over) {
-                if (rw == 1) {  // more than LDS_FILL distinct keys share 16 hash bits: give up
-                    if (tid == 0) atomicOr(overflow, 2u);
-                } else {
-                    st_lo[sp] = rlo + rw / 2;
-                    st_w[sp] = rw / 2;
-                    st_lo[sp + 1] = rlo;
-                    st_w[sp + 1] = rw / 2;
-                    sp += 2;
-                }
-                __syncthreads();
-                continue;
-            }
-            // emit the occupied slots
-            uint32_t mine = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) mine += s_k0[tid + j * BLOCK] != EMPTY_KEY;
-            s_scan[tid] = mine;
-            __syncthreads();
-            for (uint32_t off = 1; off < BLOCK; off <<= 1) {
-                const uint32_t v = tid >= off ? s_scan[tid - off] : 0;
-                __syncthreads();
-                s_scan[tid] += v;
-                __syncthreads();
-            }
-            if (tid == 0) {
-                const uint32_t total = s_scan[BLOCK - 1];
-                s_base = total ? atomicAdd(d_counter, (unsigned long long)total) : 0ull;
-                if (s_base + total > capD) atomicOr(overflow, 1u);
-            }
-            __syncthreads();
-            uint64_t o = s_base + s_scan[tid] - mine;
-#pragma unroll
-            for (uint32_t j = 0; j < PER; ++j) {
-                const uint32_t sl = tid + j * BLOCK;
-                if (s_k0[sl] == EMPTY_KEY) continue;
-                if (o < capD) {
-                    out.k0[o] = s_k0[sl];
-                    if (NW == 2) out.k1[o] = s_k1[sl];
-                    if (multi) out.lib[o] = (uint16_t)s_lib[sl];
-                    out.cnt[o] = s_cnt[sl];
-                    if (TRACK_T0) out.t0[o] = s_t0[sl];
-                }
-                ++o;
-            }
-            __syncthreads();
-        }
-    }
+    if (cur != ~0u)
+        for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
+            if (s_reg[i]) atomicMax(&regs[(size_t)cur * HLL_M + i], s_reg[i]);
 }
 
 // ---------------------------------------------------------------------------
-// Bloom tiers from the distinct k-mers
+// table passes
 // ---------------------------------------------------------------------------
 
 template <int NW, typename Ev>
-__device__ __forceinline__ Kv distinct_key(const Distinct<Ev>& d, uint64_t i) {
-    uint64_t key[NW];
-    key[0] = d.k0[i];
-    if (NW == 2) key[NW - 1] = d.k1[i];
-    return unpack_key<NW>(key);
+__global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t n_slots) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_slots;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        Slot<NW, Ev> s;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s.k[w] = EMPTY_KEY;
+        s.t0 = (Ev)~(Ev)0;
+        s.cnt = 0;
+        tab[i] = s;
+    }
 }
 
-// Per-library counter accumulated in LDS for the first LIB_LDS libraries.
-constexpr uint32_t LIB_LDS = 1024;
+// Stage 2 insert (TRACK_T0, every window) or Stage 3 recount after Hammer (SOLID_ONLY:
+// only windows whose canonical k-mer is Bloom-solid, assemble.rs:196).  Each thread
+// takes WPT windows of the tile and issues their first bucket loads together, so a wave
+// keeps WPT independent random table accesses in flight.
+template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY>
+__global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
+                                                  const uint64_t* __restrict__ act_off,
+                                                  const uint32_t* __restrict__ act_lib,
+                                                  const uint32_t* __restrict__ tile_first, uint64_t n_active,
+                                                  uint64_t n_occ, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                  const LibDesc* __restrict__ libs, uint32_t max_probe,
+                                                  LibOut* __restrict__ out) {
+    constexpr int NW = WIDE ? 2 : 1;
+    constexpr uint32_t WPT = TILE_WINDOWS / BLOCK;
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
+    const uint64_t tile = blockIdx.x;
+    const uint64_t a0 = tile_first[tile];
+    const uint64_t w0 = tile * TILE_WINDOWS;
+    const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+    const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+    __syncthreads();
+    uint64_t key[WPT][NW], bkt[WPT], pre[WPT][SPB];
+    uint32_t lib[WPT];
+    bool valid[WPT];
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j) {
+        const uint64_t w = w0 + j * BLOCK + threadIdx.x;
+        valid[j] = w < w1;
+        if (!valid[j]) continue;
+        const uint32_t lo = tile_read(s_ev, na, w);
+        lib[j] = act_lib[a0 + lo];
+        const Kv c = canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
+        if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
+            valid[j] = false;
+            continue;
+        }
+        pack_key<NW>(c, key[j]);
+        bkt[j] = libs[lib[j]].bucket0 + home_bucket<NW>(key[j], libs[lib[j]].nbuckets);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j)
+        if (valid[j]) bucket_load<NW, Ev>(tab, bkt[j], pre[j]);
+#pragma unroll
+    for (uint32_t j = 0; j < WPT; ++j) {
+        if (!valid[j]) continue;
+        const LibDesc& L = libs[lib[j]];
+        const Ev t = (Ev)(w0 + j * BLOCK + threadIdx.x - L.ev0);
+        if (!table_insert_from<NW, Ev, TRACK_T0>(tab, L.bucket0, L.nbuckets, bkt[j], pre[j], key[j], t, max_probe))
+            atomicOr(&out[lib[j]].overflow, 1u);
+    }
+}
 
-// fs[p] = min(4*t0 + j) over the distinct k-mers of the libraries [g_lo, g_hi) whose
-// first-setter arrays are resident; counts distinct k-mers per library.
+// Iterates the slots [s_begin, s_end) of the table pool in chunks of BLOCK * SCAN_ITEMS;
+// f(slot index, library, chunk's first library) for every occupied slot.
+template <int NW, typename Ev, class F, class G>
+__device__ __forceinline__ void scan_slots(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin, uint64_t s_end,
+                                           const uint64_t* __restrict__ lib_b0, uint32_t n_libs, F&& f,
+                                           G&& chunk_end) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
+    for (uint64_t c0 = s_begin + blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
+        const uint32_t lib0 = find_region(lib_b0, n_libs, c0 / SPB);
+        uint32_t lib = lib0;
+        uint64_t lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+        for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+            const uint64_t i = c0 + (uint64_t)j * BLOCK + threadIdx.x;
+            if (i >= s_end) break;
+            if (i >= lib_end) {
+                lib = find_region(lib_b0, n_libs, i / SPB);
+                lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+            }
+            f(i, lib, lib0);
+        }
+        chunk_end(lib0);
+    }
+}
+
+// fs[p] = min(4*t0 + j) over the distinct k-mers' bits; counts distinct k-mers per library.
 template <int NW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_first_setter(Distinct<Ev> d, uint64_t D, uint32_t multi, uint32_t g_lo,
-                                                        uint32_t g_hi, const LibDesc* __restrict__ libs,
+__global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
+                                                        uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                        uint32_t n_libs, const LibDesc* __restrict__ libs,
                                                         LibOut* __restrict__ out) {
-    __shared__ uint32_t s_n[LIB_LDS];
-    const uint32_t nl = min(g_hi, LIB_LDS);
-    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) s_n[i] = 0;
+    __shared__ unsigned long long s_acc[LIB_LOCAL];
+    LibAccum acc{s_acc};
+    acc.reset();
     __syncthreads();
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < D; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t lib = multi ? d.lib[i] : 0;
-        if (lib < g_lo || lib >= g_hi) continue;
-        const LibDesc& L = libs[lib];
-        const Kv v = distinct_key<NW, Ev>(d, i);
-        uint64_t p[3];
-        bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
-        Ev* fs = reinterpret_cast<Ev*>(L.fs);
-        const Ev base = (Ev)(d.t0[i] * 4);
+    scan_slots<NW, Ev>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t lib0) {
+            const Slot<NW, Ev> s = tab[i];
+            if (s.k[0] == EMPTY_KEY) return;
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            Ev* fs = reinterpret_cast<Ev*>(L.fs);
+            const Ev base = (Ev)(s.t0 * 4);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
-        if (lib < LIB_LDS) atomicAdd(&s_n[lib], 1u);
-        else atomicAdd(&out[lib].n_distinct, 1ull);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x)
-        if (s_n[i]) atomicAdd(&out[i].n_distinct, (unsigned long long)s_n[i]);
+            for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
+            acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
+        },
+        [&](uint32_t lib0) {
+            __syncthreads();
+            acc.flush(lib0, n_libs, &out[0].n_distinct, LIBOUT_STRIDE);
+            __syncthreads();
+            acc.reset();
+            __syncthreads();
+        });
 }
 
 // seen word w (pool index) = ballot(fs != inf) over its 64 bits; clears the solid word.
@@ -643,50 +415,58 @@ __global__ void k_seen_from_fs(uint64_t w_begin, uint64_t w_end, const uint64_t*
 
 // Solid tier: OR of the bits of every promoted distinct k-mer (bloom.rs:99-103).
 template <int NW, typename Ev>
-__global__ void k_promote(Distinct<Ev> d, uint64_t D, uint32_t multi, uint32_t g_lo, uint32_t g_hi,
-                          const LibDesc* __restrict__ libs) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < D; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t lib = multi ? d.lib[i] : 0;
-        if (lib < g_lo || lib >= g_hi) continue;
-        const LibDesc& L = libs[lib];
-        const Kv v = distinct_key<NW, Ev>(d, i);
-        uint64_t p[3];
-        bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
-        bool promote = d.cnt[i] >= 2;
-        if (!promote) {
-            const Ev* fs = reinterpret_cast<const Ev*>(L.fs);
-            const Ev base = (Ev)(d.t0[i] * 4);
-            promote = fs[p[0]] != base && fs[p[1]] != (Ev)(base + 1) && fs[p[2]] != (Ev)(base + 2);
-        }
-        if (promote) {
+__global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
+                                                   uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                   uint32_t n_libs, const LibDesc* __restrict__ libs) {
+    scan_slots<NW, Ev>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t) {
+            const Slot<NW, Ev> s = tab[i];
+            if (s.k[0] == EMPTY_KEY) return;
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            bool promote = s.cnt >= 2;
+            if (!promote) {
+                const Ev* fs = reinterpret_cast<const Ev*>(L.fs);
+                const Ev base = (Ev)(s.t0 * 4);
+                promote = fs[p[0]] != base && fs[p[1]] != (Ev)(base + 1) && fs[p[2]] != (Ev)(base + 2);
+            }
+            if (promote) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                atomicOr(reinterpret_cast<unsigned long long*>(&L.solid[p[j] >> 6]), 1ull << (p[j] & 63));
-        }
-    }
+                for (int j = 0; j < 3; ++j)
+                    atomicOr(reinterpret_cast<unsigned long long*>(&L.solid[p[j] >> 6]), 1ull << (p[j] & 63));
+            }
+        },
+        [](uint32_t) {});
 }
 
-// Gather the count table as sort entries: distinct k-mers that are Bloom-solid (or all of
-// them when they were counted from Bloom-solid windows only).  One atomic per chunk.
+// Gather the count table: occupied slots whose k-mer is Bloom-solid (or every occupied
+// slot when the table was built from Bloom-solid windows only), as (hi, lo, count) sort
+// entries.  Each block reserves its output range with one atomic per chunk.
 template <int NW, typename Ev, bool CHECK_SOLID>
-__global__ void __launch_bounds__(BLOCK) k_compact(Distinct<Ev> d, uint64_t D, uint32_t multi,
+__global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_end,
+                                                   const uint64_t* __restrict__ lib_b0, uint32_t n_libs,
                                                    const LibDesc* __restrict__ libs, SortLayout lay,
                                                    uint64_t* __restrict__ out_hi, uint64_t* __restrict__ out_lo,
                                                    uint32_t* __restrict__ out_cnt,
                                                    unsigned long long* __restrict__ counter) {
     __shared__ uint32_t s_scan[BLOCK];
     __shared__ unsigned long long s_base;
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
     const uint32_t tid = threadIdx.x;
-    for (uint64_t c0 = blockIdx.x * chunk; c0 < D; c0 += (uint64_t)gridDim.x * chunk) {
+    for (uint64_t c0 = blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
         uint32_t bits = 0, cnt = 0;
         for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
-            if (i >= D) break;
-            bool take = true;
-            if (CHECK_SOLID) {
-                const LibDesc& L = libs[multi ? d.lib[i] : 0];
-                take = bloom_is_solid(distinct_key<NW, Ev>(d, i), L.solid, L.M);
+            if (i >= s_end) break;
+            const Slot<NW, Ev> sl = tab[i];
+            bool take = sl.k[0] != EMPTY_KEY;
+            if (CHECK_SOLID && take) {
+                const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
+                take = bloom_is_solid(unpack_key<NW>(sl.k), L.solid, L.M);
             }
             bits |= (uint32_t)take << j;
             cnt += take;
@@ -707,11 +487,12 @@ __global__ void __launch_bounds__(BLOCK) k_compact(Distinct<Ev> d, uint64_t D, u
             const uint32_t j = __ffs(bits) - 1;
             bits &= bits - 1;
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
+            const Slot<NW, Ev> sl = tab[i];
             uint64_t hi, lo;
-            encode_entry(lay, multi ? d.lib[i] : 0, distinct_key<NW, Ev>(d, i), hi, lo);
+            encode_entry(lay, find_region(lib_b0, n_libs, i / SPB), unpack_key<NW>(sl.k), hi, lo);
             out_lo[o] = lo;
             if (!lay.single) out_hi[o] = hi;
-            out_cnt[o] = d.cnt[i];
+            out_cnt[o] = sl.cnt;
             ++o;
         }
         __syncthreads();
@@ -827,13 +608,13 @@ __global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restric
 }
 
 enum KernelId {
-    KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_PART1, KID_PART2H, KID_PART2, KID_COUNT_LDS, KID_FS_CLEAR,
-    KID_FIRST_SETTER, KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS, KID_COUNT
+    KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_HLL, KID_CLEAR, KID_INSERT, KID_FS_CLEAR, KID_FIRST_SETTER,
+    KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS, KID_COUNT
 };
 const char* const KERNEL_NAMES[KID_COUNT] = {
-    "k_nwin", "scan_windows", "select_active", "k_active", "k_part1", "k_part2_hist", "k_part2", "k_count_lds",
-    "k_fs_clear", "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_compact", "radix_sort",
-    "k_count_hist", "k_solid_flags"};
+    "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
+    "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_compact", "radix_sort", "k_count_hist",
+    "k_solid_flags"};
 
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
 struct HostLib {
@@ -843,7 +624,7 @@ struct HostLib {
     uint64_t bloom_bits = 0;
     double min_cov = 5.0;
     // placement in the device pools (set by the planner)
-    uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, fs_off = 0, est = 0;
+    uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
     uint32_t group = 0;
     kc_info info{};
     uint64_t entry0 = 0;  // first sorted entry
@@ -875,15 +656,9 @@ struct kc_ctx {
     DevBuf d_nwin, d_prefix, d_flags, d_iota, d_active, d_act_ev, d_act_off, d_act_lib, d_tile_first;
     DevBuf d_lib_read0, d_lib_ev0;
     uint64_t n_occ = 0, n_active = 0, n_tiles = 0;
-    // filter pools, first setters, library descriptors
-    DevBuf d_seen, d_solid, d_fs, d_libs, d_lib_w0, d_blooms, d_out, d_mincov;
-    uint64_t total_words = 0, fs_bytes = 0;
-    // radix-partition records (level 1, level 2) and the distinct k-mers
-    DevBuf d_r1k0, d_r1k1, d_r1t, d_r1lib, d_r2k0, d_r2k1, d_r2t, d_r2lib;
-    DevBuf d_dk0, d_dk1, d_dlib, d_dcnt, d_dt0;
-    DevBuf d_cursor1, d_hist2, d_off2, d_cursor2, d_chunk_bkt, d_chunk_start;
-    uint64_t cap1 = 0, n_distinct_all = 0, D = 0;
-    uint32_t s_bits = 0;
+    // filter pools, first setters, table pool
+    DevBuf d_seen, d_solid, d_fs, d_tab, d_libs, d_lib_b0, d_lib_w0, d_blooms, d_out, d_mincov;
+    uint64_t total_words = 0, total_buckets = 0, fs_bytes = 0;
     std::vector<std::pair<uint32_t, uint32_t>> groups;  // [lib begin, lib end) sharing d_fs
     // count entries (sorted) + scratch
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
@@ -1073,17 +848,58 @@ struct Run {
     }
     std::vector<uint64_t> lib_ev0_;
 
-    // Lay out the Bloom pools, the first-setter groups and the library descriptors.
-    template <typename Ev>
-    int plan_bloom() {
-        uint64_t w0 = 0;
+    template <bool WIDE>
+    int estimate_distinct() {
+        hipStream_t st = c->st;
+        const size_t nreg = (size_t)L() * HLL_M;
+        KC_TRY(c->d_hll.ensure(nreg * 4));
+        k_fill<uint32_t><<<grid_for(nreg), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), nreg, 0u);
+        KC_TRY(hipGetLastError());
+        const uint32_t g = (uint32_t)std::min<uint64_t>(c->n_tiles, 2048);
+        TIMED(KID_HLL, (k_hll<WIDE><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
+                                                         c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
+                                                         c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
+                                                         c->n_tiles, c->o.k, c->d_hll.as<uint32_t>())));
+        std::vector<uint32_t> regs(nreg);
+        KC_TRY(hipMemcpyAsync(regs.data(), c->d_hll.p, nreg * 4, hipMemcpyDeviceToHost, st));
+        KC_TRY(hipStreamSynchronize(st));
+        for (uint32_t l = 0; l < L(); ++l) {
+            double sum = 0;
+            uint32_t zeros = 0;
+            for (uint32_t i = 0; i < HLL_M; ++i) {
+                const uint32_t r = regs[(size_t)l * HLL_M + i];
+                sum += std::ldexp(1.0, -(int)r);
+                zeros += r == 0;
+            }
+            const double m = HLL_M;
+            double E = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
+            if (E <= 2.5 * m && zeros) E = m * std::log(m / zeros);
+            c->libs[l].est = (uint64_t)std::ceil(E);
+        }
+        return KC_OK;
+    }
+
+    // Lay out the table pool, the Bloom pools and the first-setter groups.
+    template <int NW, typename Ev>
+    int plan(const std::vector<uint32_t>& grow) {
+        const double load = (c->o.table_load > 0.05f && c->o.table_load < 0.95f) ? c->o.table_load : 0.5;
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        uint64_t b0 = 0, w0 = 0;
         for (uint32_t l = 0; l < L(); ++l) {
             auto& lib = c->libs[l];
+            const uint64_t occ = lib.info.n_occ;
+            uint64_t cap = (uint64_t)std::ceil((double)lib.est * 1.10 / load) + 64;
+            cap = std::min<uint64_t>(cap, occ * 2 + 64) << grow[l];
+            lib.nbuckets = occ ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
+            lib.bucket0 = b0;
+            b0 += lib.nbuckets;
             lib.num_words = std::max<uint64_t>((lib.bloom_bits + 63) / 64, 1);
             lib.word0 = w0;
             w0 += lib.num_words;
         }
+        c->total_buckets = b0;
         c->total_words = w0;
+        KC_TRY(c->d_tab.ensure(std::max<uint64_t>(b0, 1) * 64));
         KC_TRY(c->d_seen.ensure(w0 * 8));
         KC_TRY(c->d_solid.ensure(w0 * 8));
         // first-setter groups: consecutive libraries whose fs arrays fit the budget together
@@ -1107,14 +923,17 @@ struct Run {
         maxg = std::max(maxg, used);
         KC_TRY(c->d_fs.ensure(maxg));
         c->fs_bytes = maxg;
+        // descriptors
         std::vector<LibDesc> desc(L());
         std::vector<LibBloom> blooms(L());
-        std::vector<uint64_t> lw0(L());
+        std::vector<uint64_t> lb0(L()), lw0(L());
         std::vector<double> mc(L());
         for (uint32_t l = 0; l < L(); ++l) {
             auto& lib = c->libs[l];
             LibDesc& d = desc[l];
             d.ev0 = lib_ev0_[l];
+            d.bucket0 = lib.bucket0;
+            d.nbuckets = lib.nbuckets;
             d.word0 = lib.word0;
             d.seen = c->d_seen.as<uint64_t>() + lib.word0;
             d.solid = c->d_solid.as<uint64_t>() + lib.word0;
@@ -1122,17 +941,20 @@ struct Run {
             d.M = make_modulus(lib.num_words * 64);
             d.num_words = lib.num_words;
             blooms[l] = LibBloom{d.solid, d.M};
+            lb0[l] = lib.bucket0;
             lw0[l] = lib.word0;
             mc[l] = lib.min_cov;
         }
         KC_TRY(c->d_libs.ensure(L() * sizeof(LibDesc)));
         KC_TRY(c->d_blooms.ensure(L() * sizeof(LibBloom)));
+        KC_TRY(c->d_lib_b0.ensure(L() * 8));
         KC_TRY(c->d_lib_w0.ensure(L() * 8));
         KC_TRY(c->d_mincov.ensure(L() * 8));
         KC_TRY(c->d_out.ensure(L() * sizeof(LibOut)));
         hipStream_t st = c->st;
         KC_TRY(hipMemcpyAsync(c->d_libs.p, desc.data(), L() * sizeof(LibDesc), hipMemcpyHostToDevice, st));
         KC_TRY(hipMemcpyAsync(c->d_blooms.p, blooms.data(), L() * sizeof(LibBloom), hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_lib_b0.p, lb0.data(), L() * 8, hipMemcpyHostToDevice, st));
         KC_TRY(hipMemcpyAsync(c->d_lib_w0.p, lw0.data(), L() * 8, hipMemcpyHostToDevice, st));
         KC_TRY(hipMemcpyAsync(c->d_mincov.p, mc.data(), L() * 8, hipMemcpyHostToDevice, st));
         KC_TRY(hipStreamSynchronize(st));
@@ -1153,193 +975,57 @@ struct Run {
         return KC_OK;
     }
 
-    template <int NW, typename Ev>
-    Recs<NW, Ev> recs1() const {
-        return Recs<NW, Ev>{c->d_r1k0.as<uint64_t>(), c->d_r1k1.as<uint64_t>(), c->d_r1t.as<Ev>(),
-                            c->d_r1lib.as<uint16_t>()};
-    }
-    template <int NW, typename Ev>
-    Recs<NW, Ev> recs2() const {
-        return Recs<NW, Ev>{c->d_r2k0.as<uint64_t>(), c->d_r2k1.as<uint64_t>(), c->d_r2t.as<Ev>(),
-                            c->d_r2lib.as<uint16_t>()};
-    }
-    template <typename Ev>
-    Distinct<Ev> distinct() const {
-        return Distinct<Ev>{c->d_dk0.as<uint64_t>(), c->d_dk1.as<uint64_t>(), c->d_dlib.as<uint16_t>(),
-                            c->d_dcnt.as<uint32_t>(), c->d_dt0.as<Ev>()};
-    }
-
-    // One counting pass over the windows of `data`: every distinct (library, canonical
-    // k-mer) with its count (and first event when TRACK); SOLID_ONLY keeps Bloom-solid
-    // windows only (assemble.rs:196).  Leaves the distinct k-mers in the d_d* arrays.
-    template <bool WIDE, typename Ev, bool TRACK, bool SOLID_ONLY>
-    int count_pass(const uint8_t* data) {
-        constexpr int NW = WIDE ? 2 : 1;
-        hipStream_t st = c->st;
-        const uint32_t multi = L() > 1;
-        const uint64_t n_occ = c->n_occ;
-        unsigned long long* ctr = c->d_counters.as<unsigned long long>();
-        // ---- level 1 (and the distinct-count estimate) ----
-        uint64_t cap1 = (uint64_t)std::ceil((double)n_occ / P1_BINS * 1.1) + 65536;
-        std::vector<unsigned long long> n1(P1_BINS);
-        for (int attempt = 0;; ++attempt) {
-            const uint64_t slots = cap1 * P1_BINS;
-            KC_TRY(c->d_r1k0.ensure(slots * 8));
-            if (NW == 2) KC_TRY(c->d_r1k1.ensure(slots * 8));
-            if (TRACK) KC_TRY(c->d_r1t.ensure(slots * sizeof(Ev)));
-            if (multi) KC_TRY(c->d_r1lib.ensure(slots * 2));
-            KC_TRY(c->d_cursor1.ensure(P1_BINS * 8));
-            k_fill<unsigned long long><<<1, P1_BINS, 0, st>>>(c->d_cursor1.as<unsigned long long>(), P1_BINS, 0ull);
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-            KC_TRY(hipGetLastError());
-            if (TRACK) {
-                const size_t nreg = (size_t)L() * HLL_M;
-                KC_TRY(c->d_hll.ensure(nreg * 4));
-                k_fill<uint32_t><<<grid_for(nreg), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), nreg, 0u);
-                KC_TRY(hipGetLastError());
-            }
-            const uint32_t g = (uint32_t)std::min<uint64_t>(c->n_tiles, 4096);
-            TIMED(KID_PART1, (k_part1<WIDE, Ev, SOLID_ONLY><<<g, BLOCK, 0, st>>>(
-                                 data, c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
-                                 c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active, n_occ,
-                                 c->n_tiles, c->o.k, c->d_libs.as<LibDesc>(), recs1<NW, Ev>(), cap1,
-                                 c->d_cursor1.as<unsigned long long>(), TRACK ? c->d_hll.as<uint32_t>() : nullptr,
-                                 multi, c->d_counters.as<uint32_t>())));
-            KC_TRY(hipMemcpyAsync(n1.data(), c->d_cursor1.p, P1_BINS * 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            if (!(c->h_stage[0] & 0xFFFFFFFFull)) break;
-            if (attempt >= 2) return fail(KC_ENOMEM, "level-1 partition overflow");
-            cap1 = *std::max_element(n1.begin(), n1.end()) + 1024;  // exact sizes are known now
-        }
-        c->cap1 = cap1;
-        uint64_t N = 0;
-        for (auto v : n1) N += v;
-        if (TRACK) {
-            const size_t nreg = (size_t)L() * HLL_M;
-            std::vector<uint32_t> regs(nreg);
-            KC_TRY(hipMemcpyAsync(regs.data(), c->d_hll.p, nreg * 4, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            uint64_t tot = 0;
-            for (uint32_t l = 0; l < L(); ++l) {
-                double sum = 0;
-                uint32_t zeros = 0;
-                for (uint32_t i = 0; i < HLL_M; ++i) {
-                    const uint32_t r = regs[(size_t)l * HLL_M + i];
-                    sum += std::ldexp(1.0, -(int)r);
-                    zeros += r == 0;
-                }
-                const double m = HLL_M;
-                double E = (0.7213 / (1.0 + 1.079 / m)) * m * m / sum;
-                if (E <= 2.5 * m && zeros) E = m * std::log(m / zeros);
-                c->libs[l].est = (uint64_t)std::ceil(E);
-                tot += c->libs[l].est;
-            }
-            // final buckets of ~600 distinct k-mers: S = 2^s_bits level-2 buckets per level-1 bucket
-            uint32_t sb = 0;
-            while (sb < P2_MAX_BITS && (double)tot / (P1_BINS << sb) > 600.0) ++sb;
-            c->s_bits = sb;
-            c->n_distinct_all = tot;
-        }
-        const uint32_t sbits = c->s_bits;
-        const uint32_t S = 1u << sbits;
-        // ---- level 2 ----
-        std::vector<uint32_t> cb;
-        std::vector<uint64_t> cs;
-        for (uint32_t b = 0; b < P1_BINS; ++b)
-            for (uint64_t s0 = 0; s0 < n1[b]; s0 += P_CHUNK) {
-                cb.push_back(b);
-                cs.push_back(s0);
-            }
-        const uint64_t nchunks = cb.size();
-        const uint64_t nq = (uint64_t)P1_BINS * S;
-        KC_TRY(c->d_chunk_bkt.ensure(nchunks * 4 + 4));
-        KC_TRY(c->d_chunk_start.ensure(nchunks * 8 + 8));
-        KC_TRY(c->d_hist2.ensure((nq + 1) * 4));
-        KC_TRY(c->d_off2.ensure((nq + 1) * 8));
-        KC_TRY(c->d_cursor2.ensure(nq * 8));
-        if (nchunks) {
-            KC_TRY(hipMemcpyAsync(c->d_chunk_bkt.p, cb.data(), nchunks * 4, hipMemcpyHostToDevice, st));
-            KC_TRY(hipMemcpyAsync(c->d_chunk_start.p, cs.data(), nchunks * 8, hipMemcpyHostToDevice, st));
-        }
-        k_fill<uint32_t><<<grid_for(nq + 1), BLOCK, 0, st>>>(c->d_hist2.as<uint32_t>(), nq + 1, 0u);
-        KC_TRY(hipGetLastError());
-        if (nchunks)
-            TIMED(KID_PART2H, (k_part2_hist<NW, Ev><<<(uint32_t)nchunks, BLOCK, 0, st>>>(
-                                  recs1<NW, Ev>(), cap1, c->d_chunk_bkt.as<uint32_t>(), c->d_chunk_start.as<uint64_t>(),
-                                  c->d_cursor1.as<unsigned long long>(), sbits, multi, c->d_hist2.as<uint32_t>())));
-        size_t tb = 0;
-        KC_TRY(rocprim::exclusive_scan(nullptr, tb, c->d_hist2.as<uint32_t>(), c->d_off2.as<unsigned long long>(),
-                                       0ull, (size_t)(nq + 1), rocprim::plus<unsigned long long>(), st));
-        KC_TRY(c->d_temp.ensure(tb));
-        tb = c->d_temp.bytes;
-        KC_TRY(rocprim::exclusive_scan(c->d_temp.p, tb, c->d_hist2.as<uint32_t>(), c->d_off2.as<unsigned long long>(),
-                                       0ull, (size_t)(nq + 1), rocprim::plus<unsigned long long>(), st));
-        KC_TRY(hipMemcpyAsync(c->d_cursor2.p, c->d_off2.p, nq * 8, hipMemcpyDeviceToDevice, st));
-        KC_TRY(c->d_r2k0.ensure(N * 8 + 8));
-        if (NW == 2) KC_TRY(c->d_r2k1.ensure(N * 8 + 8));
-        if (TRACK) KC_TRY(c->d_r2t.ensure(N * sizeof(Ev) + 8));
-        if (multi) KC_TRY(c->d_r2lib.ensure(N * 2 + 8));
-        if (nchunks)
-            TIMED(KID_PART2, (k_part2<NW, Ev, TRACK><<<(uint32_t)nchunks, BLOCK, 0, st>>>(
-                                 recs1<NW, Ev>(), cap1, recs2<NW, Ev>(), c->d_chunk_bkt.as<uint32_t>(),
-                                 c->d_chunk_start.as<uint64_t>(), c->d_cursor1.as<unsigned long long>(), sbits, multi,
-                                 c->d_cursor2.as<unsigned long long>())));
-        // ---- count in LDS ----
-        uint64_t capD = TRACK ? std::min<uint64_t>(N, (uint64_t)((double)c->n_distinct_all * 1.3) + 65536) : N;
-        for (int attempt = 0;; ++attempt) {
-            capD = std::max<uint64_t>(capD, 1);
-            KC_TRY(c->d_dk0.ensure(capD * 8));
-            if (NW == 2) KC_TRY(c->d_dk1.ensure(capD * 8));
-            if (multi) KC_TRY(c->d_dlib.ensure(capD * 2));
-            KC_TRY(c->d_dcnt.ensure(capD * 4));
-            if (TRACK) KC_TRY(c->d_dt0.ensure(capD * sizeof(Ev)));
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-            KC_TRY(hipGetLastError());
-            const uint32_t g = (uint32_t)std::min<uint64_t>(nq, 16384);
-            TIMED(KID_COUNT_LDS, (k_count_lds<NW, Ev, TRACK><<<g, BLOCK, 0, st>>>(
-                                     recs2<NW, Ev>(), c->d_off2.as<unsigned long long>(), (uint32_t)nq, multi,
-                                     distinct<Ev>(), capD, ctr + 2, c->d_counters.as<uint32_t>())));
-            KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 24, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            const uint32_t of = (uint32_t)c->h_stage[0];
-            if (of & 2u) return fail(KC_EUNSUPPORTED, "a final bucket holds too many distinct k-mers for LDS");
-            if (!(of & 1u)) break;
-            if (attempt >= 1) return fail(KC_ENOMEM, "distinct k-mer array overflow");
-            capD = N;
-        }
-        c->D = c->h_stage[2];
-        return KC_OK;
-    }
-
     template <bool WIDE, typename Ev>
     int bloom_build() {
         constexpr int NW = WIDE ? 2 : 1;
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
-        int rc = plan_bloom<Ev>();
+        int rc = estimate_distinct<WIDE>();
         if (rc) return rc;
-        rc = clear_out();
-        if (rc) return rc;
-        rc = count_pass<WIDE, Ev, true, false>(c->d_data.as<uint8_t>());
-        if (rc) return rc;
-        const uint64_t D = c->D;
-        const uint32_t multi = L() > 1;
+        std::vector<uint32_t> grow(L(), 0);
+        for (int attempt = 0;; ++attempt) {
+            rc = plan<NW, Ev>(grow);
+            if (rc) return rc;
+            rc = clear_out();
+            if (rc) return rc;
+            const uint64_t n_slots = c->total_buckets * SPB;
+            TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(
+                                 c->d_tab.as<Slot<NW, Ev>>(), n_slots)));
+            TIMED(KID_INSERT, (k_insert<WIDE, Ev, true, false><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
+                                  c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
+                                  c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
+                                  c->o.k, c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), 1u << 20,
+                                  c->d_out.as<LibOut>())));
+            rc = read_out();
+            if (rc) return rc;
+            bool any = false;
+            for (uint32_t l = 0; l < L(); ++l)
+                if (c->h_out[l].overflow) {
+                    any = true;
+                    ++grow[l];
+                }
+            if (!any) break;
+            if (attempt >= 4) return fail(KC_ENOMEM, "device hash table overflow");
+        }
         // first setters -> seen tier -> solid tier, one group of libraries at a time
         for (auto& g : c->groups) {
             const auto& lb = c->libs[g.first];
             const auto& le = c->libs[g.second - 1];
+            const uint64_t s_begin = lb.bucket0 * SPB, s_end = (le.bucket0 + le.nbuckets) * SPB;
             const uint64_t w_begin = lb.word0, w_end = le.word0 + le.num_words;
             const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
             TIMED(KID_FS_CLEAR, (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
-            if (D)
-                TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev><<<grid_for(D), BLOCK, 0, st>>>(
-                                            distinct<Ev>(), D, multi, g.first, g.second, c->d_libs.as<LibDesc>(),
-                                            c->d_out.as<LibOut>())));
+            if (s_end > s_begin)
+                TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0,
+                                                                  st>>>(
+                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                                            c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>())));
             TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for((w_end - w_begin) * 64), BLOCK, 0, st>>>(
                                 w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>())));
-            if (D)
-                TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<grid_for(D), BLOCK, 0, st>>>(
-                                       distinct<Ev>(), D, multi, g.first, g.second, c->d_libs.as<LibDesc>())));
+            if (s_end > s_begin)
+                TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                       c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                                       c->d_libs.as<LibDesc>())));
         }
         return read_out();
     }
@@ -1409,10 +1095,11 @@ struct Run {
         constexpr int NW = WIDE ? 2 : 1;
         hipStream_t st = c->st;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>();
+        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
+        KC_TRY(hipGetLastError());
         int rc = clear_out();
         if (rc) return rc;
         const bool ec = c->o.error_correct != 0;
-        const uint32_t multi = L() > 1;
         c->lay = make_layout(c->o.k, L());
         if (c->lay.hi_bits > 64)
             return fail(KC_EUNSUPPORTED, "too many libraries for this k in one context (need 2k - 63 + log2(libraries) <= 64)");
@@ -1420,8 +1107,6 @@ struct Run {
             // corr = data; Hammer writes the reads it modifies
             KC_TRY(c->d_corr.ensure(c->d_data.bytes));
             KC_TRY(hipMemcpyAsync(c->d_corr.p, c->d_data.p, c->d_data.bytes, hipMemcpyDeviceToDevice, st));
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-            KC_TRY(hipGetLastError());
             if (c->n_active) {
                 TIMED(KID_HAMMER, (k_hammer<WIDE><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
                                       c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(),
@@ -1430,33 +1115,45 @@ struct Run {
                                       &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
                                       c->d_counters.as<uint32_t>() + 6)));
             }
+            const uint64_t SPB = slots_per_bucket<NW, uint32_t>();
+            const uint64_t n_slots = c->total_buckets * SPB;
+            TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
+                                 c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
+            if (c->n_tiles) {
+                TIMED(KID_RECOUNT, (k_insert<WIDE, uint32_t, false, true><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
+                                       c->d_corr.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
+                                       c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active,
+                                       c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(), c->d_libs.as<LibDesc>(),
+                                       1u << 20, c->d_out.as<LibOut>())));
+            }
+            rc = read_out();
+            if (rc) return rc;
             KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 32, hipMemcpyDeviceToHost, st));
             KC_TRY(hipStreamSynchronize(st));
             if (c->h_stage[3] & 0xFFFFFFFFull)
                 return fail(KC_EUNSUPPORTED, "error_correct supports reads of at most 1024 bases");
-            rc = count_pass<WIDE, Ev, false, true>(c->d_corr.as<uint8_t>());
-            if (rc) return rc;
+            for (uint32_t l = 0; l < L(); ++l)
+                if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
+                    return fail(KC_ENOMEM, "device hash table overflow (recount)");
         }
-        const uint64_t D = c->D;
-        const uint64_t cap = std::max<uint64_t>(D, 1);
+        const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
+        const uint64_t cap = std::max<uint64_t>(slots, 1);
         KC_TRY(c->d_hi.ensure(cap * 8));
         KC_TRY(c->d_lo.ensure(cap * 8));
         KC_TRY(c->d_cnt.ensure(cap * 4));
         KC_TRY(c->d_hi2.ensure(cap * 8));
         KC_TRY(c->d_lo2.ensure(cap * 8));
         KC_TRY(c->d_cnt2.ensure(cap * 4));
-        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-        KC_TRY(hipGetLastError());
-        if (D) {
-            if (ec) {
-                TIMED(KID_COMPACT, (k_compact<NW, Ev, false><<<grid_for(D, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
-                                       distinct<Ev>(), D, multi, c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(),
-                                       c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4)));
-            } else {
-                TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(D, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
-                                       distinct<Ev>(), D, multi, c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(),
-                                       c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4)));
-            }
+        if (ec) {
+            TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                   c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
+                                   c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
+        } else {
+            TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                   c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
+                                   c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
         }
         KC_TRY(hipMemcpyAsync(c->h_stage, ctr + 4, 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
@@ -1545,12 +1242,8 @@ void kc_destroy(kc_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->d_data,   &ctx->d_off,     &ctx->d_len,     &ctx->d_rlib,     &ctx->d_nwin,
                       &ctx->d_prefix, &ctx->d_flags,   &ctx->d_iota,    &ctx->d_active,   &ctx->d_act_ev,
                       &ctx->d_act_off, &ctx->d_act_lib, &ctx->d_tile_first, &ctx->d_lib_read0, &ctx->d_lib_ev0,
-                      &ctx->d_seen,   &ctx->d_solid,   &ctx->d_fs,      &ctx->d_libs,     &ctx->d_lib_w0,
-                      &ctx->d_blooms, &ctx->d_out,     &ctx->d_mincov,  &ctx->d_r1k0,     &ctx->d_r1k1,
-                      &ctx->d_r1t,    &ctx->d_r1lib,   &ctx->d_r2k0,    &ctx->d_r2k1,     &ctx->d_r2t,
-                      &ctx->d_r2lib,  &ctx->d_dk0,     &ctx->d_dk1,     &ctx->d_dlib,     &ctx->d_dcnt,
-                      &ctx->d_dt0,    &ctx->d_cursor1, &ctx->d_hist2,   &ctx->d_off2,     &ctx->d_cursor2,
-                      &ctx->d_chunk_bkt, &ctx->d_chunk_start,
+                      &ctx->d_seen,   &ctx->d_solid,   &ctx->d_fs,      &ctx->d_tab,      &ctx->d_libs,
+                      &ctx->d_lib_b0, &ctx->d_lib_w0,  &ctx->d_blooms,  &ctx->d_out,      &ctx->d_mincov,
                       &ctx->d_hi,     &ctx->d_lo,      &ctx->d_cnt,     &ctx->d_hi2,      &ctx->d_lo2,
                       &ctx->d_cnt2,   &ctx->d_idx,     &ctx->d_idx2,    &ctx->d_sflags,   &ctx->d_corr,
                       &ctx->d_hll,    &ctx->d_hist,    &ctx->d_counters, &ctx->d_temp,    &ctx->d_keys,
@@ -1611,9 +1304,9 @@ int kc_bloom_build(kc_ctx* ctx) {
     ctx->ev64 = false;
     for (auto& lib : ctx->libs) ctx->ev64 = ctx->ev64 || lib.info.n_occ >= (1ull << 30) - 1;
     if (ctx->n_occ == 0) {
+        std::vector<uint32_t> grow(ctx->libs.size(), 0);
         for (auto& lib : ctx->libs) lib.est = 0;
-        ctx->D = 0;
-        rc = run.plan_bloom<uint32_t>();
+        rc = ctx->wide ? run.plan<2, uint32_t>(grow) : run.plan<1, uint32_t>(grow);
         if (rc) return rc;
         k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_seen.as<uint64_t>(),
                                                                            ctx->total_words, 0ull);
@@ -1638,7 +1331,7 @@ int kc_bloom_build(kc_ctx* ctx) {
         lib.info.ms_stage2 = ms;
         lib.info.num_words = lib.num_words;
         lib.info.n_distinct = ctx->h_out[l].n_distinct;
-        lib.info.table_capacity = lib.est;  // the distinct-count estimate that sized the partition
+        lib.info.table_capacity = lib.nbuckets * (64 / (ctx->wide || ctx->ev64 ? 32 : 16));
     }
     ctx->built = true;
     return KC_OK;

@@ -150,6 +150,51 @@ int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_
                              size_t n);
 int kc_library_reads_export(kc_ctx* ctx, uint32_t lib, uint8_t* data, size_t data_len);
 
+/* Prior / trusted contigs of a library (assemble.rs:244-284: AssemblerConfig::prior_contigs
+ * plus the sequences of trusted_contigs files).  n_seqs ASCII sequences, sequence i at
+ * seqs[starts[i] .. starts[i] + lengths[i]).  After Stage 3's count, every window whose k
+ * bases are all ACGT/acgt adds prior_boost = max((u32)min_coverage * 4, 50) (u32 wrapping)
+ * to its canonical k-mer, inserting it when absent; the cutoff and solid set then see the
+ * boosted counts.  Replaces the library's previous priors (n_seqs = 0 clears them); the
+ * sequences are copied.  Takes effect at the next kc_bloom_build / kc_run, which sizes the
+ * table for the prior windows. */
+int kc_library_priors_set(kc_ctx* ctx, uint32_t lib, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts,
+                          const uint64_t* lengths, size_t n_seqs);
+int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts, const uint64_t* lengths,
+                  size_t n_seqs); /* library 0 */
+
+/* ---- Tuning and graph replay ---------------------------------------------- */
+
+/* Launch shape of the table insert and the table load factor.  Every setting gives
+ * identical results; only the speed differs. */
+typedef struct kc_tune {
+    uint32_t windows_per_thread; /* 1, 2 or 4: table probes each thread keeps in flight */
+    uint32_t blocks_per_cu;      /* 0: one workgroup per 1024-window tile; else CUs x this
+                                    workgroups, each looping over tiles */
+    float table_load;            /* device hash-table target load factor (0: kc_opts.table_load) */
+    uint32_t reserved;           /* 0 */
+    double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
+} kc_tune;
+
+int kc_tune_get(const kc_ctx* ctx, kc_tune* tune);
+int kc_tune_set(kc_ctx* ctx, const kc_tune* tune);
+/* Time Stage 2 + 3 over the uploaded libraries for a coordinate search over the load
+ * factor, the windows per thread and the workgroups per CU (reps timed runs each, after
+ * one warm-up), check every setting gives the same per-library summary, and keep the
+ * fastest.  *best (nullable) receives it. */
+int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best);
+
+/* Stage 2 + Stage 3 as one HIP graph launch.  The first call (and the first after new
+ * reads, priors, tuning or an eager kc_bloom_build / kc_count) runs the stages eagerly
+ * to fix every size (table layout, count-table entries), then captures their device work
+ * into a graph; later calls replay it.  A replay whose data-dependent sizes differ from
+ * the plan (entries, table overflow, read length under error_correct) is redone eagerly
+ * and re-captured, so results always equal kc_run's.  The per-read preparation that
+ * depends only on the uploaded lengths is done at planning time.  In info, ms_stage2
+ * holds the whole replay and ms_stage3 is 0. */
+int kc_graph_run(kc_ctx* ctx, kc_info* info);
+int kc_graph_release(kc_ctx* ctx);
+
 /* Per-kernel device time of the last run (KC_FLAG_TIMING): fills up to cap entries of
  * names (static strings), total ms and launch counts; returns the number of kernels. */
 int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap);

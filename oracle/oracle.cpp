@@ -381,12 +381,16 @@ static void get_read(const uint8_t* data, uint64_t off, uint32_t len, std::vecto
 // ---------------------------------------------------------------------------
 // assemble.rs:692-735 determine_solid_kmer_cutoff
 // ---------------------------------------------------------------------------
+// Rust's `f as u32`: truncation toward zero, saturating at both ends, NaN -> 0.
+static uint32_t f64_as_u32(double v) {
+    if (!(v > 0.0)) return 0;
+    if (v >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
 static uint32_t determine_solid_kmer_cutoff(std::vector<uint32_t> covs, double min_coverage, uint32_t* top_out) {
     if (top_out) *top_out = 0;
-    if (min_coverage > 5.0) {
-        double v = std::max(min_coverage * 0.35, 2.0);
-        return (uint32_t)v;  // `as u32` saturates; values here are small and finite
-    }
+    if (min_coverage > 5.0) return f64_as_u32(std::max(min_coverage * 0.35, 2.0));
     if (covs.size() <= 100) return 2;
     std::sort(covs.begin(), covs.end());
     uint32_t top = covs[covs.size() * 95 / 100];
@@ -449,9 +453,17 @@ static size_t next_pow2(size_t x) {
     return p;
 }
 
+// `config.min_coverage as u32` (Rust's saturating float-to-int cast), then `* 4` and
+// `.max(50)` as in assemble.rs:270 (u32 arithmetic wraps in a release build).
+static uint32_t prior_boost_of(double min_coverage) {
+    return std::max<uint32_t>(f64_as_u32(min_coverage) * 4u, 50u);
+}
+
 static Spectrum* spectrum_run(const uint8_t* data, const uint64_t* offsets, const uint32_t* lengths,
                               uint64_t n_reads, unsigned k, uint64_t bloom_bits, int error_correct,
-                              double min_coverage, int threads) {
+                              double min_coverage, int threads, const uint8_t* prior = nullptr,
+                              const uint64_t* prior_starts = nullptr, const uint64_t* prior_lens = nullptr,
+                              uint64_t n_priors = 0) {
     auto* out = new Spectrum();
     out->k = k;
     auto t0 = std::chrono::steady_clock::now();
@@ -554,12 +566,28 @@ static Spectrum* spectrum_run(const uint8_t* data, const uint64_t* offsets, cons
     }
 
     // assemble.rs:238-242 merge; outputs are sorted here so they can be compared.
-    std::vector<std::pair<Kmer256, uint32_t>> all;
+    std::unordered_map<Kmer256, uint32_t, Kmer256Hash> global;
     size_t total = 0;
     for (auto& m : shards) total += m.size();
-    all.reserve(total);
-    for (auto& m : shards)
-        for (auto& kv : m) all.push_back(kv);
+    global.reserve(total);
+    for (auto& m : shards) global.insert(m.begin(), m.end());
+    // assemble.rs:266-283 prior / trusted contigs: every valid window of every prior adds
+    // prior_boost to its canonical k-mer, inserting it when absent (u32 wraps).
+    if (n_priors) {
+        const uint32_t boost = prior_boost_of(min_coverage);
+        for (uint64_t p = 0; p < n_priors; ++p) {
+            const uint8_t* seq = prior + prior_starts[p];
+            const uint64_t len = prior_lens[p];
+            if (len < k) continue;
+            for (uint64_t i = 0; i + k <= len; ++i) {
+                Kmer256 km;
+                if (!kmer_from_bytes(seq + i, k, k, &km)) continue;
+                uint32_t& c = global[kmer_canonical(km, k, nullptr)];
+                c += boost;
+            }
+        }
+    }
+    std::vector<std::pair<Kmer256, uint32_t>> all(global.begin(), global.end());
     std::sort(all.begin(), all.end(), [](const std::pair<Kmer256, uint32_t>& a, const std::pair<Kmer256, uint32_t>& b) {
         return a.first < b.first;
     });
@@ -668,6 +696,15 @@ void* or_spectrum_run(const uint8_t* data, const uint64_t* offsets, const uint32
                       uint32_t k, uint64_t bloom_bits, int error_correct, double min_cov, int threads) {
     return spectrum_run(data, offsets, lengths, n_reads, k, bloom_bits, error_correct, min_cov, threads);
 }
+// Same, with prior / trusted contigs given as ASCII sequences (starts/lens into `prior`).
+void* or_spectrum_run_priors(const uint8_t* data, const uint64_t* offsets, const uint32_t* lengths, uint64_t n_reads,
+                             uint32_t k, uint64_t bloom_bits, int error_correct, double min_cov, int threads,
+                             const uint8_t* prior, const uint64_t* prior_starts, const uint64_t* prior_lens,
+                             uint64_t n_priors) {
+    return spectrum_run(data, offsets, lengths, n_reads, k, bloom_bits, error_correct, min_cov, threads, prior,
+                        prior_starts, prior_lens, n_priors);
+}
+uint32_t or_prior_boost(double min_cov) { return prior_boost_of(min_cov); }
 void or_spectrum_free(void* h) { delete (Spectrum*)h; }
 uint64_t or_spectrum_num_words(void* h) { return ((Spectrum*)h)->num_words; }
 uint64_t or_spectrum_num_counts(void* h) { return ((Spectrum*)h)->keys.size(); }

@@ -59,6 +59,10 @@ def lib():
             "or_get_read": (None, [_u8p, ctypes.c_uint64, ctypes.c_uint32, _u8p]),
             "or_spectrum_run": (ctypes.c_void_p, [_u8p, _u64p, _u32p, ctypes.c_uint64, ctypes.c_uint32,
                                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+            "or_spectrum_run_priors": (ctypes.c_void_p, [_u8p, _u64p, _u32p, ctypes.c_uint64, ctypes.c_uint32,
+                                                         ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                         _u8p, _u64p, _u64p, ctypes.c_uint64]),
+            "or_prior_boost": (ctypes.c_uint32, [ctypes.c_double]),
             "or_spectrum_free": (None, [ctypes.c_void_p]),
             "or_spectrum_num_words": (ctypes.c_uint64, [ctypes.c_void_p]),
             "or_spectrum_num_counts": (ctypes.c_uint64, [ctypes.c_void_p]),
@@ -249,21 +253,46 @@ def get_read(data, off, ln):
     return bytes(out[: int(ln)])
 
 
+def concat_ascii(seqs):
+    """Concatenate ASCII sequences: (buffer, starts, lengths) as numpy arrays."""
+    bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    lens = np.array([len(b) for b in bs], dtype=np.uint64)
+    starts = np.zeros(len(bs), dtype=np.uint64)
+    if len(bs) > 1:
+        starts[1:] = np.cumsum(lens)[:-1]
+    buf = np.frombuffer(b"".join(bs) + b"\0", dtype=np.uint8).copy()
+    return buf, starts, lens
+
+
+def prior_boost(min_cov):
+    """assemble.rs:270: (min_coverage as u32 * 4).max(50)."""
+    return lib().or_prior_boost(float(min_cov))
+
+
 class Spectrum:
     """Stage 2 + Stage 3 of run_assembly_with_packed_reads (assemble.rs:129-298).
 
     threads == 1: the sequential oracle; threads > 1: the reference's parallel structure
     (timed CPU baseline; its solid tier may legitimately differ, bloom.rs:95-103)."""
 
-    def __init__(self, data, offsets, lengths, k, bloom_bits, error_correct=False, min_cov=5.0, threads=1):
+    def __init__(self, data, offsets, lengths, k, bloom_bits, error_correct=False, min_cov=5.0, threads=1,
+                 priors=None):
+        """priors: prior / trusted contig sequences (ASCII bytes or str), assemble.rs:244-284."""
         data = np.ascontiguousarray(data, dtype=np.uint8)
         if data.size == 0:
             data = np.zeros(1, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
         L = lib()
-        self._h = L.or_spectrum_run(_p(data, _u8p), _p(offsets, _u64p), _p(lengths, _u32p), lengths.shape[0],
-                                    k, bloom_bits, int(bool(error_correct)), float(min_cov), int(threads))
+        if priors:
+            buf, starts, lens = concat_ascii(priors)
+            self._h = L.or_spectrum_run_priors(_p(data, _u8p), _p(offsets, _u64p), _p(lengths, _u32p),
+                                               lengths.shape[0], k, bloom_bits, int(bool(error_correct)),
+                                               float(min_cov), int(threads), _p(buf, _u8p), _p(starts, _u64p),
+                                               _p(lens, _u64p), len(priors))
+        else:
+            self._h = L.or_spectrum_run(_p(data, _u8p), _p(offsets, _u64p), _p(lengths, _u32p), lengths.shape[0],
+                                        k, bloom_bits, int(bool(error_correct)), float(min_cov), int(threads))
         self.k = k
         self.num_words = L.or_spectrum_num_words(self._h)
         self.n_counts = L.or_spectrum_num_counts(self._h)

@@ -184,18 +184,27 @@ __device__ __forceinline__ uint64_t table_hash(const uint64_t key[NW]) {
     return fmix64(h);
 }
 
+// Probe loads of table words.  Every key word goes from EMPTY to its final value once and
+// t0 only decreases, so any value a load returns is either current or an older one that
+// the protocol tolerates (an EMPTY key word is settled by the CAS, a larger t0 only costs
+// an extra atomicMin).  Workgroup scope lets the per-XCD L2 serve the line.
 template <typename T>
 __device__ __forceinline__ T ld_relaxed(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// First key word of every slot of bucket b (one 64-byte line).
+// First key word (and, when T0 is given, the first event) of every slot of bucket b (one
+// 64-byte line).
 template <int NW, typename Ev>
 __device__ __forceinline__ void bucket_load(const Slot<NW, Ev>* __restrict__ tab, uint64_t b,
-                                            uint64_t w0[slots_per_bucket<NW, Ev>()]) {
+                                            uint64_t w0[slots_per_bucket<NW, Ev>()],
+                                            Ev* t0 = nullptr) {
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
 #pragma unroll
-    for (uint32_t s = 0; s < SPB; ++s) w0[s] = ld_relaxed(&tab[b * SPB + s].k[0]);
+    for (uint32_t s = 0; s < SPB; ++s) {
+        w0[s] = ld_relaxed(&tab[b * SPB + s].k[0]);
+        if (t0) t0[s] = ld_relaxed(&tab[b * SPB + s].t0);
+    }
 }
 
 // Try to find or claim key in bucket b given the bucket's first key words w0 (possibly
@@ -228,32 +237,61 @@ __device__ __forceinline__ uint64_t bucket_find(Slot<NW, Ev>* __restrict__ tab, 
     return ~0ull;
 }
 
-// Record one occurrence (event t) in slot i.
+// Record one occurrence (event t) in slot i.  seen_t0 is a value of the slot's t0 read
+// earlier in this kernel: t0 only decreases, so t >= seen_t0 means t cannot be the
+// minimum and the atomicMin is skipped.
 template <int NW, typename Ev, bool TRACK_T0>
-__device__ __forceinline__ void slot_hit(Slot<NW, Ev>* __restrict__ tab, uint64_t i, Ev t) {
+__device__ __forceinline__ void slot_hit(Slot<NW, Ev>* __restrict__ tab, uint64_t i, Ev t, Ev seen_t0) {
     // Neither result is needed, so both are issued without return and the wave does not wait.
     (void)__hip_atomic_fetch_add(&tab[i].cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (TRACK_T0) (void)__hip_atomic_fetch_min(&tab[i].t0, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (TRACK_T0 && t < seen_t0)
+        (void)__hip_atomic_fetch_min(&tab[i].t0, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Insert one occurrence of key, starting at bucket b whose first key words are w0.
-// Returns false when max_probe buckets were inspected without finding or claiming it.
-// The table region is buckets [b0, b0 + nbuckets); probing wraps inside it.
+// Insert one occurrence of key, starting at bucket b whose first key words are w0 and
+// whose first events are t0 (when TRACK_T0).  Returns false when max_probe buckets were
+// inspected without finding or claiming it.  The table region is buckets
+// [b0, b0 + nbuckets); probing wraps inside it.
 template <int NW, typename Ev, bool TRACK_T0>
 __device__ __forceinline__ bool table_insert_from(Slot<NW, Ev>* __restrict__ tab, uint64_t b0, uint64_t nbuckets,
                                                   uint64_t b, uint64_t w0[slots_per_bucket<NW, Ev>()],
-                                                  const uint64_t key[NW], Ev t, uint32_t max_probe) {
+                                                  Ev t0[slots_per_bucket<NW, Ev>()], const uint64_t key[NW], Ev t,
+                                                  uint32_t max_probe) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     const uint64_t probes = nbuckets < max_probe ? nbuckets : max_probe;
     for (uint64_t probe = 0; probe < probes; ++probe) {
         const uint64_t i = bucket_find<NW, Ev>(tab, b, w0, key);
         if (i != ~0ull) {
-            slot_hit<NW, Ev, TRACK_T0>(tab, i, t);
+            Ev seen = (Ev)~(Ev)0;
+            if (TRACK_T0) {
+#pragma unroll
+                for (uint32_t s = 0; s < SPB; ++s)
+                    if (i == b * SPB + s) seen = t0[s];
+            }
+            slot_hit<NW, Ev, TRACK_T0>(tab, i, t, seen);
             return true;
         }
         b = (b + 1 == b0 + nbuckets) ? b0 : b + 1;
-        bucket_load<NW, Ev>(tab, b, w0);
+        bucket_load<NW, Ev>(tab, b, w0, TRACK_T0 ? t0 : nullptr);
     }
     return false;
+}
+
+// Find or claim key starting at bucket b (first key words w0); returns its slot, or ~0
+// when max_probe buckets were inspected without finding or claiming it.
+template <int NW, typename Ev>
+__device__ __forceinline__ uint64_t table_find_or_claim(Slot<NW, Ev>* __restrict__ tab, uint64_t b0,
+                                                        uint64_t nbuckets, uint64_t b,
+                                                        uint64_t w0[slots_per_bucket<NW, Ev>()],
+                                                        const uint64_t key[NW], uint32_t max_probe) {
+    const uint64_t probes = nbuckets < max_probe ? nbuckets : max_probe;
+    for (uint64_t probe = 0; probe < probes; ++probe) {
+        const uint64_t i = bucket_find<NW, Ev>(tab, b, w0, key);
+        if (i != ~0ull) return i;
+        b = (b + 1 == b0 + nbuckets) ? b0 : b + 1;
+        bucket_load<NW, Ev>(tab, b, w0);
+    }
+    return ~0ull;
 }
 
 template <int NW>

@@ -38,6 +38,7 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_merge_sort.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
@@ -69,7 +70,20 @@ struct LibDesc {
     void* fs;           // num_words * 64 first-setter entries (valid while its group runs)
     Modulus M;
     uint64_t num_words;
+    uint32_t boost;     // prior / trusted-contig boost per window (assemble.rs:270)
+    uint32_t pad;
 };
+
+// prior_boost of assemble.rs:270, (min_coverage as u32 * 4).max(50): the float-to-int
+// cast saturates (NaN and negatives give 0) and the u32 product wraps.
+uint32_t prior_boost(double min_cov) {
+    uint32_t mc;
+    if (!(min_cov > 0.0)) mc = 0;
+    else if (min_cov >= 4294967295.0) mc = 0xFFFFFFFFu;
+    else mc = (uint32_t)min_cov;
+    const uint32_t b = mc * 4u;
+    return b > 50u ? b : 50u;
+}
 
 // Per-library results written on the device.
 struct LibOut {
@@ -281,56 +295,66 @@ __global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t n_slots) 
 }
 
 // Stage 2 insert (TRACK_T0, every window) or Stage 3 recount after Hammer (SOLID_ONLY:
-// only windows whose canonical k-mer is Bloom-solid, assemble.rs:196).  Each thread
-// takes WPT windows of the tile and issues their first bucket loads together, so a wave
-// keeps WPT independent random table accesses in flight.
-template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY>
+// only windows whose canonical k-mer is Bloom-solid, assemble.rs:196).  Block b takes
+// tiles b, b + gridDim.x, ... (one tile each when the grid covers every tile).  Within a
+// tile each thread takes WPT windows at a time and issues their first bucket loads
+// together, so a wave keeps WPT independent random table accesses in flight.
+template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY, uint32_t WPT>
 __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                   const uint64_t* __restrict__ act_off,
                                                   const uint32_t* __restrict__ act_lib,
                                                   const uint32_t* __restrict__ tile_first, uint64_t n_active,
-                                                  uint64_t n_occ, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                  uint64_t n_occ, uint64_t n_tiles, uint32_t k,
+                                                  Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
                                                   const LibDesc* __restrict__ libs, uint32_t max_probe,
                                                   LibOut* __restrict__ out) {
     constexpr int NW = WIDE ? 2 : 1;
-    constexpr uint32_t WPT = TILE_WINDOWS / BLOCK;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    constexpr uint32_t PASS = BLOCK * WPT;
+    static_assert(TILE_WINDOWS % PASS == 0, "a tile is a whole number of passes");
     __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
-    const uint64_t tile = blockIdx.x;
-    const uint64_t a0 = tile_first[tile];
-    const uint64_t w0 = tile * TILE_WINDOWS;
-    const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
-    const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
-    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
-    __syncthreads();
-    uint64_t key[WPT][NW], bkt[WPT], pre[WPT][SPB];
-    uint32_t lib[WPT];
-    bool valid[WPT];
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const uint64_t a0 = tile_first[tile];
+        const uint64_t w0 = tile * TILE_WINDOWS;
+        const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+        const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+        __syncthreads();  // the previous tile's reads of s_ev are done
+        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+        __syncthreads();
+        for (uint64_t wp = w0; wp < w1; wp += PASS) {
+            uint64_t key[WPT][NW], bkt[WPT], pre[WPT][SPB];
+            Ev pre_t0[WPT][SPB];
+            uint32_t lib[WPT];
+            bool valid[WPT];
 #pragma unroll
-    for (uint32_t j = 0; j < WPT; ++j) {
-        const uint64_t w = w0 + j * BLOCK + threadIdx.x;
-        valid[j] = w < w1;
-        if (!valid[j]) continue;
-        const uint32_t lo = tile_read(s_ev, na, w);
-        lib[j] = act_lib[a0 + lo];
-        const Kv c = canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
-        if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
-            valid[j] = false;
-            continue;
+            for (uint32_t j = 0; j < WPT; ++j) {
+                const uint64_t w = wp + j * BLOCK + threadIdx.x;
+                valid[j] = w < w1;
+                if (!valid[j]) continue;
+                const uint32_t lo = tile_read(s_ev, na, w);
+                lib[j] = act_lib[a0 + lo];
+                const Kv c =
+                    canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
+                if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
+                    valid[j] = false;
+                    continue;
+                }
+                pack_key<NW>(c, key[j]);
+                bkt[j] = libs[lib[j]].bucket0 + home_bucket<NW>(key[j], libs[lib[j]].nbuckets);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < WPT; ++j)
+                if (valid[j]) bucket_load<NW, Ev>(tab, bkt[j], pre[j], TRACK_T0 ? pre_t0[j] : nullptr);
+#pragma unroll
+            for (uint32_t j = 0; j < WPT; ++j) {
+                if (!valid[j]) continue;
+                const LibDesc& L = libs[lib[j]];
+                const Ev t = (Ev)(wp + j * BLOCK + threadIdx.x - L.ev0);
+                if (!table_insert_from<NW, Ev, TRACK_T0>(tab, L.bucket0, L.nbuckets, bkt[j], pre[j], pre_t0[j], key[j],
+                                                         t, max_probe))
+                    atomicOr(&out[lib[j]].overflow, 1u);
+            }
         }
-        pack_key<NW>(c, key[j]);
-        bkt[j] = libs[lib[j]].bucket0 + home_bucket<NW>(key[j], libs[lib[j]].nbuckets);
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < WPT; ++j)
-        if (valid[j]) bucket_load<NW, Ev>(tab, bkt[j], pre[j]);
-#pragma unroll
-    for (uint32_t j = 0; j < WPT; ++j) {
-        if (!valid[j]) continue;
-        const LibDesc& L = libs[lib[j]];
-        const Ev t = (Ev)(w0 + j * BLOCK + threadIdx.x - L.ev0);
-        if (!table_insert_from<NW, Ev, TRACK_T0>(tab, L.bucket0, L.nbuckets, bkt[j], pre[j], key[j], t, max_probe))
-            atomicOr(&out[lib[j]].overflow, 1u);
     }
 }
 
@@ -445,30 +469,39 @@ __global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restric
 // Gather the count table: occupied slots whose k-mer is Bloom-solid (or every occupied
 // slot when the table was built from Bloom-solid windows only), as (hi, lo, count) sort
 // entries.  Each block reserves its output range with one atomic per chunk.
+// With priors (psum/pflag not null, assemble.rs:266-283), a flagged slot is in the table
+// whatever its Bloom state, and its count is (counted occurrences, if Bloom-solid) + the
+// boost sum, in wrapping u32 arithmetic.
 template <int NW, typename Ev, bool CHECK_SOLID>
 __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_end,
                                                    const uint64_t* __restrict__ lib_b0, uint32_t n_libs,
                                                    const LibDesc* __restrict__ libs, SortLayout lay,
                                                    uint64_t* __restrict__ out_hi, uint64_t* __restrict__ out_lo,
                                                    uint32_t* __restrict__ out_cnt,
-                                                   unsigned long long* __restrict__ counter) {
+                                                   unsigned long long* __restrict__ counter,
+                                                   const uint32_t* __restrict__ psum,
+                                                   const uint8_t* __restrict__ pflag) {
     __shared__ uint32_t s_scan[BLOCK];
     __shared__ unsigned long long s_base;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
     const uint32_t tid = threadIdx.x;
     for (uint64_t c0 = blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
-        uint32_t bits = 0, cnt = 0;
+        uint32_t bits = 0, counted = 0, cnt = 0;  // bits: in the table; counted: occurrences count
         for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
             if (i >= s_end) break;
             const Slot<NW, Ev> sl = tab[i];
-            bool take = sl.k[0] != EMPTY_KEY;
-            if (CHECK_SOLID && take) {
+            const bool occ = sl.k[0] != EMPTY_KEY;
+            bool base = occ;
+            if (CHECK_SOLID && base) {
+                // a slot claimed only by a prior holds no occurrence
                 const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
-                take = bloom_is_solid(unpack_key<NW>(sl.k), L.solid, L.M);
+                base = sl.cnt > 0 && bloom_is_solid(unpack_key<NW>(sl.k), L.solid, L.M);
             }
+            const bool take = base || (occ && pflag && pflag[i]);
             bits |= (uint32_t)take << j;
+            counted |= (uint32_t)base << j;
             cnt += take;
         }
         s_scan[tid] = cnt;
@@ -492,10 +525,96 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             encode_entry(lay, find_region(lib_b0, n_libs, i / SPB), unpack_key<NW>(sl.k), hi, lo);
             out_lo[o] = lo;
             if (!lay.single) out_hi[o] = hi;
-            out_cnt[o] = sl.cnt;
+            out_cnt[o] = ((counted >> j) & 1u ? sl.cnt : 0u) + (psum ? psum[i] : 0u);
             ++o;
         }
         __syncthreads();
+    }
+}
+
+// Prior / trusted contigs (assemble.rs:266-283): every window of a prior whose k bases
+// are all ACGT/acgt (Kmer256::from_bytes, dna.rs:299-321) adds its library's boost to the
+// slot of its canonical k-mer, claiming the slot when the k-mer is absent, and flags the
+// slot as present in the count table.  Window w belongs to prior p = the last with
+// pw0[p] <= w (every listed prior has at least one window); it starts at byte
+// pstart[p] + (w - pw0[p]) of seq.
+template <bool WIDE, typename Ev>
+__global__ void k_prior(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ pstart,
+                        const uint64_t* __restrict__ pw0, const uint32_t* __restrict__ plib, uint32_t n_priors,
+                        uint64_t n_win, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                        const LibDesc* __restrict__ libs, uint32_t* __restrict__ psum, uint8_t* __restrict__ pflag,
+                        LibOut* __restrict__ out) {
+    constexpr int NW = WIDE ? 2 : 1;
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < n_win;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = find_region(pw0, n_priors, w);
+        const uint8_t* s = seq + pstart[p] + (w - pw0[p]);
+        Kv v{0, 0};
+        bool ok = true;
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t c = s[i] | 0x20u;  // 'A' and 'a' both map to 'a', and so on
+            uint64_t code;
+            if (c == 'a') code = 0;
+            else if (c == 'c') code = 1;
+            else if (c == 'g') code = 2;
+            else if (c == 't') code = 3;
+            else {
+                ok = false;
+                break;
+            }
+            v.hi = (v.hi << 2) | (v.lo >> 62);
+            v.lo = (v.lo << 2) | code;
+        }
+        if (!ok) continue;
+        uint64_t key[NW];
+        pack_key<NW>(canonical<WIDE>(v, k), key);
+        const uint32_t l = plib[p];
+        const LibDesc& L = libs[l];
+        const uint64_t b = L.bucket0 + home_bucket<NW>(key, L.nbuckets);
+        uint64_t w0[SPB];
+        bucket_load<NW, Ev>(tab, b, w0);
+        const uint64_t i = table_find_or_claim<NW, Ev>(tab, L.bucket0, L.nbuckets, b, w0, key, 1u << 20);
+        if (i == ~0ull) {
+            atomicOr(&out[l].overflow, 1u);
+            continue;
+        }
+        atomicAdd(&psum[i], L.boost);
+        pflag[i] = 1;
+    }
+}
+
+// Byte copy (the graph's stand-in for a device-to-device memcpy): 8-byte words, then the tail.
+__global__ void k_copy_bytes(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
+    const uint64_t nw = n / 8;
+    const uint64_t* s8 = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d8 = reinterpret_cast<uint64_t*>(dst);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw; i += stride) d8[i] = s8[i];
+    for (uint64_t i = nw * 8 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Two-word sort key of a count-table entry: (hi, lo) compared as one 128-bit value.
+struct Key128 {
+    uint64_t hi, lo;
+};
+struct Key128Less {
+    __device__ __host__ bool operator()(const Key128& a, const Key128& b) const {
+        return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+    }
+};
+
+__global__ void k_pack_keys(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo, Key128* __restrict__ out,
+                            uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = Key128{hi[i], lo[i]};
+}
+
+__global__ void k_unpack_keys(const Key128* __restrict__ in, uint64_t* __restrict__ hi, uint64_t* __restrict__ lo,
+                              uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        hi[i] = in[i].hi;
+        lo[i] = in[i].lo;
     }
 }
 
@@ -609,12 +728,12 @@ __global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restric
 
 enum KernelId {
     KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_HLL, KID_CLEAR, KID_INSERT, KID_FS_CLEAR, KID_FIRST_SETTER,
-    KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS, KID_COUNT
+    KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_PRIOR, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS, KID_COUNT
 };
 const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
-    "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_compact", "radix_sort", "k_count_hist",
-    "k_solid_flags"};
+    "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_prior", "k_compact", "radix_sort",
+    "k_count_hist", "k_solid_flags"};
 
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
 struct HostLib {
@@ -623,6 +742,14 @@ struct HostLib {
     std::vector<uint32_t> len;
     uint64_t bloom_bits = 0;
     double min_cov = 5.0;
+    // prior / trusted contigs (ASCII), applied to the count table (assemble.rs:244-284)
+    std::vector<uint8_t> prior;
+    std::vector<uint64_t> prior_start, prior_len;
+    uint64_t prior_windows(uint32_t k) const {
+        uint64_t n = 0;
+        for (uint64_t len : prior_len) n += len >= k ? len - k + 1 : 0;
+        return n;
+    }
     // placement in the device pools (set by the planner)
     uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
     uint32_t group = 0;
@@ -663,8 +790,11 @@ struct kc_ctx {
     // count entries (sorted) + scratch
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
+    DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     uint64_t n_entries = 0;
     SortLayout lay{};
+    kc_tune tune{4, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
+    uint32_t n_cu = 0;
     bool wide = false, ev64 = false, built = false, counted = false;
     uint64_t* h_stage = nullptr;  // small pinned staging
     std::vector<LibOut> h_out;
@@ -679,7 +809,19 @@ struct kc_ctx {
         err = m;
         return code;
     }
-    bool timing() const { return (o.flags & KC_FLAG_TIMING) != 0; }
+    bool timing() const { return (o.flags & KC_FLAG_TIMING) != 0 && !capturing; }
+    // graph replay (kc_graph_run)
+    bool capturing = false, graph_ok = false;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t g_entries = 0;         // count-table entries of the planned run
+    uint32_t g_np = 0;              // prior sequences / windows uploaded by the last count
+    uint64_t g_nw = 0;
+    DevBuf d_gk1, d_gk2;            // 128-bit sort keys of the graph's merge sort
+    void graph_release() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        graph_ok = false;
+    }
     hipEvent_t new_event() {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -848,6 +990,29 @@ struct Run {
     }
     std::vector<uint64_t> lib_ev0_;
 
+    // k_insert over every tile with the context's tuning (windows in flight per thread,
+    // workgroups per CU; 0 = one workgroup per tile).
+    template <bool WIDE, typename Ev, bool TRACK, bool SOLID>
+    int launch_insert(const uint8_t* data, int kid) {
+        constexpr int NW = WIDE ? 2 : 1;
+        const uint64_t tiles = c->n_tiles;
+        if (!tiles) return KC_OK;
+        uint64_t g = tiles;
+        if (c->tune.blocks_per_cu) g = std::min<uint64_t>(tiles, (uint64_t)c->n_cu * c->tune.blocks_per_cu);
+        if (g > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "too many tiles for one launch");
+        hipStream_t st = c->st;
+#define KC_INSERT_LAUNCH(W)                                                                                       \
+    TIMED(kid, (k_insert<WIDE, Ev, TRACK, SOLID, W><<<(uint32_t)g, BLOCK, 0, st>>>(                              \
+                   data, c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), \
+                   c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ, tiles, c->o.k,                         \
+                   c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), 1u << 20, c->d_out.as<LibOut>())))
+        if (c->tune.windows_per_thread == 2) KC_INSERT_LAUNCH(2);
+        else if (c->tune.windows_per_thread == 1) KC_INSERT_LAUNCH(1);
+        else KC_INSERT_LAUNCH(4);
+#undef KC_INSERT_LAUNCH
+        return KC_OK;
+    }
+
     template <bool WIDE>
     int estimate_distinct() {
         hipStream_t st = c->st;
@@ -882,13 +1047,15 @@ struct Run {
     // Lay out the table pool, the Bloom pools and the first-setter groups.
     template <int NW, typename Ev>
     int plan(const std::vector<uint32_t>& grow) {
-        const double load = (c->o.table_load > 0.05f && c->o.table_load < 0.95f) ? c->o.table_load : 0.5;
+        const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
+        const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         uint64_t b0 = 0, w0 = 0;
         for (uint32_t l = 0; l < L(); ++l) {
             auto& lib = c->libs[l];
-            const uint64_t occ = lib.info.n_occ;
-            uint64_t cap = (uint64_t)std::ceil((double)lib.est * 1.10 / load) + 64;
+            const uint64_t pw = lib.prior_windows(c->o.k);
+            const uint64_t occ = lib.info.n_occ + pw;
+            uint64_t cap = (uint64_t)std::ceil((double)(lib.est + pw) * 1.10 / load) + 64;
             cap = std::min<uint64_t>(cap, occ * 2 + 64) << grow[l];
             lib.nbuckets = occ ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
             lib.bucket0 = b0;
@@ -940,6 +1107,7 @@ struct Run {
             d.fs = c->d_fs.as<uint8_t>() + lib.fs_off;
             d.M = make_modulus(lib.num_words * 64);
             d.num_words = lib.num_words;
+            d.boost = prior_boost(lib.min_cov);
             blooms[l] = LibBloom{d.solid, d.M};
             lb0[l] = lib.bucket0;
             lw0[l] = lib.word0;
@@ -991,11 +1159,8 @@ struct Run {
             const uint64_t n_slots = c->total_buckets * SPB;
             TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(
                                  c->d_tab.as<Slot<NW, Ev>>(), n_slots)));
-            TIMED(KID_INSERT, (k_insert<WIDE, Ev, true, false><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
-                                  c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
-                                  c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
-                                  c->o.k, c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), 1u << 20,
-                                  c->d_out.as<LibOut>())));
+            rc = launch_insert<WIDE, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+            if (rc) return rc;
             rc = read_out();
             if (rc) return rc;
             bool any = false;
@@ -1089,6 +1254,62 @@ struct Run {
         return KC_OK;
     }
 
+    // Upload every library's prior windows and add them to the (final) count table.
+    // Sets `any` when at least one prior window exists.
+    template <bool WIDE, typename Ev>
+    int apply_priors(uint64_t slots, bool& any) {
+        constexpr int NW = WIDE ? 2 : 1;
+        hipStream_t st = c->st;
+        const uint32_t k = c->o.k;
+        std::vector<uint8_t> seq;
+        std::vector<uint64_t> pstart, pw0;
+        std::vector<uint32_t> plib;
+        uint64_t nw = 0;
+        for (uint32_t l = 0; l < L(); ++l) {
+            const auto& lib = c->libs[l];
+            for (size_t p = 0; p < lib.prior_len.size(); ++p) {
+                const uint64_t len = lib.prior_len[p];
+                if (len < k) continue;
+                pstart.push_back(seq.size());
+                pw0.push_back(nw);
+                plib.push_back(l);
+                seq.insert(seq.end(), lib.prior.begin() + lib.prior_start[p],
+                           lib.prior.begin() + lib.prior_start[p] + len);
+                nw += len - k + 1;
+            }
+        }
+        any = nw > 0;
+        c->g_np = 0;
+        c->g_nw = 0;
+        if (!any) return KC_OK;
+        if (pstart.size() > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "too many prior sequences");
+        const uint32_t np = (uint32_t)pstart.size();
+        KC_TRY(c->d_pseq.ensure(seq.size()));
+        KC_TRY(c->d_pstart.ensure(np * 8));
+        KC_TRY(c->d_pw0.ensure(np * 8));
+        KC_TRY(c->d_plib.ensure(np * 4));
+        KC_TRY(c->d_psum.ensure(slots * 4));
+        KC_TRY(c->d_pflag.ensure(slots));
+        KC_TRY(hipMemcpyAsync(c->d_pseq.p, seq.data(), seq.size(), hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_pstart.p, pstart.data(), np * 8, hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_pw0.p, pw0.data(), np * 8, hipMemcpyHostToDevice, st));
+        KC_TRY(hipMemcpyAsync(c->d_plib.p, plib.data(), np * 4, hipMemcpyHostToDevice, st));
+        k_fill<uint32_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_psum.as<uint32_t>(), slots, 0u);
+        k_fill<uint8_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_pflag.as<uint8_t>(), slots, (uint8_t)0);
+        KC_TRY(hipGetLastError());
+        TIMED(KID_PRIOR, (k_prior<WIDE, Ev><<<grid_for(nw), BLOCK, 0, st>>>(
+                             c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
+                             c->d_plib.as<uint32_t>(), np, nw, k, c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(),
+                             c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>())));
+        int rc = read_out();  // also keeps the host vectors alive until the copies are done
+        if (rc) return rc;
+        c->g_np = np;
+        c->g_nw = nw;
+        for (uint32_t l = 0; l < L(); ++l)
+            if (c->h_out[l].overflow) return fail(KC_ENOMEM, "device hash table overflow (priors)");
+        return KC_OK;
+    }
+
     // Stage 3 after the Bloom build: optional Hammer + recount, compaction, sort, cutoff.
     template <bool WIDE, typename Ev>
     int count() {
@@ -1120,11 +1341,8 @@ struct Run {
             TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
                                  c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
             if (c->n_tiles) {
-                TIMED(KID_RECOUNT, (k_insert<WIDE, uint32_t, false, true><<<(uint32_t)c->n_tiles, BLOCK, 0, st>>>(
-                                       c->d_corr.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
-                                       c->d_act_lib.as<uint32_t>(), c->d_tile_first.as<uint32_t>(), c->n_active,
-                                       c->n_occ, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(), c->d_libs.as<LibDesc>(),
-                                       1u << 20, c->d_out.as<LibOut>())));
+                rc = launch_insert<WIDE, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+                if (rc) return rc;
             }
             rc = read_out();
             if (rc) return rc;
@@ -1137,6 +1355,11 @@ struct Run {
                     return fail(KC_ENOMEM, "device hash table overflow (recount)");
         }
         const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
+        bool priors = false;
+        rc = ec ? apply_priors<WIDE, uint32_t>(slots, priors) : apply_priors<WIDE, Ev>(slots, priors);
+        if (rc) return rc;
+        const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
+        const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
         const uint64_t cap = std::max<uint64_t>(slots, 1);
         KC_TRY(c->d_hi.ensure(cap * 8));
         KC_TRY(c->d_lo.ensure(cap * 8));
@@ -1148,12 +1371,12 @@ struct Run {
             TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                                    c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
                                    c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
+                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag)));
         } else {
             TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                                    c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
                                    c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                   c->d_cnt.as<uint32_t>(), ctr + 4)));
+                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag)));
         }
         KC_TRY(hipMemcpyAsync(c->h_stage, ctr + 4, 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
@@ -1177,6 +1400,153 @@ struct Run {
                                     c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), n, c->lay,
                                     L(), c->d_out.as<LibOut>(), c->d_sflags.as<uint8_t>(), c->d_out.as<LibOut>())));
         return read_out();
+    }
+
+    // The device work of one Stage 2 + Stage 3 run, for capture into a HIP graph: the
+    // sizes are the last eager run's (reads prep, table plan, first-setter groups, prior
+    // windows, count-table entries), nothing is read back and nothing is allocated.  The
+    // results land where the eager run's did (d_hi/d_lo/d_cnt, d_sflags, d_out); d_out
+    // and the counters are cleared once, so the caller can check the overflow flags,
+    // the read-length flag and the number of compacted entries after a replay.
+    template <bool WIDE, typename Ev>
+    int enqueue_planned() {
+        constexpr int NW = WIDE ? 2 : 1;
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        hipStream_t st = c->st;
+        const bool ec = c->o.error_correct != 0;
+        const uint32_t multi = L() > 1;
+        unsigned long long* ctr = c->d_counters.as<unsigned long long>();
+        (void)multi;
+        // ---- Stage 2 ----
+        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
+        KC_TRY(hipGetLastError());
+        int rc = clear_out();
+        if (rc) return rc;
+        const uint64_t n_slots = c->total_buckets * SPB;
+        k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(), n_slots);
+        KC_TRY(hipGetLastError());
+        rc = launch_insert<WIDE, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+        if (rc) return rc;
+        for (auto& g : c->groups) {
+            const auto& lb = c->libs[g.first];
+            const auto& le = c->libs[g.second - 1];
+            const uint64_t s_begin = lb.bucket0 * SPB, s_end = (le.bucket0 + le.nbuckets) * SPB;
+            const uint64_t w_begin = lb.word0, w_end = le.word0 + le.num_words;
+            const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
+            k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0);
+            if (s_end > s_begin)
+                k_first_setter<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                    c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                    c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>());
+            k_seen_from_fs<Ev><<<grid_for((w_end - w_begin) * 64), BLOCK, 0, st>>>(
+                w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>());
+            if (s_end > s_begin)
+                k_promote<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                    c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
+                    c->d_libs.as<LibDesc>());
+            KC_TRY(hipGetLastError());
+        }
+        // ---- Stage 3 ----
+        if (ec) {
+            k_copy_bytes<<<grid_for(c->d_data.bytes / 8 + 1), BLOCK, 0, st>>>(c->d_data.as<uint8_t>(),
+                                                                             c->d_corr.as<uint8_t>(), c->d_data.bytes);
+            if (c->n_active)
+                k_hammer<WIDE><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
+                    c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(), c->n_active,
+                    c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(), c->d_rlib.as<uint32_t>(),
+                    c->d_blooms.as<LibBloom>(), c->o.k, &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
+                    c->d_counters.as<uint32_t>() + 6);
+            const uint64_t n_slots32 = c->total_buckets * slots_per_bucket<NW, uint32_t>();
+            k_table_clear<NW, uint32_t><<<grid_for(n_slots32), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, uint32_t>>(),
+                                                                              n_slots32);
+            KC_TRY(hipGetLastError());
+            rc = launch_insert<WIDE, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+            if (rc) return rc;
+        }
+        const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : SPB);
+        const bool priors = c->g_nw > 0;
+        if (priors) {
+            k_fill<uint32_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_psum.as<uint32_t>(), slots, 0u);
+            k_fill<uint8_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_pflag.as<uint8_t>(), slots, (uint8_t)0);
+            if (ec)
+                k_prior<WIDE, uint32_t><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
+                    c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
+                    c->d_plib.as<uint32_t>(), c->g_np, c->g_nw, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(),
+                    c->d_libs.as<LibDesc>(), c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>());
+            else
+                k_prior<WIDE, Ev><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
+                    c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
+                    c->d_plib.as<uint32_t>(), c->g_np, c->g_nw, c->o.k, c->d_tab.as<Slot<NW, Ev>>(),
+                    c->d_libs.as<LibDesc>(), c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>());
+            KC_TRY(hipGetLastError());
+        }
+        const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
+        const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
+        // compaction into the second buffers; the merge sort writes d_hi / d_lo / d_cnt
+        if (ec)
+            k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                c->lay, c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag);
+        else
+            k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(), c->lay,
+                c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag);
+        KC_TRY(hipGetLastError());
+        const uint64_t n = c->g_entries;
+        if (n) {
+            size_t tb = c->d_temp.bytes;
+            if (c->lay.single) {
+                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
+                                           rocprim::less<uint64_t>(), st));
+            } else {
+                k_pack_keys<<<grid_for(n), BLOCK, 0, st>>>(c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
+                                                           c->d_gk1.as<Key128>(), n);
+                KC_TRY(hipGetLastError());
+                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_gk1.as<Key128>(), c->d_gk2.as<Key128>(),
+                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, Key128Less(),
+                                           st));
+                k_unpack_keys<<<grid_for(n), BLOCK, 0, st>>>(c->d_gk2.as<Key128>(), c->d_hi.as<uint64_t>(),
+                                                             c->d_lo.as<uint64_t>(), n);
+                KC_TRY(hipGetLastError());
+            }
+        }
+        const size_t nh = (size_t)L() * HIST_BINS;
+        k_fill<uint32_t><<<grid_for(nh), BLOCK, 0, st>>>(c->d_hist.as<uint32_t>(), nh, 0u);
+        if (n)
+            k_count_hist<<<grid_for(n, BLOCK * 16, 4096), BLOCK, 0, st>>>(c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                                                         c->d_cnt.as<uint32_t>(), n, c->lay,
+                                                                         c->d_hist.as<uint32_t>());
+        k_cutoff<<<grid_for(L(), 64), 64, 0, st>>>(c->d_hist.as<uint32_t>(), L(), c->d_mincov.as<double>(),
+                                                   c->d_out.as<LibOut>());
+        if (n)
+            k_solid_flags<<<grid_for(n, BLOCK * 16, 4096), BLOCK, 0, st>>>(
+                c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), n, c->lay, L(),
+                c->d_out.as<LibOut>(), c->d_sflags.as<uint8_t>(), c->d_out.as<LibOut>());
+        KC_TRY(hipGetLastError());
+        return KC_OK;
+    }
+
+    // Buffers the planned run needs beyond the eager run's (called before capture).
+    int prepare_planned() {
+        const uint64_t n = c->g_entries;
+        size_t need = 0;
+        if (c->lay.single) {
+            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
+                                       rocprim::less<uint64_t>(), c->st));
+        } else {
+            KC_TRY(c->d_gk1.ensure(std::max<uint64_t>(n, 1) * sizeof(Key128)));
+            KC_TRY(c->d_gk2.ensure(std::max<uint64_t>(n, 1) * sizeof(Key128)));
+            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_gk1.as<Key128>(), c->d_gk2.as<Key128>(),
+                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, Key128Less(),
+                                       c->st));
+        }
+        KC_TRY(c->d_temp.ensure(need));
+        KC_TRY(c->d_hist.ensure((size_t)L() * HIST_BINS * 4));
+        KC_TRY(c->d_sflags.ensure(n + 1));
+        if (c->o.error_correct) KC_TRY(c->d_corr.ensure(c->d_data.bytes));
+        return KC_OK;
     }
 };
 
@@ -1209,7 +1579,7 @@ static int add_library(kc_ctx* ctx, const uint8_t* data, size_t data_len, const 
     lib.bloom_bits = bloom_bits;
     lib.min_cov = min_cov;
     ctx->libs.push_back(std::move(lib));
-    ctx->uploaded = ctx->built = ctx->counted = false;
+    ctx->uploaded = ctx->built = ctx->counted = ctx->graph_ok = false;
     return KC_OK;
 }
 
@@ -1231,6 +1601,10 @@ int kc_create(kc_ctx** out, const kc_opts* opts) {
         return KC_EHIP;
     }
     ctx->wide = opts->k > 32;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, opts->device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    ctx->n_cu = (uint32_t)ncu;
     *out = ctx;
     return KC_OK;
 }
@@ -1249,6 +1623,11 @@ void kc_destroy(kc_ctx* ctx) {
                       &ctx->d_hll,    &ctx->d_hist,    &ctx->d_counters, &ctx->d_temp,    &ctx->d_keys,
                       &ctx->d_out8};
     for (DevBuf* b : bufs) b->release();
+    ctx->graph_release();
+    ctx->d_gk1.release();
+    ctx->d_gk2.release();
+    DevBuf* pbufs[] = {&ctx->d_pseq, &ctx->d_pstart, &ctx->d_pw0, &ctx->d_plib, &ctx->d_psum, &ctx->d_pflag};
+    for (DevBuf* b : pbufs) b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -1273,6 +1652,28 @@ int kc_library_add(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint
     return rc;
 }
 
+int kc_library_priors_set(kc_ctx* ctx, uint32_t lib, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts,
+                          const uint64_t* lengths, size_t n_seqs) {
+    if (!ctx) return KC_EINVAL;
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    if (n_seqs && (!starts || !lengths)) return ctx->fail(KC_EINVAL, "null starts/lengths");
+    if (seqs_len && !seqs) return ctx->fail(KC_EINVAL, "null seqs");
+    for (size_t i = 0; i < n_seqs; ++i)
+        if (starts[i] > seqs_len || lengths[i] > seqs_len - starts[i])
+            return ctx->fail(KC_EINVAL, "prior " + std::to_string(i) + " extends past seqs_len");
+    HostLib& L = ctx->libs[lib];
+    L.prior.assign(seqs, seqs + seqs_len);
+    L.prior_start.assign(starts, starts + n_seqs);
+    L.prior_len.assign(lengths, lengths + n_seqs);
+    ctx->built = ctx->counted = ctx->graph_ok = false;
+    return KC_OK;
+}
+
+int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts, const uint64_t* lengths,
+                  size_t n_seqs) {
+    return kc_library_priors_set(ctx, 0, seqs, seqs_len, starts, lengths, n_seqs);
+}
+
 int kc_library_count(const kc_ctx* ctx, uint32_t* n) {
     if (!ctx || !n) return KC_EINVAL;
     *n = (uint32_t)ctx->libs.size();
@@ -1282,6 +1683,7 @@ int kc_library_count(const kc_ctx* ctx, uint32_t* n) {
 int kc_bloom_build(kc_ctx* ctx) {
     if (!ctx) return KC_EINVAL;
     if (ctx->libs.empty()) return ctx->fail(KC_ESTATE, "no reads uploaded");
+    ctx->graph_ok = false;  // an eager run may move buffers a captured graph points at
     CTX_TRY(hipSetDevice(ctx->o.device));
     ctx->reset_marks();
     ctx->built = ctx->counted = false;
@@ -1308,6 +1710,17 @@ int kc_bloom_build(kc_ctx* ctx) {
         for (auto& lib : ctx->libs) lib.est = 0;
         rc = ctx->wide ? run.plan<2, uint32_t>(grow) : run.plan<1, uint32_t>(grow);
         if (rc) return rc;
+        // an empty table for prior windows (sized by the plan)
+        const uint64_t n_slots = ctx->total_buckets * (ctx->wide ? 2 : 4);
+        if (n_slots) {
+            if (ctx->wide)
+                k_table_clear<2, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
+                    ctx->d_tab.as<Slot<2, uint32_t>>(), n_slots);
+            else
+                k_table_clear<1, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
+                    ctx->d_tab.as<Slot<1, uint32_t>>(), n_slots);
+            CTX_TRY(hipGetLastError());
+        }
         k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_seen.as<uint64_t>(),
                                                                            ctx->total_words, 0ull);
         k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_solid.as<uint64_t>(),
@@ -1340,12 +1753,15 @@ int kc_bloom_build(kc_ctx* ctx) {
 int kc_count(kc_ctx* ctx, kc_info* info) {
     if (!ctx) return KC_EINVAL;
     if (!ctx->built) return ctx->fail(KC_ESTATE, "kc_count before kc_bloom_build");
+    ctx->graph_ok = false;
     CTX_TRY(hipSetDevice(ctx->o.device));
     hipEvent_t e0 = ctx->new_event(), e1 = ctx->new_event();
     CTX_TRY(hipEventRecord(e0, ctx->st));
     Run run(ctx);
     int rc = KC_OK;
-    if (ctx->n_occ == 0) {
+    uint64_t prior_windows = 0;
+    for (auto& lib : ctx->libs) prior_windows += lib.prior_windows(ctx->o.k);
+    if (ctx->n_occ == 0 && prior_windows == 0) {
         ctx->n_entries = 0;
         ctx->lay = make_layout(ctx->o.k, (uint32_t)ctx->libs.size());
         ctx->h_out.assign(ctx->libs.size(), LibOut{});
@@ -1392,6 +1808,89 @@ int kc_run(kc_ctx* ctx, kc_info* info) {
     int rc = kc_bloom_build(ctx);
     if (rc) return rc;
     return kc_count(ctx, info);
+}
+
+// Eager run, then capture of the planned device work into ctx->gexec.
+static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
+    int rc = kc_run(ctx, info);
+    if (rc) return rc;
+    ctx->graph_release();
+    if (ctx->n_occ == 0) return KC_OK;  // nothing to replay: kc_graph_run runs eagerly
+    ctx->g_entries = ctx->n_entries;
+    Run run(ctx);
+    rc = run.prepare_planned();
+    if (rc) return rc;
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    CTX_TRY(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    if (!ctx->wide) rc = ctx->ev64 ? run.enqueue_planned<false, uint64_t>() : run.enqueue_planned<false, uint32_t>();
+    else rc = ctx->ev64 ? run.enqueue_planned<true, uint64_t>() : run.enqueue_planned<true, uint32_t>();
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return ctx->fail(KC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->gexec = nullptr;
+        return ctx->fail(KC_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    ctx->graph_ok = true;
+    return KC_OK;
+}
+
+int kc_graph_run(kc_ctx* ctx, kc_info* info) {
+    if (!ctx) return KC_EINVAL;
+    if (ctx->libs.empty()) return ctx->fail(KC_ESTATE, "no reads uploaded");
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    if (!ctx->graph_ok || !ctx->gexec) return graph_plan_and_capture(ctx, info);
+    ctx->reset_marks();
+    hipEvent_t e0 = ctx->new_event(), e1 = ctx->new_event();
+    CTX_TRY(hipEventRecord(e0, ctx->st));
+    CTX_TRY(hipGraphLaunch(ctx->gexec, ctx->st));
+    CTX_TRY(hipEventRecord(e1, ctx->st));
+    const uint32_t L = (uint32_t)ctx->libs.size();
+    ctx->h_out.resize(L);
+    CTX_TRY(hipMemcpyAsync(ctx->h_out.data(), ctx->d_out.p, L * sizeof(LibOut), hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipMemcpyAsync(ctx->h_stage, ctx->d_counters.p, 16 * 8, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    // the plan holds when no table overflowed, no read was too long for Hammer and the
+    // compaction produced exactly the planned number of entries; otherwise plan again
+    bool ok = ctx->h_stage[4] == ctx->g_entries;
+    if (ctx->o.error_correct && (ctx->h_stage[3] & 0xFFFFFFFFull)) ok = false;
+    for (uint32_t l = 0; l < L; ++l) ok = ok && !ctx->h_out[l].overflow;
+    if (!ok) return graph_plan_and_capture(ctx, info);
+    float ms = 0;
+    CTX_TRY(hipEventElapsedTime(&ms, e0, e1));
+    uint64_t e = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        auto& lib = ctx->libs[l];
+        const LibOut& o = ctx->h_out[l];
+        lib.info.ms_stage2 = ms;  // the whole replay
+        lib.info.ms_stage3 = 0;
+        lib.info.n_distinct = o.n_distinct;
+        lib.info.n_counts = o.n_counts;
+        lib.info.n_solid = o.n_solid;
+        lib.info.cutoff = o.cutoff;
+        lib.info.top_cov = o.top;
+        lib.info.corrected_reads = o.corrected;
+        lib.entry0 = e;
+        e += o.n_counts;
+    }
+    if (e != ctx->n_entries) return ctx->fail(KC_EHIP, "per-library counts do not add up to the entries");
+    ctx->built = ctx->counted = true;
+    if (info) *info = ctx->libs[0].info;
+    return KC_OK;
+}
+
+int kc_graph_release(kc_ctx* ctx) {
+    if (!ctx) return KC_EINVAL;
+    ctx->graph_release();
+    return KC_OK;
 }
 
 int kc_library_info(const kc_ctx* ctx, uint32_t lib, kc_info* info) {
@@ -1523,6 +2022,89 @@ int kc_library_reads_export(kc_ctx* ctx, uint32_t lib, uint8_t* data, size_t dat
 
 int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len) {
     return kc_library_reads_export(ctx, 0, data, data_len);
+}
+
+int kc_tune_get(const kc_ctx* ctx, kc_tune* tune) {
+    if (!ctx || !tune) return KC_EINVAL;
+    *tune = ctx->tune;
+    return KC_OK;
+}
+
+int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
+    if (!ctx || !tune) return KC_EINVAL;
+    const uint32_t w = tune->windows_per_thread;
+    if (w != 1 && w != 2 && w != 4) return ctx->fail(KC_EINVAL, "windows_per_thread must be 1, 2 or 4");
+    if (tune->blocks_per_cu > 1024) return ctx->fail(KC_EINVAL, "blocks_per_cu must be at most 1024");
+    if (tune->table_load != 0.0f && !(tune->table_load > 0.05f && tune->table_load < 0.95f))
+        return ctx->fail(KC_EINVAL, "table_load must be 0 or in (0.05, 0.95)");
+    ctx->tune = *tune;
+    ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
+    return KC_OK;
+}
+
+int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
+    if (!ctx) return KC_EINVAL;
+    if (ctx->libs.empty()) return ctx->fail(KC_ESTATE, "no reads uploaded");
+    reps = std::max<uint32_t>(reps, 1);
+    std::vector<kc_info> ref;  // per-library summary every setting must reproduce
+    auto same = [&](const kc_info& a, const kc_info& b) {
+        return a.n_occ == b.n_occ && a.n_distinct == b.n_distinct && a.n_counts == b.n_counts &&
+               a.n_solid == b.n_solid && a.cutoff == b.cutoff && a.corrected_reads == b.corrected_reads;
+    };
+    auto trial = [&](const kc_tune& t, double* ms) -> int {
+        int rc = kc_tune_set(ctx, &t);
+        if (rc) return rc;
+        double m = 1e300;
+        for (uint32_t r = 0; r <= reps; ++r) {  // run 0 is the warm-up
+            kc_info info{};
+            rc = kc_run(ctx, &info);
+            if (rc) return rc;
+            if (r) m = std::min(m, info.ms_stage2 + info.ms_stage3);
+        }
+        for (size_t l = 0; l < ctx->libs.size(); ++l) {
+            if (ref.size() < ctx->libs.size()) ref.push_back(ctx->libs[l].info);
+            else if (!same(ref[l], ctx->libs[l].info))
+                return ctx->fail(KC_EHIP, "autotune: a setting changed the results of library " + std::to_string(l));
+        }
+        *ms = m;
+        return KC_OK;
+    };
+    kc_tune cur = ctx->tune;
+    if (cur.table_load == 0.0f) cur.table_load = (ctx->o.table_load > 0.05f && ctx->o.table_load < 0.95f)
+                                                     ? ctx->o.table_load : 0.5f;
+    double cur_ms = 0;
+    int rc = trial(cur, &cur_ms);
+    if (rc) return rc;
+    // coordinate search: load factor, then windows in flight, then workgroups per CU
+    const float loads[] = {0.35f, 0.5f, 0.65f, 0.8f};
+    const uint32_t wpts[] = {1, 2, 4};
+    const uint32_t bpcs[] = {0, 4, 8, 16, 32};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int n = dim == 0 ? 4 : (dim == 1 ? 3 : 5);
+        kc_tune base = cur;
+        for (int i = 0; i < n; ++i) {
+            kc_tune t = base;
+            if (dim == 0) t.table_load = loads[i];
+            else if (dim == 1) t.windows_per_thread = wpts[i];
+            else t.blocks_per_cu = bpcs[i];
+            if (t.table_load == base.table_load && t.windows_per_thread == base.windows_per_thread &&
+                t.blocks_per_cu == base.blocks_per_cu)
+                continue;
+            double ms = 0;
+            rc = trial(t, &ms);
+            if (rc) return rc;
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    cur.ms = cur_ms;
+    rc = kc_tune_set(ctx, &cur);
+    if (rc) return rc;
+    ctx->tune.ms = cur_ms;
+    if (best) *best = ctx->tune;
+    return KC_OK;
 }
 
 int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap) {

@@ -29,7 +29,8 @@ EXPORTS = (
     "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
     "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
-    "kc_library_counts_export", "kc_library_reads_export",
+    "kc_library_counts_export", "kc_library_reads_export", "kc_library_priors_set", "kc_priors_set",
+    "kc_tune_get", "kc_tune_set", "kc_autotune", "kc_graph_run", "kc_graph_release",
 )
 
 
@@ -53,6 +54,14 @@ class kc_info(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class kc_tune(ctypes.Structure):
+    _fields_ = [("windows_per_thread", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("table_load", ctypes.c_float), ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 _vp = ctypes.c_void_p
@@ -92,6 +101,13 @@ _SIGS = {
     "kc_library_bloom_query": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _sz, _u8p]),
     "kc_library_counts_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u32p, _u8p, _sz]),
     "kc_library_reads_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz]),
+    "kc_library_priors_set": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz, _u64p, _u64p, _sz]),
+    "kc_priors_set": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u64p, _sz]),
+    "kc_tune_get": (ctypes.c_int, [_vp, ctypes.POINTER(kc_tune)]),
+    "kc_tune_set": (ctypes.c_int, [_vp, ctypes.POINTER(kc_tune)]),
+    "kc_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_tune)]),
+    "kc_graph_run": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
+    "kc_graph_release": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

@@ -12,6 +12,7 @@ reference's Kmer256(lo: u128, hi: u128) (dna.rs:137-138).
 """
 import ctypes
 from dataclasses import dataclass
+from typing import List, Optional
 
 import numpy as np
 
@@ -134,11 +135,30 @@ class ErrorCorrector:
 
 @dataclass
 class AssemblerConfig:
-    """The AssemblerConfig fields read by Stage 2-3 (assemble.rs:35-103 defaults)."""
+    """The AssemblerConfig fields read by Stage 2-3 (assemble.rs:35-103 defaults).
+
+    prior_contigs: sequences (str or bytes) whose k-mers get the prior boost
+    (assemble.rs:244-284); trusted_contigs: FASTA/FASTQ paths whose sequences are added
+    to them (parse_reads_from_file, fastq.rs:156)."""
     k: int = 31
     min_coverage: float = 5.0
     bloom_bits: int = 512 * 1024 * 1024
     error_correct: bool = False
+    prior_contigs: Optional[List] = None
+    trusted_contigs: Optional[List[str]] = None
+
+    def all_priors(self):
+        """assemble.rs:244-264: prior contigs, then every trusted file's sequences (a file
+        that fails to parse is skipped with a warning, as the reference does)."""
+        from .packed_reads import read_sequences
+        out = [p.encode() if isinstance(p, str) else bytes(p) for p in (self.prior_contigs or [])]
+        for path in self.trusted_contigs or []:
+            try:
+                out.extend(read_sequences(path))
+            except (OSError, ValueError) as e:
+                import sys
+                print(f"  [Warning] Failed to parse trusted contigs from {path!r}: {e}", file=sys.stderr)
+        return out
 
 
 @dataclass
@@ -210,6 +230,18 @@ class SpectrumEngine:
         self._data_len.append(data.shape[0])
         return idx.value
 
+    def set_priors(self, seqs, library=0):
+        """Prior / trusted contig sequences of a library (assemble.rs:244-284); applied by the
+        next run().  An empty list clears them."""
+        bs = [q.encode() if isinstance(q, str) else bytes(q) for q in seqs]
+        lens = np.array([len(b) for b in bs], dtype=np.uint64)
+        starts = np.zeros(len(bs), dtype=np.uint64)
+        if len(bs) > 1:
+            starts[1:] = np.cumsum(lens)[:-1]
+        buf = np.frombuffer(b"".join(bs) + b"\0", dtype=np.uint8).copy()
+        self._chk(lib().kc_library_priors_set(self._h, library, ptr(buf, u8p), int(lens.sum()), ptr(starts, u64p),
+                                              ptr(lens, u64p), len(bs)))
+
     def n_libraries(self):
         n = ctypes.c_uint32(0)
         self._chk(lib().kc_library_count(self._h, ctypes.byref(n)))
@@ -227,6 +259,33 @@ class SpectrumEngine:
         info = _abi.kc_info()
         self._chk(lib().kc_run(self._h, ctypes.byref(info)))
         return info.as_dict()
+
+    def graph_run(self):
+        """Stage 2 + 3 as one HIP graph launch (captured on the first call; re-planned
+        eagerly whenever a data-dependent size no longer matches).  Same results as run()."""
+        info = _abi.kc_info()
+        self._chk(lib().kc_graph_run(self._h, ctypes.byref(info)))
+        return info.as_dict()
+
+    def graph_release(self):
+        self._chk(lib().kc_graph_release(self._h))
+
+    def tune(self):
+        t = _abi.kc_tune()
+        self._chk(lib().kc_tune_get(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, windows_per_thread=4, blocks_per_cu=0, table_load=0.0):
+        t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
+                         table_load=float(table_load))
+        self._chk(lib().kc_tune_set(self._h, ctypes.byref(t)))
+
+    def autotune(self, reps=2):
+        """Coordinate search over load factor, windows per thread and workgroups per CU on
+        the uploaded libraries; keeps and returns the fastest setting."""
+        t = _abi.kc_tune()
+        self._chk(lib().kc_autotune(self._h, int(reps), ctypes.byref(t)))
+        return t.as_dict()
 
     def info(self, library=0):
         info = _abi.kc_info()
@@ -274,6 +333,9 @@ def count_kmers(packed, config, device=0):
     eng = SpectrumEngine(config, device=device)
     try:
         eng.upload(packed)
+        priors = config.all_priors()
+        if priors:
+            eng.set_priors(priors)
         info = eng.run()
         seen, solid = eng.bloom()
         keys, cnt, mask = eng.counts()

@@ -282,3 +282,33 @@ def read_fastq_sequences(path):
         seqs.append(seq)
         i += 4
     return seqs
+
+
+def read_sequences(path):
+    """parse_reads_from_file (fastq.rs:156-358, no quality filter): the format is taken from
+    the first non-whitespace byte ('@' FASTQ, '>' FASTA, anything else is an error).  FASTA
+    records are the concatenation of their lines (CR/LF stripped); empty records and empty
+    FASTQ sequences are dropped."""
+    opener = gzip.open if str(path).lower().endswith(".gz") else open
+    with opener(path, "rb") as fh:
+        raw = fh.read()
+    body = raw.lstrip()
+    if not body:
+        return []
+    if body[:1] == b"@":
+        return [s for s in read_fastq_sequences(path) if s]
+    if body[:1] != b">":
+        raise ValueError(f"Malformed FASTQ/FASTA file {path!r}: expected '@' or '>' header, found {body[:1]!r}")
+    seqs, cur = [], []
+    lines = body.split(b"\n")
+    for line in lines[1:]:  # the rest of the first header line is discarded
+        line = line.rstrip(b"\r")
+        if line.startswith(b">"):
+            if cur:
+                seqs.append(b"".join(cur))
+            cur = []
+        else:
+            cur.append(line)
+    if cur and b"".join(cur):
+        seqs.append(b"".join(cur))
+    return [q for q in seqs if q]

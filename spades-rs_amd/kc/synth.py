@@ -12,7 +12,7 @@ then all R2 mates (packed_reads.rs:481-489).
 """
 import numpy as np
 
-from .packed_reads import PackedReads, pack_fixed_length
+from .packed_reads import PackedReads, pack_codes, pack_fixed_length
 
 
 def random_genome(rng, size, gc=0.5):
@@ -92,3 +92,51 @@ def workload_5mb_100x(seed=2, scale=1.0):
     rising linearly 0.1% -> 1.0% along each read with matching Phred qualities."""
     g = max(int(5_000_000 * scale), 2000)
     return uniform_genome_pairs(seed, g, max(int(1_666_667 * scale), 1), p_start=0.001, p_end=0.010)
+
+
+def uneven_pairs(rng, genome, n_pairs, median_len=150.0, sigma=0.25, min_len=50, max_len=250, p_err=0.003,
+                 insert_sd=30.0, chunk=1 << 16):
+    """Paired reads whose lengths are lognormal(median_len, sigma) truncated to
+    [min_len, max_len], from fragments ~ N(2.2 * median_len, insert_sd) (at least max_len),
+    uniform substitutions at p_err.  R1 mates then R2 mates, packed contiguously."""
+    G = genome.shape[0]
+    ar = np.arange(max_len, dtype=np.int64)
+    perr = np.full(max_len, p_err)
+    parts, lens1, lens2 = [[], []], [], []
+    for c0 in range(0, n_pairs, chunk):
+        m = min(chunk, n_pairs - c0)
+        L = np.clip(np.rint(median_len * np.exp(sigma * rng.standard_normal((2, m)))), min_len, max_len).astype(np.int64)
+        frag = np.clip(np.rint(rng.normal(2.2 * median_len, insert_sd, m)).astype(np.int64), max_len, G)
+        start = (rng.random(m) * (G - frag + 1)).astype(np.int64)
+        r1 = _mutate(rng, genome[start[:, None] + ar[None, :]], perr)
+        tail = start + frag - max_len
+        r2 = _mutate(rng, 3 - genome[tail[:, None] + ar[None, ::-1]], perr)
+        # R2 reads the fragment from its end: keep its first L bases as well
+        for side, (rr, LL) in enumerate(((r1, L[0]), (r2, L[1]))):
+            keep = ar[None, :] < LL[:, None]
+            parts[side].append((rr[keep], LL))
+        lens1.append(L[0])
+        lens2.append(L[1])
+    codes = [np.concatenate([c for c, _ in parts[s]]) if parts[s] else np.zeros(0, np.uint8) for s in (0, 1)]
+    lengths = np.concatenate([np.concatenate(lens1) if lens1 else np.zeros(0, np.int64),
+                              np.concatenate(lens2) if lens2 else np.zeros(0, np.int64)])
+    data, offsets = pack_codes(np.concatenate(codes), lengths)
+    return PackedReads.from_arrays(data, offsets, lengths.astype(np.uint32), pe_count=n_pairs)
+
+
+def workload_64_libraries(seed=3, scale=1.0, n_libs=64):
+    """Many independent libraries: genome sizes log-uniform in [5 kb, 6 Mb], coverage
+    uniform in [20x, 200x], read lengths lognormal (median 150) truncated to [50, 250],
+    0.3% substitutions; `scale` multiplies every genome size (coverage is kept).
+    Returns a list of (PackedReads, meta)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.exp(rng.uniform(np.log(5_000), np.log(6_000_000), n_libs))
+    covs = rng.uniform(20.0, 200.0, n_libs)
+    out = []
+    for i in range(n_libs):
+        g = max(int(sizes[i] * scale), 600)
+        genome = random_genome(rng, g, 0.5)
+        n_pairs = max(int(round(covs[i] * g / (2 * 155.0))), 1)
+        store = uneven_pairs(rng, genome, n_pairs)
+        out.append((store, {"genome_size": g, "coverage": float(covs[i]), "n_pairs": n_pairs}))
+    return out

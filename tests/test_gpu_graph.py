@@ -1,0 +1,126 @@
+"""GPU: tuning settings, the autotuner and HIP-graph replay give results identical to the
+oracle (and to the eager run): every launch shape and load factor, several libraries of
+uneven read lengths with priors, Hammer on and off.  A graph is replayed several times
+with eager exports and synchronisations between replays, and every replay is checked."""
+import sys
+
+import numpy as np
+import pytest
+
+from kc import AssemblerConfig, PackedReads, SpectrumEngine, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _uneven_library(seed, genome, pairs, min_len=50, max_len=250, err=0.003):
+    rng = np.random.default_rng(seed)
+    st, meta = synth.uniform_genome_pairs(seed=seed, genome_size=genome, n_pairs=pairs, read_len=max_len,
+                                          insert_mean=max_len * 2, insert_sd=20, p_start=err)
+    out = PackedReads()
+    for i in range(len(st)):
+        r = st.get_read(i)
+        out.pack_clean(r[: int(rng.integers(min_len, max_len + 1))])
+    return out, "".join("ACGT"[int(x)] for x in meta["genome"])
+
+
+def _snapshot(eng, n):
+    return [(eng.info(i), eng.bloom(i), eng.counts(i)) for i in range(n)]
+
+
+def _same(a, b):
+    for (ia, ba, ca), (ib, bb, cb) in zip(a, b):
+        for key in ("n_occ", "n_distinct", "n_counts", "n_solid", "cutoff", "top_cov", "corrected_reads"):
+            assert ia[key] == ib[key], key
+        assert all(np.array_equal(x, y) for x, y in zip(ba, bb))
+        assert all(np.array_equal(x, y) for x, y in zip(ca, cb))
+
+
+def _oracle_check(oracle, eng, libs, k, bits, ec, mcs, priors):
+    for i, st in enumerate(libs):
+        d, o, l = st.arrays()
+        S = oracle.Spectrum(d, o, l, k, bits, ec, mcs[i], 1, priors=priors.get(i))
+        keys, cnt, mask = eng.counts(i)
+        okeys, ocnt, omask = S.counts()
+        seen, solid = eng.bloom(i)
+        oseen, osolid = S.bloom()
+        assert np.array_equal(seen, oseen) and np.array_equal(solid, osolid)
+        assert np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt) and np.array_equal(mask, omask)
+        assert eng.info(i)["cutoff"] == S.cutoff
+
+
+@pytest.mark.parametrize("k,ec", [(33, False), (21, True), (55, True)])
+def test_graph_replays_match_eager_and_oracle(oracle, k, ec):
+    libs, genomes = zip(*[_uneven_library(700 + i, 2000 + 1700 * i, 120 + 60 * i) for i in range(4)])
+    libs = list(libs)
+    bits = 1 << 17
+    mcs = [5.0, 5.0, 14.0, 5.0]
+    priors = {1: [genomes[1][50:700], genomes[1][900:1000].lower()]}
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits, error_correct=ec))
+    for i, st in enumerate(libs):
+        eng.add_library(st, bloom_bits=bits, min_coverage=mcs[i])
+    eng.set_priors(priors[1], library=1)
+    say(f"case graph k={k} ec={ec}")
+    eng.run()
+    eager = _snapshot(eng, len(libs))
+    _oracle_check(oracle, eng, libs, k, bits, ec, mcs, priors)
+    for rep in range(4):  # call 0 plans and captures, calls 1.. replay
+        say(f"  graph call {rep}")
+        info = eng.graph_run()
+        assert info["n_occ"] == eager[0][0]["n_occ"]
+        _same(_snapshot(eng, len(libs)), eager)
+    # an eager run in between drops the graph; the next call plans again
+    eng.run()
+    _same(_snapshot(eng, len(libs)), eager)
+    eng.graph_run()
+    eng.graph_run()
+    _same(_snapshot(eng, len(libs)), eager)
+    eng.close()
+
+
+@pytest.mark.parametrize("wpt,bpc,load", [(1, 0, 0.5), (2, 8, 0.35), (4, 16, 0.8), (4, 1, 0.65), (2, 0, 0.9)])
+def test_tune_settings_are_exact(oracle, wpt, bpc, load):
+    libs, genomes = zip(*[_uneven_library(40 + i, 3000 + 900 * i, 200, err=0.006) for i in range(3)])
+    libs = list(libs)
+    k, bits = 31, (1 << 16) + 64
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits))
+    for st in libs:
+        eng.add_library(st, bloom_bits=bits)
+    eng.set_tune(windows_per_thread=wpt, blocks_per_cu=bpc, table_load=load)
+    assert eng.tune()["windows_per_thread"] == wpt
+    say(f"case tune wpt={wpt} bpc={bpc} load={load}")
+    eng.run()
+    _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * 3, {})
+    eng.close()
+
+
+def test_autotune_keeps_results(oracle):
+    libs, _ = zip(*[_uneven_library(90 + i, 4000, 300) for i in range(2)])
+    libs = list(libs)
+    k, bits = 33, 1 << 18
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits, error_correct=True))
+    for st in libs:
+        eng.add_library(st, bloom_bits=bits)
+    say("case autotune")
+    best = eng.autotune(reps=1)
+    assert best["windows_per_thread"] in (1, 2, 4) and best["ms"] > 0
+    assert eng.tune() == best
+    eng.run()
+    _oracle_check(oracle, eng, libs, k, bits, True, [5.0] * 2, {})
+    eng.graph_run()
+    eng.graph_run()
+    _oracle_check(oracle, eng, libs, k, bits, True, [5.0] * 2, {})
+    eng.close()
+
+
+def test_tune_rejects_bad_settings():
+    from kc import KcError
+    eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=1 << 12))
+    for bad in ({"windows_per_thread": 3}, {"table_load": 0.99}, {"blocks_per_cu": 5000}):
+        with pytest.raises(KcError):
+            eng.set_tune(**bad)
+    eng.close()

@@ -55,6 +55,10 @@ def test_cutoff_known_answers(oracle):
     assert oracle.cutoff([1, 2, 3], 10.0)[0] == 3
     assert oracle.cutoff([1, 2, 3], 5.5)[0] == 2
     assert oracle.cutoff([], 20.0)[0] == 7
+    # `as u32` saturates: 1e20 * 0.35 -> u32::MAX; +inf likewise
+    assert oracle.cutoff([1, 2], 1e20)[0] == 0xFFFFFFFF
+    assert oracle.cutoff([1, 2], float("inf"))[0] == 0xFFFFFFFF
+    assert oracle.cutoff([1, 2], 12345678.9)[0] == 4320987  # trunc(4320987.615)
     # <= 100 entries -> 2
     assert oracle.cutoff([50] * 100, 5.0)[0] == 2
     # top (95th percentile) < 30 -> 2

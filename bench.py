@@ -42,8 +42,9 @@ def _pmc_rows(path, counter):
 
 def traffic_from(fetch_csv, write_csv, kernel_ms, kernel="k_insert<"):
     """HBM bytes per launch of `kernel` from two rocprofv3 --pmc passes (FETCH_SIZE, then
-    WRITE_SIZE, KiB per dispatch) of `bench.py --ks K --steps 1 --warmup 0 --copy-calibration`.
-    Each counter is scaled by what it reads for the 1 GiB calibration copy."""
+    WRITE_SIZE, KiB per dispatch) of `bench.py --ks K --steps 1 --warmup 0 --copy-calibration`:
+    the last dispatch of the kernel (the timed step, after any autotuning).  Each counter
+    is scaled by what it reads for the 1 GiB calibration copy."""
     out = {}
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_rows(path, name)
@@ -52,18 +53,43 @@ def traffic_from(fetch_csv, write_csv, kernel_ms, kernel="k_insert<"):
         if not copy or not kern:
             raise SystemExit(f"{path}: no calibration copy or no {kernel} dispatch")
         scale = (COPY_BYTES / 1024.0) / max(copy)  # the copy reads and writes 1 GiB once
-        out[name] = {"kib_per_dispatch": sum(kern) / len(kern), "dispatches": len(kern), "copy_kib": max(copy),
+        out[name] = {"kib_per_dispatch": kern[-1], "dispatches": len(kern), "copy_kib": max(copy),
                      "scale": round(scale, 4)}
     b = sum(v["kib_per_dispatch"] * 1024 * v["scale"] for v in out.values())
     return {"kernel": kernel, "bytes_per_launch": int(b), "kernel_ms": kernel_ms, "counters": out}
 
 
+def atomics_from(csv_path, kernel_ms, kernel="k_insert<"):
+    """Atomic requests of the last dispatch of `kernel` from a rocprofv3 --pmc pass over
+    TCC_ATOMIC_sum and TCC_EA0_ATOMIC_sum (L2 atomics, and those forwarded to the fabric)."""
+    import csv
+    vals = {}
+    for r in csv.DictReader(open(csv_path)):
+        if kernel in r["Kernel_Name"]:
+            vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    rec = {c: v[-1] for c, v in vals.items()}
+    rec["per_s"] = {c: round(v / (kernel_ms / 1e3), 1) for c, v in rec.items() if not isinstance(v, dict)}
+    rec["kernel_ms"] = kernel_ms
+    return rec
+
+
+def tune_key(k, tune):
+    return f"{k}:w{tune['windows_per_thread']}:b{tune['blocks_per_cu']}:l{tune['table_load']:.2f}"
+
+
 def main():
-    if len(sys.argv) > 1 and sys.argv[1] == "--traffic-from":
-        # python bench.py --traffic-from K FETCH.csv WRITE.csv KERNEL_MS  -> entry K of TRAFFIC_JSON
-        rec = traffic_from(sys.argv[3], sys.argv[4], float(sys.argv[5]))
+    if len(sys.argv) > 1 and sys.argv[1] in ("--traffic-from", "--atomics-from"):
+        # python bench.py --traffic-from KEY FETCH.csv WRITE.csv KERNEL_MS
+        # python bench.py --atomics-from KEY ATOMIC.csv KERNEL_MS
+        # KEY = tune_key(k, tune) as printed by bench.py ("tune_keys")
         allrec = json.load(open(TRAFFIC_JSON)) if os.path.exists(TRAFFIC_JSON) else {}
-        allrec[str(int(sys.argv[2]))] = rec
+        key = sys.argv[2]
+        rec = allrec.get(key, {})
+        if sys.argv[1] == "--traffic-from":
+            rec.update(traffic_from(sys.argv[3], sys.argv[4], float(sys.argv[5])))
+        else:
+            rec["atomics"] = atomics_from(sys.argv[3], float(sys.argv[4]))
+        allrec[key] = rec
         with open(TRAFFIC_JSON, "w") as f:
             json.dump(allrec, f, indent=1, sort_keys=True)
         print(json.dumps(rec))
@@ -75,7 +101,10 @@ def main():
     ap.add_argument("--workload", default="5mb_100x", choices=["5mb_100x", "1mb_30x", "64lib"])
     ap.add_argument("--scale", type=float, default=0.1,
                     help="64lib: genome-size scale (1.0 = 5 kb-6 Mb genomes, ~5.4 Gb of reads)")
-    ap.add_argument("--autotune", action="store_true", help="64lib: autotune before timing")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner (it runs before the timed region by default)")
+    ap.add_argument("--no-microbench", dest="microbench", action="store_false",
+                    help="skip the random-atomic / copy microbenchmarks")
     ap.add_argument("--eager", action="store_true", help="64lib: time eager runs instead of graph replays")
     ap.add_argument("--ks", default="21,33,55")
     ap.add_argument("--bloom-bits", type=int, default=512 * 1024 * 1024)
@@ -121,6 +150,8 @@ def main():
         eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), table_load=args.table_load,
                              timing=True)
         eng.upload(store)
+        if args.autotune:
+            log(f"k={k} tune: {eng.autotune(reps=1)}")
         engines.append(eng)
     log("uploaded")
 
@@ -174,12 +205,16 @@ def main():
         # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
         # used only while the recorded launch time still matches this kernel's
         rec = None
+        dom_tune = engines[ks.index(dom_k)].tune()
+        roof["tune_key"] = tune_key(dom_k, dom_tune)
         if args.workload == "5mb_100x" and os.path.exists(TRAFFIC_JSON):
-            rec = json.load(open(TRAFFIC_JSON)).get(str(dom_k))
-        if rec:
+            rec = json.load(open(TRAFFIC_JSON)).get(roof["tune_key"])
+        if rec and "bytes_per_launch" in rec:
             if abs(rec["kernel_ms"] - dom_ms / dom_n) <= 0.1 * rec["kernel_ms"]:
                 roof["traffic"] = rec["bytes_per_launch"]
                 roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
+        if rec and "atomics" in rec:
+            roof["atomic_requests_per_s"] = rec["atomics"]["per_s"]
     # whole-path roofline with the design-independent byte count of the path
     path_b = 0
     for k in ks:
@@ -187,6 +222,11 @@ def main():
         path_b += stream_bytes(k) + inf["n_occ"] * G_BYTES + inf["n_distinct"] * 6 * G_BYTES + inf["n_counts"] * 20
     path = {"bytes_per_step": path_b, "achieved_GBs": round(path_b / (ms_per_step / 1e3) / 1e9, 1),
             "frac": round(path_b / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    micro = None
+    if args.microbench:
+        micro = microbenchmarks()
+        log(f"microbenchmarks: {micro}")
 
     cpu = None
     if not args.no_cpu:
@@ -217,12 +257,44 @@ def main():
                                                       "ms": round(per_k[k]["ms_stage2"] + per_k[k]["ms_stage3"], 3)}
                                              for k in ks}},
         "roofline": roof, "path_roofline": path, "cpu_baseline": cpu,
+        "tune": {str(k): e.tune() for k, e in zip(ks, engines)},
+        "tune_keys": [tune_key(k, e.tune()) for k, e in zip(ks, engines)],
+        "microbench": micro,
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     for e in engines:
         e.close()
     print(json.dumps(out), flush=True)
+
+
+def microbenchmarks(device=0):
+    """Measured ceilings (libkc_bench.so): random non-returning atomics (u32 add, u64 or)
+    and random 8-byte reads over a 2 GiB array (the size of a 2^29-entry u32 array), and a
+    4 GiB streaming copy."""
+    import ctypes
+    path = os.path.join(PKG, "libkc_bench.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.kc_bench_device.restype = ctypes.c_int
+    L.kc_bench_device.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    n_ops = 1 << 29
+    for name, kind, nbytes in (("atomic_add_u32", 0, 2 << 30), ("atomic_or_u64", 1, 2 << 30),
+                               ("read_64b", 2, 2 << 30), ("copy", 3, 4 << 30)):
+        ms = ctypes.c_double(0)
+        rc = L.kc_bench_device(device, kind, nbytes, n_ops, 3, ctypes.byref(ms))
+        if rc != 0:
+            out[name] = {"error": rc}
+            continue
+        if kind == 3:
+            out[name] = {"GB_s": round(2 * nbytes / (ms.value / 1e3) / 1e9, 1), "ms": round(ms.value, 3)}
+        else:
+            out[name] = {"G_ops_s": round(n_ops / (ms.value / 1e3) / 1e9, 2), "ms": round(ms.value, 3),
+                         "array_bytes": nbytes}
+    return out
 
 
 def bench_many_libraries(args):
@@ -238,7 +310,7 @@ def bench_many_libraries(args):
     reads = sum(len(st) for st, _ in libs)
     bases = sum(int(st.lengths.astype(np.int64).sum()) for st, _ in libs)
     log(f"generated {len(libs)} libraries, {reads} reads, {bases} bases in {time.time() - t0:.1f}s")
-    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), timing=False)
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), timing=args.eager)
     for st, _ in libs:
         eng.add_library(st)
     tune = eng.autotune(reps=1) if args.autotune else eng.tune()
@@ -255,6 +327,9 @@ def bench_many_libraries(args):
     ms = (time.perf_counter() - t_start) * 1e3 / args.steps
     n_occ = sum(eng.info(i)["n_occ"] for i in range(len(libs)))
     n_counts = sum(eng.info(i)["n_counts"] for i in range(len(libs)))
+    if args.eager:
+        for name, (kms, n) in sorted(eng.kernel_times().items(), key=lambda x: -x[1][0])[:12]:
+            log(f"  {name:18s} {kms / max(n, 1):9.3f} ms/launch x{n}")
     eng.close()
     out = {
         "metric": "k-mers/s counted bit-exact (Bloom tiers + sorted count table + cutoff + solid set)",

@@ -109,6 +109,11 @@ const char* kc_last_error(const kc_ctx* ctx);
 int kc_reads_upload(kc_ctx* ctx, const uint8_t* data, size_t data_len, const uint64_t* offsets,
                     const uint32_t* lengths, size_t n_reads);
 
+/* Change k for the next run, keeping every library's reads resident on the device: the
+ * multi-k loop (multik.rs:109-178) runs k = 21, 33, 55 ... over one upload.  Results of
+ * the previous k are dropped. */
+int kc_set_k(kc_ctx* ctx, uint32_t k);
+
 /* Stage 2: stream every window of every read into the two-tier filter (assemble.rs:129-161). */
 int kc_bloom_build(kc_ctx* ctx);
 /* Stage 3: optional Hammer (error_correct), exact counts of Bloom-solid k-mers, sort,
@@ -194,6 +199,47 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best);
  * holds the whole replay and ms_stage3 is 0. */
 int kc_graph_run(kc_ctx* ctx, kc_info* info);
 int kc_graph_release(kc_ctx* ctx);
+
+/* ---- Learned first pass: the network ----------------------------------------
+ * A 16 -> 64 -> 64 -> 1 MLP (ReLU, ReLU, sigmoid) that predicts a k-mer's solid label
+ * from 16 features.  bf16 weights and inputs, fp32 accumulation and biases; inference
+ * on MFMA.  Parameters are kc_mlp_num_params() floats: W1[16][64], b1[64], W2[64][64],
+ * b2[64], w3[64], b3.  No counterpart in the reference. */
+typedef struct kc_mlp kc_mlp;
+int kc_mlp_create(kc_mlp** out, int32_t device, uint64_t seed); /* seeded He initialisation */
+void kc_mlp_destroy(kc_mlp* m);
+const char* kc_mlp_last_error(const kc_mlp* m);
+uint32_t kc_mlp_num_params(void);
+int kc_mlp_get_params(kc_mlp* m, float* params, size_t n);
+int kc_mlp_set_params(kc_mlp* m, const float* params, size_t n);
+/* p[i] = sigmoid(network(X[i])), X row-major n x 16. */
+int kc_mlp_forward(kc_mlp* m, const float* X, size_t n, float* p);
+/* Gradient of mean_i w_i * BCE(p_i, y_i) with w_i = sample_w[i] (1 when NULL) times
+ * pos_weight for y_i = 1, with respect to the parameters (straight through the bf16
+ * roundings); loss (nullable) receives the loss. */
+int kc_mlp_gradient(kc_mlp* m, const float* X, const uint8_t* y, const float* sample_w, size_t n, float pos_weight,
+                    float* grad, double* loss);
+/* Adam (0.9, 0.999, 1e-8) over `epochs` shuffled passes of mini-batches; deterministic
+ * for a given seed.  last_loss (nullable) receives the last batch's loss. */
+int kc_mlp_train(kc_mlp* m, const float* X, const uint8_t* y, const float* sample_w, size_t n, uint32_t epochs,
+                 uint32_t batch, float lr, float pos_weight, uint64_t seed, double* last_loss);
+
+/* ---- Learned first pass: features and selection -------------------------------
+ * Need kc_count with error_correct off (the table then holds the uncorrected counts). */
+
+/* Per-base Phred qualities of a library's reads (one byte per base; read i's bases at
+ * quals[qoffsets[i] .. + length_i]); copied.  n_quals = 0 removes them. */
+int kc_library_quals_set(kc_ctx* ctx, uint32_t lib, const uint8_t* quals, size_t n_quals, const uint64_t* qoffsets);
+/* 16 features per count-table entry of the library, in the entries' sorted order: its
+ * count, the mean quality of its occurrences (0 without qualities), and statistics of the
+ * counts of its 3k Hamming-1 neighbours, relative to the library's cutoff; n must equal
+ * kc_info.n_counts. */
+int kc_library_features(kc_ctx* ctx, uint32_t lib, float* out, size_t n);
+/* Features -> network -> label per entry: 1 (solid) when p >= 1 - margin, 0 when
+ * p <= margin, else the exact rule count >= cutoff, and fallback[i] = 1 there.  Either
+ * output may be NULL; *ms (nullable) receives the device time of the whole pass. */
+int kc_library_learned_select(kc_ctx* ctx, uint32_t lib, kc_mlp* mlp, float margin, uint8_t* labels,
+                              uint8_t* fallback, size_t n, double* ms);
 
 /* Per-kernel device time of the last run (KC_FLAG_TIMING): fills up to cap entries of
  * names (static strings), total ms and launch counts; returns the number of kernels. */

@@ -294,6 +294,29 @@ __device__ __forceinline__ uint64_t table_find_or_claim(Slot<NW, Ev>* __restrict
     return ~0ull;
 }
 
+// Read-only lookup: the slot holding key, or ~0 when absent.  A key lives in the first
+// bucket from its home that had a free slot when it was inserted, and slots are never
+// freed, so a bucket with an EMPTY slot ends the search.
+template <int NW, typename Ev>
+__device__ __forceinline__ uint64_t table_find(const Slot<NW, Ev>* __restrict__ tab, uint64_t b0, uint64_t nbuckets,
+                                               uint64_t b, const uint64_t key[NW]) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    for (uint64_t probe = 0; probe < nbuckets; ++probe) {
+#pragma unroll
+        for (uint32_t s = 0; s < SPB; ++s) {
+            const Slot<NW, Ev>* sl = tab + b * SPB + s;
+            const uint64_t w0 = sl->k[0];
+            if (w0 == EMPTY_KEY) return ~0ull;
+            bool match = w0 == key[0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) match = match && sl->k[w] == key[w];
+            if (match) return b * SPB + s;
+        }
+        b = (b + 1 == b0 + nbuckets) ? b0 : b + 1;
+    }
+    return ~0ull;
+}
+
 template <int NW>
 __device__ __forceinline__ uint64_t home_bucket(const uint64_t key[NW], uint64_t nbuckets) {
     return __umul64hi(table_hash<NW>(key), nbuckets);

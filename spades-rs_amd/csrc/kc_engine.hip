@@ -46,6 +46,7 @@
 #include "../../include/kc.h"
 #include "kc_device.h"
 #include "kc_hammer.h"
+#include "kc_mlp.h"
 #include "kc_host.h"
 
 using namespace kc;
@@ -384,7 +385,8 @@ __device__ __forceinline__ void scan_slots(const Slot<NW, Ev>* __restrict__ tab,
 }
 
 // fs[p] = min(4*t0 + j) over the distinct k-mers' bits; counts distinct k-mers per library.
-template <int NW, typename Ev>
+// SEEN (the sparse mode): also sets the seen bits directly instead of deriving them from fs.
+template <int NW, typename Ev, bool SEEN>
 __global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
                                                         uint64_t s_end, const uint64_t* __restrict__ lib_b0,
                                                         uint32_t n_libs, const LibDesc* __restrict__ libs,
@@ -406,6 +408,11 @@ __global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __re
             const Ev base = (Ev)(s.t0 * 4);
 #pragma unroll
             for (int j = 0; j < 3; ++j) atomicMin(&fs[p[j]], (Ev)(base + j));
+            if (SEEN) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    atomicOr(reinterpret_cast<unsigned long long*>(&L.seen[p[j] >> 6]), 1ull << (p[j] & 63));
+            }
             acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
         },
         [&](uint32_t lib0) {
@@ -417,24 +424,61 @@ __global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __re
         });
 }
 
-// seen word w (pool index) = ballot(fs != inf) over its 64 bits; clears the solid word.
+// seen word w (pool index) = the 64 bits (fs != inf) of its 64 first-setter entries, read
+// as 16-byte vectors by one thread; clears the solid word.
 template <typename Ev>
-__global__ void k_seen_from_fs(uint64_t w_begin, uint64_t w_end, const uint64_t* __restrict__ lib_w0, uint32_t n_libs,
-                               const LibDesc* __restrict__ libs) {
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const uint32_t lane = __lane_id();
-    for (uint64_t w = w_begin + wave; w < w_end; w += nwaves) {
+__global__ void __launch_bounds__(BLOCK) k_seen_from_fs(uint64_t w_begin, uint64_t w_end,
+                                                        const uint64_t* __restrict__ lib_w0, uint32_t n_libs,
+                                                        const LibDesc* __restrict__ libs) {
+    constexpr int PER = 16 / sizeof(Ev);       // entries per 16-byte load
+    constexpr int LOADS = 64 / PER;
+    for (uint64_t w = w_begin + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < w_end;
+         w += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t lib = find_region(lib_w0, n_libs, w);
         const LibDesc& L = libs[lib];
         const uint64_t lw = w - L.word0;
-        const Ev v = reinterpret_cast<const Ev*>(L.fs)[lw * 64 + lane];
-        const uint64_t m = __ballot(v != (Ev)~(Ev)0);
-        if (lane == 0) {
-            L.seen[lw] = m;
-            L.solid[lw] = 0;
+        const uint4* src = reinterpret_cast<const uint4*>(reinterpret_cast<const Ev*>(L.fs) + lw * 64);
+        uint4 v[LOADS];
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) v[q] = src[q];
+        uint64_t m = 0;
+#pragma unroll
+        for (int q = 0; q < LOADS; ++q) {
+            const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+            if (sizeof(Ev) == 4) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) m |= (uint64_t)(x[e] != 0xFFFFFFFFu) << (q * 4 + e);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    m |= (uint64_t)((x[2 * e] & x[2 * e + 1]) != 0xFFFFFFFFu) << (q * 2 + e);
+            }
         }
+        L.seen[lw] = m;
+        L.solid[lw] = 0;
     }
+}
+
+// Sparse mode: puts back the "no setter" value at every first-setter entry the table's
+// k-mers touched, so the array is all ~0 again for the next run.
+template <int NW, typename Ev>
+__global__ void __launch_bounds__(BLOCK) k_fs_reset(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
+                                                    uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                    uint32_t n_libs, const LibDesc* __restrict__ libs) {
+    scan_slots<NW, Ev>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t) {
+            const Slot<NW, Ev> s = tab[i];
+            if (s.k[0] == EMPTY_KEY) return;
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            Ev* fs = reinterpret_cast<Ev*>(L.fs);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) fs[p[j]] = (Ev)~(Ev)0;
+        },
+        [](uint32_t) {});
 }
 
 // Solid tier: OR of the bits of every promoted distinct k-mer (bloom.rs:99-103).
@@ -618,6 +662,166 @@ __global__ void k_unpack_keys(const Key128* __restrict__ in, uint64_t* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// learned first pass: features of the count-table entries
+// ---------------------------------------------------------------------------
+
+constexpr int N_FEAT = 16;
+
+// Sum of the k base qualities of every window of library `lib`, added to its k-mer's slot
+// of the (uncorrected) table.  aqoff[a] = quality offset of active read a, or ~0 when
+// the read has none.
+template <bool WIDE, typename Ev>
+__global__ void __launch_bounds__(BLOCK) k_qsum(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
+                                                const uint64_t* __restrict__ act_off,
+                                                const uint32_t* __restrict__ act_lib,
+                                                const uint32_t* __restrict__ tile_first,
+                                                const uint64_t* __restrict__ aqoff, const uint8_t* __restrict__ quals,
+                                                uint64_t n_active, uint64_t n_occ, uint64_t tile0, uint64_t tile1,
+                                                uint32_t k, uint32_t lib, const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                const LibDesc* __restrict__ libs,
+                                                unsigned long long* __restrict__ qsum) {
+    constexpr int NW = WIDE ? 2 : 1;
+    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
+    const LibDesc& L = libs[lib];
+    for (uint64_t tile = tile0 + blockIdx.x; tile < tile1; tile += gridDim.x) {
+        const uint64_t a0 = tile_first[tile];
+        const uint64_t w0 = tile * TILE_WINDOWS;
+        const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+        const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+        __syncthreads();
+        for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
+            const uint32_t lo = tile_read(s_ev, na, w);
+            if (act_lib[a0 + lo] != lib) continue;
+            const uint64_t qo = aqoff[a0 + lo];
+            if (qo == ~0ull) continue;
+            const uint32_t pos = (uint32_t)(w - s_ev[lo]);
+            uint64_t key[NW];
+            pack_key<NW>(canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], pos, k), k), key);
+            const uint64_t i =
+                table_find<NW, Ev>(tab, L.bucket0, L.nbuckets, L.bucket0 + home_bucket<NW>(key, L.nbuckets), key);
+            if (i == ~0ull) continue;
+            uint32_t s = 0;
+            for (uint32_t j = 0; j < k; ++j) s += quals[qo + pos + j];
+            atomicAdd(&qsum[i], (unsigned long long)s);
+        }
+    }
+}
+
+// Raw count of a k-mer value in the library's table (0 when absent).
+template <bool WIDE, typename Ev>
+__device__ __forceinline__ uint32_t table_count(const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab, const LibDesc& L,
+                                                const Kv& v, uint32_t k, uint64_t* slot) {
+    constexpr int NW = WIDE ? 2 : 1;
+    uint64_t key[NW];
+    pack_key<NW>(canonical<WIDE>(v, k), key);
+    const uint64_t i = table_find<NW, Ev>(tab, L.bucket0, L.nbuckets, L.bucket0 + home_bucket<NW>(key, L.nbuckets), key);
+    if (slot) *slot = i;
+    return i == ~0ull ? 0u : tab[i].cnt;
+}
+
+// The N_FEAT features of count-table entries [e0, e0 + n) (one library): its count c, the
+// mean quality of its occurrences' bases, and statistics of the raw counts of its 3k
+// Hamming-1 neighbours (canonical), relative to the library's cutoff and 95th-percentile
+// count.
+template <bool WIDE, typename Ev>
+__global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo,
+                                                    const uint32_t* __restrict__ cnt, uint64_t e0, uint64_t n,
+                                                    SortLayout lay, uint32_t k, uint32_t lib,
+                                                    const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                    const LibDesc* __restrict__ libs, const LibOut* __restrict__ outs,
+                                                    const unsigned long long* __restrict__ qsum,
+                                                    float* __restrict__ feat) {
+    const LibDesc& L = libs[lib];
+    const float cutoff = (float)outs[lib].cutoff, top = (float)max(outs[lib].top, 1u);
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = e0 + t;
+        // entry key -> k-mer value (the inverse of encode_entry)
+        Kv v;
+        if (lay.single) {
+            v.lo = 2 * k >= 64 ? lo[e] : (lo[e] & ((1ull << (2 * k)) - 1));
+            v.hi = 0;
+        } else if (2 * k <= 63) {
+            v.lo = lo[e];
+            v.hi = 0;
+        } else {
+            const uint32_t vb = 2 * k - 63;
+            const uint64_t a = hi[e] & ((1ull << vb) - 1);
+            v.lo = lo[e] | (a << 63);
+            v.hi = a >> 1;
+        }
+        const float c = (float)cnt[e];
+        uint64_t slot = ~0ull;
+        (void)table_count<WIDE, Ev>(tab, L, v, k, &slot);
+        const float q = (slot != ~0ull && c > 0.f) ? (float)qsum[slot] / (c * (float)k) : 0.f;
+        uint32_t mx = 0, mx2 = 0, present = 0, above_cut = 0, above_c = 0;
+        float sum = 0.f;
+        for (uint32_t pos = 0; pos < k; ++pos) {
+            const uint32_t sh = 2 * (k - 1 - pos);  // bits of base `pos` (first base most significant)
+            const uint32_t cur = sh < 64 ? (uint32_t)(v.lo >> sh) & 3u : (uint32_t)(v.hi >> (sh - 64)) & 3u;
+            for (uint32_t b = 0; b < 4; ++b) {
+                if (b == cur) continue;
+                Kv y = v;
+                if (sh < 64) y.lo = (y.lo & ~(3ull << sh)) | ((uint64_t)b << sh);
+                else y.hi = (y.hi & ~(3ull << (sh - 64))) | ((uint64_t)b << (sh - 64));
+                const uint32_t ny = table_count<WIDE, Ev>(tab, L, y, k, nullptr);
+                if (ny > mx) {
+                    mx2 = mx;
+                    mx = ny;
+                } else if (ny > mx2) {
+                    mx2 = ny;
+                }
+                present += ny > 0;
+                above_cut += (float)ny >= cutoff;
+                above_c += (float)ny > c;
+                sum += (float)ny;
+            }
+        }
+        float* f = feat + t * N_FEAT;
+        const float lc = log2f(1.f + c), lm = log2f(1.f + (float)mx);
+        f[0] = lc;
+        f[1] = q / 40.f;
+        f[2] = lm;
+        f[3] = log2f(1.f + sum);
+        f[4] = (float)present / (3.f * (float)k);
+        f[5] = c / (c + (float)mx);
+        f[6] = lc - log2f(1.f + cutoff);
+        f[7] = c >= cutoff ? 1.f : 0.f;
+        f[8] = fminf((float)above_cut, 4.f) / 4.f;
+        f[9] = lc - lm;
+        f[10] = (q / 40.f) * (q / 40.f);
+        f[11] = log2f(1.f + top) / 8.f;
+        f[12] = c / top;
+        f[13] = fminf((float)above_c, 4.f) / 4.f;
+        f[14] = log2f(1.f + (float)mx2);
+        f[15] = 1.f;
+    }
+}
+
+// labels: the network's decision outside the margin, else the exact rule count >= cutoff.
+__global__ void k_learned_labels(const float* __restrict__ p, const uint32_t* __restrict__ cnt, uint64_t e0,
+                                 uint64_t n, const LibOut* __restrict__ outs, uint32_t lib, float margin,
+                                 uint8_t* __restrict__ lab) {
+    const uint32_t cutoff = outs[lib].cutoff;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const float pt = p[t];
+        uint8_t v;
+        if (pt >= 1.f - margin) v = 1;
+        else if (pt <= margin) v = 0;
+        else v = 2 | (cnt[e0 + t] >= cutoff ? 1 : 0);  // bit 1: the exact rule decided
+        lab[t] = v;
+    }
+}
+
+__global__ void k_aqoff(const uint32_t* __restrict__ active, uint64_t n_active, const uint64_t* __restrict__ qoff,
+                        uint64_t* __restrict__ aqoff) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n_active;
+         a += (uint64_t)gridDim.x * blockDim.x)
+        aqoff[a] = qoff[active[a]];
+}
+
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         x[i] = (uint32_t)i;
@@ -745,6 +949,10 @@ struct HostLib {
     // prior / trusted contigs (ASCII), applied to the count table (assemble.rs:244-284)
     std::vector<uint8_t> prior;
     std::vector<uint64_t> prior_start, prior_len;
+    // per-base Phred qualities (features of the learned first pass): read i's bases at
+    // quals[qoff[i] ..]
+    std::vector<uint8_t> quals;
+    std::vector<uint64_t> qoff;
     uint64_t prior_windows(uint32_t k) const {
         uint64_t n = 0;
         for (uint64_t len : prior_len) n += len >= k ? len - k + 1 : 0;
@@ -752,6 +960,7 @@ struct HostLib {
     }
     // placement in the device pools (set by the planner)
     uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
+    uint64_t ev0 = 0;  // first global window number (set by the reads preparation)
     uint32_t group = 0;
     kc_info info{};
     uint64_t entry0 = 0;  // first sorted entry
@@ -786,14 +995,17 @@ struct kc_ctx {
     // filter pools, first setters, table pool
     DevBuf d_seen, d_solid, d_fs, d_tab, d_libs, d_lib_b0, d_lib_w0, d_blooms, d_out, d_mincov;
     uint64_t total_words = 0, total_buckets = 0, fs_bytes = 0;
+    uint64_t fs_clean_bytes = 0;  // leading bytes of d_fs known to hold ~0 (sparse mode)
+    bool bloom_sparse = false;    // first-setter mode of the last Bloom build
     std::vector<std::pair<uint32_t, uint32_t>> groups;  // [lib begin, lib end) sharing d_fs
     // count entries (sorted) + scratch
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
+    DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
     uint64_t n_entries = 0;
     SortLayout lay{};
-    kc_tune tune{4, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
+    kc_tune tune{1, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
     uint32_t n_cu = 0;
     bool wide = false, ev64 = false, built = false, counted = false;
     uint64_t* h_stage = nullptr;  // small pinned staging
@@ -973,7 +1185,10 @@ struct Run {
         c->n_occ = ev0[L()];
         c->n_active = c->h_stage[2];
         c->n_tiles = (c->n_occ + TILE_WINDOWS - 1) / TILE_WINDOWS;
-        for (uint32_t l = 0; l < L(); ++l) c->libs[l].info.n_occ = ev0[l + 1] - ev0[l];
+        for (uint32_t l = 0; l < L(); ++l) {
+            c->libs[l].info.n_occ = ev0[l + 1] - ev0[l];
+            c->libs[l].ev0 = ev0[l];
+        }
         lib_ev0_ = ev0;
         if (c->n_occ == 0) return KC_OK;
         if (c->n_tiles > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "too many windows for one launch");
@@ -1088,7 +1303,9 @@ struct Run {
         }
         c->groups.push_back({gb, L()});
         maxg = std::max(maxg, used);
+        void* fs_before = c->d_fs.p;
         KC_TRY(c->d_fs.ensure(maxg));
+        if (c->d_fs.p != fs_before) c->fs_clean_bytes = 0;
         c->fs_bytes = maxg;
         // descriptors
         std::vector<LibDesc> desc(L());
@@ -1172,27 +1389,84 @@ struct Run {
             if (!any) break;
             if (attempt >= 4) return fail(KC_ENOMEM, "device hash table overflow");
         }
-        // first setters -> seen tier -> solid tier, one group of libraries at a time
+        rc = bloom_groups<NW, Ev>(true);
+        if (rc) return rc;
+        return read_out();
+    }
+
+    // First setters -> seen tier -> solid tier, one group of libraries at a time.
+    // Dense mode fills each group's first-setter array and derives the seen tier from it.
+    // Sparse mode (every group's k-mers touch under 1/32 of its entries, by the distinct
+    // estimate) keeps the array all ~0 between runs: the first setters also set the seen
+    // bits, and afterwards only the touched entries are reset.  `decide` (eager runs)
+    // picks the mode and restores a clean array when needed; a captured graph replays
+    // the mode of the run it was planned from.
+    template <int NW, typename Ev>
+    int bloom_groups(bool decide) {
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        hipStream_t st = c->st;
+        if (decide) {
+            bool sparse = true;
+            for (auto& g : c->groups) {
+                uint64_t est = 0;
+                for (uint32_t l = g.first; l < g.second; ++l) est += c->libs[l].est;
+                const auto& le = c->libs[g.second - 1];
+                const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
+                if (3 * est * 32 >= fs_n) sparse = false;
+            }
+            c->bloom_sparse = sparse;
+            if (sparse && c->fs_clean_bytes < c->fs_bytes) {
+                TIMED(KID_FS_CLEAR, (k_fill<uint8_t><<<grid_for(c->fs_bytes), BLOCK, 0, st>>>(
+                                        c->d_fs.as<uint8_t>(), c->fs_bytes, (uint8_t)0xFF)));
+                c->fs_clean_bytes = c->fs_bytes;
+            }
+        }
+        const bool sparse = c->bloom_sparse;
+        const uint64_t clean = c->fs_clean_bytes;
+        c->fs_clean_bytes = 0;  // until this run's resets are enqueued
         for (auto& g : c->groups) {
             const auto& lb = c->libs[g.first];
             const auto& le = c->libs[g.second - 1];
             const uint64_t s_begin = lb.bucket0 * SPB, s_end = (le.bucket0 + le.nbuckets) * SPB;
             const uint64_t w_begin = lb.word0, w_end = le.word0 + le.num_words;
             const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
-            TIMED(KID_FS_CLEAR, (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
-            if (s_end > s_begin)
-                TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0,
-                                                                  st>>>(
-                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
-                                            c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>())));
-            TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for((w_end - w_begin) * 64), BLOCK, 0, st>>>(
-                                w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>())));
-            if (s_end > s_begin)
-                TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
-                                       c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
-                                       c->d_libs.as<LibDesc>())));
+            const uint32_t gs = grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS);
+            if (sparse) {
+                k_fill<uint64_t><<<grid_for(w_end - w_begin), BLOCK, 0, st>>>(c->d_seen.as<uint64_t>() + w_begin,
+                                                                             w_end - w_begin, 0ull);
+                k_fill<uint64_t><<<grid_for(w_end - w_begin), BLOCK, 0, st>>>(c->d_solid.as<uint64_t>() + w_begin,
+                                                                             w_end - w_begin, 0ull);
+                KC_TRY(hipGetLastError());
+                if (s_end > s_begin) {
+                    TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev, true><<<gs, BLOCK, 0, st>>>(
+                                                c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end,
+                                                c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                                c->d_out.as<LibOut>())));
+                    TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
+                                           c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
+                                           L(), c->d_libs.as<LibDesc>())));
+                    TIMED(KID_FS_CLEAR, (k_fs_reset<NW, Ev><<<gs, BLOCK, 0, st>>>(
+                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
+                                            L(), c->d_libs.as<LibDesc>())));
+                }
+            } else {
+                TIMED(KID_FS_CLEAR,
+                      (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
+                if (s_end > s_begin)
+                    TIMED(KID_FIRST_SETTER, (k_first_setter<NW, Ev, false><<<gs, BLOCK, 0, st>>>(
+                                                c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end,
+                                                c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                                c->d_out.as<LibOut>())));
+                TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for(w_end - w_begin), BLOCK, 0, st>>>(
+                                    w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>())));
+                if (s_end > s_begin)
+                    TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
+                                           c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
+                                           L(), c->d_libs.as<LibDesc>())));
+            }
         }
-        return read_out();
+        if (sparse) c->fs_clean_bytes = clean;
+        return KC_OK;
     }
 
     // Sort entries ascending by (library, k-mer).
@@ -1427,25 +1701,8 @@ struct Run {
         KC_TRY(hipGetLastError());
         rc = launch_insert<WIDE, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
         if (rc) return rc;
-        for (auto& g : c->groups) {
-            const auto& lb = c->libs[g.first];
-            const auto& le = c->libs[g.second - 1];
-            const uint64_t s_begin = lb.bucket0 * SPB, s_end = (le.bucket0 + le.nbuckets) * SPB;
-            const uint64_t w_begin = lb.word0, w_end = le.word0 + le.num_words;
-            const uint64_t fs_n = (le.fs_off + le.num_words * 64 * sizeof(Ev)) / sizeof(Ev);
-            k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0);
-            if (s_end > s_begin)
-                k_first_setter<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
-                    c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
-                    c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>());
-            k_seen_from_fs<Ev><<<grid_for((w_end - w_begin) * 64), BLOCK, 0, st>>>(
-                w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>());
-            if (s_end > s_begin)
-                k_promote<NW, Ev><<<grid_for(s_end - s_begin, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
-                    c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(), L(),
-                    c->d_libs.as<LibDesc>());
-            KC_TRY(hipGetLastError());
-        }
+        rc = bloom_groups<NW, Ev>(false);
+        if (rc) return rc;
         // ---- Stage 3 ----
         if (ec) {
             k_copy_bytes<<<grid_for(c->d_data.bytes / 8 + 1), BLOCK, 0, st>>>(c->d_data.as<uint8_t>(),
@@ -1525,6 +1782,57 @@ struct Run {
                 c->d_out.as<LibOut>(), c->d_sflags.as<uint8_t>(), c->d_out.as<LibOut>());
         KC_TRY(hipGetLastError());
         return KC_OK;
+    }
+
+    // The learned first pass's features of library lib's count-table entries, into d_feat
+    // (entry order, N_FEAT floats each).  The table must hold the uncorrected counts.
+    template <bool WIDE, typename Ev>
+    int features(uint32_t lib) {
+        constexpr int NW = WIDE ? 2 : 1;
+        hipStream_t st = c->st;
+        const HostLib& H = c->libs[lib];
+        const uint64_t n = H.info.n_counts, e0 = H.entry0;
+        const uint64_t slots = c->total_buckets * slots_per_bucket<NW, Ev>();
+        KC_TRY(c->d_qsum.ensure(std::max<uint64_t>(slots, 1) * 8));
+        k_fill<unsigned long long><<<grid_for(slots), BLOCK, 0, st>>>(c->d_qsum.as<unsigned long long>(), slots, 0ull);
+        KC_TRY(hipGetLastError());
+        if (!H.quals.empty() && c->n_active && H.info.n_occ) {
+            std::vector<uint64_t> qo(c->n_reads, ~0ull);
+            for (size_t i = 0; i < H.len.size(); ++i) qo[H.read0 + i] = H.qoff[i];
+            KC_TRY(c->d_quals.ensure(H.quals.size() + 64));
+            KC_TRY(c->d_rqoff.ensure(c->n_reads * 8 + 8));
+            KC_TRY(c->d_aqoff.ensure(c->n_active * 8 + 8));
+            KC_TRY(hipMemcpyAsync(c->d_quals.p, H.quals.data(), H.quals.size(), hipMemcpyHostToDevice, st));
+            KC_TRY(hipMemcpyAsync(c->d_rqoff.p, qo.data(), c->n_reads * 8, hipMemcpyHostToDevice, st));
+            k_aqoff<<<grid_for(c->n_active), BLOCK, 0, st>>>(c->d_active.as<uint32_t>(), c->n_active,
+                                                           c->d_rqoff.as<uint64_t>(), c->d_aqoff.as<uint64_t>());
+            KC_TRY(hipGetLastError());
+            const uint64_t t0 = H.ev0 / TILE_WINDOWS;
+            const uint64_t t1 = (H.ev0 + H.info.n_occ + TILE_WINDOWS - 1) / TILE_WINDOWS;
+            const uint32_t g = (uint32_t)std::min<uint64_t>(t1 - t0, 4096);
+            k_qsum<WIDE, Ev><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
+                                                  c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
+                                                  c->d_tile_first.as<uint32_t>(), c->d_aqoff.as<uint64_t>(),
+                                                  c->d_quals.as<uint8_t>(), c->n_active, c->n_occ, t0, t1, c->o.k, lib,
+                                                  c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(),
+                                                  c->d_qsum.as<unsigned long long>());
+            KC_TRY(hipGetLastError());
+            KC_TRY(hipStreamSynchronize(st));  // the host vector qo is released on return
+        }
+        KC_TRY(c->d_feat.ensure(std::max<uint64_t>(n, 1) * N_FEAT * 4));
+        if (n) {
+            k_features<WIDE, Ev><<<grid_for(n), BLOCK, 0, st>>>(
+                c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), e0, n, c->lay, c->o.k, lib,
+                c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>(),
+                c->d_qsum.as<unsigned long long>(), c->d_feat.as<float>());
+            KC_TRY(hipGetLastError());
+        }
+        return KC_OK;
+    }
+
+    int features_any(uint32_t lib) {
+        if (!c->wide) return c->ev64 ? features<false, uint64_t>(lib) : features<false, uint32_t>(lib);
+        return c->ev64 ? features<true, uint64_t>(lib) : features<true, uint32_t>(lib);
     }
 
     // Buffers the planned run needs beyond the eager run's (called before capture).
@@ -1626,7 +1934,9 @@ void kc_destroy(kc_ctx* ctx) {
     ctx->graph_release();
     ctx->d_gk1.release();
     ctx->d_gk2.release();
-    DevBuf* pbufs[] = {&ctx->d_pseq, &ctx->d_pstart, &ctx->d_pw0, &ctx->d_plib, &ctx->d_psum, &ctx->d_pflag};
+    DevBuf* pbufs[] = {&ctx->d_pseq,  &ctx->d_pstart, &ctx->d_pw0,  &ctx->d_plib, &ctx->d_psum,
+                       &ctx->d_pflag, &ctx->d_quals,  &ctx->d_rqoff, &ctx->d_aqoff, &ctx->d_qsum,
+                       &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab};
     for (DevBuf* b : pbufs) b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
@@ -1672,6 +1982,16 @@ int kc_library_priors_set(kc_ctx* ctx, uint32_t lib, const uint8_t* seqs, size_t
 int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts, const uint64_t* lengths,
                   size_t n_seqs) {
     return kc_library_priors_set(ctx, 0, seqs, seqs_len, starts, lengths, n_seqs);
+}
+
+int kc_set_k(kc_ctx* ctx, uint32_t k) {
+    if (!ctx) return KC_EINVAL;
+    if (k < 1 || k > KC_MAX_K) return ctx->fail(k > KC_MAX_K ? KC_EUNSUPPORTED : KC_EINVAL, "k outside 1..63");
+    ctx->o.k = k;
+    ctx->wide = k > 32;
+    ctx->built = ctx->counted = false;
+    ctx->graph_release();
+    return KC_OK;
 }
 
 int kc_library_count(const kc_ctx* ctx, uint32_t* n) {
@@ -2001,6 +2321,83 @@ int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_
 
 int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask, size_t n) {
     return kc_library_counts_export(ctx, 0, keys4, counts, solid_mask, n);
+}
+
+int kc_library_quals_set(kc_ctx* ctx, uint32_t lib, const uint8_t* quals, size_t n_quals, const uint64_t* qoffsets) {
+    if (!ctx) return KC_EINVAL;
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    HostLib& H = ctx->libs[lib];
+    if (n_quals && !quals) return ctx->fail(KC_EINVAL, "null quals");
+    if (!H.len.empty() && n_quals && !qoffsets) return ctx->fail(KC_EINVAL, "null qoffsets");
+    if (n_quals == 0) {
+        H.quals.clear();
+        H.qoff.clear();
+        return KC_OK;
+    }
+    for (size_t i = 0; i < H.len.size(); ++i)
+        if (qoffsets[i] > n_quals || H.len[i] > n_quals - qoffsets[i])
+            return ctx->fail(KC_EINVAL, "qualities of read " + std::to_string(i) + " extend past n_quals");
+    H.quals.assign(quals, quals + n_quals);
+    H.qoff.assign(qoffsets, qoffsets + H.len.size());
+    return KC_OK;
+}
+
+static int learned_ready(kc_ctx* ctx, uint32_t lib, size_t n) {
+    if (!ctx->counted) return ctx->fail(KC_ESTATE, "features before kc_count");
+    if (ctx->o.error_correct) return ctx->fail(KC_ESTATE, "features need the uncorrected table (error_correct off)");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    if (n != ctx->libs[lib].info.n_counts) return ctx->fail(KC_EINVAL, "n must equal the library's n_counts");
+    return KC_OK;
+}
+
+int kc_library_features(kc_ctx* ctx, uint32_t lib, float* out, size_t n) {
+    if (!ctx) return KC_EINVAL;
+    int rc = learned_ready(ctx, lib, n);
+    if (rc) return rc;
+    if (n && !out) return ctx->fail(KC_EINVAL, "null out");
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    Run run(ctx);
+    rc = run.features_any(lib);
+    if (rc) return rc;
+    if (n) CTX_TRY(hipMemcpyAsync(out, ctx->d_feat.p, n * N_FEAT * 4, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    return KC_OK;
+}
+
+int kc_library_learned_select(kc_ctx* ctx, uint32_t lib, kc_mlp* mlp, float margin, uint8_t* labels,
+                              uint8_t* fallback, size_t n, double* ms) {
+    if (!ctx || !mlp) return KC_EINVAL;
+    int rc = learned_ready(ctx, lib, n);
+    if (rc) return rc;
+    if (mlp->device != ctx->o.device) return ctx->fail(KC_EINVAL, "network and context on different devices");
+    if (!(margin >= 0.f && margin < 0.5f)) return ctx->fail(KC_EINVAL, "margin must be in [0, 0.5)");
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    hipEvent_t e0 = ctx->new_event(), e1 = ctx->new_event();
+    CTX_TRY(hipEventRecord(e0, ctx->st));
+    Run run(ctx);
+    rc = run.features_any(lib);
+    if (rc) return rc;
+    CTX_TRY(ctx->d_prob.ensure(std::max<size_t>(n, 1) * 4));
+    CTX_TRY(ctx->d_lab.ensure(std::max<size_t>(n, 1)));
+    CTX_TRY(mlp_infer_device(mlp, ctx->d_feat.as<float>(), n, ctx->d_prob.as<float>(), ctx->st));
+    if (n) {
+        k_learned_labels<<<grid_for(n), BLOCK, 0, ctx->st>>>(ctx->d_prob.as<float>(), ctx->d_cnt.as<uint32_t>(),
+                                                             ctx->libs[lib].entry0, n, ctx->d_out.as<LibOut>(), lib,
+                                                             margin, ctx->d_lab.as<uint8_t>());
+        CTX_TRY(hipGetLastError());
+    }
+    CTX_TRY(hipEventRecord(e1, ctx->st));
+    std::vector<uint8_t> lab(n);
+    if (n) CTX_TRY(hipMemcpyAsync(lab.data(), ctx->d_lab.p, n, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    float t = 0;
+    CTX_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    for (size_t i = 0; i < n; ++i) {
+        if (labels) labels[i] = lab[i] & 1;
+        if (fallback) fallback[i] = (lab[i] >> 1) & 1;
+    }
+    return KC_OK;
 }
 
 int kc_library_reads_export(kc_ctx* ctx, uint32_t lib, uint8_t* data, size_t data_len) {

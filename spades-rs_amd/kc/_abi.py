@@ -30,7 +30,10 @@ EXPORTS = (
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
     "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
     "kc_library_counts_export", "kc_library_reads_export", "kc_library_priors_set", "kc_priors_set",
-    "kc_tune_get", "kc_tune_set", "kc_autotune", "kc_graph_run", "kc_graph_release",
+    "kc_tune_get", "kc_tune_set", "kc_autotune", "kc_graph_run", "kc_graph_release", "kc_set_k",
+    "kc_mlp_create", "kc_mlp_destroy", "kc_mlp_last_error", "kc_mlp_num_params", "kc_mlp_get_params",
+    "kc_mlp_set_params", "kc_mlp_forward", "kc_mlp_gradient", "kc_mlp_train",
+    "kc_library_quals_set", "kc_library_features", "kc_library_learned_select",
 )
 
 
@@ -69,6 +72,7 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _dblp = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
 _sz = ctypes.c_size_t
 
 _SIGS = {
@@ -108,6 +112,20 @@ _SIGS = {
     "kc_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_tune)]),
     "kc_graph_run": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
     "kc_graph_release": (ctypes.c_int, [_vp]),
+    "kc_set_k": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "kc_mlp_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_uint64]),
+    "kc_mlp_destroy": (None, [_vp]),
+    "kc_mlp_last_error": (ctypes.c_char_p, [_vp]),
+    "kc_mlp_num_params": (ctypes.c_uint32, []),
+    "kc_mlp_get_params": (ctypes.c_int, [_vp, _f32p, _sz]),
+    "kc_mlp_set_params": (ctypes.c_int, [_vp, _f32p, _sz]),
+    "kc_mlp_forward": (ctypes.c_int, [_vp, _f32p, _sz, _f32p]),
+    "kc_mlp_gradient": (ctypes.c_int, [_vp, _f32p, _u8p, _f32p, _sz, ctypes.c_float, _f32p, _dblp]),
+    "kc_mlp_train": (ctypes.c_int, [_vp, _f32p, _u8p, _f32p, _sz, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_uint64, _dblp]),
+    "kc_library_quals_set": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz, _u64p]),
+    "kc_library_features": (ctypes.c_int, [_vp, ctypes.c_uint32, _f32p, _sz]),
+    "kc_library_learned_select": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.c_float, _u8p, _u8p, _sz, _dblp]),
 }
 
 _lib = None
@@ -136,4 +154,4 @@ def ptr(a, t):
     return a.ctypes.data_as(t)
 
 
-u8p, u32p, u64p = _u8p, _u32p, _u64p
+u8p, u32p, u64p, f32p = _u8p, _u32p, _u64p, _f32p

@@ -260,6 +260,14 @@ class SpectrumEngine:
         self._chk(lib().kc_run(self._h, ctypes.byref(info)))
         return info.as_dict()
 
+    def set_k(self, k):
+        """Next runs count k-mers of size k over the same resident reads (multik.rs:109-178)."""
+        self._chk(lib().kc_set_k(self._h, int(k)))
+        self.config = AssemblerConfig(k=int(k), min_coverage=self.config.min_coverage,
+                                      bloom_bits=self.config.bloom_bits, error_correct=self.config.error_correct,
+                                      prior_contigs=self.config.prior_contigs,
+                                      trusted_contigs=self.config.trusted_contigs)
+
     def graph_run(self):
         """Stage 2 + 3 as one HIP graph launch (captured on the first call; re-planned
         eagerly whenever a data-dependent size no longer matches).  Same results as run()."""
@@ -286,6 +294,30 @@ class SpectrumEngine:
         t = _abi.kc_tune()
         self._chk(lib().kc_autotune(self._h, int(reps), ctypes.byref(t)))
         return t.as_dict()
+
+    def set_quals(self, quals, qoffsets, library=0):
+        """Per-base Phred qualities (features of the learned first pass)."""
+        q = np.ascontiguousarray(quals, dtype=np.uint8)
+        o = np.ascontiguousarray(qoffsets, dtype=np.uint64)
+        qq = q if q.shape[0] else np.zeros(1, np.uint8)
+        self._chk(lib().kc_library_quals_set(self._h, library, ptr(qq, u8p), q.shape[0], ptr(o, u64p)))
+
+    def features(self, library=0):
+        """(n_counts, 16) float32 features of the count-table entries, in their sorted order."""
+        n = self.info(library)["n_counts"]
+        f = np.zeros((max(n, 1), 16), dtype=np.float32)
+        self._chk(lib().kc_library_features(self._h, library, ptr(f, _abi.f32p), n))
+        return f[:n]
+
+    def learned_select(self, mlp, margin=0.05, library=0):
+        """(labels, fallback mask, device ms) of the learned first pass over the entries."""
+        n = self.info(library)["n_counts"]
+        lab = np.zeros(max(n, 1), dtype=np.uint8)
+        fb = np.zeros(max(n, 1), dtype=np.uint8)
+        ms = ctypes.c_double(0)
+        self._chk(lib().kc_library_learned_select(self._h, library, mlp._h, float(margin), ptr(lab, u8p), ptr(fb, u8p),
+                                                  n, ctypes.byref(ms)))
+        return lab[:n].astype(bool), fb[:n].astype(bool), ms.value
 
     def info(self, library=0):
         info = _abi.kc_info()

@@ -140,3 +140,65 @@ def workload_64_libraries(seed=3, scale=1.0, n_libs=64):
         store = uneven_pairs(rng, genome, n_pairs)
         out.append((store, {"genome_size": g, "coverage": float(covs[i]), "n_pairs": n_pairs}))
     return out
+
+
+def repeat_genome(rng, size, repeat_frac=0.2, fam_len=(300, 6000), copies=(2, 50), divergence=(0.01, 0.05)):
+    """A random genome in which about repeat_frac of the bases are copies of repeat
+    families (lengths log-uniform in fam_len, copies uniform in copies, each copy a
+    mutated version of its family consensus at a per-copy divergence uniform in
+    divergence)."""
+    g = random_genome(rng, size, 0.5)
+    covered = 0
+    target = int(repeat_frac * size)
+    while covered < target:
+        L = int(np.exp(rng.uniform(np.log(fam_len[0]), np.log(fam_len[1]))))
+        L = min(L, size // 4)
+        cons = random_genome(rng, L, 0.5)
+        for _ in range(int(rng.integers(copies[0], copies[1] + 1))):
+            d = rng.uniform(*divergence)
+            copy = _mutate(rng, cons[None, :], np.full(L, d))[0]
+            pos = int(rng.integers(0, size - L + 1))
+            g[pos:pos + L] = copy
+            covered += L
+            if covered >= target:
+                break
+    return g
+
+
+def repeat_pairs_with_quals(seed, genome_size=12_000_000, coverage=60.0, read_len=150, p_err=0.003,
+                            q_good=(30, 40), q_bad=(5, 20), chunk=1 << 17):
+    """Paired 2x150 reads at `coverage` from a repeat_genome, uniform substitutions at
+    p_err, and a Phred quality per base: uniform in q_good for correct bases and in q_bad
+    for substituted ones.  Returns (PackedReads, quals, meta): quals holds one byte per
+    base, read i's bases at quals[qoff[i] : qoff[i] + len_i] with qoff = meta['qoff']."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    genome = repeat_genome(rng, genome_size)
+    G = genome.shape[0]
+    n_pairs = max(int(round(coverage * G / (2 * read_len))), 1)
+    ar = np.arange(read_len, dtype=np.int64)
+    parts = [[], []]
+    qparts = [[], []]
+    for c0 in range(0, n_pairs, chunk):
+        m = min(chunk, n_pairs - c0)
+        frag = np.clip(np.rint(rng.normal(300.0, 30.0, m)).astype(np.int64), read_len, G)
+        start = (rng.random(m) * (G - frag + 1)).astype(np.int64)
+        r1 = genome[start[:, None] + ar[None, :]]
+        tail = start + frag - read_len
+        r2 = 3 - genome[tail[:, None] + ar[None, ::-1]]
+        for side, r in enumerate((r1, r2)):
+            err = rng.random(r.shape) < p_err
+            sub = rng.integers(1, 4, r.shape, dtype=np.uint8)
+            rr = np.where(err, (r + sub) & 3, r).astype(np.uint8)
+            q = np.where(err, rng.integers(q_bad[0], q_bad[1] + 1, r.shape),
+                         rng.integers(q_good[0], q_good[1] + 1, r.shape)).astype(np.uint8)
+            parts[side].append(pack_fixed_length(rr)[0])
+            qparts[side].append(q.reshape(-1))
+    nb = (read_len + 3) // 4
+    data = np.concatenate(parts[0] + parts[1])
+    quals = np.concatenate(qparts[0] + qparts[1])
+    n = 2 * n_pairs
+    offsets = np.arange(n, dtype=np.uint64) * np.uint64(nb)
+    lengths = np.full(n, read_len, dtype=np.uint32)
+    store = PackedReads.from_arrays(data, offsets, lengths, pe_count=n_pairs)
+    qoff = np.arange(n, dtype=np.uint64) * np.uint64(read_len)
+    return store, quals, {"genome": genome, "n_pairs": n_pairs, "qoff": qoff, "seed": seed}

@@ -124,3 +124,58 @@ def test_tune_rejects_bad_settings():
         with pytest.raises(KcError):
             eng.set_tune(**bad)
     eng.close()
+
+
+def test_multik_over_resident_reads(oracle):
+    """multik.rs:109-178: k = 21, 33, 55 over one upload, with the previous k's sequences
+    injected as priors; then k back to 21 (buffers sized for 55 are reused)."""
+    store, meta = synth.uniform_genome_pairs(seed=77, genome_size=6000, n_pairs=500, read_len=150,
+                                             insert_mean=300, insert_sd=20, p_start=0.004)
+    g = "".join("ACGT"[int(x)] for x in meta["genome"])
+    d, o, l = store.arrays()
+    bits = 1 << 18
+    eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=bits, error_correct=True))
+    eng.upload(store)
+    for k, priors in ((21, None), (33, [g[0:800]]), (55, [g[0:800], g[3000:4000]]), (21, None)):
+        say(f"case multik k={k}")
+        eng.set_k(k)
+        eng.set_priors(priors or [])
+        info = eng.run()
+        S = oracle.Spectrum(d, o, l, k, bits, True, 5.0, 1, priors=priors)
+        assert info["n_occ"] == S.n_occ
+        keys, cnt, mask = eng.counts()
+        okeys, ocnt, omask = S.counts()
+        assert np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt) and np.array_equal(mask, omask)
+        assert np.array_equal(eng.reads(), S.corrected_data())
+    eng.close()
+
+
+def test_sparse_first_setters(oracle):
+    """Libraries whose k-mers touch a small part of large Bloom tiers take the sparse
+    first-setter path (the first-setter array stays all ~0 between runs, only touched
+    entries are reset): repeated eager runs, a change of k, graph replays, and a switch to
+    small tiers (the dense path) and back all match the oracle."""
+    libs, genomes = zip(*[_uneven_library(500 + i, 2000 + 800 * i, 80 + 30 * i) for i in range(4)])
+    libs = list(libs)
+    big, small = 1 << 22, 1 << 12
+    eng = SpectrumEngine(AssemblerConfig(k=33, bloom_bits=big, error_correct=True))
+    for st in libs:
+        eng.add_library(st, bloom_bits=big)
+    mcs = [5.0] * len(libs)
+    for k in (33, 33, 21):
+        say(f"case sparse k={k}")
+        eng.set_k(k)
+        eng.run()
+        _oracle_check(oracle, eng, libs, k, big, True, mcs, {})
+    for rep in range(3):
+        say(f"  sparse graph call {rep}")
+        eng.graph_run()
+        _oracle_check(oracle, eng, libs, 21, big, True, mcs, {})
+    eng.close()
+    # the same libraries on small tiers (dense), then large again in a fresh context
+    eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=small))
+    for st in libs:
+        eng.add_library(st, bloom_bits=small)
+    eng.run()
+    _oracle_check(oracle, eng, libs, 21, small, False, mcs, {})
+    eng.close()

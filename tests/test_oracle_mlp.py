@@ -1,0 +1,54 @@
+"""Pins the NumPy restatement of the learned first pass's network (oracle/mlp_np.py):
+  * bf16 rounding known answers (ties to even, carries into the exponent);
+  * the analytic gradient equals central finite differences of the loss (with the bf16
+    roundings off, where the loss is differentiable), for every parameter block;
+  * p is the logistic of the logit, and the loss is the weighted BCE computed directly."""
+import numpy as np
+
+from oracle import mlp_np as M
+
+
+def test_bf16_rounding_known_answers():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 65504.0, 1.0 + 2 ** -8 + 2 ** -20,
+                  (2 - 2 ** -8)], dtype=np.float32)
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -2.5, 65536.0, 1.0 + 2 ** -7, 2.0], dtype=np.float32)
+    assert np.array_equal(M.bf16(x), want)
+    # every bf16 value is a fixed point
+    v = np.arange(0, 1 << 16, 7, dtype=np.uint32) << 16
+    f = v.view(np.float32)
+    f = f[np.isfinite(f)]
+    assert np.array_equal(M.bf16(f).view(np.uint32), f.view(np.uint32))
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    w = M.init_params(1).astype(np.float64)
+    X = rng.normal(0, 1, (40, M.N_IN))
+    y = rng.integers(0, 2, 40)
+    sw = rng.uniform(0.5, 2.0, 40)
+
+    def L(wv):
+        return M.loss_and_grad(wv, X, y, sw, pos_weight=1.7, rounding=False)[0]
+
+    _, g = M.loss_and_grad(w, X, y, sw, pos_weight=1.7, rounding=False)
+    idx = list(range(0, M.OFF_B1, 97)) + list(range(M.OFF_B1, M.OFF_W2, 13)) + \
+        list(range(M.OFF_W2, M.OFF_B2, 311)) + list(range(M.OFF_B2, M.OFF_W3, 17)) + \
+        list(range(M.OFF_W3, M.OFF_B3, 9)) + [M.OFF_B3]
+    for i in idx:
+        e = np.zeros_like(w)
+        e[i] = 1e-5
+        fd = (L(w + e) - L(w - e)) / 2e-5
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_forward_is_logistic_of_logit_and_loss_is_bce():
+    rng = np.random.default_rng(5)
+    w = M.init_params(2)
+    X = rng.normal(0, 1, (50, M.N_IN)).astype(np.float32)
+    y = rng.integers(0, 2, 50)
+    p, inter = M.forward(w, X)
+    z3 = inter[-1]
+    assert np.allclose(p, 1 / (1 + np.exp(-z3)))
+    loss, _ = M.loss_and_grad(w, X, y)
+    direct = -np.mean(y * np.log(p) + (1 - y) * np.log(1 - p))
+    assert abs(loss - direct) < 1e-9

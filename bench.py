@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "spades-rs_amd")
 for _p in (ROOT, PKG):
@@ -98,7 +100,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="5mb_100x", choices=["5mb_100x", "1mb_30x", "64lib"])
+    ap.add_argument("--workload", default="5mb_100x", choices=["5mb_100x", "1mb_30x", "64lib", "learned"])
+    ap.add_argument("--train-seeds", type=int, default=8, help="learned: training seeds (held-out seeds follow)")
+    ap.add_argument("--test-seeds", type=int, default=2)
+    ap.add_argument("--margin", type=float, default=0.05, help="learned: confidence margin of the first pass")
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--scale", type=float, default=0.1,
                     help="64lib: genome-size scale (1.0 = 5 kb-6 Mb genomes, ~5.4 Gb of reads)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -130,6 +136,8 @@ def main():
     ks = [int(x) for x in args.ks.split(",")]
     if args.workload == "64lib":
         return bench_many_libraries(args)
+    if args.workload == "learned":
+        return bench_learned(args)
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -295,6 +303,93 @@ def microbenchmarks(device=0):
             out[name] = {"G_ops_s": round(n_ops / (ms.value / 1e3) / 1e9, 2), "ms": round(ms.value, 3),
                          "array_bytes": nbytes}
     return out
+
+
+def _key_index(keys):
+    """Rows of (n, 4) k-mer limbs as one sortable 1-D array (k <= 63: limbs 2, 3 are 0)."""
+    a = np.ascontiguousarray(keys[:, :2], dtype=np.uint64)
+    return a.view([("lo", "<u8"), ("hi", "<u8")]).reshape(-1)
+
+
+def learned_seed(seed, scale, k):
+    """One repeat-rich library: the exact Hammer-on selection (the judge) and the
+    uncorrected table's features.  Returns a dict of arrays and timings."""
+    from kc import AssemblerConfig, SpectrumEngine, synth
+    g = max(int(12_000_000 * scale), 20_000)
+    store, quals, meta = synth.repeat_pairs_with_quals(seed, genome_size=g, coverage=60.0)
+    ex = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=1 << 29, error_correct=True))
+    ex.upload(store)
+    ex.run()
+    info_ex = ex.run()  # warm
+    kex, _, mex = ex.counts()
+    judge = np.sort(_key_index(kex[mex.astype(bool)]))
+    ex.close()
+    pl = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=1 << 29))
+    pl.upload(store)
+    pl.run()
+    info_pl = pl.run()
+    pl.set_quals(quals, meta["qoff"])
+    keys, cnt, mask = pl.counts()
+    X = pl.features()
+    y = np.isin(_key_index(keys), judge)
+    return {"engine": pl, "X": X, "y": y, "cnt": cnt, "cutoff": info_pl["cutoff"], "cutoff_exact": info_ex["cutoff"],
+            "judge": judge.shape[0],
+            "judge_outside": int(judge.shape[0] - y.sum()), "t_exact": info_ex["ms_stage3"],
+            "t_plain": info_pl["ms_stage3"], "reads": len(store), "n_occ": info_pl["n_occ"]}
+
+
+def bench_learned(args):
+    """Learned first pass versus the exact Hammer-on solid-k-mer selection on repeat-rich
+    genomes (20% repeats, 60x, 0.3% errors, qualities), k=21: trained on --train-seeds
+    seeds, judged on the next --test-seeds seeds."""
+    import torch
+    from kc.learn import Mlp
+    k = 21
+    scale = args.scale
+    Xs, ys = [], []
+    for s in range(args.train_seeds):
+        d = learned_seed(1000 + s, scale, k)
+        Xs.append(d["X"])
+        ys.append(d["y"])
+        d["engine"].close()
+        log(f"train seed {s}: {d['X'].shape[0]} entries, {d['y'].mean():.3f} solid after correction")
+    X, y = np.concatenate(Xs), np.concatenate(ys).astype(np.uint8)
+    m = Mlp(seed=1)
+    t0 = time.perf_counter()
+    loss = m.train(X, y, epochs=args.epochs, batch=4096, lr=2e-3, seed=7)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"trained on {X.shape[0]} rows in {train_s:.1f}s, last loss {loss:.4f}")
+    res = []
+    for s in range(args.test_seeds):
+        d = learned_seed(5000 + s, scale, k)
+        eng = d["engine"]
+        lab, fb, ms = eng.learned_select(m, args.margin)
+        ms_best = min([ms] + [eng.learned_select(m, args.margin)[2] for _ in range(2)])
+        plain = d["cnt"] >= d["cutoff"]
+        res.append({"entries": int(lab.shape[0]), "agreement": float(np.mean(lab == d["y"])),
+                    "plain_rule_agreement": float(np.mean(plain == d["y"])),
+                    "fallback_fraction": float(fb.mean()), "residual_mismatches": int(np.sum(lab != d["y"])),
+                    "judge_solid": d["judge"], "judge_solid_outside_table": d["judge_outside"],
+                    "exact_selection_ms": round(d["t_exact"], 3), "learned_pass_ms": round(ms_best, 3),
+                    "plain_rule_selection_ms": round(d["t_plain"], 3),
+                    "cutoff_uncorrected": int(d["cutoff"]), "cutoff_exact": int(d["cutoff_exact"]),
+                    "rule_at_exact_cutoff_agreement": float(np.mean((d["cnt"] >= d["cutoff_exact"]) == d["y"])),
+                    "speedup_vs_exact": round(d["t_exact"] / ms_best, 3)})
+        eng.close()
+        log(f"test seed {s}: {res[-1]}")
+    m.close()
+    agree = float(np.mean([r["agreement"] for r in res]))
+    out = {
+        "metric": "learned first pass vs exact Hammer-on solid-k-mer selection (speedup at label agreement)",
+        "value": round(float(np.mean([r["speedup_vs_exact"] for r in res])), 3), "unit": "x",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded)", "dtype": "bf16/fp32",
+        "config": {"workload": f"repeat-rich genome 12 Mb x {scale}, 20% repeats, 60x, 0.3% errors, k=21",
+                   "train_seeds": args.train_seeds, "test_seeds": args.test_seeds, "margin": args.margin,
+                   "epochs": args.epochs, "train_rows": int(X.shape[0]), "train_s": round(train_s, 2)},
+        "agreement": agree, "per_test_seed": res,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_many_libraries(args):

@@ -107,8 +107,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--scale", type=float, default=0.1,
                     help="64lib: genome-size scale (1.0 = 5 kb-6 Mb genomes, ~5.4 Gb of reads)")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip the autotuner (it runs before the timed region by default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner on each engine before the timed region (the defaults are its usual choice)")
     ap.add_argument("--no-microbench", dest="microbench", action="store_false",
                     help="skip the random-atomic / copy microbenchmarks")
     ap.add_argument("--eager", action="store_true", help="64lib: time eager runs instead of graph replays")

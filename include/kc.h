@@ -168,6 +168,29 @@ int kc_library_priors_set(kc_ctx* ctx, uint32_t lib, const uint8_t* seqs, size_t
 int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint64_t* starts, const uint64_t* lengths,
                   size_t n_seqs); /* library 0 */
 
+/* ---- FASTQ ingestion on the device (packed_reads.rs) ------------------------
+ * File contents (uncompressed) of a paired set -> the three-block PackedReads layout of
+ * PackedReads::from_paired_fastq (packed_reads.rs:325-380): FastxReader's record checks,
+ * the three N gates (terminal trim, split on internal N into >= 21-base sub-reads, the
+ * ambiguous sidecar) and 2-bit packing all run on the GPU.  r2 = NULL ingests one file
+ * as append_from_file does (every read through the gates, pe_count 0).  The device
+ * handles the plain 4-line FASTQ layout; FASTA input and blank lines between records
+ * return KC_EUNSUPPORTED (parse those on the host), malformed records KC_EINVAL;
+ * kc_fastq_last_error() gives the message. */
+typedef struct kc_packed kc_packed;
+typedef struct kc_packed_info {
+    uint64_t n_reads, data_len, pe_count; /* the PackedReads arrays and its paired block size */
+    uint64_t n_ambiguous;                 /* reads routed to the ambiguous sidecar (not packed) */
+    uint64_t r1_records, r2_records;      /* records read from each file */
+} kc_packed_info;
+int kc_fastq_pack(int32_t device, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
+                  kc_packed** out);
+const char* kc_fastq_last_error(void);
+int kc_packed_info_get(const kc_packed* p, kc_packed_info* info);
+int kc_packed_export(const kc_packed* p, uint8_t* data, size_t data_len, uint64_t* offsets, uint32_t* lengths,
+                     size_t n_reads);
+void kc_packed_destroy(kc_packed* p);
+
 /* ---- Tuning and graph replay ---------------------------------------------- */
 
 /* Launch shape of the table insert and the table load factor.  Every setting gives

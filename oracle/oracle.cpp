@@ -379,6 +379,245 @@ static void get_read(const uint8_t* data, uint64_t off, uint32_t len, std::vecto
 }
 
 // ---------------------------------------------------------------------------
+// packed_reads.rs:60-235 FastxReader over an in-memory file; 436-460, 513-578, 630-690
+// three-gate ingestion of paired files into the three-block layout
+// ---------------------------------------------------------------------------
+
+struct MemFastx {
+    const uint8_t* p;
+    size_t n, pos = 0;
+    bool is_fastq = true, eof = false;
+    std::string err;
+    // read_until(b'\n'): appends through the newline; false when nothing is left
+    bool read_line(std::vector<uint8_t>& out) {
+        out.clear();
+        if (pos >= n) return false;
+        while (pos < n) {
+            const uint8_t b = p[pos++];
+            out.push_back(b);
+            if (b == '\n') break;
+        }
+        return true;
+    }
+    static void strip_eol(std::vector<uint8_t>& v) {
+        while (!v.empty() && (v.back() == '\n' || v.back() == '\r')) v.pop_back();
+    }
+    MemFastx(const uint8_t* p_, size_t n_) : p(p_), n(n_) {
+        std::vector<uint8_t> line;
+        for (;;) {  // FastxReader::open: skip blank / whitespace-only leading lines
+            if (pos >= n) {
+                eof = true;
+                return;
+            }
+            if (p[pos] == '\n' || p[pos] == '\r') {
+                read_line(line);
+                continue;
+            }
+            size_t e = pos;
+            while (e < n && p[e] != '\n') ++e;
+            bool ws = true;
+            for (size_t i = pos; i < e; ++i) ws = ws && (p[i] == ' ' || p[i] == '\t' || p[i] == '\r' || p[i] == '\f' || p[i] == 0x0B);
+            if (ws) {
+                read_line(line);
+                continue;
+            }
+            if (p[pos] == '@') is_fastq = true;
+            else if (p[pos] == '>') is_fastq = false;
+            else {
+                err = "expected '@' or '>' header";
+                eof = true;
+                return;
+            }
+            break;
+        }
+        if (!is_fastq) read_line(line);  // the first '>' header
+    }
+    // 1: a sequence in seq, 0: end of file, -1: malformed (err)
+    int next_seq(std::vector<uint8_t>& seq) {
+        seq.clear();
+        if (eof) return 0;
+        std::vector<uint8_t> line;
+        if (is_fastq) {
+            for (;;) {
+                if (!read_line(line)) {
+                    eof = true;
+                    return 0;
+                }
+                strip_eol(line);
+                if (line.empty()) continue;
+                if (line[0] != '@') {
+                    err = "expected '@' header";
+                    return -1;
+                }
+                if (!read_line(line)) {
+                    eof = true;
+                    err = "unexpected EOF waiting for sequence line";
+                    return -1;
+                }
+                strip_eol(line);
+                seq = line;
+                if (!read_line(line)) {
+                    eof = true;
+                    err = "unexpected EOF waiting for '+' separator";
+                    return -1;
+                }
+                if (line.empty() || line[0] != '+') {
+                    err = "expected '+' separator line";
+                    return -1;
+                }
+                if (!read_line(line)) {
+                    eof = true;
+                    err = "unexpected EOF waiting for quality scores";
+                    return -1;
+                }
+                strip_eol(line);
+                if (line.size() != seq.size()) {
+                    err = "sequence length != quality length";
+                    return -1;
+                }
+                return 1;
+            }
+        }
+        for (;;) {  // FASTA
+            if (!read_line(line)) {
+                eof = true;
+                return seq.empty() ? 0 : 1;
+            }
+            strip_eol(line);
+            if (!line.empty() && line[0] == '>') return 1;
+            seq.insert(seq.end(), line.begin(), line.end());
+        }
+    }
+};
+
+struct PackedStore {
+    std::vector<uint8_t> data;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> len;
+    uint64_t ambiguous = 0;
+    void pack(const uint8_t* r, size_t n) { pack_clean(r, n, data, off, len); }
+    void append(const PackedStore& o) {
+        const uint64_t base = data.size();
+        data.insert(data.end(), o.data.begin(), o.data.end());
+        for (uint64_t x : o.off) off.push_back(x + base);
+        len.insert(len.end(), o.len.begin(), o.len.end());
+        ambiguous += o.ambiguous;
+    }
+};
+
+static bool ambig(uint8_t b) { return ascii_to_2bit(b) == 0xFF; }
+
+// trim_n_ends (packed_reads.rs:634-648): [s, e)
+static void trim_n_ends(const std::vector<uint8_t>& r, size_t& s, size_t& e) {
+    s = 0;
+    while (s < r.size() && ambig(r[s])) ++s;
+    e = r.size();
+    while (e > 0 && ambig(r[e - 1])) --e;
+    if (s >= e) s = e = 0;
+}
+
+// ingest_trimmed_unpaired (packed_reads.rs:513-535): split on N, pack segments >= MIN_K,
+// else the ambiguous sidecar
+static void ingest_trimmed(const uint8_t* t, size_t n, PackedStore& b3) {
+    const size_t MIN_K = 21;
+    if (n == 0) return;
+    bool any = false;
+    size_t i = 0;
+    while (i < n) {
+        while (i < n && ambig(t[i])) ++i;
+        size_t j = i;
+        while (j < n && !ambig(t[j])) ++j;
+        if (j > i && j - i >= MIN_K) {
+            b3.pack(t + i, j - i);
+            any = true;
+        }
+        i = j;
+    }
+    if (!any) b3.ambiguous += 1;
+}
+
+static void ingest_read(const std::vector<uint8_t>& r, PackedStore& b3) {
+    size_t s, e;
+    trim_n_ends(r, s, e);
+    ingest_trimmed(r.data() + s, e - s, b3);
+}
+
+// partition_pair_streaming (packed_reads.rs:654-680)
+static void partition_pair(const std::vector<uint8_t>& r1, const std::vector<uint8_t>& r2, PackedStore& b1,
+                           PackedStore& b2, PackedStore& b3) {
+    const size_t MIN_K = 21;
+    size_t s1, e1, s2, e2;
+    trim_n_ends(r1, s1, e1);
+    trim_n_ends(r2, s2, e2);
+    bool n1 = false, n2 = false;
+    for (size_t i = s1; i < e1; ++i) n1 = n1 || ambig(r1[i]);
+    for (size_t i = s2; i < e2; ++i) n2 = n2 || ambig(r2[i]);
+    if (!n1 && !n2 && e1 - s1 >= MIN_K && e2 - s2 >= MIN_K) {
+        b1.pack(r1.data() + s1, e1 - s1);
+        b2.pack(r2.data() + s2, e2 - s2);
+    } else {
+        ingest_trimmed(r1.data() + s1, e1 - s1, b3);
+        ingest_trimmed(r2.data() + s2, e2 - s2, b3);
+    }
+}
+
+struct PackResult {
+    PackedStore st;
+    uint64_t pe_count = 0, rec1 = 0, rec2 = 0;
+    std::string err;
+};
+
+// from_paired_fastq (packed_reads.rs:325-380) over two in-memory files; with r2 == NULL,
+// append_from_file (packed_reads.rs:490-501) of one file (every read through ingest_read).
+static PackResult* pack_fastq(const uint8_t* r1, size_t n1, const uint8_t* r2, size_t n2) {
+    auto* res = new PackResult();
+    MemFastx a(r1, n1);
+    if (!a.err.empty()) {
+        res->err = a.err;
+        return res;
+    }
+    std::vector<uint8_t> s1, s2;
+    if (!r2) {
+        for (;;) {
+            const int h = a.next_seq(s1);
+            if (h < 0) {
+                res->err = a.err;
+                return res;
+            }
+            if (h == 0) break;
+            ++res->rec1;
+            ingest_read(s1, res->st);
+        }
+        return res;
+    }
+    MemFastx b(r2, n2);
+    if (!b.err.empty()) {
+        res->err = b.err;
+        return res;
+    }
+    PackedStore b1, b2, b3;
+    for (;;) {
+        const int h1 = a.next_seq(s1);
+        const int h2 = b.next_seq(s2);
+        if (h1 < 0 || h2 < 0) {
+            res->err = h1 < 0 ? a.err : b.err;
+            return res;
+        }
+        res->rec1 += h1;
+        res->rec2 += h2;
+        if (h1 && h2) partition_pair(s1, s2, b1, b2, b3);
+        else if (h1) ingest_read(s1, b3);
+        else if (h2) ingest_read(s2, b3);
+        else break;
+    }
+    res->pe_count = b1.len.size();
+    res->st = b1;
+    res->st.append(b2);
+    res->st.append(b3);
+    return res;
+}
+
+// ---------------------------------------------------------------------------
 // assemble.rs:692-735 determine_solid_kmer_cutoff
 // ---------------------------------------------------------------------------
 // Rust's `f as u32`: truncation toward zero, saturating at both ends, NaN -> 0.
@@ -705,6 +944,26 @@ void* or_spectrum_run_priors(const uint8_t* data, const uint64_t* offsets, const
                         prior_starts, prior_lens, n_priors);
 }
 uint32_t or_prior_boost(double min_cov) { return prior_boost_of(min_cov); }
+
+void* or_pack_fastq(const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint64_t n2) { return pack_fastq(r1, n1, r2, n2); }
+void or_pack_free(void* h) { delete (PackResult*)h; }
+// sizes[0..5] = n_reads, data_len, pe_count, ambiguous, r1 records, r2 records; returns 0 or -1 (malformed)
+int or_pack_sizes(void* h, uint64_t* sizes) {
+    auto* r = (PackResult*)h;
+    sizes[0] = r->st.len.size();
+    sizes[1] = r->st.data.size();
+    sizes[2] = r->pe_count;
+    sizes[3] = r->st.ambiguous;
+    sizes[4] = r->rec1;
+    sizes[5] = r->rec2;
+    return r->err.empty() ? 0 : -1;
+}
+void or_pack_export(void* h, uint8_t* data, uint64_t* offsets, uint32_t* lengths) {
+    auto* r = (PackResult*)h;
+    if (!r->st.data.empty()) std::memcpy(data, r->st.data.data(), r->st.data.size());
+    if (!r->st.off.empty()) std::memcpy(offsets, r->st.off.data(), r->st.off.size() * 8);
+    if (!r->st.len.empty()) std::memcpy(lengths, r->st.len.data(), r->st.len.size() * 4);
+}
 void or_spectrum_free(void* h) { delete (Spectrum*)h; }
 uint64_t or_spectrum_num_words(void* h) { return ((Spectrum*)h)->num_words; }
 uint64_t or_spectrum_num_counts(void* h) { return ((Spectrum*)h)->keys.size(); }

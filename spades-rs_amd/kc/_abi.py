@@ -34,6 +34,7 @@ EXPORTS = (
     "kc_mlp_create", "kc_mlp_destroy", "kc_mlp_last_error", "kc_mlp_num_params", "kc_mlp_get_params",
     "kc_mlp_set_params", "kc_mlp_forward", "kc_mlp_gradient", "kc_mlp_train",
     "kc_library_quals_set", "kc_library_features", "kc_library_learned_select",
+    "kc_fastq_pack", "kc_fastq_last_error", "kc_packed_info_get", "kc_packed_export", "kc_packed_destroy",
 )
 
 
@@ -54,6 +55,14 @@ class kc_info(ctypes.Structure):
                 ("n_counts", ctypes.c_uint64), ("n_solid", ctypes.c_uint64), ("corrected_reads", ctypes.c_uint64),
                 ("num_words", ctypes.c_uint64), ("table_capacity", ctypes.c_uint64), ("cutoff", ctypes.c_uint32),
                 ("top_cov", ctypes.c_uint32), ("ms_stage2", ctypes.c_double), ("ms_stage3", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class kc_packed_info(ctypes.Structure):
+    _fields_ = [("n_reads", ctypes.c_uint64), ("data_len", ctypes.c_uint64), ("pe_count", ctypes.c_uint64),
+                ("n_ambiguous", ctypes.c_uint64), ("r1_records", ctypes.c_uint64), ("r2_records", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -126,6 +135,11 @@ _SIGS = {
     "kc_library_quals_set": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz, _u64p]),
     "kc_library_features": (ctypes.c_int, [_vp, ctypes.c_uint32, _f32p, _sz]),
     "kc_library_learned_select": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, ctypes.c_float, _u8p, _u8p, _sz, _dblp]),
+    "kc_fastq_pack": (ctypes.c_int, [ctypes.c_int32, _u8p, _sz, _u8p, _sz, ctypes.POINTER(_vp)]),
+    "kc_fastq_last_error": (ctypes.c_char_p, []),
+    "kc_packed_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(kc_packed_info)]),
+    "kc_packed_export": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u32p, _sz]),
+    "kc_packed_destroy": (None, [_vp]),
 }
 
 _lib = None

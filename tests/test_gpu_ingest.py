@@ -1,0 +1,82 @@
+"""GPU: FASTQ ingestion on the device against the oracle's restatement of
+from_paired_fastq / append_from_file (FastxReader, the three N gates, the three-block
+layout): bit-exact data, offsets, lengths, paired-block size and sidecar count."""
+import gzip
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from kc import KcError, _abi
+from kc.ingest import fastq_pack
+from test_oracle_fastq import fastq, random_reads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check(oracle, a, b=None):
+    data, offs, lens, pe, amb, rec1, rec2 = oracle.pack_fastq(a, b)
+    store, info = fastq_pack(a, b)
+    d, o, l = store.arrays()
+    assert info["r1_records"] == rec1 and info["r2_records"] == rec2
+    assert info["pe_count"] == pe and info["n_ambiguous"] == amb
+    assert np.array_equal(d, data) and np.array_equal(o, offs) and np.array_equal(l, lens)
+    return info
+
+
+@pytest.mark.parametrize("seed,crlf,lead,unequal,n", [(1, False, False, 0, 500), (2, True, True, 0, 800),
+                                                       (3, False, False, 9, 300), (4, True, False, -4, 1000),
+                                                       (5, False, True, 0, 0), (6, False, False, 0, 1)])
+def test_pairs_match_oracle(oracle, seed, crlf, lead, unequal, n):
+    rng = np.random.default_rng(seed)
+    r1 = random_reads(rng, n)
+    r2 = random_reads(rng, max(n + unequal, 0))
+    a = fastq(r1, crlf, False, lead) + (b"\n\n" if seed % 2 else b"")  # trailing blank lines
+    b = fastq(r2, crlf, False, lead)
+    say(f"case ingest seed={seed} n={n}")
+    info = check(oracle, a, b)
+    assert info["r1_records"] == len(r1)
+
+
+def test_single_file_and_no_final_newline(oracle):
+    rng = np.random.default_rng(8)
+    a = fastq(random_reads(rng, 700))
+    check(oracle, a)
+    check(oracle, a[:-1])  # the last quality line without its newline
+
+
+def test_ecoli_fixture(oracle):
+    from conftest import GOLDEN
+    a = gzip.open(os.path.join(GOLDEN, "ecoli_1K_1.fq.gz"), "rb").read()
+    b = gzip.open(os.path.join(GOLDEN, "ecoli_1K_2.fq.gz"), "rb").read()
+    info = check(oracle, a, b)
+    assert info["pe_count"] > 0
+
+
+def test_large_synthetic_pairs(oracle):
+    rng = np.random.default_rng(11)
+    n = 60000
+    r1 = [bytes(x) for x in rng.choice(list(b"ACGT"), (n, 150)).astype(np.uint8)]
+    r2 = [bytes(x) for x in rng.choice(list(b"ACGT"), (n, 150)).astype(np.uint8)]
+    for i in range(0, n, 97):
+        r1[i] = r1[i][:70] + b"N" + r1[i][71:]
+    check(oracle, fastq(r1), fastq(r2))
+
+
+def test_unsupported_and_malformed():
+    with pytest.raises(KcError) as e:
+        fastq_pack(fastq([b"ACGT" * 10, b"GGGG" * 10], blank_between=True), fastq([b"ACGT" * 10, b"TTTT" * 10]))
+    assert e.value.status == _abi.KC_EUNSUPPORTED
+    with pytest.raises(KcError) as e:
+        fastq_pack(b">c\nACGT\n", None)
+    assert e.value.status == _abi.KC_EUNSUPPORTED
+    for bad in (b"@r\nACGT\n+\nII\n", b"@r\nACGT\n-\nIIII\n", b"xr\nACGT\n+\nIIII\n", b"@r\nACGT\n+\n"):
+        with pytest.raises(KcError) as e:
+            fastq_pack(bad, fastq([b"ACGT"]))
+        assert e.value.status == _abi.KC_EINVAL, bad

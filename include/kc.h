@@ -182,9 +182,22 @@ typedef struct kc_packed_info {
     uint64_t n_reads, data_len, pe_count; /* the PackedReads arrays and its paired block size */
     uint64_t n_ambiguous;                 /* reads routed to the ambiguous sidecar (not packed) */
     uint64_t r1_records, r2_records;      /* records read from each file */
+    double ms_device;                     /* device time from the end of the upload to the end of the
+                                             last kernel (includes the host reads of sizes between
+                                             kernels, not the copies in and out) */
 } kc_packed_info;
+/* One-shot form: a context for the call only. */
 int kc_fastq_pack(int32_t device, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
                   kc_packed** out);
+/* A context keeps its stream and device buffers between calls (they only grow), so that
+ * repeated calls allocate nothing once the largest input has been seen.  One thread at a
+ * time per context. */
+typedef struct kc_ingest kc_ingest;
+int kc_ingest_create(int32_t device, kc_ingest** out);
+void kc_ingest_destroy(kc_ingest* ctx);
+int kc_ingest_pack(kc_ingest* ctx, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
+                   kc_packed** out);
+/* The message of the calling thread's last failed ingestion call. */
 const char* kc_fastq_last_error(void);
 int kc_packed_info_get(const kc_packed* p, kc_packed_info* info);
 int kc_packed_export(const kc_packed* p, uint8_t* data, size_t data_len, uint64_t* offsets, uint32_t* lengths,

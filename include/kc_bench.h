@@ -17,6 +17,8 @@ extern "C" {
 #define KC_BENCH_ATOMIC_OR_U64 1  /* non-returning 64-bit atomicOr at a random word */
 #define KC_BENCH_READ_64B 2       /* plain 8-byte load at a random word (one 64 B access) */
 #define KC_BENCH_COPY 3           /* copy array_bytes to a second buffer (n_ops unused) */
+#define KC_BENCH_ATOMIC_ADD_U32_WG 4  /* as 0 at workgroup scope (executed in the XCD's L2) */
+#define KC_BENCH_ATOMIC_ADD_U32_RET 5 /* as 0 with the returned value used */
 
 /* Best of `reps` timed launches (after one warm-up) of n_ops operations on a fresh
  * array of array_bytes; *ms_out receives the time of one launch.  Returns 0, or -1 (bad

@@ -29,11 +29,17 @@ __global__ void k_random(uint64_t* __restrict__ a, uint64_t words, uint64_t n_op
                                          __HIP_MEMORY_SCOPE_AGENT);
         } else if (kind == KC_BENCH_ATOMIC_OR_U64) {
             (void)__hip_atomic_fetch_or(a + w, 1ull << (i & 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (kind == KC_BENCH_ATOMIC_ADD_U32_WG) {
+            (void)__hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(a + w), 1u, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (kind == KC_BENCH_ATOMIC_ADD_U32_RET) {
+            acc += __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(a + w), 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
         } else {
             acc += a[w];
         }
     }
-    if (kind == KC_BENCH_READ_64B && acc == 0x5A5A5A5A5A5A5A5Aull) *sink = acc;  // keeps the loads
+    if (acc == 0x5A5A5A5A5A5A5A5Aull) *sink = acc;  // keeps the loads and returned values
 }
 
 __global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
@@ -51,7 +57,7 @@ __global__ void k_zero(uint64_t* __restrict__ a, uint64_t n) {
 extern "C" int kc_bench_device(int32_t device, int kind, uint64_t array_bytes, uint64_t n_ops, int reps,
                                double* ms_out) {
     if (!ms_out || array_bytes < 64 || reps < 1) return -1;
-    if (kind < KC_BENCH_ATOMIC_ADD_U32 || kind > KC_BENCH_COPY) return -1;
+    if (kind < KC_BENCH_ATOMIC_ADD_U32 || kind > KC_BENCH_ATOMIC_ADD_U32_RET) return -1;
     if (hipSetDevice(device) != hipSuccess) return -3;
     const uint64_t words = array_bytes / 8;
     const bool copy = kind == KC_BENCH_COPY;

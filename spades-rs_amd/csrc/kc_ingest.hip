@@ -3,14 +3,22 @@
 // FastxReader's record checks (packed_reads.rs:141-217) and the three N gates
 // (packed_reads.rs:513-535, 630-690).
 //
-// Pipeline (one stream): newline positions by a scan; per-line flags; per-record checks;
-// per-read trimming / N scan / segment count; output units in three-block order (block 1
-// = clean pairs' R1, block 2 = their R2, block 3 = the other pairs' segments, then the
-// longer file's extra reads), whose read and byte bases come from exclusive scans; one
-// thread per input read writes its output reads' offsets, lengths and packed bytes.
-// The device handles the 4-line FASTQ layout (blank or whitespace-only lines allowed
-// before the first record and after the last); blank lines between records and FASTA
-// give KC_EUNSUPPORTED, for the host parser.
+// Pipeline (one stream, buffers kept in a kc_ingest context between calls):
+//   1. newline index: per 4 KiB tile a newline count (16 bytes per thread, SWAR byte
+//      compare), an exclusive scan over tiles, then each tile writes its newline
+//      positions in order (block scan of the per-thread counts);
+//   2. per line (block-reduced): first line that is not whitespace only, last non-empty
+//      line; per record (4 lines): the '@' / '+' / length checks;
+//   3. per read: gate 1 (terminal non-ACGT trim), the internal-N flag and the number and
+//      packed size of the >= 21-base ACGT segments, kept in a gate array;
+//   4. per pair: clean flag (gate 2), scan -> clean rank; per read: its output unit (block
+//      1 = clean pairs' R1, block 2 = their R2, block 3 = the other pairs' segments, then
+//      the longer file's extra reads) and that unit's reads and bytes; exclusive scans give
+//      each unit's first read and first byte;
+//   5. one thread per input read writes its output reads' offsets, lengths and packed bytes.
+// The device handles the 4-line FASTQ layout (blank or whitespace-only lines allowed before
+// the first record, blank lines after the last); blank lines between records and FASTA give
+// KC_EUNSUPPORTED, for the host parser.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,7 +52,9 @@ struct Buf : DevBuf {
     ~Buf() { release(); }
 };
 
-constexpr uint32_t MIN_K = 21;  // packed_reads.rs:38
+constexpr uint32_t MIN_K = 21;     // packed_reads.rs:38
+constexpr uint32_t TILE = BLOCK * 16;  // bytes per newline-index tile
+constexpr uint64_t FILE_PAD = 64;  // zero bytes after each uploaded file
 
 __device__ __forceinline__ uint32_t code2(uint8_t b) {  // dna.rs:11-30, 4 = not ACGT
     const uint32_t c = b | 0x20u;
@@ -55,15 +65,66 @@ __device__ __forceinline__ bool is_ws(uint8_t b) {
     return b == ' ' || b == '\t' || b == '\n' || b == '\r' || b == '\f' || b == 0x0B;
 }
 
-__global__ void k_nl_flags(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ f) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        f[i] = b[i] == '\n';
+// High bit of each byte of w set where that byte is '\n' (exact, no borrow between bytes).
+__device__ __forceinline__ uint32_t nl_bits(uint32_t w) {
+    const uint32_t x = w ^ 0x0A0A0A0Au;
+    const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return ~(t | x | 0x7F7F7F7Fu);
 }
 
-__global__ void k_nl_scatter(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ pre,
-                             unsigned long long* __restrict__ nl) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        if (b[i] == '\n') nl[pre[i]] = i;
+__device__ __forceinline__ uint32_t nl_count16(const uint4 v) {
+    return __popc(nl_bits(v.x)) + __popc(nl_bits(v.y)) + __popc(nl_bits(v.z)) + __popc(nl_bits(v.w));
+}
+
+// Exclusive block scan of one value per thread (BLOCK threads); returns the block total.
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint32_t* sh) {
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < BLOCK; d <<= 1) {
+        const uint32_t a = t >= d ? sh[t - d] : 0u;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    excl = sh[t] - v;
+    const uint32_t total = sh[BLOCK - 1];
+    __syncthreads();
+    return total;
+}
+
+// Per tile of TILE bytes: its newline count.  The file is zero-padded to a multiple of 16.
+__global__ __launch_bounds__(BLOCK) void k_nl_count(const uint8_t* __restrict__ b, uint64_t n,
+                                                    uint32_t* __restrict__ tile_cnt) {
+    __shared__ uint32_t sh[BLOCK];
+    const uint64_t at = (uint64_t)blockIdx.x * TILE + threadIdx.x * 16ull;
+    uint32_t c = 0;
+    if (at < n) c = nl_count16(*reinterpret_cast<const uint4*>(b + at));
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(c, excl, sh);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+}
+
+// Per tile: writes the positions of its newlines at nl[tile_base[tile] ..] in file order.
+__global__ __launch_bounds__(BLOCK) void k_nl_write(const uint8_t* __restrict__ b, uint64_t n,
+                                                    const unsigned long long* __restrict__ tile_base,
+                                                    unsigned long long* __restrict__ nl) {
+    __shared__ uint32_t sh[BLOCK];
+    const uint64_t at = (uint64_t)blockIdx.x * TILE + threadIdx.x * 16ull;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (at < n) v = *reinterpret_cast<const uint4*>(b + at);
+    uint32_t excl;
+    block_exclusive_scan(nl_count16(v), excl, sh);
+    uint64_t o = tile_base[blockIdx.x] + excl;
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t m = nl_bits(w[q]);
+        while (m) {
+            const uint32_t byte = (uint32_t)(__ffs(m) - 1) >> 3;
+            nl[o++] = at + 4u * q + byte;
+            m &= m - 1;
+        }
+    }
 }
 
 __device__ __forceinline__ void line_span(const unsigned long long* nl, uint64_t m, uint64_t n, uint64_t j,
@@ -72,10 +133,13 @@ __device__ __forceinline__ void line_span(const unsigned long long* nl, uint64_t
     e = j < m ? nl[j] : n;
 }
 
-// Per line: 1 = empty after stripping CR/LF, 2 = whitespace only.  red[0] = first line
-// that is not whitespace only (FastxReader::open), red[1] = last non-empty line + 1.
-__global__ void k_lines(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
-                        uint64_t m, uint64_t lines, unsigned long long* __restrict__ red) {
+// red[0] = first line that is not whitespace only (FastxReader::open), red[1] = last
+// non-empty line (after stripping CR) + 1.  One atomic per block.
+__global__ __launch_bounds__(BLOCK) void k_lines(const uint8_t* __restrict__ b, uint64_t n,
+                                                 const unsigned long long* __restrict__ nl, uint64_t m,
+                                                 uint64_t lines, unsigned long long* __restrict__ red) {
+    __shared__ unsigned long long lo_sh[BLOCK], hi_sh[BLOCK];
+    unsigned long long lo = ~0ull, hi = 0;
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < lines; j += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t s, e;
         line_span(nl, m, n, j, s, e);
@@ -83,8 +147,33 @@ __global__ void k_lines(const uint8_t* __restrict__ b, uint64_t n, const unsigne
         while (t > s && (b[t - 1] == '\r' || b[t - 1] == '\n')) --t;
         bool ws = true;
         for (uint64_t i = s; i < e && ws; ++i) ws = is_ws(b[i]);
-        if (!ws) atomicMin(&red[0], (unsigned long long)j);
-        if (t > s) atomicMax(&red[1], (unsigned long long)(j + 1));
+        if (!ws && j < lo) lo = j;
+        if (t > s && j + 1 > hi) hi = j + 1;
+    }
+    lo_sh[threadIdx.x] = lo;
+    hi_sh[threadIdx.x] = hi;
+    __syncthreads();
+    for (uint32_t d = BLOCK / 2; d; d >>= 1) {
+        if (threadIdx.x < d) {
+            lo_sh[threadIdx.x] = min(lo_sh[threadIdx.x], lo_sh[threadIdx.x + d]);
+            hi_sh[threadIdx.x] = max(hi_sh[threadIdx.x], hi_sh[threadIdx.x + d]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (lo_sh[0] != ~0ull) atomicMin(&red[0], lo_sh[0]);
+        if (hi_sh[0]) atomicMax(&red[1], hi_sh[0]);
+    }
+}
+
+// Empty lines (after stripping CR/LF) among lines [j0, j1).
+__global__ void k_count_empty(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
+                              uint64_t m, uint64_t j0, uint64_t j1, unsigned long long* __restrict__ count) {
+    for (uint64_t j = j0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < j1; j += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t s, e;
+        line_span(nl, m, n, j, s, e);
+        while (e > s && (b[e - 1] == '\r' || b[e - 1] == '\n')) --e;
+        if (e == s) atomicAdd(count, 1ull);
     }
 }
 
@@ -116,11 +205,10 @@ __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsig
     }
 }
 
-// Gate 1 and the N scan of one read: trimmed span [ts, te), internal-N flag, number of
+// Gate 1 and the N scan of one read: trimmed span [ts, ts + tl), internal-N flag, number of
 // ACGT segments of at least MIN_K bases and their packed bytes.
 struct ReadGate {
-    uint32_t ts, tl, nseg;
-    uint8_t internal_n;
+    uint32_t ts, tl, nseg, internal_n;
     uint64_t seg_bytes;
 };
 
@@ -148,48 +236,64 @@ __device__ ReadGate gate(const uint8_t* __restrict__ b, uint64_t s, uint32_t l) 
     return g;
 }
 
-// Per input read: its output unit and that unit's reads / bytes; pair clean flags.
-// Reads 0..P-1 of both files are pairs (when paired), the rest extras.  Unit numbering:
-// clean pair c: units c (R1) and n_clean + c (R2); non-clean pair d: 2 n_clean + 2d (+1);
-// extra e: 2P + e.  pass 0 computes clean flags; pass 1 (with the clean prefix) the rest.
-__global__ void k_units(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
+// Input read x: reads 0..R1-1 of the first file, then R1.. of the second.
+__global__ void k_gates(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
                         const uint32_t* __restrict__ l1, uint64_t R1, const uint8_t* __restrict__ b2,
-                        const uint64_t* __restrict__ s2, const uint32_t* __restrict__ l2, uint64_t R2, uint64_t P,
-                        int pass, uint32_t* __restrict__ clean, const unsigned long long* __restrict__ clean_pre,
+                        const uint64_t* __restrict__ s2, const uint32_t* __restrict__ l2, uint64_t R2,
+                        ReadGate* __restrict__ gates) {
+    const uint64_t total = R1 + R2;
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x)
+        gates[x] = x < R1 ? gate(b1, s1[x], l1[x]) : gate(b2, s2[x - R1], l2[x - R1]);
+}
+
+// Gate 2 of pair i (packed_reads.rs:340-352): both mates free of internal non-ACGT bases
+// and at least MIN_K long after trimming.
+__global__ void k_clean(const ReadGate* __restrict__ gates, uint64_t R1, uint64_t P, uint32_t* __restrict__ clean) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < P; i += (uint64_t)gridDim.x * blockDim.x) {
+        const ReadGate g = gates[i], h = gates[R1 + i];
+        clean[i] = !g.internal_n && !h.internal_n && g.tl >= MIN_K && h.tl >= MIN_K;
+    }
+}
+
+// Output unit of input read x.  Reads 0..P-1 of both files are pairs (when paired), the
+// rest extras.  Clean pair c: units c (R1) and n_clean + c (R2); other pair d:
+// 2 n_clean + 2d (+1 for R2); extra e: 2P + e (only one file has extras).
+__device__ __forceinline__ uint64_t unit_of(uint64_t x, uint64_t R1, uint64_t P, const uint32_t* clean,
+                                            const unsigned long long* clean_pre, uint64_t n_clean, bool& whole) {
+    const bool second = x >= R1;
+    const uint64_t i = second ? x - R1 : x;
+    whole = false;
+    if (i < P) {
+        const uint64_t ci = clean_pre[i];
+        if (clean[i]) {
+            whole = true;
+            return second ? n_clean + ci : ci;
+        }
+        return 2 * n_clean + 2 * (i - ci) + (second ? 1 : 0);
+    }
+    return 2 * P + (second ? (i - P) + (R1 - P) : i - P);
+}
+
+__global__ void k_units(const ReadGate* __restrict__ gates, uint64_t R1, uint64_t R2, uint64_t P,
+                        const uint32_t* __restrict__ clean, const unsigned long long* __restrict__ clean_pre,
                         uint64_t n_clean, unsigned long long* __restrict__ unit_reads,
                         unsigned long long* __restrict__ unit_bytes, unsigned long long* __restrict__ ambiguous) {
     const uint64_t total = R1 + R2;
+    uint32_t amb = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
-        const bool second = x >= R1;
-        const uint64_t i = second ? x - R1 : x;
-        const ReadGate g = second ? gate(b2, s2[i], l2[i]) : gate(b1, s1[i], l1[i]);
-        if (i < P) {
-            const ReadGate h = second ? gate(b1, s1[i], l1[i]) : gate(b2, s2[i], l2[i]);
-            const bool c = !g.internal_n && !h.internal_n && g.tl >= MIN_K && h.tl >= MIN_K;
-            if (pass == 0) {
-                if (!second) clean[i] = c;
-                continue;
-            }
-            const uint64_t ci = clean_pre[i];
-            uint64_t u;
-            if (c) {
-                u = second ? n_clean + ci : ci;
-                unit_reads[u] = 1;
-                unit_bytes[u] = (g.tl + 3) / 4;
-            } else {
-                u = 2 * n_clean + 2 * (i - ci) + (second ? 1 : 0);
-                unit_reads[u] = g.nseg;
-                unit_bytes[u] = g.seg_bytes;
-                if (g.tl && !g.nseg) atomicAdd(ambiguous, 1ull);
-            }
-        } else if (pass == 1) {
-            const uint64_t e = second ? (i - P) + (R1 - P) : i - P;  // one file has no extras
-            const uint64_t u = 2 * P + e;
+        const ReadGate g = gates[x];
+        bool whole;
+        const uint64_t u = unit_of(x, R1, P, clean, clean_pre, n_clean, whole);
+        if (whole) {
+            unit_reads[u] = 1;
+            unit_bytes[u] = (g.tl + 3) / 4;
+        } else {
             unit_reads[u] = g.nseg;
             unit_bytes[u] = g.seg_bytes;
-            if (g.tl && !g.nseg) atomicAdd(ambiguous, 1ull);
+            amb += g.tl && !g.nseg;  // gate 3: nothing of the read reaches MIN_K
         }
     }
+    if (amb) atomicAdd(ambiguous, (unsigned long long)amb);
 }
 
 __device__ void pack_run(const uint8_t* __restrict__ b, uint64_t s, uint32_t l, uint8_t* __restrict__ out) {
@@ -202,34 +306,21 @@ __device__ void pack_run(const uint8_t* __restrict__ b, uint64_t s, uint32_t l, 
 }
 
 // Writes every input read's output reads (offsets, lengths, packed bytes).
-__global__ void k_emit(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
-                       const uint32_t* __restrict__ l1, uint64_t R1, const uint8_t* __restrict__ b2,
-                       const uint64_t* __restrict__ s2, const uint32_t* __restrict__ l2, uint64_t R2, uint64_t P,
-                       const uint32_t* __restrict__ clean, const unsigned long long* __restrict__ clean_pre,
-                       uint64_t n_clean, const unsigned long long* __restrict__ read_base,
+__global__ void k_emit(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1, uint64_t R1,
+                       const uint8_t* __restrict__ b2, const uint64_t* __restrict__ s2, uint64_t R2, uint64_t P,
+                       const ReadGate* __restrict__ gates, const uint32_t* __restrict__ clean,
+                       const unsigned long long* __restrict__ clean_pre, uint64_t n_clean,
+                       const unsigned long long* __restrict__ read_base,
                        const unsigned long long* __restrict__ byte_base, uint8_t* __restrict__ data,
                        uint64_t* __restrict__ offsets, uint32_t* __restrict__ lengths) {
     const uint64_t total = R1 + R2;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
         const bool second = x >= R1;
-        const uint64_t i = second ? x - R1 : x;
         const uint8_t* b = second ? b2 : b1;
-        const uint64_t s = second ? s2[i] : s1[i];
-        const uint32_t l = second ? l2[i] : l1[i];
-        const ReadGate g = gate(b, s, l);
-        uint64_t u;
-        bool whole = false;
-        if (i < P) {
-            const uint64_t ci = clean_pre[i];
-            if (clean[i]) {
-                u = second ? n_clean + ci : ci;
-                whole = true;
-            } else {
-                u = 2 * n_clean + 2 * (i - ci) + (second ? 1 : 0);
-            }
-        } else {
-            u = 2 * P + (second ? (i - P) + (R1 - P) : i - P);
-        }
+        const uint64_t s = second ? s2[x - R1] : s1[x];
+        const ReadGate g = gates[x];
+        bool whole;
+        const uint64_t u = unit_of(x, R1, P, clean, clean_pre, n_clean, whole);
         uint64_t rb = read_base[u], bb = byte_base[u];
         if (whole) {
             offsets[rb] = bb;
@@ -255,21 +346,22 @@ __global__ void k_emit(const uint8_t* __restrict__ b1, const uint64_t* __restric
     }
 }
 
-// Empty lines (after stripping CR/LF) among lines [j0, j1).
-__global__ void k_count_empty(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
-                              uint64_t m, uint64_t j0, uint64_t j1, unsigned long long* __restrict__ count) {
-    for (uint64_t j = j0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < j1; j += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t s, e;
-        line_span(nl, m, n, j, s, e);
-        while (e > s && (b[e - 1] == '\r' || b[e - 1] == '\n')) --e;
-        if (e == s) atomicAdd(count, 1ull);
-    }
-}
-
 struct FileIndex {
-    Buf buf, flags, pre, nl, red, seq_s, seq_l, err;
-    uint64_t R = 0;
+    Buf buf, tiles, tile_base, nl, red, seq_s, seq_l, err;
+    uint64_t n = 0, R = 0;
 };
+
+}  // namespace
+
+struct kc_ingest {
+    int32_t device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    FileIndex A, B;
+    Buf gates, clean, clean_pre, ureads, ubytes, rbase, bbase, small, scan_tmp, data, off, len;
+};
+
+namespace {
 
 #define I_TRY(expr)                                                                     \
     do {                                                                                \
@@ -285,49 +377,65 @@ int fail(int code, const std::string& m) {
     return code;
 }
 
-// Uploads one file and finds its records' sequence spans.
-int index_file(const uint8_t* host, uint64_t n, hipStream_t st, FileIndex& F, const char* name) {
-    I_TRY(F.buf.ensure(n + 64));
-    if (n) I_TRY(hipMemcpyAsync(F.buf.p, host, n, hipMemcpyHostToDevice, st));
-    I_TRY(F.flags.ensure(n * 4 + 4));
-    I_TRY(F.pre.ensure(n * 8 + 8));
+// Exclusive scan of n values into out (64-bit sums), through the context's temp buffer.
+template <typename In>
+int scan_excl(kc_ingest* c, const In* in, unsigned long long* out, uint64_t n) {
+    if (!n) return KC_OK;
+    size_t tb = 0;
+    I_TRY(rocprim::exclusive_scan(nullptr, tb, in, out, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), c->st));
+    I_TRY(c->scan_tmp.ensure(tb));
+    I_TRY(rocprim::exclusive_scan(c->scan_tmp.p, tb, in, out, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
+                                  c->st));
+    return KC_OK;
+}
+
+int upload(kc_ingest* c, const uint8_t* host, uint64_t n, FileIndex& F) {
+    F.n = n;
+    I_TRY(F.buf.ensure(n + FILE_PAD));
+    if (n) I_TRY(hipMemcpyAsync(F.buf.p, host, n, hipMemcpyHostToDevice, c->st));
+    I_TRY(hipMemsetAsync(F.buf.as<uint8_t>() + n, 0, FILE_PAD, c->st));
+    return KC_OK;
+}
+
+// Finds one uploaded file's records' sequence spans.
+int index_file(kc_ingest* c, FileIndex& F, const char* name) {
+    hipStream_t st = c->st;
+    const uint64_t n = F.n;
+    const uint8_t* b = F.buf.as<uint8_t>();
     I_TRY(F.red.ensure(64));
     uint64_t m = 0;
+    const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (n) {
-        k_nl_flags<<<grid_for(n), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), n, F.flags.as<uint32_t>());
+        I_TRY(F.tiles.ensure(ntiles * 4 + 4));
+        I_TRY(F.tile_base.ensure(ntiles * 8 + 8));
+        k_nl_count<<<(uint32_t)ntiles, BLOCK, 0, st>>>(b, n, F.tiles.as<uint32_t>());
         I_TRY(hipGetLastError());
-        size_t tb = 0;
-        Buf tmp;
-        I_TRY(rocprim::exclusive_scan(nullptr, tb, F.flags.as<uint32_t>(), F.pre.as<unsigned long long>(), 0ull,
-                                      (size_t)n, rocprim::plus<unsigned long long>(), st));
-        I_TRY(tmp.ensure(tb));
-        I_TRY(rocprim::exclusive_scan(tmp.p, tb, F.flags.as<uint32_t>(), F.pre.as<unsigned long long>(), 0ull,
-                                      (size_t)n, rocprim::plus<unsigned long long>(), st));
-        uint64_t last_pre = 0;
-        uint32_t last_flag = 0;
-        I_TRY(hipMemcpyAsync(&last_pre, F.pre.as<unsigned long long>() + n - 1, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(&last_flag, F.flags.as<uint32_t>() + n - 1, 4, hipMemcpyDeviceToHost, st));
+        int rc = scan_excl(c, F.tiles.as<uint32_t>(), F.tile_base.as<unsigned long long>(), ntiles);
+        if (rc) return rc;
+        unsigned long long last_base = 0;
+        uint32_t last_cnt = 0;
+        I_TRY(hipMemcpyAsync(&last_base, F.tile_base.as<unsigned long long>() + ntiles - 1, 8, hipMemcpyDeviceToHost,
+                             st));
+        I_TRY(hipMemcpyAsync(&last_cnt, F.tiles.as<uint32_t>() + ntiles - 1, 4, hipMemcpyDeviceToHost, st));
         I_TRY(hipStreamSynchronize(st));
-        m = last_pre + last_flag;
+        m = last_base + last_cnt;
     }
     I_TRY(F.nl.ensure(m * 8 + 8));
     if (m) {
-        k_nl_scatter<<<grid_for(n), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), n, F.pre.as<unsigned long long>(),
-                                                    F.nl.as<unsigned long long>());
+        k_nl_write<<<(uint32_t)ntiles, BLOCK, 0, st>>>(b, n, F.tile_base.as<unsigned long long>(),
+                                                      F.nl.as<unsigned long long>());
         I_TRY(hipGetLastError());
     }
     // read_until semantics: a final line without a newline still counts
     uint8_t lastb = '\n';
-    if (n) {
-        I_TRY(hipMemcpyAsync(&lastb, F.buf.as<uint8_t>() + n - 1, 1, hipMemcpyDeviceToHost, st));
-        I_TRY(hipStreamSynchronize(st));
-    }
-    const uint64_t lines = m + (n && lastb != '\n' ? 1 : 0);
+    if (n) I_TRY(hipMemcpyAsync(&lastb, b + n - 1, 1, hipMemcpyDeviceToHost, st));
     unsigned long long red[2] = {~0ull, 0ull};
     I_TRY(hipMemcpyAsync(F.red.p, red, 16, hipMemcpyHostToDevice, st));
+    I_TRY(hipStreamSynchronize(st));
+    const uint64_t lines = m + (n && lastb != '\n' ? 1 : 0);
     if (lines) {
-        k_lines<<<grid_for(lines), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), n, F.nl.as<unsigned long long>(), m, lines,
-                                                   F.red.as<unsigned long long>());
+        k_lines<<<grid_for(lines, BLOCK, 4096), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, lines,
+                                                               F.red.as<unsigned long long>());
         I_TRY(hipGetLastError());
     }
     I_TRY(hipMemcpyAsync(red, F.red.p, 16, hipMemcpyDeviceToHost, st));
@@ -345,7 +453,7 @@ int index_file(const uint8_t* host, uint64_t n, hipStream_t st, FileIndex& F, co
             I_TRY(hipStreamSynchronize(st));
         }
         const uint64_t at = first ? pos + 1 : 0;
-        I_TRY(hipMemcpyAsync(&head, F.buf.as<uint8_t>() + at, 1, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&head, b + at, 1, hipMemcpyDeviceToHost, st));
         I_TRY(hipStreamSynchronize(st));
     }
     if (head == '>') return fail(KC_EUNSUPPORTED, std::string(name) + ": FASTA input is parsed on the host");
@@ -355,35 +463,33 @@ int index_file(const uint8_t* host, uint64_t n, hipStream_t st, FileIndex& F, co
     // none, any misalignment is a malformed record; with some, a misaligned file is left
     // to the host parser.
     unsigned long long empties = 0;
-    {
-        I_TRY(hipMemcpyAsync(F.red.p, &empties, 8, hipMemcpyHostToDevice, st));
-        k_count_empty<<<grid_for(end - first), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), n, F.nl.as<unsigned long long>(),
-                                                               m, first, end, F.red.as<unsigned long long>());
-        I_TRY(hipGetLastError());
-        I_TRY(hipMemcpyAsync(&empties, F.red.p, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipStreamSynchronize(st));
-        if ((end - first) % 4 != 0) {
-            if (empties) {
-                // empty sequence and quality lines are legal; blank separator lines are not
-                // handled on the device
-                return fail(KC_EUNSUPPORTED, std::string(name) +
-                                                 ": blank lines between records: parse it on the host");
-            }
-            return fail(KC_EINVAL, std::string(name) + ": malformed FASTQ: unexpected EOF inside a record");
+    I_TRY(hipMemcpyAsync(F.red.p, &empties, 8, hipMemcpyHostToDevice, st));
+    k_count_empty<<<grid_for(end - first), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, end,
+                                                           F.red.as<unsigned long long>());
+    I_TRY(hipGetLastError());
+    I_TRY(hipMemcpyAsync(&empties, F.red.p, 8, hipMemcpyDeviceToHost, st));
+    I_TRY(hipStreamSynchronize(st));
+    // The lines after `end` are all empty, so a last record whose sequence and quality are
+    // empty ends up to three lines later; the 4-line parse takes them when they exist.
+    const uint64_t end4 = first + (end - first + 3) / 4 * 4;
+    if (end4 > lines) {
+        if (empties) {
+            // empty sequence and quality lines are legal; blank separator lines are not
+            // handled on the device
+            return fail(KC_EUNSUPPORTED, std::string(name) + ": blank lines between records: parse it on the host");
         }
+        return fail(KC_EINVAL, std::string(name) + ": malformed FASTQ: unexpected EOF inside a record");
     }
-    const uint64_t R = (end - first) / 4;
+    const uint64_t R = (end4 - first) / 4;
     I_TRY(F.seq_s.ensure(R * 8 + 8));
     I_TRY(F.seq_l.ensure(R * 4 + 4));
     I_TRY(F.err.ensure(64));
     unsigned long long init[2] = {0ull, ~0ull};
     I_TRY(hipMemcpyAsync(F.err.p, init, 16, hipMemcpyHostToDevice, st));
-    if (R) {
-        k_records<<<grid_for(R), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), n, F.nl.as<unsigned long long>(), m, first, R,
-                                                 F.seq_s.as<uint64_t>(), F.seq_l.as<uint32_t>(),
-                                                 F.err.as<uint32_t>(), F.err.as<unsigned long long>() + 1);
-        I_TRY(hipGetLastError());
-    }
+    k_records<<<grid_for(R), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, R, F.seq_s.as<uint64_t>(),
+                                             F.seq_l.as<uint32_t>(), F.err.as<uint32_t>(),
+                                             F.err.as<unsigned long long>() + 1);
+    I_TRY(hipGetLastError());
     unsigned long long out[2];
     I_TRY(hipMemcpyAsync(out, F.err.p, 16, hipMemcpyDeviceToHost, st));
     I_TRY(hipStreamSynchronize(st));
@@ -399,122 +505,103 @@ int index_file(const uint8_t* host, uint64_t n, hipStream_t st, FileIndex& F, co
     return KC_OK;
 }
 
-int scan_u64(const unsigned long long* in, unsigned long long* out, uint64_t n, hipStream_t st) {
-    if (!n) return KC_OK;
-    size_t tb = 0;
-    Buf tmp;
-    I_TRY(rocprim::exclusive_scan(nullptr, tb, in, out, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), st));
-    I_TRY(tmp.ensure(tb));
-    I_TRY(rocprim::exclusive_scan(tmp.p, tb, in, out, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), st));
-    I_TRY(hipStreamSynchronize(st));
-    return KC_OK;
-}
-
-int ingest(int32_t device, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint64_t n2, kc_packed* out) {
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(KC_EHIP, "no GPU");
-    if (device < 0 || device >= ndev) return fail(KC_EINVAL, "bad device");
-    I_TRY(hipSetDevice(device));
-    hipStream_t st = nullptr;
-    I_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    FileIndex A, B;
-    int rc = index_file(r1, n1, st, A, "R1");
-    if (rc) return rc;
+int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint64_t n2, kc_packed* out) {
+    I_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->st;
     const bool paired = r2 != nullptr;
-    if (paired) {
-        rc = index_file(r2, n2, st, B, "R2");
-        if (rc) return rc;
-    }
-    const uint64_t R1 = A.R, R2 = paired ? B.R : 0;
+    int rc = upload(c, r1, n1, c->A);
+    if (!rc && paired) rc = upload(c, r2, n2, c->B);
+    if (rc) return rc;
+    I_TRY(hipEventRecord(c->ev[0], st));
+    rc = index_file(c, c->A, "R1");
+    if (!rc && paired) rc = index_file(c, c->B, "R2");
+    if (rc) return rc;
+    FileIndex& A = c->A;
+    FileIndex& B = paired ? c->B : c->A;
+    const uint64_t R1 = A.R, R2 = paired ? c->B.R : 0;
     const uint64_t P = paired ? std::min(R1, R2) : 0;
-    const uint64_t E = R1 + R2 - 2 * P;
-    const uint64_t U = 2 * P + E;
-    Buf clean, clean_pre, ureads, ubytes, rbase, bbase, amb;
-    I_TRY(clean.ensure(P * 4 + 4));
-    I_TRY(clean_pre.ensure(P * 8 + 8));
-    I_TRY(ureads.ensure(U * 8 + 8));
-    I_TRY(ubytes.ensure(U * 8 + 8));
-    I_TRY(rbase.ensure(U * 8 + 8));
-    I_TRY(bbase.ensure(U * 8 + 8));
-    I_TRY(amb.ensure(8));
-    const unsigned long long zero = 0;
-    I_TRY(hipMemcpyAsync(amb.p, &zero, 8, hipMemcpyHostToDevice, st));
-    const uint8_t* b1 = A.buf.as<uint8_t>();
-    const uint8_t* b2 = paired ? B.buf.as<uint8_t>() : b1;
-    const uint64_t* s2 = paired ? B.seq_s.as<uint64_t>() : A.seq_s.as<uint64_t>();
-    const uint32_t* l2 = paired ? B.seq_l.as<uint32_t>() : A.seq_l.as<uint32_t>();
     const uint64_t tot = R1 + R2;
+    const uint64_t U = tot;  // one output unit per input read
+    I_TRY(c->gates.ensure(tot * sizeof(ReadGate) + sizeof(ReadGate)));
+    I_TRY(c->clean.ensure(P * 4 + 4));
+    I_TRY(c->clean_pre.ensure(P * 8 + 8));
+    I_TRY(c->ureads.ensure(U * 8 + 8));
+    I_TRY(c->ubytes.ensure(U * 8 + 8));
+    I_TRY(c->rbase.ensure(U * 8 + 8));
+    I_TRY(c->bbase.ensure(U * 8 + 8));
+    I_TRY(c->small.ensure(64));
+    const unsigned long long zero = 0;
+    I_TRY(hipMemcpyAsync(c->small.p, &zero, 8, hipMemcpyHostToDevice, st));
+    const uint8_t* b1 = A.buf.as<uint8_t>();
+    const uint8_t* b2 = B.buf.as<uint8_t>();
+    ReadGate* gates = c->gates.as<ReadGate>();
+    if (tot) {
+        k_gates<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,
+                                               B.seq_s.as<uint64_t>(), B.seq_l.as<uint32_t>(), R2, gates);
+        I_TRY(hipGetLastError());
+    }
     uint64_t n_clean = 0;
     if (P) {
-        k_units<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2, s2, l2, R2,
-                                               P, 0, clean.as<uint32_t>(), nullptr, 0, nullptr, nullptr, nullptr);
+        k_clean<<<grid_for(P), BLOCK, 0, st>>>(gates, R1, P, c->clean.as<uint32_t>());
         I_TRY(hipGetLastError());
-        size_t tb = 0;
-        Buf tmp;
-        I_TRY(rocprim::exclusive_scan(nullptr, tb, clean.as<uint32_t>(), clean_pre.as<unsigned long long>(), 0ull,
-                                      (size_t)P, rocprim::plus<unsigned long long>(), st));
-        I_TRY(tmp.ensure(tb));
-        I_TRY(rocprim::exclusive_scan(tmp.p, tb, clean.as<uint32_t>(), clean_pre.as<unsigned long long>(), 0ull,
-                                      (size_t)P, rocprim::plus<unsigned long long>(), st));
+        rc = scan_excl(c, c->clean.as<uint32_t>(), c->clean_pre.as<unsigned long long>(), P);
+        if (rc) return rc;
         unsigned long long lp = 0;
         uint32_t lc = 0;
-        I_TRY(hipMemcpyAsync(&lp, clean_pre.as<unsigned long long>() + P - 1, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(&lc, clean.as<uint32_t>() + P - 1, 4, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&lp, c->clean_pre.as<unsigned long long>() + P - 1, 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&lc, c->clean.as<uint32_t>() + P - 1, 4, hipMemcpyDeviceToHost, st));
         I_TRY(hipStreamSynchronize(st));
         n_clean = lp + lc;
     }
     if (tot) {
-        k_units<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2, s2, l2, R2,
-                                               P, 1, clean.as<uint32_t>(), clean_pre.as<unsigned long long>(), n_clean,
-                                               ureads.as<unsigned long long>(), ubytes.as<unsigned long long>(),
-                                               amb.as<unsigned long long>());
+        k_units<<<grid_for(tot), BLOCK, 0, st>>>(gates, R1, R2, P, c->clean.as<uint32_t>(),
+                                               c->clean_pre.as<unsigned long long>(), n_clean,
+                                               c->ureads.as<unsigned long long>(), c->ubytes.as<unsigned long long>(),
+                                               c->small.as<unsigned long long>());
         I_TRY(hipGetLastError());
     }
-    rc = scan_u64(ureads.as<unsigned long long>(), rbase.as<unsigned long long>(), U, st);
+    rc = scan_excl(c, c->ureads.as<unsigned long long>(), c->rbase.as<unsigned long long>(), U);
+    if (!rc) rc = scan_excl(c, c->ubytes.as<unsigned long long>(), c->bbase.as<unsigned long long>(), U);
     if (rc) return rc;
-    rc = scan_u64(ubytes.as<unsigned long long>(), bbase.as<unsigned long long>(), U, st);
-    if (rc) return rc;
-    unsigned long long tail[4] = {0, 0, 0, 0};
+    unsigned long long tail[4] = {0, 0, 0, 0}, ambiguous = 0;
     if (U) {
-        I_TRY(hipMemcpyAsync(&tail[0], rbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(&tail[1], ureads.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(&tail[2], bbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(&tail[3], ubytes.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&tail[0], c->rbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&tail[1], c->ureads.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&tail[2], c->bbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(&tail[3], c->ubytes.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
     }
-    unsigned long long ambiguous = 0;
-    I_TRY(hipMemcpyAsync(&ambiguous, amb.p, 8, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&ambiguous, c->small.p, 8, hipMemcpyDeviceToHost, st));
     I_TRY(hipStreamSynchronize(st));
     const uint64_t n_reads = tail[0] + tail[1], data_len = tail[2] + tail[3];
-    Buf d_data, d_off, d_len;
-    I_TRY(d_data.ensure(data_len + 8));
-    I_TRY(d_off.ensure(n_reads * 8 + 8));
-    I_TRY(d_len.ensure(n_reads * 4 + 4));
+    I_TRY(c->data.ensure(data_len + 8));
+    I_TRY(c->off.ensure(n_reads * 8 + 8));
+    I_TRY(c->len.ensure(n_reads * 4 + 4));
     if (tot) {
-        k_emit<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2, s2, l2, R2,
-                                              P, clean.as<uint32_t>(), clean_pre.as<unsigned long long>(), n_clean,
-                                              rbase.as<unsigned long long>(), bbase.as<unsigned long long>(),
-                                              d_data.as<uint8_t>(), d_off.as<uint64_t>(), d_len.as<uint32_t>());
+        k_emit<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), R1, b2, B.seq_s.as<uint64_t>(), R2, P, gates,
+                                              c->clean.as<uint32_t>(), c->clean_pre.as<unsigned long long>(), n_clean,
+                                              c->rbase.as<unsigned long long>(), c->bbase.as<unsigned long long>(),
+                                              c->data.as<uint8_t>(), c->off.as<uint64_t>(), c->len.as<uint32_t>());
         I_TRY(hipGetLastError());
     }
+    I_TRY(hipEventRecord(c->ev[1], st));
     out->data.resize(data_len);
     out->off.resize(n_reads);
     out->len.resize(n_reads);
-    if (data_len) I_TRY(hipMemcpyAsync(out->data.data(), d_data.p, data_len, hipMemcpyDeviceToHost, st));
+    if (data_len) I_TRY(hipMemcpyAsync(out->data.data(), c->data.p, data_len, hipMemcpyDeviceToHost, st));
     if (n_reads) {
-        I_TRY(hipMemcpyAsync(out->off.data(), d_off.p, n_reads * 8, hipMemcpyDeviceToHost, st));
-        I_TRY(hipMemcpyAsync(out->len.data(), d_len.p, n_reads * 4, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(out->off.data(), c->off.p, n_reads * 8, hipMemcpyDeviceToHost, st));
+        I_TRY(hipMemcpyAsync(out->len.data(), c->len.p, n_reads * 4, hipMemcpyDeviceToHost, st));
     }
     I_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    I_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     out->info.n_reads = n_reads;
     out->info.data_len = data_len;
     out->info.pe_count = n_clean;
     out->info.n_ambiguous = ambiguous;
     out->info.r1_records = R1;
     out->info.r2_records = R2;
+    out->info.ms_device = ms;
     return KC_OK;
 }
 
@@ -522,20 +609,62 @@ int ingest(int32_t device, const uint8_t* r1, uint64_t n1, const uint8_t* r2, ui
 
 extern "C" {
 
-int kc_fastq_pack(int32_t device, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
-                  kc_packed** out) {
-    if (!out) return KC_EINVAL;
+int kc_ingest_create(int32_t device, kc_ingest** out) {
+    if (!out) return fail(KC_EINVAL, "null out");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(KC_EHIP, "no GPU");
+    if (device < 0 || device >= ndev) return fail(KC_EINVAL, "bad device");
+    I_TRY(hipSetDevice(device));
+    auto* c = new kc_ingest();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
+    if (e != hipSuccess) {
+        kc_ingest_destroy(c);
+        return fail(KC_EHIP, hip_msg("kc_ingest_create", e));
+    }
+    *out = c;
+    return KC_OK;
+}
+
+void kc_ingest_destroy(kc_ingest* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+int kc_ingest_pack(kc_ingest* c, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
+                   kc_packed** out) {
+    if (!c || !out) return fail(KC_EINVAL, "null argument");
     *out = nullptr;
     if (r1_len && !r1) return fail(KC_EINVAL, "null r1");
     if (r2 == nullptr && r2_len) return fail(KC_EINVAL, "null r2");
     auto* p = new kc_packed();
-    const int rc = ingest(device, r1, r1_len, r2, r2_len, p);
+    const int rc = ingest(c, r1, r1_len, r2, r2_len, p);
     if (rc) {
         delete p;
         return rc;
     }
     *out = p;
     return KC_OK;
+}
+
+int kc_fastq_pack(int32_t device, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
+                  kc_packed** out) {
+    if (!out) return fail(KC_EINVAL, "null out");
+    *out = nullptr;
+    kc_ingest* c = nullptr;
+    int rc = kc_ingest_create(device, &c);
+    if (rc) return rc;
+    rc = kc_ingest_pack(c, r1, r1_len, r2, r2_len, out);
+    kc_ingest_destroy(c);
+    return rc;
 }
 
 const char* kc_fastq_last_error(void) { return g_ingest_err.c_str(); }

@@ -1,5 +1,5 @@
 """FASTQ ingestion on the GPU (packed_reads.rs from_paired_fastq / append_from_file):
-file contents -> the three-block PackedReads layout, through libkc.so's kc_fastq_pack."""
+file contents -> the three-block PackedReads layout, through libkc.so's kc_ingest_pack."""
 import ctypes
 import gzip
 
@@ -19,20 +19,50 @@ def _contents(x):
         return fh.read()
 
 
-def fastq_pack(r1, r2=None, device=0):
-    """Pack FASTQ content (bytes or a path, .gz decompressed on the host) on the device.
-    Returns (PackedReads, info dict).  Raises KcError: KC_EUNSUPPORTED for FASTA input
-    or blank lines between records (use PackedReads.from_paired_fastq), KC_EINVAL for a
-    malformed record."""
-    a = _contents(r1)
-    b = None if r2 is None else _contents(r2)
-    aa = np.frombuffer(a + b"\0", dtype=np.uint8)
-    bb = None if b is None else np.frombuffer(b + b"\0", dtype=np.uint8)
-    h = ctypes.c_void_p()
-    st = lib().kc_fastq_pack(device, ptr(aa, u8p), len(a), ptr(bb, u8p) if bb is not None else None,
-                             0 if b is None else len(b), ctypes.byref(h))
-    if st != _abi.KC_OK:
-        raise KcError(st, lib().kc_fastq_last_error().decode())
+def _u8(b):
+    """A uint8 view of b with one byte after it, so that an empty input has an address."""
+    if isinstance(b, np.ndarray):
+        return b
+    return np.frombuffer(b + b"\0", dtype=np.uint8)
+
+
+class FastqIngest:
+    """A device context for repeated ingestion (its buffers grow to the largest input and
+    are reused).  pack(r1, r2=None) takes bytes, uint8 arrays or paths (.gz decompressed
+    on the host) and returns (PackedReads, info dict).  Raises KcError: KC_EUNSUPPORTED for
+    FASTA input or blank lines between records (PackedReads.from_paired_fastq parses
+    those), KC_EINVAL for a malformed record."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        st = lib().kc_ingest_create(device, ctypes.byref(self._h))
+        if st != _abi.KC_OK:
+            raise KcError(st, lib().kc_fastq_last_error().decode())
+
+    def pack(self, r1, r2=None):
+        a = r1 if isinstance(r1, np.ndarray) else _contents(r1)
+        b = None if r2 is None else (r2 if isinstance(r2, np.ndarray) else _contents(r2))
+        aa, bb = _u8(a), None if b is None else _u8(b)
+        h = ctypes.c_void_p()
+        st = lib().kc_ingest_pack(self._h, ptr(aa, u8p), len(a), None if bb is None else ptr(bb, u8p),
+                                  0 if b is None else len(b), ctypes.byref(h))
+        if st != _abi.KC_OK:
+            raise KcError(st, lib().kc_fastq_last_error().decode())
+        return _take(h)
+
+    def close(self):
+        if self._h:
+            lib().kc_ingest_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _take(h):
     try:
         info = _abi.kc_packed_info()
         lib().kc_packed_info_get(h, ctypes.byref(info))
@@ -47,3 +77,16 @@ def fastq_pack(r1, r2=None, device=0):
         return store, info.as_dict()
     finally:
         lib().kc_packed_destroy(h)
+
+
+def fastq_pack(r1, r2=None, device=0):
+    """One-shot form of FastqIngest(device).pack(r1, r2) (kc_fastq_pack)."""
+    a = _contents(r1)
+    b = None if r2 is None else _contents(r2)
+    aa, bb = _u8(a), None if b is None else _u8(b)
+    h = ctypes.c_void_p()
+    st = lib().kc_fastq_pack(device, ptr(aa, u8p), len(a), None if bb is None else ptr(bb, u8p),
+                             0 if b is None else len(b), ctypes.byref(h))
+    if st != _abi.KC_OK:
+        raise KcError(st, lib().kc_fastq_last_error().decode())
+    return _take(h)

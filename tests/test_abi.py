@@ -31,7 +31,7 @@ def test_struct_layouts_match_header():
     assert _abi.kc_opts.bloom_bits.offset == 8 and _abi.kc_opts.min_coverage.offset == 16
     assert ctypes.sizeof(_abi.kc_info) == 8 * 8 + 4 * 2 + 8 * 2
     assert ctypes.sizeof(_abi.kc_tune) == 24 and _abi.kc_tune.ms.offset == 16
-    assert ctypes.sizeof(_abi.kc_packed_info) == 48
+    assert ctypes.sizeof(_abi.kc_packed_info) == 56
 
 
 def test_struct_layouts_match_c_compiler(tmp_path):
@@ -40,15 +40,17 @@ def test_struct_layouts_match_c_compiler(tmp_path):
     from kc import _abi
     src = tmp_path / "layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "kc.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(kc_opts), sizeof(kc_info),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(kc_opts), sizeof(kc_info),'
                    ' sizeof(kc_tune), offsetof(kc_opts, table_load), offsetof(kc_info, cutoff),'
-                   ' offsetof(kc_info, ms_stage2), offsetof(kc_tune, table_load)); return 0;}\n')
+                   ' offsetof(kc_info, ms_stage2), offsetof(kc_tune, table_load), sizeof(kc_packed_info),'
+                   ' offsetof(kc_packed_info, ms_device)); return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert got == [ctypes.sizeof(_abi.kc_opts), ctypes.sizeof(_abi.kc_info), ctypes.sizeof(_abi.kc_tune),
                    _abi.kc_opts.table_load.offset, _abi.kc_info.cutoff.offset, _abi.kc_info.ms_stage2.offset,
-                   _abi.kc_tune.table_load.offset]
+                   _abi.kc_tune.table_load.offset, ctypes.sizeof(_abi.kc_packed_info),
+                   _abi.kc_packed_info.ms_device.offset]
 
 
 def test_no_device_is_an_error_not_a_crash():

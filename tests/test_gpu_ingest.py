@@ -51,6 +51,15 @@ def test_single_file_and_no_final_newline(oracle):
     check(oracle, a[:-1])  # the last quality line without its newline
 
 
+def test_empty_last_record(oracle):
+    # the last record's sequence and quality lines are empty, with and without blank lines after
+    a = fastq([b"ACGT" * 10, b"CCGGA" * 9, b""])
+    b = fastq([b"TTGCA" * 8, b"GGCA" * 11, b"AC"])
+    check(oracle, a, b)
+    check(oracle, a + b"\n\n", b + b"\r\n")
+    check(oracle, b"@only\n\n+\n\n")
+
+
 def test_ecoli_fixture(oracle):
     from conftest import GOLDEN
     a = gzip.open(os.path.join(GOLDEN, "ecoli_1K_1.fq.gz"), "rb").read()
@@ -67,6 +76,28 @@ def test_large_synthetic_pairs(oracle):
     for i in range(0, n, 97):
         r1[i] = r1[i][:70] + b"N" + r1[i][71:]
     check(oracle, fastq(r1), fastq(r2))
+
+
+def test_context_reuse_shrinking_inputs(oracle):
+    # one context, a large input then smaller ones: the grown buffers keep no stale state
+    from kc.ingest import FastqIngest
+    ctx = FastqIngest()
+    for seed, n in ((21, 3000), (22, 40), (23, 700), (24, 0)):
+        rng = np.random.default_rng(seed)
+        a, b = fastq(random_reads(rng, n)), fastq(random_reads(rng, n + seed % 3))
+        say(f"case ingest context seed={seed} n={n}")
+        data, offs, lens, pe, amb, rec1, rec2 = oracle.pack_fastq(a, b)
+        store, info = ctx.pack(a, b)
+        d, o, l = store.arrays()
+        assert (info["pe_count"], info["n_ambiguous"], info["r1_records"], info["r2_records"]) == (pe, amb, rec1, rec2)
+        assert np.array_equal(d, data) and np.array_equal(o, offs) and np.array_equal(l, lens)
+        assert info["ms_device"] >= 0
+    # single-file mode after paired calls on the same context
+    a = fastq(random_reads(np.random.default_rng(25), 500))
+    data, offs, lens, *_ = oracle.pack_fastq(a)
+    d, o, l = ctx.pack(a)[0].arrays()
+    assert np.array_equal(d, data) and np.array_equal(o, offs) and np.array_equal(l, lens)
+    ctx.close()
 
 
 def test_unsupported_and_malformed():

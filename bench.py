@@ -118,9 +118,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-load", type=float, default=0.0)
+    ap.add_argument("--microbench-sweep", action="store_true",
+                    help="only print random-atomic / random-read rates against array size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.microbench_sweep:
+        return microbench_sweep()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
                                    "(--gpus must be 1)"}), flush=True)
@@ -305,6 +309,25 @@ def microbenchmarks(device=0):
             out[name] = {"G_ops_s": round(n_ops / (ms.value / 1e3) / 1e9, 2), "ms": round(ms.value, 3),
                          "array_bytes": nbytes}
     return out
+
+
+def microbench_sweep(device=0, n_ops=1 << 28):
+    """Random-access rates against array size (1 MiB to 8 GiB): non-returning u32 atomicAdd
+    at agent and at workgroup scope, returning u32 atomicAdd, and 8-byte random reads.
+    Prints one JSON object per size."""
+    import ctypes
+    L = ctypes.CDLL(os.path.join(PKG, "libkc_bench.so"))
+    L.kc_bench_device.restype = ctypes.c_int
+    L.kc_bench_device.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    kinds = (("atomic_add_agent", 0), ("atomic_add_workgroup", 4), ("atomic_add_returning", 5), ("read_8b", 2))
+    for mib in (1, 4, 16, 64, 128, 256, 512, 2048, 8192):
+        row = {"array_mib": mib}
+        for name, kind in kinds:
+            ms = ctypes.c_double(0)
+            rc = L.kc_bench_device(device, kind, mib << 20, n_ops, 3, ctypes.byref(ms))
+            row[name + "_G_per_s"] = round(n_ops / (ms.value / 1e3) / 1e9, 2) if rc == 0 else f"error {rc}"
+        print(json.dumps(row), flush=True)
 
 
 def _key_index(keys):

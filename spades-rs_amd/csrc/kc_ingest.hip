@@ -9,13 +9,15 @@
 //      positions in order (block scan of the per-thread counts);
 //   2. per line (block-reduced): first line that is not whitespace only, last non-empty
 //      line; per record (4 lines): the '@' / '+' / length checks;
-//   3. per read: gate 1 (terminal non-ACGT trim), the internal-N flag and the number and
-//      packed size of the >= 21-base ACGT segments, kept in a gate array;
+//   3. per read (one wavefront, 64 bases per step, run boundaries from the ballot mask):
+//      gate 1 (terminal non-ACGT trim), the internal-N flag and the number and packed size
+//      of the >= 21-base ACGT segments, kept in a gate array;
 //   4. per pair: clean flag (gate 2), scan -> clean rank; per read: its output unit (block
 //      1 = clean pairs' R1, block 2 = their R2, block 3 = the other pairs' segments, then
 //      the longer file's extra reads) and that unit's reads and bytes; exclusive scans give
 //      each unit's first read and first byte;
-//   5. one thread per input read writes its output reads' offsets, lengths and packed bytes.
+//   5. one wavefront per input read writes its output reads' offsets, lengths and packed
+//      bytes (lane q packs output bytes q, q + 64, ...).
 // The device handles the 4-line FASTQ layout (blank or whitespace-only lines allowed before
 // the first record, blank lines after the last); blank lines between records and FASTA give
 // KC_EUNSUPPORTED, for the host parser.
@@ -212,38 +214,133 @@ struct ReadGate {
     uint64_t seg_bytes;
 };
 
-__device__ ReadGate gate(const uint8_t* __restrict__ b, uint64_t s, uint32_t l) {
-    ReadGate g{};
-    uint32_t a = 0, z = l;
-    while (a < l && code2(b[s + a]) == 4u) ++a;
-    while (z > a && code2(b[s + z - 1]) == 4u) --z;
-    g.ts = a;
-    g.tl = z - a;
-    uint32_t i = a;
-    while (i < z) {
-        while (i < z && code2(b[s + i]) == 4u) {
-            g.internal_n = 1;
-            ++i;
+constexpr uint32_t WAVE = 64;
+
+// This wave's index in the grid, as a value the compiler knows to be wave-uniform (so the
+// per-read loads and the run walk below are scalar).
+__device__ __forceinline__ uint64_t wave_id() {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    return (uint64_t)blockIdx.x * (BLOCK / WAVE) + w;
+}
+
+// Calls on_run(start, len) for every maximal run of ACGT bases of the read [s, s + l), in
+// order.  One wavefront per read: each 64-base chunk is one base per lane and the wave's
+// ballot of "not ACGT"; the run boundaries come from that mask by count-trailing-zeros, so
+// every value here is wave-uniform.  Bases past l count as not ACGT, which closes a run at
+// the read's end.
+template <typename F>
+__device__ void wave_runs(const uint8_t* __restrict__ b, uint64_t s, uint32_t l, uint32_t lane, F&& on_run) {
+    bool in_run = false;
+    uint32_t run_start = 0;
+    for (uint64_t c1 = 0; c1 < l; c1 += 4 * WAVE) {
+        // four independent loads per lane before the first ballot (one memory round trip
+        // for reads of up to 256 bases)
+        uint8_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t p = c1 + j * WAVE + lane;
+            v[j] = p < l ? b[s + p] : (uint8_t)'N';
         }
-        uint32_t j = i;
-        while (j < z && code2(b[s + j]) != 4u) ++j;
-        if (j - i >= MIN_K) {
-            g.nseg += 1;
-            g.seg_bytes += (j - i + 3) / 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t c0 = c1 + j * WAVE;
+            if (c0 >= l) break;
+            const uint64_t M = __ballot(code2(v[j]) == 4u);  // past l: 'N'
+            uint32_t pos = 0;
+            while (pos < WAVE) {
+                if (in_run) {
+                    const uint64_t m = M >> pos;
+                    if (!m) break;
+                    const uint32_t e = pos + (uint32_t)__builtin_ctzll(m);
+                    on_run(run_start, (uint32_t)(c0 + e) - run_start);
+                    in_run = false;
+                    pos = e;
+                } else {
+                    const uint64_t z = ~M >> pos;
+                    if (!z) break;
+                    const uint32_t st = pos + (uint32_t)__builtin_ctzll(z);
+                    in_run = true;
+                    run_start = (uint32_t)c0 + st;
+                    pos = st;
+                }
+            }
         }
-        i = j;
     }
+    if (in_run) on_run(run_start, l - run_start);
+}
+
+// The trimmed span is the first run's start to the last run's end; an internal non-ACGT
+// base exists exactly when there are two or more runs (packed_reads.rs:513-535, 630-690).
+// Fast path: per 64-base chunk the ACGT mask gives the first and last ACGT base and the
+// number of run starts by bit arithmetic; only a read with two or more runs walks them.
+__device__ ReadGate wave_gate(const uint8_t* __restrict__ b, uint64_t s, uint32_t l, uint32_t lane) {
+    ReadGate g{};
+    uint32_t runs = 0, first = l, last_end = 0;
+    uint64_t carry = 0;  // the previous chunk's last base is ACGT
+    for (uint64_t c1 = 0; c1 < l; c1 += 4 * WAVE) {
+        uint8_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t p = c1 + j * WAVE + lane;
+            v[j] = p < l ? b[s + p] : (uint8_t)'N';
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t c0 = c1 + j * WAVE;
+            if (c0 >= l) break;
+            const uint64_t Z = __ballot(code2(v[j]) != 4u);  // ACGT bases of the chunk
+            const uint64_t starts = Z & ~((Z << 1) | carry);
+            runs += (uint32_t)__popcll(starts);
+            if (Z) {
+                if (first == l) first = (uint32_t)c0 + (uint32_t)__builtin_ctzll(Z);
+                last_end = (uint32_t)c0 + 64u - (uint32_t)__builtin_clzll(Z);
+            }
+            carry = Z >> 63;
+        }
+    }
+    if (runs >= 2) {
+        wave_runs(b, s, l, lane, [&](uint32_t, uint32_t len) {
+            if (len >= MIN_K) {
+                g.nseg += 1;
+                g.seg_bytes += (len + 3) / 4;
+            }
+        });
+    } else if (runs == 1 && last_end - first >= MIN_K) {
+        g.nseg = 1;
+        g.seg_bytes = (last_end - first + 3) / 4;
+    }
+    g.ts = runs ? first : l;
+    g.tl = runs ? last_end - first : 0;
+    g.internal_n = runs >= 2;
     return g;
 }
 
-// Input read x: reads 0..R1-1 of the first file, then R1.. of the second.
-__global__ void k_gates(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
-                        const uint32_t* __restrict__ l1, uint64_t R1, const uint8_t* __restrict__ b2,
-                        const uint64_t* __restrict__ s2, const uint32_t* __restrict__ l2, uint64_t R2,
-                        ReadGate* __restrict__ gates) {
+// Packs bases [s, s + len) (all ACGT) into out, 4 per byte, high bits first; lane q writes
+// output bytes q, q + 64, ...
+__device__ void wave_pack(const uint8_t* __restrict__ b, uint64_t s, uint32_t len, uint8_t* __restrict__ out,
+                          uint32_t lane) {
+    const uint32_t nb = (len + 3) / 4;
+    for (uint32_t q = lane; q < nb; q += WAVE) {
+        uint32_t v = 0;
+        for (uint32_t r = 0; r < 4; ++r)
+            if (4 * q + r < len) v |= code2(b[s + 4 * q + r]) << (6 - 2 * r);
+        out[q] = (uint8_t)v;
+    }
+}
+
+// Input read x: reads 0..R1-1 of the first file, then R1.. of the second.  One wave per read.
+__global__ __launch_bounds__(BLOCK) void k_gates(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
+                                                 const uint32_t* __restrict__ l1, uint64_t R1,
+                                                 const uint8_t* __restrict__ b2, const uint64_t* __restrict__ s2,
+                                                 const uint32_t* __restrict__ l2, uint64_t R2,
+                                                 ReadGate* __restrict__ gates) {
     const uint64_t total = R1 + R2;
-    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x)
-        gates[x] = x < R1 ? gate(b1, s1[x], l1[x]) : gate(b2, s2[x - R1], l2[x - R1]);
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t x = wave_id(); x < total; x += nw) {
+        const ReadGate g = x < R1 ? wave_gate(b1, s1[x], l1[x], lane) : wave_gate(b2, s2[x - R1], l2[x - R1], lane);
+        if (lane == 0) gates[x] = g;
+    }
 }
 
 // Gate 2 of pair i (packed_reads.rs:340-352): both mates free of internal non-ACGT bases
@@ -296,53 +393,59 @@ __global__ void k_units(const ReadGate* __restrict__ gates, uint64_t R1, uint64_
     if (amb) atomicAdd(ambiguous, (unsigned long long)amb);
 }
 
-__device__ void pack_run(const uint8_t* __restrict__ b, uint64_t s, uint32_t l, uint8_t* __restrict__ out) {
-    for (uint32_t q = 0; q < l; q += 4) {
-        uint32_t v = 0;
-        for (uint32_t r = 0; r < 4; ++r)
-            if (q + r < l) v |= code2(b[s + q + r]) << (6 - 2 * r);
-        out[q / 4] = (uint8_t)v;
+// place[x] = (first output read, first output byte) of input read x.
+__global__ void k_place(uint64_t total, uint64_t R1, uint64_t P, const uint32_t* __restrict__ clean,
+                        const unsigned long long* __restrict__ clean_pre, uint64_t n_clean,
+                        const unsigned long long* __restrict__ read_base,
+                        const unsigned long long* __restrict__ byte_base, ulonglong2* __restrict__ place) {
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
+        bool whole;
+        const uint64_t u = unit_of(x, R1, P, clean, clean_pre, n_clean, whole);
+        place[x] = make_ulonglong2(read_base[u], byte_base[u]);
     }
 }
 
-// Writes every input read's output reads (offsets, lengths, packed bytes).
-__global__ void k_emit(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1, uint64_t R1,
-                       const uint8_t* __restrict__ b2, const uint64_t* __restrict__ s2, uint64_t R2, uint64_t P,
-                       const ReadGate* __restrict__ gates, const uint32_t* __restrict__ clean,
-                       const unsigned long long* __restrict__ clean_pre, uint64_t n_clean,
-                       const unsigned long long* __restrict__ read_base,
-                       const unsigned long long* __restrict__ byte_base, uint8_t* __restrict__ data,
-                       uint64_t* __restrict__ offsets, uint32_t* __restrict__ lengths) {
+// Writes every input read's output reads (offsets, lengths, packed bytes).  One wave per
+// input read: a whole read is one packed span; otherwise each run of at least MIN_K bases
+// becomes an output read, in order.
+__global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, const uint64_t* __restrict__ s1,
+                                                const uint32_t* __restrict__ l1, uint64_t R1,
+                                                const uint8_t* __restrict__ b2, const uint64_t* __restrict__ s2,
+                                                const uint32_t* __restrict__ l2, uint64_t R2, uint64_t P,
+                                                const ReadGate* __restrict__ gates, const uint32_t* __restrict__ clean,
+                                                const ulonglong2* __restrict__ place, uint8_t* __restrict__ data,
+                                                uint64_t* __restrict__ offsets, uint32_t* __restrict__ lengths) {
     const uint64_t total = R1 + R2;
-    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t x = wave_id(); x < total; x += nw) {
         const bool second = x >= R1;
         const uint8_t* b = second ? b2 : b1;
         const uint64_t s = second ? s2[x - R1] : s1[x];
+        const uint32_t l = second ? l2[x - R1] : l1[x];
         const ReadGate g = gates[x];
-        bool whole;
-        const uint64_t u = unit_of(x, R1, P, clean, clean_pre, n_clean, whole);
-        uint64_t rb = read_base[u], bb = byte_base[u];
-        if (whole) {
-            offsets[rb] = bb;
-            lengths[rb] = g.tl;
-            pack_run(b, s + g.ts, g.tl, data + bb);
+        const ulonglong2 pl = place[x];
+        uint64_t rb = pl.x, bb = pl.y;
+        const uint64_t i = second ? x - R1 : x;
+        if (i < P && clean[i]) {  // a clean pair's read: one output read of its trimmed span
+            if (lane == 0) {
+                offsets[rb] = bb;
+                lengths[rb] = g.tl;
+            }
+            wave_pack(b, s + g.ts, g.tl, data + bb, lane);
             continue;
         }
-        const uint32_t z = g.ts + g.tl;
-        uint32_t p = g.ts;
-        while (p < z) {
-            while (p < z && code2(b[s + p]) == 4u) ++p;
-            uint32_t q = p;
-            while (q < z && code2(b[s + q]) != 4u) ++q;
-            if (q - p >= MIN_K) {
+        if (!g.nseg) continue;
+        wave_runs(b, s, l, lane, [&](uint32_t st, uint32_t len) {
+            if (len < MIN_K) return;
+            if (lane == 0) {
                 offsets[rb] = bb;
-                lengths[rb] = q - p;
-                pack_run(b, s + p, q - p, data + bb);
-                ++rb;
-                bb += (q - p + 3) / 4;
+                lengths[rb] = len;
             }
-            p = q;
-        }
+            wave_pack(b, s + st, len, data + bb, lane);
+            ++rb;
+            bb += (len + 3) / 4;
+        });
     }
 }
 
@@ -358,7 +461,7 @@ struct kc_ingest {
     hipStream_t st = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     FileIndex A, B;
-    Buf gates, clean, clean_pre, ureads, ubytes, rbase, bbase, small, scan_tmp, data, off, len;
+    Buf gates, clean, clean_pre, ureads, ubytes, rbase, bbase, place, small, scan_tmp, data, off, len;
 };
 
 namespace {
@@ -529,6 +632,7 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     I_TRY(c->ubytes.ensure(U * 8 + 8));
     I_TRY(c->rbase.ensure(U * 8 + 8));
     I_TRY(c->bbase.ensure(U * 8 + 8));
+    I_TRY(c->place.ensure(tot * 16 + 16));
     I_TRY(c->small.ensure(64));
     const unsigned long long zero = 0;
     I_TRY(hipMemcpyAsync(c->small.p, &zero, 8, hipMemcpyHostToDevice, st));
@@ -536,7 +640,7 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     const uint8_t* b2 = B.buf.as<uint8_t>();
     ReadGate* gates = c->gates.as<ReadGate>();
     if (tot) {
-        k_gates<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,
+        k_gates<<<grid_for(tot * WAVE), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,
                                                B.seq_s.as<uint64_t>(), B.seq_l.as<uint32_t>(), R2, gates);
         I_TRY(hipGetLastError());
     }
@@ -577,10 +681,16 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     I_TRY(c->off.ensure(n_reads * 8 + 8));
     I_TRY(c->len.ensure(n_reads * 4 + 4));
     if (tot) {
-        k_emit<<<grid_for(tot), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), R1, b2, B.seq_s.as<uint64_t>(), R2, P, gates,
-                                              c->clean.as<uint32_t>(), c->clean_pre.as<unsigned long long>(), n_clean,
-                                              c->rbase.as<unsigned long long>(), c->bbase.as<unsigned long long>(),
-                                              c->data.as<uint8_t>(), c->off.as<uint64_t>(), c->len.as<uint32_t>());
+        k_place<<<grid_for(tot), BLOCK, 0, st>>>(tot, R1, P, c->clean.as<uint32_t>(),
+                                               c->clean_pre.as<unsigned long long>(), n_clean,
+                                               c->rbase.as<unsigned long long>(), c->bbase.as<unsigned long long>(),
+                                               c->place.as<ulonglong2>());
+        I_TRY(hipGetLastError());
+        k_emit<<<grid_for(tot * WAVE), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,
+                                                      B.seq_s.as<uint64_t>(), B.seq_l.as<uint32_t>(), R2, P, gates,
+                                                      c->clean.as<uint32_t>(), c->place.as<ulonglong2>(),
+                                                      c->data.as<uint8_t>(), c->off.as<uint64_t>(),
+                                                      c->len.as<uint32_t>());
         I_TRY(hipGetLastError());
     }
     I_TRY(hipEventRecord(c->ev[1], st));

@@ -40,7 +40,7 @@ extern "C" {
 #define KC_ESTATE (-4)       /* call out of order (e.g. export before run) */
 #define KC_EUNSUPPORTED (-5) /* valid in the reference but not supported by this build */
 
-#define KC_MAX_K 63u         /* this build: 1 <= k <= 63 (reference Kmer256 allows up to 128) */
+#define KC_MAX_K 127u        /* 1 <= k <= 127 (the reference's CLI range is odd k in 11..127) */
 
 typedef struct kc_ctx kc_ctx;
 typedef struct kc_filter kc_filter;

@@ -1,9 +1,10 @@
 // kc_device.h — device primitives of the gfx950 k-mer spectrum engine.
 //
-// A k-mer (k <= 63) is held as its 2k-bit value V, first base most significant
-// (the value Kmer256::from_bytes gives for k <= 64, dna.rs:299-321), in two
-// 64-bit words: lo = bits 0..63, hi = bits 64..127.  WIDE = false is the
-// k <= 32 case (hi == 0), WIDE = true covers 33 <= k <= 63.
+// A k-mer (k <= 127) is held as its 2k-bit value V, first base most significant (the
+// order of Kmer256's derived Ord, dna.rs:137-138, 299-321), in four 64-bit words: lo =
+// bits 0..63, hi = 64..127, w2 = 128..191, w3 = 192..255.  A kernel is built for KW
+// value words: KW = 1 for k <= 32, 2 for 33 <= k <= 63, 4 for 64 <= k <= 127; the words
+// above KW are zero.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,13 +13,37 @@
 namespace kc {
 
 struct Kv {
-    uint64_t lo, hi;
+    uint64_t lo, hi, w2, w3;
 };
+
+// Table key words per value width: 63-bit words, 1, 2 or 5 of them (section below).
+__host__ __device__ constexpr int nw_of(int KW) { return KW == 4 ? 5 : KW; }
+__host__ __device__ constexpr int kw_of_nw(int NW) { return NW == 5 ? 4 : NW; }
+__host__ __device__ constexpr int kw_of_k(uint32_t k) { return k <= 32 ? 1 : k <= 63 ? 2 : 4; }
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
 
 __device__ __forceinline__ bool kv_le(const Kv& a, const Kv& b) {
+    if (a.w3 != b.w3) return a.w3 < b.w3;
+    if (a.w2 != b.w2) return a.w2 < b.w2;
     return a.hi < b.hi || (a.hi == b.hi && a.lo <= b.lo);
+}
+
+// Word j (0..3) of a value, 0 past the top.
+__device__ __forceinline__ uint64_t kv_word(const Kv& x, uint32_t j) {
+    return j == 0 ? x.lo : j == 1 ? x.hi : j == 2 ? x.w2 : j == 3 ? x.w3 : 0ull;
+}
+
+// x >> s as a 256-bit value, 0 <= s < 256.
+__device__ __forceinline__ Kv kv_shr(const Kv& x, uint32_t s) {
+    const uint32_t q = s >> 6, r = s & 63;
+    uint64_t o[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint64_t a = kv_word(x, i + q), b = kv_word(x, i + q + 1);
+        o[i] = r ? (a >> r) | (b << (64 - r)) : a;
+    }
+    return Kv{o[0], o[1], o[2], o[3]};
 }
 
 // Reverse the order of the 32 2-bit groups of x, keeping the two bits of a group in order.
@@ -29,52 +54,88 @@ __device__ __forceinline__ uint64_t rev_groups(uint64_t x) {
 }
 
 // Reverse complement of a k-mer value (complement = code ^ 3, dna.rs:1-8): the same
-// result as Kmer256::revcomp (dna.rs:152-180) for k <= 63.
-template <bool WIDE>
+// result as Kmer256::revcomp (dna.rs:152-180).  For KW = 4 the 128 groups of the
+// complemented 256-bit value are reversed (words swapped end for end, each word's groups
+// reversed), which leaves the k bases at the top, and a shift by 256 - 2k brings them down.
+template <int KW>
 __device__ __forceinline__ Kv revcomp(const Kv& v, uint32_t k) {
-    if (!WIDE) return Kv{rev_groups(~v.lo) >> (64 - 2 * k), 0};
-    const uint64_t rhi = rev_groups(~v.lo), rlo = rev_groups(~v.hi);
-    const uint32_t s2 = 128 - 2 * k;  // 2..62
-    return Kv{(rlo >> s2) | (rhi << (64 - s2)), rhi >> s2};
+    if (KW == 1) return Kv{rev_groups(~v.lo) >> (64 - 2 * k), 0};
+    if (KW == 2) {
+        const uint64_t rhi = rev_groups(~v.lo), rlo = rev_groups(~v.hi);
+        const uint32_t s2 = 128 - 2 * k;  // 2..62
+        return Kv{(rlo >> s2) | (rhi << (64 - s2)), rhi >> s2};
+    }
+    const Kv r{rev_groups(~v.w3), rev_groups(~v.w2), rev_groups(~v.hi), rev_groups(~v.lo)};
+    return kv_shr(r, 256 - 2 * k);  // 2..128 for 64 <= k <= 127
 }
 
 // Kmer256::canonical (dna.rs:185-192): min(fwd, rc), ties to the forward strand.
-template <bool WIDE>
+template <int KW>
 __device__ __forceinline__ Kv canonical(const Kv& v, uint32_t k) {
-    const Kv rc = revcomp<WIDE>(v, k);
+    const Kv rc = revcomp<KW>(v, k);
     return kv_le(v, rc) ? v : rc;
 }
 
-// The 2k bits starting at bit `sh` (0..31) of a big-endian bit stream held in the
-// 32-bit words p[0..4] (byte 0 = bits 31..24 of the first word after bswap).  p must be
-// 4-byte aligned with 3 (k <= 32) or 5 (k <= 63) readable words.
-template <bool WIDE>
+// The 2k bits starting at bit `sh` (0..31) of a big-endian bit stream held in 32-bit
+// words p[0..] (byte 0 = bits 31..24 of the first word after bswap).  p must be 4-byte
+// aligned with 3 (KW = 1), 5 (KW = 2) or 9 (KW = 4) readable words.
+template <int KW>
 __device__ __forceinline__ Kv extract_be(const uint32_t* p, uint32_t sh, uint32_t k) {
-    if (!WIDE) {
+    if (KW == 1) {
         const uint64_t w0 = __builtin_bswap32(p[0]), w1 = __builtin_bswap32(p[1]);
         const uint32_t w2 = __builtin_bswap32(p[2]);
         uint64_t top = (w0 << 32) | w1;
         top = sh ? ((top << sh) | (uint64_t)(w2 >> (32 - sh))) : top;
         return Kv{top >> (64 - 2 * k), 0};
     }
-    const uint64_t a = ((uint64_t)__builtin_bswap32(p[0]) << 32) | __builtin_bswap32(p[1]);
-    const uint64_t b = ((uint64_t)__builtin_bswap32(p[2]) << 32) | __builtin_bswap32(p[3]);
-    const uint64_t c = (uint64_t)__builtin_bswap32(p[4]) << 32;
-    const uint64_t hi = sh ? ((a << sh) | (b >> (64 - sh))) : a;
-    const uint64_t lo = sh ? ((b << sh) | (c >> (64 - sh))) : b;
-    const uint32_t s2 = 128 - 2 * k;  // 2..62 for 33 <= k <= 63
-    return Kv{(lo >> s2) | (hi << (64 - s2)), hi >> s2};
+    if (KW == 2) {
+        const uint64_t a = ((uint64_t)__builtin_bswap32(p[0]) << 32) | __builtin_bswap32(p[1]);
+        const uint64_t b = ((uint64_t)__builtin_bswap32(p[2]) << 32) | __builtin_bswap32(p[3]);
+        const uint64_t c = (uint64_t)__builtin_bswap32(p[4]) << 32;
+        const uint64_t hi = sh ? ((a << sh) | (b >> (64 - sh))) : a;
+        const uint64_t lo = sh ? ((b << sh) | (c >> (64 - sh))) : b;
+        const uint32_t s2 = 128 - 2 * k;  // 2..62 for 33 <= k <= 63
+        return Kv{(lo >> s2) | (hi << (64 - s2)), hi >> s2};
+    }
+    uint64_t a[5];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        a[i] = ((uint64_t)__builtin_bswap32(p[2 * i]) << 32) | __builtin_bswap32(p[2 * i + 1]);
+    a[4] = (uint64_t)__builtin_bswap32(p[8]) << 32;
+    uint64_t t[4];  // t[0] = the most significant word of the window's 256 leading bits
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = sh ? ((a[i] << sh) | (a[i + 1] >> (64 - sh))) : a[i];
+    return kv_shr(Kv{t[3], t[2], t[1], t[0]}, 256 - 2 * k);
+}
+
+// The four u64 limbs hash_kmer256 reads (bloom.rs:61-66) of the Kmer256 holding V:
+// for k <= 64, Kmer256(V, 0); for k > 64, Kmer256(first 64 bases, the other k - 64
+// bases) (dna.rs:134-138, 299-321).
+template <int KW>
+__device__ __forceinline__ void kmer_limbs(const Kv& v, uint32_t k, uint64_t w[4]) {
+    if (KW <= 2) {
+        w[0] = v.lo;
+        w[1] = v.hi;
+        w[2] = w[3] = 0;
+        return;
+    }
+    const uint32_t s = 2 * k - 128;  // bits of the second u128, 0..126
+    const Kv top = kv_shr(v, s);
+    w[0] = top.lo;
+    w[1] = top.hi;
+    w[2] = s >= 64 ? v.lo : (v.lo & ((1ull << s) - 1));
+    w[3] = s > 64 ? (v.hi & ((1ull << (s - 64)) - 1)) : 0ull;
 }
 
 // Window starting at base s of a read whose packed bytes start at byte `off` of `data`
 // (PackedReads layout, packed_reads.rs:547-578: base j in byte j/4, bits 6-2*(j%4)).
-// `data` must be readable 20 bytes past the last read byte.
-template <bool WIDE>
+// `data` must be readable 24 bytes past the last read byte.
+template <int KW>
 __device__ __forceinline__ Kv load_window(const uint8_t* __restrict__ data, uint64_t off, uint32_t s, uint32_t k) {
     const uint64_t b0 = off + (s >> 2);
     const uint64_t a = b0 & ~3ull;
     const uint32_t sh = (uint32_t)(b0 - a) * 8 + 2 * (s & 3);
-    return extract_be<WIDE>(reinterpret_cast<const uint32_t*>(data + a), sh, k);
+    return extract_be<KW>(reinterpret_cast<const uint32_t*>(data + a), sh, k);
 }
 
 // ---- bloom.rs:61-78 hash_kmer256 -------------------------------------------------
@@ -119,10 +180,20 @@ __device__ __forceinline__ void bloom_positions(uint64_t w0, uint64_t w1, uint64
     }
 }
 
-// bloom.rs:111-126 is_solid_kmer256 for a k <= 64 k-mer value (hi limbs zero).
-__device__ __forceinline__ bool bloom_is_solid(const Kv& v, const uint64_t* __restrict__ solid, const Modulus& M) {
+// The Bloom bit positions of a k-mer value.
+template <int KW>
+__device__ __forceinline__ void kv_positions(const Kv& v, uint32_t k, const Modulus& M, uint64_t pos[3]) {
+    uint64_t w[4];
+    kmer_limbs<KW>(v, k, w);
+    bloom_positions(w[0], w[1], w[2], w[3], M, pos);
+}
+
+// bloom.rs:111-126 is_solid_kmer256 for a k-mer value.
+template <int KW>
+__device__ __forceinline__ bool bloom_is_solid(const Kv& v, uint32_t k, const uint64_t* __restrict__ solid,
+                                               const Modulus& M) {
     uint64_t p[3];
-    bloom_positions(v.lo, v.hi, 0, 0, M, p);
+    kv_positions<KW>(v, k, M, p);
     const uint64_t a = solid[p[0] >> 6], b = solid[p[1] >> 6], c = solid[p[2] >> 6];
     return ((a >> (p[0] & 63)) & (b >> (p[1] & 63)) & (c >> (p[2] & 63)) & 1ull) != 0;
 }
@@ -133,15 +204,17 @@ __device__ __forceinline__ bool bloom_is_solid(const Kv& v, const uint64_t* __re
 // NW = 1: the key is V itself (k <= 32; a canonical 32-mer is never all ones because
 // revcomp(T^32) = A^32 < T^32, so ~0 can mark an empty word).  NW = 2 (33 <= k <= 63):
 // V (< 2^126) is split into two 63-bit words A = V >> 63, B = V & (2^63 - 1), neither of
-// which can be ~0.  A slot is claimed word by word with 64-bit CAS (A first): every word
-// goes from EMPTY to its final value exactly once, so all inserters of one key take the
-// same decision at every slot and the key lands in exactly one slot without locks.
+// which can be ~0.  NW = 5 (64 <= k <= 127): V (< 2^254) as five words of at most 63
+// bits, lowest bits first.  A slot is claimed word by word with 64-bit CAS (word 0
+// first): every word goes from EMPTY to its final value exactly once, so all inserters of
+// one key take the same decision at every slot and the key lands in exactly one slot
+// without locks.
 
-// A slot is 16 bytes (one key word, 32-bit events) or 32 bytes, so a 64-byte line holds
-// a bucket of SPB = 4 or 2 whole slots.  Probing scans a bucket in slot order, then
-// moves to the next bucket.
+// A slot is 16 bytes (one key word, 32-bit events), 32 bytes (NW <= 2) or 64 bytes
+// (NW = 5), so a 64-byte line holds a bucket of SPB = 4, 2 or 1 whole slots.  Probing
+// scans a bucket in slot order, then moves to the next bucket.
 template <int NW, typename Ev>
-struct alignas((NW == 1 && sizeof(Ev) == 4) ? 16 : 32) Slot {
+struct alignas((NW == 1 && sizeof(Ev) == 4) ? 16 : NW <= 2 ? 32 : 64) Slot {
     uint64_t k[NW];
     Ev t0;         // smallest event index (global window number) of this key
     uint32_t cnt;  // occurrences
@@ -152,20 +225,30 @@ constexpr uint32_t slots_per_bucket() {
     return 64 / sizeof(Slot<NW, Ev>);
 }
 
+constexpr uint64_t M63 = 0x7FFFFFFFFFFFFFFFull;
+
 template <int NW>
 __device__ __forceinline__ void pack_key(const Kv& v, uint64_t key[NW]) {
     if (NW == 1) {
         key[0] = v.lo;
-    } else {
+    } else if (NW == 2) {
         key[0] = (v.hi << 1) | (v.lo >> 63);
-        key[NW - 1] = v.lo & 0x7FFFFFFFFFFFFFFFull;
+        key[NW - 1] = v.lo & M63;
+    } else {  // V bits 0..62, 63..125, 126..188, 189..251, 252..253
+        key[0] = v.lo & M63;
+        key[1] = ((v.lo >> 63) | (v.hi << 1)) & M63;
+        key[2] = ((v.hi >> 62) | (v.w2 << 2)) & M63;
+        key[3] = ((v.w2 >> 61) | (v.w3 << 3)) & M63;
+        key[NW - 1] = v.w3 >> 60;
     }
 }
 
 template <int NW>
 __device__ __forceinline__ Kv unpack_key(const uint64_t* key) {
     if (NW == 1) return Kv{key[0], 0};
-    return Kv{key[NW - 1] | (key[0] << 63), key[0] >> 1};
+    if (NW == 2) return Kv{key[NW - 1] | (key[0] << 63), key[0] >> 1};
+    return Kv{key[0] | (key[1] << 63), (key[1] >> 1) | (key[2] << 62), (key[2] >> 2) | (key[3] << 61),
+              (key[3] >> 3) | (key[NW - 1] << 60)};
 }
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t x) {
@@ -181,6 +264,10 @@ template <int NW>
 __device__ __forceinline__ uint64_t table_hash(const uint64_t key[NW]) {
     uint64_t h = key[0] * 0x9E3779B97F4A7C15ull;
     if (NW == 2) h ^= key[NW - 1] + 0x632BE59BD9B4E019ull;
+    if (NW == 5) {
+#pragma unroll
+        for (int w = 1; w < NW; ++w) h = fmix64(h ^ (key[w] + 0x632BE59BD9B4E019ull * (uint64_t)w));
+    }
     return fmix64(h);
 }
 

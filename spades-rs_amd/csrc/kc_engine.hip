@@ -72,7 +72,7 @@ struct LibDesc {
     Modulus M;
     uint64_t num_words;
     uint32_t boost;     // prior / trusted-contig boost per window (assemble.rs:270)
-    uint32_t pad;
+    uint32_t k;
 };
 
 // prior_boost of assemble.rs:270, (min_coverage as u32 * 4).max(50): the float-to-int
@@ -101,11 +101,34 @@ struct SortLayout {
     uint32_t single_bits;
 };
 
+// A count-table entry for k > 63: the k-mer value V (four words, lowest first) and its
+// library; entries sort as (library, V).  The library is also written into the hi word
+// array (layout: lib_bits == hi_bits, no value bits), which the count passes read.
+struct WideEntry {
+    uint64_t v[4];
+    uint32_t lib, pad;
+};
+struct WideLess {
+    __device__ __host__ bool operator()(const WideEntry& a, const WideEntry& b) const {
+        if (a.lib != b.lib) return a.lib < b.lib;
+        for (int w = 3; w >= 0; --w)
+            if (a.v[w] != b.v[w]) return a.v[w] < b.v[w];
+        return false;
+    }
+};
+
 SortLayout make_layout(uint32_t k, uint32_t n_libs) {
     SortLayout s{};
     s.k = k;
     s.lib_bits = 0;
     while ((1u << s.lib_bits) < n_libs) ++s.lib_bits;
+    if (k > 63) {
+        s.lo_bits = 0;
+        s.hi_bits = s.lib_bits;
+        s.single = 0;
+        s.single_bits = s.lib_bits;
+        return s;
+    }
     if (2 * k <= 63) {
         s.lo_bits = 2 * k;
         s.hi_bits = s.lib_bits;
@@ -228,7 +251,7 @@ __device__ __forceinline__ uint32_t tile_read(const uint64_t* s_ev, uint32_t na,
 // distinct-count estimate per library (HyperLogLog, 2^12 registers) to size its table
 // ---------------------------------------------------------------------------
 
-template <bool WIDE>
+template <int KW>
 __global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                const uint64_t* __restrict__ act_off,
                                                const uint32_t* __restrict__ act_lib,
@@ -237,7 +260,7 @@ __global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data,
                                                uint32_t* __restrict__ regs) {
     __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
     __shared__ uint32_t s_reg[HLL_M];
-    constexpr int NW = WIDE ? 2 : 1;
+    constexpr int NW = nw_of(KW);
     uint32_t cur = ~0u;  // library whose registers s_reg holds (block-uniform)
     for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
@@ -260,9 +283,9 @@ __global__ void __launch_bounds__(BLOCK) k_hll(const uint8_t* __restrict__ data,
         __syncthreads();
         for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
             const uint32_t lo = tile_read(s_ev, na, w);
-            const Kv v = load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k);
+            const Kv v = load_window<KW>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k);
             uint64_t key[NW];
-            pack_key<NW>(canonical<WIDE>(v, k), key);
+            pack_key<NW>(canonical<KW>(v, k), key);
             const uint64_t h = table_hash<NW>(key);
             const uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
             const uint64_t hw = h << HLL_P;
@@ -300,16 +323,16 @@ __global__ void k_table_clear(Slot<NW, Ev>* __restrict__ tab, uint64_t n_slots) 
 // tiles b, b + gridDim.x, ... (one tile each when the grid covers every tile).  Within a
 // tile each thread takes WPT windows at a time and issues their first bucket loads
 // together, so a wave keeps WPT independent random table accesses in flight.
-template <bool WIDE, typename Ev, bool TRACK_T0, bool SOLID_ONLY, uint32_t WPT>
+template <int KW, typename Ev, bool TRACK_T0, bool SOLID_ONLY, uint32_t WPT>
 __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                   const uint64_t* __restrict__ act_off,
                                                   const uint32_t* __restrict__ act_lib,
                                                   const uint32_t* __restrict__ tile_first, uint64_t n_active,
                                                   uint64_t n_occ, uint64_t n_tiles, uint32_t k,
-                                                  Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                  Slot<nw_of(KW), Ev>* __restrict__ tab,
                                                   const LibDesc* __restrict__ libs, uint32_t max_probe,
                                                   LibOut* __restrict__ out) {
-    constexpr int NW = WIDE ? 2 : 1;
+    constexpr int NW = nw_of(KW);
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     constexpr uint32_t PASS = BLOCK * WPT;
     static_assert(TILE_WINDOWS % PASS == 0, "a tile is a whole number of passes");
@@ -335,8 +358,8 @@ __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ da
                 const uint32_t lo = tile_read(s_ev, na, w);
                 lib[j] = act_lib[a0 + lo];
                 const Kv c =
-                    canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
-                if (SOLID_ONLY && !bloom_is_solid(c, libs[lib[j]].solid, libs[lib[j]].M)) {
+                    canonical<KW>(load_window<KW>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k);
+                if (SOLID_ONLY && !bloom_is_solid<KW>(c, k, libs[lib[j]].solid, libs[lib[j]].M)) {
                     valid[j] = false;
                     continue;
                 }
@@ -403,7 +426,7 @@ __global__ void __launch_bounds__(BLOCK) k_first_setter(const Slot<NW, Ev>* __re
             const LibDesc& L = libs[lib];
             const Kv v = unpack_key<NW>(s.k);
             uint64_t p[3];
-            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
             Ev* fs = reinterpret_cast<Ev*>(L.fs);
             const Ev base = (Ev)(s.t0 * 4);
 #pragma unroll
@@ -473,7 +496,7 @@ __global__ void __launch_bounds__(BLOCK) k_fs_reset(const Slot<NW, Ev>* __restri
             const LibDesc& L = libs[lib];
             const Kv v = unpack_key<NW>(s.k);
             uint64_t p[3];
-            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
             Ev* fs = reinterpret_cast<Ev*>(L.fs);
 #pragma unroll
             for (int j = 0; j < 3; ++j) fs[p[j]] = (Ev)~(Ev)0;
@@ -494,7 +517,7 @@ __global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restric
             const LibDesc& L = libs[lib];
             const Kv v = unpack_key<NW>(s.k);
             uint64_t p[3];
-            bloom_positions(v.lo, v.hi, 0, 0, L.M, p);
+            kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
             bool promote = s.cnt >= 2;
             if (!promote) {
                 const Ev* fs = reinterpret_cast<const Ev*>(L.fs);
@@ -524,7 +547,8 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
                                                    uint32_t* __restrict__ out_cnt,
                                                    unsigned long long* __restrict__ counter,
                                                    const uint32_t* __restrict__ psum,
-                                                   const uint8_t* __restrict__ pflag) {
+                                                   const uint8_t* __restrict__ pflag,
+                                                   WideEntry* __restrict__ out_wide) {
     __shared__ uint32_t s_scan[BLOCK];
     __shared__ unsigned long long s_base;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
@@ -541,7 +565,7 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             if (CHECK_SOLID && base) {
                 // a slot claimed only by a prior holds no occurrence
                 const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
-                base = sl.cnt > 0 && bloom_is_solid(unpack_key<NW>(sl.k), L.solid, L.M);
+                base = sl.cnt > 0 && bloom_is_solid<kw_of_nw(NW)>(unpack_key<NW>(sl.k), L.k, L.solid, L.M);
             }
             const bool take = base || (occ && pflag && pflag[i]);
             bits |= (uint32_t)take << j;
@@ -565,10 +589,16 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             bits &= bits - 1;
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
             const Slot<NW, Ev> sl = tab[i];
-            uint64_t hi, lo;
-            encode_entry(lay, find_region(lib_b0, n_libs, i / SPB), unpack_key<NW>(sl.k), hi, lo);
-            out_lo[o] = lo;
-            if (!lay.single) out_hi[o] = hi;
+            const uint32_t lib = find_region(lib_b0, n_libs, i / SPB);
+            if (NW == 5) {
+                const Kv v = unpack_key<NW>(sl.k);
+                out_wide[o] = WideEntry{{v.lo, v.hi, v.w2, v.w3}, lib, 0u};
+            } else {
+                uint64_t hi, lo;
+                encode_entry(lay, lib, unpack_key<NW>(sl.k), hi, lo);
+                out_lo[o] = lo;
+                if (!lay.single) out_hi[o] = hi;
+            }
             out_cnt[o] = ((counted >> j) & 1u ? sl.cnt : 0u) + (psum ? psum[i] : 0u);
             ++o;
         }
@@ -582,19 +612,19 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
 // slot as present in the count table.  Window w belongs to prior p = the last with
 // pw0[p] <= w (every listed prior has at least one window); it starts at byte
 // pstart[p] + (w - pw0[p]) of seq.
-template <bool WIDE, typename Ev>
+template <int KW, typename Ev>
 __global__ void k_prior(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ pstart,
                         const uint64_t* __restrict__ pw0, const uint32_t* __restrict__ plib, uint32_t n_priors,
-                        uint64_t n_win, uint32_t k, Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                        uint64_t n_win, uint32_t k, Slot<nw_of(KW), Ev>* __restrict__ tab,
                         const LibDesc* __restrict__ libs, uint32_t* __restrict__ psum, uint8_t* __restrict__ pflag,
                         LibOut* __restrict__ out) {
-    constexpr int NW = WIDE ? 2 : 1;
+    constexpr int NW = nw_of(KW);
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < n_win;
          w += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = find_region(pw0, n_priors, w);
         const uint8_t* s = seq + pstart[p] + (w - pw0[p]);
-        Kv v{0, 0};
+        Kv v{0, 0, 0, 0};
         bool ok = true;
         for (uint32_t i = 0; i < k; ++i) {
             const uint32_t c = s[i] | 0x20u;  // 'A' and 'a' both map to 'a', and so on
@@ -607,12 +637,14 @@ __global__ void k_prior(const uint8_t* __restrict__ seq, const uint64_t* __restr
                 ok = false;
                 break;
             }
+            v.w3 = (v.w3 << 2) | (v.w2 >> 62);
+            v.w2 = (v.w2 << 2) | (v.hi >> 62);
             v.hi = (v.hi << 2) | (v.lo >> 62);
             v.lo = (v.lo << 2) | code;
         }
         if (!ok) continue;
         uint64_t key[NW];
-        pack_key<NW>(canonical<WIDE>(v, k), key);
+        pack_key<NW>(canonical<KW>(v, k), key);
         const uint32_t l = plib[p];
         const LibDesc& L = libs[l];
         const uint64_t b = L.bucket0 + home_bucket<NW>(key, L.nbuckets);
@@ -662,6 +694,12 @@ __global__ void k_unpack_keys(const Key128* __restrict__ in, uint64_t* __restric
     }
 }
 
+// The library of every sorted wide entry into the hi word array.
+__global__ void k_wide_lib(const WideEntry* __restrict__ in, uint64_t* __restrict__ hi, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        hi[i] = in[i].lib;
+}
+
 // ---------------------------------------------------------------------------
 // learned first pass: features of the count-table entries
 // ---------------------------------------------------------------------------
@@ -671,17 +709,17 @@ constexpr int N_FEAT = 16;
 // Sum of the k base qualities of every window of library `lib`, added to its k-mer's slot
 // of the (uncorrected) table.  aqoff[a] = quality offset of active read a, or ~0 when
 // the read has none.
-template <bool WIDE, typename Ev>
+template <int KW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_qsum(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
                                                 const uint64_t* __restrict__ act_off,
                                                 const uint32_t* __restrict__ act_lib,
                                                 const uint32_t* __restrict__ tile_first,
                                                 const uint64_t* __restrict__ aqoff, const uint8_t* __restrict__ quals,
                                                 uint64_t n_active, uint64_t n_occ, uint64_t tile0, uint64_t tile1,
-                                                uint32_t k, uint32_t lib, const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                uint32_t k, uint32_t lib, const Slot<nw_of(KW), Ev>* __restrict__ tab,
                                                 const LibDesc* __restrict__ libs,
                                                 unsigned long long* __restrict__ qsum) {
-    constexpr int NW = WIDE ? 2 : 1;
+    constexpr int NW = nw_of(KW);
     __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
     const LibDesc& L = libs[lib];
     for (uint64_t tile = tile0 + blockIdx.x; tile < tile1; tile += gridDim.x) {
@@ -699,7 +737,7 @@ __global__ void __launch_bounds__(BLOCK) k_qsum(const uint8_t* __restrict__ data
             if (qo == ~0ull) continue;
             const uint32_t pos = (uint32_t)(w - s_ev[lo]);
             uint64_t key[NW];
-            pack_key<NW>(canonical<WIDE>(load_window<WIDE>(data, act_off[a0 + lo], pos, k), k), key);
+            pack_key<NW>(canonical<KW>(load_window<KW>(data, act_off[a0 + lo], pos, k), k), key);
             const uint64_t i =
                 table_find<NW, Ev>(tab, L.bucket0, L.nbuckets, L.bucket0 + home_bucket<NW>(key, L.nbuckets), key);
             if (i == ~0ull) continue;
@@ -711,12 +749,12 @@ __global__ void __launch_bounds__(BLOCK) k_qsum(const uint8_t* __restrict__ data
 }
 
 // Raw count of a k-mer value in the library's table (0 when absent).
-template <bool WIDE, typename Ev>
-__device__ __forceinline__ uint32_t table_count(const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab, const LibDesc& L,
+template <int KW, typename Ev>
+__device__ __forceinline__ uint32_t table_count(const Slot<nw_of(KW), Ev>* __restrict__ tab, const LibDesc& L,
                                                 const Kv& v, uint32_t k, uint64_t* slot) {
-    constexpr int NW = WIDE ? 2 : 1;
+    constexpr int NW = nw_of(KW);
     uint64_t key[NW];
-    pack_key<NW>(canonical<WIDE>(v, k), key);
+    pack_key<NW>(canonical<KW>(v, k), key);
     const uint64_t i = table_find<NW, Ev>(tab, L.bucket0, L.nbuckets, L.bucket0 + home_bucket<NW>(key, L.nbuckets), key);
     if (slot) *slot = i;
     return i == ~0ull ? 0u : tab[i].cnt;
@@ -726,11 +764,11 @@ __device__ __forceinline__ uint32_t table_count(const Slot<WIDE ? 2 : 1, Ev>* __
 // mean quality of its occurrences' bases, and statistics of the raw counts of its 3k
 // Hamming-1 neighbours (canonical), relative to the library's cutoff and 95th-percentile
 // count.
-template <bool WIDE, typename Ev>
+template <int KW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo,
                                                     const uint32_t* __restrict__ cnt, uint64_t e0, uint64_t n,
                                                     SortLayout lay, uint32_t k, uint32_t lib,
-                                                    const Slot<WIDE ? 2 : 1, Ev>* __restrict__ tab,
+                                                    const Slot<nw_of(KW), Ev>* __restrict__ tab,
                                                     const LibDesc* __restrict__ libs, const LibOut* __restrict__ outs,
                                                     const unsigned long long* __restrict__ qsum,
                                                     float* __restrict__ feat) {
@@ -739,7 +777,7 @@ __global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = e0 + t;
         // entry key -> k-mer value (the inverse of encode_entry)
-        Kv v;
+        Kv v{0, 0, 0, 0};
         if (lay.single) {
             v.lo = 2 * k >= 64 ? lo[e] : (lo[e] & ((1ull << (2 * k)) - 1));
             v.hi = 0;
@@ -754,7 +792,7 @@ __global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__
         }
         const float c = (float)cnt[e];
         uint64_t slot = ~0ull;
-        (void)table_count<WIDE, Ev>(tab, L, v, k, &slot);
+        (void)table_count<KW, Ev>(tab, L, v, k, &slot);
         const float q = (slot != ~0ull && c > 0.f) ? (float)qsum[slot] / (c * (float)k) : 0.f;
         uint32_t mx = 0, mx2 = 0, present = 0, above_cut = 0, above_c = 0;
         float sum = 0.f;
@@ -766,7 +804,7 @@ __global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__
                 Kv y = v;
                 if (sh < 64) y.lo = (y.lo & ~(3ull << sh)) | ((uint64_t)b << sh);
                 else y.hi = (y.hi & ~(3ull << (sh - 64))) | ((uint64_t)b << (sh - 64));
-                const uint32_t ny = table_count<WIDE, Ev>(tab, L, y, k, nullptr);
+                const uint32_t ny = table_count<KW, Ev>(tab, L, y, k, nullptr);
                 if (ny > mx) {
                     mx2 = mx;
                     mx = ny;
@@ -1000,6 +1038,7 @@ struct kc_ctx {
     std::vector<std::pair<uint32_t, uint32_t>> groups;  // [lib begin, lib end) sharing d_fs
     // count entries (sorted) + scratch
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
+    DevBuf d_wide, d_wide2;  // k > 63: count-table entries as (V, library) records
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
@@ -1007,7 +1046,8 @@ struct kc_ctx {
     SortLayout lay{};
     kc_tune tune{1, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
     uint32_t n_cu = 0;
-    bool wide = false, ev64 = false, built = false, counted = false;
+    uint32_t kw = 1;  // value words of the current k (1, 2 or 4)
+    bool ev64 = false, built = false, counted = false;
     uint64_t* h_stage = nullptr;  // small pinned staging
     std::vector<LibOut> h_out;
     // timing
@@ -1020,6 +1060,11 @@ struct kc_ctx {
     int fail(int code, const std::string& m) {
         err = m;
         return code;
+    }
+    // Table slots per 64-byte bucket for the current k and event width.
+    uint64_t slots_per_bucket() const {
+        if (kw == 4) return 1;
+        return (kw == 2 || ev64) ? 2 : 4;
     }
     bool timing() const { return (o.flags & KC_FLAG_TIMING) != 0 && !capturing; }
     // graph replay (kc_graph_run)
@@ -1207,9 +1252,9 @@ struct Run {
 
     // k_insert over every tile with the context's tuning (windows in flight per thread,
     // workgroups per CU; 0 = one workgroup per tile).
-    template <bool WIDE, typename Ev, bool TRACK, bool SOLID>
+    template <int KW, typename Ev, bool TRACK, bool SOLID>
     int launch_insert(const uint8_t* data, int kid) {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         const uint64_t tiles = c->n_tiles;
         if (!tiles) return KC_OK;
         uint64_t g = tiles;
@@ -1217,7 +1262,7 @@ struct Run {
         if (g > 0xFFFFFFFFull) return fail(KC_EUNSUPPORTED, "too many tiles for one launch");
         hipStream_t st = c->st;
 #define KC_INSERT_LAUNCH(W)                                                                                       \
-    TIMED(kid, (k_insert<WIDE, Ev, TRACK, SOLID, W><<<(uint32_t)g, BLOCK, 0, st>>>(                              \
+    TIMED(kid, (k_insert<KW, Ev, TRACK, SOLID, W><<<(uint32_t)g, BLOCK, 0, st>>>(                              \
                    data, c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), \
                    c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ, tiles, c->o.k,                         \
                    c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), 1u << 20, c->d_out.as<LibOut>())))
@@ -1228,7 +1273,7 @@ struct Run {
         return KC_OK;
     }
 
-    template <bool WIDE>
+    template <int KW>
     int estimate_distinct() {
         hipStream_t st = c->st;
         const size_t nreg = (size_t)L() * HLL_M;
@@ -1236,7 +1281,7 @@ struct Run {
         k_fill<uint32_t><<<grid_for(nreg), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), nreg, 0u);
         KC_TRY(hipGetLastError());
         const uint32_t g = (uint32_t)std::min<uint64_t>(c->n_tiles, 2048);
-        TIMED(KID_HLL, (k_hll<WIDE><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
+        TIMED(KID_HLL, (k_hll<KW><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
                                                          c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
                                                          c->d_tile_first.as<uint32_t>(), c->n_active, c->n_occ,
                                                          c->n_tiles, c->o.k, c->d_hll.as<uint32_t>())));
@@ -1325,6 +1370,7 @@ struct Run {
             d.M = make_modulus(lib.num_words * 64);
             d.num_words = lib.num_words;
             d.boost = prior_boost(lib.min_cov);
+            d.k = c->o.k;
             blooms[l] = LibBloom{d.solid, d.M};
             lb0[l] = lib.bucket0;
             lw0[l] = lib.word0;
@@ -1360,12 +1406,12 @@ struct Run {
         return KC_OK;
     }
 
-    template <bool WIDE, typename Ev>
+    template <int KW, typename Ev>
     int bloom_build() {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
-        int rc = estimate_distinct<WIDE>();
+        int rc = estimate_distinct<KW>();
         if (rc) return rc;
         std::vector<uint32_t> grow(L(), 0);
         for (int attempt = 0;; ++attempt) {
@@ -1376,7 +1422,7 @@ struct Run {
             const uint64_t n_slots = c->total_buckets * SPB;
             TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(
                                  c->d_tab.as<Slot<NW, Ev>>(), n_slots)));
-            rc = launch_insert<WIDE, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+            rc = launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
             if (rc) return rc;
             rc = read_out();
             if (rc) return rc;
@@ -1475,6 +1521,20 @@ struct Run {
         if (n == 0) return KC_OK;
         const SortLayout& lay = c->lay;
         size_t tb = 0;
+        if (c->kw == 4) {  // (library, V) records: one merge sort, then the library words
+            KC_TRY(rocprim::merge_sort(nullptr, tb, c->d_wide.as<WideEntry>(), c->d_wide2.as<WideEntry>(),
+                                       c->d_cnt.as<uint32_t>(), c->d_cnt2.as<uint32_t>(), (size_t)n, WideLess(), st));
+            KC_TRY(c->d_temp.ensure(tb));
+            tb = c->d_temp.bytes;
+            TIMED(KID_SORT, KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_wide.as<WideEntry>(),
+                                                        c->d_wide2.as<WideEntry>(), c->d_cnt.as<uint32_t>(),
+                                                        c->d_cnt2.as<uint32_t>(), (size_t)n, WideLess(), st)));
+            std::swap(c->d_wide, c->d_wide2);
+            std::swap(c->d_cnt, c->d_cnt2);
+            k_wide_lib<<<grid_for(n), BLOCK, 0, st>>>(c->d_wide.as<WideEntry>(), c->d_hi.as<uint64_t>(), n);
+            KC_TRY(hipGetLastError());
+            return KC_OK;
+        }
         if (lay.single) {
             const uint32_t bits = lay.single_bits;
             KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_lo.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
@@ -1530,9 +1590,9 @@ struct Run {
 
     // Upload every library's prior windows and add them to the (final) count table.
     // Sets `any` when at least one prior window exists.
-    template <bool WIDE, typename Ev>
+    template <int KW, typename Ev>
     int apply_priors(uint64_t slots, bool& any) {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         hipStream_t st = c->st;
         const uint32_t k = c->o.k;
         std::vector<uint8_t> seq;
@@ -1571,7 +1631,7 @@ struct Run {
         k_fill<uint32_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_psum.as<uint32_t>(), slots, 0u);
         k_fill<uint8_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_pflag.as<uint8_t>(), slots, (uint8_t)0);
         KC_TRY(hipGetLastError());
-        TIMED(KID_PRIOR, (k_prior<WIDE, Ev><<<grid_for(nw), BLOCK, 0, st>>>(
+        TIMED(KID_PRIOR, (k_prior<KW, Ev><<<grid_for(nw), BLOCK, 0, st>>>(
                              c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
                              c->d_plib.as<uint32_t>(), np, nw, k, c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(),
                              c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>())));
@@ -1585,9 +1645,9 @@ struct Run {
     }
 
     // Stage 3 after the Bloom build: optional Hammer + recount, compaction, sort, cutoff.
-    template <bool WIDE, typename Ev>
+    template <int KW, typename Ev>
     int count() {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         hipStream_t st = c->st;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>();
         k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
@@ -1603,7 +1663,7 @@ struct Run {
             KC_TRY(c->d_corr.ensure(c->d_data.bytes));
             KC_TRY(hipMemcpyAsync(c->d_corr.p, c->d_data.p, c->d_data.bytes, hipMemcpyDeviceToDevice, st));
             if (c->n_active) {
-                TIMED(KID_HAMMER, (k_hammer<WIDE><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
+                TIMED(KID_HAMMER, (k_hammer<KW><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
                                       c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(),
                                       c->n_active, c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(),
                                       c->d_rlib.as<uint32_t>(), c->d_blooms.as<LibBloom>(), c->o.k,
@@ -1615,7 +1675,7 @@ struct Run {
             TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
                                  c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
             if (c->n_tiles) {
-                rc = launch_insert<WIDE, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+                rc = launch_insert<KW, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
                 if (rc) return rc;
             }
             rc = read_out();
@@ -1630,7 +1690,7 @@ struct Run {
         }
         const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
         bool priors = false;
-        rc = ec ? apply_priors<WIDE, uint32_t>(slots, priors) : apply_priors<WIDE, Ev>(slots, priors);
+        rc = ec ? apply_priors<KW, uint32_t>(slots, priors) : apply_priors<KW, Ev>(slots, priors);
         if (rc) return rc;
         const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
         const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
@@ -1641,16 +1701,20 @@ struct Run {
         KC_TRY(c->d_hi2.ensure(cap * 8));
         KC_TRY(c->d_lo2.ensure(cap * 8));
         KC_TRY(c->d_cnt2.ensure(cap * 4));
+        if (NW == 5) {
+            KC_TRY(c->d_wide.ensure(cap * sizeof(WideEntry)));
+            KC_TRY(c->d_wide2.ensure(cap * sizeof(WideEntry)));
+        }
         if (ec) {
             TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                                    c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
                                    c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag)));
+                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag, c->d_wide.as<WideEntry>())));
         } else {
             TIMED(KID_COMPACT, (k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                                    c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
                                    c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag)));
+                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag, c->d_wide.as<WideEntry>())));
         }
         KC_TRY(hipMemcpyAsync(c->h_stage, ctr + 4, 8, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
@@ -1682,9 +1746,9 @@ struct Run {
     // results land where the eager run's did (d_hi/d_lo/d_cnt, d_sflags, d_out); d_out
     // and the counters are cleared once, so the caller can check the overflow flags,
     // the read-length flag and the number of compacted entries after a replay.
-    template <bool WIDE, typename Ev>
+    template <int KW, typename Ev>
     int enqueue_planned() {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
         const bool ec = c->o.error_correct != 0;
@@ -1699,7 +1763,7 @@ struct Run {
         const uint64_t n_slots = c->total_buckets * SPB;
         k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(), n_slots);
         KC_TRY(hipGetLastError());
-        rc = launch_insert<WIDE, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+        rc = launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
         if (rc) return rc;
         rc = bloom_groups<NW, Ev>(false);
         if (rc) return rc;
@@ -1708,7 +1772,7 @@ struct Run {
             k_copy_bytes<<<grid_for(c->d_data.bytes / 8 + 1), BLOCK, 0, st>>>(c->d_data.as<uint8_t>(),
                                                                              c->d_corr.as<uint8_t>(), c->d_data.bytes);
             if (c->n_active)
-                k_hammer<WIDE><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
+                k_hammer<KW><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
                     c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(), c->n_active,
                     c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(), c->d_rlib.as<uint32_t>(),
                     c->d_blooms.as<LibBloom>(), c->o.k, &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
@@ -1717,7 +1781,7 @@ struct Run {
             k_table_clear<NW, uint32_t><<<grid_for(n_slots32), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, uint32_t>>(),
                                                                               n_slots32);
             KC_TRY(hipGetLastError());
-            rc = launch_insert<WIDE, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+            rc = launch_insert<KW, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
             if (rc) return rc;
         }
         const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : SPB);
@@ -1726,12 +1790,12 @@ struct Run {
             k_fill<uint32_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_psum.as<uint32_t>(), slots, 0u);
             k_fill<uint8_t><<<grid_for(slots), BLOCK, 0, st>>>(c->d_pflag.as<uint8_t>(), slots, (uint8_t)0);
             if (ec)
-                k_prior<WIDE, uint32_t><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
+                k_prior<KW, uint32_t><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
                     c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
                     c->d_plib.as<uint32_t>(), c->g_np, c->g_nw, c->o.k, c->d_tab.as<Slot<NW, uint32_t>>(),
                     c->d_libs.as<LibDesc>(), c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>());
             else
-                k_prior<WIDE, Ev><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
+                k_prior<KW, Ev><<<grid_for(c->g_nw), BLOCK, 0, st>>>(
                     c->d_pseq.as<uint8_t>(), c->d_pstart.as<uint64_t>(), c->d_pw0.as<uint64_t>(),
                     c->d_plib.as<uint32_t>(), c->g_np, c->g_nw, c->o.k, c->d_tab.as<Slot<NW, Ev>>(),
                     c->d_libs.as<LibDesc>(), c->d_psum.as<uint32_t>(), c->d_pflag.as<uint8_t>(), c->d_out.as<LibOut>());
@@ -1743,16 +1807,24 @@ struct Run {
         if (ec)
             k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                 c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
-                c->lay, c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag);
+                c->lay, c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag,
+                c->d_wide2.as<WideEntry>());
         else
             k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                 c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(), c->lay,
-                c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag);
+                c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag,
+                c->d_wide2.as<WideEntry>());
         KC_TRY(hipGetLastError());
         const uint64_t n = c->g_entries;
         if (n) {
             size_t tb = c->d_temp.bytes;
-            if (c->lay.single) {
+            if (NW == 5) {
+                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
+                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
+                                           st));
+                k_wide_lib<<<grid_for(n), BLOCK, 0, st>>>(c->d_wide.as<WideEntry>(), c->d_hi.as<uint64_t>(), n);
+                KC_TRY(hipGetLastError());
+            } else if (c->lay.single) {
                 KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
                                            c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
                                            rocprim::less<uint64_t>(), st));
@@ -1786,9 +1858,9 @@ struct Run {
 
     // The learned first pass's features of library lib's count-table entries, into d_feat
     // (entry order, N_FEAT floats each).  The table must hold the uncorrected counts.
-    template <bool WIDE, typename Ev>
+    template <int KW, typename Ev>
     int features(uint32_t lib) {
-        constexpr int NW = WIDE ? 2 : 1;
+        constexpr int NW = nw_of(KW);
         hipStream_t st = c->st;
         const HostLib& H = c->libs[lib];
         const uint64_t n = H.info.n_counts, e0 = H.entry0;
@@ -1810,7 +1882,7 @@ struct Run {
             const uint64_t t0 = H.ev0 / TILE_WINDOWS;
             const uint64_t t1 = (H.ev0 + H.info.n_occ + TILE_WINDOWS - 1) / TILE_WINDOWS;
             const uint32_t g = (uint32_t)std::min<uint64_t>(t1 - t0, 4096);
-            k_qsum<WIDE, Ev><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
+            k_qsum<KW, Ev><<<g, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(),
                                                   c->d_act_off.as<uint64_t>(), c->d_act_lib.as<uint32_t>(),
                                                   c->d_tile_first.as<uint32_t>(), c->d_aqoff.as<uint64_t>(),
                                                   c->d_quals.as<uint8_t>(), c->n_active, c->n_occ, t0, t1, c->o.k, lib,
@@ -1821,7 +1893,7 @@ struct Run {
         }
         KC_TRY(c->d_feat.ensure(std::max<uint64_t>(n, 1) * N_FEAT * 4));
         if (n) {
-            k_features<WIDE, Ev><<<grid_for(n), BLOCK, 0, st>>>(
+            k_features<KW, Ev><<<grid_for(n), BLOCK, 0, st>>>(
                 c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), e0, n, c->lay, c->o.k, lib,
                 c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>(),
                 c->d_qsum.as<unsigned long long>(), c->d_feat.as<float>());
@@ -1831,15 +1903,20 @@ struct Run {
     }
 
     int features_any(uint32_t lib) {
-        if (!c->wide) return c->ev64 ? features<false, uint64_t>(lib) : features<false, uint32_t>(lib);
-        return c->ev64 ? features<true, uint64_t>(lib) : features<true, uint32_t>(lib);
+        if (c->kw == 4) return fail(KC_EUNSUPPORTED, "learned-pass features support k <= 63");
+        if (c->kw == 1) return c->ev64 ? features<1, uint64_t>(lib) : features<1, uint32_t>(lib);
+        return c->ev64 ? features<2, uint64_t>(lib) : features<2, uint32_t>(lib);
     }
 
     // Buffers the planned run needs beyond the eager run's (called before capture).
     int prepare_planned() {
         const uint64_t n = c->g_entries;
         size_t need = 0;
-        if (c->lay.single) {
+        if (c->kw == 4) {
+            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
+                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
+                                       c->st));
+        } else if (c->lay.single) {
             KC_TRY(rocprim::merge_sort(nullptr, need, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
                                        c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
                                        rocprim::less<uint64_t>(), c->st));
@@ -1896,7 +1973,8 @@ extern "C" {
 int kc_create(kc_ctx** out, const kc_opts* opts) {
     if (!out || !opts) return KC_EINVAL;
     *out = nullptr;
-    if (opts->k < 1 || opts->k > KC_MAX_K) return KC_EINVAL;
+    if (opts->k < 1 || opts->k > 128) return KC_EINVAL;  // Kmer256 holds k <= 128 (dna.rs:134)
+    if (opts->k > KC_MAX_K) return KC_EUNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_EHIP;
     if (opts->device < 0 || opts->device >= ndev) return KC_EINVAL;
@@ -1908,7 +1986,7 @@ int kc_create(kc_ctx** out, const kc_opts* opts) {
         delete ctx;
         return KC_EHIP;
     }
-    ctx->wide = opts->k > 32;
+    ctx->kw = kw_of_k(opts->k);
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, opts->device) != hipSuccess || ncu <= 0)
         ncu = 256;
@@ -1986,9 +2064,10 @@ int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint6
 
 int kc_set_k(kc_ctx* ctx, uint32_t k) {
     if (!ctx) return KC_EINVAL;
-    if (k < 1 || k > KC_MAX_K) return ctx->fail(k > KC_MAX_K ? KC_EUNSUPPORTED : KC_EINVAL, "k outside 1..63");
+    if (k < 1 || k > KC_MAX_K)
+        return ctx->fail(k == 128 ? KC_EUNSUPPORTED : KC_EINVAL, "k outside 1..127");
     ctx->o.k = k;
-    ctx->wide = k > 32;
+    ctx->kw = kw_of_k(k);
     ctx->built = ctx->counted = false;
     ctx->graph_release();
     return KC_OK;
@@ -2028,17 +2107,22 @@ int kc_bloom_build(kc_ctx* ctx) {
     if (ctx->n_occ == 0) {
         std::vector<uint32_t> grow(ctx->libs.size(), 0);
         for (auto& lib : ctx->libs) lib.est = 0;
-        rc = ctx->wide ? run.plan<2, uint32_t>(grow) : run.plan<1, uint32_t>(grow);
+        rc = ctx->kw == 1 ? run.plan<1, uint32_t>(grow)
+             : ctx->kw == 2 ? run.plan<2, uint32_t>(grow)
+                            : run.plan<5, uint32_t>(grow);
         if (rc) return rc;
         // an empty table for prior windows (sized by the plan)
-        const uint64_t n_slots = ctx->total_buckets * (ctx->wide ? 2 : 4);
+        const uint64_t n_slots = ctx->total_buckets * ctx->slots_per_bucket();
         if (n_slots) {
-            if (ctx->wide)
+            if (ctx->kw == 1)
+                k_table_clear<1, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
+                    ctx->d_tab.as<Slot<1, uint32_t>>(), n_slots);
+            else if (ctx->kw == 2)
                 k_table_clear<2, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
                     ctx->d_tab.as<Slot<2, uint32_t>>(), n_slots);
             else
-                k_table_clear<1, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
-                    ctx->d_tab.as<Slot<1, uint32_t>>(), n_slots);
+                k_table_clear<5, uint32_t><<<grid_for(n_slots), BLOCK, 0, ctx->st>>>(
+                    ctx->d_tab.as<Slot<5, uint32_t>>(), n_slots);
             CTX_TRY(hipGetLastError());
         }
         k_fill<uint64_t><<<grid_for(ctx->total_words), BLOCK, 0, ctx->st>>>(ctx->d_seen.as<uint64_t>(),
@@ -2049,10 +2133,12 @@ int kc_bloom_build(kc_ctx* ctx) {
         rc = run.clear_out();
         if (rc) return rc;
         rc = run.read_out();
-    } else if (!ctx->wide) {
-        rc = ctx->ev64 ? run.bloom_build<false, uint64_t>() : run.bloom_build<false, uint32_t>();
+    } else if (ctx->kw == 1) {
+        rc = ctx->ev64 ? run.bloom_build<1, uint64_t>() : run.bloom_build<1, uint32_t>();
+    } else if (ctx->kw == 2) {
+        rc = ctx->ev64 ? run.bloom_build<2, uint64_t>() : run.bloom_build<2, uint32_t>();
     } else {
-        rc = ctx->ev64 ? run.bloom_build<true, uint64_t>() : run.bloom_build<true, uint32_t>();
+        rc = ctx->ev64 ? run.bloom_build<4, uint64_t>() : run.bloom_build<4, uint32_t>();
     }
     if (rc) return rc;
     CTX_TRY(hipEventRecord(e1, ctx->st));
@@ -2064,7 +2150,7 @@ int kc_bloom_build(kc_ctx* ctx) {
         lib.info.ms_stage2 = ms;
         lib.info.num_words = lib.num_words;
         lib.info.n_distinct = ctx->h_out[l].n_distinct;
-        lib.info.table_capacity = lib.nbuckets * (64 / (ctx->wide || ctx->ev64 ? 32 : 16));
+        lib.info.table_capacity = lib.nbuckets * ctx->slots_per_bucket();
     }
     ctx->built = true;
     return KC_OK;
@@ -2094,10 +2180,12 @@ int kc_count(kc_ctx* ctx, kc_info* info) {
             CTX_TRY(ctx->d_corr.ensure(ctx->d_data.bytes));
             CTX_TRY(hipMemcpyAsync(ctx->d_corr.p, ctx->d_data.p, ctx->d_data.bytes, hipMemcpyDeviceToDevice, ctx->st));
         }
-    } else if (!ctx->wide) {
-        rc = ctx->ev64 ? run.count<false, uint64_t>() : run.count<false, uint32_t>();
+    } else if (ctx->kw == 1) {
+        rc = ctx->ev64 ? run.count<1, uint64_t>() : run.count<1, uint32_t>();
+    } else if (ctx->kw == 2) {
+        rc = ctx->ev64 ? run.count<2, uint64_t>() : run.count<2, uint32_t>();
     } else {
-        rc = ctx->ev64 ? run.count<true, uint64_t>() : run.count<true, uint32_t>();
+        rc = ctx->ev64 ? run.count<4, uint64_t>() : run.count<4, uint32_t>();
     }
     if (rc) return rc;
     CTX_TRY(hipEventRecord(e1, ctx->st));
@@ -2143,8 +2231,9 @@ static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
     CTX_TRY(hipStreamSynchronize(ctx->st));
     CTX_TRY(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
     ctx->capturing = true;
-    if (!ctx->wide) rc = ctx->ev64 ? run.enqueue_planned<false, uint64_t>() : run.enqueue_planned<false, uint32_t>();
-    else rc = ctx->ev64 ? run.enqueue_planned<true, uint64_t>() : run.enqueue_planned<true, uint32_t>();
+    if (ctx->kw == 1) rc = ctx->ev64 ? run.enqueue_planned<1, uint64_t>() : run.enqueue_planned<1, uint32_t>();
+    else if (ctx->kw == 2) rc = ctx->ev64 ? run.enqueue_planned<2, uint64_t>() : run.enqueue_planned<2, uint32_t>();
+    else rc = ctx->ev64 ? run.enqueue_planned<4, uint64_t>() : run.enqueue_planned<4, uint32_t>();
     ctx->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->st, &g);
@@ -2270,6 +2359,29 @@ int kc_bloom_query(kc_ctx* ctx, const uint64_t* keys4, size_t n, uint8_t* out) {
     return kc_library_bloom_query(ctx, 0, keys4, n, out);
 }
 
+// The Kmer256 limbs (lo u128 = limbs 0-1, hi u128 = limbs 2-3) of the k-mer whose 2k-bit
+// value is v[0..3] (lowest word first): for k <= 64, Kmer256(V, 0); for k > 64,
+// Kmer256(V >> 2(k-64), V mod 2^(2(k-64))) (dna.rs:134-138, 299-321).
+static void kmer256_limbs_host(const uint64_t v[4], uint32_t k, uint64_t* w) {
+    if (k <= 64) {
+        w[0] = v[0];
+        w[1] = v[1];
+        w[2] = w[3] = 0;
+        return;
+    }
+    const uint32_t s = 2 * k - 128;  // 2..126
+    uint64_t t[4];
+    const uint32_t q = s >> 6, r = s & 63;
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint64_t a = i + q < 4 ? v[i + q] : 0, b = i + q + 1 < 4 ? v[i + q + 1] : 0;
+        t[i] = r ? (a >> r) | (b << (64 - r)) : a;
+    }
+    w[0] = t[0];
+    w[1] = t[1];
+    w[2] = s >= 64 ? v[0] : (v[0] & ((1ull << s) - 1));
+    w[3] = s > 64 ? (v[1] & ((1ull << (s - 64)) - 1)) : 0;
+}
+
 int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask,
                              size_t n) {
     if (!ctx) return KC_EINVAL;
@@ -2285,6 +2397,14 @@ int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_
         CTX_TRY(hipMemcpyAsync(solid_mask, ctx->d_sflags.as<uint8_t>() + e0, n, hipMemcpyDeviceToHost, ctx->st));
     std::vector<uint64_t> hi, lo;
     const SortLayout& lay = ctx->lay;
+    if (keys4 && ctx->kw == 4) {
+        std::vector<WideEntry> we(n);
+        CTX_TRY(hipMemcpyAsync(we.data(), ctx->d_wide.as<WideEntry>() + e0, n * sizeof(WideEntry),
+                               hipMemcpyDeviceToHost, ctx->st));
+        CTX_TRY(hipStreamSynchronize(ctx->st));
+        for (size_t i = 0; i < n; ++i) kmer256_limbs_host(we[i].v, lay.k, keys4 + 4 * i);
+        return KC_OK;
+    }
     if (keys4) {
         lo.resize(n);
         CTX_TRY(hipMemcpyAsync(lo.data(), ctx->d_lo.as<uint64_t>() + e0, n * 8, hipMemcpyDeviceToHost, ctx->st));

@@ -237,16 +237,15 @@ int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_
         F_TRY(d_lib.ensure(sizeof(LibBloom)));
         F_TRY(hipMemcpyAsync(d_lib.p, &lb, sizeof(LibBloom), hipMemcpyHostToDevice, f->st));
         const uint32_t g = grid_for(active.size(), 64, 65536);
-        if (k > 32)
-            k_hammer<true><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),
-                                                 active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,
-                                                 d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,
-                                                 d_ctr.as<uint32_t>() + 4);
-        else
-            k_hammer<false><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),
-                                                  active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,
-                                                  d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,
-                                                  d_ctr.as<uint32_t>() + 4);
+#define KC_HAMMER_LAUNCH(KW)                                                                                  \
+    k_hammer<KW><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),        \
+                                      active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,       \
+                                      d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,               \
+                                      d_ctr.as<uint32_t>() + 4)
+        if (kw_of_k(k) == 4) KC_HAMMER_LAUNCH(4);
+        else if (kw_of_k(k) == 2) KC_HAMMER_LAUNCH(2);
+        else KC_HAMMER_LAUNCH(1);
+#undef KC_HAMMER_LAUNCH
         F_TRY(hipGetLastError());
     }
     uint64_t ctr[2] = {0, 0};

@@ -21,11 +21,11 @@ struct LibBloom {
     Modulus M;
 };
 
-template <bool WIDE>
+template <int KW>
 __device__ __forceinline__ bool lds_window_solid(const uint32_t* rd, uint32_t s, uint32_t k,
                                                  const uint64_t* __restrict__ solid, const Modulus& M) {
-    const Kv v = extract_be<WIDE>(rd + (s >> 4), (2 * s) & 31, k);
-    return bloom_is_solid(canonical<WIDE>(v, k), solid, M);
+    const Kv v = extract_be<KW>(rd + (s >> 4), (2 * s) & 31, k);
+    return bloom_is_solid<KW>(canonical<KW>(v, k), k, solid, M);
 }
 
 __device__ __forceinline__ uint32_t lds_get_base(const uint8_t* rb, uint32_t pos) {
@@ -40,7 +40,7 @@ __device__ __forceinline__ void lds_set_base(uint8_t* rb, uint32_t pos, uint32_t
 // Corrects the reads active[0..n_active) of data into corr (corr must already hold a copy
 // of data).  read_lib (nullable) gives each read's library; n_corrected[lib] counts the
 // reads modified (stride corrected_stride); too_long is set for reads longer than HAMMER_MAX_LEN (left unchanged).
-template <bool WIDE>
+template <int KW>
 __global__ void __launch_bounds__(64) k_hammer(const uint8_t* __restrict__ data, uint8_t* __restrict__ corr,
                                                const uint32_t* __restrict__ active, uint64_t n_active,
                                                const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ lengths,
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(64) k_hammer(const uint8_t* __restrict__ data,
         for (uint32_t j = 0; j < nb; ++j) rb[j] = data[off + j];
         bool modified = false;
         for (uint32_t i = 0; i + k <= L; ++i) {
-            if (lds_window_solid<WIDE>(rd, i, k, solid, M)) continue;
+            if (lds_window_solid<KW>(rd, i, k, solid, M)) continue;
             bool have = false;
             uint32_t best_off = 0, best_c = 0, max_gain = 0;
             for (uint32_t o = 0; o < k; ++o) {
@@ -77,14 +77,14 @@ __global__ void __launch_bounds__(64) k_hammer(const uint8_t* __restrict__ data,
                 const uint32_t end = min(pos, L - k);
                 uint32_t base_solid = 0;
                 for (uint32_t s = start; s <= end; ++s)
-                    if (s != i && lds_window_solid<WIDE>(rd, s, k, solid, M)) ++base_solid;
+                    if (s != i && lds_window_solid<KW>(rd, s, k, solid, M)) ++base_solid;
                 for (uint32_t c = 0; c < 4; ++c) {
                     if (c == orig) continue;
                     lds_set_base(rb, pos, c);
-                    if (lds_window_solid<WIDE>(rd, i, k, solid, M)) {
+                    if (lds_window_solid<KW>(rd, i, k, solid, M)) {
                         uint32_t sc = 1;
                         for (uint32_t s = start; s <= end; ++s)
-                            if (s != i && lds_window_solid<WIDE>(rd, s, k, solid, M)) ++sc;
+                            if (s != i && lds_window_solid<KW>(rd, s, k, solid, M)) ++sc;
                         const uint32_t gain = sc > base_solid ? sc - base_solid : 0;
                         if (gain > max_gain) {
                             max_gain = gain;

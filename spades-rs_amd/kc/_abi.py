@@ -15,7 +15,7 @@ KC_ENOMEM = -2
 KC_EHIP = -3
 KC_ESTATE = -4
 KC_EUNSUPPORTED = -5
-KC_MAX_K = 63
+KC_MAX_K = 127
 KC_FLAG_TIMING = 1
 
 STATUS_NAMES = {KC_EINVAL: "KC_EINVAL", KC_ENOMEM: "KC_ENOMEM", KC_EHIP: "KC_EHIP",

@@ -185,7 +185,7 @@ class SpectrumEngine:
     def __init__(self, config, device=0, table_load=0.0, timing=False):
         cfg = config
         if not 1 <= cfg.k <= _abi.KC_MAX_K:
-            raise KcError(_abi.KC_EUNSUPPORTED if cfg.k > _abi.KC_MAX_K else _abi.KC_EINVAL,
+            raise KcError(_abi.KC_EUNSUPPORTED if cfg.k == 128 else _abi.KC_EINVAL,
                           f"k={cfg.k} outside 1..{_abi.KC_MAX_K}")
         o = _abi.kc_opts(device=device, k=cfg.k, bloom_bits=int(cfg.bloom_bits), min_coverage=float(cfg.min_coverage),
                          error_correct=int(bool(cfg.error_correct)), table_load=float(table_load),

@@ -64,3 +64,7 @@ def test_no_device_is_an_error_not_a_crash():
     assert _abi.lib().kc_create(ctypes.byref(h), ctypes.byref(o)) != 0
     assert _abi.lib().kc_filter_create(ctypes.byref(h), 0, 1 << 20) != 0
     assert _abi.lib().kc_create(ctypes.byref(h), None) == _abi.KC_EINVAL
+    # k is checked before the device: 1..127 supported, 128 valid for Kmer256 but not here
+    for k, want in ((0, _abi.KC_EINVAL), (128, _abi.KC_EUNSUPPORTED), (129, _abi.KC_EINVAL), (127, _abi.KC_EHIP)):
+        o = _abi.kc_opts(device=0, k=k, bloom_bits=1 << 20, min_coverage=5.0)
+        assert _abi.lib().kc_create(ctypes.byref(h), ctypes.byref(o)) == want, k

@@ -112,7 +112,8 @@ def test_hammer_reference_tests(oracle):
     assert ErrorCorrector(15).correct_read_in_place(b"ACGT", TwoTierFilter(1024)) == (False, b"ACGT")
 
 
-@pytest.mark.parametrize("k,bits", [(11, 1 << 14), (21, 1 << 22), (33, 1 << 22), (55, 1 << 12)])
+@pytest.mark.parametrize("k,bits", [(11, 1 << 14), (21, 1 << 22), (33, 1 << 22), (55, 1 << 12), (77, 1 << 20),
+                                    (101, 1 << 16)])
 def test_hammer_matches_oracle_on_noisy_reads(oracle, k, bits):
     store, _ = synth.uniform_genome_pairs(seed=100 + k, genome_size=4000, n_pairs=300, read_len=120,
                                           insert_mean=250, insert_sd=20, p_start=0.02)
@@ -148,6 +149,8 @@ def test_hammer_matches_oracle_on_noisy_reads(oracle, k, bits):
     (21, 1 << 20, False), (21, 1 << 16, False), (21, 1 << 20, True), (31, (1 << 20) + 64 * 5, False),
     (32, 1 << 18, False), (33, 1 << 20, False), (33, 1 << 20, True), (55, 1 << 20, False), (55, 4096, True),
     (63, 1 << 19, False), (11, 1 << 12, False), (5, 64 * 3, False), (1, 64, False),
+    (64, 1 << 18, False), (65, 1 << 20, False), (77, 1 << 20, True), (99, 1 << 16, False), (127, 1 << 20, True),
+    (127, 4096, False), (96, (1 << 18) + 64 * 3, True),
 ])
 def test_spectrum_parity_synthetic(oracle, k, bits, ec):
     store, _ = synth.uniform_genome_pairs(seed=7 * k + bits % 97 + ec, genome_size=20000, n_pairs=2000,
@@ -173,8 +176,8 @@ def test_spectrum_ragged_and_edge_reads(oracle):
     st = PackedReads()
     for r in modulo4_reads() + lcg_workload()[:500] + [b"A" * 21, b"T" * 40, b"ACGT" * 10]:
         st.add_read(r)
-    for k in (21, 33, 64 - 1):
-        assert_parity(oracle, st, k, 1 << 16, ec=(k == 21))
+    for k in (21, 33, 63, 64, 65, 127):
+        assert_parity(oracle, st, k, 1 << 16, ec=(k in (21, 65)))
 
 
 def test_spectrum_empty_and_all_short(oracle):
@@ -254,7 +257,8 @@ def _uneven_library(seed, genome, pairs, min_len=50, max_len=250):
     return out
 
 
-@pytest.mark.parametrize("k,ec", [(21, False), (33, True), (55, False), (31, True), (63, False)])
+@pytest.mark.parametrize("k,ec", [(21, False), (33, True), (55, False), (31, True), (63, False), (77, True),
+                                  (127, False)])
 def test_many_libraries_match_each_alone(oracle, k, ec):
     libs = [_uneven_library(1000 + i, genome=3000 + 1500 * i, pairs=150 + 40 * i) for i in range(5)]
     libs.insert(2, PackedReads())  # an empty library

@@ -53,7 +53,7 @@ def _oracle_check(oracle, eng, libs, k, bits, ec, mcs, priors):
         assert eng.info(i)["cutoff"] == S.cutoff
 
 
-@pytest.mark.parametrize("k,ec", [(33, False), (21, True), (55, True)])
+@pytest.mark.parametrize("k,ec", [(33, False), (21, True), (55, True), (99, True), (127, False)])
 def test_graph_replays_match_eager_and_oracle(oracle, k, ec):
     libs, genomes = zip(*[_uneven_library(700 + i, 2000 + 1700 * i, 120 + 60 * i) for i in range(4)])
     libs = list(libs)
@@ -127,8 +127,8 @@ def test_tune_rejects_bad_settings():
 
 
 def test_multik_over_resident_reads(oracle):
-    """multik.rs:109-178: k = 21, 33, 55 over one upload, with the previous k's sequences
-    injected as priors; then k back to 21 (buffers sized for 55 are reused)."""
+    """multik.rs:109-178: k = 21, 33, 55, 77, 127 over one upload, with the previous k's
+    sequences injected as priors; then k back to 21 (buffers sized for larger k reused)."""
     store, meta = synth.uniform_genome_pairs(seed=77, genome_size=6000, n_pairs=500, read_len=150,
                                              insert_mean=300, insert_sd=20, p_start=0.004)
     g = "".join("ACGT"[int(x)] for x in meta["genome"])
@@ -136,7 +136,8 @@ def test_multik_over_resident_reads(oracle):
     bits = 1 << 18
     eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=bits, error_correct=True))
     eng.upload(store)
-    for k, priors in ((21, None), (33, [g[0:800]]), (55, [g[0:800], g[3000:4000]]), (21, None)):
+    for k, priors in ((21, None), (33, [g[0:800]]), (55, [g[0:800], g[3000:4000]]), (77, [g[100:900]]),
+                      (127, [g[2000:2600], g[5000:5200]]), (21, None)):
         say(f"case multik k={k}")
         eng.set_k(k)
         eng.set_priors(priors or [])

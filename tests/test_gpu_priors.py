@@ -43,7 +43,8 @@ def _compare(eng, lib, S):
 
 
 @pytest.mark.parametrize("k,ec,min_cov", [(21, False, 5.0), (33, True, 5.0), (55, False, 13.0), (21, False, 1e20),
-                                          (63, True, 5.0), (11, False, 5.0), (32, False, 40.0)])
+                                          (63, True, 5.0), (11, False, 5.0), (32, False, 40.0), (77, True, 5.0),
+                                          (127, False, 5.0)])
 def test_priors_match_oracle(oracle, k, ec, min_cov):
     store, meta = synth.uniform_genome_pairs(seed=300 + k, genome_size=5000, n_pairs=400, read_len=120,
                                              insert_mean=250, insert_sd=20, p_start=0.003, p_end=0.008)
@@ -69,7 +70,7 @@ def test_priors_match_oracle(oracle, k, ec, min_cov):
     eng.close()
 
 
-@pytest.mark.parametrize("k", [21, 33])
+@pytest.mark.parametrize("k", [21, 33, 99])
 def test_priors_without_reads(oracle, k):
     store, meta = synth.uniform_genome_pairs(seed=7, genome_size=3000, n_pairs=10, read_len=100,
                                              insert_mean=200, insert_sd=20)

@@ -99,11 +99,13 @@ struct SortLayout {
     uint32_t k, lib_bits, lo_bits, hi_bits;  // hi_bits == 0: one word
     uint32_t single;                         // 1: (lib, V) packed in the lo word
     uint32_t single_bits;
+    uint32_t wide;                           // 1: (V, library) records (WideEntry)
 };
 
-// A count-table entry for k > 63: the k-mer value V (four words, lowest first) and its
-// library; entries sort as (library, V).  The library is also written into the hi word
-// array (layout: lib_bits == hi_bits, no value bits), which the count passes read.
+// A count-table entry when (library, V) does not fit two words (k > 63, or k close to 63
+// with many libraries): the k-mer value V (four words, lowest first) and its library;
+// entries sort as (library, V).  The library is also written into the hi word array
+// (layout: lib_bits == hi_bits, no value bits), which the count passes read.
 struct WideEntry {
     uint64_t v[4];
     uint32_t lib, pad;
@@ -122,7 +124,8 @@ SortLayout make_layout(uint32_t k, uint32_t n_libs) {
     s.k = k;
     s.lib_bits = 0;
     while ((1u << s.lib_bits) < n_libs) ++s.lib_bits;
-    if (k > 63) {
+    if (k > 63 || (2 * k > 63 && 2 * k - 63 + s.lib_bits > 64)) {
+        s.wide = 1;
         s.lo_bits = 0;
         s.hi_bits = s.lib_bits;
         s.single = 0;
@@ -590,7 +593,7 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
             const Slot<NW, Ev> sl = tab[i];
             const uint32_t lib = find_region(lib_b0, n_libs, i / SPB);
-            if (NW == 5) {
+            if (lay.wide) {
                 const Kv v = unpack_key<NW>(sl.k);
                 out_wide[o] = WideEntry{{v.lo, v.hi, v.w2, v.w3}, lib, 0u};
             } else {
@@ -771,14 +774,17 @@ __global__ void __launch_bounds__(BLOCK) k_features(const uint64_t* __restrict__
                                                     const Slot<nw_of(KW), Ev>* __restrict__ tab,
                                                     const LibDesc* __restrict__ libs, const LibOut* __restrict__ outs,
                                                     const unsigned long long* __restrict__ qsum,
-                                                    float* __restrict__ feat) {
+                                                    float* __restrict__ feat, const WideEntry* __restrict__ wide) {
     const LibDesc& L = libs[lib];
     const float cutoff = (float)outs[lib].cutoff, top = (float)max(outs[lib].top, 1u);
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = e0 + t;
         // entry key -> k-mer value (the inverse of encode_entry)
         Kv v{0, 0, 0, 0};
-        if (lay.single) {
+        if (lay.wide) {
+            v.lo = wide[e].v[0];
+            v.hi = wide[e].v[1];
+        } else if (lay.single) {
             v.lo = 2 * k >= 64 ? lo[e] : (lo[e] & ((1ull << (2 * k)) - 1));
             v.hi = 0;
         } else if (2 * k <= 63) {
@@ -1521,7 +1527,7 @@ struct Run {
         if (n == 0) return KC_OK;
         const SortLayout& lay = c->lay;
         size_t tb = 0;
-        if (c->kw == 4) {  // (library, V) records: one merge sort, then the library words
+        if (lay.wide) {  // (library, V) records: one merge sort, then the library words
             KC_TRY(rocprim::merge_sort(nullptr, tb, c->d_wide.as<WideEntry>(), c->d_wide2.as<WideEntry>(),
                                        c->d_cnt.as<uint32_t>(), c->d_cnt2.as<uint32_t>(), (size_t)n, WideLess(), st));
             KC_TRY(c->d_temp.ensure(tb));
@@ -1701,7 +1707,7 @@ struct Run {
         KC_TRY(c->d_hi2.ensure(cap * 8));
         KC_TRY(c->d_lo2.ensure(cap * 8));
         KC_TRY(c->d_cnt2.ensure(cap * 4));
-        if (NW == 5) {
+        if (c->lay.wide) {
             KC_TRY(c->d_wide.ensure(cap * sizeof(WideEntry)));
             KC_TRY(c->d_wide2.ensure(cap * sizeof(WideEntry)));
         }
@@ -1818,7 +1824,7 @@ struct Run {
         const uint64_t n = c->g_entries;
         if (n) {
             size_t tb = c->d_temp.bytes;
-            if (NW == 5) {
+            if (c->lay.wide) {
                 KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
                                            c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
                                            st));
@@ -1896,7 +1902,7 @@ struct Run {
             k_features<KW, Ev><<<grid_for(n), BLOCK, 0, st>>>(
                 c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), e0, n, c->lay, c->o.k, lib,
                 c->d_tab.as<Slot<NW, Ev>>(), c->d_libs.as<LibDesc>(), c->d_out.as<LibOut>(),
-                c->d_qsum.as<unsigned long long>(), c->d_feat.as<float>());
+                c->d_qsum.as<unsigned long long>(), c->d_feat.as<float>(), c->d_wide.as<WideEntry>());
             KC_TRY(hipGetLastError());
         }
         return KC_OK;
@@ -1912,7 +1918,7 @@ struct Run {
     int prepare_planned() {
         const uint64_t n = c->g_entries;
         size_t need = 0;
-        if (c->kw == 4) {
+        if (c->lay.wide) {
             KC_TRY(rocprim::merge_sort(nullptr, need, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
                                        c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
                                        c->st));
@@ -2397,7 +2403,7 @@ int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_
         CTX_TRY(hipMemcpyAsync(solid_mask, ctx->d_sflags.as<uint8_t>() + e0, n, hipMemcpyDeviceToHost, ctx->st));
     std::vector<uint64_t> hi, lo;
     const SortLayout& lay = ctx->lay;
-    if (keys4 && ctx->kw == 4) {
+    if (keys4 && lay.wide) {
         std::vector<WideEntry> we(n);
         CTX_TRY(hipMemcpyAsync(we.data(), ctx->d_wide.as<WideEntry>() + e0, n * sizeof(WideEntry),
                                hipMemcpyDeviceToHost, ctx->st));

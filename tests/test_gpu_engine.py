@@ -266,7 +266,7 @@ def test_many_libraries_match_each_alone(oracle, k, ec):
     short.pack_clean(b"ACGT" * 4)
     libs.append(short)  # reads shorter than k only
     bits = [1 << 16, (1 << 18) + 64, 4096, 1 << 20, 1 << 17, 1 << 15, 1 << 12]
-    n_libs = 2 if k == 63 else len(libs)  # 2k - 63 + log2(libraries) <= 64
+    n_libs = len(libs)  # at k = 63, (library, V) takes more than two words: the record layout
     eng = SpectrumEngine(AssemblerConfig(k=k, error_correct=ec))
     for i in range(n_libs):
         assert eng.add_library(libs[i], bloom_bits=bits[i], min_coverage=5.0 if i != 3 else 12.0) == i

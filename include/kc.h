@@ -185,6 +185,7 @@ typedef struct kc_packed_info {
     double ms_device;                     /* device time from the end of the upload to the end of the
                                              last kernel (includes the host reads of sizes between
                                              kernels, not the copies in and out) */
+    uint64_t n_quals;                     /* bytes of the quality side array (0 unless kept) */
 } kc_packed_info;
 /* One-shot form: a context for the call only. */
 int kc_fastq_pack(int32_t device, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
@@ -197,11 +198,16 @@ int kc_ingest_create(int32_t device, kc_ingest** out);
 void kc_ingest_destroy(kc_ingest* ctx);
 int kc_ingest_pack(kc_ingest* ctx, const uint8_t* r1, size_t r1_len, const uint8_t* r2, size_t r2_len,
                    kc_packed** out);
+/* Keep a quality side array (no counterpart in the reference, which drops qualities;
+ * the learned pass's features read it, kc_library_quals_set): the Phred value
+ * (character - 33, 0 below '!') of every packed base, read i's at quals[qoffsets[i] ..]. */
+int kc_ingest_keep_quals(kc_ingest* ctx, int32_t keep);
 /* The message of the calling thread's last failed ingestion call. */
 const char* kc_fastq_last_error(void);
 int kc_packed_info_get(const kc_packed* p, kc_packed_info* info);
 int kc_packed_export(const kc_packed* p, uint8_t* data, size_t data_len, uint64_t* offsets, uint32_t* lengths,
                      size_t n_reads);
+int kc_packed_export_quals(const kc_packed* p, uint8_t* quals, size_t n_quals, uint64_t* qoffsets, size_t n_reads);
 void kc_packed_destroy(kc_packed* p);
 
 /* ---- Tuning and graph replay ---------------------------------------------- */

@@ -432,9 +432,11 @@ struct MemFastx {
         }
         if (!is_fastq) read_line(line);  // the first '>' header
     }
-    // 1: a sequence in seq, 0: end of file, -1: malformed (err)
-    int next_seq(std::vector<uint8_t>& seq) {
+    // 1: a sequence in seq, 0: end of file, -1: malformed (err).  qual (FASTQ, optional)
+    // receives the record's quality line without its line end.
+    int next_seq(std::vector<uint8_t>& seq, std::vector<uint8_t>* qual = nullptr) {
         seq.clear();
+        if (qual) qual->clear();
         if (eof) return 0;
         std::vector<uint8_t> line;
         if (is_fastq) {
@@ -475,6 +477,7 @@ struct MemFastx {
                     err = "sequence length != quality length";
                     return -1;
                 }
+                if (qual) *qual = line;
                 return 1;
             }
         }
@@ -490,18 +493,31 @@ struct MemFastx {
     }
 };
 
+// The packed store, and (not part of the reference, which drops qualities) the Phred
+// values of every packed base: read i's bases at quals[qoff[i] ..], each the quality
+// character of the same base in the source record minus 33 (0 below '!').
 struct PackedStore {
     std::vector<uint8_t> data;
     std::vector<uint64_t> off;
     std::vector<uint32_t> len;
     uint64_t ambiguous = 0;
-    void pack(const uint8_t* r, size_t n) { pack_clean(r, n, data, off, len); }
+    std::vector<uint8_t> quals;
+    std::vector<uint64_t> qoff;
+    void pack(const uint8_t* r, size_t n, const uint8_t* q = nullptr) {
+        pack_clean(r, n, data, off, len);
+        if (q) {
+            qoff.push_back(quals.size());
+            for (size_t i = 0; i < n; ++i) quals.push_back(q[i] >= 33 ? (uint8_t)(q[i] - 33) : 0);
+        }
+    }
     void append(const PackedStore& o) {
-        const uint64_t base = data.size();
+        const uint64_t base = data.size(), qbase = quals.size();
         data.insert(data.end(), o.data.begin(), o.data.end());
         for (uint64_t x : o.off) off.push_back(x + base);
         len.insert(len.end(), o.len.begin(), o.len.end());
         ambiguous += o.ambiguous;
+        quals.insert(quals.end(), o.quals.begin(), o.quals.end());
+        for (uint64_t x : o.qoff) qoff.push_back(x + qbase);
     }
 };
 
@@ -518,7 +534,7 @@ static void trim_n_ends(const std::vector<uint8_t>& r, size_t& s, size_t& e) {
 
 // ingest_trimmed_unpaired (packed_reads.rs:513-535): split on N, pack segments >= MIN_K,
 // else the ambiguous sidecar
-static void ingest_trimmed(const uint8_t* t, size_t n, PackedStore& b3) {
+static void ingest_trimmed(const uint8_t* t, size_t n, PackedStore& b3, const uint8_t* tq = nullptr) {
     const size_t MIN_K = 21;
     if (n == 0) return;
     bool any = false;
@@ -528,7 +544,7 @@ static void ingest_trimmed(const uint8_t* t, size_t n, PackedStore& b3) {
         size_t j = i;
         while (j < n && !ambig(t[j])) ++j;
         if (j > i && j - i >= MIN_K) {
-            b3.pack(t + i, j - i);
+            b3.pack(t + i, j - i, tq ? tq + i : nullptr);
             any = true;
         }
         i = j;
@@ -536,15 +552,16 @@ static void ingest_trimmed(const uint8_t* t, size_t n, PackedStore& b3) {
     if (!any) b3.ambiguous += 1;
 }
 
-static void ingest_read(const std::vector<uint8_t>& r, PackedStore& b3) {
+static void ingest_read(const std::vector<uint8_t>& r, PackedStore& b3, const std::vector<uint8_t>* q = nullptr) {
     size_t s, e;
     trim_n_ends(r, s, e);
-    ingest_trimmed(r.data() + s, e - s, b3);
+    ingest_trimmed(r.data() + s, e - s, b3, q ? q->data() + s : nullptr);
 }
 
 // partition_pair_streaming (packed_reads.rs:654-680)
 static void partition_pair(const std::vector<uint8_t>& r1, const std::vector<uint8_t>& r2, PackedStore& b1,
-                           PackedStore& b2, PackedStore& b3) {
+                           PackedStore& b2, PackedStore& b3, const std::vector<uint8_t>* q1 = nullptr,
+                           const std::vector<uint8_t>* q2 = nullptr) {
     const size_t MIN_K = 21;
     size_t s1, e1, s2, e2;
     trim_n_ends(r1, s1, e1);
@@ -553,11 +570,11 @@ static void partition_pair(const std::vector<uint8_t>& r1, const std::vector<uin
     for (size_t i = s1; i < e1; ++i) n1 = n1 || ambig(r1[i]);
     for (size_t i = s2; i < e2; ++i) n2 = n2 || ambig(r2[i]);
     if (!n1 && !n2 && e1 - s1 >= MIN_K && e2 - s2 >= MIN_K) {
-        b1.pack(r1.data() + s1, e1 - s1);
-        b2.pack(r2.data() + s2, e2 - s2);
+        b1.pack(r1.data() + s1, e1 - s1, q1 ? q1->data() + s1 : nullptr);
+        b2.pack(r2.data() + s2, e2 - s2, q2 ? q2->data() + s2 : nullptr);
     } else {
-        ingest_trimmed(r1.data() + s1, e1 - s1, b3);
-        ingest_trimmed(r2.data() + s2, e2 - s2, b3);
+        ingest_trimmed(r1.data() + s1, e1 - s1, b3, q1 ? q1->data() + s1 : nullptr);
+        ingest_trimmed(r2.data() + s2, e2 - s2, b3, q2 ? q2->data() + s2 : nullptr);
     }
 }
 
@@ -569,24 +586,25 @@ struct PackResult {
 
 // from_paired_fastq (packed_reads.rs:325-380) over two in-memory files; with r2 == NULL,
 // append_from_file (packed_reads.rs:490-501) of one file (every read through ingest_read).
-static PackResult* pack_fastq(const uint8_t* r1, size_t n1, const uint8_t* r2, size_t n2) {
+static PackResult* pack_fastq(const uint8_t* r1, size_t n1, const uint8_t* r2, size_t n2, bool keep_q = false) {
     auto* res = new PackResult();
     MemFastx a(r1, n1);
     if (!a.err.empty()) {
         res->err = a.err;
         return res;
     }
-    std::vector<uint8_t> s1, s2;
+    std::vector<uint8_t> s1, s2, q1, q2;
+    std::vector<uint8_t>* qa = keep_q && a.is_fastq ? &q1 : nullptr;
     if (!r2) {
         for (;;) {
-            const int h = a.next_seq(s1);
+            const int h = a.next_seq(s1, qa);
             if (h < 0) {
                 res->err = a.err;
                 return res;
             }
             if (h == 0) break;
             ++res->rec1;
-            ingest_read(s1, res->st);
+            ingest_read(s1, res->st, qa);
         }
         return res;
     }
@@ -596,18 +614,19 @@ static PackResult* pack_fastq(const uint8_t* r1, size_t n1, const uint8_t* r2, s
         return res;
     }
     PackedStore b1, b2, b3;
+    std::vector<uint8_t>* qb = keep_q && b.is_fastq ? &q2 : nullptr;
     for (;;) {
-        const int h1 = a.next_seq(s1);
-        const int h2 = b.next_seq(s2);
+        const int h1 = a.next_seq(s1, qa);
+        const int h2 = b.next_seq(s2, qb);
         if (h1 < 0 || h2 < 0) {
             res->err = h1 < 0 ? a.err : b.err;
             return res;
         }
         res->rec1 += h1;
         res->rec2 += h2;
-        if (h1 && h2) partition_pair(s1, s2, b1, b2, b3);
-        else if (h1) ingest_read(s1, b3);
-        else if (h2) ingest_read(s2, b3);
+        if (h1 && h2) partition_pair(s1, s2, b1, b2, b3, qa, qb);
+        else if (h1) ingest_read(s1, b3, qa);
+        else if (h2) ingest_read(s2, b3, qb);
         else break;
     }
     res->pe_count = b1.len.size();
@@ -946,6 +965,15 @@ void* or_spectrum_run_priors(const uint8_t* data, const uint64_t* offsets, const
 uint32_t or_prior_boost(double min_cov) { return prior_boost_of(min_cov); }
 
 void* or_pack_fastq(const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint64_t n2) { return pack_fastq(r1, n1, r2, n2); }
+void* or_pack_fastq_q(const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint64_t n2) {
+    return pack_fastq(r1, n1, r2, n2, true);
+}
+uint64_t or_pack_n_quals(void* h) { return ((PackResult*)h)->st.quals.size(); }
+void or_pack_export_quals(void* h, uint8_t* quals, uint64_t* qoff) {
+    auto* r = (PackResult*)h;
+    if (!r->st.quals.empty()) std::memcpy(quals, r->st.quals.data(), r->st.quals.size());
+    if (!r->st.qoff.empty()) std::memcpy(qoff, r->st.qoff.data(), r->st.qoff.size() * 8);
+}
 void or_pack_free(void* h) { delete (PackResult*)h; }
 // sizes[0..5] = n_reads, data_len, pe_count, ambiguous, r1 records, r2 records; returns 0 or -1 (malformed)
 int or_pack_sizes(void* h, uint64_t* sizes) {

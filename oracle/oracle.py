@@ -64,6 +64,9 @@ def lib():
                                                          _u8p, _u64p, _u64p, ctypes.c_uint64]),
             "or_prior_boost": (ctypes.c_uint32, [ctypes.c_double]),
             "or_pack_fastq": (ctypes.c_void_p, [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64]),
+            "or_pack_fastq_q": (ctypes.c_void_p, [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64]),
+            "or_pack_n_quals": (ctypes.c_uint64, [ctypes.c_void_p]),
+            "or_pack_export_quals": (None, [ctypes.c_void_p, _u8p, _u64p]),
             "or_pack_free": (None, [ctypes.c_void_p]),
             "or_pack_sizes": (ctypes.c_int, [ctypes.c_void_p, _u64p]),
             "or_pack_export": (None, [ctypes.c_void_p, _u8p, _u64p, _u32p]),
@@ -334,14 +337,16 @@ class Spectrum:
             self._h = None
 
 
-def pack_fastq(r1, r2=None):
+def pack_fastq(r1, r2=None, quals=False):
     """packed_reads.rs from_paired_fastq (or append_from_file when r2 is None) over file
-    contents: (data, offsets, lengths, pe_count, n_ambiguous, r1_records, r2_records).
-    Raises ValueError for a malformed file."""
+    contents: (data, offsets, lengths, pe_count, n_ambiguous, r1_records, r2_records), and
+    with quals=True also (quals, qoff): the Phred value of every packed base (quality
+    character - 33), read i's at quals[qoff[i]:].  Raises ValueError for a malformed file."""
     L = lib()
     a = np.frombuffer(bytes(r1) + b"\0", dtype=np.uint8)
     b = None if r2 is None else np.frombuffer(bytes(r2) + b"\0", dtype=np.uint8)
-    h = L.or_pack_fastq(_p(a, _u8p), len(r1), _p(b, _u8p) if b is not None else None, 0 if r2 is None else len(r2))
+    fn = L.or_pack_fastq_q if quals else L.or_pack_fastq
+    h = fn(_p(a, _u8p), len(r1), _p(b, _u8p) if b is not None else None, 0 if r2 is None else len(r2))
     try:
         sizes = np.zeros(6, dtype=np.uint64)
         if L.or_pack_sizes(h, _p(sizes, _u64p)) != 0:
@@ -351,6 +356,13 @@ def pack_fastq(r1, r2=None):
         offs = np.zeros(max(n, 1), dtype=np.uint64)
         lens = np.zeros(max(n, 1), dtype=np.uint32)
         L.or_pack_export(h, _p(data, _u8p), _p(offs, _u64p), _p(lens, _u32p))
-        return data[:dl], offs[:n], lens[:n], int(sizes[2]), int(sizes[3]), int(sizes[4]), int(sizes[5])
+        out = (data[:dl], offs[:n], lens[:n], int(sizes[2]), int(sizes[3]), int(sizes[4]), int(sizes[5]))
+        if not quals:
+            return out
+        nq = int(L.or_pack_n_quals(h))
+        q = np.zeros(max(nq, 1), dtype=np.uint8)
+        qo = np.zeros(max(n, 1), dtype=np.uint64)
+        L.or_pack_export_quals(h, _p(q, _u8p), _p(qo, _u64p))
+        return out + (q[:nq], qo[:n])
     finally:
         L.or_pack_free(h)

@@ -39,6 +39,8 @@ struct kc_packed {
     std::vector<uint8_t> data;
     std::vector<uint64_t> off;
     std::vector<uint32_t> len;
+    std::vector<uint8_t> quals;  // when kept: Phred value of every packed base
+    std::vector<uint64_t> qoff;
     kc_packed_info info{};
 };
 
@@ -184,7 +186,7 @@ __global__ void k_count_empty(const uint8_t* __restrict__ b, uint64_t n, const u
 // header line (blank lines between records); bad[0] = first bad record.
 __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
                           uint64_t m, uint64_t first, uint64_t R, uint64_t* __restrict__ seq_s,
-                          uint32_t* __restrict__ seq_l, uint32_t* __restrict__ err,
+                          uint32_t* __restrict__ seq_l, uint64_t* __restrict__ qual_s, uint32_t* __restrict__ err,
                           unsigned long long* __restrict__ bad) {
     for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t s[4], e[4];
@@ -199,6 +201,7 @@ __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsig
         if (e[3] - s[3] != e[1] - s[1]) f |= 4u;
         if (e[1] - s[1] > 0xFFFFFFFFull) f |= 4u;
         seq_s[r] = s[1];
+        qual_s[r] = s[3];
         seq_l[r] = (uint32_t)(e[1] - s[1]);
         if (f) {
             atomicOr(err, f);
@@ -211,7 +214,7 @@ __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsig
 // ACGT segments of at least MIN_K bases and their packed bytes.
 struct ReadGate {
     uint32_t ts, tl, nseg, internal_n;
-    uint64_t seg_bytes;
+    uint64_t seg_bytes, seg_bases;
 };
 
 constexpr uint32_t WAVE = 64;
@@ -303,11 +306,13 @@ __device__ ReadGate wave_gate(const uint8_t* __restrict__ b, uint64_t s, uint32_
             if (len >= MIN_K) {
                 g.nseg += 1;
                 g.seg_bytes += (len + 3) / 4;
+                g.seg_bases += len;
             }
         });
     } else if (runs == 1 && last_end - first >= MIN_K) {
         g.nseg = 1;
         g.seg_bytes = (last_end - first + 3) / 4;
+        g.seg_bases = last_end - first;
     }
     g.ts = runs ? first : l;
     g.tl = runs ? last_end - first : 0;
@@ -374,7 +379,8 @@ __device__ __forceinline__ uint64_t unit_of(uint64_t x, uint64_t R1, uint64_t P,
 __global__ void k_units(const ReadGate* __restrict__ gates, uint64_t R1, uint64_t R2, uint64_t P,
                         const uint32_t* __restrict__ clean, const unsigned long long* __restrict__ clean_pre,
                         uint64_t n_clean, unsigned long long* __restrict__ unit_reads,
-                        unsigned long long* __restrict__ unit_bytes, unsigned long long* __restrict__ ambiguous) {
+                        unsigned long long* __restrict__ unit_bytes, unsigned long long* __restrict__ unit_bases,
+                        unsigned long long* __restrict__ ambiguous) {
     const uint64_t total = R1 + R2;
     uint32_t amb = 0;
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
@@ -384,24 +390,40 @@ __global__ void k_units(const ReadGate* __restrict__ gates, uint64_t R1, uint64_
         if (whole) {
             unit_reads[u] = 1;
             unit_bytes[u] = (g.tl + 3) / 4;
+            if (unit_bases) unit_bases[u] = g.tl;
         } else {
             unit_reads[u] = g.nseg;
             unit_bytes[u] = g.seg_bytes;
+            if (unit_bases) unit_bases[u] = g.seg_bases;
             amb += g.tl && !g.nseg;  // gate 3: nothing of the read reaches MIN_K
         }
     }
     if (amb) atomicAdd(ambiguous, (unsigned long long)amb);
 }
 
-// place[x] = (first output read, first output byte) of input read x.
+// Where input read x's output goes: first output read, first packed byte, first quality.
+struct Place {
+    uint64_t rb, bb, qb;
+};
+
 __global__ void k_place(uint64_t total, uint64_t R1, uint64_t P, const uint32_t* __restrict__ clean,
                         const unsigned long long* __restrict__ clean_pre, uint64_t n_clean,
                         const unsigned long long* __restrict__ read_base,
-                        const unsigned long long* __restrict__ byte_base, ulonglong2* __restrict__ place) {
+                        const unsigned long long* __restrict__ byte_base,
+                        const unsigned long long* __restrict__ base_base, Place* __restrict__ place) {
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < total; x += (uint64_t)gridDim.x * blockDim.x) {
         bool whole;
         const uint64_t u = unit_of(x, R1, P, clean, clean_pre, n_clean, whole);
-        place[x] = make_ulonglong2(read_base[u], byte_base[u]);
+        place[x] = Place{read_base[u], byte_base[u], base_base ? base_base[u] : 0ull};
+    }
+}
+
+// Quality characters [s, s + len) -> Phred values (character - 33, 0 below '!').
+__device__ void wave_quals(const uint8_t* __restrict__ b, uint64_t s, uint32_t len, uint8_t* __restrict__ out,
+                           uint32_t lane) {
+    for (uint32_t i = lane; i < len; i += WAVE) {
+        const uint8_t c = b[s + i];
+        out[i] = c >= 33 ? (uint8_t)(c - 33) : (uint8_t)0;
     }
 }
 
@@ -413,8 +435,10 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, 
                                                 const uint8_t* __restrict__ b2, const uint64_t* __restrict__ s2,
                                                 const uint32_t* __restrict__ l2, uint64_t R2, uint64_t P,
                                                 const ReadGate* __restrict__ gates, const uint32_t* __restrict__ clean,
-                                                const ulonglong2* __restrict__ place, uint8_t* __restrict__ data,
-                                                uint64_t* __restrict__ offsets, uint32_t* __restrict__ lengths) {
+                                                const Place* __restrict__ place, uint8_t* __restrict__ data,
+                                                uint64_t* __restrict__ offsets, uint32_t* __restrict__ lengths,
+                                                const uint64_t* __restrict__ q1, const uint64_t* __restrict__ q2,
+                                                uint8_t* __restrict__ quals, uint64_t* __restrict__ qoff) {
     const uint64_t total = R1 + R2;
     const uint32_t lane = threadIdx.x % WAVE;
     const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / WAVE);
@@ -424,15 +448,18 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, 
         const uint64_t s = second ? s2[x - R1] : s1[x];
         const uint32_t l = second ? l2[x - R1] : l1[x];
         const ReadGate g = gates[x];
-        const ulonglong2 pl = place[x];
-        uint64_t rb = pl.x, bb = pl.y;
+        const Place pl = place[x];
+        uint64_t rb = pl.rb, bb = pl.bb, qb = pl.qb;
         const uint64_t i = second ? x - R1 : x;
+        const uint64_t qs = quals ? (second ? q2[x - R1] : q1[x]) : 0;  // the record's quality line
         if (i < P && clean[i]) {  // a clean pair's read: one output read of its trimmed span
             if (lane == 0) {
                 offsets[rb] = bb;
                 lengths[rb] = g.tl;
+                if (quals) qoff[rb] = qb;
             }
             wave_pack(b, s + g.ts, g.tl, data + bb, lane);
+            if (quals) wave_quals(b, qs + g.ts, g.tl, quals + qb, lane);
             continue;
         }
         if (!g.nseg) continue;
@@ -441,16 +468,19 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, 
             if (lane == 0) {
                 offsets[rb] = bb;
                 lengths[rb] = len;
+                if (quals) qoff[rb] = qb;
             }
             wave_pack(b, s + st, len, data + bb, lane);
+            if (quals) wave_quals(b, qs + st, len, quals + qb, lane);
             ++rb;
             bb += (len + 3) / 4;
+            qb += len;
         });
     }
 }
 
 struct FileIndex {
-    Buf buf, tiles, tile_base, nl, red, seq_s, seq_l, err;
+    Buf buf, tiles, tile_base, nl, red, seq_s, seq_l, qual_s, err;
     uint64_t n = 0, R = 0;
 };
 
@@ -462,6 +492,8 @@ struct kc_ingest {
     hipEvent_t ev[2] = {nullptr, nullptr};
     FileIndex A, B;
     Buf gates, clean, clean_pre, ureads, ubytes, rbase, bbase, place, small, scan_tmp, data, off, len;
+    Buf ubases, qbase, quals, qoff;  // quality side array (keep_quals)
+    bool keep_quals = false;
 };
 
 namespace {
@@ -586,11 +618,12 @@ int index_file(kc_ingest* c, FileIndex& F, const char* name) {
     const uint64_t R = (end4 - first) / 4;
     I_TRY(F.seq_s.ensure(R * 8 + 8));
     I_TRY(F.seq_l.ensure(R * 4 + 4));
+    I_TRY(F.qual_s.ensure(R * 8 + 8));
     I_TRY(F.err.ensure(64));
     unsigned long long init[2] = {0ull, ~0ull};
     I_TRY(hipMemcpyAsync(F.err.p, init, 16, hipMemcpyHostToDevice, st));
     k_records<<<grid_for(R), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, R, F.seq_s.as<uint64_t>(),
-                                             F.seq_l.as<uint32_t>(), F.err.as<uint32_t>(),
+                                             F.seq_l.as<uint32_t>(), F.qual_s.as<uint64_t>(), F.err.as<uint32_t>(),
                                              F.err.as<unsigned long long>() + 1);
     I_TRY(hipGetLastError());
     unsigned long long out[2];
@@ -632,7 +665,12 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     I_TRY(c->ubytes.ensure(U * 8 + 8));
     I_TRY(c->rbase.ensure(U * 8 + 8));
     I_TRY(c->bbase.ensure(U * 8 + 8));
-    I_TRY(c->place.ensure(tot * 16 + 16));
+    I_TRY(c->place.ensure((tot + 1) * sizeof(Place)));
+    const bool kq = c->keep_quals;
+    if (kq) {
+        I_TRY(c->ubases.ensure(U * 8 + 8));
+        I_TRY(c->qbase.ensure(U * 8 + 8));
+    }
     I_TRY(c->small.ensure(64));
     const unsigned long long zero = 0;
     I_TRY(hipMemcpyAsync(c->small.p, &zero, 8, hipMemcpyHostToDevice, st));
@@ -661,36 +699,50 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
         k_units<<<grid_for(tot), BLOCK, 0, st>>>(gates, R1, R2, P, c->clean.as<uint32_t>(),
                                                c->clean_pre.as<unsigned long long>(), n_clean,
                                                c->ureads.as<unsigned long long>(), c->ubytes.as<unsigned long long>(),
+                                               kq ? c->ubases.as<unsigned long long>() : nullptr,
                                                c->small.as<unsigned long long>());
         I_TRY(hipGetLastError());
     }
     rc = scan_excl(c, c->ureads.as<unsigned long long>(), c->rbase.as<unsigned long long>(), U);
     if (!rc) rc = scan_excl(c, c->ubytes.as<unsigned long long>(), c->bbase.as<unsigned long long>(), U);
+    if (!rc && kq) rc = scan_excl(c, c->ubases.as<unsigned long long>(), c->qbase.as<unsigned long long>(), U);
     if (rc) return rc;
-    unsigned long long tail[4] = {0, 0, 0, 0}, ambiguous = 0;
+    unsigned long long tail[6] = {0, 0, 0, 0, 0, 0}, ambiguous = 0;
     if (U) {
         I_TRY(hipMemcpyAsync(&tail[0], c->rbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
         I_TRY(hipMemcpyAsync(&tail[1], c->ureads.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
         I_TRY(hipMemcpyAsync(&tail[2], c->bbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
         I_TRY(hipMemcpyAsync(&tail[3], c->ubytes.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+        if (kq) {
+            I_TRY(hipMemcpyAsync(&tail[4], c->qbase.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost, st));
+            I_TRY(hipMemcpyAsync(&tail[5], c->ubases.as<unsigned long long>() + U - 1, 8, hipMemcpyDeviceToHost,
+                                 st));
+        }
     }
     I_TRY(hipMemcpyAsync(&ambiguous, c->small.p, 8, hipMemcpyDeviceToHost, st));
     I_TRY(hipStreamSynchronize(st));
-    const uint64_t n_reads = tail[0] + tail[1], data_len = tail[2] + tail[3];
+    const uint64_t n_reads = tail[0] + tail[1], data_len = tail[2] + tail[3], n_quals = tail[4] + tail[5];
     I_TRY(c->data.ensure(data_len + 8));
+    if (kq) {
+        I_TRY(c->quals.ensure(n_quals + 8));
+        I_TRY(c->qoff.ensure(n_reads * 8 + 8));
+    }
     I_TRY(c->off.ensure(n_reads * 8 + 8));
     I_TRY(c->len.ensure(n_reads * 4 + 4));
     if (tot) {
         k_place<<<grid_for(tot), BLOCK, 0, st>>>(tot, R1, P, c->clean.as<uint32_t>(),
                                                c->clean_pre.as<unsigned long long>(), n_clean,
                                                c->rbase.as<unsigned long long>(), c->bbase.as<unsigned long long>(),
-                                               c->place.as<ulonglong2>());
+                                               kq ? c->qbase.as<unsigned long long>() : nullptr,
+                                               c->place.as<Place>());
         I_TRY(hipGetLastError());
         k_emit<<<grid_for(tot * WAVE), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,
                                                       B.seq_s.as<uint64_t>(), B.seq_l.as<uint32_t>(), R2, P, gates,
-                                                      c->clean.as<uint32_t>(), c->place.as<ulonglong2>(),
+                                                      c->clean.as<uint32_t>(), c->place.as<Place>(),
                                                       c->data.as<uint8_t>(), c->off.as<uint64_t>(),
-                                                      c->len.as<uint32_t>());
+                                                      c->len.as<uint32_t>(), A.qual_s.as<uint64_t>(),
+                                                      B.qual_s.as<uint64_t>(), kq ? c->quals.as<uint8_t>() : nullptr,
+                                                      kq ? c->qoff.as<uint64_t>() : nullptr);
         I_TRY(hipGetLastError());
     }
     I_TRY(hipEventRecord(c->ev[1], st));
@@ -702,6 +754,12 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
         I_TRY(hipMemcpyAsync(out->off.data(), c->off.p, n_reads * 8, hipMemcpyDeviceToHost, st));
         I_TRY(hipMemcpyAsync(out->len.data(), c->len.p, n_reads * 4, hipMemcpyDeviceToHost, st));
     }
+    if (kq) {
+        out->quals.resize(n_quals);
+        out->qoff.resize(n_reads);
+        if (n_quals) I_TRY(hipMemcpyAsync(out->quals.data(), c->quals.p, n_quals, hipMemcpyDeviceToHost, st));
+        if (n_reads) I_TRY(hipMemcpyAsync(out->qoff.data(), c->qoff.p, n_reads * 8, hipMemcpyDeviceToHost, st));
+    }
     I_TRY(hipStreamSynchronize(st));
     float ms = 0.f;
     I_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -712,6 +770,7 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     out->info.r1_records = R1;
     out->info.r2_records = R2;
     out->info.ms_device = ms;
+    out->info.n_quals = kq ? n_quals : 0;
     return KC_OK;
 }
 
@@ -793,6 +852,22 @@ int kc_packed_export(const kc_packed* p, uint8_t* data, size_t data_len, uint64_
     std::copy(p->data.begin(), p->data.end(), data);
     std::copy(p->off.begin(), p->off.end(), offsets);
     std::copy(p->len.begin(), p->len.end(), lengths);
+    return KC_OK;
+}
+
+int kc_ingest_keep_quals(kc_ingest* c, int32_t keep) {
+    if (!c) return fail(KC_EINVAL, "null context");
+    c->keep_quals = keep != 0;
+    return KC_OK;
+}
+
+int kc_packed_export_quals(const kc_packed* p, uint8_t* quals, size_t n_quals, uint64_t* qoffsets, size_t n_reads) {
+    if (!p) return KC_EINVAL;
+    if (p->qoff.size() != p->len.size()) return fail(KC_ESTATE, "qualities were not kept (kc_ingest_keep_quals)");
+    if (n_quals != p->quals.size() || n_reads != p->qoff.size()) return KC_EINVAL;
+    if ((n_quals && !quals) || (n_reads && !qoffsets)) return KC_EINVAL;
+    std::copy(p->quals.begin(), p->quals.end(), quals);
+    std::copy(p->qoff.begin(), p->qoff.end(), qoffsets);
     return KC_OK;
 }
 

@@ -35,7 +35,7 @@ EXPORTS = (
     "kc_mlp_set_params", "kc_mlp_forward", "kc_mlp_gradient", "kc_mlp_train",
     "kc_library_quals_set", "kc_library_features", "kc_library_learned_select",
     "kc_fastq_pack", "kc_fastq_last_error", "kc_packed_info_get", "kc_packed_export", "kc_packed_destroy",
-    "kc_ingest_create", "kc_ingest_destroy", "kc_ingest_pack",
+    "kc_ingest_create", "kc_ingest_destroy", "kc_ingest_pack", "kc_ingest_keep_quals", "kc_packed_export_quals",
 )
 
 
@@ -64,7 +64,7 @@ class kc_info(ctypes.Structure):
 class kc_packed_info(ctypes.Structure):
     _fields_ = [("n_reads", ctypes.c_uint64), ("data_len", ctypes.c_uint64), ("pe_count", ctypes.c_uint64),
                 ("n_ambiguous", ctypes.c_uint64), ("r1_records", ctypes.c_uint64), ("r2_records", ctypes.c_uint64),
-                ("ms_device", ctypes.c_double)]
+                ("ms_device", ctypes.c_double), ("n_quals", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -145,6 +145,8 @@ _SIGS = {
     "kc_ingest_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_vp)]),
     "kc_ingest_destroy": (None, [_vp]),
     "kc_ingest_pack": (ctypes.c_int, [_vp, _u8p, _sz, _u8p, _sz, ctypes.POINTER(_vp)]),
+    "kc_ingest_keep_quals": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "kc_packed_export_quals": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _sz]),
 }
 
 _lib = None

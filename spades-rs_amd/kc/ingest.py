@@ -29,15 +29,20 @@ def _u8(b):
 class FastqIngest:
     """A device context for repeated ingestion (its buffers grow to the largest input and
     are reused).  pack(r1, r2=None) takes bytes, uint8 arrays or paths (.gz decompressed
-    on the host) and returns (PackedReads, info dict).  Raises KcError: KC_EUNSUPPORTED for
-    FASTA input or blank lines between records (PackedReads.from_paired_fastq parses
-    those), KC_EINVAL for a malformed record."""
+    on the host) and returns (PackedReads, info dict).  With keep_quals, info also holds
+    "quals" (the Phred value of every packed base) and "qoff" (read i's at quals[qoff[i]:]),
+    the arrays SpectrumEngine.set_quals takes.  Raises KcError: KC_EUNSUPPORTED for FASTA
+    input or blank lines between records (PackedReads.from_paired_fastq parses those),
+    KC_EINVAL for a malformed record."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, keep_quals=False):
         self._h = ctypes.c_void_p()
         st = lib().kc_ingest_create(device, ctypes.byref(self._h))
         if st != _abi.KC_OK:
             raise KcError(st, lib().kc_fastq_last_error().decode())
+        if keep_quals:
+            lib().kc_ingest_keep_quals(self._h, 1)
+        self.keep_quals = bool(keep_quals)
 
     def pack(self, r1, r2=None):
         a = r1 if isinstance(r1, np.ndarray) else _contents(r1)
@@ -48,7 +53,7 @@ class FastqIngest:
                                   0 if b is None else len(b), ctypes.byref(h))
         if st != _abi.KC_OK:
             raise KcError(st, lib().kc_fastq_last_error().decode())
-        return _take(h)
+        return _take(h, self.keep_quals)
 
     def close(self):
         if self._h:
@@ -62,7 +67,7 @@ class FastqIngest:
             pass
 
 
-def _take(h):
+def _take(h, quals=False):
     try:
         info = _abi.kc_packed_info()
         lib().kc_packed_info_get(h, ctypes.byref(info))
@@ -74,7 +79,16 @@ def _take(h):
         if st != _abi.KC_OK:
             raise KcError(st, "kc_packed_export failed")
         store = PackedReads.from_arrays(data[:dl], offs[:n], lens[:n], pe_count=info.pe_count)
-        return store, info.as_dict()
+        out = info.as_dict()
+        if quals:
+            nq = info.n_quals
+            q = np.zeros(max(nq, 1), dtype=np.uint8)
+            qo = np.zeros(max(n, 1), dtype=np.uint64)
+            st = lib().kc_packed_export_quals(h, ptr(q, u8p), nq, ptr(qo, u64p), n)
+            if st != _abi.KC_OK:
+                raise KcError(st, "kc_packed_export_quals failed")
+            out["quals"], out["qoff"] = q[:nq], qo[:n]
+        return store, out
     finally:
         lib().kc_packed_destroy(h)
 

@@ -100,6 +100,31 @@ def test_context_reuse_shrinking_inputs(oracle):
     ctx.close()
 
 
+def test_quality_side_array_matches_oracle(oracle):
+    from kc.ingest import FastqIngest
+    from test_oracle_fastq import fastq_q
+    rng = np.random.default_rng(41)
+    ctx = FastqIngest(keep_quals=True)
+    for n, crlf in ((700, False), (1500, True), (0, False)):
+        a, _ = fastq_q(random_reads(rng, n), rng, crlf)
+        b, _ = fastq_q(random_reads(rng, n + 3), rng)
+        say(f"case ingest quals n={n}")
+        data, offs, lens, pe, amb, rec1, rec2, q, qo = oracle.pack_fastq(a, b, quals=True)
+        store, info = ctx.pack(a, b)
+        d, o, l = store.arrays()
+        assert np.array_equal(d, data) and np.array_equal(o, offs) and np.array_equal(l, lens)
+        assert info["n_quals"] == len(q)
+        assert np.array_equal(info["quals"], q) and np.array_equal(info["qoff"], qo)
+    a, _ = fastq_q(random_reads(rng, 400), rng)
+    *_, q, qo = oracle.pack_fastq(a, quals=True)
+    _, info = ctx.pack(a)
+    assert np.array_equal(info["quals"], q) and np.array_equal(info["qoff"], qo)
+    ctx.close()
+    # without keep_quals there is no quality array
+    _, info = fastq_pack(a)
+    assert info["n_quals"] == 0
+
+
 def test_unsupported_and_malformed():
     with pytest.raises(KcError) as e:
         fastq_pack(fastq([b"ACGT" * 10, b"GGGG" * 10], blank_between=True), fastq([b"ACGT" * 10, b"TTTT" * 10]))

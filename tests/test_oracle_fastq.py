@@ -101,3 +101,39 @@ def test_single_file_ingest_matches_host_mirror(oracle):
 def test_malformed_is_an_error(oracle, text):
     with pytest.raises(ValueError):
         oracle.pack_fastq(text, fastq([b"ACGT"]))
+
+
+def fastq_q(reads, rng, crlf=False):
+    """FASTQ with a random quality character per base; returns (text, [(seq, qual), ...])."""
+    eol = b"\r\n" if crlf else b"\n"
+    out, recs = [], []
+    for i, r in enumerate(reads):
+        q = bytes(rng.integers(33, 75, len(r)).astype(np.uint8))
+        out.append(b"@r%d" % i + eol + r + eol + b"+" + eol + q + eol)
+        recs.append((r.upper(), q))
+    return b"".join(out), recs
+
+
+def test_quality_side_array_follows_the_packed_bases(oracle):
+    # Not a reference feature (the reference drops qualities): every packed base carries
+    # its own quality - 33.  Pinned by locating each output read in its source records.
+    rng = np.random.default_rng(31)
+    r1, r2 = random_reads(rng, 300), random_reads(rng, 305)
+    a, ra = fastq_q(r1, rng, crlf=True)
+    b, rb = fastq_q(r2, rng)
+    plain = oracle.pack_fastq(a, b)
+    data, offs, lens, pe, amb, rec1, rec2, q, qo = oracle.pack_fastq(a, b, quals=True)
+    for x, y in zip(plain, (data, offs, lens, pe, amb, rec1, rec2)):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    assert len(q) == int(lens.astype(np.int64).sum())
+    assert np.array_equal(qo, np.concatenate([[0], np.cumsum(lens.astype(np.uint64))[:-1]]).astype(np.uint64))
+    s = P.PackedReads.from_arrays(data, offs, lens, pe_count=pe)
+    recs = ra + rb
+    for i in range(len(s)):
+        seq = s.get_read(i)
+        qual = bytes((q[int(qo[i]):int(qo[i]) + len(seq)] + 33).astype(np.uint8))
+        assert any(p >= 0 and src[1][p:p + len(seq)] == qual
+                   for src in recs for p in [src[0].find(seq)]), i
+    # single-file mode keeps the same pairing of bases and qualities
+    d1, o1, l1, *_, q1, qo1 = oracle.pack_fastq(a, quals=True)
+    assert len(q1) == int(l1.astype(np.int64).sum())

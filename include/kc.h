@@ -210,6 +210,32 @@ int kc_packed_export(const kc_packed* p, uint8_t* data, size_t data_len, uint64_
 int kc_packed_export_quals(const kc_packed* p, uint8_t* quals, size_t n_quals, uint64_t* qoffsets, size_t n_reads);
 void kc_packed_destroy(kc_packed* p);
 
+/* ---- Unique-anchor k-mer indexes over contigs / unitigs ---------------------
+ * The maps canonical k-mer -> (sequence, position, is_rc) that the reference builds from
+ * ASCII sequences with the counting path's k-mer extraction (windows with a base other
+ * than ACGT/acgt are skipped):
+ *   KC_ANCHOR_POLISHER     Polisher::build_kmer_index (polisher.rs:33-60): a second
+ *                          sighting drops the k-mer, for k < 11 only one in another contig;
+ *   KC_ANCHOR_PAIRED_INFO  PairedInfoIndex::build_kmer_index (paired_info.rs:34-70): any
+ *                          second sighting drops it;
+ *   KC_ANCHOR_SPLITTER     the splitter's anchor step (splitter.rs:57-70): a sighting in
+ *                          another contig drops it (the value used there is the contig).
+ * The value is the k-mer's first occurrence.  kc_anchor_export gives the anchors sorted
+ * by k-mer as Kmer256 limbs; kc_anchor_query looks up canonical k-mers (limbs) and gives
+ * the anchor's index in that order, or -1. */
+#define KC_ANCHOR_POLISHER 0
+#define KC_ANCHOR_PAIRED_INFO 1
+#define KC_ANCHOR_SPLITTER 2
+typedef struct kc_anchor kc_anchor;
+int kc_anchor_build(int32_t device, uint32_t k, int32_t kind, const uint8_t* seqs, size_t seqs_len,
+                    const uint64_t* starts, const uint64_t* lengths, size_t n_seqs, kc_anchor** out);
+int kc_anchor_count(const kc_anchor* a, uint64_t* n);
+int kc_anchor_export(kc_anchor* a, uint64_t* keys4, uint32_t* seq_ids, uint32_t* positions, uint8_t* is_rc,
+                     size_t n);
+int kc_anchor_query(kc_anchor* a, const uint64_t* keys4, size_t n, int64_t* index);
+const char* kc_anchor_last_error(void);
+void kc_anchor_destroy(kc_anchor* a);
+
 /* ---- Tuning and graph replay ---------------------------------------------- */
 
 /* Launch shape of the table insert and the table load factor.  Every setting gives

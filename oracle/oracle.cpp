@@ -29,6 +29,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 typedef unsigned __int128 u128;
@@ -872,6 +873,119 @@ static Spectrum* spectrum_run(const uint8_t* data, const uint64_t* offsets, cons
 // ---------------------------------------------------------------------------
 // C ABI for the ctypes test wrapper (oracle/oracle.py)
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Unique-anchor k-mer indexes over contigs / unitigs (ASCII sequences)
+// ---------------------------------------------------------------------------
+// The value of one anchor: sequence index, window position, and whether the canonical
+// form of that window is its reverse complement.
+struct Anchor {
+    uint32_t seq, pos;
+    bool is_rc;
+};
+using AnchorMap = std::unordered_map<Kmer256, Anchor, Kmer256Hash>;
+using KmerSet = std::unordered_set<Kmer256, Kmer256Hash>;
+
+// polisher.rs:33-60 Polisher::build_kmer_index: a second sighting removes the k-mer (for
+// k < 11 only when it is in another contig); removed k-mers are never re-added.
+static AnchorMap polisher_index(unsigned k, const std::vector<std::vector<uint8_t>>& contigs) {
+    AnchorMap pos;
+    KmerSet repeats;
+    for (size_t c = 0; c < contigs.size(); ++c) {
+        const auto& sq = contigs[c];
+        if (sq.size() < k) continue;
+        for (size_t i = 0; i + k <= sq.size(); ++i) {
+            Kmer256 km;
+            if (!kmer_from_bytes(sq.data() + i, k, k, &km)) continue;
+            int rc = 0;
+            const Kmer256 can = kmer_canonical(km, k, &rc);
+            if (repeats.count(can)) continue;
+            auto it = pos.find(can);
+            if (it != pos.end()) {
+                if (k >= 11 || it->second.seq != c) {
+                    pos.erase(it);
+                    repeats.insert(can);
+                }
+            } else {
+                pos[can] = Anchor{(uint32_t)c, (uint32_t)i, rc != 0};
+            }
+        }
+    }
+    return pos;
+}
+
+// paired_info.rs:34-70 PairedInfoIndex::build_kmer_index: any second sighting removes.
+static AnchorMap paired_info_index(unsigned k, const std::vector<std::vector<uint8_t>>& unitigs) {
+    AnchorMap map;
+    if (k == 0 || k > 128) return map;
+    KmerSet repeats;
+    for (size_t u = 0; u < unitigs.size(); ++u) {
+        const auto& sq = unitigs[u];
+        if (sq.size() < k) continue;
+        for (size_t i = 0; i + k <= sq.size(); ++i) {
+            Kmer256 km;
+            if (!kmer_from_bytes(sq.data() + i, k, k, &km)) continue;
+            int rc = 0;
+            const Kmer256 can = kmer_canonical(km, k, &rc);
+            if (repeats.count(can)) continue;
+            if (map.count(can)) {
+                map.erase(can);
+                repeats.insert(can);
+            } else {
+                map[can] = Anchor{(uint32_t)u, (uint32_t)i, rc != 0};
+            }
+        }
+    }
+    return map;
+}
+
+// splitter.rs:57-70: a sighting in another contig removes; the value is the contig.
+static AnchorMap splitter_index(unsigned k, const std::vector<std::vector<uint8_t>>& contigs) {
+    AnchorMap map;
+    KmerSet ambiguous;
+    for (size_t c = 0; c < contigs.size(); ++c) {
+        const auto& sq = contigs[c];
+        if (sq.size() < k) continue;
+        for (size_t i = 0; i + k <= sq.size(); ++i) {
+            Kmer256 km;
+            if (!kmer_from_bytes(sq.data() + i, k, k, &km)) continue;
+            int rc = 0;
+            const Kmer256 can = kmer_canonical(km, k, &rc);
+            if (ambiguous.count(can)) continue;
+            auto it = map.find(can);
+            if (it != map.end()) {
+                if (it->second.seq != c) {
+                    map.erase(it);
+                    ambiguous.insert(can);
+                }
+            } else {
+                map[can] = Anchor{(uint32_t)c, (uint32_t)i, rc != 0};
+            }
+        }
+    }
+    return map;
+}
+
+struct AnchorResult {
+    std::vector<Kmer256> keys;  // sorted
+    std::vector<Anchor> vals;
+};
+
+static AnchorResult* anchor_run(int which, unsigned k, const uint8_t* seqs, const uint64_t* starts,
+                                const uint64_t* lengths, uint64_t n) {
+    std::vector<std::vector<uint8_t>> v(n);
+    for (uint64_t i = 0; i < n; ++i) v[i].assign(seqs + starts[i], seqs + starts[i] + lengths[i]);
+    const AnchorMap m = which == 0 ? polisher_index(k, v) : which == 1 ? paired_info_index(k, v) : splitter_index(k, v);
+    auto* r = new AnchorResult();
+    std::vector<std::pair<Kmer256, Anchor>> e(m.begin(), m.end());
+    std::sort(e.begin(), e.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (auto& x : e) {
+        r->keys.push_back(x.first);
+        r->vals.push_back(x.second);
+    }
+    return r;
+}
+
 extern "C" {
 
 uint8_t or_ascii_to_2bit(uint8_t b) { return ascii_to_2bit(b); }
@@ -992,6 +1106,22 @@ void or_pack_export(void* h, uint8_t* data, uint64_t* offsets, uint32_t* lengths
     if (!r->st.off.empty()) std::memcpy(offsets, r->st.off.data(), r->st.off.size() * 8);
     if (!r->st.len.empty()) std::memcpy(lengths, r->st.len.data(), r->st.len.size() * 4);
 }
+// which: 0 = polisher.rs, 1 = paired_info.rs, 2 = splitter.rs
+void* or_anchor_run(int which, uint32_t k, const uint8_t* seqs, const uint64_t* starts, const uint64_t* lengths,
+                    uint64_t n) {
+    return anchor_run(which, k, seqs, starts, lengths, n);
+}
+uint64_t or_anchor_size(void* h) { return ((AnchorResult*)h)->keys.size(); }
+void or_anchor_export(void* h, uint64_t* keys4, uint32_t* seq, uint32_t* pos, uint8_t* is_rc) {
+    auto* r = (AnchorResult*)h;
+    for (size_t i = 0; i < r->keys.size(); ++i) {
+        kmer_to_words(r->keys[i], keys4 + 4 * i);
+        seq[i] = r->vals[i].seq;
+        pos[i] = r->vals[i].pos;
+        is_rc[i] = r->vals[i].is_rc;
+    }
+}
+void or_anchor_free(void* h) { delete (AnchorResult*)h; }
 void or_spectrum_free(void* h) { delete (Spectrum*)h; }
 uint64_t or_spectrum_num_words(void* h) { return ((Spectrum*)h)->num_words; }
 uint64_t or_spectrum_num_counts(void* h) { return ((Spectrum*)h)->keys.size(); }

@@ -64,6 +64,10 @@ def lib():
                                                          _u8p, _u64p, _u64p, ctypes.c_uint64]),
             "or_prior_boost": (ctypes.c_uint32, [ctypes.c_double]),
             "or_pack_fastq": (ctypes.c_void_p, [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64]),
+            "or_anchor_run": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_uint32, _u8p, _u64p, _u64p, ctypes.c_uint64]),
+            "or_anchor_size": (ctypes.c_uint64, [ctypes.c_void_p]),
+            "or_anchor_export": (None, [ctypes.c_void_p, _u64p, _u32p, _u32p, _u8p]),
+            "or_anchor_free": (None, [ctypes.c_void_p]),
             "or_pack_fastq_q": (ctypes.c_void_p, [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64]),
             "or_pack_n_quals": (ctypes.c_uint64, [ctypes.c_void_p]),
             "or_pack_export_quals": (None, [ctypes.c_void_p, _u8p, _u64p]),
@@ -366,3 +370,25 @@ def pack_fastq(r1, r2=None, quals=False):
         return out + (q[:nq], qo[:n])
     finally:
         L.or_pack_free(h)
+
+
+ANCHOR_KINDS = {"polisher": 0, "paired_info": 1, "splitter": 2}
+
+
+def anchor_index(kind, k, seqs):
+    """The unique-anchor k-mer index of polisher.rs:33-60, paired_info.rs:34-70 or
+    splitter.rs:57-70 over ASCII sequences: (keys (n, 4) Kmer256 limbs sorted, sequence
+    index, position, is_rc) of every k-mer the reference's map keeps."""
+    L = lib()
+    data, starts, lens = concat_ascii(seqs)
+    h = L.or_anchor_run(ANCHOR_KINDS[kind], k, _p(data, _u8p), _p(starts, _u64p), _p(lens, _u64p), len(seqs))
+    try:
+        n = int(L.or_anchor_size(h))
+        keys = np.zeros((max(n, 1), 4), dtype=np.uint64)
+        seq = np.zeros(max(n, 1), dtype=np.uint32)
+        pos = np.zeros(max(n, 1), dtype=np.uint32)
+        rc = np.zeros(max(n, 1), dtype=np.uint8)
+        L.or_anchor_export(h, _p(keys, _u64p), _p(seq, _u32p), _p(pos, _u32p), _p(rc, _u8p))
+        return keys[:n], seq[:n], pos[:n], rc[:n]
+    finally:
+        L.or_anchor_free(h)

@@ -16,6 +16,7 @@ KC_EHIP = -3
 KC_ESTATE = -4
 KC_EUNSUPPORTED = -5
 KC_MAX_K = 127
+KC_ANCHOR_POLISHER, KC_ANCHOR_PAIRED_INFO, KC_ANCHOR_SPLITTER = 0, 1, 2
 KC_FLAG_TIMING = 1
 
 STATUS_NAMES = {KC_EINVAL: "KC_EINVAL", KC_ENOMEM: "KC_ENOMEM", KC_EHIP: "KC_EHIP",
@@ -36,6 +37,8 @@ EXPORTS = (
     "kc_library_quals_set", "kc_library_features", "kc_library_learned_select",
     "kc_fastq_pack", "kc_fastq_last_error", "kc_packed_info_get", "kc_packed_export", "kc_packed_destroy",
     "kc_ingest_create", "kc_ingest_destroy", "kc_ingest_pack", "kc_ingest_keep_quals", "kc_packed_export_quals",
+    "kc_anchor_build", "kc_anchor_count", "kc_anchor_export", "kc_anchor_query", "kc_anchor_last_error",
+    "kc_anchor_destroy",
 )
 
 
@@ -146,6 +149,13 @@ _SIGS = {
     "kc_ingest_destroy": (None, [_vp]),
     "kc_ingest_pack": (ctypes.c_int, [_vp, _u8p, _sz, _u8p, _sz, ctypes.POINTER(_vp)]),
     "kc_ingest_keep_quals": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "kc_anchor_build": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32, _u8p, _sz, _u64p, _u64p, _sz,
+                                       ctypes.POINTER(_vp)]),
+    "kc_anchor_count": (ctypes.c_int, [_vp, _u64p]),
+    "kc_anchor_export": (ctypes.c_int, [_vp, _u64p, _u32p, _u32p, _u8p, _sz]),
+    "kc_anchor_query": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.POINTER(ctypes.c_int64)]),
+    "kc_anchor_last_error": (ctypes.c_char_p, []),
+    "kc_anchor_destroy": (None, [_vp]),
     "kc_packed_export_quals": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _sz]),
 }
 

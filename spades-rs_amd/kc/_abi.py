@@ -75,10 +75,10 @@ class kc_packed_info(ctypes.Structure):
 
 class kc_tune(ctypes.Structure):
     _fields_ = [("windows_per_thread", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("table_load", ctypes.c_float), ("reserved", ctypes.c_uint32), ("ms", ctypes.c_double)]
+                ("table_load", ctypes.c_float), ("insert_path", ctypes.c_uint32), ("ms", ctypes.c_double)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _vp = ctypes.c_void_p

@@ -117,10 +117,43 @@ def test_autotune_keeps_results(oracle):
     eng.close()
 
 
+@pytest.mark.parametrize("k,ec,bits", [(21, False, 1 << 20), (21, True, 1 << 16), (11, False, 1 << 14),
+                                       (31, False, (1 << 18) + 64), (32, True, 1 << 20), (33, False, 1 << 20),
+                                       (55, True, 4096), (63, False, 1 << 19), (5, False, 64 * 3), (1, False, 64)])
+def test_insert_paths_are_exact(oracle, k, ec, bits):
+    """Stage 2 through the partitioned count (windows routed by hash, counted in LDS) and
+    through per-window table atomics: both equal the oracle, on one library."""
+    store, _ = synth.uniform_genome_pairs(seed=300 + k, genome_size=20000, n_pairs=2000, read_len=150, p_start=0.005)
+    d, o, l = store.arrays()
+    S = oracle.Spectrum(d, o, l, k, bits, ec, 5.0, 1)
+    okeys, ocnt, omask = S.counts()
+    oseen, osolid = S.bloom()
+    distinct = set()
+    for path in (2, 1):
+        say(f"case insert path={path} k={k} ec={ec}")
+        eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits, error_correct=ec))
+        eng.set_tune(insert_path=path)
+        eng.upload(store)
+        info = eng.run()
+        keys, cnt, mask = eng.counts()
+        seen, solid = eng.bloom()
+        distinct.add(info["n_distinct"])
+        assert np.array_equal(seen, oseen) and np.array_equal(solid, osolid)
+        assert np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt) and np.array_equal(mask, omask)
+        assert info["cutoff"] == S.cutoff and info["n_solid"] == S.n_solid
+        if ec:
+            assert np.array_equal(eng.reads(), S.corrected_data())
+        # a second run on the same context (buffers reused) gives the same tables
+        eng.run()
+        assert np.array_equal(eng.counts()[0], okeys) and np.array_equal(eng.bloom()[1], osolid)
+        eng.close()
+    assert len(distinct) == 1  # both paths hold the same distinct k-mers
+
+
 def test_tune_rejects_bad_settings():
     from kc import KcError
     eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=1 << 12))
-    for bad in ({"windows_per_thread": 3}, {"table_load": 0.99}, {"blocks_per_cu": 5000}):
+    for bad in ({"windows_per_thread": 3}, {"table_load": 0.99}, {"blocks_per_cu": 5000}, {"insert_path": 3}):
         with pytest.raises(KcError):
             eng.set_tune(**bad)
     eng.close()

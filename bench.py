@@ -358,6 +358,7 @@ def learned_seed(seed, scale, k):
     X = pl.features()
     y = np.isin(_key_index(keys), judge)
     return {"engine": pl, "X": X, "y": y, "cnt": cnt, "cutoff": info_pl["cutoff"], "cutoff_exact": info_ex["cutoff"],
+            "top": info_pl["top_cov"],
             "judge": judge.shape[0],
             "judge_outside": int(judge.shape[0] - y.sum()), "t_exact": info_ex["ms_stage3"],
             "t_plain": info_pl["ms_stage3"], "reads": len(store), "n_occ": info_pl["n_occ"]}
@@ -392,7 +393,16 @@ def bench_learned(args):
         lab, fb, ms = eng.learned_select(m, args.margin)
         ms_best = min([ms] + [eng.learned_select(m, args.margin)[2] for _ in range(2)])
         plain = d["cnt"] >= d["cutoff"]
+        # Library-level margin: when the uncorrected count histogram shows no valley (the
+        # cutoff rule fell back to 2 although the 95th-percentile count is at least 30),
+        # correction decides where the valley lands, which no per-k-mer feature sees; the
+        # whole library then takes the exact selection.
+        lib_fb = d["cutoff"] == 2 and d["top"] >= 30
+        final = d["y"] if lib_fb else lab
+        cost = d["t_plain"] + (d["t_exact"] if lib_fb else ms_best)
         res.append({"entries": int(lab.shape[0]), "agreement": float(np.mean(lab == d["y"])),
+                    "library_fallback": bool(lib_fb), "agreement_after_fallback": float(np.mean(final == d["y"])),
+                    "end_to_end_ms": round(cost, 3),
                     "plain_rule_agreement": float(np.mean(plain == d["y"])),
                     "fallback_fraction": float(fb.mean()), "residual_mismatches": int(np.sum(lab != d["y"])),
                     "judge_solid": d["judge"], "judge_solid_outside_table": d["judge_outside"],
@@ -404,15 +414,18 @@ def bench_learned(args):
         eng.close()
         log(f"test seed {s}: {res[-1]}")
     m.close()
-    agree = float(np.mean([r["agreement"] for r in res]))
+    agree = float(np.mean([r["agreement_after_fallback"] for r in res]))
+    speed = sum(r["exact_selection_ms"] for r in res) / sum(r["end_to_end_ms"] for r in res)
     out = {
         "metric": "learned first pass vs exact Hammer-on solid-k-mer selection (speedup at label agreement)",
-        "value": round(float(np.mean([r["speedup_vs_exact"] for r in res])), 3), "unit": "x",
+        "value": round(speed, 3), "unit": "x",
         "n_gpus": 1, "higher_is_better": True, "data": "synthetic (seeded)", "dtype": "bf16/fp32",
         "config": {"workload": f"repeat-rich genome 12 Mb x {scale}, 20% repeats, 60x, 0.3% errors, k=21",
                    "train_seeds": args.train_seeds, "test_seeds": args.test_seeds, "margin": args.margin,
                    "epochs": args.epochs, "train_rows": int(X.shape[0]), "train_s": round(train_s, 2)},
-        "agreement": agree, "per_test_seed": res,
+        "agreement": agree, "min_agreement": float(min(r["agreement_after_fallback"] for r in res)),
+        "per_kmer_only_agreement": float(np.mean([r["agreement"] for r in res])),
+        "library_fallbacks": int(sum(r["library_fallback"] for r in res)), "per_test_seed": res,
     }
     print(json.dumps(out), flush=True)
 

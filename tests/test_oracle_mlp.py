@@ -52,3 +52,29 @@ def test_forward_is_logistic_of_logit_and_loss_is_bce():
     loss, _ = M.loss_and_grad(w, X, y)
     direct = -np.mean(y * np.log(p) + (1 - y) * np.log(1 - p))
     assert abs(loss - direct) < 1e-9
+
+
+def test_cutoff_fragility_host_rule(oracle):
+    """kc.learn's valley search equals the oracle's determine_solid_kmer_cutoff
+    (assemble.rs:692-735) on random count vectors, and the fragility test flags the
+    default cutoff and is reproducible."""
+    from kc.learn import _valley, cutoff_is_fragile
+    rng = np.random.default_rng(4)
+    for t in range(300):
+        n = int(rng.integers(101, 3000))
+        counts = np.concatenate([rng.geometric(0.5, n) + 1, rng.poisson(int(rng.integers(30, 90)), n)]).astype(np.uint32)
+        covs = np.sort(counts)
+        top = int(covs[len(covs) * 95 // 100])
+        want = oracle.cutoff(counts, 5.0)[0]
+        if top < 30:
+            assert want == 2
+            continue
+        mvs = min(max(top // 4, 10), 35)
+        hist = np.bincount(np.minimum(covs.astype(np.int64), mvs + 1), minlength=mvs + 2)[:mvs + 1]
+        got, found = _valley(hist)
+        assert got == want, t
+        if not found:
+            assert cutoff_is_fragile(counts)
+        assert cutoff_is_fragile(counts) == cutoff_is_fragile(counts)
+    assert not cutoff_is_fragile(np.arange(500), min_coverage=12.0)
+    assert cutoff_is_fragile(np.full(50, 40))

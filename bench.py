@@ -100,7 +100,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="5mb_100x", choices=["5mb_100x", "1mb_30x", "64lib", "learned", "ingest"])
+    ap.add_argument("--workload", default="5mb_100x",
+                    choices=["5mb_100x", "1mb_30x", "64lib", "learned", "ingest", "unitigs"])
     ap.add_argument("--train-seeds", type=int, default=8, help="learned: training seeds (held-out seeds follow)")
     ap.add_argument("--test-seeds", type=int, default=2)
     ap.add_argument("--margin", type=float, default=0.05, help="learned: confidence margin of the first pass")
@@ -144,6 +145,8 @@ def main():
         return bench_learned(args)
     if args.workload == "ingest":
         return bench_ingest(args)
+    if args.workload == "unitigs":
+        return bench_unitigs(args)
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -572,6 +575,82 @@ def bench_ingest(args):
                                                          f"single-thread FastxReader + gates",
                                "device_matches_on_sample": same}
     ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+def bench_unitigs(args):
+    """Compacted unitigs (CompactedGraph::build, graph.rs:65-234) of the 5 Mb / 100x solid
+    set at k = the first of --ks, built on the device from the context's count table.
+    value = solid k-mers compacted per second of device time (the count table already
+    resident); the dominant stage is the adjacency pass (8 hash lookups per k-mer)."""
+    from kc import AssemblerConfig, SpectrumEngine, synth
+    k = int(args.ks.split(",")[0])
+    t0 = time.time()
+    store, meta = synth.workload_5mb_100x(seed=2)
+    log(f"generated {len(store)} reads in {time.time() - t0:.1f}s")
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits, min_coverage=5.0))
+    eng.upload(store)
+    del store
+    info = eng.run()
+    n = int(info["n_solid"])
+    log(f"k={k}: {n} solid k-mers (cutoff {info['cutoff']})")
+    for _ in range(args.warmup):
+        eng.unitigs()
+    devs, walls, stages = [], [], []
+    g = None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        g = eng.unitigs()
+        walls.append((time.perf_counter() - t) * 1e3)
+        devs.append(g.ms_device)
+        stages.append(g.ms_stage)
+    ms_dev = float(np.median(devs))
+    st_med = {name: round(float(np.median([x[name] for x in stages])), 3) for name in g.ms_stage}
+    ok = bool(int(g.kmers.sum()) == n and g.seq.shape[0] == n + len(g) * (k - 1))
+    log(f"device {ms_dev:.3f} ms ({st_med}), {len(g)} unitigs, {g.seq.shape[0]} bases, invariants {ok}")
+    kw = 1 if k <= 32 else 2 if k <= 63 else 4
+    # adjacency pass, per solid k-mer: its value, 8 hash-slot reads, the keys of the found
+    # neighbours (about 2 on a genome path), degree + two neighbour ids out
+    adj_bytes = n * (8 * kw + 8 * 8 + 2 * 8 * kw + 9)
+    adj_ms = st_med["adjacency+links"]
+    res = {
+        "metric": "solid k-mers compacted into unitigs per second (device-resident count table)",
+        "value": round(n / (ms_dev / 1e3), 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_dev, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded)",
+        "config": {"workload": f"unitigs of the 5 Mb genome, 2x150 bp at 100x solid set, k={k}", "k": k,
+                   "n_solid": n, "n_unitigs": len(g), "n_bases": int(g.seq.shape[0]), "stage_ms": st_med,
+                   "wall_ms": round(float(np.median(walls)), 3), "invariants_hold": ok},
+        "roofline": {"bound": "hbm", "kernel": "k_adjacent + k_links (adjacency+links stage)",
+                     "achieved": round(adj_bytes / (adj_ms / 1e3) / 1e9, 1) if adj_ms > 0 else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(adj_bytes / (adj_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if adj_ms > 0 else None,
+                     "traffic": None, "algorithmic_bytes_per_launch": adj_bytes,
+                     "random_lookups_per_s": round(8 * n / (adj_ms / 1e3), 1) if adj_ms > 0 else None},
+    }
+    if not args.no_cpu and k <= 32:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        keys, cnt, mask = eng.counts()
+        solid = keys[mask.astype(bool), 0]
+        gen = meta["genome"][: meta["genome"].shape[0] // 8].astype(np.uint64)
+        m = gen.shape[0] - k + 1
+        fw = np.zeros(m, np.uint64)
+        rv = np.zeros(m, np.uint64)
+        for i in range(k):
+            fw = (fw << np.uint64(2)) | gen[i:i + m]
+            rv = (rv << np.uint64(2)) | (np.uint64(3) - gen[k - 1 - i:k - 1 - i + m])
+        samp = np.intersect1d(np.minimum(fw, rv), solid)
+        sk = np.zeros((samp.size, 4), np.uint64)
+        sk[:, 0] = samp
+        t = time.perf_counter()
+        ref = O.compacted_graph(k, sk, keys, cnt)
+        cpu_s = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": round(samp.size / cpu_s, 1), "unit": "k-mers/s", "cores": 1, "kind": "port",
+                               "sample": f"the {samp.size} solid k-mers of the genome's first 1/8 "
+                                         f"({gen.shape[0]} bases), {len(ref)} unitigs, oracle CompactedGraph::build "
+                                         f"(single thread, ascending order) in {cpu_s:.1f} s"}
+    eng.close()
     print(json.dumps(res), flush=True)
 
 

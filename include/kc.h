@@ -236,6 +236,31 @@ int kc_anchor_query(kc_anchor* a, const uint64_t* keys4, size_t n, int64_t* inde
 const char* kc_anchor_last_error(void);
 void kc_anchor_destroy(kc_anchor* a);
 
+/* ---- Compacted unitigs of a solid set (graph.rs:65-234) ---------------------
+ * CompactedGraph::build over library lib's solid set after kc_count, on the context's
+ * device: the maximal non-branching paths of solid canonical k-mers, each as its ASCII
+ * sequence (the first k-mer's k bases, then the last base of every further k-mer), its
+ * k-mer count and its mean count (the library's counts; graph.rs:159-178).  Unitigs come
+ * ordered by their first k-mer's rank in the sorted count table; a path's strand may be
+ * the reverse complement of the reference's walk (which follows hash order), and a cycle
+ * starts at its smallest k-mer, forward strand (the reference iterating in ascending
+ * order, graph.rs:216-230).  Odd k only (KC_EUNSUPPORTED otherwise).  Build errors are
+ * reported through kc_last_error(ctx); the result is independent of the context. */
+typedef struct kc_unitigs kc_unitigs;
+typedef struct kc_unitigs_info {
+    uint64_t n_unitigs, n_bases, n_kmers;
+    double ms_device;   /* device time of the build */
+    double ms_stage[4]; /* of which: decode + index, adjacency + links, ranking, emit */
+} kc_unitigs_info;
+int kc_unitigs_build(kc_ctx* ctx, uint32_t lib, kc_unitigs** out);
+int kc_unitigs_info_get(const kc_unitigs* u, kc_unitigs_info* info);
+/* seq: n_bases bytes (unitig i at seq[offsets[i] .. offsets[i+1])); offsets: n_unitigs + 1;
+ * kmers, mean_cov: n_unitigs.  Any pointer may be NULL. */
+int kc_unitigs_export(kc_unitigs* u, uint8_t* seq, size_t n_bases, uint64_t* offsets, uint64_t* kmers,
+                      double* mean_cov, size_t n_unitigs);
+const char* kc_unitigs_last_error(const kc_unitigs* u);
+void kc_unitigs_destroy(kc_unitigs* u);
+
 /* ---- Tuning and graph replay ---------------------------------------------- */
 
 /* Launch shape of the table insert and the table load factor.  Every setting gives

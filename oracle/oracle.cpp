@@ -966,6 +966,130 @@ static AnchorMap splitter_index(unsigned k, const std::vector<std::vector<uint8_
     return map;
 }
 
+
+// ---------------------------------------------------------------------------
+// graph.rs:65-234 CompactedGraph::build
+// ---------------------------------------------------------------------------
+// The reference iterates a hash set; this restatement iterates the solid k-mers in
+// ascending order, which is one of the orders the reference can take.
+struct OKmer {
+    Kmer256 kmer;
+    bool is_rc;
+};
+struct Unitig {
+    std::string seq;
+    double mean_cov;
+    uint64_t kmers;
+};
+
+static std::vector<Unitig> compacted_graph(unsigned k, const std::vector<Kmer256>& solid_sorted,
+                                           const std::unordered_map<Kmer256, uint32_t, Kmer256Hash>& cov) {
+    std::vector<Unitig> out;
+    if (k == 0 || solid_sorted.empty()) return out;
+    KmerSet solid(solid_sorted.begin(), solid_sorted.end());
+    auto explicit_seq = [&](const OKmer& o) { return o.is_rc ? kmer_revcomp(o.kmer, k) : o.kmer; };
+    auto succs = [&](const OKmer& o) {
+        std::vector<OKmer> r;
+        const Kmer256 s = explicit_seq(o);
+        for (uint8_t b = 0; b < 4; ++b) {
+            int rc = 0;
+            const Kmer256 c = kmer_canonical(kmer_extend_right(s, b, k), k, &rc);
+            if (solid.count(c)) r.push_back(OKmer{c, rc != 0});
+        }
+        return r;
+    };
+    auto preds = [&](const OKmer& o) {
+        std::vector<OKmer> r;
+        const Kmer256 s = explicit_seq(o);
+        for (uint8_t a = 0; a < 4; ++a) {
+            int rc = 0;
+            const Kmer256 c = kmer_canonical(kmer_prepend_left(s, a, k), k, &rc);
+            if (solid.count(c)) r.push_back(OKmer{c, rc != 0});
+        }
+        return r;
+    };
+    KmerSet visited;
+    auto extract = [&](OKmer start) {
+        std::vector<OKmer> path;
+        OKmer cur = start;
+        while (!visited.count(cur.kmer)) {
+            visited.insert(cur.kmer);
+            path.push_back(cur);
+            const auto sc = succs(cur);
+            if (sc.size() != 1) break;
+            const OKmer nx = sc[0];
+            if (visited.count(nx.kmer)) break;
+            if (preds(nx).size() != 1) break;
+            cur = nx;
+        }
+        if (path.empty()) return path;
+        std::vector<OKmer> back;
+        OKmer bc = path[0];
+        for (;;) {
+            const auto pr = preds(bc);
+            if (pr.size() != 1) break;
+            const OKmer pv = pr[0];
+            if (visited.count(pv.kmer)) break;
+            if (succs(pv).size() != 1) break;
+            visited.insert(pv.kmer);
+            back.push_back(pv);
+            bc = pv;
+        }
+        std::reverse(back.begin(), back.end());
+        back.insert(back.end(), path.begin(), path.end());
+        return back;
+    };
+    auto build = [&](const std::vector<OKmer>& path) {
+        std::vector<uint8_t> buf(k);
+        kmer_to_string(explicit_seq(path[0]), k, buf.data());
+        Unitig u;
+        u.seq.assign(buf.begin(), buf.end());
+        auto covof = [&](const Kmer256& x) {
+            auto it = cov.find(x);
+            return it == cov.end() ? 2.0 : (double)it->second;
+        };
+        double total = covof(path[0].kmer);
+        for (size_t i = 1; i < path.size(); ++i) {
+            kmer_to_string(explicit_seq(path[i]), k, buf.data());
+            u.seq.push_back((char)buf[k - 1]);  // last_base
+            total += covof(path[i].kmer);
+        }
+        u.mean_cov = total / (double)path.size();
+        u.kmers = path.size();
+        return u;
+    };
+    for (const Kmer256& x : solid_sorted) {  // pass 1: paths from boundary nodes
+        if (visited.count(x)) continue;
+        OKmer start{x, false};
+        const auto pf = preds(start);
+        const auto prv = preds(OKmer{x, true});
+        if (pf.size() == 1) {
+            const OKmer p = pf[0];
+            if (succs(p).size() == 1 && !visited.count(p.kmer)) {
+                if (prv.size() == 1) {
+                    const OKmer q = prv[0];
+                    if (succs(q).size() == 1 && !visited.count(q.kmer)) continue;  // interior node
+                    start.is_rc = true;
+                } else {
+                    start.is_rc = true;
+                }
+            }
+        }
+        const auto path = extract(start);
+        if (!path.empty()) out.push_back(build(path));
+    }
+    for (const Kmer256& x : solid_sorted) {  // pass 2: isolated cycles and the rest
+        if (visited.count(x)) continue;
+        const auto path = extract(OKmer{x, false});
+        if (!path.empty()) out.push_back(build(path));
+    }
+    return out;
+}
+
+struct GraphResult {
+    std::vector<Unitig> u;
+};
+
 struct AnchorResult {
     std::vector<Kmer256> keys;  // sorted
     std::vector<Anchor> vals;
@@ -1122,6 +1246,38 @@ void or_anchor_export(void* h, uint64_t* keys4, uint32_t* seq, uint32_t* pos, ui
     }
 }
 void or_anchor_free(void* h) { delete (AnchorResult*)h; }
+// solid keys (any order; sorted here), their counts (cov: keys4c/counts over all count-
+// table entries)
+void* or_graph_build(uint32_t k, const uint64_t* solid4, uint64_t n_solid, const uint64_t* cov4,
+                     const uint32_t* cov_counts, uint64_t n_cov) {
+    std::vector<Kmer256> s(n_solid);
+    for (uint64_t i = 0; i < n_solid; ++i) s[i] = kmer_from_words(solid4 + 4 * i);
+    std::sort(s.begin(), s.end());
+    std::unordered_map<Kmer256, uint32_t, Kmer256Hash> cov;
+    for (uint64_t i = 0; i < n_cov; ++i) cov[kmer_from_words(cov4 + 4 * i)] = cov_counts[i];
+    auto* r = new GraphResult();
+    r->u = compacted_graph(k, s, cov);
+    return r;
+}
+uint64_t or_graph_n(void* h) { return ((GraphResult*)h)->u.size(); }
+uint64_t or_graph_bases(void* h) {
+    uint64_t n = 0;
+    for (auto& u : ((GraphResult*)h)->u) n += u.seq.size();
+    return n;
+}
+void or_graph_export(void* h, uint8_t* seq, uint64_t* off, uint64_t* kmers, double* mean_cov) {
+    auto* r = (GraphResult*)h;
+    uint64_t o = 0;
+    for (size_t i = 0; i < r->u.size(); ++i) {
+        off[i] = o;
+        std::memcpy(seq + o, r->u[i].seq.data(), r->u[i].seq.size());
+        o += r->u[i].seq.size();
+        kmers[i] = r->u[i].kmers;
+        mean_cov[i] = r->u[i].mean_cov;
+    }
+    off[r->u.size()] = o;
+}
+void or_graph_free(void* h) { delete (GraphResult*)h; }
 void or_spectrum_free(void* h) { delete (Spectrum*)h; }
 uint64_t or_spectrum_num_words(void* h) { return ((Spectrum*)h)->num_words; }
 uint64_t or_spectrum_num_counts(void* h) { return ((Spectrum*)h)->keys.size(); }

@@ -68,6 +68,12 @@ def lib():
             "or_anchor_size": (ctypes.c_uint64, [ctypes.c_void_p]),
             "or_anchor_export": (None, [ctypes.c_void_p, _u64p, _u32p, _u32p, _u8p]),
             "or_anchor_free": (None, [ctypes.c_void_p]),
+            "or_graph_build": (ctypes.c_void_p, [ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p, _u32p,
+                                                 ctypes.c_uint64]),
+            "or_graph_n": (ctypes.c_uint64, [ctypes.c_void_p]),
+            "or_graph_bases": (ctypes.c_uint64, [ctypes.c_void_p]),
+            "or_graph_export": (None, [ctypes.c_void_p, _u8p, _u64p, _u64p, ctypes.POINTER(ctypes.c_double)]),
+            "or_graph_free": (None, [ctypes.c_void_p]),
             "or_pack_fastq_q": (ctypes.c_void_p, [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64]),
             "or_pack_n_quals": (ctypes.c_uint64, [ctypes.c_void_p]),
             "or_pack_export_quals": (None, [ctypes.c_void_p, _u8p, _u64p]),
@@ -392,3 +398,29 @@ def anchor_index(kind, k, seqs):
         return keys[:n], seq[:n], pos[:n], rc[:n]
     finally:
         L.or_anchor_free(h)
+
+
+def compacted_graph(k, solid_keys4, cov_keys4, cov_counts):
+    """CompactedGraph::build (graph.rs:65-234) over the solid k-mers (Kmer256 limbs) with
+    coverage from the count table; iterates the solid k-mers in ascending order.  Returns
+    a list of (sequence bytes, kmers_count, mean_coverage) in build order."""
+    L = lib()
+    s = np.ascontiguousarray(solid_keys4, dtype=np.uint64).reshape(-1, 4)
+    c = np.ascontiguousarray(cov_keys4, dtype=np.uint64).reshape(-1, 4)
+    n = np.ascontiguousarray(cov_counts, dtype=np.uint32)
+    s1 = s if s.shape[0] else np.zeros((1, 4), np.uint64)
+    c1 = c if c.shape[0] else np.zeros((1, 4), np.uint64)
+    n1 = n if n.shape[0] else np.zeros(1, np.uint32)
+    h = L.or_graph_build(k, _p(s1, _u64p), s.shape[0], _p(c1, _u64p), _p(n1, _u32p), c.shape[0])
+    try:
+        m, nb = int(L.or_graph_n(h)), int(L.or_graph_bases(h))
+        seq = np.zeros(max(nb, 1), np.uint8)
+        off = np.zeros(m + 1, np.uint64)
+        km = np.zeros(max(m, 1), np.uint64)
+        cov = np.zeros(max(m, 1), np.float64)
+        L.or_graph_export(h, _p(seq, _u8p), _p(off, _u64p), _p(km, _u64p),
+                          cov.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return [(seq[int(off[i]):int(off[i + 1])].tobytes(), int(km[i]), float(cov[i])) for i in range(m)]
+    finally:
+        L.or_graph_free(h)
+

@@ -42,6 +42,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "../../include/kc.h"
 #include "kc_device.h"
@@ -1343,6 +1344,7 @@ struct kc_ctx {
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
+    DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
     uint64_t n_entries = 0;
     SortLayout lay{};
     kc_tune tune{1, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
@@ -2471,8 +2473,12 @@ void kc_destroy(kc_ctx* ctx) {
     ctx->d_gk2.release();
     DevBuf* pbufs[] = {&ctx->d_pseq,  &ctx->d_pstart, &ctx->d_pw0,  &ctx->d_plib, &ctx->d_psum,
                        &ctx->d_pflag, &ctx->d_quals,  &ctx->d_rqoff, &ctx->d_aqoff, &ctx->d_qsum,
-                       &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab};
+                       &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab,   &ctx->d_wide, &ctx->d_wide2,
+                       &ctx->d_p1k0,  &ctx->d_p1k1,   &ctx->d_p1t,   &ctx->d_p2k0, &ctx->d_p2k1,
+                       &ctx->d_p2t,   &ctx->d_cur1,   &ctx->d_cur2,  &ctx->d_pchunk_b, &ctx->d_pchunk_s,
+                       &ctx->d_base1, &ctx->d_lim1,   &ctx->d_base2, &ctx->d_lim2};
     for (DevBuf* b : pbufs) b->release();
+    for (DevBuf& b : ctx->d_gscratch) b.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -2899,6 +2905,89 @@ int kc_library_counts_export(kc_ctx* ctx, uint32_t lib, uint64_t* keys4, uint32_
 int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* solid_mask, size_t n) {
     return kc_library_counts_export(ctx, 0, keys4, counts, solid_mask, n);
 }
+
+}  // extern "C"
+
+// ---- A library's solid set on the device, for the unitig builder (kc_graph.hip) -------
+
+// Solid entry i (entry e0 + idx[i] of the sorted count table) as its value V in vw words,
+// lowest first, and its count.
+__global__ void k_solid_values(SortLayout lay, const uint64_t* __restrict__ lo, const uint64_t* __restrict__ hi,
+                               const WideEntry* __restrict__ wide, const uint32_t* __restrict__ cnt,
+                               const uint64_t* __restrict__ idx, uint64_t n, uint64_t e0, uint32_t vw,
+                               uint64_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+    const uint32_t k = lay.k;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = e0 + idx[i];
+        uint64_t v[4] = {0, 0, 0, 0};
+        if (lay.wide) {
+            for (int w = 0; w < 4; ++w) v[w] = wide[e].v[w];
+        } else if (lay.single) {  // (lib << 2k) | V, or V itself when 2k == 64
+            v[0] = 2 * k >= 64 ? lo[e] : (lo[e] & ((1ull << (2 * k)) - 1));
+        } else if (2 * k <= 63) {
+            v[0] = lo[e];
+        } else {  // hi = (lib << (2k-63)) | (V >> 63), lo = V mod 2^63
+            const uint64_t a = hi[e] & ((1ull << (2 * k - 63)) - 1);
+            v[0] = lo[e] | (a << 63);
+            v[1] = a >> 1;
+        }
+        for (uint32_t w = 0; w < vw; ++w) vals[i * vw + w] = v[w];
+        counts[i] = cnt[e];
+    }
+}
+
+namespace kc {
+
+int ctx_fail(kc_ctx* ctx, int code, const std::string& msg) { return ctx->fail(code, msg); }
+
+DevBuf* graph_scratch(kc_ctx* ctx) { return ctx->d_gscratch; }
+
+int solid_set(kc_ctx* ctx, uint32_t lib, SolidSet* s) {
+    if (!ctx || !s) return KC_EINVAL;
+    if (!ctx->counted) return ctx->fail(KC_ESTATE, "solid set before kc_count");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    s->device = ctx->o.device;
+    s->stream = ctx->st;
+    s->k = ctx->lay.k;
+    s->n = ctx->libs[lib].info.n_solid;
+    return KC_OK;
+}
+
+int solid_values(kc_ctx* ctx, uint32_t lib, uint32_t vw, uint64_t* d_vals, uint32_t* d_counts, DevBuf& idx,
+                 DevBuf& tmp) {
+    SolidSet s{};
+    const int rc = solid_set(ctx, lib, &s);
+    if (rc != KC_OK) return rc;
+    const HostLib& L = ctx->libs[lib];
+    const uint64_t n = L.info.n_counts, e0 = L.entry0;
+    if (s.n == 0 || n == 0) return KC_OK;
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    CTX_TRY(idx.ensure((n + 1) * 8));
+    uint64_t* d_nsel = idx.as<uint64_t>() + n;
+    rocprim::counting_iterator<uint64_t> iota(0);
+    size_t tb = 0;
+    CTX_TRY(rocprim::select(nullptr, tb, iota, ctx->d_sflags.as<uint8_t>() + e0, idx.as<uint64_t>(), d_nsel,
+                            (size_t)n, ctx->st));
+    CTX_TRY(tmp.ensure(tb));
+    tb = tmp.bytes;
+    CTX_TRY(rocprim::select(tmp.p, tb, iota, ctx->d_sflags.as<uint8_t>() + e0, idx.as<uint64_t>(), d_nsel,
+                            (size_t)n, ctx->st));
+    uint64_t got = 0;
+    CTX_TRY(hipMemcpyAsync(&got, d_nsel, 8, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    if (got != s.n) return ctx->fail(KC_EHIP, "solid flags disagree with the solid count");
+    const bool wide = ctx->lay.wide != 0;
+    k_solid_values<<<grid_for(s.n), BLOCK, 0, ctx->st>>>(
+        ctx->lay, ctx->d_lo.as<uint64_t>(), wide ? nullptr : ctx->d_hi.as<uint64_t>(),
+        wide ? ctx->d_wide.as<WideEntry>() : nullptr, ctx->d_cnt.as<uint32_t>(), idx.as<uint64_t>(), s.n, e0, vw,
+        d_vals, d_counts);
+    CTX_TRY(hipGetLastError());
+    return KC_OK;
+}
+
+}  // namespace kc
+
+extern "C" {
 
 int kc_library_quals_set(kc_ctx* ctx, uint32_t lib, const uint8_t* quals, size_t n_quals, const uint64_t* qoffsets) {
     if (!ctx) return KC_EINVAL;

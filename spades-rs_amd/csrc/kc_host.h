@@ -62,7 +62,30 @@ inline std::string hip_msg(const char* what, hipError_t e) { return std::string(
 
 }  // namespace kc
 
+struct kc_ctx;
+
 namespace kc {
+// One library's solid set after kc_count (kc_engine.hip): its size, k, and the context's
+// device and stream.
+struct SolidSet {
+    int device;
+    hipStream_t stream;
+    uint32_t k;
+    uint64_t n;
+};
+int solid_set(kc_ctx* ctx, uint32_t lib, SolidSet* s);
+// Its k-mers in ascending order as vw value words each (V, lowest word first) and their
+// counts, written on the context's stream into device buffers of SolidSet::n entries;
+// idx and tmp are scratch.
+int solid_values(kc_ctx* ctx, uint32_t lib, uint32_t vw, uint64_t* d_vals, uint32_t* d_counts, DevBuf& idx,
+                 DevBuf& tmp);
+// Record an error on the context (kc_last_error) and return code.
+int ctx_fail(kc_ctx* ctx, int code, const std::string& msg);
+// GRAPH_SCRATCH growable device buffers of the context that the unitig builder reuses
+// from call to call (released with the context).
+constexpr int GRAPH_SCRATCH = 24;
+DevBuf* graph_scratch(kc_ctx* ctx);
+
 // is_solid_kmer256 (bloom.rs:111-126) for n device-resident keys (kc_filter.hip).
 hipError_t query_solid(const uint64_t* d_keys4, uint64_t n, const Modulus& M, const uint64_t* d_solid, uint8_t* d_out,
                        hipStream_t st);

@@ -39,6 +39,7 @@ EXPORTS = (
     "kc_ingest_create", "kc_ingest_destroy", "kc_ingest_pack", "kc_ingest_keep_quals", "kc_packed_export_quals",
     "kc_anchor_build", "kc_anchor_count", "kc_anchor_export", "kc_anchor_query", "kc_anchor_last_error",
     "kc_anchor_destroy",
+    "kc_unitigs_build", "kc_unitigs_info_get", "kc_unitigs_export", "kc_unitigs_last_error", "kc_unitigs_destroy",
 )
 
 
@@ -71,6 +72,16 @@ class kc_packed_info(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class kc_unitigs_info(ctypes.Structure):
+    _fields_ = [("n_unitigs", ctypes.c_uint64), ("n_bases", ctypes.c_uint64), ("n_kmers", ctypes.c_uint64),
+                ("ms_device", ctypes.c_double), ("ms_stage", ctypes.c_double * 4)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["ms_stage"] = list(self.ms_stage)
+        return d
 
 
 class kc_tune(ctypes.Structure):
@@ -156,6 +167,11 @@ _SIGS = {
     "kc_anchor_query": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.POINTER(ctypes.c_int64)]),
     "kc_anchor_last_error": (ctypes.c_char_p, []),
     "kc_anchor_destroy": (None, [_vp]),
+    "kc_unitigs_build": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "kc_unitigs_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(kc_unitigs_info)]),
+    "kc_unitigs_export": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u64p, ctypes.POINTER(ctypes.c_double), _sz]),
+    "kc_unitigs_last_error": (ctypes.c_char_p, [_vp]),
+    "kc_unitigs_destroy": (None, [_vp]),
     "kc_packed_export_quals": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _sz]),
 }
 

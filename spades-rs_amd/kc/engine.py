@@ -353,6 +353,11 @@ class SpectrumEngine:
         self._chk(lib().kc_library_reads_export(self._h, library, ptr(out, u8p), dl))
         return out[:dl]
 
+    def unitigs(self, library=0):
+        """CompactedGraph::build (graph.rs:65-234) over the library's solid set, on the GPU."""
+        from .graph import CompactedGraph
+        return CompactedGraph.build(self, library)
+
     def kernel_times(self):
         names = (ctypes.c_char_p * 32)()
         ms = (ctypes.c_double * 32)()

@@ -32,6 +32,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.kc_info) == 8 * 8 + 4 * 2 + 8 * 2
     assert ctypes.sizeof(_abi.kc_tune) == 24 and _abi.kc_tune.ms.offset == 16
     assert ctypes.sizeof(_abi.kc_packed_info) == 64
+    assert ctypes.sizeof(_abi.kc_unitigs_info) == 64 and _abi.kc_unitigs_info.ms_stage.offset == 32
 
 
 def test_struct_layouts_match_c_compiler(tmp_path):

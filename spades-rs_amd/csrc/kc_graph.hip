@@ -12,11 +12,11 @@
 //   1. index    open-addressed hash of the sorted solid values -> rank
 //   2. adjacent 4 successor + 4 predecessor lookups per k-mer (both strands follow)
 //   3. links    the link condition from the neighbours' degrees
-//   4. rank     pointer jumping over the predecessor links of the 2n oriented k-mers:
-//               the head of each chain and the distance to it, and the smallest rank on
-//               each cycle; a cycle is opened before its smallest k-mer in forward
-//               orientation (where the reference's pass 2 starts when it iterates in
-//               ascending order, graph.rs:216-230) and ranked again
+//   4. rank     list ranking of the 2n oriented k-mers: sampled rulers walk their
+//               sublists, pointer jumping ranks the rulers; each k-mer gets its chain's
+//               head and its distance from it.  A cycle is opened before its smallest
+//               k-mer in forward orientation (where the reference's pass 2 starts when
+//               it iterates in ascending order, graph.rs:216-230)
 //   5. emit     one strand per chain (the one whose head has the smaller oriented id),
 //               the unitig's bases and the per-position counts for its mean coverage
 //
@@ -189,10 +189,10 @@ __device__ __forceinline__ uint32_t in_deg(const uint8_t* deg, uint32_t u) {
 }
 
 // The forward strand's outgoing and incoming links (graph.rs:112-126 forward, 138-150
-// backward): nxt[x] / prv[x] are oriented ids or NONE.  The reverse strand's links are the
-// twins: nxt(2x+1) = twin(prv[x]), prv(2x+1) = twin(nxt[x]).
+// backward) as one word per k-mer: rec[x] = nxt | prv << 32, oriented ids or NONE.  The
+// reverse strand's links are the twins: nxt(2x+1) = twin(prv), prv(2x+1) = twin(nxt).
 __global__ void k_links(uint64_t n, const uint8_t* __restrict__ deg, const uint32_t* __restrict__ succ,
-                        const uint32_t* __restrict__ pred, uint32_t* __restrict__ nxt, uint32_t* __restrict__ prv) {
+                        const uint32_t* __restrict__ pred, unsigned long long* __restrict__ rec) {
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
         const uint8_t d = deg[x];
         uint32_t a = NONE, b = NONE;
@@ -204,71 +204,207 @@ __global__ void k_links(uint64_t n, const uint8_t* __restrict__ deg, const uint3
             const uint32_t p = pred[x];
             if ((p >> 1) != x && out_deg(deg, p) == 1) b = p;
         }
-        nxt[x] = a;
-        prv[x] = b;
+        rec[x] = (unsigned long long)a | ((unsigned long long)b << 32);
     }
 }
 
-__device__ __forceinline__ uint32_t prv_of(const uint32_t* nxt, const uint32_t* prv, uint32_t u) {
-    const uint32_t x = u >> 1;
-    if (!(u & 1)) return prv[x];
-    const uint32_t t = nxt[x];
+__device__ __forceinline__ uint32_t rec_nxt(unsigned long long r, uint32_t u) {
+    if (!(u & 1)) return (uint32_t)r;
+    const uint32_t p = (uint32_t)(r >> 32);
+    return p == NONE ? NONE : p ^ 1;
+}
+__device__ __forceinline__ uint32_t rec_prv(unsigned long long r, uint32_t u) {
+    if (!(u & 1)) return (uint32_t)(r >> 32);
+    const uint32_t t = (uint32_t)r;
     return t == NONE ? NONE : t ^ 1;
 }
 
-// Pointer-jumping state of an oriented k-mer: the farthest predecessor reached (itself
-// at a chain's head), the number of links to it, and the smallest rank on the way.
+// ---- list ranking ------------------------------------------------------------------
+// Rulers: every chain head (no incoming link) and both strands of a sampled 1/2^s of the
+// k-mers (Fibonacci hash of the rank).  Each ruler walks its sublist (up to the next
+// ruler) once, labelling every k-mer with (ruler, offset): O(n) dependent reads in all,
+// about 2^s per walk.  Pointer jumping then ranks the short list of rulers.  A cycle with
+// rulers is opened by arithmetic modulo its length; a cycle without any ruler (rare at
+// 1/16 sampling unless short) is walked by each of its own k-mers.
+__device__ __forceinline__ bool is_ruler(unsigned long long r, uint32_t u, uint32_t shift) {
+    return rec_prv(r, u) == NONE || (((uint64_t)(u >> 1) * 0x9E3779B97F4A7C15ull) >> shift) == 0;
+}
+
+__global__ void k_ruler_flags(uint64_t n2, const unsigned long long* __restrict__ rec, uint32_t shift,
+                              uint32_t* __restrict__ flag) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n2; u += (uint64_t)gridDim.x * blockDim.x)
+        flag[u] = is_ruler(rec[u >> 1], (uint32_t)u, shift) ? 1u : 0u;
+}
+
+__global__ void k_ruler_list(uint64_t n2, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rid,
+                             uint32_t* __restrict__ rnode) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n2; u += (uint64_t)gridDim.x * blockDim.x)
+        if (flag[u]) rnode[rid[u]] = (uint32_t)u;
+}
+
+// own[u] = ruler index | offset << 32 for every k-mer a walk reaches (~0 otherwise);
+// rlen = k-mers in the sublist, rmin = its smallest rank, rnext = the next ruler or NONE.
+__global__ void k_walk(uint64_t nr, const uint32_t* __restrict__ rnode, const unsigned long long* __restrict__ rec,
+                       const uint32_t* __restrict__ rid, uint32_t shift, unsigned long long* __restrict__ own,
+                       uint32_t* __restrict__ rlen, uint32_t* __restrict__ rmin, uint32_t* __restrict__ rnext) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nr; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t cur = rnode[i], mn = cur >> 1, off = 0, rn = NONE;
+        own[cur] = i;
+        unsigned long long r = rec[cur >> 1];
+        for (;;) {
+            const uint32_t nx = rec_nxt(r, cur);
+            if (nx == NONE) break;
+            const unsigned long long q = rec[nx >> 1];
+            if (is_ruler(q, nx, shift)) {
+                rn = rid[nx];
+                break;
+            }
+            ++off;
+            own[nx] = i | ((unsigned long long)off << 32);
+            mn = min(mn, nx >> 1);
+            cur = nx;
+            r = q;
+        }
+        rlen[i] = off + 1;
+        rmin[i] = mn;
+        rnext[i] = rn;
+    }
+}
+
+__global__ void k_ruler_prev(uint64_t nr, const uint32_t* __restrict__ rnext, uint32_t* __restrict__ rprv) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nr; i += (uint64_t)gridDim.x * blockDim.x)
+        if (rnext[i] != NONE) rprv[rnext[i]] = (uint32_t)i;
+}
+
+// Pointer-jumping state of a ruler: the farthest predecessor ruler reached (itself at a
+// chain's head), the k-mers from its sublist's start to this one's, and the smallest rank
+// over the sublists covered.
 struct Jump {
     uint32_t j, d, m, pad;
 };
 
-__global__ void k_jump_init(uint64_t n2, const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ prv,
-                            Jump* __restrict__ st) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n2; u += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t p = prv_of(nxt, prv, (uint32_t)u);
-        st[u] = Jump{p == NONE ? (uint32_t)u : p, p == NONE ? 0u : 1u, (uint32_t)(u >> 1), 0u};
+__global__ void k_jump_init(uint64_t nr, const uint32_t* __restrict__ rprv, const uint32_t* __restrict__ rlen,
+                            const uint32_t* __restrict__ rmin, Jump* __restrict__ st) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nr; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = rprv[i];
+        st[i] = Jump{p == NONE ? (uint32_t)i : p, p == NONE ? 0u : rlen[p], rmin[i], 0u};
     }
 }
 
-__global__ void k_jump(uint64_t n2, const Jump* __restrict__ in, Jump* __restrict__ out) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n2; u += (uint64_t)gridDim.x * blockDim.x) {
-        const Jump a = in[u];
+__global__ void k_jump(uint64_t nr, const Jump* __restrict__ in, Jump* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nr; i += (uint64_t)gridDim.x * blockDim.x) {
+        const Jump a = in[i];
         const Jump b = in[a.j];
-        out[u] = Jump{b.j, a.d + b.d, a.m < b.m ? a.m : b.m, 0u};
+        out[i] = Jump{b.j, a.d + b.d, a.m < b.m ? a.m : b.m, 0u};
     }
 }
 
-// After enough rounds every k-mer on a path points at its head (a fixed point); one
-// still pointing at a non-head lies on a cycle, and m is the cycle's smallest rank.  The
-// cycle is opened before that k-mer's forward strand: the link p -> (m, fwd) goes, and
-// with it its twin (m, rc) -> twin(p).  counter[0] counts the cycles.
-__global__ void k_open_cycles(uint64_t n, const Jump* __restrict__ st, uint32_t* __restrict__ nxt,
-                              uint32_t* __restrict__ prv, unsigned long long* __restrict__ counter) {
-    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
-        const Jump a = st[2 * x];
-        if (st[a.j].j == a.j || a.m != (uint32_t)x) continue;
-        const uint32_t p = prv[x];
-        prv[x] = NONE;
-        if (p & 1) prv[p >> 1] = NONE;  // p = (j, rc): its outgoing link is twin(prv[j])
-        else nxt[p >> 1] = NONE;
+// After enough rounds a ruler on a path points at its chain's head, which has no
+// predecessor ruler; on a cycle it does not, and m is the cycle's smallest rank.  The
+// cycle of oriented k-mers through (m, fwd) opens before (m, fwd), where the reference's
+// walk starts it (graph.rs:216-230 in ascending order); its twin cycle opens before
+// twin(p) for the link p -> (m, fwd), i.e. after (m, rc).  The ruler owning that k-mer
+// becomes the head of its ruler list: cut = k-mer | its offset << 32, tail = the ruler
+// before it.  A thread reads rprv only at its own pointer target and writes only its own
+// entry, and the one ruler of a cycle that cuts is the only writer there.
+__global__ void k_cut_cycles(uint64_t nr, const Jump* __restrict__ st, uint32_t* __restrict__ rprv,
+                             const unsigned long long* __restrict__ own, const unsigned long long* __restrict__ rec,
+                             unsigned long long* __restrict__ cut, uint32_t* __restrict__ tail,
+                             unsigned long long* __restrict__ counter) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nr; i += (uint64_t)gridDim.x * blockDim.x) {
+        const Jump a = st[i];
+        if (rprv[a.j] == NONE) continue;
+        const uint32_t m = a.m;
+        uint32_t c = 2 * m;
+        if ((uint32_t)own[c] != (uint32_t)i) {
+            const uint32_t p = (uint32_t)(rec[m] >> 32);  // prv of (m, fwd): on a cycle, never NONE
+            c = p == NONE ? NONE : (p ^ 1);
+            if (c == NONE || (uint32_t)own[c] != (uint32_t)i) continue;
+        }
+        tail[i] = rprv[i];
+        rprv[i] = NONE;
+        cut[i] = (unsigned long long)c | ((own[c] >> 32) << 32);
         atomicAdd(counter, 1ull);
     }
 }
 
+// A k-mer that no walk reached lies on a cycle without rulers: it walks the cycle for its
+// length, smallest rank and that k-mer's strand, then to the cycle's opening k-mer
+// (chosen as above).  small[u] = head | position << 32.  Both walks are bounded by 2n
+// steps and a walk off a chain end marks the k-mer invalid (~0), which cannot happen for
+// k-mers on a cycle.
+__global__ void k_small_cycles(uint64_t n2, const unsigned long long* __restrict__ own,
+                               const unsigned long long* __restrict__ rec, unsigned long long* __restrict__ small) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n2; u += (uint64_t)gridDim.x * blockDim.x) {
+        if (own[u] != ~0ull) continue;
+        uint32_t cur = (uint32_t)u, mn = cur >> 1, mo = cur & 1;
+        uint64_t len = 0;
+        bool ok = true;
+        do {
+            cur = rec_nxt(rec[cur >> 1], cur);
+            if (cur == NONE || ++len > n2) {
+                ok = false;
+                break;
+            }
+            if ((cur >> 1) < mn) {
+                mn = cur >> 1;
+                mo = cur & 1;
+            }
+        } while (cur != (uint32_t)u);
+        if (!ok) {
+            small[u] = ~0ull;
+            continue;
+        }
+        const uint32_t c = mo == 0 ? 2 * mn : (((uint32_t)(rec[mn] >> 32)) ^ 1);
+        uint64_t s = 0;
+        for (cur = (uint32_t)u; cur != c && s <= len; ++s) cur = rec_nxt(rec[cur >> 1], cur);
+        small[u] = s > len ? ~0ull : ((unsigned long long)c | ((unsigned long long)((len - s) % len) << 32));
+    }
+}
+
+// Head and position of an oriented k-mer from its ruler's ranking.
+__device__ __forceinline__ unsigned long long head_pos(uint32_t u, const unsigned long long* __restrict__ own,
+                                                       const Jump* __restrict__ st,
+                                                       const uint32_t* __restrict__ rnode,
+                                                       const unsigned long long* __restrict__ cut,
+                                                       const uint32_t* __restrict__ tail,
+                                                       const uint32_t* __restrict__ rlen,
+                                                       const unsigned long long* __restrict__ small) {
+    const unsigned long long o = own[u];
+    if (o == ~0ull) return small[u];
+    const uint32_t i = (uint32_t)o, off = (uint32_t)(o >> 32);
+    const Jump a = st[i];
+    const unsigned long long cu = cut[a.j];
+    const uint64_t P = (uint64_t)a.d + off;
+    if (cu == ~0ull) return (unsigned long long)rnode[a.j] | (P << 32);
+    const uint32_t t = tail[a.j];
+    const uint64_t C = (uint64_t)st[t].d + rlen[t], co = cu >> 32;
+    const uint64_t pos = P >= co ? P - co : P + C - co;
+    return (cu & 0xFFFFFFFFull) | (pos << 32);
+}
+
 // One strand per chain: the one whose head has the smaller oriented id.  rep[x] = head
-// rank | position << 32 | strand << 63; the chain's tail writes the chain's length at its
-// head's rank.
-__global__ void k_represent(uint64_t n, const Jump* __restrict__ st, const uint32_t* __restrict__ nxt,
-                            const uint32_t* __restrict__ prv, unsigned long long* __restrict__ rep,
-                            uint32_t* __restrict__ len) {
+// rank | position << 32 | strand << 63.  A chain's two strands put x at positions p and
+// L - 1 - p, so the head (position 0) writes the length L = p0 + p1 + 1.  bad[0] is set
+// when a k-mer has no valid position.
+__global__ void k_represent(uint64_t n, const unsigned long long* __restrict__ own, const Jump* __restrict__ st,
+                            const uint32_t* __restrict__ rnode, const unsigned long long* __restrict__ cut,
+                            const uint32_t* __restrict__ tail, const uint32_t* __restrict__ rlen,
+                            const unsigned long long* __restrict__ small, unsigned long long* __restrict__ rep,
+                            uint32_t* __restrict__ len, unsigned long long* __restrict__ bad) {
     for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
-        const Jump a = st[2 * x], b = st[2 * x + 1];
-        const uint32_t o = b.j < a.j ? 1u : 0u;
-        const Jump r = o ? b : a;
-        const uint32_t head = r.j >> 1;
-        rep[x] = (unsigned long long)head | ((unsigned long long)r.d << 32) | ((unsigned long long)o << 63);
-        const uint32_t out = o ? prv[x] : nxt[x];
-        if (out == NONE) len[head] = r.d + 1;
+        const unsigned long long h0 = head_pos(2 * (uint32_t)x, own, st, rnode, cut, tail, rlen, small);
+        const unsigned long long h1 = head_pos(2 * (uint32_t)x + 1, own, st, rnode, cut, tail, rlen, small);
+        if (h0 == ~0ull || h1 == ~0ull) {
+            atomicAdd(bad, 1ull);
+            rep[x] = 0;
+            continue;
+        }
+        const uint32_t o = (uint32_t)h1 < (uint32_t)h0 ? 1u : 0u;
+        const unsigned long long h = o ? h1 : h0;
+        const uint32_t pos = (uint32_t)(h >> 32);
+        rep[x] = ((h & 0xFFFFFFFFull) >> 1) | ((unsigned long long)pos << 32) | ((unsigned long long)o << 63);
+        if (pos == 0) len[x] = (uint32_t)(h0 >> 32) + (uint32_t)(h1 >> 32) + 1;
     }
 }
 
@@ -346,14 +482,18 @@ namespace {
 
 // Scratch buffers, held by the engine context between calls.
 struct Work {
-    DevBuf &vals, &counts, &idx, &tmp, &slots, &deg, &succ, &pred, &nxt, &prv, &st0, &st1, &rep, &len, &is_head,
-        &bases, &boff, &uid, &pos_cov, &cov_scan;
+    DevBuf &vals, &counts, &idx, &tmp, &slots, &deg, &succ, &pred, &rec, &flag, &rid, &own, &small, &rnode, &rlen,
+        &rmin, &rnext, &rprv, &cut, &tail, &st0, &st1, &rep, &len, &is_head, &bases, &boff, &uid, &pos_cov, &cov_scan;
     explicit Work(DevBuf* b)
-        : vals(b[0]), counts(b[1]), idx(b[2]), tmp(b[3]), slots(b[4]), deg(b[5]), succ(b[6]), pred(b[7]), nxt(b[8]),
-          prv(b[9]), st0(b[10]), st1(b[11]), rep(b[12]), len(b[13]), is_head(b[14]), bases(b[15]), boff(b[16]),
-          uid(b[17]), pos_cov(b[18]), cov_scan(b[19]) {}
+        : vals(b[0]), counts(b[1]), idx(b[2]), tmp(b[3]), slots(b[4]), deg(b[5]), succ(b[6]), pred(b[7]), rec(b[8]),
+          flag(b[9]), rid(b[10]), own(b[11]), small(b[12]), rnode(b[13]), rlen(b[14]), rmin(b[15]), rnext(b[16]),
+          rprv(b[17]), cut(b[18]), tail(b[19]), st0(b[20]), st1(b[21]), rep(b[22]), len(b[23]), is_head(b[24]),
+          bases(b[25]), boff(b[26]), uid(b[27]), pos_cov(b[28]), cov_scan(b[29]) {}
 };
-static_assert(GRAPH_SCRATCH >= 20, "scratch buffers");
+static_assert(GRAPH_SCRATCH >= 30, "scratch buffers");
+
+// Sampled rulers: both strands of 1 in 2^RULER_LOG2 k-mers (walks of ~16 steps).
+constexpr uint32_t RULER_LOG2 = 4;
 
 #define G_TRY(expr)                                                                                          \
     do {                                                                                                     \
@@ -381,10 +521,11 @@ int build(kc_ctx* ctx, uint32_t lib, const SolidSet& S, kc_unitigs* U, Work& w, 
     G_TRY(w.deg.ensure(n));
     G_TRY(w.succ.ensure(n * 4));
     G_TRY(w.pred.ensure(n * 4));
-    G_TRY(w.nxt.ensure(n * 4));
-    G_TRY(w.prv.ensure(n * 4));
-    G_TRY(w.st0.ensure(n2 * sizeof(Jump)));
-    G_TRY(w.st1.ensure(n2 * sizeof(Jump)));
+    G_TRY(w.rec.ensure(n * 8));
+    G_TRY(w.flag.ensure(n2 * 4));
+    G_TRY(w.rid.ensure(n2 * 4));
+    G_TRY(w.own.ensure(n2 * 8));
+    G_TRY(w.small.ensure(n2 * 8));
     G_TRY(w.rep.ensure(n * 8));
     G_TRY(w.len.ensure(n * 4));
     G_TRY(w.is_head.ensure(n * 4));
@@ -400,37 +541,77 @@ int build(kc_ctx* ctx, uint32_t lib, const SolidSet& S, kc_unitigs* U, Work& w, 
     (void)hipEventRecord(ev[1], st);
     k_adjacent<KW><<<grid_for(n), BLOCK, 0, st>>>(vals, n, k, slots, cap - 1, w.deg.as<uint8_t>(),
                                                   w.succ.as<uint32_t>(), w.pred.as<uint32_t>());
-    uint32_t* nxt = w.nxt.as<uint32_t>();
-    uint32_t* prv = w.prv.as<uint32_t>();
-    k_links<<<grid_for(n), BLOCK, 0, st>>>(n, w.deg.as<uint8_t>(), w.succ.as<uint32_t>(), w.pred.as<uint32_t>(), nxt,
-                                           prv);
+    auto* rec = w.rec.as<unsigned long long>();
+    k_links<<<grid_for(n), BLOCK, 0, st>>>(n, w.deg.as<uint8_t>(), w.succ.as<uint32_t>(), w.pred.as<uint32_t>(), rec);
     G_TRY(hipGetLastError());
     (void)hipEventRecord(ev[2], st);
-    // 2^rounds >= n bounds every chain's length, so paths reach their fixed point and
-    // every cycle's window covers the whole cycle.
-    uint32_t rounds = 1;
-    while ((1ull << rounds) < n) ++rounds;
+    // rulers, their sublists, and the ranking of the ruler list
+    const uint32_t shift = 64 - RULER_LOG2;
+    uint32_t* flag = w.flag.as<uint32_t>();
+    uint32_t* rid = w.rid.as<uint32_t>();
+    k_ruler_flags<<<grid_for(n2), BLOCK, 0, st>>>(n2, rec, shift, flag);
+    size_t tb = 0;
+    G_TRY(rocprim::exclusive_scan(nullptr, tb, flag, rid, 0u, (size_t)n2, rocprim::plus<uint32_t>(), st));
+    G_TRY(w.tmp.ensure(tb));
+    tb = w.tmp.bytes;
+    G_TRY(rocprim::exclusive_scan(w.tmp.p, tb, flag, rid, 0u, (size_t)n2, rocprim::plus<uint32_t>(), st));
+    uint32_t last_f = 0, last_r = 0;
+    G_TRY(hipMemcpyAsync(&last_f, flag + n2 - 1, 4, hipMemcpyDeviceToHost, st));
+    G_TRY(hipMemcpyAsync(&last_r, rid + n2 - 1, 4, hipMemcpyDeviceToHost, st));
+    G_TRY(hipStreamSynchronize(st));
+    const uint64_t nr = (uint64_t)last_f + last_r;
+    G_TRY(w.rnode.ensure(nr * 4));
+    G_TRY(w.rlen.ensure(nr * 4));
+    G_TRY(w.rmin.ensure(nr * 4));
+    G_TRY(w.rnext.ensure(nr * 4));
+    G_TRY(w.rprv.ensure(nr * 4));
+    G_TRY(w.cut.ensure(nr * 8));
+    G_TRY(w.tail.ensure(nr * 4));
+    G_TRY(w.st0.ensure(nr * sizeof(Jump)));
+    G_TRY(w.st1.ensure(nr * sizeof(Jump)));
+    auto* own = w.own.as<unsigned long long>();
+    uint32_t* rnode = w.rnode.as<uint32_t>();
+    uint32_t* rlen = w.rlen.as<uint32_t>();
+    uint32_t* rprv = w.rprv.as<uint32_t>();
+    auto* cut = w.cut.as<unsigned long long>();
+    auto* counters = w.idx.as<unsigned long long>();  // [0] cycles cut, [1] k-mers without a position
+    k_fill<unsigned long long><<<grid_for(n2), BLOCK, 0, st>>>(own, n2, ~0ull);
+    k_fill<unsigned long long><<<1, 64, 0, st>>>(counters, 2, 0ull);
     Jump* a = w.st0.as<Jump>();
     Jump* b = w.st1.as<Jump>();
-    auto rank = [&]() -> hipError_t {
-        k_jump_init<<<grid_for(n2), BLOCK, 0, st>>>(n2, nxt, prv, a);
-        for (uint32_t r = 0; r < rounds; ++r) {
-            k_jump<<<grid_for(n2), BLOCK, 0, st>>>(n2, a, b);
-            std::swap(a, b);
-        }
-        return hipGetLastError();
-    };
-    G_TRY(rank());
-    auto* counter = w.uid.as<unsigned long long>();  // scratch before the scans
-    k_fill<unsigned long long><<<1, 64, 0, st>>>(counter, 1, 0ull);
-    k_open_cycles<<<grid_for(n), BLOCK, 0, st>>>(n, a, nxt, prv, counter);
-    uint64_t cycles = 0;
-    G_TRY(hipMemcpyAsync(&cycles, counter, 8, hipMemcpyDeviceToHost, st));
-    G_TRY(hipStreamSynchronize(st));
-    if (cycles) G_TRY(rank());
-    (void)hipEventRecord(ev[3], st);
+    if (nr) {
+        k_ruler_list<<<grid_for(n2), BLOCK, 0, st>>>(n2, flag, rid, rnode);
+        k_walk<<<grid_for(nr), BLOCK, 0, st>>>(nr, rnode, rec, rid, shift, own, rlen, w.rmin.as<uint32_t>(),
+                                               w.rnext.as<uint32_t>());
+        k_fill<uint32_t><<<grid_for(nr), BLOCK, 0, st>>>(rprv, nr, NONE);
+        k_fill<unsigned long long><<<grid_for(nr), BLOCK, 0, st>>>(cut, nr, ~0ull);
+        k_ruler_prev<<<grid_for(nr), BLOCK, 0, st>>>(nr, w.rnext.as<uint32_t>(), rprv);
+        // 2^rounds >= nr bounds every ruler list, so paths reach their head and every
+        // cycle's window covers the whole cycle
+        uint32_t rounds = 0;
+        while ((1ull << rounds) < nr) ++rounds;
+        auto rank = [&]() -> hipError_t {
+            k_jump_init<<<grid_for(nr), BLOCK, 0, st>>>(nr, rprv, rlen, w.rmin.as<uint32_t>(), a);
+            for (uint32_t r = 0; r < rounds; ++r) {
+                k_jump<<<grid_for(nr), BLOCK, 0, st>>>(nr, a, b);
+                std::swap(a, b);
+            }
+            return hipGetLastError();
+        };
+        G_TRY(rank());
+        k_cut_cycles<<<grid_for(nr), BLOCK, 0, st>>>(nr, a, rprv, own, rec, cut, w.tail.as<uint32_t>(), counters);
+        uint64_t cycles = 0;
+        G_TRY(hipMemcpyAsync(&cycles, counters, 8, hipMemcpyDeviceToHost, st));
+        G_TRY(hipStreamSynchronize(st));
+        if (cycles) G_TRY(rank());
+    }
+    k_small_cycles<<<grid_for(n2), BLOCK, 0, st>>>(n2, own, rec, w.small.as<unsigned long long>());
     auto* rep = w.rep.as<unsigned long long>();
-    k_represent<<<grid_for(n), BLOCK, 0, st>>>(n, a, nxt, prv, rep, w.len.as<uint32_t>());
+    k_represent<<<grid_for(n), BLOCK, 0, st>>>(n, own, a, rnode, cut, w.tail.as<uint32_t>(), rlen,
+                                               w.small.as<unsigned long long>(), rep, w.len.as<uint32_t>(),
+                                               counters + 1);
+    G_TRY(hipGetLastError());
+    (void)hipEventRecord(ev[3], st);
     k_heads<<<grid_for(n), BLOCK, 0, st>>>(n, k, rep, w.len.as<uint32_t>(), w.is_head.as<uint32_t>(),
                                            w.bases.as<unsigned long long>());
     G_TRY(hipGetLastError());
@@ -458,7 +639,10 @@ int build(kc_ctx* ctx, uint32_t lib, const SolidSet& S, kc_unitigs* U, Work& w, 
     G_TRY(hipMemcpyAsync(&last_bo, boff + n - 1, 8, hipMemcpyDeviceToHost, st));
     G_TRY(hipMemcpyAsync(&last_h, w.is_head.as<uint32_t>() + n - 1, 4, hipMemcpyDeviceToHost, st));
     G_TRY(hipMemcpyAsync(&last_u, uid + n - 1, 4, hipMemcpyDeviceToHost, st));
+    uint64_t unplaced = 0;
+    G_TRY(hipMemcpyAsync(&unplaced, counters + 1, 8, hipMemcpyDeviceToHost, st));
     G_TRY(hipStreamSynchronize(st));
+    if (unplaced) return ctx_fail(ctx, KC_EHIP, "k-mers left without a unitig position");
     const uint64_t n_bases = last_bo + last_b, n_unitigs = (uint64_t)last_u + last_h;
     if (n_bases != n + n_unitigs * (k - 1))
         return ctx_fail(ctx, KC_EHIP, "unitig chains do not cover the solid set");

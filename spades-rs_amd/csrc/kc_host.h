@@ -83,7 +83,7 @@ int solid_values(kc_ctx* ctx, uint32_t lib, uint32_t vw, uint64_t* d_vals, uint3
 int ctx_fail(kc_ctx* ctx, int code, const std::string& msg);
 // GRAPH_SCRATCH growable device buffers of the context that the unitig builder reuses
 // from call to call (released with the context).
-constexpr int GRAPH_SCRATCH = 24;
+constexpr int GRAPH_SCRATCH = 32;
 DevBuf* graph_scratch(kc_ctx* ctx);
 
 // is_solid_kmer256 (bloom.rs:111-126) for n device-resident keys (kc_filter.hip).

@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-load", type=float, default=0.0)
+    ap.add_argument("--insert-path", type=int, default=0,
+                    help="Stage 2 count: 0 table atomics, 2 partitioned, 3 minimizer-binned")
     ap.add_argument("--microbench-sweep", action="store_true",
                     help="only print random-atomic / random-read rates against array size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -166,6 +168,8 @@ def main():
     for k in ks:
         eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), table_load=args.table_load,
                              timing=True)
+        if args.insert_path:
+            eng.set_tune(insert_path=args.insert_path)
         eng.upload(store)
         if args.autotune:
             log(f"k={k} tune: {eng.autotune(reps=1)}")

@@ -49,6 +49,7 @@
 #include "kc_hammer.h"
 #include "kc_mlp.h"
 #include "kc_host.h"
+#include "kc_superkmer.h"
 
 using namespace kc;
 
@@ -675,6 +676,284 @@ __global__ void __launch_bounds__(BLOCK) k_part3(PartRecs<NW, Ev> r2, const unsi
     }
 }
 
+// ---------------------------------------------------------------------------
+// Minimizer-binned Stage 2 count (one library, k <= 63)
+// ---------------------------------------------------------------------------
+//
+// Every window is routed to one of 2^BIN_BITS bins by the hash of its canonical
+// minimizer: the smallest mix of the canonical m-mers it contains, which is the same for a
+// k-mer and its reverse complement, so all occurrences of a canonical k-mer share a bin.
+// A read is cut into super-k-mers, runs of consecutive windows in one bin, each an 8-byte
+// record (active read, first window, window count); the records are radix-sorted by bin.
+// One workgroup then counts a bin exactly in an LDS table (count and first event of
+// every distinct k-mer) and writes the distinct k-mers densely to the table pool with
+// plain stores: no global atomic per window, one per flushed pass.  A bin with too many
+// distinct k-mers for the LDS table is split by key-hash bits into passes (2, 4, ...)
+// that each count a disjoint subset.  The table then holds exactly the entries k_insert
+// leaves (every distinct key with its count and first event), packed at the start of the
+// library's region with no hash placement, which is all the Stage 2 / 3 scans need.
+
+// Super-k-mers, one wave per active read (superkmers() in kc_superkmer.h is the
+// definition).  Windows are taken 64 at a time: lane l holds the keys of m-mers c0 + l
+// and c0 + 64 + l, a sparse-table sliding minimum over shuffles gives window c0 + l its
+// minimizer key, and a ballot marks the windows whose bin differs from the previous
+// window's.  The lane at such a start accounts for the run that ends before it; the
+// read's last run is accounted for after the loop.  Counting (EMIT false) stores the
+// read's record count in cnt[a]; emitting writes its records from sk_off[a] on: bin key,
+// and a | first window << 32 | windows << 56.  No lane diverges on the data.
+template <bool EMIT>
+__global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
+                                                   const uint64_t* __restrict__ act_off,
+                                                   const uint32_t* __restrict__ active,
+                                                   const uint64_t* __restrict__ nwin, uint64_t n_active, uint32_t k,
+                                                   uint32_t* __restrict__ cnt, const uint64_t* __restrict__ sk_off,
+                                                   uint32_t* __restrict__ keys, uint64_t* __restrict__ recs) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    const uint32_t m = minimizer_len(k), w = k - m + 1;
+    uint32_t p = 1;  // largest power of two <= w
+    while (2 * p <= w) p <<= 1;
+    const uint64_t mmask = (1ull << (2 * m)) - 1;
+    const uint32_t* dw = reinterpret_cast<const uint32_t*>(data);
+    for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
+        const uint64_t bit0 = act_off[a] * 8;
+        const uint32_t nw = (uint32_t)nwin[active[a]];
+        const uint32_t nm = nw + w - 1;  // m-mers of the read
+        auto key_at = [&](uint32_t i) -> uint32_t {
+            if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
+            const uint64_t bp = bit0 + 2ull * i;
+            const uint64_t q = bp >> 5;
+            const uint32_t sh = (uint32_t)(bp & 31);
+            const uint64_t win = ((uint64_t)__builtin_bswap32(dw[q]) << 32) | __builtin_bswap32(dw[q + 1]);
+            const uint64_t f = (win >> (64 - sh - 2 * m)) & mmask;
+            const uint64_t r = rev_groups(~f) >> (64 - 2 * m);
+            return mmer_key(f < r ? f : r);
+        };
+        uint32_t open = 0, carry_bin = 0, total = 0;
+        uint64_t out = EMIT ? sk_off[a] : 0;
+        for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
+            // A = positions c0 + lane, B = positions c0 + 64 + lane (up to c0 + 63 + w - 1)
+            uint32_t A = key_at(c0 + lane), B = lane + 1 < w ? key_at(c0 + 64 + lane) : 0xFFFFFFFFu;
+            for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
+                const uint32_t src = (lane + d) & 63;
+                const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
+                const bool wrap = lane + d >= 64;
+                A = min(A, wrap ? tB : tA);
+                if (!wrap) B = min(B, tB);
+            }
+            const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
+            const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
+            const uint32_t bin = minimizer_bin(min(A, e >= 64 ? tB : tA));
+            const uint32_t prev = __shfl(bin, (lane + 63) & 63);
+            const uint32_t pb = lane ? prev : carry_bin;
+            const bool valid = c0 + lane < nw;
+            const bool start = valid && c0 + lane > 0 && bin != pb;
+            const uint64_t sm = __ballot(start);
+            uint32_t nrec = 0, rs = 0, rl = 0;
+            if (start) {
+                const uint64_t below = sm & ((1ull << lane) - 1);
+                rs = below ? c0 + (63 - __clzll(below)) : open;
+                rl = c0 + lane - rs;
+                nrec = (rl + SK_MAX_WINDOWS - 1) / SK_MAX_WINDOWS;
+            }
+            if (EMIT) {
+                uint32_t incl = nrec;
+                for (uint32_t d = 1; d < 64; d <<= 1) {
+                    const uint32_t t = __shfl_up(incl, d);
+                    if (lane >= d) incl += t;
+                }
+                uint64_t o = out + incl - nrec;
+                for (uint32_t t = 0; t < nrec; ++t, ++o) {
+                    const uint32_t f0 = rs + t * SK_MAX_WINDOWS;
+                    keys[o] = pb;
+                    recs[o] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
+                }
+                out += __shfl(incl, 63);
+            } else {
+                total += nrec;
+            }
+            if (sm) open = c0 + (63 - __clzll(sm));
+            carry_bin = __shfl(bin, min(63u, nw - 1 - c0));
+        }
+        // the last run [open, nw)
+        const uint32_t rl = nw - open, nlast = (rl + SK_MAX_WINDOWS - 1) / SK_MAX_WINDOWS;
+        if (EMIT) {
+            for (uint32_t t = lane; t < nlast; t += 64) {
+                const uint32_t f0 = open + t * SK_MAX_WINDOWS;
+                keys[out + t] = carry_bin;
+                recs[out + t] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
+            }
+        } else {
+            for (uint32_t d = 32; d >= 1; d >>= 1) total += __shfl_xor(total, d);
+            if (lane == 0) cnt[a] = total + nlast;
+        }
+    }
+}
+
+// bounds[b] = first sorted record of bin b (b <= BIN_COUNT).
+__global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ bounds) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= BIN_COUNT; b += gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < b) lo = mid + 1;
+            else hi = mid;
+        }
+        bounds[b] = lo;
+    }
+}
+
+// One bin per workgroup pass (see above).  The bin's records are staged in LDS in chunks
+// with the prefix sum of their window counts, and the chunk's windows are dealt to the
+// threads in order: neighbouring lanes take neighbouring windows of one record (distinct
+// k-mers), rather than the same window of reads from the same locus, which would send
+// the whole wave's LDS atomics to one slot.  ctr[0] = entries written, ctr[1] = set when
+// a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts the library with
+// the table insert instead).
+constexpr uint32_t BIN3_RCHUNK = 512;  // records staged per chunk
+
+template <int KW, typename Ev>
+__global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict__ recs,
+                                                     const uint64_t* __restrict__ bounds,
+                                                     const uint8_t* __restrict__ data,
+                                                     const uint64_t* __restrict__ act_ev,
+                                                     const uint64_t* __restrict__ act_off, uint32_t k, uint64_t ev0,
+                                                     Slot<nw_of(KW), Ev>* __restrict__ out,
+                                                     unsigned long long* __restrict__ ctr) {
+    constexpr int NW = nw_of(KW);
+    static_assert(NW <= 2, "k <= 63");
+    __shared__ uint64_t s_k0[BIN3_CAP];
+    __shared__ uint64_t s_k1[NW == 2 ? BIN3_CAP : 1];
+    __shared__ uint32_t s_cnt[BIN3_CAP];
+    __shared__ Ev s_t0[BIN3_CAP];
+    __shared__ uint64_t s_rec[BIN3_RCHUNK];
+    __shared__ uint32_t s_woff[BIN3_RCHUNK + 1];
+    __shared__ uint32_t s_scan[BLOCK];
+    static_assert(BIN3_RCHUNK == 2 * BLOCK, "two staged records per thread");
+    __shared__ uint32_t s_fill, s_over, s_idx, s_np;
+    __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
+    __shared__ unsigned long long s_base;
+    for (uint32_t bin = blockIdx.x; bin < BIN_COUNT; bin += gridDim.x) {
+        const uint64_t rb = bounds[bin], re = bounds[bin + 1];
+        if (rb == re) continue;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_np = 1;
+            s_pass[0] = 1;
+            s_pass[1] = 0;
+        }
+        __syncthreads();
+        while (s_np) {
+            const uint32_t parts = s_pass[2 * (s_np - 1)], part = s_pass[2 * (s_np - 1) + 1];
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < BIN3_CAP; i += blockDim.x) {
+                s_k0[i] = EMPTY_KEY;
+                if (NW == 2) s_k1[i] = EMPTY_KEY;
+                s_cnt[i] = 0;
+                s_t0[i] = (Ev)~(Ev)0;
+            }
+            if (threadIdx.x == 0) {
+                s_fill = 0;
+                s_over = 0;
+                s_idx = 0;
+                --s_np;
+            }
+            for (uint64_t c0 = rb; c0 < re; c0 += BIN3_RCHUNK) {
+                const uint32_t nr = (uint32_t)min<uint64_t>(BIN3_RCHUNK, re - c0);
+                __syncthreads();  // the previous chunk's staging reads are done
+                if (s_over) break;
+                for (uint32_t i = threadIdx.x; i < nr; i += blockDim.x) s_rec[i] = recs[c0 + i];
+                __syncthreads();
+                {  // window offsets of the staged records: each thread sums two, block scan
+                    const uint32_t i0 = 2 * threadIdx.x;
+                    const uint32_t n0 = i0 < nr ? (uint32_t)(s_rec[i0] >> 56) : 0u;
+                    const uint32_t n1 = i0 + 1 < nr ? (uint32_t)(s_rec[i0 + 1] >> 56) : 0u;
+                    s_scan[threadIdx.x] = n0 + n1;
+                    __syncthreads();
+                    for (uint32_t d = 1; d < BLOCK; d <<= 1) {
+                        const uint32_t add = threadIdx.x >= d ? s_scan[threadIdx.x - d] : 0u;
+                        __syncthreads();
+                        s_scan[threadIdx.x] += add;
+                        __syncthreads();
+                    }
+                    const uint32_t excl = s_scan[threadIdx.x] - (n0 + n1);
+                    if (i0 < nr) s_woff[i0] = excl;
+                    if (i0 + 1 < nr) s_woff[i0 + 1] = excl + n0;
+                    if (threadIdx.x == BLOCK - 1) s_woff[nr] = s_scan[BLOCK - 1];
+                }
+                __syncthreads();
+                const uint32_t nwin = s_woff[nr];
+                for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
+                    if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                    uint32_t lo = 0, hi = nr - 1;  // the record holding window x
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (s_woff[mid] <= x) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint64_t rec = s_rec[lo];
+                    const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+                    const uint32_t j = first + (x - s_woff[lo]);
+                    uint64_t key[NW];
+                    pack_key<NW>(canonical<KW>(load_window<KW>(data, act_off[a], j, k), k), key);
+                    const uint64_t h = table_hash<NW>(key);
+                    if (parts > 1 && ((uint32_t)(h >> 40) & (parts - 1)) != part) continue;
+                    const Ev t = (Ev)(act_ev[a] + j - ev0);
+                    uint32_t slot = (uint32_t)h & (BIN3_CAP - 1);
+                    bool done = false;
+                    for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
+                        bool claimed = false;
+                        bool hit = lds_claim_word(&s_k0[slot], key[0], claimed);
+                        if (claimed && atomicAdd(&s_fill, 1u) + 1 > BIN3_FILL) atomicOr(&s_over, 1u);
+                        if (hit && NW == 2) {
+                            bool c2 = false;
+                            hit = lds_claim_word(&s_k1[slot], key[NW - 1], c2);
+                        }
+                        if (hit) {
+                            atomicAdd(&s_cnt[slot], 1u);
+                            atomicMin(&s_t0[slot], t);
+                            done = true;
+                            break;
+                        }
+                        slot = (slot + 1) & (BIN3_CAP - 1);
+                    }
+                    if (!done) atomicOr(&s_over, 1u);
+                }
+            }
+            __syncthreads();
+            if (s_over) {
+                // split this pass in two by the next key-hash bit
+                if (threadIdx.x == 0) {
+                    if (2 * parts > BIN3_MAX_SPLIT || s_np + 2 > 32) {
+                        atomicOr(reinterpret_cast<unsigned int*>(&ctr[1]), 1u);
+                        s_np = 0;
+                    } else {
+                        s_pass[2 * s_np] = 2 * parts;
+                        s_pass[2 * s_np + 1] = part;
+                        s_pass[2 * s_np + 2] = 2 * parts;
+                        s_pass[2 * s_np + 3] = part + parts;
+                        s_np += 2;
+                    }
+                }
+                __syncthreads();
+                continue;
+            }
+            if (threadIdx.x == 0) s_base = atomicAdd(&ctr[0], (unsigned long long)s_fill);
+            __syncthreads();
+            for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
+                if (s_k0[sl] == EMPTY_KEY) continue;
+                Slot<NW, Ev> e;
+                e.k[0] = s_k0[sl];
+                if (NW == 2) e.k[NW - 1] = s_k1[sl];
+                e.t0 = s_t0[sl];
+                e.cnt = s_cnt[sl];
+                out[s_base + atomicAdd(&s_idx, 1u)] = e;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Iterates the slots [s_begin, s_end) of the table pool in chunks of BLOCK * SCAN_ITEMS;
 // f(slot index, library, chunk's first library) for every occupied slot.
 template <int NW, typename Ev, class F, class G>
@@ -1267,12 +1546,12 @@ __global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restric
 enum KernelId {
     KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_HLL, KID_CLEAR, KID_INSERT, KID_FS_CLEAR, KID_FIRST_SETTER,
     KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_PRIOR, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS,
-    KID_PART1, KID_PART2, KID_PART3, KID_COUNT
+    KID_PART1, KID_PART2, KID_PART3, KID_SK, KID_BIN_SORT, KID_BIN_COUNT, KID_COUNT
 };
 const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
     "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_prior", "k_compact", "radix_sort",
-    "k_count_hist", "k_solid_flags", "k_part1", "k_part2", "k_part3"};
+    "k_count_hist", "k_solid_flags", "k_part1", "k_part2", "k_part3", "k_superkmers", "bin_sort", "k_bin_count"};
 
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
 struct HostLib {
@@ -1296,6 +1575,7 @@ struct HostLib {
     // placement in the device pools (set by the planner)
     uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
     uint64_t ev0 = 0;  // first global window number (set by the reads preparation)
+    uint32_t max_len = 0;  // longest read
     uint32_t group = 0;
     kc_info info{};
     uint64_t entry0 = 0;  // first sorted entry
@@ -1340,6 +1620,11 @@ struct kc_ctx {
     DevBuf d_p1k0, d_p1k1, d_p1t, d_p2k0, d_p2k1, d_p2t, d_cur1, d_cur2, d_pchunk_b, d_pchunk_s;
     DevBuf d_base1, d_lim1, d_base2, d_lim2;
     bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
+    // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
+    // bounds, per-read counts and offsets
+    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko;
+    bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
+    bool graph_planning = false;  // the eager run that plans a graph capture
     uint64_t p_nq = 0;
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
@@ -1761,9 +2046,98 @@ struct Run {
         return KC_OK;
     }
 
-    // Lay out the table pool, the Bloom pools and the first-setter groups.
+    // Whether Stage 2 takes the minimizer-binned count: only when the tuning asks for it
+    // (insert_path 3), with one library, k <= 63, reads shorter than 2^24, no Hammer (its
+    // recount hashes into the table), no priors (looked up by hash), and not for a graph
+    // capture or the eager run that plans one.
+    bool use_binned() const {
+        if (L() != 1 || c->kw == 4 || c->capturing || c->graph_planning || c->tune.insert_path != 3) return false;
+        if (c->n_active == 0) return false;
+        if (c->o.error_correct) return false;
+        const HostLib& H = c->libs[0];
+        return H.prior_windows(c->o.k) == 0 && H.max_len < SK_MAX_LEN;
+    }
+
+    // The binned count into the table pool laid out by plan(.., dense = true); on success
+    // the library's region is cut to the entries written.  BIN_FALLBACK: a bin needed more
+    // passes than allowed, so the caller counts with the table insert instead.
+    static constexpr int BIN_FALLBACK = 1;
+    template <int KW, typename Ev>
+    int binned_count() {
+        constexpr int NW = nw_of(KW);
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        hipStream_t st = c->st;
+        const uint64_t na = c->n_active;
+        const uint32_t k = c->o.k;
+        KC_TRY(c->d_skc.ensure(na * 4 + 4));
+        KC_TRY(c->d_sko.ensure(na * 8 + 8));
+        const uint32_t gw = grid_for(na, BLOCK / 64);
+        TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
+                                                               c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
+                                                               na, k, c->d_skc.as<uint32_t>(), nullptr, nullptr,
+                                                               nullptr)));
+        size_t tb = 0;
+        KC_TRY(rocprim::exclusive_scan(nullptr, tb, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(), (uint64_t)0,
+                                       (size_t)na, rocprim::plus<uint64_t>(), st));
+        KC_TRY(c->d_temp.ensure(tb));
+        tb = c->d_temp.bytes;
+        KC_TRY(rocprim::exclusive_scan(c->d_temp.p, tb, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
+                                       (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
+        KC_TRY(hipMemcpyAsync(c->h_stage, c->d_sko.as<uint64_t>() + na - 1, 8, hipMemcpyDeviceToHost, st));
+        KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_skc.as<uint32_t>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+        KC_TRY(hipStreamSynchronize(st));
+        const uint64_t nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
+        KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
+        KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
+        KC_TRY(c->d_br.ensure(nrec * 8 + 8));
+        KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
+        KC_TRY(c->d_bbounds.ensure((BIN_COUNT + 1) * 8));
+        TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
+                                                              c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
+                                                              na, k, nullptr, c->d_sko.as<uint64_t>(),
+                                                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
+        tb = 0;
+        KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
+                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS,
+                                         st));
+        KC_TRY(c->d_temp.ensure(tb));
+        tb = c->d_temp.bytes;
+        TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
+                                                              c->d_bk2.as<uint32_t>(), c->d_br.as<uint64_t>(),
+                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS, st)));
+        k_bin_bounds<<<grid_for(BIN_COUNT + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec,
+                                                               c->d_bbounds.as<uint64_t>());
+        unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
+        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 2, 0ull);
+        KC_TRY(hipGetLastError());
+        const uint32_t g = (uint32_t)std::min<uint64_t>(BIN_COUNT, (uint64_t)c->n_cu * 8);
+        TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
+                                 c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
+                                 c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k, lib_ev0_[0],
+                                 c->d_tab.as<Slot<NW, Ev>>(), ctr)));
+        KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 16, hipMemcpyDeviceToHost, st));
+        KC_TRY(hipStreamSynchronize(st));
+        if (c->h_stage[1]) return BIN_FALLBACK;
+        const uint64_t d = c->h_stage[0];
+        if (d > c->libs[0].nbuckets * SPB) return fail(KC_EHIP, "binned count wrote past the table region");
+        // the region becomes the entries written, the rest of its last bucket empty
+        HostLib& H = c->libs[0];
+        H.est = d;  // exact: the first-setter mode decision reads it
+        H.nbuckets = d ? (d + SPB - 1) / SPB : 0;
+        c->total_buckets = H.nbuckets;
+        if (H.nbuckets * SPB > d)
+            k_table_clear<NW, Ev><<<1, BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>() + d, H.nbuckets * SPB - d);
+        KC_TRY(hipGetLastError());
+        KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets, 8,
+                              hipMemcpyHostToDevice, st));
+        KC_TRY(hipStreamSynchronize(st));
+        return KC_OK;
+    }
+
+    // Lay out the table pool, the Bloom pools and the first-setter groups.  dense: the
+    // binned count's layout, one slot per window (an upper bound on the distinct keys).
     template <int NW, typename Ev>
-    int plan(const std::vector<uint32_t>& grow) {
+    int plan(const std::vector<uint32_t>& grow, bool dense = false) {
         const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
         const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
@@ -1774,6 +2148,7 @@ struct Run {
             const uint64_t occ = lib.info.n_occ + pw;
             uint64_t cap = (uint64_t)std::ceil((double)(lib.est + pw) * 1.10 / load) + 64;
             cap = std::min<uint64_t>(cap, occ * 2 + 64) << grow[l];
+            if (dense) cap = occ;
             lib.nbuckets = occ ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
             lib.bucket0 = b0;
             b0 += lib.nbuckets;
@@ -1868,6 +2243,25 @@ struct Run {
         constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
+        c->last_binned = false;
+        if constexpr (KW <= 2) {
+            if (use_binned()) {
+                std::vector<uint32_t> none(L(), 0);
+                int rc = plan<NW, Ev>(none, true);
+                if (rc) return rc;
+                rc = clear_out();
+                if (rc) return rc;
+                rc = binned_count<KW, Ev>();
+                if (rc == KC_OK) {
+                    c->last_partitioned = false;
+                    c->last_binned = true;
+                    rc = bloom_groups<NW, Ev>(true);
+                    if (rc) return rc;
+                    return read_out();
+                }
+                if (rc != BIN_FALLBACK) return rc;
+            }
+        }
         const bool part = KW <= 2 && use_partition();
         c->last_partitioned = part;
         int rc = part ? partition<KW, Ev>() : estimate_distinct<KW>();
@@ -2420,6 +2814,7 @@ static int add_library(kc_ctx* ctx, const uint8_t* data, size_t data_len, const 
     lib.data.assign(data, data + data_len);
     lib.off.assign(offsets, offsets + n_reads);
     lib.len.assign(lengths, lengths + n_reads);
+    lib.max_len = n_reads ? *std::max_element(lengths, lengths + n_reads) : 0;
     lib.bloom_bits = bloom_bits;
     lib.min_cov = min_cov;
     ctx->libs.push_back(std::move(lib));
@@ -2479,6 +2874,8 @@ void kc_destroy(kc_ctx* ctx) {
                        &ctx->d_base1, &ctx->d_lim1,   &ctx->d_base2, &ctx->d_lim2};
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
+    for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko})
+        b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -2683,7 +3080,9 @@ int kc_run(kc_ctx* ctx, kc_info* info) {
 
 // Eager run, then capture of the planned device work into ctx->gexec.
 static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
+    ctx->graph_planning = true;
     int rc = kc_run(ctx, info);
+    ctx->graph_planning = false;
     if (rc) return rc;
     ctx->graph_release();
     if (ctx->n_occ == 0) return KC_OK;  // nothing to replay: kc_graph_run runs eagerly
@@ -3011,6 +3410,8 @@ int kc_library_quals_set(kc_ctx* ctx, uint32_t lib, const uint8_t* quals, size_t
 static int learned_ready(kc_ctx* ctx, uint32_t lib, size_t n) {
     if (!ctx->counted) return ctx->fail(KC_ESTATE, "features before kc_count");
     if (ctx->o.error_correct) return ctx->fail(KC_ESTATE, "features need the uncorrected table (error_correct off)");
+    if (ctx->last_binned)
+        return ctx->fail(KC_ESTATE, "features look k-mers up by hash; the binned count (insert_path 3) packs the table");
     if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
     if (n != ctx->libs[lib].info.n_counts) return ctx->fail(KC_EINVAL, "n must equal the library's n_counts");
     return KC_OK;
@@ -3100,7 +3501,7 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->blocks_per_cu > 1024) return ctx->fail(KC_EINVAL, "blocks_per_cu must be at most 1024");
     if (tune->table_load != 0.0f && !(tune->table_load > 0.05f && tune->table_load < 0.95f))
         return ctx->fail(KC_EINVAL, "table_load must be 0 or in (0.05, 0.95)");
-    if (tune->insert_path > 2) return ctx->fail(KC_EINVAL, "insert_path must be 0, 1 or 2");
+    if (tune->insert_path > 3) return ctx->fail(KC_EINVAL, "insert_path must be 0, 1, 2 or 3");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
     return KC_OK;

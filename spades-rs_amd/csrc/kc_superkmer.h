@@ -1,0 +1,125 @@
+// kc_superkmer.h — cutting a packed read into super-k-mers by canonical-minimizer bin
+// (the minimizer-binned Stage 2 count in kc_engine.hip).  Host and device: the host
+// build is what the CPU tests of the cut run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kc {
+
+constexpr uint32_t BIN_BITS = 17;
+constexpr uint32_t BIN_COUNT = 1u << BIN_BITS;
+constexpr uint32_t BIN3_CAP = 2048;  // LDS slots per pass
+constexpr uint32_t BIN3_FILL = BIN3_CAP * 3 / 4;
+constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving up
+constexpr uint32_t SK_MAX_WINDOWS = 255;       // windows per record
+constexpr uint32_t SK_MAX_LEN = 1u << 24;      // read length the record can address
+
+// Minimizer length: the whole k-mer up to 11, then 11 (k <= 32) or 13.
+__host__ __device__ constexpr uint32_t minimizer_len(uint32_t k) { return k <= 11 ? k : k <= 32 ? 11 : 13; }
+
+__host__ __device__ __forceinline__ uint64_t mmer_mix(uint64_t c) {
+    c ^= c >> 31;
+    c *= 0x7fb5d329728ea185ull;
+    c ^= c >> 27;
+    c *= 0x81dadef4bc2dd44dull;
+    c ^= c >> 33;
+    return c;
+}
+
+// Sequential reader of the 2-bit bases of a packed read (first base in bits 7..6 of its
+// first byte): keeps the current 32-bit word, so a scan loads one word per 16 bases.
+// The packed data must be 4-byte aligned and readable 3 bytes past the read's end.
+struct BaseReader {
+    const uint32_t* w;
+    uint64_t bit0;
+    uint64_t idx = ~0ull;
+    uint32_t word = 0;
+    __host__ __device__ BaseReader(const uint8_t* data, uint64_t off)
+        : w(reinterpret_cast<const uint32_t*>(data)), bit0(off * 8) {}
+    __host__ __device__ __forceinline__ uint32_t operator()(uint32_t j) {
+        const uint64_t bp = bit0 + 2ull * j;
+        if ((bp >> 5) != idx) {
+            idx = bp >> 5;
+            word = __builtin_bswap32(w[idx]);
+        }
+        return (word >> (30 - (uint32_t)(bp & 31))) & 3u;
+    }
+};
+
+// The 32-bit order key of a canonical m-mer (the top half of its mix).
+__host__ __device__ __forceinline__ uint32_t mmer_key(uint64_t canonical_mmer) {
+    return (uint32_t)(mmer_mix(canonical_mmer) >> 32);
+}
+
+// The bin of a window from its minimizer's key (the smallest key of its m-mers).  The
+// minimum of w keys sits near 0, so its top bits would crowd the low bins: it is mixed
+// once more.
+__host__ __device__ __forceinline__ uint32_t minimizer_bin(uint32_t minkey) {
+    return (uint32_t)(mmer_mix(minkey ^ 0x9e3779b97f4a7c15ull) >> (64 - BIN_BITS));
+}
+
+// Calls emit(first window, windows, bin) for every super-k-mer of a read of len >= k
+// bases at byte offset off: the maximal runs of consecutive windows in one bin, cut after
+// SK_MAX_WINDOWS windows.  A window's bin is minimizer_bin of the smallest mmer_key over
+// its w = k - m + 1 canonical m-mers, which a k-mer and its reverse complement share.
+// This sequential form is the definition (and the host reference); the device cuts reads
+// one wave per read (k_sk_wave).
+template <class F>
+__host__ __device__ __forceinline__ void superkmers(const uint8_t* __restrict__ data, uint64_t off, uint32_t len, uint32_t k,
+                                           F&& emit) {
+    const uint32_t m = minimizer_len(k), w = k - m + 1, nwin = len - k + 1;
+    const uint64_t mask = (1ull << (2 * m)) - 1;
+    const uint32_t top = 2 * (m - 1);
+    BaseReader base(data, off);
+    uint64_t f = 0, r = 0, minh = ~0ull;  // minh: a key, or ~0 before the first
+    uint32_t minp = 0;
+    // minimum over the m-mers [i0, i0 + w) of the window starting at i0; leaves f, r at
+    // the last of them
+    auto rescan = [&](uint32_t i0) {
+        f = r = 0;
+        for (uint32_t j = i0; j < i0 + m - 1; ++j) {
+            const uint32_t b = base(j);
+            f = ((f << 2) | b) & mask;
+            r = (r >> 2) | ((uint64_t)(3 - b) << top);
+        }
+        minh = ~0ull;
+        for (uint32_t i = i0; i < i0 + w; ++i) {
+            const uint32_t b = base(i + m - 1);
+            f = ((f << 2) | b) & mask;
+            r = (r >> 2) | ((uint64_t)(3 - b) << top);
+            const uint64_t h = mmer_key(f < r ? f : r);
+            if (h < minh) {
+                minh = h;
+                minp = i;
+            }
+        }
+    };
+    rescan(0);
+    uint32_t bin = minimizer_bin((uint32_t)minh), first = 0;
+    for (uint32_t j = 1; j < nwin; ++j) {
+        if (minp < j) {
+            rescan(j);
+        } else {
+            const uint32_t b = base(j + k - 1);
+            f = ((f << 2) | b) & mask;
+            r = (r >> 2) | ((uint64_t)(3 - b) << top);
+            const uint64_t h = mmer_key(f < r ? f : r);
+            if (h < minh) {
+                minh = h;
+                minp = j + w - 1;
+            }
+        }
+        const uint32_t nb = minimizer_bin((uint32_t)minh);
+        if (nb != bin || j - first == SK_MAX_WINDOWS) {
+            emit(first, j - first, bin);
+            first = j;
+            bin = nb;
+        }
+    }
+    emit(first, nwin - first, bin);
+}
+
+
+}  // namespace kc

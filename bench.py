@@ -81,16 +81,19 @@ def tune_key(k, tune):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] in ("--traffic-from", "--atomics-from"):
-        # python bench.py --traffic-from KEY FETCH.csv WRITE.csv KERNEL_MS
-        # python bench.py --atomics-from KEY ATOMIC.csv KERNEL_MS
-        # KEY = tune_key(k, tune) as printed by bench.py ("tune_keys")
+        # python bench.py --traffic-from KEY FETCH.csv WRITE.csv KERNEL_MS [KERNEL]
+        # python bench.py --atomics-from KEY ATOMIC.csv KERNEL_MS [KERNEL]
+        # KEY = tune_key(k, tune) as printed by bench.py ("tune_keys") for k_insert, or
+        # "<kernel>:<k>" (e.g. k_bin_count:55); KERNEL = a substring of the kernel's name
         allrec = json.load(open(TRAFFIC_JSON)) if os.path.exists(TRAFFIC_JSON) else {}
         key = sys.argv[2]
         rec = allrec.get(key, {})
         if sys.argv[1] == "--traffic-from":
-            rec.update(traffic_from(sys.argv[3], sys.argv[4], float(sys.argv[5])))
+            kern = sys.argv[6] if len(sys.argv) > 6 else "k_insert<"
+            rec.update(traffic_from(sys.argv[3], sys.argv[4], float(sys.argv[5]), kern))
         else:
-            rec["atomics"] = atomics_from(sys.argv[3], float(sys.argv[4]))
+            kern = sys.argv[5] if len(sys.argv) > 5 else "k_insert<"
+            rec["atomics"] = atomics_from(sys.argv[3], float(sys.argv[4]), kern)
         allrec[key] = rec
         with open(TRAFFIC_JSON, "w") as f:
             json.dump(allrec, f, indent=1, sort_keys=True)
@@ -236,6 +239,27 @@ def main():
                 roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
         if rec and "atomics" in rec:
             roof["atomic_requests_per_s"] = rec["atomics"]["per_s"]
+    elif dom_name in ("k_bin_count", "k_first_setter"):
+        inf = per_k[dom_k]
+        per_launch_s = dom_ms / dom_n / 1e3
+        slot = 16 if dom_k <= 31 else 32  # table slot bytes (one / two key words, 32-bit events)
+        if dom_name == "k_bin_count":
+            # irreducible: the packed reads once, and the distinct entries written out; the
+            # bound is LDS atomics and instruction issue (about 3 LDS atomics per window)
+            alg = stream_bytes(dom_k) + inf["n_distinct"] * slot
+            extra = {"lds_atomic_windows_per_s": round(inf["n_occ"] / per_launch_s, 1)}
+        else:
+            # each distinct key: its slot read, and three random first-setter updates (64 B lines)
+            alg = inf["n_distinct"] * (slot + 3 * G_BYTES)
+            extra = {"atomic_requests_per_s": round(3 * inf["n_distinct"] / per_launch_s, 1)}
+        achieved = alg / per_launch_s / 1e9
+        roof = {"bound": "hbm", "kernel": f"{dom_name} (k={dom_k})", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(dom_ms / dom_n, 4), **extra}
+        rec = json.load(open(TRAFFIC_JSON)).get(f"{dom_name}:{dom_k}") if os.path.exists(TRAFFIC_JSON) else None
+        if rec and "bytes_per_launch" in rec and abs(rec["kernel_ms"] - dom_ms / dom_n) <= 0.1 * rec["kernel_ms"]:
+            roof["traffic"] = rec["bytes_per_launch"]
+            roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
     # whole-path roofline with the design-independent byte count of the path
     path_b = 0
     for k in ks:

@@ -270,9 +270,13 @@ typedef struct kc_tune {
     uint32_t blocks_per_cu;      /* 0: one workgroup per 1024-window tile; else CUs x this
                                     workgroups, each looping over tiles */
     float table_load;            /* device hash-table target load factor (0: kc_opts.table_load) */
-    uint32_t insert_path;        /* Stage 2 count table: 0 or 1 = device-scope atomics per
-                                    window (default), 2 = partitioned (windows routed by hash
-                                    in two levels and counted in LDS) for one library and
+    uint32_t insert_path;        /* Stage 2 count table.  0 (default) or 3 = minimizer-binned:
+                                    reads cut into super-k-mers, radix-sorted by bin, each bin
+                                    counted in LDS, the table packed; taken for one library,
+                                    k <= 63, no Hammer, no priors, outside graph capture, and
+                                    otherwise the table insert below.  1 = device-scope table
+                                    atomics per window.  2 = partitioned (windows routed by
+                                    hash in two levels and counted in LDS), one library,
                                     k <= 63 */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;

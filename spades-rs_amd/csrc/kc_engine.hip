@@ -954,6 +954,30 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     }
 }
 
+// Packed entries (binned count) re-inserted with hash placement into library 0's region:
+// each key is distinct, so its claimed slot has one writer for t0 and cnt.
+template <int NW, typename Ev>
+__global__ void k_rehash(const Slot<NW, Ev>* __restrict__ src, uint64_t n, Slot<NW, Ev>* __restrict__ tab,
+                         const LibDesc* __restrict__ libs, unsigned long long* __restrict__ overflow) {
+    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
+    const LibDesc& L = libs[0];
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const Slot<NW, Ev> e = src[i];
+        if (e.k[0] == EMPTY_KEY) continue;
+        uint64_t key[NW], w0[SPB];
+        for (int w = 0; w < NW; ++w) key[w] = e.k[w];
+        const uint64_t b = L.bucket0 + home_bucket<NW>(key, L.nbuckets);
+        bucket_load<NW, Ev>(tab, b, w0);
+        const uint64_t j = table_find_or_claim<NW, Ev>(tab, L.bucket0, L.nbuckets, b, w0, key, 1u << 20);
+        if (j == ~0ull) {
+            atomicOr(reinterpret_cast<unsigned int*>(overflow), 1u);
+            continue;
+        }
+        tab[j].t0 = e.t0;
+        tab[j].cnt = e.cnt;
+    }
+}
+
 // Iterates the slots [s_begin, s_end) of the table pool in chunks of BLOCK * SCAN_ITEMS;
 // f(slot index, library, chunk's first library) for every occupied slot.
 template <int NW, typename Ev, class F, class G>
@@ -1622,7 +1646,7 @@ struct kc_ctx {
     bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
-    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko;
+    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack;
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
     uint64_t p_nq = 0;
@@ -1901,7 +1925,7 @@ struct Run {
     // Whether Stage 2 takes the partitioned insert: only when the tuning asks for it
     // (insert_path 2), with one library and k <= 63.  On the 5 Mb / 100x workload it is
     // slower than the per-window table atomics (its two scatter levels cost more than the
-    // atomics they remove), so the table insert is the default.
+    // atomics they remove) and than the binned count.
     bool use_partition() const {
         return L() == 1 && c->kw != 4 && !c->capturing && c->tune.insert_path == 2;
     }
@@ -2046,12 +2070,13 @@ struct Run {
         return KC_OK;
     }
 
-    // Whether Stage 2 takes the minimizer-binned count: only when the tuning asks for it
-    // (insert_path 3), with one library, k <= 63, reads shorter than 2^24, no Hammer (its
+    // Whether Stage 2 takes the minimizer-binned count: the default (insert_path 0) and
+    // insert_path 3, with one library, k <= 63, reads shorter than 2^24, no Hammer (its
     // recount hashes into the table), no priors (looked up by hash), and not for a graph
     // capture or the eager run that plans one.
     bool use_binned() const {
-        if (L() != 1 || c->kw == 4 || c->capturing || c->graph_planning || c->tune.insert_path != 3) return false;
+        const uint32_t ip = c->tune.insert_path;
+        if (L() != 1 || c->kw == 4 || c->capturing || c->graph_planning || (ip != 0 && ip != 3)) return false;
         if (c->n_active == 0) return false;
         if (c->o.error_correct) return false;
         const HostLib& H = c->libs[0];
@@ -2131,6 +2156,46 @@ struct Run {
         KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets, 8,
                               hipMemcpyHostToDevice, st));
         KC_TRY(hipStreamSynchronize(st));
+        return KC_OK;
+    }
+
+    // A packed table (left by the binned count) back to hash placement, for the passes
+    // that look k-mers up: the entries are copied aside and re-inserted into a region sized
+    // as plan() sizes one for that many distinct keys.
+    template <int NW, typename Ev>
+    int rehash() {
+        if (!c->last_binned) return KC_OK;
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        hipStream_t st = c->st;
+        HostLib& H = c->libs[0];
+        const uint64_t packed = H.nbuckets * SPB;
+        KC_TRY(c->d_pack.ensure(std::max<uint64_t>(packed, 1) * sizeof(Slot<NW, Ev>)));
+        if (packed)
+            KC_TRY(hipMemcpyAsync(c->d_pack.p, c->d_tab.p, packed * sizeof(Slot<NW, Ev>), hipMemcpyDeviceToDevice, st));
+        const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
+        const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
+        unsigned long long* flag = c->d_counters.as<unsigned long long>() + 14;
+        for (uint32_t grow = 0;; ++grow) {
+            const uint64_t cap = ((uint64_t)std::ceil((double)H.est * 1.10 / load) + 64) << grow;
+            H.nbuckets = std::max<uint64_t>((cap + SPB - 1) / SPB, 1);
+            c->total_buckets = H.nbuckets;
+            KC_TRY(c->d_tab.ensure(H.nbuckets * 64));
+            KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets,
+                                  8, hipMemcpyHostToDevice, st));
+            k_fill<unsigned long long><<<1, 64, 0, st>>>(flag, 1, 0ull);
+            k_table_clear<NW, Ev><<<grid_for(H.nbuckets * SPB), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(),
+                                                                                H.nbuckets * SPB);
+            if (packed)
+                k_rehash<NW, Ev><<<grid_for(packed), BLOCK, 0, st>>>(c->d_pack.as<Slot<NW, Ev>>(), packed,
+                                                                     c->d_tab.as<Slot<NW, Ev>>(),
+                                                                     c->d_libs.as<LibDesc>(), flag);
+            KC_TRY(hipGetLastError());
+            KC_TRY(hipMemcpyAsync(c->h_stage, flag, 8, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipStreamSynchronize(st));
+            if (!c->h_stage[0]) break;
+            if (grow >= 4) return fail(KC_ENOMEM, "device hash table overflow");
+        }
+        c->last_binned = false;
         return KC_OK;
     }
 
@@ -2715,6 +2780,7 @@ struct Run {
     int features(uint32_t lib) {
         constexpr int NW = nw_of(KW);
         hipStream_t st = c->st;
+        if (int rc = rehash<NW, Ev>()) return rc;
         const HostLib& H = c->libs[lib];
         const uint64_t n = H.info.n_counts, e0 = H.entry0;
         const uint64_t slots = c->total_buckets * slots_per_bucket<NW, Ev>();
@@ -2874,7 +2940,8 @@ void kc_destroy(kc_ctx* ctx) {
                        &ctx->d_base1, &ctx->d_lim1,   &ctx->d_base2, &ctx->d_lim2};
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
-    for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko})
+    for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
+                      &ctx->d_pack})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
@@ -3410,8 +3477,6 @@ int kc_library_quals_set(kc_ctx* ctx, uint32_t lib, const uint8_t* quals, size_t
 static int learned_ready(kc_ctx* ctx, uint32_t lib, size_t n) {
     if (!ctx->counted) return ctx->fail(KC_ESTATE, "features before kc_count");
     if (ctx->o.error_correct) return ctx->fail(KC_ESTATE, "features need the uncorrected table (error_correct off)");
-    if (ctx->last_binned)
-        return ctx->fail(KC_ESTATE, "features look k-mers up by hash; the binned count (insert_path 3) packs the table");
     if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
     if (n != ctx->libs[lib].info.n_counts) return ctx->fail(KC_EINVAL, "n must equal the library's n_counts");
     return KC_OK;

@@ -284,7 +284,9 @@ class SpectrumEngine:
         return t.as_dict()
 
     def set_tune(self, windows_per_thread=1, blocks_per_cu=0, table_load=0.0, insert_path=0):
-        """insert_path: 0 or 1 per-window table atomics (default), 2 partitioned (LDS) count."""
+        """insert_path: 0 (default) or 3 minimizer-binned count where it applies (one library,
+        k <= 63, no Hammer, no priors), else per-window table atomics; 1 per-window table
+        atomics; 2 partitioned (LDS) count."""
         t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
                          table_load=float(table_load), insert_path=insert_path)
         self._chk(lib().kc_tune_set(self._h, ctypes.byref(t)))

@@ -208,7 +208,7 @@ def main():
 
     # dominant kernel and its roofline
     tot = sorted(((v[0], k, name, v[1]) for (k, name), v in kernel_ms.items()), reverse=True)
-    for ms, k, name, n in tot[:12]:
+    for ms, k, name, n in tot[:40]:
         log(f"  k={k:3d} {name:18s} {ms / max(n, 1):9.3f} ms/launch x{n}")
     dom_ms, dom_k, dom_name, dom_n = tot[0]
     per_k = {k: inf for k, inf in zip(ks, infos)}

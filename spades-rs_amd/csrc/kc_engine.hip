@@ -700,7 +700,9 @@ __global__ void __launch_bounds__(BLOCK) k_part3(PartRecs<NW, Ev> r2, const unsi
 // window's.  The lane at such a start accounts for the run that ends before it; the
 // read's last run is accounted for after the loop.  Counting (EMIT false) stores the
 // read's record count in cnt[a]; emitting writes its records from sk_off[a] on: bin key,
-// and a | first window << 32 | windows << 56.  No lane diverges on the data.
+// and a | first window << 32 | windows << 56.  No lane diverges on the data.  (Reserving
+// room chunk by chunk on one global counter instead of counting first was measured 40x
+// slower: the same-address atomics serialize.)
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
                                                    const uint64_t* __restrict__ act_off,
@@ -2094,34 +2096,38 @@ struct Run {
         hipStream_t st = c->st;
         const uint64_t na = c->n_active;
         const uint32_t k = c->o.k;
-        KC_TRY(c->d_skc.ensure(na * 4 + 4));
-        KC_TRY(c->d_sko.ensure(na * 8 + 8));
         const uint32_t gw = grid_for(na, BLOCK / 64);
-        TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
-                                                               c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
-                                                               na, k, c->d_skc.as<uint32_t>(), nullptr, nullptr,
-                                                               nullptr)));
-        size_t tb = 0;
-        KC_TRY(rocprim::exclusive_scan(nullptr, tb, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(), (uint64_t)0,
-                                       (size_t)na, rocprim::plus<uint64_t>(), st));
-        KC_TRY(c->d_temp.ensure(tb));
-        tb = c->d_temp.bytes;
-        KC_TRY(rocprim::exclusive_scan(c->d_temp.p, tb, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
-                                       (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
-        KC_TRY(hipMemcpyAsync(c->h_stage, c->d_sko.as<uint64_t>() + na - 1, 8, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_skc.as<uint32_t>() + na - 1, 4, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipStreamSynchronize(st));
-        const uint64_t nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
-        KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
-        KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
-        KC_TRY(c->d_br.ensure(nrec * 8 + 8));
-        KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
+        unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
+        uint64_t nrec = 0;
+        {
+            // records per read, their offsets, then the records in read order
+            KC_TRY(c->d_skc.ensure(na * 4 + 4));
+            KC_TRY(c->d_sko.ensure(na * 8 + 8));
+            TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(
+                              c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
+                              c->d_nwin.as<uint64_t>(), na, k, c->d_skc.as<uint32_t>(), nullptr, nullptr, nullptr)));
+            size_t tb0 = 0;
+            KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
+                                           (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
+            KC_TRY(c->d_temp.ensure(tb0));
+            tb0 = c->d_temp.bytes;
+            KC_TRY(rocprim::exclusive_scan(c->d_temp.p, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
+                                           (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
+            KC_TRY(hipMemcpyAsync(c->h_stage, c->d_sko.as<uint64_t>() + na - 1, 8, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_skc.as<uint32_t>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipStreamSynchronize(st));
+            nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
+            KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
+            KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
+            KC_TRY(c->d_br.ensure(nrec * 8 + 8));
+            KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
+            TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(
+                              c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
+                              c->d_nwin.as<uint64_t>(), na, k, nullptr, c->d_sko.as<uint64_t>(),
+                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
+        }
         KC_TRY(c->d_bbounds.ensure((BIN_COUNT + 1) * 8));
-        TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
-                                                              c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
-                                                              na, k, nullptr, c->d_sko.as<uint64_t>(),
-                                                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
-        tb = 0;
+        size_t tb = 0;
         KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
                                          c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS,
                                          st));
@@ -2132,7 +2138,6 @@ struct Run {
                                                               c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS, st)));
         k_bin_bounds<<<grid_for(BIN_COUNT + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec,
                                                                c->d_bbounds.as<uint64_t>());
-        unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
         k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 2, 0ull);
         KC_TRY(hipGetLastError());
         const uint32_t g = (uint32_t)std::min<uint64_t>(BIN_COUNT, (uint64_t)c->n_cu * 8);

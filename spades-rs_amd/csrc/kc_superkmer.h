@@ -8,9 +8,9 @@
 
 namespace kc {
 
-constexpr uint32_t BIN_BITS = 17;
+constexpr uint32_t BIN_BITS = 18;
 constexpr uint32_t BIN_COUNT = 1u << BIN_BITS;
-constexpr uint32_t BIN3_CAP = 2048;  // LDS slots per pass
+constexpr uint32_t BIN3_CAP = 1024;  // LDS slots per pass
 constexpr uint32_t BIN3_FILL = BIN3_CAP * 3 / 4;
 constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving up
 constexpr uint32_t SK_MAX_WINDOWS = 255;       // windows per record

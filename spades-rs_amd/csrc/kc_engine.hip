@@ -708,8 +708,9 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                                                    const uint64_t* __restrict__ act_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint64_t* __restrict__ nwin, uint64_t n_active, uint32_t k,
-                                                   uint32_t* __restrict__ cnt, const uint64_t* __restrict__ sk_off,
-                                                   uint32_t* __restrict__ keys, uint64_t* __restrict__ recs) {
+                                                   uint32_t bits, uint32_t* __restrict__ cnt,
+                                                   const uint64_t* __restrict__ sk_off, uint32_t* __restrict__ keys,
+                                                   uint64_t* __restrict__ recs) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
     const uint32_t m = minimizer_len(k), w = k - m + 1;
@@ -745,7 +746,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
             }
             const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
             const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
-            const uint32_t bin = minimizer_bin(min(A, e >= 64 ? tB : tA));
+            const uint32_t bin = minimizer_bin(min(A, e >= 64 ? tB : tA), bits);
             const uint32_t prev = __shfl(bin, (lane + 63) & 63);
             const uint32_t pb = lane ? prev : carry_bin;
             const bool valid = c0 + lane < nw;
@@ -792,9 +793,10 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     }
 }
 
-// bounds[b] = first sorted record of bin b (b <= BIN_COUNT).
-__global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ bounds) {
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= BIN_COUNT; b += gridDim.x * blockDim.x) {
+// bounds[b] = first sorted record of bin b (b <= nbins).
+__global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint32_t nbins,
+                             uint64_t* __restrict__ bounds) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nbins; b += gridDim.x * blockDim.x) {
         uint64_t lo = 0, hi = n;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
@@ -820,7 +822,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                                                      const uint8_t* __restrict__ data,
                                                      const uint64_t* __restrict__ act_ev,
                                                      const uint64_t* __restrict__ act_off, uint32_t k, uint64_t ev0,
-                                                     Slot<nw_of(KW), Ev>* __restrict__ out,
+                                                     uint32_t nbins, Slot<nw_of(KW), Ev>* __restrict__ out,
                                                      unsigned long long* __restrict__ ctr) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
@@ -835,7 +837,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     __shared__ uint32_t s_fill, s_over, s_idx, s_np;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
-    for (uint32_t bin = blockIdx.x; bin < BIN_COUNT; bin += gridDim.x) {
+    for (uint32_t bin = blockIdx.x; bin < nbins; bin += gridDim.x) {
         const uint64_t rb = bounds[bin], re = bounds[bin + 1];
         if (rb == re) continue;
         __syncthreads();
@@ -913,7 +915,9 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         }
                         if (hit) {
                             atomicAdd(&s_cnt[slot], 1u);
-                            atomicMin(&s_t0[slot], t);
+                            // t0 only falls: skip the atomic when it cannot lower it
+                            if (__hip_atomic_load(&s_t0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > t)
+                                atomicMin(&s_t0[slot], t);
                             done = true;
                             break;
                         }
@@ -1602,6 +1606,7 @@ struct HostLib {
     uint64_t read0 = 0, data0 = 0, word0 = 0, num_words = 0, bucket0 = 0, nbuckets = 0, fs_off = 0, est = 0;
     uint64_t ev0 = 0;  // first global window number (set by the reads preparation)
     uint32_t max_len = 0;  // longest read
+    uint32_t est_k = 0;    // k of an exact `est` (binned count)
     uint32_t group = 0;
     kc_info info{};
     uint64_t entry0 = 0;  // first sorted entry
@@ -2097,6 +2102,11 @@ struct Run {
         const uint64_t na = c->n_active;
         const uint32_t k = c->o.k;
         const uint32_t gw = grid_for(na, BLOCK / 64);
+        // bins for about BIN_TARGET distinct k-mers each: the last exact count at this k,
+        // else a quarter of the windows
+        HostLib& H0 = c->libs[0];
+        const uint32_t bits = bin_bits_for(H0.est_k == k && H0.est ? H0.est : c->n_occ / 4);
+        const uint32_t nbins = 1u << bits;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
         uint64_t nrec = 0;
         {
@@ -2105,7 +2115,8 @@ struct Run {
             KC_TRY(c->d_sko.ensure(na * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), na, k, c->d_skc.as<uint32_t>(), nullptr, nullptr, nullptr)));
+                              c->d_nwin.as<uint64_t>(), na, k, bits, c->d_skc.as<uint32_t>(), nullptr, nullptr,
+                              nullptr)));
             size_t tb0 = 0;
             KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                            (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
@@ -2123,27 +2134,27 @@ struct Run {
             KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), na, k, nullptr, c->d_sko.as<uint64_t>(),
+                              c->d_nwin.as<uint64_t>(), na, k, bits, nullptr, c->d_sko.as<uint64_t>(),
                               c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
         }
-        KC_TRY(c->d_bbounds.ensure((BIN_COUNT + 1) * 8));
+        KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
         KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
-                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS,
+                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, bits,
                                          st));
         KC_TRY(c->d_temp.ensure(tb));
         tb = c->d_temp.bytes;
         TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
                                                               c->d_bk2.as<uint32_t>(), c->d_br.as<uint64_t>(),
-                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, BIN_BITS, st)));
-        k_bin_bounds<<<grid_for(BIN_COUNT + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec,
-                                                               c->d_bbounds.as<uint64_t>());
+                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, bits, st)));
+        k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec, nbins,
+                                                                     c->d_bbounds.as<uint64_t>());
         k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 2, 0ull);
         KC_TRY(hipGetLastError());
-        const uint32_t g = (uint32_t)std::min<uint64_t>(BIN_COUNT, (uint64_t)c->n_cu * 8);
+        const uint32_t g = (uint32_t)std::min<uint64_t>(nbins, (uint64_t)c->n_cu * 8);
         TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
                                  c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
-                                 c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k, lib_ev0_[0],
+                                 c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k, lib_ev0_[0], nbins,
                                  c->d_tab.as<Slot<NW, Ev>>(), ctr)));
         KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 16, hipMemcpyDeviceToHost, st));
         KC_TRY(hipStreamSynchronize(st));
@@ -2152,7 +2163,8 @@ struct Run {
         if (d > c->libs[0].nbuckets * SPB) return fail(KC_EHIP, "binned count wrote past the table region");
         // the region becomes the entries written, the rest of its last bucket empty
         HostLib& H = c->libs[0];
-        H.est = d;  // exact: the first-setter mode decision reads it
+        H.est = d;  // exact: the first-setter mode decision and the next run's bins read it
+        H.est_k = k;
         H.nbuckets = d ? (d + SPB - 1) / SPB : 0;
         c->total_buckets = H.nbuckets;
         if (H.nbuckets * SPB > d)

@@ -79,45 +79,6 @@ __device__ __forceinline__ uint64_t vhash(const Kv& v) {
     return h;
 }
 
-// Bits [0, 2k) of x kept, the rest cleared.
-template <int KW>
-__device__ __forceinline__ Kv kv_mask(Kv x, uint32_t k) {
-    const uint32_t b = 2 * k;
-    uint64_t w[4] = {x.lo, x.hi, x.w2, x.w3};
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) {
-        const uint32_t lo = 64 * i;
-        if (b <= lo) w[i] = 0;
-        else if (b < lo + 64) w[i] &= (1ull << (b - lo)) - 1;
-    }
-    return Kv{w[0], w[1], w[2], w[3]};
-}
-
-// Kmer256::extend_right (dna.rs): drop the first base, append b.
-template <int KW>
-__device__ __forceinline__ Kv extend_right(const Kv& v, uint32_t b, uint32_t k) {
-    Kv r{(v.lo << 2) | b, 0, 0, 0};
-    if (KW >= 2) r.hi = (v.hi << 2) | (v.lo >> 62);
-    if (KW == 4) {
-        r.w2 = (v.w2 << 2) | (v.hi >> 62);
-        r.w3 = (v.w3 << 2) | (v.w2 >> 62);
-    }
-    return kv_mask<KW>(r, k);
-}
-
-// Kmer256::prepend_left (dna.rs): drop the last base, put a first.
-template <int KW>
-__device__ __forceinline__ Kv prepend_left(const Kv& v, uint32_t a, uint32_t k) {
-    Kv r = kv_shr(v, 2);
-    const uint32_t bit = 2 * k - 2, w = bit >> 6;
-    const uint64_t m = (uint64_t)a << (bit & 63);
-    if (w == 0) r.lo |= m;
-    else if (w == 1) r.hi |= m;
-    else if (w == 2) r.w2 |= m;
-    else r.w3 |= m;
-    return r;
-}
-
 // Slot = (hash tag << 32) | rank; an absent key ends at an empty slot.
 template <int KW>
 __global__ void k_index_insert(const uint64_t* __restrict__ vals, uint64_t n, unsigned long long* __restrict__ slots,

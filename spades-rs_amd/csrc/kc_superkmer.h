@@ -8,8 +8,9 @@
 
 namespace kc {
 
-constexpr uint32_t BIN_BITS = 18;
-constexpr uint32_t BIN_COUNT = 1u << BIN_BITS;
+// Bins: 2^bits, bits chosen per run (bin_bits_for) for about BIN_TARGET distinct k-mers a bin.
+constexpr uint32_t BIN_BITS_MIN = 14, BIN_BITS_MAX = 20;
+constexpr uint32_t BIN_TARGET = 160;
 constexpr uint32_t BIN3_CAP = 1024;  // LDS slots per pass
 constexpr uint32_t BIN3_FILL = BIN3_CAP * 3 / 4;
 constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving up
@@ -26,6 +27,11 @@ __host__ __device__ __forceinline__ uint64_t mmer_mix(uint64_t c) {
     c *= 0x81dadef4bc2dd44dull;
     c ^= c >> 33;
     return c;
+}
+
+// Base j of a packed read at byte offset off (first base in bits 7..6).
+__host__ __device__ __forceinline__ uint32_t packed_base(const uint8_t* __restrict__ data, uint64_t off, uint32_t j) {
+    return (uint32_t)(data[off + (j >> 2)] >> (6 - 2 * (j & 3))) & 3u;
 }
 
 // Sequential reader of the 2-bit bases of a packed read (first base in bits 7..6 of its
@@ -56,19 +62,26 @@ __host__ __device__ __forceinline__ uint32_t mmer_key(uint64_t canonical_mmer) {
 // The bin of a window from its minimizer's key (the smallest key of its m-mers).  The
 // minimum of w keys sits near 0, so its top bits would crowd the low bins: it is mixed
 // once more.
-__host__ __device__ __forceinline__ uint32_t minimizer_bin(uint32_t minkey) {
-    return (uint32_t)(mmer_mix(minkey ^ 0x9e3779b97f4a7c15ull) >> (64 - BIN_BITS));
+__host__ __device__ __forceinline__ uint32_t minimizer_bin(uint32_t minkey, uint32_t bits) {
+    return (uint32_t)(mmer_mix(minkey ^ 0x9e3779b97f4a7c15ull) >> (64 - bits));
+}
+
+// Bin bits for an expected number of distinct k-mers.
+inline uint32_t bin_bits_for(uint64_t distinct) {
+    uint32_t b = BIN_BITS_MIN;
+    while (b < BIN_BITS_MAX && ((uint64_t)BIN_TARGET << b) < distinct) ++b;
+    return b;
 }
 
 // Calls emit(first window, windows, bin) for every super-k-mer of a read of len >= k
 // bases at byte offset off: the maximal runs of consecutive windows in one bin, cut after
-// SK_MAX_WINDOWS windows.  A window's bin is minimizer_bin of the smallest mmer_key over
+// SK_MAX_WINDOWS windows.  A window's bin is minimizer_bin (2^bits bins) of the smallest mmer_key over
 // its w = k - m + 1 canonical m-mers, which a k-mer and its reverse complement share.
 // This sequential form is the definition (and the host reference); the device cuts reads
 // one wave per read (k_sk_wave).
 template <class F>
 __host__ __device__ __forceinline__ void superkmers(const uint8_t* __restrict__ data, uint64_t off, uint32_t len, uint32_t k,
-                                           F&& emit) {
+                                           uint32_t bits, F&& emit) {
     const uint32_t m = minimizer_len(k), w = k - m + 1, nwin = len - k + 1;
     const uint64_t mask = (1ull << (2 * m)) - 1;
     const uint32_t top = 2 * (m - 1);
@@ -97,7 +110,7 @@ __host__ __device__ __forceinline__ void superkmers(const uint8_t* __restrict__ 
         }
     };
     rescan(0);
-    uint32_t bin = minimizer_bin((uint32_t)minh), first = 0;
+    uint32_t bin = minimizer_bin((uint32_t)minh, bits), first = 0;
     for (uint32_t j = 1; j < nwin; ++j) {
         if (minp < j) {
             rescan(j);
@@ -111,7 +124,7 @@ __host__ __device__ __forceinline__ void superkmers(const uint8_t* __restrict__ 
                 minp = j + w - 1;
             }
         }
-        const uint32_t nb = minimizer_bin((uint32_t)minh);
+        const uint32_t nb = minimizer_bin((uint32_t)minh, bits);
         if (nb != bin || j - first == SK_MAX_WINDOWS) {
             emit(first, j - first, bin);
             first = j;

@@ -814,7 +814,7 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // the whole wave's LDS atomics to one slot.  ctr[0] = entries written, ctr[1] = set when
 // a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts the library with
 // the table insert instead).
-constexpr uint32_t BIN3_RCHUNK = 512;  // records staged per chunk
+constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
 
 template <int KW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict__ recs,
@@ -830,10 +830,11 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     __shared__ uint64_t s_k1[NW == 2 ? BIN3_CAP : 1];
     __shared__ uint32_t s_cnt[BIN3_CAP];
     __shared__ Ev s_t0[BIN3_CAP];
-    __shared__ uint64_t s_rec[BIN3_RCHUNK];
-    __shared__ uint32_t s_woff[BIN3_RCHUNK + 1];
+    __shared__ uint64_t s_off[BIN3_RCHUNK];  // staged record: byte offset of its read,
+    __shared__ uint64_t s_ev[BIN3_RCHUNK];   // event of its first window (library-relative),
+    __shared__ uint32_t s_j0[BIN3_RCHUNK];   // its first window,
+    __shared__ uint32_t s_woff[BIN3_RCHUNK + 1];  // and its first window in the chunk
     __shared__ uint32_t s_scan[BLOCK];
-    static_assert(BIN3_RCHUNK == 2 * BLOCK, "two staged records per thread");
     __shared__ uint32_t s_fill, s_over, s_idx, s_np;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
@@ -866,24 +867,27 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 const uint32_t nr = (uint32_t)min<uint64_t>(BIN3_RCHUNK, re - c0);
                 __syncthreads();  // the previous chunk's staging reads are done
                 if (s_over) break;
-                for (uint32_t i = threadIdx.x; i < nr; i += blockDim.x) s_rec[i] = recs[c0 + i];
-                __syncthreads();
-                {  // window offsets of the staged records: each thread sums two, block scan
-                    const uint32_t i0 = 2 * threadIdx.x;
-                    const uint32_t n0 = i0 < nr ? (uint32_t)(s_rec[i0] >> 56) : 0u;
-                    const uint32_t n1 = i0 + 1 < nr ? (uint32_t)(s_rec[i0 + 1] >> 56) : 0u;
-                    s_scan[threadIdx.x] = n0 + n1;
+                {  // stage one record per thread; block scan of the window counts
+                    const uint32_t i = threadIdx.x;
+                    uint32_t n0 = 0;
+                    if (i < nr) {
+                        const uint64_t rec = recs[c0 + i];
+                        const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+                        n0 = (uint32_t)(rec >> 56);
+                        s_off[i] = act_off[a];
+                        s_ev[i] = act_ev[a] + first - ev0;
+                        s_j0[i] = first;
+                    }
+                    s_scan[i] = n0;
                     __syncthreads();
                     for (uint32_t d = 1; d < BLOCK; d <<= 1) {
-                        const uint32_t add = threadIdx.x >= d ? s_scan[threadIdx.x - d] : 0u;
+                        const uint32_t add = i >= d ? s_scan[i - d] : 0u;
                         __syncthreads();
-                        s_scan[threadIdx.x] += add;
+                        s_scan[i] += add;
                         __syncthreads();
                     }
-                    const uint32_t excl = s_scan[threadIdx.x] - (n0 + n1);
-                    if (i0 < nr) s_woff[i0] = excl;
-                    if (i0 + 1 < nr) s_woff[i0 + 1] = excl + n0;
-                    if (threadIdx.x == BLOCK - 1) s_woff[nr] = s_scan[BLOCK - 1];
+                    if (i < nr) s_woff[i] = s_scan[i] - n0;
+                    if (i == BLOCK - 1) s_woff[nr] = s_scan[BLOCK - 1];
                 }
                 __syncthreads();
                 const uint32_t nwin = s_woff[nr];
@@ -895,14 +899,12 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         if (s_woff[mid] <= x) lo = mid;
                         else hi = mid - 1;
                     }
-                    const uint64_t rec = s_rec[lo];
-                    const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
-                    const uint32_t j = first + (x - s_woff[lo]);
+                    const uint32_t dj = x - s_woff[lo];
                     uint64_t key[NW];
-                    pack_key<NW>(canonical<KW>(load_window<KW>(data, act_off[a], j, k), k), key);
+                    pack_key<NW>(canonical<KW>(load_window<KW>(data, s_off[lo], s_j0[lo] + dj, k), k), key);
                     const uint64_t h = table_hash<NW>(key);
                     if (parts > 1 && ((uint32_t)(h >> 40) & (parts - 1)) != part) continue;
-                    const Ev t = (Ev)(act_ev[a] + j - ev0);
+                    const Ev t = (Ev)(s_ev[lo] + dj);
                     uint32_t slot = (uint32_t)h & (BIN3_CAP - 1);
                     bool done = false;
                     for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {

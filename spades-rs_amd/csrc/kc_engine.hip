@@ -699,18 +699,21 @@ __global__ void __launch_bounds__(BLOCK) k_part3(PartRecs<NW, Ev> r2, const unsi
 // minimizer key, and a ballot marks the windows whose bin differs from the previous
 // window's.  The lane at such a start accounts for the run that ends before it; the
 // read's last run is accounted for after the loop.  Counting (EMIT false) stores the
-// read's record count in cnt[a]; emitting writes its records from sk_off[a] on: bin key,
-// and a | first window << 32 | windows << 56.  No lane diverges on the data.  (Reserving
-// room chunk by chunk on one global counter instead of counting first was measured 40x
-// slower: the same-address atomics serialize.)
+// read's record count in cnt[a] and every window's bin in wbin (at its global window
+// number act_ev[a] + j); emitting reads the bins back instead of recomputing them and
+// writes the read's records from sk_off[a] on: bin key, and a | first window << 32 |
+// windows << 56.  No lane diverges on the data.  (Reserving room chunk by chunk on one
+// global counter instead of counting first was measured 40x slower: the same-address
+// atomics serialize.)
 template <bool EMIT>
 __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
                                                    const uint64_t* __restrict__ act_off,
                                                    const uint32_t* __restrict__ active,
-                                                   const uint64_t* __restrict__ nwin, uint64_t n_active, uint32_t k,
+                                                   const uint64_t* __restrict__ nwin,
+                                                   const uint64_t* __restrict__ act_ev, uint64_t n_active, uint32_t k,
                                                    uint32_t bits, uint32_t* __restrict__ cnt,
-                                                   const uint64_t* __restrict__ sk_off, uint32_t* __restrict__ keys,
-                                                   uint64_t* __restrict__ recs) {
+                                                   uint32_t* __restrict__ wbin, const uint64_t* __restrict__ sk_off,
+                                                   uint32_t* __restrict__ keys, uint64_t* __restrict__ recs) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
     const uint32_t m = minimizer_len(k), w = k - m + 1;
@@ -721,6 +724,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
         const uint64_t bit0 = act_off[a] * 8;
         const uint32_t nw = (uint32_t)nwin[active[a]];
+        uint32_t* wb = wbin + act_ev[a];
         const uint32_t nm = nw + w - 1;  // m-mers of the read
         auto key_at = [&](uint32_t i) -> uint32_t {
             if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
@@ -735,21 +739,27 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
         uint32_t open = 0, carry_bin = 0, total = 0;
         uint64_t out = EMIT ? sk_off[a] : 0;
         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
-            // A = positions c0 + lane, B = positions c0 + 64 + lane (up to c0 + 63 + w - 1)
-            uint32_t A = key_at(c0 + lane), B = lane + 1 < w ? key_at(c0 + 64 + lane) : 0xFFFFFFFFu;
-            for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
-                const uint32_t src = (lane + d) & 63;
-                const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
-                const bool wrap = lane + d >= 64;
-                A = min(A, wrap ? tB : tA);
-                if (!wrap) B = min(B, tB);
+            const bool valid = c0 + lane < nw;
+            uint32_t bin;
+            if (EMIT) {
+                bin = valid ? wb[c0 + lane] : 0u;
+            } else {
+                // A = positions c0 + lane, B = positions c0 + 64 + lane (up to c0 + 63 + w - 1)
+                uint32_t A = key_at(c0 + lane), B = lane + 1 < w ? key_at(c0 + 64 + lane) : 0xFFFFFFFFu;
+                for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
+                    const uint32_t src = (lane + d) & 63;
+                    const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
+                    const bool wrap = lane + d >= 64;
+                    A = min(A, wrap ? tB : tA);
+                    if (!wrap) B = min(B, tB);
+                }
+                const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
+                const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
+                bin = minimizer_bin(min(A, e >= 64 ? tB : tA), bits);
+                if (valid) wb[c0 + lane] = bin;
             }
-            const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
-            const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
-            const uint32_t bin = minimizer_bin(min(A, e >= 64 ? tB : tA), bits);
             const uint32_t prev = __shfl(bin, (lane + 63) & 63);
             const uint32_t pb = lane ? prev : carry_bin;
-            const bool valid = c0 + lane < nw;
             const bool start = valid && c0 + lane > 0 && bin != pb;
             const uint64_t sm = __ballot(start);
             uint32_t nrec = 0, rs = 0, rl = 0;
@@ -1655,7 +1665,7 @@ struct kc_ctx {
     bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
-    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack;
+    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin;
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
     uint64_t p_nq = 0;
@@ -2115,10 +2125,11 @@ struct Run {
             // records per read, their offsets, then the records in read order
             KC_TRY(c->d_skc.ensure(na * 4 + 4));
             KC_TRY(c->d_sko.ensure(na * 8 + 8));
+            KC_TRY(c->d_wbin.ensure(c->n_occ * 4 + 4));
             TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), na, k, bits, c->d_skc.as<uint32_t>(), nullptr, nullptr,
-                              nullptr)));
+                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na, k, bits,
+                              c->d_skc.as<uint32_t>(), c->d_wbin.as<uint32_t>(), nullptr, nullptr, nullptr)));
             size_t tb0 = 0;
             KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                            (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
@@ -2136,8 +2147,9 @@ struct Run {
             KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), na, k, bits, nullptr, c->d_sko.as<uint64_t>(),
-                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
+                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na, k, bits, nullptr,
+                              c->d_wbin.as<uint32_t>(), c->d_sko.as<uint64_t>(), c->d_bk.as<uint32_t>(),
+                              c->d_br.as<uint64_t>())));
         }
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
@@ -2960,7 +2972,7 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
-                      &ctx->d_pack})
+                      &ctx->d_pack, &ctx->d_wbin})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);

@@ -2090,16 +2090,32 @@ struct Run {
     }
 
     // Whether Stage 2 takes the minimizer-binned count: the default (insert_path 0) and
-    // insert_path 3, with one library, k <= 63, reads shorter than 2^24, no Hammer (its
-    // recount hashes into the table), no priors (looked up by hash), and not for a graph
-    // capture or the eager run that plans one.
+    // insert_path 3, with one library, k <= 63, reads shorter than 2^24, and not for a
+    // graph capture or the eager run that plans one.  Hammer's recount and the priors hash
+    // into the table, so Stage 3 lays it out for hashing first (hashed_layout, rehash).
     bool use_binned() const {
         const uint32_t ip = c->tune.insert_path;
         if (L() != 1 || c->kw == 4 || c->capturing || c->graph_planning || (ip != 0 && ip != 3)) return false;
         if (c->n_active == 0) return false;
-        if (c->o.error_correct) return false;
-        const HostLib& H = c->libs[0];
-        return H.prior_windows(c->o.k) == 0 && H.max_len < SK_MAX_LEN;
+        return c->libs[0].max_len < SK_MAX_LEN;
+    }
+
+    // Library 0's region sized for hash placement of `keys` keys (as plan() sizes one),
+    // grown 2^grow times; the device descriptor follows.
+    template <int NW, typename Ev>
+    int hashed_layout(uint64_t keys, uint32_t grow) {
+        constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
+        const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
+        HostLib& H = c->libs[0];
+        const uint64_t cap = ((uint64_t)std::ceil((double)keys * 1.10 / load) + 64) << grow;
+        H.nbuckets = std::max<uint64_t>((cap + SPB - 1) / SPB, 1);
+        c->total_buckets = H.nbuckets;
+        KC_TRY(c->d_tab.ensure(H.nbuckets * 64));
+        KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets, 8,
+                              hipMemcpyHostToDevice, c->st));
+        KC_TRY(hipStreamSynchronize(c->st));
+        return KC_OK;
     }
 
     // The binned count into the table pool laid out by plan(.., dense = true); on success
@@ -2191,8 +2207,9 @@ struct Run {
     }
 
     // A packed table (left by the binned count) back to hash placement, for the passes
-    // that look k-mers up: the entries are copied aside and re-inserted into a region sized
-    // as plan() sizes one for that many distinct keys.
+    // that look k-mers up (priors, features): the entries are copied aside and re-inserted
+    // into a region sized as plan() sizes one for that many distinct keys and the prior
+    // windows.
     template <int NW, typename Ev>
     int rehash() {
         if (!c->last_binned) return KC_OK;
@@ -2203,16 +2220,9 @@ struct Run {
         KC_TRY(c->d_pack.ensure(std::max<uint64_t>(packed, 1) * sizeof(Slot<NW, Ev>)));
         if (packed)
             KC_TRY(hipMemcpyAsync(c->d_pack.p, c->d_tab.p, packed * sizeof(Slot<NW, Ev>), hipMemcpyDeviceToDevice, st));
-        const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
-        const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
         unsigned long long* flag = c->d_counters.as<unsigned long long>() + 14;
         for (uint32_t grow = 0;; ++grow) {
-            const uint64_t cap = ((uint64_t)std::ceil((double)H.est * 1.10 / load) + 64) << grow;
-            H.nbuckets = std::max<uint64_t>((cap + SPB - 1) / SPB, 1);
-            c->total_buckets = H.nbuckets;
-            KC_TRY(c->d_tab.ensure(H.nbuckets * 64));
-            KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets,
-                                  8, hipMemcpyHostToDevice, st));
+            if (int rc = hashed_layout<NW, Ev>(H.est + H.prior_windows(c->o.k), grow)) return rc;
             k_fill<unsigned long long><<<1, 64, 0, st>>>(flag, 1, 0ull);
             k_table_clear<NW, Ev><<<grid_for(H.nbuckets * SPB), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(),
                                                                                 H.nbuckets * SPB);
@@ -2619,6 +2629,11 @@ struct Run {
                                       &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
                                       c->d_counters.as<uint32_t>() + 6)));
             }
+            if (c->last_binned) {  // the recount hashes: a region sized for the first count's keys
+                rc = hashed_layout<NW, uint32_t>(c->libs[0].est + c->libs[0].prior_windows(c->o.k), 0);
+                if (rc) return rc;
+                c->last_binned = false;
+            }
             const uint64_t SPB = slots_per_bucket<NW, uint32_t>();
             const uint64_t n_slots = c->total_buckets * SPB;
             TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
@@ -2636,6 +2651,10 @@ struct Run {
             for (uint32_t l = 0; l < L(); ++l)
                 if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
                     return fail(KC_ENOMEM, "device hash table overflow (recount)");
+        }
+        if (!ec && c->last_binned && L() == 1 && c->libs[0].prior_windows(c->o.k)) {
+            rc = rehash<NW, Ev>();  // the priors find or claim their k-mers by hash
+            if (rc) return rc;
         }
         const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
         bool priors = false;

@@ -710,7 +710,8 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                                                    const uint64_t* __restrict__ act_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint64_t* __restrict__ nwin,
-                                                   const uint64_t* __restrict__ act_ev, uint64_t n_active, uint32_t k,
+                                                   const uint64_t* __restrict__ act_ev,
+                                                   const uint32_t* __restrict__ act_lib, uint64_t n_active, uint32_t k,
                                                    uint32_t bits, uint32_t* __restrict__ cnt,
                                                    uint32_t* __restrict__ wbin, const uint64_t* __restrict__ sk_off,
                                                    uint32_t* __restrict__ keys, uint64_t* __restrict__ recs) {
@@ -725,6 +726,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
         const uint64_t bit0 = act_off[a] * 8;
         const uint32_t nw = (uint32_t)nwin[active[a]];
         uint32_t* wb = wbin + act_ev[a];
+        const uint32_t lkey = act_lib[a] << bits;  // sort key: (library, bin)
         const uint32_t nm = nw + w - 1;  // m-mers of the read
         auto key_at = [&](uint32_t i) -> uint32_t {
             if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
@@ -778,7 +780,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                 uint64_t o = out + incl - nrec;
                 for (uint32_t t = 0; t < nrec; ++t, ++o) {
                     const uint32_t f0 = rs + t * SK_MAX_WINDOWS;
-                    keys[o] = pb;
+                    keys[o] = lkey | pb;
                     recs[o] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
                 }
                 out += __shfl(incl, 63);
@@ -793,7 +795,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
         if (EMIT) {
             for (uint32_t t = lane; t < nlast; t += 64) {
                 const uint32_t f0 = open + t * SK_MAX_WINDOWS;
-                keys[out + t] = carry_bin;
+                keys[out + t] = lkey | carry_bin;
                 recs[out + t] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
             }
         } else {
@@ -817,13 +819,15 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
     }
 }
 
-// One bin per workgroup pass (see above).  The bin's records are staged in LDS in chunks
+// One (library, bin) per workgroup pass (see above).  The bin's records are staged in LDS in chunks
 // with the prefix sum of their window counts, and the chunk's windows are dealt to the
 // threads in order: neighbouring lanes take neighbouring windows of one record (distinct
 // k-mers), rather than the same window of reads from the same locus, which would send
-// the whole wave's LDS atomics to one slot.  ctr[0] = entries written, ctr[1] = set when
-// a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts the library with
-// the table insert instead).
+// the whole wave's LDS atomics to one slot.  Library l's entries go to out[cap0[l] ..],
+// reserved per pass with one atomicAdd on lcnt[l]; a pass that would pass capl[l] writes
+// nothing and sets ctr[2] (lcnt still totals the library's distinct k-mers).  ctr[1] is set
+// when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
+// insert instead).
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
 
 template <int KW, typename Ev>
@@ -831,8 +835,12 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                                                      const uint64_t* __restrict__ bounds,
                                                      const uint8_t* __restrict__ data,
                                                      const uint64_t* __restrict__ act_ev,
-                                                     const uint64_t* __restrict__ act_off, uint32_t k, uint64_t ev0,
-                                                     uint32_t nbins, Slot<nw_of(KW), Ev>* __restrict__ out,
+                                                     const uint64_t* __restrict__ act_off, uint32_t k,
+                                                     const LibDesc* __restrict__ libs, uint32_t bits, uint32_t nbins,
+                                                     Slot<nw_of(KW), Ev>* __restrict__ out,
+                                                     const uint64_t* __restrict__ cap0,
+                                                     const uint64_t* __restrict__ capl,
+                                                     unsigned long long* __restrict__ lcnt,
                                                      unsigned long long* __restrict__ ctr) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
@@ -851,6 +859,8 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     for (uint32_t bin = blockIdx.x; bin < nbins; bin += gridDim.x) {
         const uint64_t rb = bounds[bin], re = bounds[bin + 1];
         if (rb == re) continue;
+        const uint32_t lib = bin >> bits;
+        const uint64_t ev0 = libs[lib].ev0;
         __syncthreads();
         if (threadIdx.x == 0) {
             s_np = 1;
@@ -956,29 +966,39 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 __syncthreads();
                 continue;
             }
-            if (threadIdx.x == 0) s_base = atomicAdd(&ctr[0], (unsigned long long)s_fill);
+            if (threadIdx.x == 0) {
+                s_base = atomicAdd(&lcnt[lib], (unsigned long long)s_fill);
+                if (s_base + s_fill > capl[lib]) {
+                    atomicOr(reinterpret_cast<unsigned int*>(&ctr[2]), 1u);
+                    s_base = ~0ull;
+                } else {
+                    s_base += cap0[lib];
+                }
+            }
             __syncthreads();
-            for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
-                if (s_k0[sl] == EMPTY_KEY) continue;
-                Slot<NW, Ev> e;
-                e.k[0] = s_k0[sl];
-                if (NW == 2) e.k[NW - 1] = s_k1[sl];
-                e.t0 = s_t0[sl];
-                e.cnt = s_cnt[sl];
-                out[s_base + atomicAdd(&s_idx, 1u)] = e;
+            if (s_base != ~0ull) {
+                for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
+                    if (s_k0[sl] == EMPTY_KEY) continue;
+                    Slot<NW, Ev> e;
+                    e.k[0] = s_k0[sl];
+                    if (NW == 2) e.k[NW - 1] = s_k1[sl];
+                    e.t0 = s_t0[sl];
+                    e.cnt = s_cnt[sl];
+                    out[s_base + atomicAdd(&s_idx, 1u)] = e;
+                }
             }
             __syncthreads();
         }
     }
 }
 
-// Packed entries (binned count) re-inserted with hash placement into library 0's region:
+// Packed entries (binned count) of library `lib` re-inserted with hash placement into its region:
 // each key is distinct, so its claimed slot has one writer for t0 and cnt.
 template <int NW, typename Ev>
 __global__ void k_rehash(const Slot<NW, Ev>* __restrict__ src, uint64_t n, Slot<NW, Ev>* __restrict__ tab,
-                         const LibDesc* __restrict__ libs, unsigned long long* __restrict__ overflow) {
+                         const LibDesc* __restrict__ libs, uint32_t lib, unsigned long long* __restrict__ overflow) {
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
-    const LibDesc& L = libs[0];
+    const LibDesc& L = libs[lib];
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const Slot<NW, Ev> e = src[i];
         if (e.k[0] == EMPTY_KEY) continue;
@@ -1665,7 +1685,7 @@ struct kc_ctx {
     bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
-    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin;
+    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
     uint64_t p_nq = 0;
@@ -1830,7 +1850,10 @@ struct Run {
         r0[L()] = n;
         KC_TRY(hipMemcpyAsync(c->d_lib_read0.p, r0.data(), r0.size() * 8, hipMemcpyHostToDevice, st));
         if (n == 0) {
-            for (auto& lib : c->libs) lib.info.n_occ = 0;
+            for (auto& lib : c->libs) {
+                lib.info.n_occ = 0;
+                lib.ev0 = 0;
+            }
             lib_ev0_.assign(L() + 1, 0);
             KC_TRY(hipStreamSynchronize(st));
             return KC_OK;
@@ -2090,51 +2113,69 @@ struct Run {
     }
 
     // Whether Stage 2 takes the minimizer-binned count: the default (insert_path 0) and
-    // insert_path 3, with one library, k <= 63, reads shorter than 2^24, and not for a
-    // graph capture or the eager run that plans one.  Hammer's recount and the priors hash
+    // insert_path 3, k <= 63, reads shorter than 2^24, and not for a graph capture or the
+    // eager run that plans one.  Hammer's recount and the priors hash
     // into the table, so Stage 3 lays it out for hashing first (hashed_layout, rehash).
     bool use_binned() const {
         const uint32_t ip = c->tune.insert_path;
-        if (L() != 1 || c->kw == 4 || c->capturing || c->graph_planning || (ip != 0 && ip != 3)) return false;
+        if (c->kw == 4 || c->capturing || c->graph_planning || (ip != 0 && ip != 3)) return false;
         if (c->n_active == 0) return false;
-        return c->libs[0].max_len < SK_MAX_LEN;
+        for (const HostLib& H : c->libs)
+            if (H.max_len >= SK_MAX_LEN) return false;
+        return true;
     }
 
-    // Library 0's region sized for hash placement of `keys` keys (as plan() sizes one),
-    // grown 2^grow times; the device descriptor follows.
+    // Every library's region sized for hash placement of its distinct keys (est) and
+    // prior windows, as plan() sizes one, grown 2^grow times; the descriptors follow.
     template <int NW, typename Ev>
-    int hashed_layout(uint64_t keys, uint32_t grow) {
+    int hashed_layout(uint32_t grow) {
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
         const double load = (tl > 0.05f && tl < 0.95f) ? tl : 0.5;
-        HostLib& H = c->libs[0];
-        const uint64_t cap = ((uint64_t)std::ceil((double)keys * 1.10 / load) + 64) << grow;
-        H.nbuckets = std::max<uint64_t>((cap + SPB - 1) / SPB, 1);
-        c->total_buckets = H.nbuckets;
-        KC_TRY(c->d_tab.ensure(H.nbuckets * 64));
-        KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets, 8,
-                              hipMemcpyHostToDevice, c->st));
-        KC_TRY(hipStreamSynchronize(c->st));
-        return KC_OK;
+        uint64_t b0 = 0;
+        for (HostLib& H : c->libs) {
+            const uint64_t keys = H.est + H.prior_windows(c->o.k);
+            const uint64_t cap = ((uint64_t)std::ceil((double)keys * 1.10 / load) + 64) << grow;
+            H.nbuckets = std::max<uint64_t>((cap + SPB - 1) / SPB, 1);
+            H.bucket0 = b0;
+            b0 += H.nbuckets;
+        }
+        c->total_buckets = b0;
+        KC_TRY(c->d_tab.ensure(b0 * 64));
+        return upload_desc();
     }
 
-    // The binned count into the table pool laid out by plan(.., dense = true); on success
-    // the library's region is cut to the entries written.  BIN_FALLBACK: a bin needed more
-    // passes than allowed, so the caller counts with the table insert instead.
+    // The binned count of every library into the table pool.  On success each library's
+    // region holds exactly its entries, packed, regions in library order.  BIN_FALLBACK: a
+    // bin needed more passes than allowed (or the keys need more than 31 bits), so the
+    // caller counts with the table insert instead.
     static constexpr int BIN_FALLBACK = 1;
     template <int KW, typename Ev>
     int binned_count() {
         constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
+        using S = Slot<NW, Ev>;
         hipStream_t st = c->st;
         const uint64_t na = c->n_active;
-        const uint32_t k = c->o.k;
+        const uint32_t k = c->o.k, nL = L();
         const uint32_t gw = grid_for(na, BLOCK / 64);
-        // bins for about BIN_TARGET distinct k-mers each: the last exact count at this k,
-        // else a quarter of the windows
-        HostLib& H0 = c->libs[0];
-        const uint32_t bits = bin_bits_for(H0.est_k == k && H0.est ? H0.est : c->n_occ / 4);
-        const uint32_t nbins = 1u << bits;
+        // bins for about BIN_TARGET distinct k-mers each in the largest library, and output
+        // room per library: from the last exact count at this k (plus 1/8), else a third of
+        // the library's windows; a run that overflows is counted again with the exact sizes
+        uint32_t lib_bits = 0;
+        while ((1u << lib_bits) < nL) ++lib_bits;
+        std::vector<uint64_t> cap(nL), cap0(nL + 1, 0), lcnt(nL, 0);
+        uint64_t est_max = 1;
+        for (uint32_t l = 0; l < nL; ++l) {
+            const HostLib& H = c->libs[l];
+            const bool exact = H.est_k == k && H.est;
+            const uint64_t e = exact ? H.est : H.info.n_occ / 3;
+            est_max = std::max(est_max, e);
+            cap[l] = (exact ? e + e / 8 : e) + 1024;
+        }
+        const uint32_t bits = bin_bits_for(est_max);
+        if (lib_bits + bits > 31) return BIN_FALLBACK;
+        const uint32_t nbins = nL << bits;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
         uint64_t nrec = 0;
         {
@@ -2144,8 +2185,8 @@ struct Run {
             KC_TRY(c->d_wbin.ensure(c->n_occ * 4 + 4));
             TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na, k, bits,
-                              c->d_skc.as<uint32_t>(), c->d_wbin.as<uint32_t>(), nullptr, nullptr, nullptr)));
+                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na, k,
+                              bits, c->d_skc.as<uint32_t>(), c->d_wbin.as<uint32_t>(), nullptr, nullptr, nullptr)));
             size_t tb0 = 0;
             KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                            (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
@@ -2163,47 +2204,75 @@ struct Run {
             KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(
                               c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na, k, bits, nullptr,
-                              c->d_wbin.as<uint32_t>(), c->d_sko.as<uint64_t>(), c->d_bk.as<uint32_t>(),
-                              c->d_br.as<uint64_t>())));
+                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na, k,
+                              bits, nullptr, c->d_wbin.as<uint32_t>(), c->d_sko.as<uint64_t>(),
+                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
         }
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
+        const uint32_t key_bits = lib_bits + bits;
         KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
-                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, bits,
+                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
                                          st));
         KC_TRY(c->d_temp.ensure(tb));
         tb = c->d_temp.bytes;
         TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
                                                               c->d_bk2.as<uint32_t>(), c->d_br.as<uint64_t>(),
-                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, bits, st)));
+                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
+                                                              st)));
         k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec, nbins,
                                                                      c->d_bbounds.as<uint64_t>());
-        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 2, 0ull);
         KC_TRY(hipGetLastError());
+        // count: one library writes straight into the table pool, several into staging
+        const bool direct = nL == 1;
         const uint32_t g = (uint32_t)std::min<uint64_t>(nbins, (uint64_t)c->n_cu * 8);
-        TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
-                                 c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
-                                 c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k, lib_ev0_[0], nbins,
-                                 c->d_tab.as<Slot<NW, Ev>>(), ctr)));
-        KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 16, hipMemcpyDeviceToHost, st));
-        KC_TRY(hipStreamSynchronize(st));
-        if (c->h_stage[1]) return BIN_FALLBACK;
-        const uint64_t d = c->h_stage[0];
-        if (d > c->libs[0].nbuckets * SPB) return fail(KC_EHIP, "binned count wrote past the table region");
-        // the region becomes the entries written, the rest of its last bucket empty
-        HostLib& H = c->libs[0];
-        H.est = d;  // exact: the first-setter mode decision and the next run's bins read it
-        H.est_k = k;
-        H.nbuckets = d ? (d + SPB - 1) / SPB : 0;
-        c->total_buckets = H.nbuckets;
-        if (H.nbuckets * SPB > d)
-            k_table_clear<NW, Ev><<<1, BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>() + d, H.nbuckets * SPB - d);
+        for (int attempt = 0;; ++attempt) {
+            for (uint32_t l = 0; l < nL; ++l) cap0[l + 1] = cap0[l] + cap[l];
+            if (direct) KC_TRY(c->d_tab.ensure((cap0[nL] + SPB - 1) / SPB * 64));
+            else KC_TRY(c->d_pack.ensure(std::max<uint64_t>(cap0[nL], 1) * sizeof(S)));
+            KC_TRY(c->d_bcap.ensure(nL * 16 + 16));
+            KC_TRY(c->d_bcnt.ensure(nL * 8 + 8));
+            KC_TRY(hipMemcpyAsync(c->d_bcap.p, cap0.data(), nL * 8, hipMemcpyHostToDevice, st));
+            KC_TRY(hipMemcpyAsync(c->d_bcap.as<uint64_t>() + nL, cap.data(), nL * 8, hipMemcpyHostToDevice, st));
+            k_fill<unsigned long long><<<grid_for(nL), BLOCK, 0, st>>>(c->d_bcnt.as<unsigned long long>(), nL, 0ull);
+            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 3, 0ull);
+            TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
+                                     c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
+                                     c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k,
+                                     c->d_libs.as<LibDesc>(), bits, nbins,
+                                     direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
+                                     c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr)));
+            KC_TRY(hipMemcpyAsync(lcnt.data(), c->d_bcnt.p, nL * 8, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 24, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipStreamSynchronize(st));
+            if (c->h_stage[1]) return BIN_FALLBACK;
+            if (!c->h_stage[2]) break;
+            if (attempt) return fail(KC_EHIP, "binned count passed its exact output sizes");
+            cap = lcnt;  // the counters totalled every library's distinct k-mers
+        }
+        // pack: library l's entries from bucket0_l, the rest of its last bucket empty
+        uint64_t b0 = 0;
+        for (uint32_t l = 0; l < nL; ++l) {
+            HostLib& H = c->libs[l];
+            H.est = lcnt[l];  // exact: the first-setter mode decision and the next run's bins read it
+            H.est_k = k;
+            H.bucket0 = b0;
+            H.nbuckets = (lcnt[l] + SPB - 1) / SPB;
+            b0 += H.nbuckets;
+        }
+        c->total_buckets = b0;
+        if (!direct) KC_TRY(c->d_tab.ensure(std::max<uint64_t>(b0, 1) * 64));
+        for (uint32_t l = 0; l < nL; ++l) {
+            const HostLib& H = c->libs[l];
+            S* dst = c->d_tab.as<S>() + H.bucket0 * SPB;
+            if (!direct && lcnt[l])
+                KC_TRY(hipMemcpyAsync(dst, c->d_pack.as<S>() + cap0[l], lcnt[l] * sizeof(S), hipMemcpyDeviceToDevice,
+                                      st));
+            if (H.nbuckets * SPB > lcnt[l])
+                k_table_clear<NW, Ev><<<1, BLOCK, 0, st>>>(dst + lcnt[l], H.nbuckets * SPB - lcnt[l]);
+        }
         KC_TRY(hipGetLastError());
-        KC_TRY(hipMemcpyAsync(reinterpret_cast<uint8_t*>(c->d_libs.p) + offsetof(LibDesc, nbuckets), &H.nbuckets, 8,
-                              hipMemcpyHostToDevice, st));
-        KC_TRY(hipStreamSynchronize(st));
-        return KC_OK;
+        return upload_desc();
     }
 
     // A packed table (left by the binned count) back to hash placement, for the passes
@@ -2215,21 +2284,26 @@ struct Run {
         if (!c->last_binned) return KC_OK;
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
-        HostLib& H = c->libs[0];
-        const uint64_t packed = H.nbuckets * SPB;
+        const uint64_t packed = c->total_buckets * SPB;
+        std::vector<uint64_t> pb0(L()), pn(L());  // packed regions, in slots
+        for (uint32_t l = 0; l < L(); ++l) {
+            pb0[l] = c->libs[l].bucket0 * SPB;
+            pn[l] = c->libs[l].nbuckets * SPB;
+        }
         KC_TRY(c->d_pack.ensure(std::max<uint64_t>(packed, 1) * sizeof(Slot<NW, Ev>)));
         if (packed)
             KC_TRY(hipMemcpyAsync(c->d_pack.p, c->d_tab.p, packed * sizeof(Slot<NW, Ev>), hipMemcpyDeviceToDevice, st));
         unsigned long long* flag = c->d_counters.as<unsigned long long>() + 14;
         for (uint32_t grow = 0;; ++grow) {
-            if (int rc = hashed_layout<NW, Ev>(H.est + H.prior_windows(c->o.k), grow)) return rc;
+            if (int rc = hashed_layout<NW, Ev>(grow)) return rc;
             k_fill<unsigned long long><<<1, 64, 0, st>>>(flag, 1, 0ull);
-            k_table_clear<NW, Ev><<<grid_for(H.nbuckets * SPB), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(),
-                                                                                H.nbuckets * SPB);
-            if (packed)
-                k_rehash<NW, Ev><<<grid_for(packed), BLOCK, 0, st>>>(c->d_pack.as<Slot<NW, Ev>>(), packed,
-                                                                     c->d_tab.as<Slot<NW, Ev>>(),
-                                                                     c->d_libs.as<LibDesc>(), flag);
+            const uint64_t slots = c->total_buckets * SPB;
+            k_table_clear<NW, Ev><<<grid_for(slots), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(), slots);
+            for (uint32_t l = 0; l < L(); ++l)
+                if (pn[l])
+                    k_rehash<NW, Ev><<<grid_for(pn[l]), BLOCK, 0, st>>>(c->d_pack.as<Slot<NW, Ev>>() + pb0[l], pn[l],
+                                                                        c->d_tab.as<Slot<NW, Ev>>(),
+                                                                        c->d_libs.as<LibDesc>(), l, flag);
             KC_TRY(hipGetLastError());
             KC_TRY(hipMemcpyAsync(c->h_stage, flag, 8, hipMemcpyDeviceToHost, st));
             KC_TRY(hipStreamSynchronize(st));
@@ -2241,7 +2315,7 @@ struct Run {
     }
 
     // Lay out the table pool, the Bloom pools and the first-setter groups.  dense: the
-    // binned count's layout, one slot per window (an upper bound on the distinct keys).
+    // binned count's layout, whose table regions binned_count() sets once it has counted.
     template <int NW, typename Ev>
     int plan(const std::vector<uint32_t>& grow, bool dense = false) {
         const float tl = c->tune.table_load > 0.0f ? c->tune.table_load : c->o.table_load;
@@ -2254,8 +2328,7 @@ struct Run {
             const uint64_t occ = lib.info.n_occ + pw;
             uint64_t cap = (uint64_t)std::ceil((double)(lib.est + pw) * 1.10 / load) + 64;
             cap = std::min<uint64_t>(cap, occ * 2 + 64) << grow[l];
-            if (dense) cap = occ;
-            lib.nbuckets = occ ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
+            lib.nbuckets = occ && !dense ? std::max<uint64_t>((cap + SPB - 1) / SPB, 1) : 0;
             lib.bucket0 = b0;
             b0 += lib.nbuckets;
             lib.num_words = std::max<uint64_t>((lib.bloom_bits + 63) / 64, 1);
@@ -2264,7 +2337,7 @@ struct Run {
         }
         c->total_buckets = b0;
         c->total_words = w0;
-        KC_TRY(c->d_tab.ensure(std::max<uint64_t>(b0, 1) * 64));
+        if (!dense) KC_TRY(c->d_tab.ensure(std::max<uint64_t>(b0, 1) * 64));
         KC_TRY(c->d_seen.ensure(w0 * 8));
         KC_TRY(c->d_solid.ensure(w0 * 8));
         // first-setter groups: consecutive libraries whose fs arrays fit the budget together
@@ -2290,7 +2363,12 @@ struct Run {
         KC_TRY(c->d_fs.ensure(maxg));
         if (c->d_fs.p != fs_before) c->fs_clean_bytes = 0;
         c->fs_bytes = maxg;
-        // descriptors
+        return upload_desc();
+    }
+
+    // The libraries' descriptors (table regions, Bloom pools, first-setter slices) to the
+    // device, from the host state.
+    int upload_desc() {
         std::vector<LibDesc> desc(L());
         std::vector<LibBloom> blooms(L());
         std::vector<uint64_t> lb0(L()), lw0(L());
@@ -2298,7 +2376,7 @@ struct Run {
         for (uint32_t l = 0; l < L(); ++l) {
             auto& lib = c->libs[l];
             LibDesc& d = desc[l];
-            d.ev0 = lib_ev0_[l];
+            d.ev0 = lib.ev0;  // kept per library: a Run made after the reads preparation has no lib_ev0_
             d.bucket0 = lib.bucket0;
             d.nbuckets = lib.nbuckets;
             d.word0 = lib.word0;
@@ -2629,8 +2707,8 @@ struct Run {
                                       &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
                                       c->d_counters.as<uint32_t>() + 6)));
             }
-            if (c->last_binned) {  // the recount hashes: a region sized for the first count's keys
-                rc = hashed_layout<NW, uint32_t>(c->libs[0].est + c->libs[0].prior_windows(c->o.k), 0);
+            if (c->last_binned) {  // the recount hashes: regions sized for the first count's keys
+                rc = hashed_layout<NW, uint32_t>(0);
                 if (rc) return rc;
                 c->last_binned = false;
             }
@@ -2652,7 +2730,9 @@ struct Run {
                 if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
                     return fail(KC_ENOMEM, "device hash table overflow (recount)");
         }
-        if (!ec && c->last_binned && L() == 1 && c->libs[0].prior_windows(c->o.k)) {
+        bool any_prior = false;
+        for (const HostLib& H : c->libs) any_prior = any_prior || H.prior_windows(c->o.k) > 0;
+        if (!ec && c->last_binned && any_prior) {
             rc = rehash<NW, Ev>();  // the priors find or claim their k-mers by hash
             if (rc) return rc;
         }
@@ -2991,7 +3071,7 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
-                      &ctx->d_pack, &ctx->d_wbin})
+                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);

@@ -115,7 +115,7 @@ def main():
                     help="run the autotuner on each engine before the timed region (the defaults are its usual choice)")
     ap.add_argument("--no-microbench", dest="microbench", action="store_false",
                     help="skip the random-atomic / copy microbenchmarks")
-    ap.add_argument("--eager", action="store_true", help="64lib: time eager runs instead of graph replays")
+    ap.add_argument("--graph", action="store_true", help="64lib: value from graph replays instead of eager runs")
     ap.add_argument("--ks", default="21,33,55")
     ap.add_argument("--bloom-bits", type=int, default=512 * 1024 * 1024)
     ap.add_argument("--cpu-sample-reads", type=int, default=300_000)
@@ -462,7 +462,8 @@ def bench_learned(args):
 def bench_many_libraries(args):
     """64 independent libraries (uneven read lengths 50-250 bp, 5 kb-6 Mb genomes x scale,
     20-200x, 0.3% errors) counted at k=33 in one context: Stage 2 + 3 of every library per
-    step, replayed as one HIP graph (or eager runs with --eager)."""
+    step.  value: eager runs (the binned count); with --graph, HIP graph replays (the table
+    insert, whose sizes a capture can fix) -- the other mode is timed beside it."""
     import numpy as np
     import torch
     from kc import AssemblerConfig, SpectrumEngine, synth
@@ -472,26 +473,35 @@ def bench_many_libraries(args):
     reads = sum(len(st) for st, _ in libs)
     bases = sum(int(st.lengths.astype(np.int64).sum()) for st, _ in libs)
     log(f"generated {len(libs)} libraries, {reads} reads, {bases} bases in {time.time() - t0:.1f}s")
-    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), timing=args.eager)
+    eager = not args.graph
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), timing=eager)
     for st, _ in libs:
         eng.add_library(st)
     tune = eng.autotune(reps=1) if args.autotune else eng.tune()
     log(f"tune: {tune}")
-    first = eng.run() if args.eager else eng.graph_run()  # plans (and captures)
-    step = eng.run if args.eager else eng.graph_run
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t_start) * 1e3 / args.steps
-    n_occ = sum(eng.info(i)["n_occ"] for i in range(len(libs)))
-    n_counts = sum(eng.info(i)["n_counts"] for i in range(len(libs)))
-    if args.eager:
+
+    def timed(step):
+        first = step()  # an eager run sizes its bins; a graph call plans and captures
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t_start) * 1e3 / args.steps, first
+
+    ms, first = timed(eng.run if eager else eng.graph_run)
+    snap = [eng.counts(i) for i in range(len(libs))]
+    if eager:
         for name, (kms, n) in sorted(eng.kernel_times().items(), key=lambda x: -x[1][0])[:12]:
             log(f"  {name:18s} {kms / max(n, 1):9.3f} ms/launch x{n}")
+    other_ms, _ = timed(eng.graph_run if eager else eng.run)
+    same = all(all(np.array_equal(a, b) for a, b in zip(x, eng.counts(i))) for i, x in enumerate(snap))
+    log(f"{'eager' if eager else 'graph'} {ms:.3f} ms, {'graph' if eager else 'eager'} {other_ms:.3f} ms, "
+        f"same tables {same}")
+    n_occ = sum(eng.info(i)["n_occ"] for i in range(len(libs)))
+    n_counts = sum(eng.info(i)["n_counts"] for i in range(len(libs)))
     eng.close()
     out = {
         "metric": "k-mers/s counted bit-exact (Bloom tiers + sorted count table + cutoff + solid set)",
@@ -500,7 +510,10 @@ def bench_many_libraries(args):
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded)",
         "config": {"workload": f"64 libraries, genomes 5 kb-6 Mb x {args.scale}, 20-200x, reads 50-250 bp "
                                f"(lognormal, median 150), 0.3% errors", "k": k, "reads": reads, "bases": bases,
-                   "n_occ": n_occ, "n_counts": n_counts, "mode": "eager" if args.eager else "hip graph replay",
+                   "n_occ": n_occ, "n_counts": n_counts,
+                   "mode": "eager (binned count)" if eager else "hip graph replay (table insert)",
+                   ("graph_replay_ms_per_step" if eager else "eager_ms_per_step"): round(other_ms, 3),
+                   "both_modes_same_tables": same,
                    "tune": tune, "first_call_ms": round(first["ms_stage2"] + first["ms_stage3"], 3)},
     }
     print(json.dumps(out), flush=True)

@@ -227,6 +227,19 @@ __device__ __forceinline__ void kv_positions(const Kv& v, uint32_t k, const Modu
     bloom_positions(w[0], w[1], w[2], w[3], M, pos);
 }
 
+// The same test reading the next bit only while every earlier one is set: a sparse
+// solid tier rejects most k-mers on their first bit, so most lanes issue one load, not
+// three (scans over many distinct k-mers are bound by these random reads).
+template <int KW>
+__device__ __forceinline__ bool bloom_is_solid_lazy(const Kv& v, uint32_t k, const uint64_t* __restrict__ solid,
+                                                    const Modulus& M) {
+    uint64_t p[3];
+    kv_positions<KW>(v, k, M, p);
+    if (!((solid[p[0] >> 6] >> (p[0] & 63)) & 1ull)) return false;
+    if (!((solid[p[1] >> 6] >> (p[1] & 63)) & 1ull)) return false;
+    return ((solid[p[2] >> 6] >> (p[2] & 63)) & 1ull) != 0;
+}
+
 // bloom.rs:111-126 is_solid_kmer256 for a k-mer value.
 template <int KW>
 __device__ __forceinline__ bool bloom_is_solid(const Kv& v, uint32_t k, const uint64_t* __restrict__ solid,

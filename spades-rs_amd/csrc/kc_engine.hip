@@ -1153,10 +1153,12 @@ __global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restric
             uint64_t p[3];
             kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
             bool promote = s.cnt >= 2;
-            if (!promote) {
+            if (!promote) {  // each later first setter is read only while the earlier ones are others'
                 const Ev* fs = reinterpret_cast<const Ev*>(L.fs);
                 const Ev base = (Ev)(s.t0 * 4);
-                promote = fs[p[0]] != base && fs[p[1]] != (Ev)(base + 1) && fs[p[2]] != (Ev)(base + 2);
+                promote = fs[p[0]] != base;
+                if (promote) promote = fs[p[1]] != (Ev)(base + 1);
+                if (promote) promote = fs[p[2]] != (Ev)(base + 2);
             }
             if (promote) {
 #pragma unroll
@@ -1199,7 +1201,7 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             if (CHECK_SOLID && base) {
                 // a slot claimed only by a prior holds no occurrence
                 const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
-                base = sl.cnt > 0 && bloom_is_solid<kw_of_nw(NW)>(unpack_key<NW>(sl.k), L.k, L.solid, L.M);
+                base = sl.cnt > 0 && bloom_is_solid_lazy<kw_of_nw(NW)>(unpack_key<NW>(sl.k), L.k, L.solid, L.M);
             }
             const bool take = base || (occ && pflag && pflag[i]);
             bits |= (uint32_t)take << j;

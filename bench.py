@@ -123,7 +123,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-load", type=float, default=0.0)
     ap.add_argument("--insert-path", type=int, default=0,
-                    help="Stage 2 count: 0 table atomics, 2 partitioned, 3 minimizer-binned")
+                    help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics, 2 partitioned")
+    ap.add_argument("--setter-path", type=int, default=0,
+                    help="Bloom first setters: 0 sorted by partition with minima in LDS where it applies, 1 atomics")
     ap.add_argument("--microbench-sweep", action="store_true",
                     help="only print random-atomic / random-read rates against array size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -171,8 +173,8 @@ def main():
     for k in ks:
         eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), table_load=args.table_load,
                              timing=True)
-        if args.insert_path:
-            eng.set_tune(insert_path=args.insert_path)
+        if args.insert_path or args.setter_path:
+            eng.set_tune(insert_path=args.insert_path, setter_path=args.setter_path)
         eng.upload(store)
         if args.autotune:
             log(f"k={k} tune: {eng.autotune(reps=1)}")

@@ -278,6 +278,13 @@ typedef struct kc_tune {
                                     atomics per window.  2 = partitioned (windows routed by
                                     hash in two levels and counted in LDS), one library,
                                     k <= 63 */
+    uint32_t setter_path;        /* Bloom first setters.  0 (default) = sorted: three
+                                    (entry, setter) items per distinct k-mer radix-sorted by
+                                    16384-entry partition, each partition's minima taken in
+                                    LDS and written in order; taken in the dense mode with
+                                    32-bit events and first-setter groups under 2^32 entries,
+                                    and otherwise the atomics below.  1 = three device-scope
+                                    atomicMin per distinct k-mer */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;
 

@@ -1116,6 +1116,112 @@ __global__ void __launch_bounds__(BLOCK) k_seen_from_fs(uint64_t w_begin, uint64
     }
 }
 
+// Sorted first setters (dense mode, 32-bit events, a group array of under 2^32 entries).
+// Item of (distinct k-mer x, hash j) = e << 32 | (4*t0(x) + j), with e the bit's entry in the
+// group's first-setter array; slot i of the scanned range writes items 3i .. 3i+2, and an
+// empty slot writes ~0, whose partition key sorts past every partition.  Sorted by partition
+// (e >> FS_PART_LOG2), each partition's minima are taken in LDS by one workgroup and its
+// first setters and seen words written in order, in place of three random device-scope
+// atomicMin per k-mer.  The minimum does not depend on the order of the items, so the
+// result is the one of the atomics.
+constexpr uint32_t FS_PART_LOG2 = 14;  // entries per partition: 64 KiB of LDS
+constexpr uint32_t FS_PART = 1u << FS_PART_LOG2;
+constexpr uint32_t FS_KEY_SHIFT = 32 + FS_PART_LOG2;
+constexpr uint32_t FS_BLOCK = 1024;
+
+template <int NW>
+__global__ void __launch_bounds__(BLOCK) k_fs_items(const Slot<NW, uint32_t>* __restrict__ tab, uint64_t s_begin,
+                                                    uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                    uint32_t n_libs, const LibDesc* __restrict__ libs,
+                                                    LibOut* __restrict__ out, const uint8_t* __restrict__ fs_base,
+                                                    uint64_t* __restrict__ items) {
+    __shared__ unsigned long long s_acc[LIB_LOCAL];
+    LibAccum acc{s_acc};
+    acc.reset();
+    __syncthreads();
+    scan_slots<NW, uint32_t>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t lib0) {
+            const Slot<NW, uint32_t> s = tab[i];
+            uint64_t* o = items + 3 * (i - s_begin);
+            if (s.k[0] == EMPTY_KEY) {
+                o[0] = o[1] = o[2] = ~0ull;
+                return;
+            }
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
+            const uint64_t e0 = (uint64_t)(reinterpret_cast<const uint8_t*>(L.fs) - fs_base) / sizeof(uint32_t);
+            const uint32_t base = s.t0 * 4u;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) o[j] = (e0 + p[j]) << 32 | (uint32_t)(base + j);
+            acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
+        },
+        [&](uint32_t lib0) {
+            __syncthreads();
+            acc.flush(lib0, n_libs, &out[0].n_distinct, LIBOUT_STRIDE);
+            __syncthreads();
+            acc.reset();
+            __syncthreads();
+        });
+}
+
+// bounds[b] = the first sorted item whose partition is >= b, for b = 0 .. nparts.
+__global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint32_t nparts,
+                            uint64_t* __restrict__ bounds) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nparts; b += gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if ((items[mid] >> FS_KEY_SHIFT) < b) lo = mid + 1;
+            else hi = mid;
+        }
+        bounds[b] = lo;
+    }
+}
+
+// One partition per workgroup pass: the minimum setter of each of its FS_PART entries in
+// LDS, then, one wave per 64 entries, the entries to fs, their seen word (entry != ~0) by
+// ballot, and a cleared solid word.  fs_n is a multiple of 64.
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict__ items,
+                                                     const uint64_t* __restrict__ bounds, uint32_t nparts,
+                                                     uint32_t* __restrict__ fs, uint64_t fs_n,
+                                                     uint64_t* __restrict__ seen, uint64_t* __restrict__ solid) {
+    __shared__ uint32_t m[FS_PART];
+    constexpr uint32_t U = 4;  // items in flight per thread
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
+        for (uint32_t e = threadIdx.x; e < FS_PART; e += FS_BLOCK) m[e] = ~0u;
+        __syncthreads();
+        const uint64_t b = bounds[part], end = bounds[part + 1];
+        for (uint64_t i0 = b; i0 < end; i0 += (uint64_t)FS_BLOCK * U) {
+            uint64_t it[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t i = i0 + u * FS_BLOCK + threadIdx.x;
+                it[u] = i < end ? items[i] : ~0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (it[u] != ~0ull) atomicMin(&m[(uint32_t)(it[u] >> 32) & (FS_PART - 1)], (uint32_t)it[u]);
+        }
+        __syncthreads();
+        const uint64_t e0 = (uint64_t)part * FS_PART;
+        const uint32_t nw = (uint32_t)(min<uint64_t>(FS_PART, fs_n - e0) / 64);
+        for (uint32_t w = wave; w < nw; w += FS_BLOCK / 64) {
+            const uint32_t v = m[w * 64 + lane];
+            fs[e0 + w * 64 + lane] = v;
+            const uint64_t bits = __ballot(v != ~0u);
+            if (lane == 0) {
+                seen[e0 / 64 + w] = bits;
+                solid[e0 / 64 + w] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Sparse mode: puts back the "no setter" value at every first-setter entry the table's
 // k-mers touched, so the array is all ~0 again for the next run.
 template <int NW, typename Ev>
@@ -1610,12 +1716,13 @@ __global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restric
 enum KernelId {
     KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_HLL, KID_CLEAR, KID_INSERT, KID_FS_CLEAR, KID_FIRST_SETTER,
     KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_PRIOR, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS,
-    KID_PART1, KID_PART2, KID_PART3, KID_SK, KID_BIN_SORT, KID_BIN_COUNT, KID_COUNT
+    KID_PART1, KID_PART2, KID_PART3, KID_SK, KID_BIN_SORT, KID_BIN_COUNT, KID_FS_ITEMS, KID_FS_SORT, KID_FS_MIN, KID_COUNT
 };
 const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
     "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_prior", "k_compact", "radix_sort",
-    "k_count_hist", "k_solid_flags", "k_part1", "k_part2", "k_part3", "k_superkmers", "bin_sort", "k_bin_count"};
+    "k_count_hist", "k_solid_flags", "k_part1", "k_part2", "k_part3", "k_superkmers", "bin_sort", "k_bin_count",
+    "k_fs_items", "fs_sort", "k_fs_min"};
 
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
 struct HostLib {
@@ -1688,6 +1795,7 @@ struct kc_ctx {
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;
+    DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
     uint64_t p_nq = 0;
@@ -1697,7 +1805,7 @@ struct kc_ctx {
     DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
     uint64_t n_entries = 0;
     SortLayout lay{};
-    kc_tune tune{1, 0, 0.0f, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
+    kc_tune tune{1, 0, 0.0f, 0, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
     uint32_t n_cu = 0;
     uint32_t kw = 1;  // value words of the current k (1, 2 or 4)
     bool ev64 = false, built = false, counted = false;
@@ -2479,6 +2587,14 @@ struct Run {
         return read_out();
     }
 
+    // Whether a dense group's first setters are taken by sorting (k_fs_items, fs_sort,
+    // k_fs_min) rather than by device atomics: 32-bit events, item entries under 2^32 - FS_PART
+    // (so no partition key is all ones), and the setting allows it.
+    template <typename Ev>
+    bool sorted_setters(uint64_t fs_n) const {
+        return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32);
+    }
+
     // First setters -> seen tier -> solid tier, one group of libraries at a time.
     // Dense mode fills each group's first-setter array and derives the seen tier from it.
     // Sparse mode (every group's k-mers touch under 1/32 of its entries, by the distinct
@@ -2534,6 +2650,41 @@ struct Run {
                                             c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
                                             L(), c->d_libs.as<LibDesc>())));
                 }
+            } else if (sorted_setters<Ev>(fs_n)) {
+                const uint64_t n_items = 3 * (s_end - s_begin);
+                const uint32_t nparts = (uint32_t)((fs_n + FS_PART - 1) / FS_PART);
+                uint32_t key_bits = 1;  // partition keys 0 .. nparts - 1; empty slots' items all ones
+                while ((1ull << key_bits) <= nparts) ++key_bits;
+                KC_TRY(c->d_fsi.ensure(std::max<uint64_t>(n_items, 1) * 8));
+                KC_TRY(c->d_fsi2.ensure(std::max<uint64_t>(n_items, 1) * 8));
+                KC_TRY(c->d_fsb.ensure(((uint64_t)nparts + 1) * 8));
+                if (n_items) {
+                    TIMED(KID_FS_ITEMS, (k_fs_items<NW><<<gs, BLOCK, 0, st>>>(
+                                                c->d_tab.as<Slot<NW, uint32_t>>(), s_begin, s_end,
+                                                c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                                c->d_out.as<LibOut>(), c->d_fs.as<uint8_t>(),
+                                                c->d_fsi.as<uint64_t>())));
+                    size_t tb = 0;
+                    KC_TRY(rocprim::radix_sort_keys(nullptr, tb, c->d_fsi.as<uint64_t>(), c->d_fsi2.as<uint64_t>(),
+                                                    (size_t)n_items, FS_KEY_SHIFT, FS_KEY_SHIFT + key_bits, st));
+                    KC_TRY(c->d_temp.ensure(tb));
+                    tb = c->d_temp.bytes;
+                    TIMED(KID_FS_SORT,
+                          KC_TRY(rocprim::radix_sort_keys(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
+                                                          c->d_fsi2.as<uint64_t>(), (size_t)n_items, FS_KEY_SHIFT,
+                                                          FS_KEY_SHIFT + key_bits, st)));
+                }
+                k_fs_bounds<<<grid_for((uint64_t)nparts + 1), BLOCK, 0, st>>>(
+                    c->d_fsi2.as<uint64_t>(), n_items, nparts, c->d_fsb.as<uint64_t>());
+                KC_TRY(hipGetLastError());
+                TIMED(KID_FS_MIN, (k_fs_min<<<nparts, FS_BLOCK, 0, st>>>(
+                                      c->d_fsi2.as<uint64_t>(), c->d_fsb.as<uint64_t>(), nparts,
+                                      c->d_fs.as<uint32_t>(), fs_n, c->d_seen.as<uint64_t>() + w_begin,
+                                      c->d_solid.as<uint64_t>() + w_begin)));
+                if (s_end > s_begin)
+                    TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
+                                           c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
+                                           L(), c->d_libs.as<LibDesc>())));
             } else {
                 TIMED(KID_FS_CLEAR,
                       (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
@@ -3072,7 +3223,8 @@ void kc_destroy(kc_ctx* ctx) {
                        &ctx->d_base1, &ctx->d_lim1,   &ctx->d_base2, &ctx->d_lim2};
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
-    for (DevBuf* b : {&ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2, &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
+    for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
+                      &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
                       &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -3699,6 +3851,7 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->table_load != 0.0f && !(tune->table_load > 0.05f && tune->table_load < 0.95f))
         return ctx->fail(KC_EINVAL, "table_load must be 0 or in (0.05, 0.95)");
     if (tune->insert_path > 3) return ctx->fail(KC_EINVAL, "insert_path must be 0, 1, 2 or 3");
+    if (tune->setter_path > 1) return ctx->fail(KC_EINVAL, "setter_path must be 0 or 1");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
     return KC_OK;

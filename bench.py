@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--table-load", type=float, default=0.0)
     ap.add_argument("--insert-path", type=int, default=0,
                     help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics, 2 partitioned")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="workgroups per CU of the table insert / bin count (0: default shape)")
     ap.add_argument("--setter-path", type=int, default=0,
                     help="Bloom first setters: 0 sorted by partition with minima in LDS where it applies, 1 atomics")
     ap.add_argument("--microbench-sweep", action="store_true",
@@ -173,8 +175,8 @@ def main():
     for k in ks:
         eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), table_load=args.table_load,
                              timing=True)
-        if args.insert_path or args.setter_path:
-            eng.set_tune(insert_path=args.insert_path, setter_path=args.setter_path)
+        if args.insert_path or args.setter_path or args.blocks_per_cu:
+            eng.set_tune(insert_path=args.insert_path, setter_path=args.setter_path, blocks_per_cu=args.blocks_per_cu)
         eng.upload(store)
         if args.autotune:
             log(f"k={k} tune: {eng.autotune(reps=1)}")

@@ -268,7 +268,9 @@ void kc_unitigs_destroy(kc_unitigs* u);
 typedef struct kc_tune {
     uint32_t windows_per_thread; /* 1, 2 or 4: table probes each thread keeps in flight */
     uint32_t blocks_per_cu;      /* 0: one workgroup per 1024-window tile; else CUs x this
-                                    workgroups, each looping over tiles */
+                                    workgroups, each looping over tiles.  The binned count:
+                                    0 = as many workgroups as are resident at once, else CUs x
+                                    this, each looping over bins */
     float table_load;            /* device hash-table target load factor (0: kc_opts.table_load) */
     uint32_t insert_path;        /* Stage 2 count table.  0 (default) or 3 = minimizer-binned:
                                     reads cut into super-k-mers, radix-sorted by bin, each bin

@@ -823,7 +823,10 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // with the prefix sum of their window counts, and the chunk's windows are dealt to the
 // threads in order: neighbouring lanes take neighbouring windows of one record (distinct
 // k-mers), rather than the same window of reads from the same locus, which would send
-// the whole wave's LDS atomics to one slot.  Library l's entries go to out[cap0[l] ..],
+// the whole wave's LDS atomics to one slot.  (Runs of 4-8 windows per thread, the first
+// extracted and the others rolled in base by base, were measured slower: a wave then
+// spans 8x more records, and copies of one super-k-mer collide on a slot.)  Library l's
+// entries go to out[cap0[l] ..],
 // reserved per pass with one atomicAdd on lcnt[l]; a pass that would pass capl[l] writes
 // nothing and sets ctr[2] (lcnt still totals the library's distinct k-mers).  ctr[1] is set
 // when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
@@ -852,10 +855,24 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     __shared__ uint64_t s_ev[BIN3_RCHUNK];   // event of its first window (library-relative),
     __shared__ uint32_t s_j0[BIN3_RCHUNK];   // its first window,
     __shared__ uint32_t s_woff[BIN3_RCHUNK + 1];  // and its first window in the chunk
-    __shared__ uint32_t s_scan[BLOCK];
-    __shared__ uint32_t s_fill, s_over, s_idx, s_np;
+    __shared__ uint16_t s_list[BIN3_FILL];        // the pass's claimed slots, in claim order
+    __shared__ uint32_t s_wsum[BLOCK / 64];
+    __shared__ uint32_t s_fill, s_over, s_np;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
+    static_assert(BIN3_CAP <= 65536, "slot numbers are 16-bit");
+    // The table starts empty and every pass leaves it empty: its claimed slots are listed
+    // and reset as they are written out; a pass that overflows (the list is then
+    // incomplete) clears the whole table.
+    auto clear_table = [&]() {
+        for (uint32_t i = threadIdx.x; i < BIN3_CAP; i += blockDim.x) {
+            s_k0[i] = EMPTY_KEY;
+            if (NW == 2) s_k1[i] = EMPTY_KEY;
+            s_cnt[i] = 0;
+            s_t0[i] = (Ev)~(Ev)0;
+        }
+    };
+    clear_table();
     for (uint32_t bin = blockIdx.x; bin < nbins; bin += gridDim.x) {
         const uint64_t rb = bounds[bin], re = bounds[bin + 1];
         if (rb == re) continue;
@@ -871,24 +888,17 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
         while (s_np) {
             const uint32_t parts = s_pass[2 * (s_np - 1)], part = s_pass[2 * (s_np - 1) + 1];
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < BIN3_CAP; i += blockDim.x) {
-                s_k0[i] = EMPTY_KEY;
-                if (NW == 2) s_k1[i] = EMPTY_KEY;
-                s_cnt[i] = 0;
-                s_t0[i] = (Ev)~(Ev)0;
-            }
             if (threadIdx.x == 0) {
                 s_fill = 0;
                 s_over = 0;
-                s_idx = 0;
                 --s_np;
             }
             for (uint64_t c0 = rb; c0 < re; c0 += BIN3_RCHUNK) {
                 const uint32_t nr = (uint32_t)min<uint64_t>(BIN3_RCHUNK, re - c0);
                 __syncthreads();  // the previous chunk's staging reads are done
                 if (s_over) break;
-                {  // stage one record per thread; block scan of the window counts
-                    const uint32_t i = threadIdx.x;
+                {  // stage one record per thread; scan of the window counts (waves, then across)
+                    const uint32_t i = threadIdx.x, lane = i & 63, wave = i >> 6;
                     uint32_t n0 = 0;
                     if (i < nr) {
                         const uint64_t rec = recs[c0 + i];
@@ -898,16 +908,23 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         s_ev[i] = act_ev[a] + first - ev0;
                         s_j0[i] = first;
                     }
-                    s_scan[i] = n0;
-                    __syncthreads();
-                    for (uint32_t d = 1; d < BLOCK; d <<= 1) {
-                        const uint32_t add = i >= d ? s_scan[i - d] : 0u;
-                        __syncthreads();
-                        s_scan[i] += add;
-                        __syncthreads();
+                    uint32_t inc = n0;
+#pragma unroll
+                    for (uint32_t d = 1; d < 64; d <<= 1) {
+                        const uint32_t v = (uint32_t)__shfl_up((int)inc, d, 64);
+                        if (lane >= d) inc += v;
                     }
-                    if (i < nr) s_woff[i] = s_scan[i] - n0;
-                    if (i == BLOCK - 1) s_woff[nr] = s_scan[BLOCK - 1];
+                    if (lane == 63) s_wsum[wave] = inc;
+                    __syncthreads();
+                    uint32_t before = 0, total = 0;
+#pragma unroll
+                    for (uint32_t w = 0; w < BLOCK / 64; ++w) {
+                        const uint32_t t = s_wsum[w];
+                        before += w < wave ? t : 0u;
+                        total += t;
+                    }
+                    if (i < nr) s_woff[i] = before + inc - n0;
+                    if (i == 0) s_woff[nr] = total;
                 }
                 __syncthreads();
                 const uint32_t nwin = s_woff[nr];
@@ -930,7 +947,11 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
                         bool claimed = false;
                         bool hit = lds_claim_word(&s_k0[slot], key[0], claimed);
-                        if (claimed && atomicAdd(&s_fill, 1u) + 1 > BIN3_FILL) atomicOr(&s_over, 1u);
+                        if (claimed) {
+                            const uint32_t f = atomicAdd(&s_fill, 1u);
+                            if (f < BIN3_FILL) s_list[f] = (uint16_t)slot;
+                            else atomicOr(&s_over, 1u);
+                        }
                         if (hit && NW == 2) {
                             bool c2 = false;
                             hit = lds_claim_word(&s_k1[slot], key[NW - 1], c2);
@@ -950,6 +971,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
             }
             __syncthreads();
             if (s_over) {
+                clear_table();
                 // split this pass in two by the next key-hash bit
                 if (threadIdx.x == 0) {
                     if (2 * parts > BIN3_MAX_SPLIT || s_np + 2 > 32) {
@@ -976,16 +998,21 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 }
             }
             __syncthreads();
-            if (s_base != ~0ull) {
-                for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
-                    if (s_k0[sl] == EMPTY_KEY) continue;
+            const unsigned long long base = s_base;
+            for (uint32_t f = threadIdx.x; f < s_fill; f += blockDim.x) {
+                const uint32_t sl = s_list[f];
+                if (base != ~0ull) {
                     Slot<NW, Ev> e;
                     e.k[0] = s_k0[sl];
                     if (NW == 2) e.k[NW - 1] = s_k1[sl];
                     e.t0 = s_t0[sl];
                     e.cnt = s_cnt[sl];
-                    out[s_base + atomicAdd(&s_idx, 1u)] = e;
+                    out[base + f] = e;
                 }
+                s_k0[sl] = EMPTY_KEY;
+                if (NW == 2) s_k1[sl] = EMPTY_KEY;
+                s_cnt[sl] = 0;
+                s_t0[sl] = (Ev)~(Ev)0;
             }
             __syncthreads();
         }
@@ -2335,7 +2362,10 @@ struct Run {
         KC_TRY(hipGetLastError());
         // count: one library writes straight into the table pool, several into staging
         const bool direct = nL == 1;
-        const uint32_t g = (uint32_t)std::min<uint64_t>(nbins, (uint64_t)c->n_cu * 8);
+        // workgroups: one per bin, so that the dispatcher balances the uneven bins (measured
+        // faster than any CU-multiple grid looping over bins); kc_tune.blocks_per_cu overrides
+        const uint64_t g_max = c->tune.blocks_per_cu ? (uint64_t)c->n_cu * c->tune.blocks_per_cu : nbins;
+        const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nbins, g_max));
         for (int attempt = 0;; ++attempt) {
             for (uint32_t l = 0; l < nL; ++l) cap0[l + 1] = cap0[l] + cap[l];
             if (direct) KC_TRY(c->d_tab.ensure((cap0[nL] + SPB - 1) / SPB * 64));

@@ -10,7 +10,7 @@ namespace kc {
 
 // Bins: 2^bits, bits chosen per run (bin_bits_for) for about BIN_TARGET distinct k-mers a bin.
 constexpr uint32_t BIN_BITS_MIN = 14, BIN_BITS_MAX = 20;
-constexpr uint32_t BIN_TARGET = 160;
+constexpr uint32_t BIN_TARGET = 320;
 constexpr uint32_t BIN3_CAP = 1024;  // LDS slots per pass
 constexpr uint32_t BIN3_FILL = BIN3_CAP * 3 / 4;
 constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving up

@@ -740,14 +740,18 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
         };
         uint32_t open = 0, carry_bin = 0, total = 0;
         uint64_t out = EMIT ? sk_off[a] : 0;
+        uint32_t K = EMIT ? 0u : key_at(lane);  // keys of positions c0 + lane
         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
             const bool valid = c0 + lane < nw;
             uint32_t bin;
             if (EMIT) {
                 bin = valid ? wb[c0 + lane] : 0u;
             } else {
-                // A = positions c0 + lane, B = positions c0 + 64 + lane (up to c0 + 63 + w - 1)
-                uint32_t A = key_at(c0 + lane), B = lane + 1 < w ? key_at(c0 + 64 + lane) : 0xFFFFFFFFu;
+                // A = positions c0 + lane, B = positions c0 + 64 + lane (the windows read B
+                // up to lane w - 2; B is the next chunk's A)
+                const uint32_t Kn = key_at(c0 + 64 + lane);
+                uint32_t A = K, B = Kn;
+                K = Kn;
                 for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
                     const uint32_t src = (lane + d) & 63;
                     const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
@@ -1053,16 +1057,26 @@ __device__ __forceinline__ void scan_slots(const Slot<NW, Ev>* __restrict__ tab,
     const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
     for (uint64_t c0 = s_begin + blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
         const uint32_t lib0 = find_region(lib_b0, n_libs, c0 / SPB);
-        uint32_t lib = lib0;
-        uint64_t lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
-        for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
-            const uint64_t i = c0 + (uint64_t)j * BLOCK + threadIdx.x;
-            if (i >= s_end) break;
-            if (i >= lib_end) {
-                lib = find_region(lib_b0, n_libs, i / SPB);
-                lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+        uint64_t lib_end = (lib0 + 1 < n_libs) ? lib_b0[lib0 + 1] * SPB : ~0ull;
+        if (lib_end >= min(c0 + chunk, s_end)) {
+            // the chunk lies in one library: f sees a workgroup-uniform library, so its
+            // descriptor loads are scalar and leave the per-slot dependency chain
+            for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+                const uint64_t i = c0 + (uint64_t)j * BLOCK + threadIdx.x;
+                if (i >= s_end) break;
+                f(i, lib0, lib0);
             }
-            f(i, lib, lib0);
+        } else {
+            uint32_t lib = lib0;
+            for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
+                const uint64_t i = c0 + (uint64_t)j * BLOCK + threadIdx.x;
+                if (i >= s_end) break;
+                if (i >= lib_end) {
+                    lib = find_region(lib_b0, n_libs, i / SPB);
+                    lib_end = (lib + 1 < n_libs) ? lib_b0[lib + 1] * SPB : ~0ull;
+                }
+                f(i, lib, lib0);
+            }
         }
         chunk_end(lib0);
     }
@@ -1318,13 +1332,16 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
                                                    const uint32_t* __restrict__ psum,
                                                    const uint8_t* __restrict__ pflag,
                                                    WideEntry* __restrict__ out_wide) {
-    __shared__ uint32_t s_scan[BLOCK];
+    __shared__ uint32_t s_scan[BLOCK / 64];
     __shared__ unsigned long long s_base;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
     const uint64_t chunk = (uint64_t)BLOCK * SCAN_ITEMS;
     const uint32_t tid = threadIdx.x;
     for (uint64_t c0 = blockIdx.x * chunk; c0 < s_end; c0 += (uint64_t)gridDim.x * chunk) {
         uint32_t bits = 0, counted = 0, cnt = 0;  // bits: in the table; counted: occurrences count
+        // a chunk inside one library reads its descriptor once (scalar loads)
+        const uint32_t lib0 = find_region(lib_b0, n_libs, c0 / SPB);
+        const bool single = lib0 + 1 >= n_libs || lib_b0[lib0 + 1] * SPB >= min(c0 + chunk, s_end);
         for (uint32_t j = 0; j < SCAN_ITEMS; ++j) {
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
             if (i >= s_end) break;
@@ -1333,32 +1350,45 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             bool base = occ;
             if (CHECK_SOLID && base) {
                 // a slot claimed only by a prior holds no occurrence
-                const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
-                base = sl.cnt > 0 && bloom_is_solid_lazy<kw_of_nw(NW)>(unpack_key<NW>(sl.k), L.k, L.solid, L.M);
+                const Kv v = unpack_key<NW>(sl.k);
+                if (single) {
+                    const LibDesc& L = libs[lib0];
+                    base = sl.cnt > 0 && bloom_is_solid_lazy<kw_of_nw(NW)>(v, L.k, L.solid, L.M);
+                } else {
+                    const LibDesc& L = libs[find_region(lib_b0, n_libs, i / SPB)];
+                    base = sl.cnt > 0 && bloom_is_solid_lazy<kw_of_nw(NW)>(v, L.k, L.solid, L.M);
+                }
             }
             const bool take = base || (occ && pflag && pflag[i]);
             bits |= (uint32_t)take << j;
             counted |= (uint32_t)base << j;
             cnt += take;
         }
-        s_scan[tid] = cnt;
-        __syncthreads();
-        for (uint32_t off = 1; off < BLOCK; off <<= 1) {
-            const uint32_t v = tid >= off ? s_scan[tid - off] : 0;
-            __syncthreads();
-            s_scan[tid] += v;
-            __syncthreads();
+        uint32_t inc = cnt;  // inclusive scan: in the wave, then across the waves
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, d, 64);
+            if (lane >= d) inc += v;
         }
-        const uint32_t total = s_scan[BLOCK - 1];
+        if (lane == 63) s_scan[wave] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < BLOCK / 64; ++w) {
+            const uint32_t t = s_scan[w];
+            before += w < wave ? t : 0u;
+            total += t;
+        }
         if (tid == 0) s_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
         __syncthreads();
-        uint64_t o = s_base + s_scan[tid] - cnt;
+        uint64_t o = s_base + before + inc - cnt;
         while (bits) {
             const uint32_t j = __ffs(bits) - 1;
             bits &= bits - 1;
             const uint64_t i = c0 + (uint64_t)j * BLOCK + tid;
             const Slot<NW, Ev> sl = tab[i];
-            const uint32_t lib = find_region(lib_b0, n_libs, i / SPB);
+            const uint32_t lib = single ? lib0 : find_region(lib_b0, n_libs, i / SPB);
             if (lay.wide) {
                 const Kv v = unpack_key<NW>(sl.k);
                 out_wide[o] = WideEntry{{v.lo, v.hi, v.w2, v.w3}, lib, 0u};

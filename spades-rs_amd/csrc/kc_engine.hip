@@ -1164,13 +1164,15 @@ __global__ void __launch_bounds__(BLOCK) k_seen_from_fs(uint64_t w_begin, uint64
 // (e >> FS_PART_LOG2), each partition's minima are taken in LDS by one workgroup and its
 // first setters and seen words written in order, in place of three random device-scope
 // atomicMin per k-mer.  The minimum does not depend on the order of the items, so the
-// result is the one of the atomics.
+// result is the one of the atomics.  When the partition keys end below bit 63 (MULTI), bit
+// 63 of an item marks a k-mer counted at least twice, which always promotes: k_fs_min
+// then also writes those k-mers' solid bits, and k_promote skips them.
 constexpr uint32_t FS_PART_LOG2 = 14;  // entries per partition: 64 KiB of LDS
 constexpr uint32_t FS_PART = 1u << FS_PART_LOG2;
 constexpr uint32_t FS_KEY_SHIFT = 32 + FS_PART_LOG2;
 constexpr uint32_t FS_BLOCK = 1024;
 
-template <int NW>
+template <int NW, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) k_fs_items(const Slot<NW, uint32_t>* __restrict__ tab, uint64_t s_begin,
                                                     uint64_t s_end, const uint64_t* __restrict__ lib_b0,
                                                     uint32_t n_libs, const LibDesc* __restrict__ libs,
@@ -1195,8 +1197,9 @@ __global__ void __launch_bounds__(BLOCK) k_fs_items(const Slot<NW, uint32_t>* __
             kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
             const uint64_t e0 = (uint64_t)(reinterpret_cast<const uint8_t*>(L.fs) - fs_base) / sizeof(uint32_t);
             const uint32_t base = s.t0 * 4u;
+            const uint64_t multi = MULTI && s.cnt >= 2 ? 1ull << 63 : 0ull;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) o[j] = (e0 + p[j]) << 32 | (uint32_t)(base + j);
+            for (int j = 0; j < 3; ++j) o[j] = multi | (e0 + p[j]) << 32 | (uint32_t)(base + j);
             acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
         },
         [&](uint32_t lib0) {
@@ -1208,14 +1211,15 @@ __global__ void __launch_bounds__(BLOCK) k_fs_items(const Slot<NW, uint32_t>* __
         });
 }
 
-// bounds[b] = the first sorted item whose partition is >= b, for b = 0 .. nparts.
-__global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint32_t nparts,
+// bounds[b] = the first sorted item whose partition (key bits under kmask) is >= b, for
+// b = 0 .. nparts.
+__global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint32_t nparts, uint32_t kmask,
                             uint64_t* __restrict__ bounds) {
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nparts; b += gridDim.x * blockDim.x) {
         uint64_t lo = 0, hi = n;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
-            if ((items[mid] >> FS_KEY_SHIFT) < b) lo = mid + 1;
+            if (((uint32_t)(items[mid] >> FS_KEY_SHIFT) & kmask) < b) lo = mid + 1;
             else hi = mid;
         }
         bounds[b] = lo;
@@ -1223,17 +1227,22 @@ __global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint
 }
 
 // One partition per workgroup pass: the minimum setter of each of its FS_PART entries in
-// LDS, then, one wave per 64 entries, the entries to fs, their seen word (entry != ~0) by
-// ballot, and a cleared solid word.  fs_n is a multiple of 64.
+// LDS (and, MULTI, the solid bits of the items marked counted twice), then, one wave per
+// 64 entries, the entries to fs, their seen word (entry != ~0) by ballot, and the solid
+// word.  fs_n is a multiple of 64.
+template <bool MULTI>
 __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict__ items,
                                                      const uint64_t* __restrict__ bounds, uint32_t nparts,
                                                      uint32_t* __restrict__ fs, uint64_t fs_n,
                                                      uint64_t* __restrict__ seen, uint64_t* __restrict__ solid) {
     __shared__ uint32_t m[FS_PART];
+    __shared__ uint32_t sol[MULTI ? FS_PART / 32 : 1];
     constexpr uint32_t U = 4;  // items in flight per thread
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
         for (uint32_t e = threadIdx.x; e < FS_PART; e += FS_BLOCK) m[e] = ~0u;
+        if (MULTI)
+            for (uint32_t e = threadIdx.x; e < FS_PART / 32; e += FS_BLOCK) sol[e] = 0;
         __syncthreads();
         const uint64_t b = bounds[part], end = bounds[part + 1];
         for (uint64_t i0 = b; i0 < end; i0 += (uint64_t)FS_BLOCK * U) {
@@ -1244,8 +1253,12 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict_
                 it[u] = i < end ? items[i] : ~0ull;
             }
 #pragma unroll
-            for (uint32_t u = 0; u < U; ++u)
-                if (it[u] != ~0ull) atomicMin(&m[(uint32_t)(it[u] >> 32) & (FS_PART - 1)], (uint32_t)it[u]);
+            for (uint32_t u = 0; u < U; ++u) {
+                if (it[u] == ~0ull) continue;
+                const uint32_t e = (uint32_t)(it[u] >> 32) & (FS_PART - 1);
+                atomicMin(&m[e], (uint32_t)it[u]);
+                if (MULTI && (it[u] >> 63)) atomicOr(&sol[e >> 5], 1u << (e & 31));
+            }
         }
         __syncthreads();
         const uint64_t e0 = (uint64_t)part * FS_PART;
@@ -1256,7 +1269,7 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict_
             const uint64_t bits = __ballot(v != ~0u);
             if (lane == 0) {
                 seen[e0 / 64 + w] = bits;
-                solid[e0 / 64 + w] = 0;
+                solid[e0 / 64 + w] = MULTI ? (uint64_t)sol[2 * w] | (uint64_t)sol[2 * w + 1] << 32 : 0ull;
             }
         }
         __syncthreads();
@@ -1289,12 +1302,14 @@ __global__ void __launch_bounds__(BLOCK) k_fs_reset(const Slot<NW, Ev>* __restri
 template <int NW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restrict__ tab, uint64_t s_begin,
                                                    uint64_t s_end, const uint64_t* __restrict__ lib_b0,
-                                                   uint32_t n_libs, const LibDesc* __restrict__ libs) {
+                                                   uint32_t n_libs, const LibDesc* __restrict__ libs,
+                                                   bool skip_multi) {
     scan_slots<NW, Ev>(
         tab, s_begin, s_end, lib_b0, n_libs,
         [&](uint64_t i, uint32_t lib, uint32_t) {
             const Slot<NW, Ev> s = tab[i];
             if (s.k[0] == EMPTY_KEY) return;
+            if (skip_multi && s.cnt >= 2) return;  // already set by the sorted first-setter pass
             const LibDesc& L = libs[lib];
             const Kv v = unpack_key<NW>(s.k);
             uint64_t p[3];
@@ -2705,7 +2720,7 @@ struct Run {
                                                 c->d_out.as<LibOut>())));
                     TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
-                                           L(), c->d_libs.as<LibDesc>())));
+                                           L(), c->d_libs.as<LibDesc>(), false)));
                     TIMED(KID_FS_CLEAR, (k_fs_reset<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                             c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
                                             L(), c->d_libs.as<LibDesc>())));
@@ -2715,15 +2730,16 @@ struct Run {
                 const uint32_t nparts = (uint32_t)((fs_n + FS_PART - 1) / FS_PART);
                 uint32_t key_bits = 1;  // partition keys 0 .. nparts - 1; empty slots' items all ones
                 while ((1ull << key_bits) <= nparts) ++key_bits;
+                const bool multi = FS_KEY_SHIFT + key_bits <= 63;  // bit 63 free for the counted-twice mark
                 KC_TRY(c->d_fsi.ensure(std::max<uint64_t>(n_items, 1) * 8));
                 KC_TRY(c->d_fsi2.ensure(std::max<uint64_t>(n_items, 1) * 8));
                 KC_TRY(c->d_fsb.ensure(((uint64_t)nparts + 1) * 8));
                 if (n_items) {
-                    TIMED(KID_FS_ITEMS, (k_fs_items<NW><<<gs, BLOCK, 0, st>>>(
-                                                c->d_tab.as<Slot<NW, uint32_t>>(), s_begin, s_end,
-                                                c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
-                                                c->d_out.as<LibOut>(), c->d_fs.as<uint8_t>(),
-                                                c->d_fsi.as<uint64_t>())));
+                    auto items = multi ? k_fs_items<NW, true> : k_fs_items<NW, false>;
+                    TIMED(KID_FS_ITEMS, (items<<<gs, BLOCK, 0, st>>>(
+                                            c->d_tab.as<Slot<NW, uint32_t>>(), s_begin, s_end,
+                                            c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                            c->d_out.as<LibOut>(), c->d_fs.as<uint8_t>(), c->d_fsi.as<uint64_t>())));
                     size_t tb = 0;
                     KC_TRY(rocprim::radix_sort_keys(nullptr, tb, c->d_fsi.as<uint64_t>(), c->d_fsi2.as<uint64_t>(),
                                                     (size_t)n_items, FS_KEY_SHIFT, FS_KEY_SHIFT + key_bits, st));
@@ -2735,16 +2751,18 @@ struct Run {
                                                           FS_KEY_SHIFT + key_bits, st)));
                 }
                 k_fs_bounds<<<grid_for((uint64_t)nparts + 1), BLOCK, 0, st>>>(
-                    c->d_fsi2.as<uint64_t>(), n_items, nparts, c->d_fsb.as<uint64_t>());
+                    c->d_fsi2.as<uint64_t>(), n_items, nparts, (uint32_t)((1ull << key_bits) - 1),
+                    c->d_fsb.as<uint64_t>());
                 KC_TRY(hipGetLastError());
-                TIMED(KID_FS_MIN, (k_fs_min<<<nparts, FS_BLOCK, 0, st>>>(
+                auto fsmin = multi ? k_fs_min<true> : k_fs_min<false>;
+                TIMED(KID_FS_MIN, (fsmin<<<nparts, FS_BLOCK, 0, st>>>(
                                       c->d_fsi2.as<uint64_t>(), c->d_fsb.as<uint64_t>(), nparts,
                                       c->d_fs.as<uint32_t>(), fs_n, c->d_seen.as<uint64_t>() + w_begin,
                                       c->d_solid.as<uint64_t>() + w_begin)));
                 if (s_end > s_begin)
                     TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
-                                           L(), c->d_libs.as<LibDesc>())));
+                                           L(), c->d_libs.as<LibDesc>(), multi)));
             } else {
                 TIMED(KID_FS_CLEAR,
                       (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));
@@ -2758,7 +2776,7 @@ struct Run {
                 if (s_end > s_begin)
                     TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
-                                           L(), c->d_libs.as<LibDesc>())));
+                                           L(), c->d_libs.as<LibDesc>(), false)));
             }
         }
         if (sparse) c->fs_clean_bytes = clean;

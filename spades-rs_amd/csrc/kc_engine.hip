@@ -697,24 +697,27 @@ __global__ void __launch_bounds__(BLOCK) k_part3(PartRecs<NW, Ev> r2, const unsi
 // definition).  Windows are taken 64 at a time: lane l holds the keys of m-mers c0 + l
 // and c0 + 64 + l, a sparse-table sliding minimum over shuffles gives window c0 + l its
 // minimizer key, and a ballot marks the windows whose bin differs from the previous
-// window's.  The lane at such a start accounts for the run that ends before it; the
-// read's last run is accounted for after the loop.  Counting (EMIT false) stores the
-// read's record count in cnt[a] and every window's bin in wbin (at its global window
-// number act_ev[a] + j); emitting reads the bins back instead of recomputing them and
-// writes the read's records from sk_off[a] on: bin key, and a | first window << 32 |
-// windows << 56.  No lane diverges on the data.  (Reserving room chunk by chunk on one
-// global counter instead of counting first was measured 40x slower: the same-address
-// atomics serialize.)
-template <bool EMIT>
+// window's.  The lane at such a start writes the record of the run that ends before it;
+// the read's last run is written after the loop.  A read has at most as many records as
+// windows, so its records go to scratch from its global window number act_ev[a] on, as
+// bin | first window << 20 | windows << 44 (bins < 2^20, first window < 2^24), and its
+// record count to cnt[a]; k_sk_copy then packs them in read order.  No lane diverges on
+// the data.  (Reserving packed room chunk by chunk on one global counter was measured
+// 40x slower: the same-address atomics serialize.  Counting first and cutting again, the
+// second pass reading back every window's bin, was about 0.6 ms a k slower than this
+// scratch on 5 Mb / 100x.)
+static_assert(BIN_BITS_MAX <= 20, "scratch records hold 20-bit bins");
+__device__ __forceinline__ uint64_t sk_scratch(uint32_t bin, uint32_t first, uint32_t nw) {
+    return (uint64_t)bin | (uint64_t)first << 20 | (uint64_t)nw << 44;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
                                                    const uint64_t* __restrict__ act_off,
                                                    const uint32_t* __restrict__ active,
                                                    const uint64_t* __restrict__ nwin,
-                                                   const uint64_t* __restrict__ act_ev,
-                                                   const uint32_t* __restrict__ act_lib, uint64_t n_active, uint32_t k,
+                                                   const uint64_t* __restrict__ act_ev, uint64_t n_active, uint32_t k,
                                                    uint32_t bits, uint32_t* __restrict__ cnt,
-                                                   uint32_t* __restrict__ wbin, const uint64_t* __restrict__ sk_off,
-                                                   uint32_t* __restrict__ keys, uint64_t* __restrict__ recs) {
+                                                   uint64_t* __restrict__ scratch) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
     const uint32_t m = minimizer_len(k), w = k - m + 1;
@@ -725,8 +728,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
         const uint64_t bit0 = act_off[a] * 8;
         const uint32_t nw = (uint32_t)nwin[active[a]];
-        uint32_t* wb = wbin + act_ev[a];
-        const uint32_t lkey = act_lib[a] << bits;  // sort key: (library, bin)
+        uint64_t* out = scratch + act_ev[a];
         const uint32_t nm = nw + w - 1;  // m-mers of the read
         auto key_at = [&](uint32_t i) -> uint32_t {
             if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
@@ -739,31 +741,24 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
             return mmer_key(f < r ? f : r);
         };
         uint32_t open = 0, carry_bin = 0, total = 0;
-        uint64_t out = EMIT ? sk_off[a] : 0;
-        uint32_t K = EMIT ? 0u : key_at(lane);  // keys of positions c0 + lane
+        uint32_t K = key_at(lane);  // keys of positions c0 + lane
         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
             const bool valid = c0 + lane < nw;
-            uint32_t bin;
-            if (EMIT) {
-                bin = valid ? wb[c0 + lane] : 0u;
-            } else {
-                // A = positions c0 + lane, B = positions c0 + 64 + lane (the windows read B
-                // up to lane w - 2; B is the next chunk's A)
-                const uint32_t Kn = key_at(c0 + 64 + lane);
-                uint32_t A = K, B = Kn;
-                K = Kn;
-                for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
-                    const uint32_t src = (lane + d) & 63;
-                    const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
-                    const bool wrap = lane + d >= 64;
-                    A = min(A, wrap ? tB : tA);
-                    if (!wrap) B = min(B, tB);
-                }
-                const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
-                const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
-                bin = minimizer_bin(min(A, e >= 64 ? tB : tA), bits);
-                if (valid) wb[c0 + lane] = bin;
+            // A = positions c0 + lane, B = positions c0 + 64 + lane (the windows read B up to
+            // lane w - 2; B is the next chunk's A)
+            const uint32_t Kn = key_at(c0 + 64 + lane);
+            uint32_t A = K, B = Kn;
+            K = Kn;
+            for (uint32_t d = 1; d < p; d <<= 1) {  // A, B: minimum over [pos, pos + 2d)
+                const uint32_t src = (lane + d) & 63;
+                const uint32_t tA = __shfl(A, src), tB = __shfl(B, src);
+                const bool wrap = lane + d >= 64;
+                A = min(A, wrap ? tB : tA);
+                if (!wrap) B = min(B, tB);
             }
+            const uint32_t e = lane + w - p;  // [lane, lane + w) = [lane, +p) and [e, e + p)
+            const uint32_t tA = __shfl(A, e & 63), tB = __shfl(B, e & 63);
+            const uint32_t bin = minimizer_bin(min(A, e >= 64 ? tB : tA), bits);
             const uint32_t prev = __shfl(bin, (lane + 63) & 63);
             const uint32_t pb = lane ? prev : carry_bin;
             const bool start = valid && c0 + lane > 0 && bin != pb;
@@ -775,36 +770,46 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                 rl = c0 + lane - rs;
                 nrec = (rl + SK_MAX_WINDOWS - 1) / SK_MAX_WINDOWS;
             }
-            if (EMIT) {
-                uint32_t incl = nrec;
-                for (uint32_t d = 1; d < 64; d <<= 1) {
-                    const uint32_t t = __shfl_up(incl, d);
-                    if (lane >= d) incl += t;
-                }
-                uint64_t o = out + incl - nrec;
-                for (uint32_t t = 0; t < nrec; ++t, ++o) {
-                    const uint32_t f0 = rs + t * SK_MAX_WINDOWS;
-                    keys[o] = lkey | pb;
-                    recs[o] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
-                }
-                out += __shfl(incl, 63);
-            } else {
-                total += nrec;
+            uint32_t incl = nrec;
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint32_t t = __shfl_up(incl, d);
+                if (lane >= d) incl += t;
             }
+            uint32_t o = total + incl - nrec;
+            for (uint32_t t = 0; t < nrec; ++t, ++o)
+                out[o] = sk_scratch(pb, rs + t * SK_MAX_WINDOWS, min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS));
+            total += __shfl(incl, 63);
             if (sm) open = c0 + (63 - __clzll(sm));
             carry_bin = __shfl(bin, min(63u, nw - 1 - c0));
         }
         // the last run [open, nw)
         const uint32_t rl = nw - open, nlast = (rl + SK_MAX_WINDOWS - 1) / SK_MAX_WINDOWS;
-        if (EMIT) {
-            for (uint32_t t = lane; t < nlast; t += 64) {
-                const uint32_t f0 = open + t * SK_MAX_WINDOWS;
-                keys[out + t] = lkey | carry_bin;
-                recs[out + t] = a | ((uint64_t)f0 << 32) | ((uint64_t)min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS) << 56);
-            }
-        } else {
-            for (uint32_t d = 32; d >= 1; d >>= 1) total += __shfl_xor(total, d);
-            if (lane == 0) cnt[a] = total + nlast;
+        for (uint32_t t = lane; t < nlast; t += 64)
+            out[total + t] = sk_scratch(carry_bin, open + t * SK_MAX_WINDOWS,
+                                        min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS));
+        if (lane == 0) cnt[a] = total + nlast;
+    }
+}
+
+// The records of each read from scratch to their packed place sk_off[a] ..: bin key
+// (library << bits | bin), and a | first window << 32 | windows << 56.  One wave per read.
+__global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ act_ev,
+                                                   const uint32_t* __restrict__ act_lib, uint64_t n_active,
+                                                   uint32_t bits, const uint32_t* __restrict__ cnt,
+                                                   const uint64_t* __restrict__ sk_off,
+                                                   const uint64_t* __restrict__ scratch, uint32_t* __restrict__ keys,
+                                                   uint64_t* __restrict__ recs) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
+        const uint32_t n = cnt[a];
+        const uint64_t* src = scratch + act_ev[a];
+        const uint64_t o = sk_off[a];
+        const uint32_t lkey = act_lib[a] << bits;
+        for (uint32_t t = lane; t < n; t += 64) {
+            const uint64_t r = src[t];
+            keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFu);
+            recs[o + t] = a | ((r >> 20) & 0xFFFFFFull) << 32 | (r >> 44) << 56;
         }
     }
 }
@@ -1866,7 +1871,7 @@ struct kc_ctx {
     bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
-    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;
+    DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
@@ -2364,11 +2369,11 @@ struct Run {
             // records per read, their offsets, then the records in read order
             KC_TRY(c->d_skc.ensure(na * 4 + 4));
             KC_TRY(c->d_sko.ensure(na * 8 + 8));
-            KC_TRY(c->d_wbin.ensure(c->n_occ * 4 + 4));
-            TIMED(KID_SK, (k_sk_wave<false><<<gw, BLOCK, 0, st>>>(
-                              c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na, k,
-                              bits, c->d_skc.as<uint32_t>(), c->d_wbin.as<uint32_t>(), nullptr, nullptr, nullptr)));
+            KC_TRY(c->d_wbin.ensure(c->n_occ * 8 + 8));
+            TIMED(KID_SK, (k_sk_wave<<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
+                                                          c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
+                                                          c->d_act_ev.as<uint64_t>(), na, k, bits,
+                                                          c->d_skc.as<uint32_t>(), c->d_wbin.as<uint64_t>())));
             size_t tb0 = 0;
             KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                            (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
@@ -2384,11 +2389,10 @@ struct Run {
             KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
             KC_TRY(c->d_br.ensure(nrec * 8 + 8));
             KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
-            TIMED(KID_SK, (k_sk_wave<true><<<gw, BLOCK, 0, st>>>(
-                              c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(), c->d_active.as<uint32_t>(),
-                              c->d_nwin.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na, k,
-                              bits, nullptr, c->d_wbin.as<uint32_t>(), c->d_sko.as<uint64_t>(),
-                              c->d_bk.as<uint32_t>(), c->d_br.as<uint64_t>())));
+            TIMED(KID_SK, (k_sk_copy<<<gw, BLOCK, 0, st>>>(c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na,
+                                                          bits, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
+                                                          c->d_wbin.as<uint64_t>(), c->d_bk.as<uint32_t>(),
+                                                          c->d_br.as<uint64_t>())));
         }
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;

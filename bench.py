@@ -28,7 +28,7 @@ for _p in (ROOT, PKG):
 
 HBM_PEAK_GBS = 8000.0
 G_BYTES = 64  # one random 64 B access per table RMW
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_insert_5mb_100x_traffic.json")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_5mb_100x.json")
 COPY_BYTES = 1 << 30  # the calibration copy that --copy-calibration runs first
 
 

@@ -770,15 +770,22 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                 rl = c0 + lane - rs;
                 nrec = (rl + SK_MAX_WINDOWS - 1) / SK_MAX_WINDOWS;
             }
-            uint32_t incl = nrec;
-            for (uint32_t d = 1; d < 64; d <<= 1) {
-                const uint32_t t = __shfl_up(incl, d);
-                if (lane >= d) incl += t;
+            uint32_t o, nchunk;  // the lane's first output; the chunk's records
+            if (__ballot(nrec > 1) == 0) {  // one record per start: rank the starts
+                o = total + (uint32_t)__popcll(sm & ((1ull << lane) - 1));
+                nchunk = (uint32_t)__popcll(sm);
+            } else {  // a run longer than SK_MAX_WINDOWS: scan the record counts
+                uint32_t incl = nrec;
+                for (uint32_t d = 1; d < 64; d <<= 1) {
+                    const uint32_t t = __shfl_up(incl, d);
+                    if (lane >= d) incl += t;
+                }
+                o = total + incl - nrec;
+                nchunk = __shfl(incl, 63);
             }
-            uint32_t o = total + incl - nrec;
             for (uint32_t t = 0; t < nrec; ++t, ++o)
                 out[o] = sk_scratch(pb, rs + t * SK_MAX_WINDOWS, min(SK_MAX_WINDOWS, rl - t * SK_MAX_WINDOWS));
-            total += __shfl(incl, 63);
+            total += nchunk;
             if (sm) open = c0 + (63 - __clzll(sm));
             carry_bin = __shfl(bin, min(63u, nw - 1 - c0));
         }

@@ -19,8 +19,9 @@
 //   5. one wavefront per input read writes its output reads' offsets, lengths and packed
 //      bytes (lane q packs output bytes q, q + 64, ...).
 // The device handles the 4-line FASTQ layout (blank or whitespace-only lines allowed before
-// the first record, blank lines after the last); blank lines between records and FASTA give
-// KC_EUNSUPPORTED, for the host parser.
+// the first record, blank lines after the last) and FASTA (single- or multi-line: each
+// record's sequence lines are copied together first); blank lines between FASTQ records
+// give KC_EUNSUPPORTED, for the host parser.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -59,6 +60,7 @@ struct Buf : DevBuf {
 constexpr uint32_t MIN_K = 21;     // packed_reads.rs:38
 constexpr uint32_t TILE = BLOCK * 16;  // bytes per newline-index tile
 constexpr uint64_t FILE_PAD = 64;  // zero bytes after each uploaded file
+constexpr uint32_t WAVE = 64;
 
 __device__ __forceinline__ uint32_t code2(uint8_t b) {  // dna.rs:11-30, 4 = not ACGT
     const uint32_t c = b | 0x20u;
@@ -210,6 +212,60 @@ __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsig
     }
 }
 
+// FASTA (FastxReader::next_seq, packed_reads.rs:217-235): after the first '>' header, a
+// record's sequence is the concatenation of its lines (CR/LF stripped) up to the next line
+// that starts with '>' (after stripping) or the end of the file; a last record with no
+// bases is no record.  Line j = first + i: hdr[i] = it is a header, len[i] = the bases it
+// contributes (0 for a header); err bit 1 = a line of 2^32 bytes or more.
+__global__ void k_fa_lines(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
+                           uint64_t m, uint64_t first, uint64_t nlines, uint32_t* __restrict__ hdr,
+                           uint32_t* __restrict__ len, uint32_t* __restrict__ err) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nlines; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t s, e;
+        line_span(nl, m, n, first + i, s, e);
+        while (e > s && (b[e - 1] == '\r' || b[e - 1] == '\n')) --e;
+        const bool h = e > s && b[s] == '>';
+        const uint64_t l = h ? 0 : e - s;
+        if (l > 0xFFFFFFFFull) atomicOr(err, 1u);
+        hdr[i] = h;
+        len[i] = (uint32_t)l;
+    }
+}
+
+// One wave per sequence line: its bases to out[off[i] ..].  Record r (the r-th header,
+// rec[i] = headers before line i) starts at its header's offset.
+__global__ __launch_bounds__(BLOCK) void k_fa_copy(const uint8_t* __restrict__ b, uint64_t n,
+                                                   const unsigned long long* __restrict__ nl, uint64_t m,
+                                                   uint64_t first, uint64_t nlines, const uint32_t* __restrict__ hdr,
+                                                   const uint32_t* __restrict__ len,
+                                                   const unsigned long long* __restrict__ rec,
+                                                   const unsigned long long* __restrict__ off,
+                                                   uint8_t* __restrict__ out, uint64_t* __restrict__ seq_s) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t i = blockIdx.x * (uint64_t)(BLOCK / WAVE) + threadIdx.x / WAVE; i < nlines; i += waves) {
+        if (hdr[i]) {
+            if (lane == 0) seq_s[rec[i]] = off[i];
+            continue;
+        }
+        uint64_t s, e;
+        line_span(nl, m, n, first + i, s, e);
+        const uint64_t o = off[i];
+        for (uint64_t q = lane; q < len[i]; q += WAVE) out[o + q] = b[s + q];
+    }
+}
+
+// seq_l[r] = the next record's start (or the total) - seq_s[r]; err bit 1 = a record of
+// 2^32 bases or more.
+__global__ void k_fa_lens(const uint64_t* __restrict__ seq_s, uint64_t R, uint64_t total,
+                          uint32_t* __restrict__ seq_l, uint32_t* __restrict__ err) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t l = (r + 1 < R ? seq_s[r + 1] : total) - seq_s[r];
+        if (l > 0xFFFFFFFFull) atomicOr(err, 1u);
+        seq_l[r] = (uint32_t)l;
+    }
+}
+
 // Gate 1 and the N scan of one read: trimmed span [ts, ts + tl), internal-N flag, number of
 // ACGT segments of at least MIN_K bases and their packed bytes.
 struct ReadGate {
@@ -217,7 +273,6 @@ struct ReadGate {
     uint64_t seg_bytes, seg_bases;
 };
 
-constexpr uint32_t WAVE = 64;
 
 // This wave's index in the grid, as a value the compiler knows to be wave-uniform (so the
 // per-read loads and the run walk below are scalar).
@@ -481,7 +536,10 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, 
 
 struct FileIndex {
     Buf buf, tiles, tile_base, nl, red, seq_s, seq_l, qual_s, err;
+    Buf fa_hdr, fa_len, fa_rec, fa_off, fa_seq;  // FASTA: line flags and lengths, their scans, the sequences
     uint64_t n = 0, R = 0;
+    bool fasta = false;
+    const uint8_t* seqs() const { return fasta ? fa_seq.as<uint8_t>() : buf.as<uint8_t>(); }
 };
 
 }  // namespace
@@ -532,12 +590,70 @@ int upload(kc_ingest* c, const uint8_t* host, uint64_t n, FileIndex& F) {
     return KC_OK;
 }
 
+// FASTA records of one uploaded file whose first non-blank line (`first`) is a '>' header:
+// the sequence lines copied together into F.fa_seq, the records' spans into it.
+int index_fasta(kc_ingest* c, FileIndex& F, const char* name, uint64_t m, uint64_t first, uint64_t lines) {
+    hipStream_t st = c->st;
+    const uint64_t n = F.n, nlines = lines - first;
+    const uint8_t* b = F.buf.as<uint8_t>();
+    if (c->keep_quals) return fail(KC_EUNSUPPORTED, std::string(name) + ": FASTA has no qualities to keep");
+    I_TRY(F.fa_hdr.ensure(nlines * 4 + 4));
+    I_TRY(F.fa_len.ensure(nlines * 4 + 4));
+    I_TRY(F.fa_rec.ensure(nlines * 8 + 8));
+    I_TRY(F.fa_off.ensure(nlines * 8 + 8));
+    I_TRY(F.err.ensure(64));
+    const uint32_t zero = 0;
+    I_TRY(hipMemcpyAsync(F.err.p, &zero, 4, hipMemcpyHostToDevice, st));
+    k_fa_lines<<<grid_for(nlines), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, nlines,
+                                                   F.fa_hdr.as<uint32_t>(), F.fa_len.as<uint32_t>(),
+                                                   F.err.as<uint32_t>());
+    I_TRY(hipGetLastError());
+    int rc = scan_excl(c, F.fa_hdr.as<uint32_t>(), F.fa_rec.as<unsigned long long>(), nlines);
+    if (!rc) rc = scan_excl(c, F.fa_len.as<uint32_t>(), F.fa_off.as<unsigned long long>(), nlines);
+    if (rc) return rc;
+    unsigned long long last[2];
+    uint32_t lastv[2], err = 0;
+    I_TRY(hipMemcpyAsync(&last[0], F.fa_rec.as<unsigned long long>() + nlines - 1, 8, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&last[1], F.fa_off.as<unsigned long long>() + nlines - 1, 8, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&lastv[0], F.fa_hdr.as<uint32_t>() + nlines - 1, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&lastv[1], F.fa_len.as<uint32_t>() + nlines - 1, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&err, F.err.p, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipStreamSynchronize(st));
+    if (err) return fail(KC_EINVAL, std::string(name) + ": a FASTA line of 2^32 bytes or more");
+    uint64_t R = last[0] + lastv[0];  // headers; line `first` is one
+    const uint64_t total = last[1] + lastv[1];
+    I_TRY(F.fa_seq.ensure(total + FILE_PAD));
+    I_TRY(hipMemsetAsync(F.fa_seq.as<uint8_t>() + total, 0, FILE_PAD, st));
+    I_TRY(F.seq_s.ensure(R * 8 + 8));
+    I_TRY(F.seq_l.ensure(R * 4 + 4));
+    I_TRY(F.qual_s.ensure(R * 8 + 8));
+    k_fa_copy<<<grid_for(nlines * WAVE), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, nlines,
+                                                         F.fa_hdr.as<uint32_t>(), F.fa_len.as<uint32_t>(),
+                                                         F.fa_rec.as<unsigned long long>(),
+                                                         F.fa_off.as<unsigned long long>(), F.fa_seq.as<uint8_t>(),
+                                                         F.seq_s.as<uint64_t>());
+    I_TRY(hipGetLastError());
+    k_fa_lens<<<grid_for(R), BLOCK, 0, st>>>(F.seq_s.as<uint64_t>(), R, total, F.seq_l.as<uint32_t>(),
+                                             F.err.as<uint32_t>());
+    I_TRY(hipGetLastError());
+    uint32_t last_len = 0;
+    I_TRY(hipMemcpyAsync(&last_len, F.seq_l.as<uint32_t>() + R - 1, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&err, F.err.p, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipStreamSynchronize(st));
+    if (err) return fail(KC_EINVAL, std::string(name) + ": a FASTA record of 2^32 bases or more");
+    if (last_len == 0) --R;  // next_seq returns the last record only when it has bases
+    F.fasta = true;
+    F.R = R;
+    return KC_OK;
+}
+
 // Finds one uploaded file's records' sequence spans.
 int index_file(kc_ingest* c, FileIndex& F, const char* name) {
     hipStream_t st = c->st;
     const uint64_t n = F.n;
     const uint8_t* b = F.buf.as<uint8_t>();
     I_TRY(F.red.ensure(64));
+    F.fasta = false;
     uint64_t m = 0;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (n) {
@@ -591,7 +707,7 @@ int index_file(kc_ingest* c, FileIndex& F, const char* name) {
         I_TRY(hipMemcpyAsync(&head, b + at, 1, hipMemcpyDeviceToHost, st));
         I_TRY(hipStreamSynchronize(st));
     }
-    if (head == '>') return fail(KC_EUNSUPPORTED, std::string(name) + ": FASTA input is parsed on the host");
+    if (head == '>') return index_fasta(c, F, name, m, first, lines);
     if (head != '@') return fail(KC_EINVAL, std::string(name) + ": expected '@' or '>' header");
     // Empty lines between the first and the last record are either legal empty sequence /
     // quality lines or blank separators, which FastxReader skips before a header.  With
@@ -674,8 +790,8 @@ int ingest(kc_ingest* c, const uint8_t* r1, uint64_t n1, const uint8_t* r2, uint
     I_TRY(c->small.ensure(64));
     const unsigned long long zero = 0;
     I_TRY(hipMemcpyAsync(c->small.p, &zero, 8, hipMemcpyHostToDevice, st));
-    const uint8_t* b1 = A.buf.as<uint8_t>();
-    const uint8_t* b2 = B.buf.as<uint8_t>();
+    const uint8_t* b1 = A.seqs();
+    const uint8_t* b2 = B.seqs();
     ReadGate* gates = c->gates.as<ReadGate>();
     if (tot) {
         k_gates<<<grid_for(tot * WAVE), BLOCK, 0, st>>>(b1, A.seq_s.as<uint64_t>(), A.seq_l.as<uint32_t>(), R1, b2,

@@ -10,7 +10,7 @@ import pytest
 
 from kc import KcError, _abi
 from kc.ingest import fastq_pack
-from test_oracle_fastq import fastq, random_reads
+from test_oracle_fastq import fasta, fastq, random_reads
 
 pytestmark = pytest.mark.gpu
 
@@ -129,10 +129,33 @@ def test_unsupported_and_malformed():
     with pytest.raises(KcError) as e:
         fastq_pack(fastq([b"ACGT" * 10, b"GGGG" * 10], blank_between=True), fastq([b"ACGT" * 10, b"TTTT" * 10]))
     assert e.value.status == _abi.KC_EUNSUPPORTED
-    with pytest.raises(KcError) as e:
-        fastq_pack(b">c\nACGT\n", None)
-    assert e.value.status == _abi.KC_EUNSUPPORTED
     for bad in (b"@r\nACGT\n+\nII\n", b"@r\nACGT\n-\nIIII\n", b"xr\nACGT\n+\nIIII\n", b"@r\nACGT\n+\n"):
         with pytest.raises(KcError) as e:
             fastq_pack(bad, fastq([b"ACGT"]))
         assert e.value.status == _abi.KC_EINVAL, bad
+
+
+@pytest.mark.parametrize("seed,width,crlf,blank,lead", [(41, 0, False, False, False), (42, 60, True, True, True),
+                                                        (43, 7, False, True, False), (44, 150, False, False, True)])
+def test_fasta_matches_oracle(oracle, seed, width, crlf, blank, lead):
+    """FASTA on the device: each record's sequence lines copied together, then the same gates
+    and packing; pairs, single files, an empty last record, and a FASTA R1 with a FASTQ R2."""
+    say(f"case fasta seed={seed} width={width}")
+    rng = np.random.default_rng(seed)
+    r1 = random_reads(rng, 700) + [b""]
+    r2 = random_reads(rng, 650)
+    a, b = fasta(r1, width, crlf, blank, lead), fasta(r2, width, crlf, blank, lead)
+    info = check(oracle, a, b)
+    assert info["r1_records"] == len(r1) - 1 and info["r2_records"] == len(r2)
+    check(oracle, a)
+    check(oracle, b, fastq(r1[:-1], crlf))
+    check(oracle, b">only\n")  # one record without bases: none
+    check(oracle, b">a\n>b\n" + b"ACGT" * 8 + b"\n")  # an empty record, then one
+    with pytest.raises(KcError) as e:  # no qualities to keep
+        from kc.ingest import FastqIngest
+        ctx = FastqIngest(keep_quals=True)
+        try:
+            ctx.pack(a)
+        finally:
+            ctx.close()
+    assert e.value.status == _abi.KC_EUNSUPPORTED

@@ -18,6 +18,26 @@ def fastq(reads, crlf=False, blank_between=False, lead_blank=False):
     return b"".join(out)
 
 
+def fasta(reads, width=0, crlf=False, blank_inside=False, lead_blank=False):
+    """FASTA text of the reads: a '>' header each, the sequence in lines of `width` bases
+    (one line when 0; none for an empty read), optionally with empty lines inside."""
+    eol = b"\r\n" if crlf else b"\n"
+    out = [eol, b"  " + eol] if lead_blank else []
+    for i, r in enumerate(reads):
+        out.append(b">r%d extra" % i + eol)
+        w = width or max(len(r), 1)
+        for j in range(0, len(r), w):
+            out.append(r[j:j + w] + eol)
+            if blank_inside and j % (3 * w) == 0:
+                out.append(eol)
+    return b"".join(out)
+
+
+def fasta_reads(reads):
+    """The records FastxReader returns for fasta(reads): a last record without bases is none."""
+    return reads[:-1] if reads and not reads[-1] else reads
+
+
 def store_of(res):
     data, offs, lens, pe, amb, _, _ = res
     return P.PackedReads.from_arrays(data, offs, lens, pe_count=pe), amb
@@ -137,3 +157,26 @@ def test_quality_side_array_follows_the_packed_bases(oracle):
     # single-file mode keeps the same pairing of bases and qualities
     d1, o1, l1, *_, q1, qo1 = oracle.pack_fastq(a, quals=True)
     assert len(q1) == int(l1.astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("seed,width,crlf,blank,lead", [(31, 0, False, False, False), (32, 60, True, True, True),
+                                                        (33, 7, False, True, False), (34, 1, True, False, True)])
+def test_fasta_pairs_match_host_mirror(oracle, seed, width, crlf, blank, lead):
+    """FastxReader's FASTA branch (packed_reads.rs:217-235): single- and multi-line records,
+    empty lines inside a sequence, empty records, CRLF, leading blank lines; an empty last
+    record is dropped.  Pinned against the host mirror fed the reads themselves."""
+    rng = np.random.default_rng(seed)
+    r1 = random_reads(rng, 300) + [b""]
+    r2 = random_reads(rng, 280)
+    a, b = fasta(r1, width, crlf, blank, lead), fasta(r2, width, crlf, blank, lead)
+    res = oracle.pack_fastq(a, b)
+    assert (res[5], res[6]) == (len(r1) - 1, len(r2))  # records read from each file
+    s, _ = store_of(res)
+    m = mirror(fasta_reads(r1), fasta_reads(r2))
+    assert s.pe_count == m.pe_count and len(s) == len(m)
+    assert all(s.get_read(i) == m.get_read(i) for i in range(len(s)))
+    # the reference test's multi-line records (tests/test_gfa_and_fastq.rs:72-90), made long
+    # enough to pass the gates: lines concatenate in order
+    text = b">seq1\n" + b"ACGT" * 6 + b"\n" + b"ACGT" * 6 + b"\n>seq2\nGGGGGGGGGG\nTTTTTTTTTT\nAAAAAAAAAA\n"
+    s, amb = store_of(oracle.pack_fastq(text))
+    assert [s.get_read(i) for i in range(len(s))] == [b"ACGT" * 12, b"G" * 10 + b"T" * 10 + b"A" * 10] and amb == 0

@@ -127,3 +127,35 @@ def test_1mb_30x_paths_agree():
     a, b = out
     assert a[:3] == b[:3]
     assert all(np.array_equal(x, y) for x, y in zip(a[3:], b[3:]))
+
+
+def test_group_over_2_31_entries_without_the_mark():
+    """Two libraries with 2^31 - 2^20-bit tiers share one first-setter array of 2^32 - 2^21
+    entries (2^18 - 2^7 partitions): the partition keys then reach bit 63, so the items
+    carry no counted-twice mark (k_fs_min writes cleared solid words, k_promote takes every
+    k-mer).  The 5 Mb / 100x reads in both libraries keep it dense (98M distinct k-mers);
+    both setter paths give identical tiers, tables and cutoffs."""
+    store, _ = synth.workload_5mb_100x(seed=3)
+    bits = (1 << 31) - (1 << 20)
+    out = []
+    for path in (0, 1):
+        say(f"case two 2^31-bit libraries setter path={path}")
+        eng = SpectrumEngine(AssemblerConfig(k=21), timing=True)
+        try:
+            eng.set_tune(setter_path=path)
+            eng.add_library(store, bloom_bits=bits)
+            eng.add_library(store, bloom_bits=bits)
+            eng.run()
+            assert sorted_launched(eng) == (path == 0)
+            res = []
+            for i in range(2):
+                info = eng.info(i)
+                res.append(((info["n_distinct"], info["n_solid"], info["cutoff"]), eng.bloom(i), eng.counts(i)))
+            out.append(res)
+        finally:
+            eng.close()
+    for (sa, ba, ca), (sb, bb, cb) in zip(*out):
+        assert sa == sb
+        assert all(np.array_equal(x, y) for x, y in zip(ba, bb))
+        assert all(np.array_equal(x, y) for x, y in zip(ca, cb))
+    assert out[0][0][0] == out[0][1][0]  # the same reads give the same summary in both libraries

@@ -174,10 +174,10 @@ int kc_priors_set(kc_ctx* ctx, const uint8_t* seqs, size_t seqs_len, const uint6
  * the three N gates (terminal trim, split on internal N into >= 21-base sub-reads, the
  * ambiguous sidecar) and 2-bit packing all run on the GPU.  r2 = NULL ingests one file
  * as append_from_file does (every read through the gates, pe_count 0).  The device
- * handles the plain 4-line FASTQ layout and FASTA (single- or multi-line records,
- * packed_reads.rs:217-235); blank lines between FASTQ records, and FASTA with
- * kc_ingest_keep_quals, return KC_EUNSUPPORTED (parse those on the host), malformed
- * records KC_EINVAL; kc_fastq_last_error() gives the message. */
+ * handles FASTQ (blank lines between records skipped where a header is due, as
+ * FastxReader does) and FASTA (single- or multi-line records, packed_reads.rs:217-235);
+ * FASTA with kc_ingest_keep_quals returns KC_EUNSUPPORTED, malformed records KC_EINVAL;
+ * kc_fastq_last_error() gives the message. */
 typedef struct kc_packed kc_packed;
 typedef struct kc_packed_info {
     uint64_t n_reads, data_len, pe_count; /* the PackedReads arrays and its paired block size */

@@ -18,10 +18,10 @@
 //      each unit's first read and first byte;
 //   5. one wavefront per input read writes its output reads' offsets, lengths and packed
 //      bytes (lane q packs output bytes q, q + 64, ...).
-// The device handles the 4-line FASTQ layout (blank or whitespace-only lines allowed before
-// the first record, blank lines after the last) and FASTA (single- or multi-line: each
-// record's sequence lines are copied together first); blank lines between FASTQ records
-// give KC_EUNSUPPORTED, for the host parser.
+// The device handles FASTQ (blank or whitespace-only lines before the first record; blank
+// lines between records, skipped where a header is due, found by a pointer-doubling chain
+// of headers) and FASTA (single- or multi-line: each record's sequence lines are copied
+// together first).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -183,17 +183,25 @@ __global__ void k_count_empty(const uint8_t* __restrict__ b, uint64_t n, const u
     }
 }
 
-// Record r = lines first + 4r .. + 3.  Writes the sequence span (CR/LF stripped) and sets
-// err bits: 1 = bad '@' header, 2 = bad '+' line, 4 = quality length, 8 = an empty
-// header line (blank lines between records); bad[0] = first bad record.
+// Record r = lines h .. h + 3, h = hdr[r] (or first + 4r without hdr).  Writes the sequence
+// span (CR/LF stripped) and sets err bits: 1 = bad '@' header, 2 = bad '+' line, 4 =
+// quality length, 8 = an empty header line (blank lines between records), 16 = the file
+// ends inside the record; bad[0] = first bad record.
 __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
-                          uint64_t m, uint64_t first, uint64_t R, uint64_t* __restrict__ seq_s,
-                          uint32_t* __restrict__ seq_l, uint64_t* __restrict__ qual_s, uint32_t* __restrict__ err,
+                          uint64_t m, uint64_t first, uint64_t R, const unsigned long long* __restrict__ hdr,
+                          uint64_t lines, uint64_t* __restrict__ seq_s, uint32_t* __restrict__ seq_l,
+                          uint64_t* __restrict__ qual_s, uint32_t* __restrict__ err,
                           unsigned long long* __restrict__ bad) {
     for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t h = hdr ? hdr[r] : first + 4 * r;
+        if (h + 3 >= lines) {  // FastxReader: unexpected EOF inside the record
+            atomicOr(err, 16u);
+            atomicMin(bad, (unsigned long long)r);
+            continue;
+        }
         uint64_t s[4], e[4];
         for (int q = 0; q < 4; ++q) {
-            line_span(nl, m, n, first + 4 * r + q, s[q], e[q]);
+            line_span(nl, m, n, h + q, s[q], e[q]);
             while (e[q] > s[q] && (b[e[q] - 1] == '\r' || b[e[q] - 1] == '\n')) --e[q];
         }
         uint32_t f = 0;
@@ -210,6 +218,61 @@ __global__ void k_records(const uint8_t* __restrict__ b, uint64_t n, const unsig
             atomicMin(bad, (unsigned long long)r);
         }
     }
+}
+
+// FASTQ with blank lines between records (FastxReader::next_seq, packed_reads.rs:154-166:
+// empty lines are skipped where a header is expected, and a record then takes the next four
+// lines whatever they hold).  The headers are the chain h0 = first, h(t+1) = nb(h(t) + 4),
+// nb(i) = the first non-empty line at or after i; it is found by pointer doubling.
+// rev[t] = line lines - 1 - t if it is not empty (after stripping CR/LF), else `lines`; a
+// minimum scan of rev gives nb.
+__global__ void k_nb_init(const uint8_t* __restrict__ b, uint64_t n, const unsigned long long* __restrict__ nl,
+                          uint64_t m, uint64_t lines, unsigned long long* __restrict__ rev) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < lines; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = lines - 1 - t;
+        uint64_t s, e;
+        line_span(nl, m, n, i, s, e);
+        while (e > s && (b[e - 1] == '\r' || b[e - 1] == '\n')) --e;
+        rev[t] = e > s ? i : lines;
+    }
+}
+
+// jump[i] = nb(i + 4) (`lines` past the end), on[i] = (i == first).
+__global__ void k_chain_init(const unsigned long long* __restrict__ nbrev, uint64_t lines, uint64_t first,
+                             unsigned long long* __restrict__ jump, uint32_t* __restrict__ on) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lines; i += (uint64_t)gridDim.x * blockDim.x) {
+        jump[i] = i + 4 < lines ? nbrev[lines - 1 - (i + 4)] : lines;
+        on[i] = i == first;
+    }
+}
+
+// One doubling round: every marked line marks its jump target (any mark is a header: it
+// is reached from a marked header), then jump2 = jump o jump.  *grew = a mark was added.
+__global__ void k_chain_mark(const unsigned long long* __restrict__ jump, uint64_t lines, uint32_t* __restrict__ on,
+                             uint32_t* __restrict__ grew) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lines; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!on[i]) continue;
+        const uint64_t j = jump[i];
+        if (j < lines && !on[j]) {
+            on[j] = 1;
+            *grew = 1;
+        }
+    }
+}
+
+__global__ void k_chain_jump(const unsigned long long* __restrict__ jump, uint64_t lines,
+                             unsigned long long* __restrict__ jump2) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lines; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j = jump[i];
+        jump2[i] = j < lines ? jump[j] : lines;
+    }
+}
+
+// hdr[pre[i]] = i for every marked line.
+__global__ void k_chain_list(const uint32_t* __restrict__ on, const unsigned long long* __restrict__ pre,
+                             uint64_t lines, unsigned long long* __restrict__ hdr) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < lines; i += (uint64_t)gridDim.x * blockDim.x)
+        if (on[i]) hdr[pre[i]] = i;
 }
 
 // FASTA (FastxReader::next_seq, packed_reads.rs:217-235): after the first '>' header, a
@@ -537,6 +600,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit(const uint8_t* __restrict__ b1, 
 struct FileIndex {
     Buf buf, tiles, tile_base, nl, red, seq_s, seq_l, qual_s, err;
     Buf fa_hdr, fa_len, fa_rec, fa_off, fa_seq;  // FASTA: line flags and lengths, their scans, the sequences
+    Buf ch_a, ch_b, ch_c, ch_h, ch_on;           // FASTQ header chain: scans, jumps, header lines, marks
     uint64_t n = 0, R = 0;
     bool fasta = false;
     const uint8_t* seqs() const { return fasta ? fa_seq.as<uint8_t>() : buf.as<uint8_t>(); }
@@ -647,6 +711,56 @@ int index_fasta(kc_ingest* c, FileIndex& F, const char* name, uint64_t m, uint64
     return KC_OK;
 }
 
+// The header lines of a FASTQ file with blank lines between records (k_nb_init ..
+// k_chain_list): F.ch_h[0 .. R).
+int fastq_chain(kc_ingest* c, FileIndex& F, uint64_t m, uint64_t first, uint64_t lines, uint64_t& R) {
+    hipStream_t st = c->st;
+    const uint64_t L = lines;
+    I_TRY(F.ch_a.ensure(L * 8 + 8));
+    I_TRY(F.ch_b.ensure(L * 8 + 8));
+    I_TRY(F.ch_c.ensure(L * 8 + 8));
+    I_TRY(F.ch_h.ensure(L * 8 + 8));
+    I_TRY(F.ch_on.ensure(L * 4 + 4));
+    I_TRY(F.err.ensure(64));
+    using ull = unsigned long long;
+    k_nb_init<<<grid_for(L), BLOCK, 0, st>>>(F.buf.as<uint8_t>(), F.n, F.nl.as<ull>(), m, L, F.ch_a.as<ull>());
+    I_TRY(hipGetLastError());
+    size_t tb = 0;
+    I_TRY(rocprim::inclusive_scan(nullptr, tb, F.ch_a.as<ull>(), F.ch_b.as<ull>(), (size_t)L, rocprim::minimum<ull>(),
+                                  st));
+    I_TRY(c->scan_tmp.ensure(tb));
+    I_TRY(rocprim::inclusive_scan(c->scan_tmp.p, tb, F.ch_a.as<ull>(), F.ch_b.as<ull>(), (size_t)L,
+                                  rocprim::minimum<ull>(), st));
+    k_chain_init<<<grid_for(L), BLOCK, 0, st>>>(F.ch_b.as<ull>(), L, first, F.ch_a.as<ull>(), F.ch_on.as<uint32_t>());
+    I_TRY(hipGetLastError());
+    ull* jump = F.ch_a.as<ull>();
+    ull* jump2 = F.ch_c.as<ull>();
+    for (;;) {  // each round doubles the chain's marked prefix; a round that adds none ends it
+        const uint32_t zero = 0;
+        uint32_t grew = 0;
+        I_TRY(hipMemcpyAsync(F.err.p, &zero, 4, hipMemcpyHostToDevice, st));
+        k_chain_mark<<<grid_for(L), BLOCK, 0, st>>>(jump, L, F.ch_on.as<uint32_t>(), F.err.as<uint32_t>());
+        I_TRY(hipGetLastError());
+        k_chain_jump<<<grid_for(L), BLOCK, 0, st>>>(jump, L, jump2);
+        I_TRY(hipGetLastError());
+        std::swap(jump, jump2);
+        I_TRY(hipMemcpyAsync(&grew, F.err.p, 4, hipMemcpyDeviceToHost, st));
+        I_TRY(hipStreamSynchronize(st));
+        if (!grew) break;
+    }
+    int rc = scan_excl(c, F.ch_on.as<uint32_t>(), F.ch_b.as<ull>(), L);
+    if (rc) return rc;
+    ull last = 0;
+    uint32_t lon = 0;
+    I_TRY(hipMemcpyAsync(&last, F.ch_b.as<ull>() + L - 1, 8, hipMemcpyDeviceToHost, st));
+    I_TRY(hipMemcpyAsync(&lon, F.ch_on.as<uint32_t>() + L - 1, 4, hipMemcpyDeviceToHost, st));
+    I_TRY(hipStreamSynchronize(st));
+    R = last + lon;
+    k_chain_list<<<grid_for(L), BLOCK, 0, st>>>(F.ch_on.as<uint32_t>(), F.ch_b.as<ull>(), L, F.ch_h.as<ull>());
+    I_TRY(hipGetLastError());
+    return KC_OK;
+}
+
 // Finds one uploaded file's records' sequence spans.
 int index_file(kc_ingest* c, FileIndex& F, const char* name) {
     hipStream_t st = c->st;
@@ -711,8 +825,8 @@ int index_file(kc_ingest* c, FileIndex& F, const char* name) {
     if (head != '@') return fail(KC_EINVAL, std::string(name) + ": expected '@' or '>' header");
     // Empty lines between the first and the last record are either legal empty sequence /
     // quality lines or blank separators, which FastxReader skips before a header.  With
-    // none, any misalignment is a malformed record; with some, a misaligned file is left
-    // to the host parser.
+    // none, records are 4-line aligned from the first header; with some, the headers are
+    // found as FastxReader finds them (fastq_chain).
     unsigned long long empties = 0;
     I_TRY(hipMemcpyAsync(F.red.p, &empties, 8, hipMemcpyHostToDevice, st));
     k_count_empty<<<grid_for(end - first), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, end,
@@ -720,37 +834,41 @@ int index_file(kc_ingest* c, FileIndex& F, const char* name) {
     I_TRY(hipGetLastError());
     I_TRY(hipMemcpyAsync(&empties, F.red.p, 8, hipMemcpyDeviceToHost, st));
     I_TRY(hipStreamSynchronize(st));
-    // The lines after `end` are all empty, so a last record whose sequence and quality are
-    // empty ends up to three lines later; the 4-line parse takes them when they exist.
-    const uint64_t end4 = first + (end - first + 3) / 4 * 4;
-    if (end4 > lines) {
-        if (empties) {
-            // empty sequence and quality lines are legal; blank separator lines are not
-            // handled on the device
-            return fail(KC_EUNSUPPORTED, std::string(name) + ": blank lines between records: parse it on the host");
-        }
-        return fail(KC_EINVAL, std::string(name) + ": malformed FASTQ: unexpected EOF inside a record");
+    const unsigned long long* hdr = nullptr;
+    uint64_t R = 0;
+    if (empties) {
+        int rc = fastq_chain(c, F, m, first, lines, R);
+        if (rc) return rc;
+        hdr = F.ch_h.as<unsigned long long>();
+    } else {
+        // The lines after `end` are all empty, so a last record whose sequence and quality
+        // are empty ends up to three lines later; the 4-line parse takes them when they exist.
+        const uint64_t end4 = first + (end - first + 3) / 4 * 4;
+        if (end4 > lines)
+            return fail(KC_EINVAL, std::string(name) + ": malformed FASTQ: unexpected EOF inside a record");
+        R = (end4 - first) / 4;
     }
-    const uint64_t R = (end4 - first) / 4;
     I_TRY(F.seq_s.ensure(R * 8 + 8));
     I_TRY(F.seq_l.ensure(R * 4 + 4));
     I_TRY(F.qual_s.ensure(R * 8 + 8));
     I_TRY(F.err.ensure(64));
     unsigned long long init[2] = {0ull, ~0ull};
     I_TRY(hipMemcpyAsync(F.err.p, init, 16, hipMemcpyHostToDevice, st));
-    k_records<<<grid_for(R), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, R, F.seq_s.as<uint64_t>(),
-                                             F.seq_l.as<uint32_t>(), F.qual_s.as<uint64_t>(), F.err.as<uint32_t>(),
-                                             F.err.as<unsigned long long>() + 1);
+    if (R)
+        k_records<<<grid_for(R), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, R, hdr, lines,
+                                                 F.seq_s.as<uint64_t>(), F.seq_l.as<uint32_t>(),
+                                                 F.qual_s.as<uint64_t>(), F.err.as<uint32_t>(),
+                                                 F.err.as<unsigned long long>() + 1);
     I_TRY(hipGetLastError());
     unsigned long long out[2];
     I_TRY(hipMemcpyAsync(out, F.err.p, 16, hipMemcpyDeviceToHost, st));
     I_TRY(hipStreamSynchronize(st));
     const uint32_t e = (uint32_t)out[0];
-    if (e && ((e & 8u) || empties))
-        return fail(KC_EUNSUPPORTED, std::string(name) + ": blank lines between records: parse it on the host");
-    if (e) {
-        const char* what = (e & 1u) ? "expected '@' header" : (e & 2u) ? "expected '+' separator line"
-                                                                        : "sequence length != quality length";
+    if (e) {  // (an empty header line, bit 8, cannot occur: headers are non-empty lines)
+        const char* what = (e & 9u)    ? "expected '@' header"
+                           : (e & 16u) ? "unexpected EOF inside the record"
+                           : (e & 2u)  ? "expected '+' separator line"
+                                       : "sequence length != quality length";
         return fail(KC_EINVAL, std::string(name) + ": malformed FASTQ record " + std::to_string(out[1]) + ": " + what);
     }
     F.R = R;

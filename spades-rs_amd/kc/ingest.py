@@ -31,10 +31,9 @@ class FastqIngest:
     are reused).  pack(r1, r2=None) takes bytes, uint8 arrays or paths (.gz decompressed
     on the host) and returns (PackedReads, info dict).  With keep_quals, info also holds
     "quals" (the Phred value of every packed base) and "qoff" (read i's at quals[qoff[i]:]),
-    the arrays SpectrumEngine.set_quals takes.  FASTA files (single- or multi-line) are
-    parsed on the device too.  Raises KcError: KC_EUNSUPPORTED for blank lines between FASTQ
-    records (PackedReads.from_paired_fastq parses those) or keep_quals with FASTA input,
-    KC_EINVAL for a malformed record."""
+    the arrays SpectrumEngine.set_quals takes.  FASTQ (with blank lines between records) and
+    FASTA (single- or multi-line) are parsed on the device.  Raises KcError:
+    KC_EUNSUPPORTED for keep_quals with FASTA input, KC_EINVAL for a malformed record."""
 
     def __init__(self, device=0, keep_quals=False):
         self._h = ctypes.c_void_p()

@@ -126,9 +126,6 @@ def test_quality_side_array_matches_oracle(oracle):
 
 
 def test_unsupported_and_malformed():
-    with pytest.raises(KcError) as e:
-        fastq_pack(fastq([b"ACGT" * 10, b"GGGG" * 10], blank_between=True), fastq([b"ACGT" * 10, b"TTTT" * 10]))
-    assert e.value.status == _abi.KC_EUNSUPPORTED
     for bad in (b"@r\nACGT\n+\nII\n", b"@r\nACGT\n-\nIIII\n", b"xr\nACGT\n+\nIIII\n", b"@r\nACGT\n+\n"):
         with pytest.raises(KcError) as e:
             fastq_pack(bad, fastq([b"ACGT"]))
@@ -159,3 +156,36 @@ def test_fasta_matches_oracle(oracle, seed, width, crlf, blank, lead):
         finally:
             ctx.close()
     assert e.value.status == _abi.KC_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("seed,crlf,n", [(51, False, 600), (52, True, 900), (53, False, 1)])
+def test_blank_lines_between_records_match_oracle(oracle, seed, crlf, n):
+    """Blank lines between FASTQ records (skipped where FastxReader expects a header): the
+    header chain found on the device by pointer doubling; records with empty sequence and
+    quality lines, runs of blank lines, leading and trailing blanks."""
+    say(f"case blank lines seed={seed} n={n}")
+    rng = np.random.default_rng(seed)
+    eol = b"\r\n" if crlf else b"\n"
+
+    def blanky(reads):
+        out = [eol]
+        for i, r in enumerate(reads):
+            out.append(eol * int(rng.integers(0, 3)))
+            out.append(b"@r%d" % i + eol + r + eol + b"+" + eol + b"I" * len(r) + eol)
+        out.append(eol * 2)
+        return b"".join(out)
+
+    r1 = random_reads(rng, n) + [b""]
+    r2 = random_reads(rng, n)
+    a, b = blanky(r1), blanky(r2)
+    info = check(oracle, a, b)
+    assert info["r1_records"] == len(r1) and info["r2_records"] == len(r2)
+    check(oracle, a)
+    check(oracle, fastq([b"ACGT" * 10, b"GGGG" * 10], blank_between=True), fastq([b"ACGT" * 10, b"TTTT" * 10]))
+    for bad in (b"@r\nACGT\n+\nIIII\n\n  \n@s\nACGT\n+\nIIII\n",  # a whitespace line where a header is due
+                b"@r\nACGT\n+\nIIII\n\n@s\nACGT\n+\n"):  # the file ends inside a record
+        with pytest.raises(KcError) as e:
+            fastq_pack(bad)
+        assert e.value.status == _abi.KC_EINVAL, bad
+        with pytest.raises(ValueError):
+            oracle.pack_fastq(bad)

@@ -286,7 +286,7 @@ typedef struct kc_tune {
                                     16384-entry partition, each partition's minima taken in
                                     LDS and written in order; taken in the dense mode with
                                     32-bit events and first-setter groups under 2^32 entries,
-                                    and otherwise the atomics below.  1 = three device-scope
+                                    outside graph capture, and otherwise the atomics below.  1 = three device-scope
                                     atomicMin per distinct k-mer */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;

@@ -2675,10 +2675,13 @@ struct Run {
 
     // Whether a dense group's first setters are taken by sorting (k_fs_items, fs_sort,
     // k_fs_min) rather than by device atomics: 32-bit events, item entries under 2^32 - FS_PART
-    // (so no partition key is all ones), and the setting allows it.
+    // (so no partition key is all ones), the setting allows it, and not for a graph capture
+    // or the eager run that plans one (rocPRIM's radix sort records memset nodes, which are
+    // kept out of graphs).
     template <typename Ev>
     bool sorted_setters(uint64_t fs_n) const {
-        return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32);
+        return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32) && !c->capturing &&
+               !c->graph_planning;
     }
 
     // First setters -> seen tier -> solid tier, one group of libraries at a time.

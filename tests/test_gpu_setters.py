@@ -72,7 +72,7 @@ def test_several_libraries_uneven_arrays(oracle, k, ec):
             if mode == "eager":
                 eng.run()
                 assert sorted_launched(eng)
-            else:
+            else:  # graphs take the atomics (no radix-sort memset nodes in a graph)
                 eng.graph_run()
             for i, (st, b) in enumerate(zip(stores, bits)):
                 d, o, l = st.arrays()

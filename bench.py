@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--table-load", type=float, default=0.0)
     ap.add_argument("--insert-path", type=int, default=0,
                     help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics, 2 partitioned")
+    ap.add_argument("--fasta", action="store_true", help="ingest: the pairs as multi-line FASTA files")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="workgroups per CU of the table insert / bin count (0: default shape)")
     ap.add_argument("--setter-path", type=int, default=0,
@@ -523,6 +524,27 @@ def bench_many_libraries(args):
     print(json.dumps(out), flush=True)
 
 
+def fasta_text(store, n_pairs, read_len, seed, width=60):
+    """The same pairs and N's as fastq_text, as two FASTA files: headers '>p<8 digits>/1'
+    and '/2', sequences in lines of `width` bases."""
+    files = []
+    for f, tag in zip(fastq_text(store, n_pairs, read_len, seed), (b"1", b"2")):
+        rows = f.reshape(n_pairs, -1)
+        seq = rows[:, 13:13 + read_len]
+        cuts = list(range(0, read_len, width))
+        out = np.empty((n_pairs, 13 + read_len + len(cuts)), dtype=np.uint8)
+        out[:, :13] = rows[:, :13]
+        out[:, 0] = ord(">")
+        o = 13
+        for c in cuts:
+            w = min(width, read_len - c)
+            out[:, o:o + w] = seq[:, c:c + w]
+            out[:, o + w] = 10
+            o += w + 1
+        files.append(out.reshape(-1))
+    return files
+
+
 def fastq_text(store, n_pairs, read_len, seed):
     """The fixed-length pairs of `store` as two FASTQ files (uint8 arrays): headers
     '@p<8 digits>/1' and '/2', Phred+33 qualities 'I', and, seeded, an internal N in 1 of
@@ -570,10 +592,11 @@ def bench_ingest(args):
     t0 = time.time()
     store, meta = synth.workload_5mb_100x(seed=2)
     n_pairs, L = meta["n_pairs"], meta["read_len"]
-    r1, r2 = fastq_text(store, n_pairs, L, seed=5)
+    fmt = "FASTA" if args.fasta else "FASTQ"
+    r1, r2 = (fasta_text if args.fasta else fastq_text)(store, n_pairs, L, seed=5)
     del store
     n_in = int(r1.size + r2.size)
-    log(f"generated FASTQ pair, {n_in / 1e9:.3f} GB, in {time.time() - t0:.1f}s")
+    log(f"generated {fmt} pair, {n_in / 1e9:.3f} GB, in {time.time() - t0:.1f}s")
     ctx = FastqIngest()
     for _ in range(args.warmup):
         ctx.pack(r1, r2)
@@ -589,12 +612,13 @@ def bench_ingest(args):
     alg = n_in + out_bytes
     log(f"device {ms_dev:.3f} ms, wall {ms_wall:.3f} ms, {info}")
     res = {
-        "metric": "FASTQ bytes/s ingested into the three-block packed layout (device-resident)",
+        "metric": f"{fmt} bytes/s ingested into the three-block packed layout (device-resident)",
         "value": round(n_in / (ms_dev / 1e3), 1), "unit": "bytes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_dev, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded)",
-        "config": {"workload": "5 Mb genome, 2x150 bp at 100x as two FASTQ files (1% internal N in R1, "
-                               "1% leading NNN in R2)", "input_bytes": n_in, "pairs": n_pairs,
+        "config": {"workload": f"5 Mb genome, 2x150 bp at 100x as two {fmt} files (1% internal N in R1, "
+                               "1% leading NNN in R2" + ("; 60-base lines)" if args.fasta else ")"),
+                   "input_bytes": n_in, "pairs": n_pairs,
                    "info": info, "host_to_host_ms": round(ms_wall, 3),
                    "host_to_host_bytes_per_s": round(n_in / (ms_wall / 1e3), 1)},
         "roofline": {"bound": "hbm", "kernel": "whole device pipeline (12 kernels + 4 scans)",
@@ -606,7 +630,7 @@ def bench_ingest(args):
         sys.path.insert(0, ROOT)
         from oracle import oracle as O
         nrec = min(200_000, n_pairs)
-        rec = 17 + 2 * L
+        rec = r1.size // n_pairs  # bytes per record (fixed-size rows)
         a, b = r1[:nrec * rec].tobytes(), r2[:nrec * rec].tobytes()
         t = time.perf_counter()
         ref = O.pack_fastq(a, b)

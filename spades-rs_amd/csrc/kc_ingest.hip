@@ -295,8 +295,9 @@ __global__ void k_fa_lines(const uint8_t* __restrict__ b, uint64_t n, const unsi
     }
 }
 
-// One wave per sequence line: its bases to out[off[i] ..].  Record r (the r-th header,
-// rec[i] = headers before line i) starts at its header's offset.
+// Sixteen lanes per sequence line (four lines a wave: FASTA lines are mostly 60-80 bytes):
+// its bases to out[off[i] ..].  Record r (the r-th header, rec[i] = headers before line i)
+// starts at its header's offset.
 __global__ __launch_bounds__(BLOCK) void k_fa_copy(const uint8_t* __restrict__ b, uint64_t n,
                                                    const unsigned long long* __restrict__ nl, uint64_t m,
                                                    uint64_t first, uint64_t nlines, const uint32_t* __restrict__ hdr,
@@ -304,17 +305,18 @@ __global__ __launch_bounds__(BLOCK) void k_fa_copy(const uint8_t* __restrict__ b
                                                    const unsigned long long* __restrict__ rec,
                                                    const unsigned long long* __restrict__ off,
                                                    uint8_t* __restrict__ out, uint64_t* __restrict__ seq_s) {
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / WAVE);
-    for (uint64_t i = blockIdx.x * (uint64_t)(BLOCK / WAVE) + threadIdx.x / WAVE; i < nlines; i += waves) {
+    constexpr uint32_t LANES = 16;
+    const uint32_t sub = threadIdx.x & (LANES - 1);
+    const uint64_t groups = (uint64_t)gridDim.x * (BLOCK / LANES);
+    for (uint64_t i = blockIdx.x * (uint64_t)(BLOCK / LANES) + threadIdx.x / LANES; i < nlines; i += groups) {
         if (hdr[i]) {
-            if (lane == 0) seq_s[rec[i]] = off[i];
+            if (sub == 0) seq_s[rec[i]] = off[i];
             continue;
         }
         uint64_t s, e;
         line_span(nl, m, n, first + i, s, e);
         const uint64_t o = off[i];
-        for (uint64_t q = lane; q < len[i]; q += WAVE) out[o + q] = b[s + q];
+        for (uint64_t q = sub; q < len[i]; q += LANES) out[o + q] = b[s + q];
     }
 }
 
@@ -691,7 +693,7 @@ int index_fasta(kc_ingest* c, FileIndex& F, const char* name, uint64_t m, uint64
     I_TRY(F.seq_s.ensure(R * 8 + 8));
     I_TRY(F.seq_l.ensure(R * 4 + 4));
     I_TRY(F.qual_s.ensure(R * 8 + 8));
-    k_fa_copy<<<grid_for(nlines * WAVE), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, nlines,
+    k_fa_copy<<<grid_for(nlines * 16), BLOCK, 0, st>>>(b, n, F.nl.as<unsigned long long>(), m, first, nlines,
                                                          F.fa_hdr.as<uint32_t>(), F.fa_len.as<uint32_t>(),
                                                          F.fa_rec.as<unsigned long long>(),
                                                          F.fa_off.as<unsigned long long>(), F.fa_seq.as<uint8_t>(),

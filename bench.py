@@ -36,69 +36,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _pmc_rows(path, counter):
-    import csv
-    rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
-    return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in rows]
-
-
-def traffic_from(fetch_csv, write_csv, kernel_ms, kernel="k_insert<"):
-    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc passes (FETCH_SIZE, then
-    WRITE_SIZE, KiB per dispatch) of `bench.py --ks K --steps 1 --warmup 0 --copy-calibration`:
-    the last dispatch of the kernel (the timed step, after any autotuning).  Each counter
-    is scaled by what it reads for the 1 GiB calibration copy."""
-    out = {}
-    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
-        rows = _pmc_rows(path, name)
-        copy = [v for n, v in rows if "copyBuffer" in n and v > 0]
-        kern = [v for n, v in rows if kernel in n]
-        if not copy or not kern:
-            raise SystemExit(f"{path}: no calibration copy or no {kernel} dispatch")
-        scale = (COPY_BYTES / 1024.0) / max(copy)  # the copy reads and writes 1 GiB once
-        out[name] = {"kib_per_dispatch": kern[-1], "dispatches": len(kern), "copy_kib": max(copy),
-                     "scale": round(scale, 4)}
-    b = sum(v["kib_per_dispatch"] * 1024 * v["scale"] for v in out.values())
-    return {"kernel": kernel, "bytes_per_launch": int(b), "kernel_ms": kernel_ms, "counters": out}
-
-
-def atomics_from(csv_path, kernel_ms, kernel="k_insert<"):
-    """Atomic requests of the last dispatch of `kernel` from a rocprofv3 --pmc pass over
-    TCC_ATOMIC_sum and TCC_EA0_ATOMIC_sum (L2 atomics, and those forwarded to the fabric)."""
-    import csv
-    vals = {}
-    for r in csv.DictReader(open(csv_path)):
-        if kernel in r["Kernel_Name"]:
-            vals.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-    rec = {c: v[-1] for c, v in vals.items()}
-    rec["per_s"] = {c: round(v / (kernel_ms / 1e3), 1) for c, v in rec.items() if not isinstance(v, dict)}
-    rec["kernel_ms"] = kernel_ms
-    return rec
-
-
-def tune_key(k, tune):
-    return f"{k}:w{tune['windows_per_thread']}:b{tune['blocks_per_cu']}:l{tune['table_load']:.2f}"
-
-
 def main():
-    if len(sys.argv) > 1 and sys.argv[1] in ("--traffic-from", "--atomics-from"):
-        # python bench.py --traffic-from KEY FETCH.csv WRITE.csv KERNEL_MS [KERNEL]
-        # python bench.py --atomics-from KEY ATOMIC.csv KERNEL_MS [KERNEL]
-        # KEY = tune_key(k, tune) as printed by bench.py ("tune_keys") for k_insert, or
-        # "<kernel>:<k>" (e.g. k_bin_count:55); KERNEL = a substring of the kernel's name
-        allrec = json.load(open(TRAFFIC_JSON)) if os.path.exists(TRAFFIC_JSON) else {}
-        key = sys.argv[2]
-        rec = allrec.get(key, {})
-        if sys.argv[1] == "--traffic-from":
-            kern = sys.argv[6] if len(sys.argv) > 6 else "k_insert<"
-            rec.update(traffic_from(sys.argv[3], sys.argv[4], float(sys.argv[5]), kern))
-        else:
-            kern = sys.argv[5] if len(sys.argv) > 5 else "k_insert<"
-            rec["atomics"] = atomics_from(sys.argv[3], float(sys.argv[4]), kern)
-        allrec[key] = rec
-        with open(TRAFFIC_JSON, "w") as f:
-            json.dump(allrec, f, indent=1, sort_keys=True)
-        print(json.dumps(rec))
-        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -118,12 +56,12 @@ def main():
     ap.add_argument("--graph", action="store_true", help="64lib: value from graph replays instead of eager runs")
     ap.add_argument("--ks", default="21,33,55")
     ap.add_argument("--bloom-bits", type=int, default=512 * 1024 * 1024)
-    ap.add_argument("--cpu-sample-reads", type=int, default=300_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-load", type=float, default=0.0)
     ap.add_argument("--insert-path", type=int, default=0,
-                    help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics, 2 partitioned")
+                    help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics")
     ap.add_argument("--fasta", action="store_true", help="ingest: the pairs as multi-line FASTA files")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="workgroups per CU of the table insert / bin count (0: default shape)")
@@ -187,8 +125,12 @@ def main():
     def step():
         return [e.run() for e in engines]
 
+    cold_ms = None
     for i in range(args.warmup):
+        c0 = time.perf_counter()
         infos = step()
+        if i == 0:
+            cold_ms = (time.perf_counter() - c0) * 1e3
         log(f"warmup {i}: " + ", ".join(f"k{k}: {inf['ms_stage2'] + inf['ms_stage3']:.1f} ms"
                                         for k, inf in zip(ks, infos)))
     torch.cuda.synchronize()
@@ -211,7 +153,11 @@ def main():
     value = n_occ / (ms_per_step / 1e3)
     log(f"steps: {', '.join(f'{x:.1f}' for x in step_ms)} ms")
 
-    # dominant kernel and its roofline
+    # dominant kernel and its roofline.  Algorithmic bytes per unit of work: one k-mer
+    # occurrence moves s_k packed-stream bytes plus one random 64 B count-table access
+    # (the counting kernels); one distinct k-mer touches 3 seen and 3 solid Bloom words
+    # (the first-setter kernels).  "irreducible_bytes" is what the kernel cannot avoid
+    # moving (the packed reads once and the distinct entries written out).
     tot = sorted(((v[0], k, name, v[1]) for (k, name), v in kernel_ms.items()), reverse=True)
     for ms, k, name, n in tot[:40]:
         log(f"  k={k:3d} {name:18s} {ms / max(n, 1):9.3f} ms/launch x{n}")
@@ -222,49 +168,34 @@ def main():
     def stream_bytes(k):
         return int(((lens + 3) // 4)[lens >= k].sum())
 
-    roof = None
-    if dom_name in ("k_insert", "k_recount"):
-        inf = per_k[dom_k]
+    inf = per_k[dom_k]
+    per_launch_s = dom_ms / dom_n / 1e3
+    slot = 16 if dom_k <= 31 else 32  # table slot bytes (one / two key words, 32-bit events)
+    setter_kernels = ("k_first_setter", "fs_sort", "k_fs_items", "k_fs_min", "k_promote", "k_seen_from_fs")
+    if dom_name in setter_kernels:
+        alg = inf["n_distinct"] * 6 * G_BYTES
+        irreducible = inf["n_distinct"] * (slot + 2 * 8)
+        unit = "distinct k-mer: 3 seen + 3 solid Bloom words (6 x 64 B)"
+    else:
         alg = stream_bytes(dom_k) + inf["n_occ"] * G_BYTES
-        per_launch_s = dom_ms / dom_n / 1e3
-        achieved = alg / per_launch_s / 1e9
-        roof = {"bound": "hbm", "kernel": f"{dom_name} (k={dom_k})", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(dom_ms / dom_n, 4)}
-        # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
-        # used only while the recorded launch time still matches this kernel's
-        rec = None
-        dom_tune = engines[ks.index(dom_k)].tune()
-        roof["tune_key"] = tune_key(dom_k, dom_tune)
-        if args.workload == "5mb_100x" and os.path.exists(TRAFFIC_JSON):
-            rec = json.load(open(TRAFFIC_JSON)).get(roof["tune_key"])
-        if rec and "bytes_per_launch" in rec:
-            if abs(rec["kernel_ms"] - dom_ms / dom_n) <= 0.1 * rec["kernel_ms"]:
-                roof["traffic"] = rec["bytes_per_launch"]
-                roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
-        if rec and "atomics" in rec:
-            roof["atomic_requests_per_s"] = rec["atomics"]["per_s"]
-    elif dom_name in ("k_bin_count", "k_first_setter"):
-        inf = per_k[dom_k]
-        per_launch_s = dom_ms / dom_n / 1e3
-        slot = 16 if dom_k <= 31 else 32  # table slot bytes (one / two key words, 32-bit events)
-        if dom_name == "k_bin_count":
-            # irreducible: the packed reads once, and the distinct entries written out; the
-            # bound is LDS atomics and instruction issue (about 3 LDS atomics per window)
-            alg = stream_bytes(dom_k) + inf["n_distinct"] * slot
-            extra = {"lds_atomic_windows_per_s": round(inf["n_occ"] / per_launch_s, 1)}
-        else:
-            # each distinct key: its slot read, and three random first-setter updates (64 B lines)
-            alg = inf["n_distinct"] * (slot + 3 * G_BYTES)
-            extra = {"atomic_requests_per_s": round(3 * inf["n_distinct"] / per_launch_s, 1)}
-        achieved = alg / per_launch_s / 1e9
-        roof = {"bound": "hbm", "kernel": f"{dom_name} (k={dom_k})", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(dom_ms / dom_n, 4), **extra}
-        rec = json.load(open(TRAFFIC_JSON)).get(f"{dom_name}:{dom_k}") if os.path.exists(TRAFFIC_JSON) else None
-        if rec and "bytes_per_launch" in rec and abs(rec["kernel_ms"] - dom_ms / dom_n) <= 0.1 * rec["kernel_ms"]:
-            roof["traffic"] = rec["bytes_per_launch"]
-            roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
+        irreducible = stream_bytes(dom_k) + inf["n_distinct"] * slot
+        unit = "k-mer occurrence: s_k packed bytes + one 64 B count-table access"
+    achieved = alg / per_launch_s / 1e9
+    roof = {"bound": "hbm", "kernel": f"{dom_name} (k={dom_k})", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": alg, "bytes_unit": unit, "avg_launch_ms": round(dom_ms / dom_n, 4),
+            "irreducible_bytes_per_launch": irreducible,
+            "irreducible_GBs": round(irreducible / per_launch_s / 1e9, 1),
+            "windows_per_s": round(inf["n_occ"] / per_launch_s, 1)}
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    # used only while the recorded launch time still matches this kernel's
+    rec = None
+    if args.workload == "5mb_100x" and os.path.exists(TRAFFIC_JSON):
+        rec = json.load(open(TRAFFIC_JSON)).get(f"{dom_name}:{dom_k}")
+    if rec and "bytes_per_launch" in rec and abs(rec["kernel_ms"] - dom_ms / dom_n) <= 0.1 * rec["kernel_ms"]:
+        roof["traffic"] = rec["bytes_per_launch"]
+        roof["traffic_over_algorithmic"] = round(rec["bytes_per_launch"] / alg, 2)
+        roof["traffic_over_irreducible"] = round(rec["bytes_per_launch"] / irreducible, 2)
     # whole-path roofline with the design-independent byte count of the path
     path_b = 0
     for k in ks:
@@ -273,6 +204,11 @@ def main():
     path = {"bytes_per_step": path_b, "achieved_GBs": round(path_b / (ms_per_step / 1e3) / 1e9, 1),
             "frac": round(path_b / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    # bit-exactness of the timed configuration: digests of every output of the last step
+    # against the sequential oracle's on the same reads (tests/golden/full_size_digests.json)
+    exact = verify_digests(args, engines, ks, infos)
+    log(f"exact: {exact}")
+
     micro = None
     if args.microbench:
         micro = microbenchmarks()
@@ -280,35 +216,23 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        n = min(args.cpu_sample_reads, len(store))
-        d, o, l = store.arrays()
-        end = int(o[n - 1]) + (int(l[n - 1]) + 3) // 4
-        cpu_occ, cpu_s = 0, 0.0
-        for k in ks:
-            S = O.Spectrum(d[:end], o[:n], l[:n], k, args.bloom_bits, False, 5.0, threads)
-            cpu_occ += S.n_occ
-            cpu_s += S.stage2_secs + S.stage3_secs
-            log(f"cpu k={k}: {S.n_occ} windows in {S.stage2_secs + S.stage3_secs:.2f}s")
-        cpu = {"value": round(cpu_occ / cpu_s, 1), "unit": "k-mers/s", "cores": threads, "kind": "port",
-               "sample": f"first {n} reads of the same store, k={args.ks}, {args.bloom_bits}-bit tiers, "
-                         f"Stage 2+3 of the oracle's multi-thread mode (reference chunking + sharded maps)"}
+        cpu = cpu_baseline(args, store, ks)
 
     out = {
         "metric": "k-mers/s counted bit-exact (Bloom tiers + sorted count table + cutoff + solid set)",
         "value": round(value, 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic (seeded)",
+        "exact": exact["all"] if exact else None,
         "config": {"workload": wdesc, "reads": len(store), "read_len": 150, "ks": ks, "bloom_bits": args.bloom_bits,
                    "n_occ": n_occ, "per_k": {str(k): {"n_occ": per_k[k]["n_occ"], "n_distinct": per_k[k]["n_distinct"],
                                                       "n_counts": per_k[k]["n_counts"], "cutoff": per_k[k]["cutoff"],
                                                       "n_solid": per_k[k]["n_solid"],
                                                       "ms": round(per_k[k]["ms_stage2"] + per_k[k]["ms_stage3"], 3)}
-                                             for k in ks}},
-        "roofline": roof, "path_roofline": path, "cpu_baseline": cpu,
+                                             for k in ks},
+                   "cold_first_step_ms": round(cold_ms, 3) if cold_ms is not None else None},
+        "roofline": roof, "path_roofline": path, "cpu_baseline": cpu, "exact_detail": exact,
         "tune": {str(k): e.tune() for k, e in zip(ks, engines)},
-        "tune_keys": [tune_key(k, e.tune()) for k, e in zip(ks, engines)],
         "microbench": micro,
     }
     if cpu:
@@ -316,6 +240,87 @@ def main():
     for e in engines:
         e.close()
     print(json.dumps(out), flush=True)
+
+
+def verify_digests(args, engines, ks, infos):
+    """Per k: the engine's last outputs against the recorded oracle digests of the same
+    workload, k and tier size; None when no digest is recorded for the configuration."""
+    from kc.digest import diff, engine_digests
+    path = os.path.join(ROOT, "tests", "golden", "full_size_digests.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    res = {}
+    for k, e, inf in zip(ks, engines, infos):
+        want = rec.get(f"{args.workload}_k{k}")
+        if not want or want["bloom_bits"] != args.bloom_bits:
+            res[str(k)] = None
+            continue
+        bad = diff(engine_digests(e, inf), want)
+        res[str(k)] = not bad
+        if bad:
+            log(f"k={k}: outputs differ from the oracle in {bad}")
+    checked = [v for v in res.values() if v is not None]
+    res["all"] = bool(checked) and all(checked) if len(checked) == len(ks) else None
+    return res
+
+
+def cpu_info():
+    """Host CPU as the run saw it: nproc, the CPUs this process may use, the model name."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def cpu_baseline(args, store, ks):
+    """The oracle's multi-thread mode (the reference's rayon chunking and sharded maps) on
+    a bounded sample of the same reads: one warm-up, then the median of 3 runs, Stage 2+3
+    only, on every CPU this process may use (at most 16, the host share of one GPU here);
+    the single-thread sequential oracle (the reference at -t 1) on a smaller sample beside it."""
+    from oracle import oracle as O
+    hw = cpu_info()
+    threads = args.cpu_threads or min(hw["usable"], 16)
+    d, o, l = store.arrays()
+
+    def sample(n):
+        n = min(n, len(store))
+        end = int(o[n - 1]) + (int(l[n - 1]) + 3) // 4
+        return n, d[:end], o[:n], l[:n]
+
+    def timed(n, th):
+        n, dd, oo, ll = sample(n)
+        occ, secs = 0, 0.0
+        for k in ks:
+            S = O.Spectrum(dd, oo, ll, k, args.bloom_bits, False, 5.0, th)
+            occ += S.n_occ
+            secs += S.stage2_secs + S.stage3_secs
+        return occ, secs
+
+    n_mt = args.cpu_sample_reads
+    timed(n_mt, threads)  # warm-up
+    runs = [timed(n_mt, threads) for _ in range(3)]
+    occ = runs[0][0]
+    med = float(np.median([s for _, s in runs]))
+    n_1 = max(n_mt // 5, 1)
+    occ1, s1 = timed(n_1, 1)
+    log(f"cpu: {threads} threads {occ} windows, runs {[round(s, 2) for _, s in runs]} s; 1 thread {occ1} in {s1:.2f} s")
+    return {"value": round(occ / med, 1), "unit": "k-mers/s", "cores": threads, "kind": "port",
+            "sample": f"first {min(n_mt, len(store))} reads of the same store, k={','.join(map(str, ks))}, "
+                      f"{args.bloom_bits}-bit tiers, Stage 2+3 of the oracle's multi-thread mode (reference "
+                      f"chunking + sharded maps); median of 3 runs after one warm-up",
+            "runs_s": [round(s, 3) for _, s in runs], "host": hw,
+            "single_thread": {"value": round(occ1 / s1, 1), "unit": "k-mers/s", "cores": 1,
+                              "sample": f"first {min(n_1, len(store))} reads, sequential oracle (-t 1 order)"}}
 
 
 def microbenchmarks(device=0):

@@ -275,12 +275,10 @@ typedef struct kc_tune {
     float table_load;            /* device hash-table target load factor (0: kc_opts.table_load) */
     uint32_t insert_path;        /* Stage 2 count table.  0 (default) or 3 = minimizer-binned:
                                     reads cut into super-k-mers, radix-sorted by bin, each bin
-                                    counted in LDS, the table packed; taken for one library,
-                                    k <= 63, no Hammer, no priors, outside graph capture, and
+                                    counted in LDS, the table packed; taken for k <= 63 and
+                                    reads shorter than 2^24 bases outside graph capture, and
                                     otherwise the table insert below.  1 = device-scope table
-                                    atomics per window.  2 = partitioned (windows routed by
-                                    hash in two levels and counted in LDS), one library,
-                                    k <= 63 */
+                                    atomics per window.  Other values give KC_EINVAL */
     uint32_t setter_path;        /* Bloom first setters.  0 (default) = sorted: three
                                     (entry, setter) items per distinct k-mer radix-sorted by
                                     16384-entry partition, each partition's minima taken in

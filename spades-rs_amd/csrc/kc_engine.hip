@@ -387,191 +387,6 @@ __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ da
     }
 }
 
-// ---------------------------------------------------------------------------
-// Partitioned Stage 2 insert (one library, k <= 63)
-// ---------------------------------------------------------------------------
-//
-// k_insert spends one device-scope atomicAdd per window on a random table slot, and the
-// device's atomic units cap that at about 27 G/s.  Here the windows are routed by the top
-// bits of their table hash instead: 256 level-1 buckets, then S sub-buckets each.  Each
-// final bucket is counted in an LDS hash table (count and first event of every distinct
-// k-mer), and only the distinct k-mers go to the global table, one claim and two plain
-// stores each (a key lives in one final bucket only, so its slot has one writer).  The
-// table then holds what k_insert leaves: every distinct key with its count and first
-// event.  A final bucket whose distinct keys do not fit the LDS table sends its records
-// through k_insert's global protocol instead.  Level 1 is fused with the HyperLogLog
-// pass: each chunk of tiles is swept twice, first for the registers and the bucket
-// sizes, then to write the records.
-
-constexpr uint32_t PART1_BINS = 256;     // level-1 buckets: the top 8 bits of the table hash
-constexpr uint32_t PART1_TILES = 16;     // tiles per level-1 chunk
-constexpr uint32_t PART2_CHUNK = 16384;  // records per level-2 chunk
-constexpr uint32_t PART2_MAX_BITS = 10;  // at most 1024 sub-buckets per level-1 bucket
-constexpr uint32_t PART3_CAP = 2048;     // LDS table slots per final bucket
-constexpr uint32_t PART3_FILL = PART3_CAP * 3 / 4;
-
-// Records (one per window): key words and event, structure of arrays.
-template <int NW, typename Ev>
-struct PartRecs {
-    uint64_t* k0;
-    uint64_t* k1;  // NW == 2 only
-    Ev* t;
-};
-
-__device__ __forceinline__ uint32_t part_sub(uint64_t h, uint32_t sbits) {
-    return sbits ? (uint32_t)(h >> (56 - sbits)) & ((1u << sbits) - 1) : 0u;
-}
-
-// Loads the first events of tile `tile`'s reads into s_ev; returns its window range.
-__device__ __forceinline__ uint32_t load_tile(uint64_t tile, const uint64_t* __restrict__ act_ev,
-                                              const uint32_t* __restrict__ tile_first, uint64_t n_active,
-                                              uint64_t n_occ, uint64_t* s_ev, uint64_t& a0, uint64_t& w0,
-                                              uint64_t& w1) {
-    a0 = tile_first[tile];
-    w0 = tile * TILE_WINDOWS;
-    w1 = min(w0 + TILE_WINDOWS, n_occ);
-    const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
-    __syncthreads();  // earlier readers of s_ev are done
-    for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
-    __syncthreads();
-    return na;
-}
-
-// Level 1 (+ HyperLogLog registers of library 0).  Chunk c covers tiles
-// [c * PART1_TILES, (c + 1) * PART1_TILES).  Bucket b holds records
-// [base1[b], base1[b] + cursor1[b]); a record past lim1[b] sets *overflow instead.
-template <int KW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_part1(const uint8_t* __restrict__ data, const uint64_t* __restrict__ act_ev,
-                                                 const uint64_t* __restrict__ act_off,
-                                                 const uint32_t* __restrict__ tile_first, uint64_t n_active,
-                                                 uint64_t n_occ, uint64_t n_tiles, uint32_t k, uint64_t ev0,
-                                                 uint32_t* __restrict__ regs, PartRecs<nw_of(KW), Ev> r1,
-                                                 const unsigned long long* __restrict__ base1,
-                                                 const unsigned long long* __restrict__ lim1,
-                                                 unsigned long long* __restrict__ cursor1,
-                                                 uint32_t* __restrict__ overflow) {
-    constexpr int NW = nw_of(KW);
-    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
-    __shared__ uint32_t s_reg[HLL_M];
-    __shared__ uint32_t s_cnt[PART1_BINS];
-    __shared__ unsigned long long s_base[PART1_BINS];
-    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) s_reg[i] = 0;
-    const uint64_t n_chunks = (n_tiles + PART1_TILES - 1) / PART1_TILES;
-    for (uint64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
-        const uint64_t t_begin = ch * PART1_TILES;
-        const uint64_t t_end = min<uint64_t>(t_begin + PART1_TILES, n_tiles);
-        __syncthreads();  // the previous chunk's sweep B is done with s_cnt / s_base
-        for (uint32_t i = threadIdx.x; i < PART1_BINS; i += blockDim.x) s_cnt[i] = 0;
-        // sweep A: registers and bucket sizes
-        for (uint64_t tile = t_begin; tile < t_end; ++tile) {
-            uint64_t a0, w0, w1;
-            const uint32_t na = load_tile(tile, act_ev, tile_first, n_active, n_occ, s_ev, a0, w0, w1);
-            for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
-                const uint32_t lo = tile_read(s_ev, na, w);
-                uint64_t key[NW];
-                pack_key<NW>(canonical<KW>(load_window<KW>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k),
-                             key);
-                const uint64_t h = table_hash<NW>(key);
-                const uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
-                const uint64_t hw = h << HLL_P;
-                const uint32_t rho = hw ? (uint32_t)__clzll((long long)hw) + 1 : 64 - HLL_P + 1;
-                atomicMax(&s_reg[idx], rho);
-                atomicAdd(&s_cnt[h >> 56], 1u);
-            }
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < PART1_BINS; i += blockDim.x) {
-            const uint32_t n = s_cnt[i];
-            s_base[i] = n ? atomicAdd(&cursor1[i], (unsigned long long)n) : 0ull;
-            s_cnt[i] = 0;
-        }
-        __syncthreads();
-        // sweep B: the same windows in the same buckets, written at base + rank
-        for (uint64_t tile = t_begin; tile < t_end; ++tile) {
-            uint64_t a0, w0, w1;
-            const uint32_t na = load_tile(tile, act_ev, tile_first, n_active, n_occ, s_ev, a0, w0, w1);
-            for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
-                const uint32_t lo = tile_read(s_ev, na, w);
-                uint64_t key[NW];
-                pack_key<NW>(canonical<KW>(load_window<KW>(data, act_off[a0 + lo], (uint32_t)(w - s_ev[lo]), k), k),
-                             key);
-                const uint32_t b = (uint32_t)(table_hash<NW>(key) >> 56);
-                const uint64_t pos = s_base[b] + atomicAdd(&s_cnt[b], 1u);
-                if (pos >= lim1[b]) {
-                    atomicOr(overflow, 1u);
-                    continue;
-                }
-                const uint64_t o = base1[b] + pos;
-                r1.k0[o] = key[0];
-                if (NW == 2) r1.k1[o] = key[NW - 1];
-                r1.t[o] = (Ev)(w - ev0);
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x)
-        if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
-}
-
-template <int NW, typename Ev>
-__device__ __forceinline__ uint64_t rec_hash(const PartRecs<NW, Ev>& r, uint64_t o, uint64_t key[NW]) {
-    key[0] = r.k0[o];
-    if (NW == 2) key[NW - 1] = r.k1[o];
-    return table_hash<NW>(key);
-}
-
-// Level 2: chunk ch holds records [chunk_start[ch], min(+PART2_CHUNK, n1[b])) of level-1
-// bucket b = chunk_bkt[ch] (from base1[b]); sub-bucket q = b * 2^sbits + sub holds records
-// [base2[q], base2[q] + cursor2[q]), at most lim2[q] of them.
-template <int NW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_part2(PartRecs<NW, Ev> r1, const unsigned long long* __restrict__ base1,
-                                                 const unsigned long long* __restrict__ n1,
-                                                 const uint32_t* __restrict__ chunk_bkt,
-                                                 const uint64_t* __restrict__ chunk_start, uint64_t n_chunks,
-                                                 uint32_t sbits, PartRecs<NW, Ev> r2,
-                                                 const unsigned long long* __restrict__ base2,
-                                                 const unsigned long long* __restrict__ lim2,
-                                                 unsigned long long* __restrict__ cursor2,
-                                                 uint32_t* __restrict__ overflow) {
-    __shared__ uint32_t s_cnt[1u << PART2_MAX_BITS];
-    __shared__ unsigned long long s_base[1u << PART2_MAX_BITS];
-    const uint32_t S = 1u << sbits;
-    for (uint64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
-        const uint32_t b = chunk_bkt[ch];
-        const uint64_t st = chunk_start[ch];
-        const uint64_t en = min<uint64_t>(st + PART2_CHUNK, n1[b]);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) s_cnt[i] = 0;
-        __syncthreads();
-        for (uint64_t i = st + threadIdx.x; i < en; i += blockDim.x) {
-            uint64_t key[NW];
-            atomicAdd(&s_cnt[part_sub(rec_hash<NW, Ev>(r1, base1[b] + i, key), sbits)], 1u);
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-            const uint32_t n = s_cnt[i];
-            s_base[i] = n ? atomicAdd(&cursor2[(uint64_t)b * S + i], (unsigned long long)n) : 0ull;
-            s_cnt[i] = 0;
-        }
-        __syncthreads();
-        for (uint64_t i = st + threadIdx.x; i < en; i += blockDim.x) {
-            const uint64_t src = base1[b] + i;
-            uint64_t key[NW];
-            const uint32_t sub = part_sub(rec_hash<NW, Ev>(r1, src, key), sbits);
-            const uint64_t q = (uint64_t)b * S + sub;
-            const uint64_t pos = s_base[sub] + atomicAdd(&s_cnt[sub], 1u);
-            if (pos >= lim2[q]) {
-                atomicOr(overflow, 1u);
-                continue;
-            }
-            const uint64_t o = base2[q] + pos;
-            r2.k0[o] = key[0];
-            if (NW == 2) r2.k1[o] = key[NW - 1];
-            r2.t[o] = r1.t[src];
-        }
-    }
-}
-
 // One 64-bit key word of an LDS slot: EMPTY -> v exactly once.  `claimed` is set when this
 // call made the change.
 __device__ __forceinline__ bool lds_claim_word(uint64_t* w, uint64_t v, bool& claimed) {
@@ -585,95 +400,6 @@ __device__ __forceinline__ bool lds_claim_word(uint64_t* w, uint64_t v, bool& cl
         }
     }
     return cur == v;
-}
-
-// Level 3: final bucket q (records [base2[q], + n2[q])) counted in LDS, then its distinct
-// keys claimed in the global table of library L with their count and first event.
-template <int NW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_part3(PartRecs<NW, Ev> r2, const unsigned long long* __restrict__ base2,
-                                                 const unsigned long long* __restrict__ n2, uint64_t nq,
-                                                 Slot<NW, Ev>* __restrict__ tab, const LibDesc* __restrict__ libs,
-                                                 uint32_t max_probe, LibOut* __restrict__ out) {
-    constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
-    __shared__ uint64_t s_k0[PART3_CAP];
-    __shared__ uint64_t s_k1[NW == 2 ? PART3_CAP : 1];
-    __shared__ uint32_t s_cnt[PART3_CAP];
-    __shared__ Ev s_t0[PART3_CAP];
-    __shared__ uint32_t s_fill, s_over;
-    const LibDesc& L = libs[0];
-    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x) {
-        const uint64_t n = n2[q];
-        if (n == 0) continue;
-        const uint64_t beg = base2[q];
-        __syncthreads();  // the previous bucket's LDS reads are done
-        for (uint32_t i = threadIdx.x; i < PART3_CAP; i += blockDim.x) {
-            s_k0[i] = EMPTY_KEY;
-            if (NW == 2) s_k1[i] = EMPTY_KEY;
-            s_cnt[i] = 0;
-            s_t0[i] = (Ev)~(Ev)0;
-        }
-        if (threadIdx.x == 0) {
-            s_fill = 0;
-            s_over = 0;
-        }
-        __syncthreads();
-        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            uint64_t key[NW];
-            const uint64_t h = rec_hash<NW, Ev>(r2, beg + i, key);
-            const Ev t = r2.t[beg + i];
-            uint32_t slot = (uint32_t)h & (PART3_CAP - 1);
-            bool done = false;
-            for (uint32_t probe = 0; probe < PART3_CAP; ++probe) {
-                bool claimed = false;
-                bool m = lds_claim_word(&s_k0[slot], key[0], claimed);
-                if (claimed && atomicAdd(&s_fill, 1u) + 1 > PART3_FILL) atomicOr(&s_over, 1u);
-                if (m && NW == 2) {
-                    bool c2 = false;
-                    m = lds_claim_word(&s_k1[slot], key[NW - 1], c2);
-                }
-                if (m) {
-                    atomicAdd(&s_cnt[slot], 1u);
-                    atomicMin(&s_t0[slot], t);
-                    done = true;
-                    break;
-                }
-                slot = (slot + 1) & (PART3_CAP - 1);
-            }
-            if (!done) atomicOr(&s_over, 1u);
-        }
-        __syncthreads();
-        if (s_over) {
-            // too many distinct keys for LDS: k_insert's global protocol for every record
-            for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-                uint64_t key[NW], w0[SPB];
-                Ev t0[SPB];
-                key[0] = r2.k0[beg + i];
-                if (NW == 2) key[NW - 1] = r2.k1[beg + i];
-                const uint64_t b = L.bucket0 + home_bucket<NW>(key, L.nbuckets);
-                bucket_load<NW, Ev>(tab, b, w0, t0);
-                if (!table_insert_from<NW, Ev, true>(tab, L.bucket0, L.nbuckets, b, w0, t0, key, r2.t[beg + i],
-                                                     max_probe))
-                    atomicOr(&out[0].overflow, 1u);
-            }
-            continue;
-        }
-        for (uint32_t sl = threadIdx.x; sl < PART3_CAP; sl += blockDim.x) {
-            if (s_k0[sl] == EMPTY_KEY) continue;
-            uint64_t key[NW], w0[SPB];
-            key[0] = s_k0[sl];
-            if (NW == 2) key[NW - 1] = s_k1[sl];
-            const uint64_t b = L.bucket0 + home_bucket<NW>(key, L.nbuckets);
-            bucket_load<NW, Ev>(tab, b, w0);
-            const uint64_t i = table_find_or_claim<NW, Ev>(tab, L.bucket0, L.nbuckets, b, w0, key, max_probe);
-            if (i == ~0ull) {
-                atomicOr(&out[0].overflow, 1u);
-                continue;
-            }
-            tab[i].cnt = s_cnt[sl];  // the key's only writer: its keys live in this bucket alone
-            tab[i].t0 = s_t0[sl];
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1800,12 +1526,12 @@ __global__ void __launch_bounds__(BLOCK) k_solid_flags(const uint64_t* __restric
 enum KernelId {
     KID_NWIN, KID_SCAN, KID_SELECT, KID_ACTIVE, KID_HLL, KID_CLEAR, KID_INSERT, KID_FS_CLEAR, KID_FIRST_SETTER,
     KID_SEEN, KID_PROMOTE, KID_HAMMER, KID_RECOUNT, KID_PRIOR, KID_COMPACT, KID_SORT, KID_HIST, KID_FLAGS,
-    KID_PART1, KID_PART2, KID_PART3, KID_SK, KID_BIN_SORT, KID_BIN_COUNT, KID_FS_ITEMS, KID_FS_SORT, KID_FS_MIN, KID_COUNT
+    KID_SK, KID_BIN_SORT, KID_BIN_COUNT, KID_FS_ITEMS, KID_FS_SORT, KID_FS_MIN, KID_COUNT
 };
 const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_nwin", "scan_windows", "select_active", "k_active", "k_hll", "k_table_clear", "k_insert", "k_fs_clear",
     "k_first_setter", "k_seen_from_fs", "k_promote", "k_hammer", "k_recount", "k_prior", "k_compact", "radix_sort",
-    "k_count_hist", "k_solid_flags", "k_part1", "k_part2", "k_part3", "k_superkmers", "bin_sort", "k_bin_count",
+    "k_count_hist", "k_solid_flags", "k_superkmers", "bin_sort", "k_bin_count",
     "k_fs_items", "fs_sort", "k_fs_min"};
 
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
@@ -1873,16 +1599,12 @@ struct kc_ctx {
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
     DevBuf d_wide, d_wide2;  // k > 63: count-table entries as (V, library) records
     // partitioned Stage 2 insert: level-1 / level-2 records, cursors, level-2 chunks
-    DevBuf d_p1k0, d_p1k1, d_p1t, d_p2k0, d_p2k1, d_p2t, d_cur1, d_cur2, d_pchunk_b, d_pchunk_s;
-    DevBuf d_base1, d_lim1, d_base2, d_lim2;
-    bool last_partitioned = false;  // the last Stage 2 used the partitioned insert
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
-    uint64_t p_nq = 0;
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
@@ -2155,154 +1877,6 @@ struct Run {
             if (E <= 2.5 * m && zeros) E = m * std::log(m / zeros);
             c->libs[l].est = (uint64_t)std::ceil(E);
         }
-        return KC_OK;
-    }
-
-    // Whether Stage 2 takes the partitioned insert: only when the tuning asks for it
-    // (insert_path 2), with one library and k <= 63.  On the 5 Mb / 100x workload it is
-    // slower than the per-window table atomics (its two scatter levels cost more than the
-    // atomics they remove) and than the binned count.
-    bool use_partition() const {
-        return L() == 1 && c->kw != 4 && !c->capturing && c->tune.insert_path == 2;
-    }
-
-    // Levels 1 and 2 of the partitioned insert (and the distinct-count estimate); leaves
-    // the final buckets in d_p2* (from d_base2) with their sizes in d_cur2.  Bucket regions
-    // start uniform with some headroom; a region that overflows is re-run with the exact
-    // sizes the cursors counted.
-    template <int KW, typename Ev>
-    int partition() {
-        constexpr int NW = nw_of(KW);
-        hipStream_t st = c->st;
-        const uint64_t n_occ = c->n_occ;
-        unsigned long long* ctr = c->d_counters.as<unsigned long long>();
-        auto recs = [&](DevBuf& k0, DevBuf& k1, DevBuf& t) {
-            return PartRecs<NW, Ev>{k0.as<uint64_t>(), NW == 2 ? k1.as<uint64_t>() : nullptr, t.as<Ev>()};
-        };
-        // regions: base[i], lim[i] for each bucket; total records the buffers hold
-        auto uniform = [](std::vector<unsigned long long>& base, std::vector<unsigned long long>& lim, uint64_t cap) {
-            for (size_t i = 0; i < base.size(); ++i) {
-                base[i] = i * cap;
-                lim[i] = cap;
-            }
-            return (uint64_t)base.size() * cap;
-        };
-        auto exact = [](std::vector<unsigned long long>& base, std::vector<unsigned long long>& lim,
-                        const std::vector<unsigned long long>& n) {
-            uint64_t o = 0;
-            for (size_t i = 0; i < base.size(); ++i) {
-                base[i] = o;
-                lim[i] = n[i];
-                o += n[i];
-            }
-            return o;
-        };
-        auto put = [&](DevBuf& d, const std::vector<unsigned long long>& v) -> int {
-            KC_TRY(d.ensure(v.size() * 8 + 8));
-            KC_TRY(hipMemcpyAsync(d.p, v.data(), v.size() * 8, hipMemcpyHostToDevice, st));
-            return KC_OK;
-        };
-        // ---- level 1 + registers ----
-        std::vector<unsigned long long> n1(PART1_BINS), base1(PART1_BINS), lim1(PART1_BINS);
-        uint64_t total1 = uniform(base1, lim1, (uint64_t)std::ceil((double)n_occ / PART1_BINS * 1.08) + 4096);
-        KC_TRY(c->d_hll.ensure(HLL_M * 4));
-        KC_TRY(c->d_cur1.ensure(PART1_BINS * 8));
-        for (int attempt = 0;; ++attempt) {
-            KC_TRY(c->d_p1k0.ensure(total1 * 8 + 8));
-            if (NW == 2) KC_TRY(c->d_p1k1.ensure(total1 * 8 + 8));
-            KC_TRY(c->d_p1t.ensure(total1 * sizeof(Ev) + 8));
-            int rc = put(c->d_base1, base1);
-            if (!rc) rc = put(c->d_lim1, lim1);
-            if (rc) return rc;
-            k_fill<uint32_t><<<grid_for(HLL_M), BLOCK, 0, st>>>(c->d_hll.as<uint32_t>(), HLL_M, 0u);
-            k_fill<unsigned long long><<<1, PART1_BINS, 0, st>>>(c->d_cur1.as<unsigned long long>(), PART1_BINS, 0ull);
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-            KC_TRY(hipGetLastError());
-            const uint64_t n_chunks = (c->n_tiles + PART1_TILES - 1) / PART1_TILES;
-            const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_chunks, (uint64_t)c->n_cu * 8));
-            TIMED(KID_PART1, (k_part1<KW, Ev><<<g, BLOCK, 0, st>>>(
-                                 c->d_data.as<uint8_t>(), c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(),
-                                 c->d_tile_first.as<uint32_t>(), c->n_active, n_occ, c->n_tiles, c->o.k,
-                                 lib_ev0_[0], c->d_hll.as<uint32_t>(), recs(c->d_p1k0, c->d_p1k1, c->d_p1t),
-                                 c->d_base1.as<unsigned long long>(), c->d_lim1.as<unsigned long long>(),
-                                 c->d_cur1.as<unsigned long long>(), c->d_counters.as<uint32_t>())));
-            KC_TRY(hipMemcpyAsync(n1.data(), c->d_cur1.p, PART1_BINS * 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            if (!(c->h_stage[0] & 0xFFFFFFFFull)) break;
-            if (attempt >= 1) return fail(KC_ENOMEM, "level-1 partition overflow");
-            total1 = exact(base1, lim1, n1);  // the cursors counted every record
-        }
-        int rc = hll_read();
-        if (rc) return rc;
-        // ---- level 2: final buckets of a few hundred distinct k-mers ----
-        uint32_t sbits = 0;
-        while (sbits < PART2_MAX_BITS && (double)c->libs[0].est / (double)(PART1_BINS << sbits) > 600.0) ++sbits;
-        const uint64_t S = 1ull << sbits, nq = PART1_BINS * S;
-        std::vector<uint32_t> cb;
-        std::vector<uint64_t> cs;
-        uint64_t n1max = 0;
-        for (uint32_t b = 0; b < PART1_BINS; ++b) {
-            n1max = std::max<uint64_t>(n1max, n1[b]);
-            for (uint64_t s0 = 0; s0 < n1[b]; s0 += PART2_CHUNK) {
-                cb.push_back(b);
-                cs.push_back(s0);
-            }
-        }
-        const uint64_t n_chunks = cb.size();
-        KC_TRY(c->d_pchunk_b.ensure(n_chunks * 4 + 4));
-        KC_TRY(c->d_pchunk_s.ensure(n_chunks * 8 + 8));
-        if (n_chunks) {
-            KC_TRY(hipMemcpyAsync(c->d_pchunk_b.p, cb.data(), n_chunks * 4, hipMemcpyHostToDevice, st));
-            KC_TRY(hipMemcpyAsync(c->d_pchunk_s.p, cs.data(), n_chunks * 8, hipMemcpyHostToDevice, st));
-        }
-        std::vector<unsigned long long> n2(nq), base2(nq), lim2(nq);
-        uint64_t total2 = uniform(base2, lim2, (uint64_t)std::ceil((double)n1max / (double)S * 1.25) + 256);
-        KC_TRY(c->d_cur2.ensure(nq * 8));
-        for (int attempt = 0;; ++attempt) {
-            KC_TRY(c->d_p2k0.ensure(total2 * 8 + 8));
-            if (NW == 2) KC_TRY(c->d_p2k1.ensure(total2 * 8 + 8));
-            KC_TRY(c->d_p2t.ensure(total2 * sizeof(Ev) + 8));
-            rc = put(c->d_base2, base2);
-            if (!rc) rc = put(c->d_lim2, lim2);
-            if (rc) return rc;
-            k_fill<unsigned long long><<<grid_for(nq), BLOCK, 0, st>>>(c->d_cur2.as<unsigned long long>(), nq, 0ull);
-            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
-            KC_TRY(hipGetLastError());
-            if (n_chunks) {
-                const uint32_t g = (uint32_t)std::min<uint64_t>(n_chunks, (uint64_t)c->n_cu * 8);
-                TIMED(KID_PART2, (k_part2<NW, Ev><<<g, BLOCK, 0, st>>>(
-                                     recs(c->d_p1k0, c->d_p1k1, c->d_p1t), c->d_base1.as<unsigned long long>(),
-                                     c->d_cur1.as<unsigned long long>(), c->d_pchunk_b.as<uint32_t>(),
-                                     c->d_pchunk_s.as<uint64_t>(), n_chunks, sbits,
-                                     recs(c->d_p2k0, c->d_p2k1, c->d_p2t), c->d_base2.as<unsigned long long>(),
-                                     c->d_lim2.as<unsigned long long>(), c->d_cur2.as<unsigned long long>(),
-                                     c->d_counters.as<uint32_t>())));
-            }
-            KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            if (!(c->h_stage[0] & 0xFFFFFFFFull)) break;
-            if (attempt >= 1) return fail(KC_ENOMEM, "level-2 partition overflow");
-            KC_TRY(hipMemcpyAsync(n2.data(), c->d_cur2.p, nq * 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            total2 = exact(base2, lim2, n2);
-        }
-        c->p_nq = nq;
-        return KC_OK;
-    }
-
-    // Level 3 into the (cleared) table.
-    template <int KW, typename Ev>
-    int partition_insert() {
-        constexpr int NW = nw_of(KW);
-        const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c->p_nq, (uint64_t)c->n_cu * 8));
-        hipStream_t st = c->st;
-        TIMED(KID_PART3, (k_part3<NW, Ev><<<g, BLOCK, 0, st>>>(
-                             PartRecs<NW, Ev>{c->d_p2k0.as<uint64_t>(), NW == 2 ? c->d_p2k1.as<uint64_t>() : nullptr,
-                                              c->d_p2t.as<Ev>()},
-                             c->d_base2.as<unsigned long long>(), c->d_cur2.as<unsigned long long>(), c->p_nq,
-                             c->d_tab.as<Slot<NW, Ev>>(),
-                             c->d_libs.as<LibDesc>(), 1u << 20, c->d_out.as<LibOut>())));
         return KC_OK;
     }
 
@@ -2633,7 +2207,6 @@ struct Run {
                 if (rc) return rc;
                 rc = binned_count<KW, Ev>();
                 if (rc == KC_OK) {
-                    c->last_partitioned = false;
                     c->last_binned = true;
                     rc = bloom_groups<NW, Ev>(true);
                     if (rc) return rc;
@@ -2642,9 +2215,7 @@ struct Run {
                 if (rc != BIN_FALLBACK) return rc;
             }
         }
-        const bool part = KW <= 2 && use_partition();
-        c->last_partitioned = part;
-        int rc = part ? partition<KW, Ev>() : estimate_distinct<KW>();
+        int rc = estimate_distinct<KW>();
         if (rc) return rc;
         std::vector<uint32_t> grow(L(), 0);
         for (int attempt = 0;; ++attempt) {
@@ -2655,7 +2226,7 @@ struct Run {
             const uint64_t n_slots = c->total_buckets * SPB;
             TIMED(KID_CLEAR, (k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(
                                  c->d_tab.as<Slot<NW, Ev>>(), n_slots)));
-            rc = part ? partition_insert<KW, Ev>() : launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+            rc = launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
             if (rc) return rc;
             rc = read_out();
             if (rc) return rc;
@@ -3309,10 +2880,7 @@ void kc_destroy(kc_ctx* ctx) {
     ctx->d_gk2.release();
     DevBuf* pbufs[] = {&ctx->d_pseq,  &ctx->d_pstart, &ctx->d_pw0,  &ctx->d_plib, &ctx->d_psum,
                        &ctx->d_pflag, &ctx->d_quals,  &ctx->d_rqoff, &ctx->d_aqoff, &ctx->d_qsum,
-                       &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab,   &ctx->d_wide, &ctx->d_wide2,
-                       &ctx->d_p1k0,  &ctx->d_p1k1,   &ctx->d_p1t,   &ctx->d_p2k0, &ctx->d_p2k1,
-                       &ctx->d_p2t,   &ctx->d_cur1,   &ctx->d_cur2,  &ctx->d_pchunk_b, &ctx->d_pchunk_s,
-                       &ctx->d_base1, &ctx->d_lim1,   &ctx->d_base2, &ctx->d_lim2};
+                       &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab,   &ctx->d_wide, &ctx->d_wide2};
     for (DevBuf* b : pbufs) b->release();
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
@@ -3942,7 +3510,8 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->blocks_per_cu > 1024) return ctx->fail(KC_EINVAL, "blocks_per_cu must be at most 1024");
     if (tune->table_load != 0.0f && !(tune->table_load > 0.05f && tune->table_load < 0.95f))
         return ctx->fail(KC_EINVAL, "table_load must be 0 or in (0.05, 0.95)");
-    if (tune->insert_path > 3) return ctx->fail(KC_EINVAL, "insert_path must be 0, 1, 2 or 3");
+    if (tune->insert_path > 3 || tune->insert_path == 2)
+        return ctx->fail(KC_EINVAL, "insert_path must be 0, 1 or 3 (the partitioned count, 2, was removed)");
     if (tune->setter_path > 1) return ctx->fail(KC_EINVAL, "setter_path must be 0 or 1");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor

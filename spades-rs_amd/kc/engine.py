@@ -284,10 +284,9 @@ class SpectrumEngine:
         return t.as_dict()
 
     def set_tune(self, windows_per_thread=1, blocks_per_cu=0, table_load=0.0, insert_path=0, setter_path=0):
-        """insert_path: 0 (default) or 3 minimizer-binned count where it applies (one library,
-        k <= 63, no Hammer, no priors), else per-window table atomics; 1 per-window table
-        atomics; 2 partitioned (LDS) count.  setter_path: 0 (default) Bloom first setters by
-        sorting (entry, setter) items by partition and taking minima in LDS where it applies
+        """insert_path: 0 (default) or 3 minimizer-binned count where it applies (k <= 63,
+        reads under 2^24 bases, no graph capture), else per-window table atomics; 1 per-window
+        table atomics.  setter_path: 0 (default) Bloom first setters by sorting (entry, setter) items by partition and taking minima in LDS where it applies
         (dense mode, 32-bit events), else device atomics; 1 device atomics."""
         t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
                          table_load=float(table_load), insert_path=insert_path, setter_path=setter_path)

@@ -27,7 +27,7 @@ def sorted_launched(eng):
 @pytest.mark.parametrize("k,ec,bits,ip", [(21, False, 1 << 20, 0), (33, False, (1 << 18) + 64, 0),
                                           (55, True, 4096, 0), (31, False, 64, 0), (21, True, 1 << 16, 1),
                                           (77, False, (1 << 17) + 192, 0), (127, True, 1 << 18, 0),
-                                          (45, False, 1 << 19, 2)])
+                                          (45, False, 1 << 19, 1)])
 def test_sorted_setters_match_atomics_and_oracle(oracle, k, ec, bits, ip):
     store, _ = synth.uniform_genome_pairs(seed=900 + k, genome_size=20000, n_pairs=2000, read_len=150,
                                           p_start=0.005)

@@ -55,6 +55,8 @@ def main():
                     help="skip the random-atomic / copy microbenchmarks")
     ap.add_argument("--graph", action="store_true", help="64lib: value from graph replays instead of eager runs")
     ap.add_argument("--ks", default="21,33,55")
+    ap.add_argument("--error-correct", action="store_true",
+                    help="run Hammer before counting (AssemblerConfig::error_correct)")
     ap.add_argument("--bloom-bits", type=int, default=512 * 1024 * 1024)
     ap.add_argument("--cpu-sample-reads", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -112,8 +114,8 @@ def main():
 
     engines = []
     for k in ks:
-        eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits), table_load=args.table_load,
-                             timing=True)
+        eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=args.bloom_bits, error_correct=args.error_correct),
+                             table_load=args.table_load, timing=True)
         if args.insert_path or args.setter_path or args.blocks_per_cu:
             eng.set_tune(insert_path=args.insert_path, setter_path=args.setter_path, blocks_per_cu=args.blocks_per_cu)
         eng.upload(store)
@@ -172,7 +174,12 @@ def main():
     per_launch_s = dom_ms / dom_n / 1e3
     slot = 16 if dom_k <= 31 else 32  # table slot bytes (one / two key words, 32-bit events)
     setter_kernels = ("k_first_setter", "fs_sort", "k_fs_items", "k_fs_min", "k_promote", "k_seen_from_fs")
-    if dom_name in setter_kernels:
+    if dom_name == "k_hammer":
+        # one random 64 B access of the solid tier per is_solid test (at least its first word)
+        alg = inf["hammer_queries"] * G_BYTES
+        irreducible = stream_bytes(dom_k) * 2
+        unit = "is_solid test: one random 64 B solid-tier access"
+    elif dom_name in setter_kernels:
         alg = inf["n_distinct"] * 6 * G_BYTES
         irreducible = inf["n_distinct"] * (slot + 2 * 8)
         unit = "distinct k-mer: 3 seen + 3 solid Bloom words (6 x 64 B)"
@@ -187,6 +194,13 @@ def main():
             "irreducible_bytes_per_launch": irreducible,
             "irreducible_GBs": round(irreducible / per_launch_s / 1e9, 1),
             "windows_per_s": round(inf["n_occ"] / per_launch_s, 1)}
+    if args.error_correct:
+        hk = kernel_ms.get((dom_k, "k_hammer"))
+        if hk:
+            roof["hammer"] = {"k": dom_k, "ms_per_launch": round(hk[0] / hk[1], 4),
+                              "queries_per_launch": inf["hammer_queries"],
+                              "queries_per_s": round(inf["hammer_queries"] / (hk[0] / hk[1] / 1e3), 1),
+                              "corrected_reads": inf["corrected_reads"]}
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated),
     # used only while the recorded launch time still matches this kernel's
     rec = None
@@ -225,6 +239,7 @@ def main():
         "dtype": "u64", "data": "synthetic (seeded)",
         "exact": exact["all"] if exact else None,
         "config": {"workload": wdesc, "reads": len(store), "read_len": 150, "ks": ks, "bloom_bits": args.bloom_bits,
+                   "error_correct": bool(args.error_correct),
                    "n_occ": n_occ, "per_k": {str(k): {"n_occ": per_k[k]["n_occ"], "n_distinct": per_k[k]["n_distinct"],
                                                       "n_counts": per_k[k]["n_counts"], "cutoff": per_k[k]["cutoff"],
                                                       "n_solid": per_k[k]["n_solid"],
@@ -252,11 +267,11 @@ def verify_digests(args, engines, ks, infos):
     rec = json.load(open(path))
     res = {}
     for k, e, inf in zip(ks, engines, infos):
-        want = rec.get(f"{args.workload}_k{k}")
+        want = rec.get(f"{args.workload}_k{k}" + ("_hammer" if args.error_correct else ""))
         if not want or want["bloom_bits"] != args.bloom_bits:
             res[str(k)] = None
             continue
-        bad = diff(engine_digests(e, inf), want)
+        bad = diff(engine_digests(e, inf, with_reads=args.error_correct), want)
         res[str(k)] = not bad
         if bad:
             log(f"k={k}: outputs differ from the oracle in {bad}")
@@ -301,7 +316,7 @@ def cpu_baseline(args, store, ks):
         n, dd, oo, ll = sample(n)
         occ, secs = 0, 0.0
         for k in ks:
-            S = O.Spectrum(dd, oo, ll, k, args.bloom_bits, False, 5.0, th)
+            S = O.Spectrum(dd, oo, ll, k, args.bloom_bits, bool(args.error_correct), 5.0, th)
             occ += S.n_occ
             secs += S.stage2_secs + S.stage3_secs
         return occ, secs
@@ -316,7 +331,8 @@ def cpu_baseline(args, store, ks):
     log(f"cpu: {threads} threads {occ} windows, runs {[round(s, 2) for _, s in runs]} s; 1 thread {occ1} in {s1:.2f} s")
     return {"value": round(occ / med, 1), "unit": "k-mers/s", "cores": threads, "kind": "port",
             "sample": f"first {min(n_mt, len(store))} reads of the same store, k={','.join(map(str, ks))}, "
-                      f"{args.bloom_bits}-bit tiers, Stage 2+3 of the oracle's multi-thread mode (reference "
+                      f"{args.bloom_bits}-bit tiers, Hammer {'on' if args.error_correct else 'off'}, Stage 2+3 of the "
+                      f"oracle's multi-thread mode (reference "
                       f"chunking + sharded maps); median of 3 runs after one warm-up",
             "runs_s": [round(s, 3) for _, s in runs], "host": hw,
             "single_thread": {"value": round(occ1 / s1, 1), "unit": "k-mers/s", "cores": 1,

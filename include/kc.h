@@ -73,6 +73,8 @@ typedef struct kc_info {
     uint32_t top_cov;          /* 95th-percentile count used by the cutoff, capped at 1023 */
     double ms_stage2;          /* device time of the Bloom build */
     double ms_stage3;          /* device time of correction + counting + sort + cutoff */
+    uint64_t hammer_queries;   /* is_solid tests Hammer made in the last kc_count (error_correct;
+                                  all libraries of the context) */
 } kc_info;
 
 /* ---- Standalone two-tier filter (bloom.rs) ------------------------------- */
@@ -93,8 +95,7 @@ int kc_filter_is_solid_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, ui
 int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words);
 /* ErrorCorrector::new(k).correct_read_in_place (hammer.rs:28-120) for every read of a
  * PackedReads-layout store against this filter; `out` receives the corrected store (same
- * offsets/lengths), n_corrected the number of reads modified (hammer.rs:123-143).
- * Reads of at most 1024 bases. */
+ * offsets/lengths), n_corrected the number of reads modified (hammer.rs:123-143). */
 int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_t data_len, const uint64_t* offsets,
                             const uint32_t* lengths, size_t n_reads, uint8_t* out, uint64_t* n_corrected);
 

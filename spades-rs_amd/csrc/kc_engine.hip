@@ -197,6 +197,8 @@ struct LibAccum {
 };
 
 constexpr size_t LIBOUT_STRIDE = sizeof(LibOut) / sizeof(unsigned long long);
+// Counter slot (of the context's 64) receiving the number of is_solid tests Hammer makes.
+constexpr uint32_t HAMMER_QUERIES = 20;
 static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
@@ -2515,13 +2517,13 @@ struct Run {
             // corr = data; Hammer writes the reads it modifies
             KC_TRY(c->d_corr.ensure(c->d_data.bytes));
             KC_TRY(hipMemcpyAsync(c->d_corr.p, c->d_data.p, c->d_data.bytes, hipMemcpyDeviceToDevice, st));
+            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr + HAMMER_QUERIES, 1, 0ull);
             if (c->n_active) {
-                TIMED(KID_HAMMER, (k_hammer<KW><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
+                TIMED(KID_HAMMER, (k_hammer<KW><<<hammer_grid(c->n_active), 64, 0, st>>>(
                                       c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(),
                                       c->n_active, c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(),
                                       c->d_rlib.as<uint32_t>(), c->d_blooms.as<LibBloom>(), c->o.k,
-                                      &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
-                                      c->d_counters.as<uint32_t>() + 6)));
+                                      &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE, ctr + HAMMER_QUERIES)));
             }
             if (c->last_binned) {  // the recount hashes: regions sized for the first count's keys
                 rc = hashed_layout<NW, uint32_t>(0);
@@ -2538,10 +2540,6 @@ struct Run {
             }
             rc = read_out();
             if (rc) return rc;
-            KC_TRY(hipMemcpyAsync(c->h_stage, c->d_counters.p, 32, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            if (c->h_stage[3] & 0xFFFFFFFFull)
-                return fail(KC_EUNSUPPORTED, "error_correct supports reads of at most 1024 bases");
             for (uint32_t l = 0; l < L(); ++l)
                 if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
                     return fail(KC_ENOMEM, "device hash table overflow (recount)");
@@ -2635,12 +2633,13 @@ struct Run {
         if (ec) {
             k_copy_bytes<<<grid_for(c->d_data.bytes / 8 + 1), BLOCK, 0, st>>>(c->d_data.as<uint8_t>(),
                                                                              c->d_corr.as<uint8_t>(), c->d_data.bytes);
+            k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr + HAMMER_QUERIES, 1, 0ull);
             if (c->n_active)
-                k_hammer<KW><<<grid_for(c->n_active, 64, 65536), 64, 0, st>>>(
+                k_hammer<KW><<<hammer_grid(c->n_active), 64, 0, st>>>(
                     c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(), c->n_active,
                     c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(), c->d_rlib.as<uint32_t>(),
                     c->d_blooms.as<LibBloom>(), c->o.k, &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
-                    c->d_counters.as<uint32_t>() + 6);
+                    ctr + HAMMER_QUERIES);
             const uint64_t n_slots32 = c->total_buckets * slots_per_bucket<NW, uint32_t>();
             k_table_clear<NW, uint32_t><<<grid_for(n_slots32), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, uint32_t>>(),
                                                                               n_slots32);
@@ -3063,11 +3062,15 @@ int kc_count(kc_ctx* ctx, kc_info* info) {
     CTX_TRY(hipStreamSynchronize(ctx->st));
     float ms = 0;
     CTX_TRY(hipEventElapsedTime(&ms, e0, e1));
+    uint64_t hq = 0;
+    if (ctx->o.error_correct)
+        CTX_TRY(hipMemcpy(&hq, ctx->d_counters.as<unsigned long long>() + HAMMER_QUERIES, 8, hipMemcpyDeviceToHost));
     uint64_t e = 0;
     for (uint32_t l = 0; l < ctx->libs.size(); ++l) {
         auto& lib = ctx->libs[l];
         const LibOut& o = ctx->h_out[l];
         lib.info.ms_stage3 = ms;
+        lib.info.hammer_queries = hq;
         lib.info.n_counts = o.n_counts;
         lib.info.n_solid = o.n_solid;
         lib.info.cutoff = o.cutoff;
@@ -3140,10 +3143,9 @@ int kc_graph_run(kc_ctx* ctx, kc_info* info) {
     CTX_TRY(hipMemcpyAsync(ctx->h_out.data(), ctx->d_out.p, L * sizeof(LibOut), hipMemcpyDeviceToHost, ctx->st));
     CTX_TRY(hipMemcpyAsync(ctx->h_stage, ctx->d_counters.p, 16 * 8, hipMemcpyDeviceToHost, ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
-    // the plan holds when no table overflowed, no read was too long for Hammer and the
-    // compaction produced exactly the planned number of entries; otherwise plan again
+    // the plan holds when no table overflowed and the compaction produced exactly the
+    // planned number of entries; otherwise plan again
     bool ok = ctx->h_stage[4] == ctx->g_entries;
-    if (ctx->o.error_correct && (ctx->h_stage[3] & 0xFFFFFFFFull)) ok = false;
     for (uint32_t l = 0; l < L; ++l) ok = ok && !ctx->h_out[l].overflow;
     if (!ok) return graph_plan_and_capture(ctx, info);
     float ms = 0;

@@ -212,7 +212,6 @@ int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_
     for (size_t i = 0; i < n_reads; ++i) {
         const uint64_t nb = ((uint64_t)lengths[i] + 3) / 4;
         if (offsets[i] > data_len || nb > data_len - offsets[i]) return f->fail(KC_EINVAL, "read extends past data_len");
-        if (lengths[i] > HAMMER_MAX_LEN) return f->fail(KC_EUNSUPPORTED, "reads of at most 1024 bases");
         if (lengths[i] >= k) active.push_back((uint32_t)i);
     }
     F_TRY(hipSetDevice(f->device));
@@ -236,12 +235,12 @@ int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_
         LibBloom lb{f->d_solid.as<uint64_t>(), f->M};
         F_TRY(d_lib.ensure(sizeof(LibBloom)));
         F_TRY(hipMemcpyAsync(d_lib.p, &lb, sizeof(LibBloom), hipMemcpyHostToDevice, f->st));
-        const uint32_t g = grid_for(active.size(), 64, 65536);
+        const uint32_t g = hammer_grid(active.size());
 #define KC_HAMMER_LAUNCH(KW)                                                                                  \
     k_hammer<KW><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),        \
                                       active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,       \
-                                      d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,               \
-                                      d_ctr.as<uint32_t>() + 4)
+                                      d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,         \
+                                      d_ctr.as<unsigned long long>() + 1)
         if (kw_of_k(k) == 4) KC_HAMMER_LAUNCH(4);
         else if (kw_of_k(k) == 2) KC_HAMMER_LAUNCH(2);
         else KC_HAMMER_LAUNCH(1);

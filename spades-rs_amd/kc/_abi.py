@@ -59,7 +59,8 @@ class kc_info(ctypes.Structure):
     _fields_ = [("n_reads", ctypes.c_uint64), ("n_occ", ctypes.c_uint64), ("n_distinct", ctypes.c_uint64),
                 ("n_counts", ctypes.c_uint64), ("n_solid", ctypes.c_uint64), ("corrected_reads", ctypes.c_uint64),
                 ("num_words", ctypes.c_uint64), ("table_capacity", ctypes.c_uint64), ("cutoff", ctypes.c_uint32),
-                ("top_cov", ctypes.c_uint32), ("ms_stage2", ctypes.c_double), ("ms_stage3", ctypes.c_double)]
+                ("top_cov", ctypes.c_uint32), ("ms_stage2", ctypes.c_double), ("ms_stage3", ctypes.c_double),
+                ("hammer_queries", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

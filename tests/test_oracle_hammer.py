@@ -56,3 +56,24 @@ def test_planted_substitution_is_restored(oracle, k):
         assert mod
         restored += out == genome[s:s + 100]
     assert restored == 40
+
+
+# ---- hand-derived known answers (tests/hammer_kats.py) --------------------------
+
+from hammer_kats import all_cases, preconditions  # noqa: E402
+
+
+@pytest.mark.parametrize("case", all_cases(), ids=lambda c: c[0])
+def test_known_answers(oracle, case):
+    name, k, kmers, read, expected, states = case
+    f = oracle.TwoTierFilter(1 << 24)
+    for _ in range(2):
+        for x in kmers:
+            f.insert_kmer256(oracle.kmer_canonical(oracle.kmer_from_bytes(x.encode(), k), k)[0])
+
+    def is_solid(w):
+        return f.is_solid_kmer256(oracle.kmer_canonical(oracle.kmer_from_bytes(w.encode(), k), k)[0])
+
+    preconditions(k, kmers, read, expected, states, is_solid)
+    mod, out = oracle.correct_read(f, k, read.encode())
+    assert (mod, out.decode()) == expected

@@ -93,6 +93,21 @@ int kc_filter_insert_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint
 int kc_filter_is_solid_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* out);
 /* Copy both tiers (num_words u64 each). */
 int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words);
+
+/* The same correction as kc_filter_correct_reads, for callers that cannot allocate (code
+ * that runs under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  No counterpart in the reference.  Every pointer except n_corrected is
+ * DEVICE memory on the filter's device: d_data (data_len bytes, PackedReads layout),
+ * d_offsets / d_lengths (n_reads each) and d_out (data_len bytes, receives the corrected
+ * store).  The call allocates nothing: its scratch (counters, the active-read list) is one
+ * static device array per GPU of 512 + 4 * KC_NOALLOC_MAX_READS bytes, held until the
+ * process ends and shared by every call on that GPU, so calls that overlap on one GPU
+ * race.  More than KC_NOALLOC_MAX_READS reads give KC_EINVAL, as does a read past
+ * data_len.  Synchronises the filter's stream before returning. */
+#define KC_NOALLOC_MAX_READS (1u << 22)
+int kc_filter_correct_reads_noalloc(kc_filter* f, uint32_t k, const uint8_t* d_data, size_t data_len,
+                                    const uint64_t* d_offsets, const uint32_t* d_lengths, size_t n_reads,
+                                    uint8_t* d_out, uint64_t* n_corrected);
 /* ErrorCorrector::new(k).correct_read_in_place (hammer.rs:28-120) for every read of a
  * PackedReads-layout store against this filter; `out` receives the corrected store (same
  * offsets/lengths), n_corrected the number of reads modified (hammer.rs:123-143). */

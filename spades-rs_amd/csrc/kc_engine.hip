@@ -2523,7 +2523,7 @@ struct Run {
                                       c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(),
                                       c->n_active, c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(),
                                       c->d_rlib.as<uint32_t>(), c->d_blooms.as<LibBloom>(), c->o.k,
-                                      &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE, ctr + HAMMER_QUERIES)));
+                                      &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE, ctr + HAMMER_QUERIES, nullptr)));
             }
             if (c->last_binned) {  // the recount hashes: regions sized for the first count's keys
                 rc = hashed_layout<NW, uint32_t>(0);
@@ -2639,7 +2639,7 @@ struct Run {
                     c->d_data.as<uint8_t>(), c->d_corr.as<uint8_t>(), c->d_active.as<uint32_t>(), c->n_active,
                     c->d_off.as<uint64_t>(), c->d_len.as<uint32_t>(), c->d_rlib.as<uint32_t>(),
                     c->d_blooms.as<LibBloom>(), c->o.k, &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE,
-                    ctr + HAMMER_QUERIES);
+                    ctr + HAMMER_QUERIES, nullptr);
             const uint64_t n_slots32 = c->total_buckets * slots_per_bucket<NW, uint32_t>();
             k_table_clear<NW, uint32_t><<<grid_for(n_slots32), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, uint32_t>>(),
                                                                               n_slots32);

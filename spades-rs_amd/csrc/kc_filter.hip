@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -80,6 +81,66 @@ __global__ void k_query_kernel(const uint64_t* __restrict__ keys4, uint64_t n, M
     }
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free Hammer (kc_filter_correct_reads_noalloc): its scratch is one static
+// device array per GPU, shared by every call on that GPU (calls that overlap on one GPU
+// race on it).  Layout: counters (corrected reads, is_solid tests, active reads, a bounds
+// error flag), the solid-tier descriptor, then the active-read list.
+// ---------------------------------------------------------------------------
+
+constexpr size_t NA_COUNTERS = 0, NA_BLOOM = 256, NA_ACTIVE = 512;
+constexpr size_t NA_SCRATCH_BYTES = NA_ACTIVE + (size_t)KC_NOALLOC_MAX_READS * 4;
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+// Counters cleared and the descriptor written (a kernel, so no memset node or host copy).
+__global__ void k_na_init(uint8_t* __restrict__ sc, LibBloom lb) {
+    if (threadIdx.x < 4) reinterpret_cast<unsigned long long*>(sc + NA_COUNTERS)[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *reinterpret_cast<LibBloom*>(sc + NA_BLOOM) = lb;
+}
+
+__global__ void k_na_copy(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Reads of at least k bases, in any order (each read is corrected on its own), appended
+// one wave at a time; a read past data_len sets the error flag and is left out.
+__global__ void k_na_active(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ lengths, uint64_t n,
+                            uint64_t data_len, uint32_t k, uint8_t* __restrict__ sc) {
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(sc + NA_COUNTERS);
+    uint32_t* act = reinterpret_cast<uint32_t*>(sc + NA_ACTIVE);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n_round = (n + stride - 1) / stride * stride;  // every lane takes part in each wave_append
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n_round; r += stride) {
+        bool take = false;
+        if (r < n) {
+            const uint64_t nb = ((uint64_t)lengths[r] + 3) / 4;
+            const bool inside = offsets[r] <= data_len && nb <= data_len - offsets[r];
+            if (!inside) atomicOr(&ctr[3], 1ull);
+            take = inside && lengths[r] >= k;
+        }
+        const uint64_t slot = wave_append(take, &ctr[2]);
+        if (take) act[slot] = (uint32_t)r;
+    }
+}
+
+void* g_scratch[64] = {};  // per device: the address of its copy of `scratch`
+std::mutex g_scratch_mu;
+
+// The device address of this device's `scratch` (the device must be current).
+hipError_t scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 namespace kc {
@@ -102,6 +163,7 @@ struct kc_filter {
     uint64_t num_words = 0;
     Modulus M{};
     DevBuf d_seen, d_solid, d_fs, d_keys, d_prom;
+    uint8_t* scratch = nullptr;  // this device's allocation-free Hammer scratch
     std::string err;
     int fail(int code, const std::string& m) {
         err = m;
@@ -142,6 +204,12 @@ int kc_filter_create(kc_filter** out, int32_t device, uint64_t capacity_bits) {
         kc_filter_destroy(f);
         return KC_EHIP;
     }
+    void* sp = nullptr;
+    if (scratch_address(device, &sp) != hipSuccess) {
+        kc_filter_destroy(f);
+        return KC_EHIP;
+    }
+    f->scratch = static_cast<uint8_t*>(sp);
     *out = f;
     return KC_OK;
 }
@@ -240,7 +308,7 @@ int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_
     k_hammer<KW><<<g, 64, 0, f->st>>>(d_data.as<uint8_t>(), d_corr.as<uint8_t>(), d_act.as<uint32_t>(),        \
                                       active.size(), d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,       \
                                       d_lib.as<LibBloom>(), k, d_ctr.as<unsigned long long>(), 1,         \
-                                      d_ctr.as<unsigned long long>() + 1)
+                                      d_ctr.as<unsigned long long>() + 1, nullptr)
         if (kw_of_k(k) == 4) KC_HAMMER_LAUNCH(4);
         else if (kw_of_k(k) == 2) KC_HAMMER_LAUNCH(2);
         else KC_HAMMER_LAUNCH(1);
@@ -252,6 +320,43 @@ int kc_filter_correct_reads(kc_filter* f, uint32_t k, const uint8_t* data, size_
     if (data_len) F_TRY(hipMemcpyAsync(out, d_corr.p, data_len, hipMemcpyDeviceToHost, f->st));
     F_TRY(hipStreamSynchronize(f->st));
     for (DevBuf* b : {&d_data, &d_corr, &d_off, &d_len, &d_act, &d_ctr, &d_lib}) b->release();
+    if (n_corrected) *n_corrected = ctr[0];
+    return KC_OK;
+}
+
+int kc_filter_correct_reads_noalloc(kc_filter* f, uint32_t k, const uint8_t* d_data, size_t data_len,
+                                    const uint64_t* d_offsets, const uint32_t* d_lengths, size_t n_reads,
+                                    uint8_t* d_out, uint64_t* n_corrected) {
+    if (!f) return KC_EINVAL;
+    if (k < 1 || k > KC_MAX_K) return f->fail(KC_EINVAL, "k out of range");
+    if (n_reads > KC_NOALLOC_MAX_READS)
+        return f->fail(KC_EINVAL, "kc_filter_correct_reads_noalloc takes at most KC_NOALLOC_MAX_READS (" +
+                                      std::to_string(KC_NOALLOC_MAX_READS) + ") reads a call");
+    if (n_reads && (!d_offsets || !d_lengths)) return f->fail(KC_EINVAL, "null offsets/lengths");
+    if (data_len && (!d_data || !d_out)) return f->fail(KC_EINVAL, "null data/out");
+    if (!f->scratch) return f->fail(KC_ESTATE, "no scratch address for this device");
+    F_TRY(hipSetDevice(f->device));
+    uint8_t* sc = f->scratch;
+    k_na_init<<<1, 64, 0, f->st>>>(sc, LibBloom{f->d_solid.as<uint64_t>(), f->M});
+    if (data_len) k_na_copy<<<grid_for(data_len), BLOCK, 0, f->st>>>(d_data, d_out, data_len);
+    if (n_reads) {
+        k_na_active<<<grid_for(n_reads), BLOCK, 0, f->st>>>(d_offsets, d_lengths, n_reads, data_len, k, sc);
+        unsigned long long* ctr = reinterpret_cast<unsigned long long*>(sc + NA_COUNTERS);
+        const uint32_t g = hammer_grid(n_reads);
+#define KC_HAMMER_LAUNCH(KW)                                                                                        \
+    k_hammer<KW><<<g, 64, 0, f->st>>>(d_data, d_out, reinterpret_cast<const uint32_t*>(sc + NA_ACTIVE), n_reads,    \
+                                      d_offsets, d_lengths, nullptr, reinterpret_cast<const LibBloom*>(sc + NA_BLOOM), \
+                                      k, ctr, 1, ctr + 1, ctr + 2)
+        if (kw_of_k(k) == 4) KC_HAMMER_LAUNCH(4);
+        else if (kw_of_k(k) == 2) KC_HAMMER_LAUNCH(2);
+        else KC_HAMMER_LAUNCH(1);
+#undef KC_HAMMER_LAUNCH
+    }
+    F_TRY(hipGetLastError());
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    F_TRY(hipMemcpyAsync(ctr, sc + NA_COUNTERS, sizeof(ctr), hipMemcpyDeviceToHost, f->st));
+    F_TRY(hipStreamSynchronize(f->st));
+    if (ctr[3]) return f->fail(KC_EINVAL, "read extends past data_len");
     if (n_corrected) *n_corrected = ctr[0];
     return KC_OK;
 }

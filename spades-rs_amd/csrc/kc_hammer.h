@@ -64,20 +64,23 @@ __device__ __forceinline__ bool hm_solid(const Kv& v, uint32_t k, const uint64_t
 // of data; only modified reads are written).  Every active read has length >= k.
 // read_lib (nullable) gives each read's library; n_corrected[lib * corrected_stride]
 // counts the reads modified (hammer.rs:123-143); n_queries (nullable) receives the number
-// of is_solid tests made.  Launch with 64 threads per workgroup.
+// of is_solid tests made.  When n_active_dev is set, the number of active reads is read
+// from it instead of n_active.  Launch with 64 threads per workgroup.
 template <int KW>
 __global__ void __launch_bounds__(64) k_hammer(const uint8_t* __restrict__ data, uint8_t* __restrict__ corr,
                                                const uint32_t* __restrict__ active, uint64_t n_active,
                                                const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ lengths,
                                                const uint32_t* __restrict__ read_lib, const LibBloom* __restrict__ libs,
                                                uint32_t k, unsigned long long* __restrict__ n_corrected,
-                                               size_t corrected_stride, unsigned long long* __restrict__ n_queries) {
+                                               size_t corrected_stride, unsigned long long* __restrict__ n_queries,
+                                               const unsigned long long* __restrict__ n_active_dev) {
     __shared__ uint32_t s_rd[HAMMER_CHUNK_WORDS + HAMMER_PAD_WORDS];
     __shared__ uint8_t s_ok[HAMMER_CHUNK];  // s_ok[s - b0]: window s of the current read is solid
     uint8_t* rb = reinterpret_cast<uint8_t*>(s_rd);
     const uint32_t lane = threadIdx.x;
     const uint32_t nc = 3 * k;  // candidates of one window
     uint64_t nq = 0;            // is_solid tests of this wave (wave-uniform)
+    if (n_active_dev) n_active = *n_active_dev;
     for (uint64_t a = blockIdx.x; a < n_active; a += gridDim.x) {
         const uint32_t r = active[a];
         const uint32_t L = lengths[r];

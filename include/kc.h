@@ -58,6 +58,9 @@ typedef struct kc_opts {
 } kc_opts;
 
 #define KC_FLAG_TIMING 1u  /* record per-kernel HIP events (kc_kernel_times) */
+#define KC_FLAG_EVENTS64 2u /* 64-bit event indices (global window numbers) even when 32 bits would
+                              do: the path a library of 2^30 - 1 windows or more takes, selectable
+                              on any input (the results are the same) */
 
 /* Summary of the last run. */
 typedef struct kc_info {
@@ -91,6 +94,9 @@ uint64_t kc_filter_num_words(const kc_filter* f);
 int kc_filter_insert_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* promoted);
 /* TwoTierFilter::is_solid_kmer256 (bloom.rs:111-126) for each key. */
 int kc_filter_is_solid_kmer256(kc_filter* f, const uint64_t* keys4, size_t n, uint8_t* out);
+/* hamming_distance_2bit (hammer.rs:13-18) for n pairs: out[i] = the number of bases (2-bit
+ * groups) in which a[i] and b[i] differ, popcount((d | d >> 1) & 0x5555...) with d = a ^ b. */
+int kc_hamming_distance_2bit(int32_t device, const uint64_t* a, const uint64_t* b, size_t n, uint32_t* out);
 /* Copy both tiers (num_words u64 each). */
 int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words);
 

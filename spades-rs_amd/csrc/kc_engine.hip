@@ -2972,7 +2972,9 @@ int kc_bloom_build(kc_ctx* ctx) {
         lib.info.n_occ = occ;
         lib.info.n_reads = lib.len.size();
     }
-    ctx->ev64 = false;
+    // 32-bit events hold 4 t + j for t < 2^30 - 1; larger libraries (or KC_FLAG_EVENTS64)
+    // take 64-bit events
+    ctx->ev64 = (ctx->o.flags & KC_FLAG_EVENTS64) != 0;
     for (auto& lib : ctx->libs) ctx->ev64 = ctx->ev64 || lib.info.n_occ >= (1ull << 30) - 1;
     if (ctx->n_occ == 0) {
         std::vector<uint32_t> grow(ctx->libs.size(), 0);

@@ -141,6 +141,15 @@ hipError_t scratch_address(int device, void** out) {
     return hipSuccess;
 }
 
+// hamming_distance_2bit (hammer.rs:13-18): bases that differ between two 2-bit packed words.
+__global__ void k_hamming(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, uint64_t n,
+                          uint32_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = a[i] ^ b[i];
+        out[i] = (uint32_t)__popcll((d | (d >> 1)) & 0x5555555555555555ull);
+    }
+}
+
 }  // namespace
 
 namespace kc {
@@ -359,6 +368,33 @@ int kc_filter_correct_reads_noalloc(kc_filter* f, uint32_t k, const uint8_t* d_d
     if (ctr[3]) return f->fail(KC_EINVAL, "read extends past data_len");
     if (n_corrected) *n_corrected = ctr[0];
     return KC_OK;
+}
+
+int kc_hamming_distance_2bit(int32_t device, const uint64_t* a, const uint64_t* b, size_t n, uint32_t* out) {
+    if (n == 0) return KC_OK;
+    if (!a || !b || !out) return KC_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KC_EHIP;
+    if (device < 0 || device >= ndev) return KC_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return KC_EHIP;
+    DevBuf da, db, dout;
+    hipStream_t st = nullptr;
+    int rc = KC_OK;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return KC_EHIP;
+    if (da.ensure(n * 8) || db.ensure(n * 8) || dout.ensure(n * 4)) rc = KC_ENOMEM;
+    if (!rc && (hipMemcpyAsync(da.p, a, n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(db.p, b, n * 8, hipMemcpyHostToDevice, st) != hipSuccess))
+        rc = KC_EHIP;
+    if (!rc) {
+        k_hamming<<<grid_for(n), BLOCK, 0, st>>>(da.as<uint64_t>(), db.as<uint64_t>(), n, dout.as<uint32_t>());
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = KC_EHIP;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    return rc;
 }
 
 int kc_filter_export(kc_filter* f, uint64_t* seen, uint64_t* solid, size_t num_words) {

@@ -18,6 +18,7 @@ KC_EUNSUPPORTED = -5
 KC_MAX_K = 127
 KC_ANCHOR_POLISHER, KC_ANCHOR_PAIRED_INFO, KC_ANCHOR_SPLITTER = 0, 1, 2
 KC_FLAG_TIMING = 1
+KC_FLAG_EVENTS64 = 2
 KC_NOALLOC_MAX_READS = 1 << 22
 
 STATUS_NAMES = {KC_EINVAL: "KC_EINVAL", KC_ENOMEM: "KC_ENOMEM", KC_EHIP: "KC_EHIP",
@@ -27,7 +28,7 @@ STATUS_NAMES = {KC_EINVAL: "KC_EINVAL", KC_ENOMEM: "KC_ENOMEM", KC_EHIP: "KC_EHI
 EXPORTS = (
     "kc_filter_create", "kc_filter_destroy", "kc_filter_last_error", "kc_filter_memory_usage_bytes",
     "kc_filter_num_words", "kc_filter_insert_kmer256", "kc_filter_is_solid_kmer256", "kc_filter_export",
-    "kc_filter_correct_reads", "kc_filter_correct_reads_noalloc",
+    "kc_filter_correct_reads", "kc_filter_correct_reads_noalloc", "kc_hamming_distance_2bit",
     "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
     "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
@@ -113,6 +114,8 @@ _SIGS = {
     "kc_filter_is_solid_kmer256": (ctypes.c_int, [_vp, _u64p, _sz, _u8p]),
     "kc_filter_export": (ctypes.c_int, [_vp, _u64p, _u64p, _sz]),
     "kc_filter_correct_reads": (ctypes.c_int, [_vp, ctypes.c_uint32, _u8p, _sz, _u64p, _u32p, _sz, _u8p, _u64p]),
+    "kc_filter_correct_reads_noalloc": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _sz, _vp, _vp, _sz, _vp, _u64p]),
+    "kc_hamming_distance_2bit": (ctypes.c_int, [ctypes.c_int32, _u64p, _u64p, _sz, _u32p]),
     "kc_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(kc_opts)]),
     "kc_destroy": (None, [_vp]),
     "kc_last_error": (ctypes.c_char_p, [_vp]),

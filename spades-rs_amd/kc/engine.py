@@ -100,6 +100,20 @@ class TwoTierFilter:
         return seen, solid
 
 
+def hamming_distance_2bit(a, b, device=0):
+    """hammer.rs:13-18 on the GPU for arrays of 2-bit packed u64 words: the number of bases
+    in which a[i] and b[i] differ."""
+    x = np.ascontiguousarray(np.atleast_1d(a), dtype=np.uint64)
+    y = np.ascontiguousarray(np.atleast_1d(b), dtype=np.uint64)
+    if x.shape != y.shape:
+        raise ValueError("a and b must have the same shape")
+    out = np.zeros(max(x.size, 1), dtype=np.uint32)
+    st = lib().kc_hamming_distance_2bit(device, ptr(x, u64p), ptr(y, u64p), x.size, ptr(out, u32p))
+    if st != _abi.KC_OK:
+        raise KcError(st, "kc_hamming_distance_2bit failed")
+    return out[:x.size]
+
+
 class ErrorCorrector:
     """hammer.rs:20-144: single-base correction of reads against a TwoTierFilter."""
 
@@ -182,14 +196,16 @@ class KmerSpectrum:
 class SpectrumEngine:
     """One engine context: reads stay resident in HBM across runs."""
 
-    def __init__(self, config, device=0, table_load=0.0, timing=False):
+    def __init__(self, config, device=0, table_load=0.0, timing=False, events64=False):
+        """events64: 64-bit event indices on any input (the path libraries of 2^30 - 1
+        windows or more take; the results are the same)."""
         cfg = config
         if not 1 <= cfg.k <= _abi.KC_MAX_K:
             raise KcError(_abi.KC_EUNSUPPORTED if cfg.k == 128 else _abi.KC_EINVAL,
                           f"k={cfg.k} outside 1..{_abi.KC_MAX_K}")
         o = _abi.kc_opts(device=device, k=cfg.k, bloom_bits=int(cfg.bloom_bits), min_coverage=float(cfg.min_coverage),
                          error_correct=int(bool(cfg.error_correct)), table_load=float(table_load),
-                         flags=_abi.KC_FLAG_TIMING if timing else 0)
+                         flags=(_abi.KC_FLAG_TIMING if timing else 0) | (_abi.KC_FLAG_EVENTS64 if events64 else 0))
         h = ctypes.c_void_p()
         st = lib().kc_create(ctypes.byref(h), ctypes.byref(o))
         if st != _abi.KC_OK:

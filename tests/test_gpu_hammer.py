@@ -150,3 +150,20 @@ def test_fasta_records_of_kilobases_with_hammer(oracle, width):
     assert info["corrected_reads"] == S.corrected_reads > 0
     assert np.array_equal(corr, S.corrected_data())
     assert np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt) and np.array_equal(mask, omask)
+
+
+def test_hamming_distance_2bit_on_device(oracle):
+    """hamming_distance_2bit (hammer.rs:13-18): the reference's known answers
+    (hammer.rs:150-158), and random words against the oracle and a per-base count."""
+    from kc import hamming_distance_2bit
+    assert list(hamming_distance_2bit([0b00011011, 0b00011011, 0b00000000],
+                                      [0b00011011, 0b00011000, 0b01010101])) == [0, 1, 4]
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2**63, 100000, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 100000, dtype=np.uint64)
+    b = a.copy()
+    flip = rng.random((100000, 32)) < 0.1
+    for j in range(32):
+        b ^= np.where(flip[:, j], rng.integers(1, 4, 100000, dtype=np.uint64) << np.uint64(2 * j), np.uint64(0))
+    got = hamming_distance_2bit(a, b)
+    assert np.array_equal(got, flip.sum(axis=1).astype(np.uint32))
+    assert all(int(got[i]) == oracle.hamming_2bit(int(a[i]), int(b[i])) for i in range(0, 100000, 997))

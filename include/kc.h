@@ -157,6 +157,23 @@ int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* so
  * as uploaded; data_len must equal the uploaded data_len. */
 int kc_reads_export(kc_ctx* ctx, uint8_t* data, size_t data_len);
 
+/* Checks of a finished run of library lib that recompute its closed forms on the device,
+ * window by window, independently of how the run built them (no counterpart in the
+ * reference; for runs too large to compare with a CPU restatement):
+ *   seen_words_differing  words of the seen tier that differ from the OR of every window's
+ *                         three bits (bloom.rs:95-97): 0 for a correct run;
+ *   solid_windows         windows of the counted reads (corrected when error_correct)
+ *                         whose canonical k-mer is Bloom-solid; without priors it equals
+ *                         sum_counts, the sum of the count table's counts (assemble.rs:196);
+ *   unordered_pairs       adjacent entries of the sorted table not strictly ascending: 0;
+ *   mask_mismatches       entries whose solid flag differs from count >= cutoff: 0;
+ *   checksum              the sum over entries i of a 64-bit mix of (i, k-mer value words,
+ *                         count): equal tables give equal checksums. */
+typedef struct kc_verify {
+    uint64_t windows, solid_windows, sum_counts, seen_words_differing, unordered_pairs, mask_mismatches, checksum;
+} kc_verify;
+int kc_library_verify(kc_ctx* ctx, uint32_t lib, kc_verify* out);
+
 /* ---- Several libraries in one context ------------------------------------
  * A context may hold many independent libraries (read sets).  Each is counted exactly as
  * if it were alone in its own context (own filter, event order, table, cutoff and solid

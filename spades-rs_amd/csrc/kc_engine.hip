@@ -576,6 +576,8 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
 // insert instead).
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
+// Transient buffers above this size are released after use instead of kept for reuse.
+constexpr uint64_t BIG_SCRATCH = 8ull << 30;
 
 template <int KW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict__ recs,
@@ -1536,6 +1538,150 @@ const char* const KERNEL_NAMES[KID_COUNT] = {
     "k_count_hist", "k_solid_flags", "k_superkmers", "bin_sort", "k_bin_count",
     "k_fs_items", "fs_sort", "k_fs_min"};
 
+// ---------------------------------------------------------------------------
+// verification passes (kc_library_verify): closed forms recomputed window by window,
+// independently of the first-setter construction and of the count's table layout
+// ---------------------------------------------------------------------------
+
+// Per library: every window's three seen bits ORed into `recount` (the seen tier is
+// the OR over all occurrences, bloom.rs:95-97), and the windows of the counted reads
+// (corrected when Hammer ran) whose canonical k-mer is Bloom-solid counted into
+// solid_win (the count table holds exactly their multiplicities, assemble.rs:196).
+template <int KW>
+__global__ void __launch_bounds__(BLOCK) k_verify_windows(const uint8_t* __restrict__ data,
+                                                          const uint8_t* __restrict__ counted,
+                                                          const uint64_t* __restrict__ act_ev,
+                                                          const uint64_t* __restrict__ act_off,
+                                                          const uint32_t* __restrict__ act_lib,
+                                                          const uint32_t* __restrict__ tile_first, uint64_t n_active,
+                                                          uint64_t n_occ, uint64_t n_tiles, uint32_t k,
+                                                          const LibDesc* __restrict__ libs,
+                                                          unsigned long long* __restrict__ recount,
+                                                          unsigned long long* __restrict__ solid_win) {
+    __shared__ uint64_t s_ev[TILE_WINDOWS + 1];
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const uint64_t a0 = tile_first[tile];
+        const uint32_t lib0 = act_lib[a0];
+        const uint64_t w0 = tile * TILE_WINDOWS;
+        const uint64_t w1 = min(w0 + TILE_WINDOWS, n_occ);
+        const uint32_t na = (uint32_t)min<uint64_t>(TILE_WINDOWS + 1, n_active - a0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < na; i += blockDim.x) s_ev[i] = act_ev[a0 + i];
+        __syncthreads();
+        uint32_t mine = 0;  // solid windows of lib0 seen by this thread
+        for (uint64_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
+            const uint32_t lo = tile_read(s_ev, na, w);
+            const uint32_t lib = act_lib[a0 + lo];
+            const LibDesc& L = libs[lib];
+            const uint32_t j = (uint32_t)(w - s_ev[lo]);
+            const Kv v = canonical<KW>(load_window<KW>(data, act_off[a0 + lo], j, k), k);
+            uint64_t p[3];
+            kv_positions<KW>(v, k, L.M, p);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) atomicOr(&recount[L.word0 + (p[q] >> 6)], 1ull << (p[q] & 63));
+            const Kv u = counted == data ? v : canonical<KW>(load_window<KW>(counted, act_off[a0 + lo], j, k), k);
+            if (bloom_is_solid<KW>(u, k, L.solid, L.M)) {
+                if (lib == lib0) ++mine;
+                else atomicAdd(&solid_win[lib], 1ull);
+            }
+        }
+        for (uint32_t d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d);
+        if ((threadIdx.x & 63) == 0 && mine) atomicAdd(&solid_win[lib0], (unsigned long long)mine);
+    }
+}
+
+// Words [0, n) of the seen tier that differ from the recount.
+__global__ void k_verify_seen(const uint64_t* __restrict__ seen, const unsigned long long* __restrict__ recount,
+                              uint64_t n, unsigned long long* __restrict__ differing) {
+    uint32_t mine = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        mine += seen[i] != recount[i];
+    for (uint32_t d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(differing, (unsigned long long)mine);
+}
+
+// The k-mer value of sorted entry i (as the counts export decodes it).
+__device__ __forceinline__ void entry_value(const SortLayout& lay, const uint64_t* __restrict__ hi,
+                                            const uint64_t* __restrict__ lo, const WideEntry* __restrict__ wide,
+                                            uint64_t i, uint64_t v[4]) {
+    const uint32_t k = lay.k;
+    v[2] = v[3] = 0;
+    if (lay.wide) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) v[w] = wide[i].v[w];
+    } else if (lay.single) {
+        v[0] = 2 * k >= 64 ? lo[i] : (lo[i] & ((1ull << (2 * k)) - 1));
+        v[1] = 0;
+    } else if (2 * k <= 63) {
+        v[0] = lo[i];
+        v[1] = 0;
+    } else {
+        const uint32_t vb = 2 * k - 63;
+        const uint64_t a = hi[i] & ((1ull << vb) - 1);
+        v[0] = lo[i] | (a << 63);
+        v[1] = a >> 1;
+    }
+}
+
+// A 64-bit mix of (position, value, count); the table checksum is its sum over entries.
+__device__ __host__ __forceinline__ uint64_t entry_mix(uint64_t i, const uint64_t v[4], uint32_t cnt) {
+    auto f = [](uint64_t x) {
+        x ^= x >> 33;
+        x *= 0xff51afd7ed558ccdull;
+        x ^= x >> 33;
+        x *= 0xc4ceb9fe1a85ec53ull;
+        x ^= x >> 33;
+        return x;
+    };
+    uint64_t x = f(i + 0x9E3779B97F4A7C15ull * (v[0] + 1));
+    x = f(x ^ v[1]);
+    x = f(x ^ v[2]);
+    x = f(x ^ v[3]);
+    return f(x ^ cnt);
+}
+
+// Over entries [e0, e0 + n): out[0] += counts, out[1] += pairs (i, i + 1) not strictly
+// ascending, out[2] += solid flags that differ from count >= cutoff, out[3] += checksum.
+__global__ void __launch_bounds__(BLOCK) k_verify_table(SortLayout lay, const uint64_t* __restrict__ hi,
+                                                        const uint64_t* __restrict__ lo,
+                                                        const WideEntry* __restrict__ wide,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const uint8_t* __restrict__ sflags, uint64_t e0, uint64_t n,
+                                                        uint32_t cutoff, unsigned long long* __restrict__ out) {
+    unsigned long long sum = 0, bad = 0, mism = 0, ck = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t v[4];
+        entry_value(lay, hi, lo, wide, e0 + i, v);
+        const uint32_t c = cnt[e0 + i];
+        sum += c;
+        mism += (sflags[e0 + i] != 0) != (c >= cutoff);
+        ck += entry_mix(i, v, c);
+        if (i + 1 < n) {
+            uint64_t u[4];
+            entry_value(lay, hi, lo, wide, e0 + i + 1, u);
+            bool less = false, eq = true;
+            for (int w = 3; w >= 0 && eq; --w)
+                if (v[w] != u[w]) {
+                    less = v[w] < u[w];
+                    eq = false;
+                }
+            bad += !less;
+        }
+    }
+    for (uint32_t d = 32; d >= 1; d >>= 1) {
+        sum += __shfl_xor(sum, d);
+        bad += __shfl_xor(bad, d);
+        mism += __shfl_xor(mism, d);
+        ck += __shfl_xor(ck, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], sum);
+        if (bad) atomicAdd(&out[1], bad);
+        if (mism) atomicAdd(&out[2], mism);
+        atomicAdd(&out[3], ck);
+    }
+}
+
 // A library as uploaded: its reads (host copy, PackedReads layout) and options.
 struct HostLib {
     std::vector<uint8_t> data;
@@ -1600,7 +1746,6 @@ struct kc_ctx {
     // count entries (sorted) + scratch
     DevBuf d_hi, d_lo, d_cnt, d_hi2, d_lo2, d_cnt2, d_idx, d_idx2, d_sflags;
     DevBuf d_wide, d_wide2;  // k > 63: count-table entries as (V, library) records
-    // partitioned Stage 2 insert: level-1 / level-2 records, cursors, level-2 chunks
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
@@ -1608,6 +1753,7 @@ struct kc_ctx {
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
+    DevBuf d_vrec, d_vout;  // kc_library_verify: seen-tier recount, per-library results
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
     DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
@@ -1976,6 +2122,12 @@ struct Run {
                                                           bits, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                                           c->d_wbin.as<uint64_t>(), c->d_bk.as<uint32_t>(),
                                                           c->d_br.as<uint64_t>())));
+            // the cut's per-window scratch is kept for the next run unless it is large
+            // (metagenome-scale inputs), where Stage 3 needs the room
+            if (c->d_wbin.bytes > BIG_SCRATCH) {
+                KC_TRY(hipStreamSynchronize(st));
+                c->d_wbin.release();
+            }
         }
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
@@ -2022,6 +2174,8 @@ struct Run {
             if (attempt) return fail(KC_EHIP, "binned count passed its exact output sizes");
             cap = lcnt;  // the counters totalled every library's distinct k-mers
         }
+        if (c->d_bk.bytes + c->d_bk2.bytes + c->d_br.bytes + c->d_br2.bytes > BIG_SCRATCH)
+            for (DevBuf* b : {&c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2}) b->release();
         // pack: library l's entries from bucket0_l, the rest of its last bucket empty
         uint64_t b0 = 0;
         for (uint32_t l = 0; l < nL; ++l) {
@@ -3176,6 +3330,57 @@ int kc_graph_run(kc_ctx* ctx, kc_info* info) {
 int kc_graph_release(kc_ctx* ctx) {
     if (!ctx) return KC_EINVAL;
     ctx->graph_release();
+    return KC_OK;
+}
+
+int kc_library_verify(kc_ctx* ctx, uint32_t lib, kc_verify* out) {
+    if (!ctx || !out) return KC_EINVAL;
+    if (!ctx->counted) return ctx->fail(KC_ESTATE, "verify before kc_count");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    hipStream_t st = ctx->st;
+    const uint32_t L = (uint32_t)ctx->libs.size();
+    const HostLib& H = ctx->libs[lib];
+    const uint64_t words = ctx->total_words;
+    CTX_TRY(ctx->d_vrec.ensure(std::max<uint64_t>(words, 1) * 8));
+    CTX_TRY(ctx->d_vout.ensure(((size_t)L + 8) * 8));
+    unsigned long long* vo = ctx->d_vout.as<unsigned long long>();
+    k_fill<unsigned long long><<<grid_for(words), BLOCK, 0, st>>>(ctx->d_vrec.as<unsigned long long>(), words, 0ull);
+    k_fill<unsigned long long><<<1, 64, 0, st>>>(vo, (uint64_t)L + 8, 0ull);
+    CTX_TRY(hipGetLastError());
+    const uint8_t* counted = ctx->o.error_correct ? ctx->d_corr.as<uint8_t>() : ctx->d_data.as<uint8_t>();
+    if (ctx->n_tiles) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(ctx->n_tiles, 8192);
+#define KC_VERIFY_LAUNCH(KW)                                                                                       \
+    k_verify_windows<KW><<<g, BLOCK, 0, st>>>(ctx->d_data.as<uint8_t>(), counted, ctx->d_act_ev.as<uint64_t>(),   \
+                                              ctx->d_act_off.as<uint64_t>(), ctx->d_act_lib.as<uint32_t>(),      \
+                                              ctx->d_tile_first.as<uint32_t>(), ctx->n_active, ctx->n_occ,       \
+                                              ctx->n_tiles, ctx->o.k, ctx->d_libs.as<LibDesc>(),                 \
+                                              ctx->d_vrec.as<unsigned long long>(), vo + 8)
+        if (ctx->kw == 1) KC_VERIFY_LAUNCH(1);
+        else if (ctx->kw == 2) KC_VERIFY_LAUNCH(2);
+        else KC_VERIFY_LAUNCH(4);
+#undef KC_VERIFY_LAUNCH
+    }
+    k_verify_seen<<<grid_for(H.num_words), BLOCK, 0, st>>>(ctx->d_seen.as<uint64_t>() + H.word0,
+                                                           ctx->d_vrec.as<unsigned long long>() + H.word0,
+                                                           H.num_words, vo + 4);
+    const uint64_t n = H.info.n_counts;
+    if (n)
+        k_verify_table<<<grid_for(n), BLOCK, 0, st>>>(ctx->lay, ctx->d_hi.as<uint64_t>(), ctx->d_lo.as<uint64_t>(),
+                                                     ctx->d_wide.as<WideEntry>(), ctx->d_cnt.as<uint32_t>(),
+                                                     ctx->d_sflags.as<uint8_t>(), H.entry0, n, H.info.cutoff, vo);
+    CTX_TRY(hipGetLastError());
+    std::vector<unsigned long long> h((size_t)L + 8);
+    CTX_TRY(hipMemcpyAsync(h.data(), vo, h.size() * 8, hipMemcpyDeviceToHost, st));
+    CTX_TRY(hipStreamSynchronize(st));
+    out->windows = H.info.n_occ;
+    out->solid_windows = h[8 + lib];
+    out->sum_counts = h[0];
+    out->seen_words_differing = h[4];
+    out->unordered_pairs = h[1];
+    out->mask_mismatches = h[2];
+    out->checksum = h[3];
     return KC_OK;
 }
 

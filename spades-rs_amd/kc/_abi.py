@@ -31,7 +31,7 @@ EXPORTS = (
     "kc_filter_correct_reads", "kc_filter_correct_reads_noalloc", "kc_hamming_distance_2bit",
     "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
-    "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
+    "kc_library_verify", "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
     "kc_library_counts_export", "kc_library_reads_export", "kc_library_priors_set", "kc_priors_set",
     "kc_tune_get", "kc_tune_set", "kc_autotune", "kc_graph_run", "kc_graph_release", "kc_set_k",
     "kc_mlp_create", "kc_mlp_destroy", "kc_mlp_last_error", "kc_mlp_num_params", "kc_mlp_get_params",
@@ -55,6 +55,14 @@ class kc_opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("k", ctypes.c_uint32), ("bloom_bits", ctypes.c_uint64),
                 ("min_coverage", ctypes.c_double), ("error_correct", ctypes.c_int32),
                 ("table_load", ctypes.c_float), ("flags", ctypes.c_uint32)]
+
+
+class kc_verify(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("windows", "solid_windows", "sum_counts", "seen_words_differing",
+                                                "unordered_pairs", "mask_mismatches", "checksum")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class kc_info(ctypes.Structure):
@@ -132,6 +140,7 @@ _SIGS = {
     "kc_library_add": (ctypes.c_int, [_vp, _u8p, _sz, _u64p, _u32p, _sz, ctypes.c_uint64, ctypes.c_double, _u32p]),
     "kc_library_count": (ctypes.c_int, [_vp, _u32p]),
     "kc_library_info": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_info)]),
+    "kc_library_verify": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_verify)]),
     "kc_library_bloom_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u64p, _sz]),
     "kc_library_bloom_query": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _sz, _u8p]),
     "kc_library_counts_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u32p, _u8p, _sz]),

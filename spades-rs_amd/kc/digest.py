@@ -41,3 +41,29 @@ OUTPUTS = ("seen", "solid", "keys", "counts", "mask", "n_counts", "n_solid", "cu
 def diff(got, want):
     """Names of the output fields recorded in `want` that differ in `got` (empty when equal)."""
     return [f for f in OUTPUTS if f in want and got.get(f) != want[f]]
+
+
+M64 = (1 << 64) - 1
+
+
+def _fmix(x):
+    x = x ^ (x >> np.uint64(33))
+    x = x * np.uint64(0xff51afd7ed558ccd)
+    x = x ^ (x >> np.uint64(33))
+    x = x * np.uint64(0xc4ceb9fe1a85ec53)
+    return x ^ (x >> np.uint64(33))
+
+
+def table_checksum(values, counts):
+    """The checksum kc_library_verify reports for a sorted table: the sum mod 2^64 over
+    entries i of mix(i, V words, count), with V the k-mer value in four words, lowest first
+    (for k <= 64 the Kmer256 limbs)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64).reshape(-1, 4)
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    i = np.arange(v.shape[0], dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = _fmix(i + np.uint64(0x9E3779B97F4A7C15) * (v[:, 0] + np.uint64(1)))
+        for w in (1, 2, 3):
+            x = _fmix(x ^ v[:, w])
+        x = _fmix(x ^ c)
+        return int(x.sum(dtype=np.uint64))

@@ -339,6 +339,14 @@ class SpectrumEngine:
                                                   n, ctypes.byref(ms)))
         return lab[:n].astype(bool), fb[:n].astype(bool), ms.value
 
+    def verify(self, library=0):
+        """Closed-form checks of the last run recomputed window by window on the device
+        (kc_library_verify): seen-tier recount, Bloom-solid windows against the sum of the
+        counts, sortedness, solid flags, and a checksum of the sorted table."""
+        v = _abi.kc_verify()
+        self._chk(lib().kc_library_verify(self._h, library, ctypes.byref(v)))
+        return v.as_dict()
+
     def info(self, library=0):
         info = _abi.kc_info()
         self._chk(lib().kc_library_info(self._h, library, ctypes.byref(info)))

@@ -94,6 +94,38 @@ def workload_5mb_100x(seed=2, scale=1.0):
     return uniform_genome_pairs(seed, g, max(int(1_666_667 * scale), 1), p_start=0.001, p_end=0.010)
 
 
+def workload_metagenome(seed=4, scale=1.0, n_genomes=20, n_pairs=30_000_000, sigma=1.0, p_err=0.005):
+    """A metagenome sample: n_genomes random genomes of 1-6 Mb (uniform; x scale), cell
+    abundances lognormal(0, sigma), n_pairs x scale 2x150 bp pairs drawn from the genomes
+    in proportion to abundance x genome length (insert N(300, 30), uniform substitutions
+    at p_err), the pairs in a seeded random order; one library, R1 mates then R2 mates.
+    Returns (PackedReads, meta)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.maximum((rng.uniform(1e6, 6e6, n_genomes) * scale).astype(np.int64), 2000)
+    ab = np.exp(sigma * rng.standard_normal(n_genomes))
+    total = max(int(n_pairs * scale), n_genomes)
+    w = ab * sizes
+    pairs = np.floor(w / w.sum() * total).astype(np.int64)
+    pairs[int(np.argmax(w))] += total - int(pairs.sum())
+    nb = 38  # bytes of a 150-base read
+    r1, r2 = [], []
+    for g in range(n_genomes):
+        genome = random_genome(rng, int(sizes[g]), 0.5)
+        st, _ = uniform_genome_pairs(seed=seed * 1000 + g, genome_size=int(sizes[g]), n_pairs=int(pairs[g]),
+                                     p_start=p_err, genome=genome)
+        d = st.data.reshape(2 * int(pairs[g]), nb)
+        r1.append(d[: pairs[g]])
+        r2.append(d[pairs[g]:])
+        del st
+    perm = rng.permutation(total)
+    data = np.concatenate([np.concatenate(r1)[perm].reshape(-1), np.concatenate(r2)[perm].reshape(-1)])
+    del r1, r2
+    offsets = np.arange(2 * total, dtype=np.uint64) * np.uint64(nb)
+    lengths = np.full(2 * total, 150, dtype=np.uint32)
+    store = PackedReads.from_arrays(data, offsets, lengths, pe_count=total)
+    return store, {"genome_sizes": sizes, "abundance": ab, "pairs": pairs, "n_pairs": total}
+
+
 def uneven_pairs(rng, genome, n_pairs, median_len=150.0, sigma=0.25, min_len=50, max_len=250, p_err=0.003,
                  insert_sd=30.0, chunk=1 << 16):
     """Paired reads whose lengths are lognormal(median_len, sigma) truncated to

@@ -7,6 +7,8 @@ reads with Hammer on) on the full-size synthetic workloads:
              2^29-bit tiers, Hammer off (assemble.rs:129-298 at -t 1)
   1mb_30x    the 1 Mb / 30x pairs (kc.synth.workload_1mb_30x, seed 1), k = 21, 2^29-bit
              tiers, Hammer off and on
+  meta_0.05  the metagenome sample at 1/20 scale (kc.synth.workload_metagenome, seed 4,
+             scale 0.05: 20 genomes totalling 4.1 Mb, 1.5M pairs), k = 55, 2^29-bit tiers
 
 The oracle runs single-threaded (the reference's sequential order); the 5 Mb cases take a
 few minutes each.  Usage: python tests/golden/make_full_digests.py [--out PATH] [case ...]
@@ -30,6 +32,7 @@ CASES = {
     "5mb_100x_k55": ("5mb_100x", 55, False),
     "1mb_30x_k21": ("1mb_30x", 21, False),
     "1mb_30x_k21_hammer": ("1mb_30x", 21, True),
+    "meta_0.05_k55": ("meta_0.05", 55, False),
 }
 
 
@@ -37,6 +40,8 @@ def workload(name):
     from kc import synth
     if name == "5mb_100x":
         return synth.workload_5mb_100x(seed=2)[0]
+    if name.startswith("meta_"):
+        return synth.workload_metagenome(seed=4, scale=float(name[5:]))[0]
     return synth.workload_1mb_30x(seed=1)[0]
 
 

@@ -430,7 +430,7 @@ def bench_learned(args):
     genomes (20% repeats, 60x, 0.3% errors, qualities), k=21: trained on --train-seeds
     seeds, judged on the next --test-seeds seeds."""
     import torch
-    from kc.learn import Mlp, cutoff_is_fragile
+    from kc.learn import Mlp
     k = 21
     scale = args.scale
     Xs, ys = [], []
@@ -454,9 +454,9 @@ def bench_learned(args):
         lab, fb, ms = eng.learned_select(m, args.margin)
         ms_best = min([ms] + [eng.learned_select(m, args.margin)[2] for _ in range(2)])
         plain = d["cnt"] >= d["cutoff"]
-        # Library-level margin: a fragile cutoff (kc.learn.cutoff_is_fragile) sends the
+        # Library-level margin: a fragile cutoff (kc_library_cutoff_fragile) sends the
         # whole library to the exact selection.
-        lib_fb = bool(cutoff_is_fragile(d["cnt"]))
+        lib_fb = eng.cutoff_fragile()
         final = d["y"] if lib_fb else lab
         cost = d["t_plain"] + (d["t_exact"] if lib_fb else ms_best)
         res.append({"entries": int(lab.shape[0]), "agreement": float(np.mean(lab == d["y"])),

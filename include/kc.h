@@ -388,6 +388,20 @@ int kc_library_features(kc_ctx* ctx, uint32_t lib, float* out, size_t n);
 int kc_library_learned_select(kc_ctx* ctx, uint32_t lib, kc_mlp* mlp, float margin, uint8_t* labels,
                               uint8_t* fallback, size_t n, double* ms);
 
+/* The library-level margin of the learned first pass (no counterpart in the reference):
+ * whether the library's cutoff (determine_solid_kmer_cutoff, assemble.rs:692-735) is too
+ * fragile to trust a learned selection, so that the caller runs the exact Hammer-on
+ * selection for it instead.  Fragile: the valley search finds no valley (cutoff 2 by
+ * default), there are at most 100 entries or the 95th-percentile count is under 30, or the
+ * search's outcome changes on any of `resamples` Poisson resamplings of the histogram
+ * (seeded; the same seed gives the same decision).  Never fragile for min_coverage > 5,
+ * where the cutoff is fixed.  kc_library_cutoff_fragile reads the library's count
+ * histogram after kc_count; kc_cutoff_is_fragile takes one on the host (hist[c] = entries
+ * with count c, the last of n_bins bins those with more) and needs no GPU. */
+int kc_library_cutoff_fragile(kc_ctx* ctx, uint32_t lib, uint32_t resamples, uint64_t seed, int32_t* fragile);
+int kc_cutoff_is_fragile(const uint64_t* hist, uint32_t n_bins, uint64_t n_counts, double min_coverage,
+                         uint32_t resamples, uint64_t seed, int32_t* fragile);
+
 /* Per-kernel device time of the last run (KC_FLAG_TIMING): fills up to cap entries of
  * names (static strings), total ms and launch counts; returns the number of kernels. */
 int kc_kernel_times(const kc_ctx* ctx, const char** names, double* ms, uint32_t* launches, int cap);

@@ -3527,6 +3527,126 @@ int kc_counts_export(kc_ctx* ctx, uint64_t* keys4, uint32_t* counts, uint8_t* so
 
 }  // extern "C"
 
+// ---- learned first pass: the library-level margin -------------------------------
+//
+// Hammer correction changes the count histogram, and where determine_solid_kmer_cutoff's
+// valley search (assemble.rs:692-735) lands then decides a whole band of labels, which no
+// per-k-mer feature sees.  A library is fragile, and takes the exact selection, when the
+// search gives the default cutoff 2 without finding a valley (the labels then depend on
+// which low-count k-mers survive correction), or when its found / not-found outcome
+// changes on any of `resamples` Poisson resamplings of the histogram.  The resamplings
+// draw from SplitMix64 streams keyed by (seed, resampling), so the decision is
+// reproducible.
+
+namespace {
+
+bool valley_found(const uint64_t* h, uint32_t mvs) {
+    uint32_t valley = 2;
+    uint64_t min_val = ~0ull;
+    for (uint32_t c = 2; c <= mvs; ++c) {
+        if (h[c] <= min_val) {
+            min_val = h[c];
+            valley = c;
+        } else if (h[c] > min_val * 2 && c > valley + 2) {
+            return true;
+        }
+    }
+    return false;
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// Poisson(lam): the product of uniforms (Knuth) below lam = 10, Hormann's transformed
+// rejection with squeeze (PTRS, 1993) above.
+uint64_t poisson(double lam, uint64_t& s) {
+    if (lam <= 0.0) return 0;
+    if (lam < 10.0) {
+        const double L = std::exp(-lam);
+        uint64_t k = 0;
+        double p = uniform01(s);
+        while (p > L) {
+            ++k;
+            p *= uniform01(s);
+        }
+        return k;
+    }
+    const double slam = std::sqrt(lam), loglam = std::log(lam);
+    const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (;;) {
+        const double U = uniform01(s) - 0.5, V = uniform01(s);
+        const double us = 0.5 - std::fabs(U);
+        const double k = std::floor((2.0 * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return (uint64_t)k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (std::log(V) + std::log(invalpha) - std::log(a / (us * us) + b) <= -lam + k * loglam - std::lgamma(k + 1.0))
+            return (uint64_t)k;
+    }
+}
+
+}  // namespace
+
+// hist[c] = entries with count c for c < n_bins - 1, hist[n_bins - 1] those with more.
+static int cutoff_fragile(const uint64_t* hist, uint32_t n_bins, uint64_t n, double min_cov, uint32_t resamples,
+                          uint64_t seed) {
+    if (min_cov > 5.0) return 0;  // a fixed cutoff, max(0.35 min_cov, 2): no valley search
+    if (n <= 100) return 1;
+    const uint64_t rank = n * 95 / 100;
+    uint64_t cum = 0;
+    uint32_t top = n_bins - 1;
+    for (uint32_t c = 0; c < n_bins; ++c) {
+        cum += hist[c];
+        if (cum > rank) {
+            top = c;
+            break;
+        }
+    }
+    if (top < 30) return 1;
+    const uint32_t mvs = std::min<uint32_t>(std::max<uint32_t>(top / 4, 10), 35);
+    if (mvs >= n_bins - 1) return 1;  // the histogram does not resolve the search
+    if (!valley_found(hist, mvs)) return 1;
+    std::vector<uint64_t> h(mvs + 1, 0);
+    for (uint32_t r = 0; r < resamples; ++r) {
+        uint64_t st = seed ^ (0xD1B54A32D192ED03ull * (r + 1));
+        for (uint32_t c = 2; c <= mvs; ++c) h[c] = poisson((double)hist[c], st);
+        if (!valley_found(h.data(), mvs)) return 1;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int kc_cutoff_is_fragile(const uint64_t* hist, uint32_t n_bins, uint64_t n_counts, double min_coverage,
+                         uint32_t resamples, uint64_t seed, int32_t* fragile) {
+    if (!fragile || (n_bins && !hist) || n_bins < 2) return KC_EINVAL;
+    *fragile = cutoff_fragile(hist, n_bins, n_counts, min_coverage, resamples, seed);
+    return KC_OK;
+}
+
+int kc_library_cutoff_fragile(kc_ctx* ctx, uint32_t lib, uint32_t resamples, uint64_t seed, int32_t* fragile) {
+    if (!ctx || !fragile) return KC_EINVAL;
+    if (!ctx->counted) return ctx->fail(KC_ESTATE, "cutoff margin before kc_count");
+    if (lib >= ctx->libs.size()) return ctx->fail(KC_EINVAL, "no such library");
+    CTX_TRY(hipSetDevice(ctx->o.device));
+    std::vector<uint32_t> h32(HIST_BINS);
+    CTX_TRY(hipMemcpyAsync(h32.data(), ctx->d_hist.as<uint32_t>() + (size_t)lib * HIST_BINS, HIST_BINS * 4,
+                           hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
+    std::vector<uint64_t> h(h32.begin(), h32.end());
+    const HostLib& H = ctx->libs[lib];
+    *fragile = cutoff_fragile(h.data(), HIST_BINS, H.info.n_counts, H.min_cov, resamples, seed);
+    return KC_OK;
+}
+
+}  // extern "C"
+
 // ---- A library's solid set on the device, for the unitig builder (kc_graph.hip) -------
 
 // Solid entry i (entry e0 + idx[i] of the sorted count table) as its value V in vw words,

@@ -36,7 +36,8 @@ EXPORTS = (
     "kc_tune_get", "kc_tune_set", "kc_autotune", "kc_graph_run", "kc_graph_release", "kc_set_k",
     "kc_mlp_create", "kc_mlp_destroy", "kc_mlp_last_error", "kc_mlp_num_params", "kc_mlp_get_params",
     "kc_mlp_set_params", "kc_mlp_forward", "kc_mlp_gradient", "kc_mlp_train",
-    "kc_library_quals_set", "kc_library_features", "kc_library_learned_select",
+    "kc_library_quals_set", "kc_library_features", "kc_library_learned_select", "kc_library_cutoff_fragile",
+    "kc_cutoff_is_fragile",
     "kc_fastq_pack", "kc_fastq_last_error", "kc_packed_info_get", "kc_packed_export", "kc_packed_destroy",
     "kc_ingest_create", "kc_ingest_destroy", "kc_ingest_pack", "kc_ingest_keep_quals", "kc_packed_export_quals",
     "kc_anchor_build", "kc_anchor_count", "kc_anchor_export", "kc_anchor_query", "kc_anchor_last_error",
@@ -141,6 +142,10 @@ _SIGS = {
     "kc_library_count": (ctypes.c_int, [_vp, _u32p]),
     "kc_library_info": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_info)]),
     "kc_library_verify": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(kc_verify)]),
+    "kc_library_cutoff_fragile": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                                 ctypes.POINTER(ctypes.c_int32)]),
+    "kc_cutoff_is_fragile": (ctypes.c_int, [_u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint32,
+                                            ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]),
     "kc_library_bloom_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u64p, _sz]),
     "kc_library_bloom_query": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _sz, _u8p]),
     "kc_library_counts_export": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u32p, _u8p, _sz]),

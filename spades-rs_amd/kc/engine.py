@@ -347,6 +347,13 @@ class SpectrumEngine:
         self._chk(lib().kc_library_verify(self._h, library, ctypes.byref(v)))
         return v.as_dict()
 
+    def cutoff_fragile(self, library=0, resamples=32, seed=0):
+        """The learned pass's library-level margin (kc_library_cutoff_fragile): True when the
+        library's cutoff is too fragile for a learned selection."""
+        f = ctypes.c_int32(0)
+        self._chk(lib().kc_library_cutoff_fragile(self._h, library, int(resamples), int(seed), ctypes.byref(f)))
+        return bool(f.value)
+
     def info(self, library=0):
         info = _abi.kc_info()
         self._chk(lib().kc_library_info(self._h, library, ctypes.byref(info)))

@@ -79,40 +79,22 @@ class Mlp:
         return loss.value
 
 
-def _valley(hist):
-    """The valley search of determine_solid_kmer_cutoff (assemble.rs:692-735) over a count
-    histogram hist[0..mvs]: (cutoff, found)."""
-    valley, low = 2, None
-    for c in range(2, len(hist)):
-        h = int(hist[c])
-        if low is None or h <= low:
-            low, valley = h, c
-        elif h > low * 2 and c > valley + 2:
-            return valley, True
-    return 2, False
+def count_histogram(counts, n_bins=1024):
+    """hist[c] = entries with count c for c < n_bins - 1, hist[n_bins - 1] those with more."""
+    c = np.minimum(np.asarray(counts, dtype=np.int64), n_bins - 1)
+    return np.bincount(c, minlength=n_bins).astype(np.uint64)
 
 
 def cutoff_is_fragile(counts, min_coverage=5.0, resamples=32, seed=0):
-    """Library-level margin of the learned first pass.  Hammer correction changes the count
-    histogram, and where the cutoff's valley search lands (assemble.rs:692-735) then
-    decides a whole band of labels, which no per-k-mer feature sees.  The library is
-    fragile, and takes the exact selection, when the uncorrected cutoff is the default 2
-    (the labels then depend on which low-count k-mers survive correction), or when the
-    search's found / not-found outcome changes on any of `resamples` Poisson resamplings
-    of the histogram (seeded, so the decision is reproducible)."""
+    """The library-level margin (kc_cutoff_is_fragile in libkc.so) over a count vector:
+    True when the cutoff's valley search (assemble.rs:692-735) is too fragile for a
+    learned selection.  SpectrumEngine.cutoff_fragile gives the same for a counted
+    library without copying its counts."""
     counts = np.asarray(counts)
-    if min_coverage > 5.0:
-        return False  # a fixed cutoff: max(0.35 * min_coverage, 2), no valley search
-    if counts.shape[0] <= 100:
-        return True
-    covs = np.sort(counts.astype(np.int64))
-    top = int(covs[covs.shape[0] * 95 // 100])
-    if top < 30:
-        return True
-    mvs = min(max(top // 4, 10), 35)
-    hist = np.bincount(np.minimum(covs, mvs + 1), minlength=mvs + 2)[:mvs + 1]
-    _, found = _valley(hist)
-    if not found:
-        return True
-    rng = np.random.default_rng(seed)
-    return any(_valley(rng.poisson(hist))[1] != found for _ in range(resamples))
+    hist = count_histogram(counts)
+    f = ctypes.c_int32(0)
+    st = lib().kc_cutoff_is_fragile(ptr(hist, _abi.u64p), hist.shape[0], counts.shape[0], float(min_coverage),
+                                    int(resamples), int(seed), ctypes.byref(f))
+    if st != _abi.KC_OK:
+        raise KcError(st, "kc_cutoff_is_fragile failed")
+    return bool(f.value)

@@ -105,7 +105,8 @@ def test_network_gain_over_the_count_rule():
     count >= uncorrected cutoff keeps every low-count k-mer and misses about 10% of the
     exact labels; the network, trained on other seeds' exact Hammer-on labels, reaches
     the exact labels.  Both libraries are ones the library-level margin would send to
-    the exact selection (kc.learn.cutoff_is_fragile), which the test also checks."""
+    the exact selection (kc_library_cutoff_fragile on the device's histogram, and
+    kc_cutoff_is_fragile on the exported counts), which the test also checks."""
     from kc.learn import cutoff_is_fragile
     g = 240_000
     Xs, ys = [], []
@@ -123,7 +124,26 @@ def test_network_gain_over_the_count_rule():
         net = float(np.mean(lab == y))
         plain = float(np.mean((cnt >= cutoff) == y))
         say(f"  cutoff {cutoff}: network {net:.5f}, count rule {plain:.5f}, fallback {fb.mean():.2e}")
-        assert cutoff == 2 and cutoff_is_fragile(cnt)
+        assert cutoff == 2 and cutoff_is_fragile(cnt) and eng.cutoff_fragile()
         assert net >= 0.999 and net >= plain + 0.05
         eng.close()
     m.close()
+
+
+@pytest.mark.parametrize("seed,cov,min_cov", [(11, 15.0, 5.0), (12, 60.0, 5.0), (13, 200.0, 5.0), (14, 60.0, 12.0)])
+def test_library_margin_matches_host_rule(seed, cov, min_cov):
+    """kc_library_cutoff_fragile (the device count histogram of a counted library) gives the
+    decision kc_cutoff_is_fragile gives for the library's exported counts, for any seed."""
+    from kc.learn import cutoff_is_fragile
+    g = 200_000
+    store, _ = synth.uniform_genome_pairs(seed=seed, genome_size=g, n_pairs=int(cov * g / 300), p_start=0.005)
+    say(f"case library margin seed={seed} cov={cov} min_cov={min_cov}")
+    eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=1 << 24, min_coverage=min_cov))
+    eng.upload(store)
+    eng.run()
+    _, cnt, _ = eng.counts()
+    for rs in (0, 3):
+        assert eng.cutoff_fragile(seed=rs) == cutoff_is_fragile(cnt, min_coverage=min_cov, seed=rs)
+    if min_cov > 5.0:
+        assert not eng.cutoff_fragile()
+    eng.close()

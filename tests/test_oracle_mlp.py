@@ -54,11 +54,27 @@ def test_forward_is_logistic_of_logit_and_loss_is_bce():
     assert abs(loss - direct) < 1e-9
 
 
-def test_cutoff_fragility_host_rule(oracle):
-    """kc.learn's valley search equals the oracle's determine_solid_kmer_cutoff
-    (assemble.rs:692-735) on random count vectors, and the fragility test flags the
-    default cutoff and is reproducible."""
-    from kc.learn import _valley, cutoff_is_fragile
+def _valley(hist):
+    """The valley search of determine_solid_kmer_cutoff (assemble.rs:692-735) restated
+    over hist[0..mvs]: (cutoff, found)."""
+    valley, low = 2, None
+    for c in range(2, len(hist)):
+        h = int(hist[c])
+        if low is None or h <= low:
+            low, valley = h, c
+        elif h > low * 2 and c > valley + 2:
+            return valley, True
+    return 2, False
+
+
+def test_cutoff_fragility_rule(oracle):
+    """The library-level margin (kc_cutoff_is_fragile, host code of libkc.so, no GPU):
+    every count vector whose valley search finds nothing (the oracle's cutoff, restated
+    above, gives 2 without a valley) is fragile, and so is any with <= 100 entries or a
+    95th percentile under 30; a fixed cutoff (min_coverage > 5) never is; a deep valley
+    between large bins survives every resampling and is not; a valley that one Poisson
+    step removes is; and the decision is reproducible."""
+    from kc.learn import count_histogram, cutoff_is_fragile
     rng = np.random.default_rng(4)
     for t in range(300):
         n = int(rng.integers(101, 3000))
@@ -67,7 +83,7 @@ def test_cutoff_fragility_host_rule(oracle):
         top = int(covs[len(covs) * 95 // 100])
         want = oracle.cutoff(counts, 5.0)[0]
         if top < 30:
-            assert want == 2
+            assert want == 2 and cutoff_is_fragile(counts)
             continue
         mvs = min(max(top // 4, 10), 35)
         hist = np.bincount(np.minimum(covs.astype(np.int64), mvs + 1), minlength=mvs + 2)[:mvs + 1]
@@ -76,5 +92,17 @@ def test_cutoff_fragility_host_rule(oracle):
         if not found:
             assert cutoff_is_fragile(counts)
         assert cutoff_is_fragile(counts) == cutoff_is_fragile(counts)
+        assert cutoff_is_fragile(counts, seed=5) == cutoff_is_fragile(counts, seed=5)
     assert not cutoff_is_fragile(np.arange(500), min_coverage=12.0)
     assert cutoff_is_fragile(np.full(50, 40))
+    assert cutoff_is_fragile(np.full(5000, 20))
+    def of(bins):
+        return np.concatenate([np.full(n, c) for c, n in bins.items()])
+
+    # error k-mers at 2..4, a valley of 10 at 7, genomic k-mers around 60: robust
+    deep = of({2: 200000, 3: 50000, 4: 20000, 5: 100, 6: 40, 7: 10, 8: 30, 9: 100, 10: 1000, 11: 2000, 12: 3000,
+               13: 4000, 14: 5000, 15: 6000, 60: 100000})
+    assert _valley(count_histogram(deep)[:16]) == (7, True) and not cutoff_is_fragile(deep)
+    # the same shape with single-digit bins: Poisson resamplings move the valley past it
+    thin = of({2: 40, 3: 9, 4: 5, 5: 4, 6: 3, 7: 5, 8: 6, 9: 7, 60: 600})
+    assert _valley(count_histogram(thin)[:16]) == (6, True) and cutoff_is_fragile(thin)

@@ -579,12 +579,43 @@ constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per th
 // Transient buffers above this size are released after use instead of kept for reuse.
 constexpr uint64_t BIG_SCRATCH = 8ull << 30;
 
+// A window's packed words as loaded (3 for k <= 32, 5 for k <= 63) and its bit shift;
+// raw_extract gives the k-mer value load_window would.
+template <int KW>
+struct RawWindow {
+    uint32_t w[KW == 1 ? 3 : 5];
+    uint32_t sh;
+};
+
+template <int KW>
+__device__ __forceinline__ void raw_fetch(const uint8_t* __restrict__ data, uint64_t off, uint32_t s, RawWindow<KW>& r) {
+    static_assert(KW <= 2, "k <= 63");
+    const uint64_t b0 = off + (s >> 2);
+    const uint64_t a = b0 & ~3ull;
+    r.sh = (uint32_t)(b0 - a) * 8 + 2 * (s & 3);
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(data + a);
+#pragma unroll
+    for (int i = 0; i < (KW == 1 ? 3 : 5); ++i) r.w[i] = p[i];
+}
+
+template <int KW>
+__device__ __forceinline__ Kv raw_extract(const RawWindow<KW>& r, uint32_t k) {
+    return extract_be<KW>(r.w, r.sh, k);
+}
+
+// (byte offset, first global window) of every active read side by side, so that a
+// staged record gathers both with one 16-byte load.
+__global__ void k_pair_oe(const uint64_t* __restrict__ act_off, const uint64_t* __restrict__ act_ev, uint64_t n,
+                          ulonglong2* __restrict__ oe) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        oe[a] = make_ulonglong2(act_off[a], act_ev[a]);
+}
+
 template <int KW, typename Ev>
 __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict__ recs,
                                                      const uint64_t* __restrict__ bounds,
                                                      const uint8_t* __restrict__ data,
-                                                     const uint64_t* __restrict__ act_ev,
-                                                     const uint64_t* __restrict__ act_off, uint32_t k,
+                                                     const ulonglong2* __restrict__ act_oe, uint32_t k,
                                                      const LibDesc* __restrict__ libs, uint32_t bits, uint32_t nbins,
                                                      Slot<nw_of(KW), Ev>* __restrict__ out,
                                                      const uint64_t* __restrict__ cap0,
@@ -650,8 +681,9 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         const uint64_t rec = recs[c0 + i];
                         const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
                         n0 = (uint32_t)(rec >> 56);
-                        s_off[i] = act_off[a];
-                        s_ev[i] = act_ev[a] + first - ev0;
+                        const ulonglong2 oe = act_oe[a];
+                        s_off[i] = oe.x;
+                        s_ev[i] = oe.y + first - ev0;
                         s_j0[i] = first;
                     }
                     uint32_t inc = n0;
@@ -674,20 +706,38 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 }
                 __syncthreads();
                 const uint32_t nwin = s_woff[nr];
-                for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
-                    if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                    uint32_t lo = 0, hi = nr - 1;  // the record holding window x
+                // the record holding window x, and its window in that record
+                auto locate = [&](uint32_t x, uint32_t& rec, uint32_t& dj) {
+                    uint32_t lo = 0, hi = nr - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi + 1) >> 1;
                         if (s_woff[mid] <= x) lo = mid;
                         else hi = mid - 1;
                     }
-                    const uint32_t dj = x - s_woff[lo];
+                    rec = lo;
+                    dj = x - s_woff[lo];
+                };
+                // The packed words of the next window are fetched one iteration ahead, so
+                // their load is in flight while this window's LDS work runs.
+                RawWindow<KW> raw;
+                uint32_t rec = 0, dj = 0;
+                if (threadIdx.x < nwin) {
+                    locate(threadIdx.x, rec, dj);
+                    raw_fetch<KW>(data, s_off[rec], s_j0[rec] + dj, raw);
+                }
+                for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
+                    if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                    const RawWindow<KW> cur = raw;
+                    const uint32_t lo = rec, cdj = dj;
+                    if (x + blockDim.x < nwin) {
+                        locate(x + blockDim.x, rec, dj);
+                        raw_fetch<KW>(data, s_off[rec], s_j0[rec] + dj, raw);
+                    }
                     uint64_t key[NW];
-                    pack_key<NW>(canonical<KW>(load_window<KW>(data, s_off[lo], s_j0[lo] + dj, k), k), key);
+                    pack_key<NW>(canonical<KW>(raw_extract<KW>(cur, k), k), key);
                     const uint64_t h = table_hash<NW>(key);
                     if (parts > 1 && ((uint32_t)(h >> 40) & (parts - 1)) != part) continue;
-                    const Ev t = (Ev)(s_ev[lo] + dj);
+                    const Ev t = (Ev)(s_ev[lo] + cdj);
                     uint32_t slot = (uint32_t)h & (BIN3_CAP - 1);
                     bool done = false;
                     for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
@@ -1108,7 +1158,11 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
             if (CHECK_SOLID && base) {
                 // a slot claimed only by a prior holds no occurrence
                 const Kv v = unpack_key<NW>(sl.k);
-                if (single) {
+                // a k-mer inserted twice found its three bits set the second time and
+                // promoted (bloom.rs:95-103), so it is solid without a look
+                if (sl.cnt >= 2) {
+                    base = true;
+                } else if (single) {
                     const LibDesc& L = libs[lib0];
                     base = sl.cnt > 0 && bloom_is_solid_lazy<kw_of_nw(NW)>(v, L.k, L.solid, L.M);
                 } else {
@@ -1754,6 +1808,7 @@ struct kc_ctx {
     bool graph_planning = false;  // the eager run that plans a graph capture
     DevBuf d_corr, d_hll, d_hist, d_counters, d_temp, d_keys, d_out8;
     DevBuf d_vrec, d_vout;  // kc_library_verify: seen-tier recount, per-library results
+    DevBuf d_act_oe;        // binned count: (byte offset, first window) per active read
     DevBuf d_pseq, d_pstart, d_pw0, d_plib, d_psum, d_pflag;  // prior contigs
     DevBuf d_quals, d_rqoff, d_aqoff, d_qsum, d_feat, d_prob, d_lab;  // learned first pass
     DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
@@ -2144,6 +2199,9 @@ struct Run {
         k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec, nbins,
                                                                      c->d_bbounds.as<uint64_t>());
         KC_TRY(hipGetLastError());
+        KC_TRY(c->d_act_oe.ensure(na * 16 + 16));
+        k_pair_oe<<<grid_for(na), BLOCK, 0, st>>>(c->d_act_off.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na,
+                                                 c->d_act_oe.as<ulonglong2>());
         // count: one library writes straight into the table pool, several into staging
         const bool direct = nL == 1;
         // workgroups: one per bin, so that the dispatcher balances the uneven bins (measured
@@ -2162,7 +2220,7 @@ struct Run {
             k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 3, 0ull);
             TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
                                      c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
-                                     c->d_act_ev.as<uint64_t>(), c->d_act_off.as<uint64_t>(), k,
+                                     c->d_act_oe.as<ulonglong2>(), k,
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr)));
@@ -3026,7 +3084,7 @@ void kc_destroy(kc_ctx* ctx) {
                       &ctx->d_hi,     &ctx->d_lo,      &ctx->d_cnt,     &ctx->d_hi2,      &ctx->d_lo2,
                       &ctx->d_cnt2,   &ctx->d_idx,     &ctx->d_idx2,    &ctx->d_sflags,   &ctx->d_corr,
                       &ctx->d_hll,    &ctx->d_hist,    &ctx->d_counters, &ctx->d_temp,    &ctx->d_keys,
-                      &ctx->d_out8};
+                      &ctx->d_out8,   &ctx->d_vrec,    &ctx->d_vout,    &ctx->d_act_oe};
     for (DevBuf* b : bufs) b->release();
     ctx->graph_release();
     ctx->d_gk1.release();

@@ -499,7 +499,9 @@ def bench_many_libraries(args):
     from kc import AssemblerConfig, SpectrumEngine, synth
     k = 33
     t0 = time.time()
-    libs = synth.workload_64_libraries(seed=3, scale=args.scale)
+    libs = synth.workload_64_libraries(
+        seed=3, scale=args.scale,
+        progress=lambda i, st: log(f"library {i}: {len(st)} reads ({time.time() - t0:.0f} s)") if i % 8 == 7 else None)
     reads = sum(len(st) for st, _ in libs)
     bases = sum(int(st.lengths.astype(np.int64).sum()) for st, _ in libs)
     log(f"generated {len(libs)} libraries, {reads} reads, {bases} bases in {time.time() - t0:.1f}s")
@@ -532,12 +534,30 @@ def bench_many_libraries(args):
         f"same tables {same}")
     n_occ = sum(eng.info(i)["n_occ"] for i in range(len(libs)))
     n_counts = sum(eng.info(i)["n_counts"] for i in range(len(libs)))
+    # exactness: the oracle's digests of the libraries recorded for this scale, and the
+    # device closed-form checks (kc_library_verify) of every library
+    from kc.digest import diff, engine_digests
+    path = os.path.join(ROOT, "tests", "golden", "full_size_digests.json")
+    rec = json.load(open(path)) if os.path.exists(path) else {}
+    sampled = {}
+    for i in range(len(libs)):
+        want = rec.get(f"64lib_{args.scale}_lib{i}_k{k}")
+        if want and want["bloom_bits"] == args.bloom_bits:
+            sampled[str(i)] = not diff(engine_digests(eng, eng.info(i), library=i), want)
+    verify_ok = 0
+    for i in range(len(libs)):
+        v = eng.verify(i)
+        verify_ok += (v["seen_words_differing"] == 0 and v["solid_windows"] == v["sum_counts"]
+                      and v["unordered_pairs"] == 0 and v["mask_mismatches"] == 0)
+    log(f"oracle digests of sampled libraries: {sampled}; device checks hold for {verify_ok} of {len(libs)}")
     eng.close()
     out = {
         "metric": "k-mers/s counted bit-exact (Bloom tiers + sorted count table + cutoff + solid set)",
         "value": round(n_occ / (ms / 1e3), 1), "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded)",
+        "exact": (all(sampled.values()) and verify_ok == len(libs)) if sampled else None,
+        "exact_sampled_libraries": sampled, "device_checks_hold": f"{verify_ok}/{len(libs)}",
         "config": {"workload": f"64 libraries, genomes 5 kb-6 Mb x {args.scale}, 20-200x, reads 50-250 bp "
                                f"(lognormal, median 150), 0.3% errors", "k": k, "reads": reads, "bases": bases,
                    "n_occ": n_occ, "n_counts": n_counts,

@@ -578,6 +578,9 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
 // Transient buffers above this size are released after use instead of kept for reuse.
 constexpr uint64_t BIG_SCRATCH = 8ull << 30;
+// Device bytes Stage 3 takes per count-table entry (two copies of key words and counts,
+// sort indices and temporary storage).
+constexpr uint64_t STAGE3_BYTES_PER_ENTRY = 64;
 
 // A window's packed words as loaded (3 for k <= 32, 5 for k <= 63) and its bit shift;
 // raw_extract gives the k-mer value load_window would.
@@ -1837,6 +1840,14 @@ struct kc_ctx {
         return (kw == 2 || ev64) ? 2 : 4;
     }
     bool timing() const { return (o.flags & KC_FLAG_TIMING) != 0 && !capturing; }
+    // Whether `need` more device bytes are free now, with a 4 GiB margin.  Large transient
+    // buffers are kept between runs (reallocating tens of GB costs hundreds of ms) unless
+    // this says the next stage would lack the room.
+    bool fits(uint64_t need) const {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return true;
+        return free_b > need + (4ull << 30);
+    }
     // graph replay (kc_graph_run)
     bool capturing = false, graph_ok = false;
     hipGraphExec_t gexec = nullptr;
@@ -2177,9 +2188,9 @@ struct Run {
                                                           bits, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                                           c->d_wbin.as<uint64_t>(), c->d_bk.as<uint32_t>(),
                                                           c->d_br.as<uint64_t>())));
-            // the cut's per-window scratch is kept for the next run unless it is large
-            // (metagenome-scale inputs), where Stage 3 needs the room
-            if (c->d_wbin.bytes > BIG_SCRATCH) {
+            // the cut's per-window scratch is kept for the next run unless the device is
+            // short of room for what follows (metagenome-scale inputs)
+            if (c->d_wbin.bytes > BIG_SCRATCH && !c->fits(c->d_wbin.bytes)) {
                 KC_TRY(hipStreamSynchronize(st));
                 c->d_wbin.release();
             }
@@ -2207,7 +2218,9 @@ struct Run {
         // workgroups: one per bin, so that the dispatcher balances the uneven bins (measured
         // faster than any CU-multiple grid looping over bins); kc_tune.blocks_per_cu overrides
         const uint64_t g_max = c->tune.blocks_per_cu ? (uint64_t)c->n_cu * c->tune.blocks_per_cu : nbins;
-        const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nbins, g_max));
+        // (a launch holds at most 2^32 - 1 work-items: many libraries with many bins each
+        // loop over their bins instead)
+        const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({nbins, g_max, (uint64_t)1 << 23}));
         for (int attempt = 0;; ++attempt) {
             for (uint32_t l = 0; l < nL; ++l) cap0[l + 1] = cap0[l] + cap[l];
             if (direct) KC_TRY(c->d_tab.ensure((cap0[nL] + SPB - 1) / SPB * 64));
@@ -2232,8 +2245,13 @@ struct Run {
             if (attempt) return fail(KC_EHIP, "binned count passed its exact output sizes");
             cap = lcnt;  // the counters totalled every library's distinct k-mers
         }
-        if (c->d_bk.bytes + c->d_bk2.bytes + c->d_br.bytes + c->d_br2.bytes > BIG_SCRATCH)
-            for (DevBuf* b : {&c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2}) b->release();
+        {  // the sorted records are dead now: dropped only when Stage 3 would lack the room
+            uint64_t tot = 0;
+            for (uint32_t l = 0; l < nL; ++l) tot += lcnt[l];
+            const uint64_t rec_bytes = c->d_bk.bytes + c->d_bk2.bytes + c->d_br.bytes + c->d_br2.bytes;
+            if (rec_bytes > BIG_SCRATCH && !c->fits(tot * STAGE3_BYTES_PER_ENTRY))
+                for (DevBuf* b : {&c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2}) b->release();
+        }
         // pack: library l's entries from bucket0_l, the rest of its last bucket empty
         uint64_t b0 = 0;
         for (uint32_t l = 0; l < nL; ++l) {
@@ -2769,6 +2787,12 @@ struct Run {
         const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
         const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
         const uint64_t cap = std::max<uint64_t>(slots, 1);
+        if (c->d_hi.bytes < cap * 8 && !c->fits(cap * STAGE3_BYTES_PER_ENTRY)) {
+            // Stage 2's transients are dead here
+            KC_TRY(hipStreamSynchronize(st));
+            for (DevBuf* b : {&c->d_wbin, &c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2, &c->d_pack, &c->d_fsi, &c->d_fsi2})
+                b->release();
+        }
         KC_TRY(c->d_hi.ensure(cap * 8));
         KC_TRY(c->d_lo.ensure(cap * 8));
         KC_TRY(c->d_cnt.ensure(cap * 4));

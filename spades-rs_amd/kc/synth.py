@@ -134,6 +134,7 @@ def uneven_pairs(rng, genome, n_pairs, median_len=150.0, sigma=0.25, min_len=50,
     G = genome.shape[0]
     ar = np.arange(max_len, dtype=np.int64)
     perr = np.full(max_len, p_err)
+    nbmax = (max_len + 3) // 4
     parts, lens1, lens2 = [[], []], [], []
     for c0 in range(0, n_pairs, chunk):
         m = min(chunk, n_pairs - c0)
@@ -143,24 +144,30 @@ def uneven_pairs(rng, genome, n_pairs, median_len=150.0, sigma=0.25, min_len=50,
         r1 = _mutate(rng, genome[start[:, None] + ar[None, :]], perr)
         tail = start + frag - max_len
         r2 = _mutate(rng, 3 - genome[tail[:, None] + ar[None, ::-1]], perr)
-        # R2 reads the fragment from its end: keep its first L bases as well
+        # R2 reads the fragment from its end: keep its first L bases as well; each read's
+        # bases past L are zeroed and its packed row cut to ceil(L / 4) bytes
         for side, (rr, LL) in enumerate(((r1, L[0]), (r2, L[1]))):
-            keep = ar[None, :] < LL[:, None]
-            parts[side].append((rr[keep], LL))
+            rr = np.where(ar[None, :] < LL[:, None], rr, 0).astype(np.uint8)
+            packed = pack_fixed_length(rr)[0].reshape(m, nbmax)
+            parts[side].append(packed[np.arange(nbmax)[None, :] < ((LL + 3) // 4)[:, None]])
         lens1.append(L[0])
         lens2.append(L[1])
-    codes = [np.concatenate([c for c, _ in parts[s]]) if parts[s] else np.zeros(0, np.uint8) for s in (0, 1)]
     lengths = np.concatenate([np.concatenate(lens1) if lens1 else np.zeros(0, np.int64),
                               np.concatenate(lens2) if lens2 else np.zeros(0, np.int64)])
-    data, offsets = pack_codes(np.concatenate(codes), lengths)
+    data = np.concatenate(parts[0] + parts[1]) if n_pairs else np.zeros(0, np.uint8)
+    nbytes = (lengths + 3) // 4
+    offsets = np.zeros(lengths.shape[0], dtype=np.uint64)
+    if lengths.shape[0]:
+        offsets[1:] = np.cumsum(nbytes)[:-1].astype(np.uint64)
     return PackedReads.from_arrays(data, offsets, lengths.astype(np.uint32), pe_count=n_pairs)
 
 
-def workload_64_libraries(seed=3, scale=1.0, n_libs=64):
+def workload_64_libraries(seed=3, scale=1.0, n_libs=64, progress=None):
     """Many independent libraries: genome sizes log-uniform in [5 kb, 6 Mb], coverage
     uniform in [20x, 200x], read lengths lognormal (median 150) truncated to [50, 250],
     0.3% substitutions; `scale` multiplies every genome size (coverage is kept).
-    Returns a list of (PackedReads, meta)."""
+    progress (optional) is called with (index, store) after each library.  Returns a list
+    of (PackedReads, meta)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sizes = np.exp(rng.uniform(np.log(5_000), np.log(6_000_000), n_libs))
     covs = rng.uniform(20.0, 200.0, n_libs)
@@ -171,6 +178,8 @@ def workload_64_libraries(seed=3, scale=1.0, n_libs=64):
         n_pairs = max(int(round(covs[i] * g / (2 * 155.0))), 1)
         store = uneven_pairs(rng, genome, n_pairs)
         out.append((store, {"genome_size": g, "coverage": float(covs[i]), "n_pairs": n_pairs}))
+        if progress:
+            progress(i, store)
     return out
 
 

@@ -199,6 +199,8 @@ struct LibAccum {
 constexpr size_t LIBOUT_STRIDE = sizeof(LibOut) / sizeof(unsigned long long);
 // Counter slot (of the context's 64) receiving the number of is_solid tests Hammer makes.
 constexpr uint32_t HAMMER_QUERIES = 20;
+// Counter slot a replayed graph sets when its data-dependent sizes leave the plan.
+constexpr uint32_t PLAN_GUARD = 22;
 static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
@@ -533,7 +535,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ 
                                                    uint32_t bits, const uint32_t* __restrict__ cnt,
                                                    const uint64_t* __restrict__ sk_off,
                                                    const uint64_t* __restrict__ scratch, uint32_t* __restrict__ keys,
-                                                   uint64_t* __restrict__ recs) {
+                                                   uint64_t* __restrict__ recs, uint64_t cap) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
     for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
@@ -541,11 +543,29 @@ __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ 
         const uint64_t* src = scratch + act_ev[a];
         const uint64_t o = sk_off[a];
         const uint32_t lkey = act_lib[a] << bits;
-        for (uint32_t t = lane; t < n; t += 64) {
+        // (cap: the record buffers' room; a replayed plan that no longer fits writes
+        // nothing past it, and its guard sends the run back to planning)
+        for (uint32_t t = lane; t < n && o + t < cap; t += 64) {
             const uint64_t r = src[t];
             keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFu);
             recs[o + t] = a | ((r >> 20) & 0xFFFFFFull) << 32 | (r >> 44) << 56;
         }
+    }
+}
+
+// Guard of a replayed binned count: sets *guard when the replay's sizes leave the plan
+// (record count, per-library distinct k-mers, a bin past its split limit, an output
+// region overflow), so that the caller plans again.
+__global__ void k_plan_guard(const uint64_t* __restrict__ sko, const uint32_t* __restrict__ skc, uint64_t na,
+                             uint64_t nrec_plan, const unsigned long long* __restrict__ lcnt,
+                             const uint64_t* __restrict__ lcnt_plan, uint32_t n_libs,
+                             const unsigned long long* __restrict__ bin_ctr, unsigned long long* __restrict__ guard) {
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l <= n_libs; l += gridDim.x * blockDim.x) {
+        bool bad = false;
+        if (l < n_libs) bad = lcnt[l] != lcnt_plan[l];
+        else bad = (na && sko[na - 1] + skc[na - 1] != nrec_plan) || (bin_ctr[1] & 0xFFFFFFFFull) ||
+                   (bin_ctr[2] & 0xFFFFFFFFull);
+        if (bad) atomicOr(guard, 1ull);
     }
 }
 
@@ -1283,30 +1303,6 @@ __global__ void k_copy_bytes(const uint8_t* __restrict__ src, uint8_t* __restric
     for (uint64_t i = nw * 8 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-// Two-word sort key of a count-table entry: (hi, lo) compared as one 128-bit value.
-struct Key128 {
-    uint64_t hi, lo;
-};
-struct Key128Less {
-    __device__ __host__ bool operator()(const Key128& a, const Key128& b) const {
-        return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
-    }
-};
-
-__global__ void k_pack_keys(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo, Key128* __restrict__ out,
-                            uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = Key128{hi[i], lo[i]};
-}
-
-__global__ void k_unpack_keys(const Key128* __restrict__ in, uint64_t* __restrict__ hi, uint64_t* __restrict__ lo,
-                              uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        hi[i] = in[i].hi;
-        lo[i] = in[i].lo;
-    }
-}
-
 // The library of every sorted wide entry into the hi word array.
 __global__ void k_wide_lib(const WideEntry* __restrict__ in, uint64_t* __restrict__ hi, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -1852,9 +1848,16 @@ struct kc_ctx {
     bool capturing = false, graph_ok = false;
     hipGraphExec_t gexec = nullptr;
     uint64_t g_entries = 0;         // count-table entries of the planned run
+    // the planned run's binned count (a capture replays it): bins, records, per-library room
+    // and distinct k-mers (also on the device for the replay guard)
+    bool g_binned = false;
+    uint32_t g_memset_nodes = 0;  // memset nodes the last capture recorded (replaced by fills)
+    uint32_t g_bin_bits = 0;
+    uint64_t g_bin_nrec = 0;
+    std::vector<uint64_t> g_bin_cap, g_bin_lcnt;
+    DevBuf d_glcnt;
     uint32_t g_np = 0;              // prior sequences / windows uploaded by the last count
     uint64_t g_nw = 0;
-    DevBuf d_gk1, d_gk2;            // 128-bit sort keys of the graph's merge sort
     void graph_release() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
@@ -2100,8 +2103,15 @@ struct Run {
     // into the table, so Stage 3 lays it out for hashing first (hashed_layout, rehash).
     bool use_binned() const {
         const uint32_t ip = c->tune.insert_path;
-        if (c->kw == 4 || c->capturing || c->graph_planning || (ip != 0 && ip != 3)) return false;
+        if (c->kw == 4 || (ip != 0 && ip != 3)) return false;
         if (c->n_active == 0) return false;
+        // a graph replays the binned count without Hammer's recount or priors (both re-lay
+        // the table out for hashing, which a capture cannot size in advance)
+        if (c->capturing || c->graph_planning) {
+            if (c->o.error_correct) return false;
+            for (const HostLib& H : c->libs)
+                if (H.prior_windows(c->o.k)) return false;
+        }
         for (const HostLib& H : c->libs)
             if (H.max_len >= SK_MAX_LEN) return false;
         return true;
@@ -2132,8 +2142,11 @@ struct Run {
     // bin needed more passes than allowed (or the keys need more than 31 bits), so the
     // caller counts with the table insert instead.
     static constexpr int BIN_FALLBACK = 1;
+    // planned (a graph capture): the bins, record count and per-library sizes are those
+    // the planning run recorded (g_bin_*), nothing is read back, allocated or copied from
+    // the host, and k_plan_guard flags a replay whose sizes differ.
     template <int KW, typename Ev>
-    int binned_count() {
+    int binned_count(bool planned = false) {
         constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         using S = Slot<NW, Ev>;
@@ -2155,7 +2168,7 @@ struct Run {
             est_max = std::max(est_max, e);
             cap[l] = (exact ? e + e / 8 : e) + 1024;
         }
-        const uint32_t bits = bin_bits_for(est_max);
+        const uint32_t bits = planned ? c->g_bin_bits : bin_bits_for(est_max);
         if (lib_bits + bits > 31) return BIN_FALLBACK;
         const uint32_t nbins = nL << bits;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
@@ -2176,10 +2189,15 @@ struct Run {
             tb0 = c->d_temp.bytes;
             KC_TRY(rocprim::exclusive_scan(c->d_temp.p, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                            (uint64_t)0, (size_t)na, rocprim::plus<uint64_t>(), st));
-            KC_TRY(hipMemcpyAsync(c->h_stage, c->d_sko.as<uint64_t>() + na - 1, 8, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_skc.as<uint32_t>() + na - 1, 4, hipMemcpyDeviceToHost, st));
-            KC_TRY(hipStreamSynchronize(st));
-            nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
+            if (planned) {
+                nrec = c->g_bin_nrec;
+            } else {
+                KC_TRY(hipMemcpyAsync(c->h_stage, c->d_sko.as<uint64_t>() + na - 1, 8, hipMemcpyDeviceToHost, st));
+                KC_TRY(hipMemcpyAsync(c->h_stage + 1, c->d_skc.as<uint32_t>() + na - 1, 4, hipMemcpyDeviceToHost,
+                                      st));
+                KC_TRY(hipStreamSynchronize(st));
+                nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
+            }
             KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
             KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
             KC_TRY(c->d_br.ensure(nrec * 8 + 8));
@@ -2187,10 +2205,10 @@ struct Run {
             TIMED(KID_SK, (k_sk_copy<<<gw, BLOCK, 0, st>>>(c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na,
                                                           bits, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                                           c->d_wbin.as<uint64_t>(), c->d_bk.as<uint32_t>(),
-                                                          c->d_br.as<uint64_t>())));
+                                                          c->d_br.as<uint64_t>(), nrec)));
             // the cut's per-window scratch is kept for the next run unless the device is
             // short of room for what follows (metagenome-scale inputs)
-            if (c->d_wbin.bytes > BIG_SCRATCH && !c->fits(c->d_wbin.bytes)) {
+            if (!planned && !c->graph_planning && c->d_wbin.bytes > BIG_SCRATCH && !c->fits(c->d_wbin.bytes)) {
                 KC_TRY(hipStreamSynchronize(st));
                 c->d_wbin.release();
             }
@@ -2221,14 +2239,20 @@ struct Run {
         // (a launch holds at most 2^32 - 1 work-items: many libraries with many bins each
         // loop over their bins instead)
         const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({nbins, g_max, (uint64_t)1 << 23}));
+        if (planned) {
+            cap = c->g_bin_cap;
+            lcnt = c->g_bin_lcnt;
+        }
         for (int attempt = 0;; ++attempt) {
             for (uint32_t l = 0; l < nL; ++l) cap0[l + 1] = cap0[l] + cap[l];
             if (direct) KC_TRY(c->d_tab.ensure((cap0[nL] + SPB - 1) / SPB * 64));
             else KC_TRY(c->d_pack.ensure(std::max<uint64_t>(cap0[nL], 1) * sizeof(S)));
             KC_TRY(c->d_bcap.ensure(nL * 16 + 16));
             KC_TRY(c->d_bcnt.ensure(nL * 8 + 8));
-            KC_TRY(hipMemcpyAsync(c->d_bcap.p, cap0.data(), nL * 8, hipMemcpyHostToDevice, st));
-            KC_TRY(hipMemcpyAsync(c->d_bcap.as<uint64_t>() + nL, cap.data(), nL * 8, hipMemcpyHostToDevice, st));
+            if (!planned) {  // a replay finds the plan's room in d_bcap
+                KC_TRY(hipMemcpyAsync(c->d_bcap.p, cap0.data(), nL * 8, hipMemcpyHostToDevice, st));
+                KC_TRY(hipMemcpyAsync(c->d_bcap.as<uint64_t>() + nL, cap.data(), nL * 8, hipMemcpyHostToDevice, st));
+            }
             k_fill<unsigned long long><<<grid_for(nL), BLOCK, 0, st>>>(c->d_bcnt.as<unsigned long long>(), nL, 0ull);
             k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 3, 0ull);
             TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
@@ -2237,6 +2261,13 @@ struct Run {
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr)));
+            if (planned) {
+                k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
+                    c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
+                    c->d_glcnt.as<uint64_t>(), nL, ctr, c->d_counters.as<unsigned long long>() + PLAN_GUARD);
+                KC_TRY(hipGetLastError());
+                break;
+            }
             KC_TRY(hipMemcpyAsync(lcnt.data(), c->d_bcnt.p, nL * 8, hipMemcpyDeviceToHost, st));
             KC_TRY(hipMemcpyAsync(c->h_stage, ctr, 24, hipMemcpyDeviceToHost, st));
             KC_TRY(hipStreamSynchronize(st));
@@ -2245,12 +2276,22 @@ struct Run {
             if (attempt) return fail(KC_EHIP, "binned count passed its exact output sizes");
             cap = lcnt;  // the counters totalled every library's distinct k-mers
         }
-        {  // the sorted records are dead now: dropped only when Stage 3 would lack the room
+        if (!planned && !c->graph_planning) {
+            // the sorted records are dead now: dropped only when Stage 3 would lack the room
             uint64_t tot = 0;
             for (uint32_t l = 0; l < nL; ++l) tot += lcnt[l];
             const uint64_t rec_bytes = c->d_bk.bytes + c->d_bk2.bytes + c->d_br.bytes + c->d_br2.bytes;
             if (rec_bytes > BIG_SCRATCH && !c->fits(tot * STAGE3_BYTES_PER_ENTRY))
                 for (DevBuf* b : {&c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2}) b->release();
+        }
+        if (!planned) {  // what a capture of this run replays
+            c->g_bin_bits = bits;
+            c->g_bin_nrec = nrec;
+            c->g_bin_cap = cap;
+            c->g_bin_lcnt = lcnt;
+            KC_TRY(c->d_glcnt.ensure(nL * 8 + 8));
+            KC_TRY(hipMemcpyAsync(c->d_glcnt.p, lcnt.data(), nL * 8, hipMemcpyHostToDevice, st));
+            KC_TRY(hipStreamSynchronize(st));
         }
         // pack: library l's entries from bucket0_l, the rest of its last bucket empty
         uint64_t b0 = 0;
@@ -2267,14 +2308,16 @@ struct Run {
         for (uint32_t l = 0; l < nL; ++l) {
             const HostLib& H = c->libs[l];
             S* dst = c->d_tab.as<S>() + H.bucket0 * SPB;
-            if (!direct && lcnt[l])
-                KC_TRY(hipMemcpyAsync(dst, c->d_pack.as<S>() + cap0[l], lcnt[l] * sizeof(S), hipMemcpyDeviceToDevice,
-                                      st));
+            if (!direct && lcnt[l]) {
+                const uint64_t nb = lcnt[l] * sizeof(S);
+                k_copy_bytes<<<grid_for(nb / 8 + 1), BLOCK, 0, st>>>(
+                    reinterpret_cast<const uint8_t*>(c->d_pack.as<S>() + cap0[l]), reinterpret_cast<uint8_t*>(dst), nb);
+            }
             if (H.nbuckets * SPB > lcnt[l])
                 k_table_clear<NW, Ev><<<1, BLOCK, 0, st>>>(dst + lcnt[l], H.nbuckets * SPB - lcnt[l]);
         }
         KC_TRY(hipGetLastError());
-        return upload_desc();
+        return planned ? KC_OK : upload_desc();
     }
 
     // A packed table (left by the binned count) back to hash placement, for the passes
@@ -2430,6 +2473,7 @@ struct Run {
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         hipStream_t st = c->st;
         c->last_binned = false;
+        c->g_binned = false;
         if constexpr (KW <= 2) {
             if (use_binned()) {
                 std::vector<uint32_t> none(L(), 0);
@@ -2440,6 +2484,7 @@ struct Run {
                 rc = binned_count<KW, Ev>();
                 if (rc == KC_OK) {
                     c->last_binned = true;
+                    c->g_binned = true;
                     rc = bloom_groups<NW, Ev>(true);
                     if (rc) return rc;
                     return read_out();
@@ -2483,8 +2528,7 @@ struct Run {
     // kept out of graphs).
     template <typename Ev>
     bool sorted_setters(uint64_t fs_n) const {
-        return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32) && !c->capturing &&
-               !c->graph_planning;
+        return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32);
     }
 
     // First setters -> seen tier -> solid tier, one group of libraries at a time.
@@ -2787,7 +2831,7 @@ struct Run {
         const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
         const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
         const uint64_t cap = std::max<uint64_t>(slots, 1);
-        if (c->d_hi.bytes < cap * 8 && !c->fits(cap * STAGE3_BYTES_PER_ENTRY)) {
+        if (c->d_hi.bytes < cap * 8 && !c->graph_planning && !c->fits(cap * STAGE3_BYTES_PER_ENTRY)) {
             // Stage 2's transients are dead here
             KC_TRY(hipStreamSynchronize(st));
             for (DevBuf* b : {&c->d_wbin, &c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2, &c->d_pack, &c->d_fsi, &c->d_fsi2})
@@ -2854,15 +2898,23 @@ struct Run {
         unsigned long long* ctr = c->d_counters.as<unsigned long long>();
         (void)multi;
         // ---- Stage 2 ----
-        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 16, 0ull);
+        k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 32, 0ull);
         KC_TRY(hipGetLastError());
         int rc = clear_out();
         if (rc) return rc;
-        const uint64_t n_slots = c->total_buckets * SPB;
-        k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(), n_slots);
-        KC_TRY(hipGetLastError());
-        rc = launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
-        if (rc) return rc;
+        if constexpr (KW <= 2) {
+            if (c->g_binned) {
+                rc = binned_count<KW, Ev>(true);
+                if (rc) return rc == BIN_FALLBACK ? fail(KC_EHIP, "planned binned count fell back") : rc;
+            }
+        }
+        if (!c->g_binned) {
+            const uint64_t n_slots = c->total_buckets * SPB;
+            k_table_clear<NW, Ev><<<grid_for(n_slots), BLOCK, 0, st>>>(c->d_tab.as<Slot<NW, Ev>>(), n_slots);
+            KC_TRY(hipGetLastError());
+            rc = launch_insert<KW, Ev, true, false>(c->d_data.as<uint8_t>(), KID_INSERT);
+            if (rc) return rc;
+        }
         rc = bloom_groups<NW, Ev>(false);
         if (rc) return rc;
         // ---- Stage 3 ----
@@ -2902,43 +2954,22 @@ struct Run {
         }
         const uint32_t* psum = priors ? c->d_psum.as<uint32_t>() : nullptr;
         const uint8_t* pflag = priors ? c->d_pflag.as<uint8_t>() : nullptr;
-        // compaction into the second buffers; the merge sort writes d_hi / d_lo / d_cnt
+        // compaction, then the radix sort of the eager run (its memset nodes become fill
+        // kernels after the capture)
         if (ec)
             k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                 c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
-                c->lay, c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag,
-                c->d_wide2.as<WideEntry>());
+                c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag,
+                c->d_wide.as<WideEntry>());
         else
             k_compact<NW, Ev, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                 c->d_tab.as<Slot<NW, Ev>>(), slots, c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(), c->lay,
-                c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(), c->d_cnt2.as<uint32_t>(), ctr + 4, psum, pflag,
-                c->d_wide2.as<WideEntry>());
+                c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag,
+                c->d_wide.as<WideEntry>());
         KC_TRY(hipGetLastError());
         const uint64_t n = c->g_entries;
-        if (n) {
-            size_t tb = c->d_temp.bytes;
-            if (c->lay.wide) {
-                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
-                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
-                                           st));
-                k_wide_lib<<<grid_for(n), BLOCK, 0, st>>>(c->d_wide.as<WideEntry>(), c->d_hi.as<uint64_t>(), n);
-                KC_TRY(hipGetLastError());
-            } else if (c->lay.single) {
-                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
-                                           rocprim::less<uint64_t>(), st));
-            } else {
-                k_pack_keys<<<grid_for(n), BLOCK, 0, st>>>(c->d_hi2.as<uint64_t>(), c->d_lo2.as<uint64_t>(),
-                                                           c->d_gk1.as<Key128>(), n);
-                KC_TRY(hipGetLastError());
-                KC_TRY(rocprim::merge_sort(c->d_temp.p, tb, c->d_gk1.as<Key128>(), c->d_gk2.as<Key128>(),
-                                           c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, Key128Less(),
-                                           st));
-                k_unpack_keys<<<grid_for(n), BLOCK, 0, st>>>(c->d_gk2.as<Key128>(), c->d_hi.as<uint64_t>(),
-                                                             c->d_lo.as<uint64_t>(), n);
-                KC_TRY(hipGetLastError());
-            }
-        }
+        rc = sort_entries(n);
+        if (rc) return rc;
         const size_t nh = (size_t)L() * HIST_BINS;
         k_fill<uint32_t><<<grid_for(nh), BLOCK, 0, st>>>(c->d_hist.as<uint32_t>(), nh, 0u);
         if (n)
@@ -3010,24 +3041,8 @@ struct Run {
 
     // Buffers the planned run needs beyond the eager run's (called before capture).
     int prepare_planned() {
+        // the planning run sized the sort buffers and temporary storage for these entries
         const uint64_t n = c->g_entries;
-        size_t need = 0;
-        if (c->lay.wide) {
-            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_wide2.as<WideEntry>(), c->d_wide.as<WideEntry>(),
-                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, WideLess(),
-                                       c->st));
-        } else if (c->lay.single) {
-            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_lo2.as<uint64_t>(), c->d_lo.as<uint64_t>(),
-                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n,
-                                       rocprim::less<uint64_t>(), c->st));
-        } else {
-            KC_TRY(c->d_gk1.ensure(std::max<uint64_t>(n, 1) * sizeof(Key128)));
-            KC_TRY(c->d_gk2.ensure(std::max<uint64_t>(n, 1) * sizeof(Key128)));
-            KC_TRY(rocprim::merge_sort(nullptr, need, c->d_gk1.as<Key128>(), c->d_gk2.as<Key128>(),
-                                       c->d_cnt2.as<uint32_t>(), c->d_cnt.as<uint32_t>(), (size_t)n, Key128Less(),
-                                       c->st));
-        }
-        KC_TRY(c->d_temp.ensure(need));
         KC_TRY(c->d_hist.ensure((size_t)L() * HIST_BINS * 4));
         KC_TRY(c->d_sflags.ensure(n + 1));
         if (c->o.error_correct) KC_TRY(c->d_corr.ensure(c->d_data.bytes));
@@ -3111,8 +3126,6 @@ void kc_destroy(kc_ctx* ctx) {
                       &ctx->d_out8,   &ctx->d_vrec,    &ctx->d_vout,    &ctx->d_act_oe};
     for (DevBuf* b : bufs) b->release();
     ctx->graph_release();
-    ctx->d_gk1.release();
-    ctx->d_gk2.release();
     DevBuf* pbufs[] = {&ctx->d_pseq,  &ctx->d_pstart, &ctx->d_pw0,  &ctx->d_plib, &ctx->d_psum,
                        &ctx->d_pflag, &ctx->d_quals,  &ctx->d_rqoff, &ctx->d_aqoff, &ctx->d_qsum,
                        &ctx->d_feat,  &ctx->d_prob,   &ctx->d_lab,   &ctx->d_wide, &ctx->d_wide2};
@@ -3331,6 +3344,64 @@ int kc_run(kc_ctx* ctx, kc_info* info) {
 }
 
 // Eager run, then capture of the planned device work into ctx->gexec.
+// A memset node of a captured graph (library calls such as rocPRIM's radix sort record
+// them) as a fill kernel: width elements of elementSize bytes in each of height rows.
+__global__ void k_graph_memset(uint8_t* __restrict__ dst, uint64_t pitch, uint32_t value, uint32_t esize,
+                               uint64_t width, uint64_t height) {
+    const uint64_t n = width * height;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / width, col = i - r * width;
+        uint8_t* p = dst + r * pitch + col * esize;
+        if (esize == 4) *reinterpret_cast<uint32_t*>(p) = value;
+        else if (esize == 2) *reinterpret_cast<uint16_t*>(p) = (uint16_t)value;
+        else *p = (uint8_t)value;
+    }
+}
+
+// Every memset node of g replaced by a k_graph_memset kernel node with the same
+// dependencies: memset nodes have given wrong results on replays here, fill kernels not.
+static hipError_t replace_memset_nodes(hipGraph_t g, uint32_t* replaced) {
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
+    if (e != hipSuccess) return e;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && (e = hipGraphGetNodes(g, nodes.data(), &n)) != hipSuccess) return e;
+    *replaced = 0;
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(node, &t)) != hipSuccess) return e;
+        if (t != hipGraphNodeTypeMemset) continue;
+        hipMemsetParams mp{};
+        if ((e = hipGraphMemsetNodeGetParams(node, &mp)) != hipSuccess) return e;
+        size_t nd = 0, no = 0;
+        if ((e = hipGraphNodeGetDependencies(node, nullptr, &nd)) != hipSuccess) return e;
+        std::vector<hipGraphNode_t> deps(nd);
+        if (nd && (e = hipGraphNodeGetDependencies(node, deps.data(), &nd)) != hipSuccess) return e;
+        if ((e = hipGraphNodeGetDependentNodes(node, nullptr, &no)) != hipSuccess) return e;
+        std::vector<hipGraphNode_t> outs(no);
+        if (no && (e = hipGraphNodeGetDependentNodes(node, outs.data(), &no)) != hipSuccess) return e;
+        uint8_t* dst = static_cast<uint8_t*>(mp.dst);
+        uint32_t value = mp.value, esize = mp.elementSize ? mp.elementSize : 1;
+        uint64_t width = mp.width, height = mp.height ? mp.height : 1;
+        uint64_t pitch = mp.pitch ? mp.pitch : width * esize;
+        void* args[] = {&dst, &pitch, &value, &esize, &width, &height};
+        hipKernelNodeParams kp{};
+        kp.func = reinterpret_cast<void*>(k_graph_memset);
+        kp.gridDim = dim3(grid_for(width * height));
+        kp.blockDim = dim3(BLOCK);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t kn;
+        if ((e = hipGraphAddKernelNode(&kn, g, nd ? deps.data() : nullptr, nd, &kp)) != hipSuccess) return e;
+        for (hipGraphNode_t o : outs)
+            if ((e = hipGraphAddDependencies(g, &kn, &o, 1)) != hipSuccess) return e;
+        if ((e = hipGraphDestroyNode(node)) != hipSuccess) return e;
+        ++*replaced;
+    }
+    return hipSuccess;
+}
+
 static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
     ctx->graph_planning = true;
     int rc = kc_run(ctx, info);
@@ -3356,12 +3427,24 @@ static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
         return rc;
     }
     if (ec != hipSuccess) return ctx->fail(KC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    uint32_t n_memset = 0;
+    const hipError_t em = replace_memset_nodes(g, &n_memset);
+    if (em != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return ctx->fail(KC_EHIP, std::string("replacing memset nodes: ") + hipGetErrorString(em));
+    }
+    ctx->g_memset_nodes = n_memset;
     const hipError_t ei = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
         ctx->gexec = nullptr;
         return ctx->fail(KC_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
+    // The capture moved the sort's buffer roles on the host (it sorts into the spare
+    // buffers and swaps them in), so the planning run's results are no longer where the
+    // exports look: one replay puts them there.
+    CTX_TRY(hipGraphLaunch(ctx->gexec, ctx->st));
+    CTX_TRY(hipStreamSynchronize(ctx->st));
     ctx->graph_ok = true;
     return KC_OK;
 }
@@ -3379,11 +3462,11 @@ int kc_graph_run(kc_ctx* ctx, kc_info* info) {
     const uint32_t L = (uint32_t)ctx->libs.size();
     ctx->h_out.resize(L);
     CTX_TRY(hipMemcpyAsync(ctx->h_out.data(), ctx->d_out.p, L * sizeof(LibOut), hipMemcpyDeviceToHost, ctx->st));
-    CTX_TRY(hipMemcpyAsync(ctx->h_stage, ctx->d_counters.p, 16 * 8, hipMemcpyDeviceToHost, ctx->st));
+    CTX_TRY(hipMemcpyAsync(ctx->h_stage, ctx->d_counters.p, 32 * 8, hipMemcpyDeviceToHost, ctx->st));
     CTX_TRY(hipStreamSynchronize(ctx->st));
-    // the plan holds when no table overflowed and the compaction produced exactly the
-    // planned number of entries; otherwise plan again
-    bool ok = ctx->h_stage[4] == ctx->g_entries;
+    // the plan holds when no table overflowed, the binned count kept the planned sizes and
+    // the compaction produced exactly the planned number of entries; otherwise plan again
+    bool ok = ctx->h_stage[4] == ctx->g_entries && ctx->h_stage[PLAN_GUARD] == 0;
     for (uint32_t l = 0; l < L; ++l) ok = ok && !ctx->h_out[l].overflow;
     if (!ok) return graph_plan_and_capture(ctx, info);
     float ms = 0;

@@ -218,3 +218,26 @@ def test_sparse_first_setters(oracle):
     eng.run()
     _oracle_check(oracle, eng, libs, 21, small, False, mcs, {})
     eng.close()
+
+
+@pytest.mark.parametrize("k,bits,nlib", [(21, 1 << 18, 1), (33, 1 << 17, 4), (55, (1 << 16) + 64, 3), (63, 1 << 19, 2)])
+def test_graph_replays_the_binned_count(oracle, k, bits, nlib):
+    """Without Hammer or priors a captured graph runs the minimizer-binned count and the
+    sorted first setters with the sizes of the run that planned it (guarded on the device;
+    rocPRIM's memset nodes become fill kernels): every replay equals the eager run and the
+    oracle."""
+    libs, _ = zip(*[_uneven_library(1200 + 7 * i + k, 3000 + 2500 * i, 200 + 60 * i) for i in range(nlib)])
+    libs = list(libs)
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits), timing=True)
+    for st in libs:
+        eng.add_library(st, bloom_bits=bits)
+    say(f"case binned graph k={k} libraries={nlib}")
+    eng.run()
+    assert eng.kernel_times().get("k_bin_count", (0.0, 0))[1] > 0
+    eager = _snapshot(eng, nlib)
+    _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * nlib, {})
+    for rep in range(4):
+        say(f"  graph call {rep}")
+        eng.graph_run()
+        _same(_snapshot(eng, nlib), eager)
+    eng.close()

@@ -492,8 +492,9 @@ def bench_learned(args):
 def bench_many_libraries(args):
     """64 independent libraries (uneven read lengths 50-250 bp, 5 kb-6 Mb genomes x scale,
     20-200x, 0.3% errors) counted at k=33 in one context: Stage 2 + 3 of every library per
-    step.  value: eager runs (the binned count); with --graph, HIP graph replays (the table
-    insert, whose sizes a capture can fix) -- the other mode is timed beside it."""
+    step.  value: eager runs; with --graph, HIP graph replays (the same binned count, its
+    sizes planned by the capturing call and guarded on the device) -- the other mode is
+    timed beside it."""
     import numpy as np
     import torch
     from kc import AssemblerConfig, SpectrumEngine, synth
@@ -561,7 +562,7 @@ def bench_many_libraries(args):
         "config": {"workload": f"64 libraries, genomes 5 kb-6 Mb x {args.scale}, 20-200x, reads 50-250 bp "
                                f"(lognormal, median 150), 0.3% errors", "k": k, "reads": reads, "bases": bases,
                    "n_occ": n_occ, "n_counts": n_counts,
-                   "mode": "eager (binned count)" if eager else "hip graph replay (table insert)",
+                   "mode": "eager (binned count)" if eager else "hip graph replay (binned count)",
                    ("graph_replay_ms_per_step" if eager else "eager_ms_per_step"): round(other_ms, 3),
                    "both_modes_same_tables": same,
                    "tune": tune, "first_call_ms": round(first["ms_stage2"] + first["ms_stage3"], 3)},

@@ -325,15 +325,19 @@ typedef struct kc_tune {
                                     32-bit events and first-setter groups under 2^32 entries,
                                     outside graph capture, and otherwise the atomics below.  1 = three device-scope
                                     atomicMin per distinct k-mer */
+    uint32_t kmers_per_bin;      /* binned count: distinct k-mers a bin is sized for (0: 320;
+                                    else 64..4096) */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;
 
 int kc_tune_get(const kc_ctx* ctx, kc_tune* tune);
 int kc_tune_set(kc_ctx* ctx, const kc_tune* tune);
-/* Time Stage 2 + 3 over the uploaded libraries for a coordinate search over the load
- * factor, the windows per thread and the workgroups per CU (reps timed runs each, after
- * one warm-up), check every setting gives the same per-library summary, and keep the
- * fastest.  *best (nullable) receives it. */
+/* Time Stage 2 + 3 over the uploaded libraries for a coordinate search (reps timed runs
+ * each, after one warm-up) over the knobs of the Stage 2 path the run takes: with the
+ * binned count, the distinct k-mers a bin is sized for and the workgroups per CU; with
+ * the table insert, the load factor, the windows per thread and the workgroups per CU;
+ * then the first-setter path.  Every setting must give the same per-library summary; the
+ * fastest is kept.  *best (nullable) receives it. */
 int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best);
 
 /* Stage 2 + Stage 3 as one HIP graph launch.  The first call (and the first after new

@@ -1813,7 +1813,7 @@ struct kc_ctx {
     DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
     uint64_t n_entries = 0;
     SortLayout lay{};
-    kc_tune tune{1, 0, 0.0f, 0, 0, 0.0};  // k_insert shape and table load (kc_tune_set / kc_autotune)
+    kc_tune tune{1, 0, 0.0f, 0, 0, 0, 0.0};  // launch shapes, table load, paths, bin size (kc_tune_set / kc_autotune)
     uint32_t n_cu = 0;
     uint32_t kw = 1;  // value words of the current k (1, 2 or 4)
     bool ev64 = false, built = false, counted = false;
@@ -1850,7 +1850,7 @@ struct kc_ctx {
     uint64_t g_entries = 0;         // count-table entries of the planned run
     // the planned run's binned count (a capture replays it): bins, records, per-library room
     // and distinct k-mers (also on the device for the replay guard)
-    bool g_binned = false;
+    bool g_binned = false;        // the last Stage 2 took the binned count
     uint32_t g_memset_nodes = 0;  // memset nodes the last capture recorded (replaced by fills)
     uint32_t g_bin_bits = 0;
     uint64_t g_bin_nrec = 0;
@@ -2098,9 +2098,9 @@ struct Run {
     }
 
     // Whether Stage 2 takes the minimizer-binned count: the default (insert_path 0) and
-    // insert_path 3, k <= 63, reads shorter than 2^24, and not for a graph capture or the
-    // eager run that plans one.  Hammer's recount and the priors hash
-    // into the table, so Stage 3 lays it out for hashing first (hashed_layout, rehash).
+    // insert_path 3, k <= 63, reads shorter than 2^24; in a graph capture (and the eager
+    // run that plans one) only without Hammer and priors.  Hammer's recount and the priors
+    // hash into the table, so Stage 3 lays it out for hashing first (hashed_layout, rehash).
     bool use_binned() const {
         const uint32_t ip = c->tune.insert_path;
         if (c->kw == 4 || (ip != 0 && ip != 3)) return false;
@@ -2168,7 +2168,9 @@ struct Run {
             est_max = std::max(est_max, e);
             cap[l] = (exact ? e + e / 8 : e) + 1024;
         }
-        const uint32_t bits = planned ? c->g_bin_bits : bin_bits_for(est_max);
+        const uint32_t bits = planned ? c->g_bin_bits
+                                      : bin_bits_for(est_max, c->tune.kmers_per_bin ? c->tune.kmers_per_bin
+                                                                                   : BIN_TARGET);
         if (lib_bits + bits > 31) return BIN_FALLBACK;
         const uint32_t nbins = nL << bits;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
@@ -4007,6 +4009,8 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->insert_path > 3 || tune->insert_path == 2)
         return ctx->fail(KC_EINVAL, "insert_path must be 0, 1 or 3 (the partitioned count, 2, was removed)");
     if (tune->setter_path > 1) return ctx->fail(KC_EINVAL, "setter_path must be 0 or 1");
+    if (tune->kmers_per_bin && (tune->kmers_per_bin < 64 || tune->kmers_per_bin > 4096))
+        return ctx->fail(KC_EINVAL, "kmers_per_bin must be 0 or in 64..4096");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
     return KC_OK;
@@ -4025,7 +4029,7 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
         int rc = kc_tune_set(ctx, &t);
         if (rc) return rc;
         double m = 1e300;
-        for (uint32_t r = 0; r <= reps; ++r) {  // run 0 is the warm-up
+        for (uint32_t r = 0; r <= reps; ++r) {  // run 0 is the warm-up (and sizes the bins)
             kc_info info{};
             rc = kc_run(ctx, &info);
             if (rc) return rc;
@@ -4040,26 +4044,55 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
         return KC_OK;
     };
     kc_tune cur = ctx->tune;
-    if (cur.table_load == 0.0f) cur.table_load = (ctx->o.table_load > 0.05f && ctx->o.table_load < 0.95f)
-                                                     ? ctx->o.table_load : 0.5f;
     double cur_ms = 0;
     int rc = trial(cur, &cur_ms);
     if (rc) return rc;
-    // coordinate search: load factor, then windows in flight, then workgroups per CU
-    const float loads[] = {0.35f, 0.5f, 0.65f, 0.8f};
-    const uint32_t wpts[] = {1, 2, 4};
-    const uint32_t bpcs[] = {0, 4, 8, 16, 32};
-    for (int dim = 0; dim < 3; ++dim) {
-        const int n = dim == 0 ? 4 : (dim == 1 ? 3 : 5);
-        kc_tune base = cur;
-        for (int i = 0; i < n; ++i) {
+    // The knobs of the Stage 2 path this input takes.  Binned count: distinct k-mers a bin
+    // is sized for (passes over the LDS table against workgroups), then workgroups per CU
+    // (0: one per bin).  Table insert: load factor, windows in flight, workgroups per CU.
+    // Then the first-setter path for either.
+    const bool binned = ctx->g_binned;  // the last Stage 2 took the binned count
+    if (!binned && cur.table_load == 0.0f)
+        cur.table_load = (ctx->o.table_load > 0.05f && ctx->o.table_load < 0.95f) ? ctx->o.table_load : 0.5f;
+    struct Dim {
+        int field;
+        std::vector<double> values;
+    };
+    enum { F_LOAD, F_WPT, F_BPC, F_KPB, F_SETTER };
+    std::vector<Dim> dims;
+    if (binned) {
+        dims.push_back({F_KPB, {192, 256, 320, 448, 640}});
+        dims.push_back({F_BPC, {0, 4, 8, 16}});
+    } else {
+        dims.push_back({F_LOAD, {0.35, 0.5, 0.65, 0.8}});
+        dims.push_back({F_WPT, {1, 2, 4}});
+        dims.push_back({F_BPC, {0, 4, 8, 16, 32}});
+    }
+    dims.push_back({F_SETTER, {0, 1}});
+    auto get = [](const kc_tune& t, int f) -> double {
+        switch (f) {
+            case F_LOAD: return t.table_load;
+            case F_WPT: return t.windows_per_thread;
+            case F_BPC: return t.blocks_per_cu;
+            case F_KPB: return t.kmers_per_bin ? t.kmers_per_bin : BIN_TARGET;
+            default: return t.setter_path;
+        }
+    };
+    auto set = [](kc_tune& t, int f, double v) {
+        switch (f) {
+            case F_LOAD: t.table_load = (float)v; break;
+            case F_WPT: t.windows_per_thread = (uint32_t)v; break;
+            case F_BPC: t.blocks_per_cu = (uint32_t)v; break;
+            case F_KPB: t.kmers_per_bin = (uint32_t)v; break;
+            default: t.setter_path = (uint32_t)v; break;
+        }
+    };
+    for (const Dim& d : dims) {
+        const kc_tune base = cur;
+        for (double v : d.values) {
+            if (v == get(base, d.field)) continue;
             kc_tune t = base;
-            if (dim == 0) t.table_load = loads[i];
-            else if (dim == 1) t.windows_per_thread = wpts[i];
-            else t.blocks_per_cu = bpcs[i];
-            if (t.table_load == base.table_load && t.windows_per_thread == base.windows_per_thread &&
-                t.blocks_per_cu == base.blocks_per_cu)
-                continue;
+            set(t, d.field, v);
             double ms = 0;
             rc = trial(t, &ms);
             if (rc) return rc;

@@ -75,10 +75,10 @@ __host__ __device__ __forceinline__ uint32_t minimizer_bin(uint32_t minkey, uint
     return (uint32_t)(c >> (64 - bits));
 }
 
-// Bin bits for an expected number of distinct k-mers.
-inline uint32_t bin_bits_for(uint64_t distinct) {
+// Bin bits for an expected number of distinct k-mers, about `target` a bin.
+inline uint32_t bin_bits_for(uint64_t distinct, uint32_t target = BIN_TARGET) {
     uint32_t b = BIN_BITS_MIN;
-    while (b < BIN_BITS_MAX && ((uint64_t)BIN_TARGET << b) < distinct) ++b;
+    while (b < BIN_BITS_MAX && ((uint64_t)target << b) < distinct) ++b;
     return b;
 }
 

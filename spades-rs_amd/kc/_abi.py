@@ -99,7 +99,7 @@ class kc_unitigs_info(ctypes.Structure):
 class kc_tune(ctypes.Structure):
     _fields_ = [("windows_per_thread", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("table_load", ctypes.c_float), ("insert_path", ctypes.c_uint32),
-                ("setter_path", ctypes.c_uint32), ("ms", ctypes.c_double)]
+                ("setter_path", ctypes.c_uint32), ("kmers_per_bin", ctypes.c_uint32), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

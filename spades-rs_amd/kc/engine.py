@@ -299,18 +299,25 @@ class SpectrumEngine:
         self._chk(lib().kc_tune_get(self._h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, windows_per_thread=1, blocks_per_cu=0, table_load=0.0, insert_path=0, setter_path=0):
+    def set_tune(self, windows_per_thread=1, blocks_per_cu=0, table_load=0.0, insert_path=0, setter_path=0,
+                 kmers_per_bin=0):
         """insert_path: 0 (default) or 3 minimizer-binned count where it applies (k <= 63,
-        reads under 2^24 bases, no graph capture), else per-window table atomics; 1 per-window
-        table atomics.  setter_path: 0 (default) Bloom first setters by sorting (entry, setter) items by partition and taking minima in LDS where it applies
-        (dense mode, 32-bit events), else device atomics; 1 device atomics."""
+        reads under 2^24 bases, no prior contigs or Hammer recount inside a graph capture),
+        else per-window table atomics; 1 per-window table atomics.  setter_path: 0 (default)
+        Bloom first setters by sorting (entry, setter) items by partition and taking minima
+        in LDS where it applies (dense mode, 32-bit events), else device atomics; 1 device
+        atomics.  kmers_per_bin: distinct k-mers a bin of the binned count is sized for (0:
+        320).  Every setting gives the same results."""
         t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
-                         table_load=float(table_load), insert_path=insert_path, setter_path=setter_path)
+                         table_load=float(table_load), insert_path=insert_path, setter_path=setter_path,
+                         kmers_per_bin=kmers_per_bin)
         self._chk(lib().kc_tune_set(self._h, ctypes.byref(t)))
 
     def autotune(self, reps=2):
-        """Coordinate search over load factor, windows per thread and workgroups per CU on
-        the uploaded libraries; keeps and returns the fastest setting."""
+        """Coordinate search over the knobs of the Stage 2 path the run takes (binned
+        count: k-mers per bin and workgroups per CU; table insert: load factor, windows per
+        thread and workgroups per CU), then the first-setter path, on the uploaded
+        libraries; every setting must give the same results; keeps and returns the fastest."""
         t = _abi.kc_tune()
         self._chk(lib().kc_autotune(self._h, int(reps), ctypes.byref(t)))
         return t.as_dict()

@@ -82,19 +82,32 @@ def test_graph_replays_match_eager_and_oracle(oracle, k, ec):
     eng.close()
 
 
-@pytest.mark.parametrize("wpt,bpc,load", [(1, 0, 0.5), (2, 8, 0.35), (4, 16, 0.8), (4, 1, 0.65), (2, 0, 0.9)])
-def test_tune_settings_are_exact(oracle, wpt, bpc, load):
+@pytest.mark.parametrize("wpt,bpc,load,kpb,ip", [(1, 0, 0.5, 0, 1), (2, 8, 0.35, 0, 1), (4, 16, 0.8, 0, 1),
+                                               (4, 1, 0.65, 0, 1), (2, 0, 0.9, 0, 1), (1, 0, 0.0, 64, 0),
+                                               (1, 4, 0.0, 192, 0), (1, 16, 0.0, 640, 0), (1, 1, 0.0, 4096, 0)])
+def test_tune_settings_are_exact(oracle, wpt, bpc, load, kpb, ip):
+    """Launch shapes and load factors of the table insert (ip 1), and bin sizes and
+    workgroups per CU of the binned count (ip 0): every setting equals the oracle."""
     libs, genomes = zip(*[_uneven_library(40 + i, 3000 + 900 * i, 200, err=0.006) for i in range(3)])
     libs = list(libs)
     k, bits = 31, (1 << 16) + 64
     eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits))
     for st in libs:
         eng.add_library(st, bloom_bits=bits)
-    eng.set_tune(windows_per_thread=wpt, blocks_per_cu=bpc, table_load=load)
-    assert eng.tune()["windows_per_thread"] == wpt
-    say(f"case tune wpt={wpt} bpc={bpc} load={load}")
-    eng.run()
-    _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * 3, {})
+    eng.set_tune(windows_per_thread=wpt, blocks_per_cu=bpc, table_load=load, kmers_per_bin=kpb, insert_path=ip)
+    assert eng.tune()["windows_per_thread"] == wpt and eng.tune()["kmers_per_bin"] == kpb
+    say(f"case tune wpt={wpt} bpc={bpc} load={load} kpb={kpb} ip={ip}")
+    for rep in range(2):  # the second run sizes its bins from the first's exact count
+        eng.run()
+        _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * 3, {})
+    eng.close()
+
+
+def test_tune_rejects_bad_bin_size():
+    eng = SpectrumEngine(AssemblerConfig(k=21, bloom_bits=1 << 16))
+    for bad in (1, 63, 4097):
+        with pytest.raises(Exception, match="kmers_per_bin"):
+            eng.set_tune(kmers_per_bin=bad)
     eng.close()
 
 
@@ -114,6 +127,30 @@ def test_autotune_keeps_results(oracle):
     eng.graph_run()
     eng.graph_run()
     _oracle_check(oracle, eng, libs, k, bits, True, [5.0] * 2, {})
+    eng.close()
+
+
+def test_autotune_on_the_binned_path(oracle):
+    """Without Hammer the input takes the binned count: the search covers its bin size and
+    workgroups per CU and the first-setter path, and the chosen setting stays exact,
+    eagerly and replayed."""
+    libs, _ = zip(*[_uneven_library(95 + i, 5000, 300) for i in range(3)])
+    libs = list(libs)
+    k, bits = 33, 1 << 18
+    eng = SpectrumEngine(AssemblerConfig(k=k, bloom_bits=bits), timing=True)
+    for st in libs:
+        eng.add_library(st, bloom_bits=bits)
+    say("case autotune binned")
+    best = eng.autotune(reps=1)
+    assert best["ms"] > 0 and best["kmers_per_bin"] in (0, 192, 256, 320, 448, 640)
+    assert best["blocks_per_cu"] in (0, 4, 8, 16) and best["setter_path"] in (0, 1)
+    assert eng.tune() == best
+    eng.run()
+    assert "k_bin_count" in eng.kernel_times()
+    _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * 3, {})
+    eng.graph_run()
+    eng.graph_run()
+    _oracle_check(oracle, eng, libs, k, bits, False, [5.0] * 3, {})
     eng.close()
 
 

@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--insert-path", type=int, default=0,
                     help="Stage 2 count: 0 or 3 minimizer-binned where it applies, 1 table atomics")
     ap.add_argument("--fasta", action="store_true", help="ingest: the pairs as multi-line FASTA files")
+    ap.add_argument("--sequential", action="store_true",
+                    help="default workloads: run the per-k contexts one after another instead of together "
+                         "(kc_run_many)")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="workgroups per CU of the table insert / bin count (0: default shape)")
     ap.add_argument("--setter-path", type=int, default=0,
@@ -128,9 +131,18 @@ def main():
         engines.append(eng)
     log("uploaded")
 
-    def step():
+    from kc import run_many
+
+    def step_sequential():
         return [e.run() for e in engines]
 
+    def step_concurrent():
+        return run_many(engines)
+
+    # one context per k over the same resident reads; without prior contigs the ks are
+    # independent, so by default they run together (kc_run_many: a host thread and stream
+    # each) and a step ends when all three have
+    step = step_sequential if args.sequential else step_concurrent
     cold_ms = None
     for i in range(args.warmup):
         c0 = time.perf_counter()
@@ -158,6 +170,26 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = n_occ / (ms_per_step / 1e3)
     log(f"steps: {', '.join(f'{x:.1f}' for x in step_ms)} ms")
+    seq_ms = ms_per_step if args.sequential else None
+    if not args.sequential:
+        # Kernels of contexts running together share the device, so their event times are
+        # not one kernel's own duration: the per-kernel figures (and the dominant kernel's
+        # roofline) come from the same number of steps run one context after another.
+        kernel_ms = {}
+        for e in engines:
+            e.kernel_times()
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for i in range(args.steps):
+            step_sequential()
+            for k, e in zip(ks, engines):
+                for name, (ms, n) in e.kernel_times().items():
+                    acc = kernel_ms.setdefault((k, name), [0.0, 0])
+                    acc[0] += ms
+                    acc[1] += n
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - s0) * 1e3 / args.steps
+        log(f"sequential steps: {seq_ms:.2f} ms a step")
 
     # dominant kernel and its roofline.  Algorithmic bytes per unit of work: one k-mer
     # occurrence moves s_k packed-stream bytes plus one random 64 B count-table access
@@ -249,7 +281,10 @@ def main():
                                                       "n_solid": per_k[k]["n_solid"],
                                                       "ms": round(per_k[k]["ms_stage2"] + per_k[k]["ms_stage3"], 3)}
                                              for k in ks},
-                   "cold_first_step_ms": round(cold_ms, 3) if cold_ms is not None else None},
+                   "cold_first_step_ms": round(cold_ms, 3) if cold_ms is not None else None,
+                   "schedule": "sequential contexts" if args.sequential else
+                               "one context per k, run together (kc_run_many)",
+                   "ms_per_step_sequential": round(seq_ms, 3)},
         "roofline": roof, "path_roofline": path, "cpu_baseline": cpu, "exact_detail": exact,
         "tune": {str(k): e.tune() for k, e in zip(ks, engines)},
         "microbench": micro,
@@ -581,7 +616,9 @@ def bench_meta(args):
     from kc import AssemblerConfig, SpectrumEngine, synth
     k = 55
     t0 = time.time()
-    store, meta = synth.workload_metagenome(seed=4, scale=args.meta_scale)
+    store, meta = synth.workload_metagenome(
+        seed=4, scale=args.meta_scale,
+        progress=lambda g, n: log(f"genome {g}: {n} pairs drawn ({time.time() - t0:.1f}s)"))
     log(f"generated {len(store)} reads ({int(meta['genome_sizes'].sum())} genome bases) in {time.time() - t0:.1f}s")
     lens = store.lengths.astype(np.int64)
     stream = int(((lens + 3) // 4)[lens >= k].sum())

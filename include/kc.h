@@ -143,6 +143,13 @@ int kc_bloom_build(kc_ctx* ctx);
 int kc_count(kc_ctx* ctx, kc_info* info);
 /* Stage 2 + Stage 3. */
 int kc_run(kc_ctx* ctx, kc_info* info);
+/* kc_run of n independent contexts at once, each from its own host thread on its own
+ * stream, so that one context's kernels fill the device while another waits on a size
+ * read-back or runs a kernel bound by another resource (the multi-k loop without prior
+ * contigs: one context per k over the same reads).  infos may be NULL, else n entries.
+ * Each context's results are those of kc_run alone.  Returns the first failing context's
+ * status (the others still finish); ctxs[i] must be distinct. */
+int kc_run_many(kc_ctx* const* ctxs, uint32_t n, kc_info* infos);
 int kc_info_get(const kc_ctx* ctx, kc_info* info);
 
 /* Both Bloom tiers after Stage 2 (num_words must equal kc_info.num_words). */
@@ -327,6 +334,8 @@ typedef struct kc_tune {
                                     atomicMin per distinct k-mer */
     uint32_t kmers_per_bin;      /* binned count: distinct k-mers a bin is sized for (0: 320;
                                     else 64..4096) */
+    uint32_t minimizer_len;      /* binned count: m of the canonical m-mers whose minimum bins a
+                                    window (0: the default for k; else 1..16, at most k) */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;
 

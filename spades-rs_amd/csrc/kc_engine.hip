@@ -36,6 +36,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <rocprim/device/device_merge_sort.hpp>
@@ -430,15 +431,15 @@ __device__ __forceinline__ bool lds_claim_word(uint64_t* w, uint64_t v, bool& cl
 // window's.  The lane at such a start writes the record of the run that ends before it;
 // the read's last run is written after the loop.  A read has at most as many records as
 // windows, so its records go to scratch from its global window number act_ev[a] on, as
-// bin | first window << 20 | windows << 44 (bins < 2^20, first window < 2^24), and its
+// bin | first window << 24 | windows << 48 (bins < 2^24, first window < 2^24), and its
 // record count to cnt[a]; k_sk_copy then packs them in read order.  No lane diverges on
 // the data.  (Reserving packed room chunk by chunk on one global counter was measured
 // 40x slower: the same-address atomics serialize.  Counting first and cutting again, the
 // second pass reading back every window's bin, was about 0.6 ms a k slower than this
 // scratch on 5 Mb / 100x.)
-static_assert(BIN_BITS_MAX <= 20, "scratch records hold 20-bit bins");
+static_assert(BIN_BITS_MAX <= 24, "scratch records hold 24-bit bins");
 __device__ __forceinline__ uint64_t sk_scratch(uint32_t bin, uint32_t first, uint32_t nw) {
-    return (uint64_t)bin | (uint64_t)first << 20 | (uint64_t)nw << 44;
+    return (uint64_t)bin | (uint64_t)first << 24 | (uint64_t)nw << 48;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
@@ -446,11 +447,11 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
                                                    const uint32_t* __restrict__ active,
                                                    const uint64_t* __restrict__ nwin,
                                                    const uint64_t* __restrict__ act_ev, uint64_t n_active, uint32_t k,
-                                                   uint32_t bits, uint32_t* __restrict__ cnt,
+                                                   uint32_t m, uint32_t bits, uint32_t* __restrict__ cnt,
                                                    uint64_t* __restrict__ scratch) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
-    const uint32_t m = minimizer_len(k), w = k - m + 1;
+    const uint32_t w = k - m + 1;
     uint32_t p = 1;  // largest power of two <= w
     while (2 * p <= w) p <<= 1;
     const uint64_t mmask = (1ull << (2 * m)) - 1;
@@ -547,8 +548,8 @@ __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ 
         // nothing past it, and its guard sends the run back to planning)
         for (uint32_t t = lane; t < n && o + t < cap; t += 64) {
             const uint64_t r = src[t];
-            keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFu);
-            recs[o + t] = a | ((r >> 20) & 0xFFFFFFull) << 32 | (r >> 44) << 56;
+            keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFFu);
+            recs[o + t] = a | ((r >> 24) & 0xFFFFFFull) << 32 | (r >> 48) << 56;
         }
     }
 }
@@ -596,6 +597,7 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
 // insert instead).
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
+constexpr uint32_t BIN3_MAP = 2048;      // windows per chunk (the window -> record map)
 // Transient buffers above this size are released after use instead of kept for reuse.
 constexpr uint64_t BIG_SCRATCH = 8ull << 30;
 // Device bytes Stage 3 takes per count-table entry (two copies of key words and counts,
@@ -610,16 +612,28 @@ struct RawWindow {
     uint32_t sh;
 };
 
+// The window whose first base is base number `pos` of the packed stream (4 bases a byte).
 template <int KW>
-__device__ __forceinline__ void raw_fetch(const uint8_t* __restrict__ data, uint64_t off, uint32_t s, RawWindow<KW>& r) {
+__device__ __forceinline__ void raw_fetch(const uint8_t* __restrict__ data, uint64_t pos, RawWindow<KW>& r) {
     static_assert(KW <= 2, "k <= 63");
-    const uint64_t b0 = off + (s >> 2);
+    const uint64_t b0 = pos >> 2;
     const uint64_t a = b0 & ~3ull;
-    r.sh = (uint32_t)(b0 - a) * 8 + 2 * (s & 3);
+    r.sh = (uint32_t)(b0 - a) * 8 + 2 * (uint32_t)(pos & 3);
     const uint32_t* p = reinterpret_cast<const uint32_t*>(data + a);
 #pragma unroll
     for (int i = 0; i < (KW == 1 ? 3 : 5); ++i) r.w[i] = p[i];
 }
+
+// Start slot (bits 0..9) and split-pass bits (20..31) of a key in a bin's LDS table: the
+// key words folded to 32 bits and mixed by fmix32.  Only the probe start and the pass of a
+// key depend on it, never the result, so a cheap hash serves (the keys are compared whole).
+template <int NW>
+__device__ __forceinline__ uint32_t bin_slot_hash(const uint64_t key[NW]) {
+    uint32_t x = (uint32_t)key[0] ^ ((uint32_t)(key[0] >> 32) * 0x9E3779B1u);
+    if (NW == 2) x ^= ((uint32_t)key[NW - 1] + 0x7F4A7C15u) * 0x85EBCA77u ^ (uint32_t)(key[NW - 1] >> 32);
+    return mix32(x);
+}
+static_assert(BIN3_CAP <= (1u << 20) && BIN3_MAX_SPLIT <= (1u << 12), "slot and pass bits of bin_slot_hash");
 
 template <int KW>
 __device__ __forceinline__ Kv raw_extract(const RawWindow<KW>& r, uint32_t k) {
@@ -651,16 +665,17 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     __shared__ uint64_t s_k1[NW == 2 ? BIN3_CAP : 1];
     __shared__ uint32_t s_cnt[BIN3_CAP];
     __shared__ Ev s_t0[BIN3_CAP];
-    __shared__ uint64_t s_off[BIN3_RCHUNK];  // staged record: byte offset of its read,
-    __shared__ uint64_t s_ev[BIN3_RCHUNK];   // event of its first window (library-relative),
-    __shared__ uint32_t s_j0[BIN3_RCHUNK];   // its first window,
-    __shared__ uint32_t s_woff[BIN3_RCHUNK + 1];  // and its first window in the chunk
-    __shared__ uint16_t s_list[BIN3_FILL];        // the pass's claimed slots, in claim order
-    __shared__ uint32_t s_wsum[BLOCK / 64];
-    __shared__ uint32_t s_fill, s_over, s_np;
+    __shared__ uint64_t s_pos[BIN3_RCHUNK];  // staged record: stream base number of its first window,
+    __shared__ Ev s_ev[BIN3_RCHUNK];         // event of its first window (library-relative),
+    __shared__ uint32_t s_woff[BIN3_RCHUNK];  // and its first window in the chunk
+    __shared__ uint8_t s_map[BIN3_MAP];       // the staged record of every window of the chunk
+    __shared__ uint16_t s_list[BIN3_FILL];    // the pass's claimed slots, in claim order
+    __shared__ uint32_t s_wsum[BLOCK / 64], s_wfit[BLOCK / 64];
+    __shared__ uint32_t s_fill, s_over, s_np, s_nr, s_nwin;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
     static_assert(BIN3_CAP <= 65536, "slot numbers are 16-bit");
+    static_assert(BIN3_RCHUNK <= 256 && BIN3_MAP >= SK_MAX_WINDOWS, "map entries are 8-bit, a record fits a chunk");
     // The table starts empty and every pass leaves it empty: its claimed slots are listed
     // and reset as they are written out; a pass that overflows (the list is then
     // incomplete) clears the whole table.
@@ -693,11 +708,12 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 s_over = 0;
                 --s_np;
             }
-            for (uint64_t c0 = rb; c0 < re; c0 += BIN3_RCHUNK) {
+            for (uint64_t c0 = rb; c0 < re;) {
                 const uint32_t nr = (uint32_t)min<uint64_t>(BIN3_RCHUNK, re - c0);
                 __syncthreads();  // the previous chunk's staging reads are done
                 if (s_over) break;
-                {  // stage one record per thread; scan of the window counts (waves, then across)
+                {  // stage one record per thread; scan of the window counts (waves, then across);
+                   // the chunk takes the leading records whose windows fit the map
                     const uint32_t i = threadIdx.x, lane = i & 63, wave = i >> 6;
                     uint32_t n0 = 0;
                     if (i < nr) {
@@ -705,9 +721,8 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
                         n0 = (uint32_t)(rec >> 56);
                         const ulonglong2 oe = act_oe[a];
-                        s_off[i] = oe.x;
-                        s_ev[i] = oe.y + first - ev0;
-                        s_j0[i] = first;
+                        s_pos[i] = oe.x * 4 + first;
+                        s_ev[i] = (Ev)(oe.y + first - ev0);
                     }
                     uint32_t inc = n0;
 #pragma unroll
@@ -717,28 +732,34 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     }
                     if (lane == 63) s_wsum[wave] = inc;
                     __syncthreads();
-                    uint32_t before = 0, total = 0;
+                    uint32_t before = 0;
 #pragma unroll
-                    for (uint32_t w = 0; w < BLOCK / 64; ++w) {
-                        const uint32_t t = s_wsum[w];
-                        before += w < wave ? t : 0u;
-                        total += t;
+                    for (uint32_t w = 0; w < BLOCK / 64; ++w) before += w < wave ? s_wsum[w] : 0u;
+                    const uint32_t end = before + inc;  // the windows up to and including record i
+                    const bool fits = i < nr && end <= BIN3_MAP;  // a prefix of the records
+                    const uint64_t fit = __ballot(fits);
+                    if (lane == 0) s_wfit[wave] = (uint32_t)__popcll(fit);
+                    if (i < nr) s_woff[i] = end - n0;
+                    __syncthreads();
+                    uint32_t nfit = 0;
+#pragma unroll
+                    for (uint32_t w = 0; w < BLOCK / 64; ++w) nfit += s_wfit[w];
+                    // nfit >= 1: a record has at most SK_MAX_WINDOWS <= BIN3_MAP windows
+                    if (fits) {
+                        for (uint32_t j = end - n0; j < end; ++j) s_map[j] = (uint8_t)i;
+                        if (i + 1 == nfit) {
+                            s_nwin = end;
+                            s_nr = nfit;
+                        }
                     }
-                    if (i < nr) s_woff[i] = before + inc - n0;
-                    if (i == 0) s_woff[nr] = total;
                 }
                 __syncthreads();
-                const uint32_t nwin = s_woff[nr];
+                const uint32_t nwin = s_nwin;
+                c0 += s_nr;
                 // the record holding window x, and its window in that record
                 auto locate = [&](uint32_t x, uint32_t& rec, uint32_t& dj) {
-                    uint32_t lo = 0, hi = nr - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi + 1) >> 1;
-                        if (s_woff[mid] <= x) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    rec = lo;
-                    dj = x - s_woff[lo];
+                    rec = s_map[x];
+                    dj = x - s_woff[rec];
                 };
                 // The packed words of the next window are fetched one iteration ahead, so
                 // their load is in flight while this window's LDS work runs.
@@ -746,7 +767,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 uint32_t rec = 0, dj = 0;
                 if (threadIdx.x < nwin) {
                     locate(threadIdx.x, rec, dj);
-                    raw_fetch<KW>(data, s_off[rec], s_j0[rec] + dj, raw);
+                    raw_fetch<KW>(data, s_pos[rec] + dj, raw);
                 }
                 for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
                     if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
@@ -754,14 +775,14 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     const uint32_t lo = rec, cdj = dj;
                     if (x + blockDim.x < nwin) {
                         locate(x + blockDim.x, rec, dj);
-                        raw_fetch<KW>(data, s_off[rec], s_j0[rec] + dj, raw);
+                        raw_fetch<KW>(data, s_pos[rec] + dj, raw);
                     }
                     uint64_t key[NW];
                     pack_key<NW>(canonical<KW>(raw_extract<KW>(cur, k), k), key);
-                    const uint64_t h = table_hash<NW>(key);
-                    if (parts > 1 && ((uint32_t)(h >> 40) & (parts - 1)) != part) continue;
+                    const uint32_t h = bin_slot_hash<NW>(key);
+                    if (parts > 1 && ((h >> 20) & (parts - 1)) != part) continue;
                     const Ev t = (Ev)(s_ev[lo] + cdj);
-                    uint32_t slot = (uint32_t)h & (BIN3_CAP - 1);
+                    uint32_t slot = h & (BIN3_CAP - 1);
                     bool done = false;
                     for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
                         bool claimed = false;
@@ -1813,7 +1834,7 @@ struct kc_ctx {
     DevBuf d_gscratch[GRAPH_SCRATCH];  // unitig builder (kc_graph.hip), reused across calls
     uint64_t n_entries = 0;
     SortLayout lay{};
-    kc_tune tune{1, 0, 0.0f, 0, 0, 0, 0.0};  // launch shapes, table load, paths, bin size (kc_tune_set / kc_autotune)
+    kc_tune tune{1, 0, 0.0f, 0, 0, 0, 0, 0.0};  // launch shapes, table load, paths, bin size, m (kc_tune_set / kc_autotune)
     uint32_t n_cu = 0;
     uint32_t kw = 1;  // value words of the current k (1, 2 or 4)
     bool ev64 = false, built = false, counted = false;
@@ -2182,7 +2203,11 @@ struct Run {
             KC_TRY(c->d_wbin.ensure(c->n_occ * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
                                                           c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
-                                                          c->d_act_ev.as<uint64_t>(), na, k, bits,
+                                                          c->d_act_ev.as<uint64_t>(), na, k,
+                                                          c->tune.minimizer_len && c->tune.minimizer_len <= k
+                                                              ? c->tune.minimizer_len
+                                                              : minimizer_len(k),
+                                                          bits,
                                                           c->d_skc.as<uint32_t>(), c->d_wbin.as<uint64_t>())));
             size_t tb0 = 0;
             KC_TRY(rocprim::exclusive_scan(nullptr, tb0, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
@@ -3345,6 +3370,25 @@ int kc_run(kc_ctx* ctx, kc_info* info) {
     return kc_count(ctx, info);
 }
 
+int kc_run_many(kc_ctx* const* ctxs, uint32_t n, kc_info* infos) {
+    if (!ctxs && n) return KC_EINVAL;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!ctxs[i]) return KC_EINVAL;
+        for (uint32_t j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return ctxs[i]->fail(KC_EINVAL, "kc_run_many: a context is listed twice");
+    }
+    std::vector<int> rc(n, KC_OK);
+    std::vector<std::thread> th;
+    th.reserve(n ? n - 1 : 0);
+    for (uint32_t i = 1; i < n; ++i)
+        th.emplace_back([&, i]() { rc[i] = kc_run(ctxs[i], infos ? &infos[i] : nullptr); });
+    if (n) rc[0] = kc_run(ctxs[0], infos ? &infos[0] : nullptr);
+    for (std::thread& t : th) t.join();
+    for (uint32_t i = 0; i < n; ++i)
+        if (rc[i]) return rc[i];
+    return KC_OK;
+}
+
 // Eager run, then capture of the planned device work into ctx->gexec.
 // A memset node of a captured graph (library calls such as rocPRIM's radix sort record
 // them) as a fill kernel: width elements of elementSize bytes in each of height rows.
@@ -4011,6 +4055,8 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->setter_path > 1) return ctx->fail(KC_EINVAL, "setter_path must be 0 or 1");
     if (tune->kmers_per_bin && (tune->kmers_per_bin < 64 || tune->kmers_per_bin > 4096))
         return ctx->fail(KC_EINVAL, "kmers_per_bin must be 0 or in 64..4096");
+    if (tune->minimizer_len > 16 || (tune->minimizer_len && tune->minimizer_len > ctx->o.k))
+        return ctx->fail(KC_EINVAL, "minimizer_len must be 0 or in 1..16 and at most k");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
     return KC_OK;

@@ -9,7 +9,7 @@
 namespace kc {
 
 // Bins: 2^bits, bits chosen per run (bin_bits_for) for about BIN_TARGET distinct k-mers a bin.
-constexpr uint32_t BIN_BITS_MIN = 14, BIN_BITS_MAX = 20;
+constexpr uint32_t BIN_BITS_MIN = 14, BIN_BITS_MAX = 24;
 constexpr uint32_t BIN_TARGET = 320;
 constexpr uint32_t BIN3_CAP = 1024;  // LDS slots per pass
 constexpr uint32_t BIN3_FILL = BIN3_CAP * 3 / 4;
@@ -17,11 +17,15 @@ constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving u
 constexpr uint32_t SK_MAX_WINDOWS = 255;       // windows per record
 constexpr uint32_t SK_MAX_LEN = 1u << 24;      // read length the record can address
 
-// Minimizer length: the whole k-mer up to 11, then 11 (k <= 32) or 13.
-__host__ __device__ constexpr uint32_t minimizer_len(uint32_t k) { return k <= 11 ? k : k <= 32 ? 11 : 13; }
+// Minimizer length: the whole k-mer up to 11, then 11 (k <= 32) or 16.  A longer m for the
+// long k-mers shortens the super-k-mers and moves more of an error's k-mers out of the
+// genomic super-k-mer's bin, so fewer bins pass the LDS table's fill (5 Mb / 100x at k =
+// 55, 2^19 bins: the windows counted, split passes included, fall from 1.39 to 1.22 times
+// the windows, and the bin count from 8.4 to 7.0 ms; k = 33: 5.26 to 4.80 ms).
+__host__ __device__ constexpr uint32_t minimizer_len(uint32_t k) { return k <= 11 ? k : k <= 32 ? 11 : 16; }
 
 // MurmurHash3's 32-bit finalizer (fmix32): a bijection of 32-bit values, two 32-bit
-// multiplies (a canonical m-mer, m <= 13, fits 26 bits).
+// multiplies (a canonical m-mer, m <= 16, fits 32 bits).
 __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x85ebca6bu;

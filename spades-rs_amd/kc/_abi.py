@@ -29,7 +29,7 @@ EXPORTS = (
     "kc_filter_create", "kc_filter_destroy", "kc_filter_last_error", "kc_filter_memory_usage_bytes",
     "kc_filter_num_words", "kc_filter_insert_kmer256", "kc_filter_is_solid_kmer256", "kc_filter_export",
     "kc_filter_correct_reads", "kc_filter_correct_reads_noalloc", "kc_hamming_distance_2bit",
-    "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run",
+    "kc_create", "kc_destroy", "kc_last_error", "kc_reads_upload", "kc_bloom_build", "kc_count", "kc_run", "kc_run_many",
     "kc_info_get", "kc_bloom_export", "kc_bloom_query", "kc_counts_export", "kc_reads_export", "kc_kernel_times",
     "kc_library_verify", "kc_library_add", "kc_library_count", "kc_library_info", "kc_library_bloom_export", "kc_library_bloom_query",
     "kc_library_counts_export", "kc_library_reads_export", "kc_library_priors_set", "kc_priors_set",
@@ -99,7 +99,8 @@ class kc_unitigs_info(ctypes.Structure):
 class kc_tune(ctypes.Structure):
     _fields_ = [("windows_per_thread", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("table_load", ctypes.c_float), ("insert_path", ctypes.c_uint32),
-                ("setter_path", ctypes.c_uint32), ("kmers_per_bin", ctypes.c_uint32), ("ms", ctypes.c_double)]
+                ("setter_path", ctypes.c_uint32), ("kmers_per_bin", ctypes.c_uint32),
+                ("minimizer_len", ctypes.c_uint32), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -132,6 +133,7 @@ _SIGS = {
     "kc_bloom_build": (ctypes.c_int, [_vp]),
     "kc_count": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
     "kc_run": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
+    "kc_run_many": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(kc_info)]),
     "kc_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(kc_info)]),
     "kc_bloom_export": (ctypes.c_int, [_vp, _u64p, _u64p, _sz]),
     "kc_bloom_query": (ctypes.c_int, [_vp, _u64p, _sz, _u8p]),

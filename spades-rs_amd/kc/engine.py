@@ -300,7 +300,7 @@ class SpectrumEngine:
         return t.as_dict()
 
     def set_tune(self, windows_per_thread=1, blocks_per_cu=0, table_load=0.0, insert_path=0, setter_path=0,
-                 kmers_per_bin=0):
+                 kmers_per_bin=0, minimizer_len=0):
         """insert_path: 0 (default) or 3 minimizer-binned count where it applies (k <= 63,
         reads under 2^24 bases, no prior contigs or Hammer recount inside a graph capture),
         else per-window table atomics; 1 per-window table atomics.  setter_path: 0 (default)
@@ -310,7 +310,7 @@ class SpectrumEngine:
         320).  Every setting gives the same results."""
         t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
                          table_load=float(table_load), insert_path=insert_path, setter_path=setter_path,
-                         kmers_per_bin=kmers_per_bin)
+                         kmers_per_bin=kmers_per_bin, minimizer_len=minimizer_len)
         self._chk(lib().kc_tune_set(self._h, ctypes.byref(t)))
 
     def autotune(self, reps=2):
@@ -405,6 +405,20 @@ class SpectrumEngine:
         launches = (ctypes.c_uint32 * 32)()
         n = lib().kc_kernel_times(self._h, names, ms, launches, 32)
         return {names[i].decode(): (ms[i], launches[i]) for i in range(max(n, 0))}
+
+
+def run_many(engines):
+    """kc_run of several engines at once (kc_run_many): one host thread and stream per
+    engine, each engine's results those of its run() alone.  Returns their infos."""
+    engines = list(engines)
+    n = len(engines)
+    hs = (_abi._vp * n)(*[e._h for e in engines])
+    infos = (_abi.kc_info * n)()
+    st = lib().kc_run_many(hs, n, infos)
+    if st != _abi.KC_OK:
+        msgs = [lib().kc_last_error(e._h) for e in engines]
+        raise KcError(st, "; ".join(m.decode() for m in msgs if m))
+    return [infos[i].as_dict() for i in range(n)]
 
 
 def count_kmers(packed, config, device=0):

@@ -94,12 +94,12 @@ def workload_5mb_100x(seed=2, scale=1.0):
     return uniform_genome_pairs(seed, g, max(int(1_666_667 * scale), 1), p_start=0.001, p_end=0.010)
 
 
-def workload_metagenome(seed=4, scale=1.0, n_genomes=20, n_pairs=30_000_000, sigma=1.0, p_err=0.005):
+def workload_metagenome(seed=4, scale=1.0, n_genomes=20, n_pairs=30_000_000, sigma=1.0, p_err=0.005, progress=None):
     """A metagenome sample: n_genomes random genomes of 1-6 Mb (uniform; x scale), cell
     abundances lognormal(0, sigma), n_pairs x scale 2x150 bp pairs drawn from the genomes
     in proportion to abundance x genome length (insert N(300, 30), uniform substitutions
     at p_err), the pairs in a seeded random order; one library, R1 mates then R2 mates.
-    Returns (PackedReads, meta)."""
+    progress(g, pairs) is called after genome g's pairs are drawn.  Returns (PackedReads, meta)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sizes = np.maximum((rng.uniform(1e6, 6e6, n_genomes) * scale).astype(np.int64), 2000)
     ab = np.exp(sigma * rng.standard_normal(n_genomes))
@@ -117,6 +117,8 @@ def workload_metagenome(seed=4, scale=1.0, n_genomes=20, n_pairs=30_000_000, sig
         r1.append(d[: pairs[g]])
         r2.append(d[pairs[g]:])
         del st
+        if progress:
+            progress(g, int(pairs[g]))
     perm = rng.permutation(total)
     data = np.concatenate([np.concatenate(r1)[perm].reshape(-1), np.concatenate(r2)[perm].reshape(-1)])
     del r1, r2

@@ -30,7 +30,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_abi.kc_opts) == 40
     assert _abi.kc_opts.bloom_bits.offset == 8 and _abi.kc_opts.min_coverage.offset == 16
     assert ctypes.sizeof(_abi.kc_info) == 8 * 8 + 4 * 2 + 8 * 2 + 8
-    assert ctypes.sizeof(_abi.kc_tune) == 32 and _abi.kc_tune.ms.offset == 24
+    assert ctypes.sizeof(_abi.kc_tune) == 40 and _abi.kc_tune.ms.offset == 32
     assert ctypes.sizeof(_abi.kc_packed_info) == 64
     assert ctypes.sizeof(_abi.kc_unitigs_info) == 64 and _abi.kc_unitigs_info.ms_stage.offset == 32
 

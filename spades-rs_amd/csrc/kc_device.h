@@ -46,11 +46,17 @@ __device__ __forceinline__ Kv kv_shr(const Kv& x, uint32_t s) {
     return Kv{o[0], o[1], o[2], o[3]};
 }
 
+// Reverse the order of the 16 2-bit groups of a 32-bit word, keeping the two bits of a
+// group in order: a full bit reversal (v_bfrev_b32) puts the groups in place with their
+// two bits swapped, and one masked merge of the word shifted each way swaps them back.
+__device__ __forceinline__ uint32_t rev_groups32(uint32_t x) {
+    const uint32_t y = __builtin_bitreverse32(x);
+    return ((y >> 1) & 0x55555555u) | ((y << 1) & 0xAAAAAAAAu);
+}
+
 // Reverse the order of the 32 2-bit groups of x, keeping the two bits of a group in order.
 __device__ __forceinline__ uint64_t rev_groups(uint64_t x) {
-    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
-    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
-    return __builtin_bswap64(x);
+    return ((uint64_t)rev_groups32((uint32_t)x) << 32) | rev_groups32((uint32_t)(x >> 32));
 }
 
 // Reverse complement of a k-mer value (complement = code ^ 3, dna.rs:1-8): the same

@@ -343,7 +343,8 @@ int kc_tune_get(const kc_ctx* ctx, kc_tune* tune);
 int kc_tune_set(kc_ctx* ctx, const kc_tune* tune);
 /* Time Stage 2 + 3 over the uploaded libraries for a coordinate search (reps timed runs
  * each, after one warm-up) over the knobs of the Stage 2 path the run takes: with the
- * binned count, the distinct k-mers a bin is sized for and the workgroups per CU; with
+ * binned count, the distinct k-mers a bin is sized for, the minimizer length and the
+ * workgroups per CU; with
  * the table insert, the load factor, the windows per thread and the workgroups per CU;
  * then the first-setter path.  Every setting must give the same per-library summary; the
  * fastest is kept.  *best (nullable) receives it. */

@@ -4104,10 +4104,15 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
         int field;
         std::vector<double> values;
     };
-    enum { F_LOAD, F_WPT, F_BPC, F_KPB, F_SETTER };
+    enum { F_LOAD, F_WPT, F_BPC, F_KPB, F_M, F_SETTER };
     std::vector<Dim> dims;
     if (binned) {
         dims.push_back({F_KPB, {192, 256, 320, 448, 640}});
+        // minimizer lengths around the default for this k (0 = the default itself)
+        std::vector<double> ms{0};
+        for (uint32_t m : {11u, 13u, 15u, 16u})
+            if (m <= ctx->o.k && m != minimizer_len(ctx->o.k)) ms.push_back(m);
+        dims.push_back({F_M, ms});
         dims.push_back({F_BPC, {0, 4, 8, 16}});
     } else {
         dims.push_back({F_LOAD, {0.35, 0.5, 0.65, 0.8}});
@@ -4121,6 +4126,7 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
             case F_WPT: return t.windows_per_thread;
             case F_BPC: return t.blocks_per_cu;
             case F_KPB: return t.kmers_per_bin ? t.kmers_per_bin : BIN_TARGET;
+            case F_M: return t.minimizer_len;
             default: return t.setter_path;
         }
     };
@@ -4130,6 +4136,7 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
             case F_WPT: t.windows_per_thread = (uint32_t)v; break;
             case F_BPC: t.blocks_per_cu = (uint32_t)v; break;
             case F_KPB: t.kmers_per_bin = (uint32_t)v; break;
+            case F_M: t.minimizer_len = (uint32_t)v; break;
             default: t.setter_path = (uint32_t)v; break;
         }
     };

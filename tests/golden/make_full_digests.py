@@ -5,8 +5,8 @@ reads with Hammer on) on the full-size synthetic workloads:
 
   5mb_100x   the 5 Mb / 100x pairs (kc.synth.workload_5mb_100x, seed 2), k = 21, 33, 55,
              2^29-bit tiers, Hammer off (assemble.rs:129-298 at -t 1)
-  1mb_30x    the 1 Mb / 30x pairs (kc.synth.workload_1mb_30x, seed 1), k = 21, 2^29-bit
-             tiers, Hammer off and on
+  1mb_30x    the 1 Mb / 30x pairs (kc.synth.workload_1mb_30x, seed 1), k = 21, 33, 55,
+             2^29-bit tiers, Hammer off; k = 21 also with Hammer on
   64lib_1.0  the 64 uneven libraries at full scale (kc.synth.workload_64_libraries, seed 3;
              5 kb-6 Mb genomes, 20-200x, 50-250 bp reads), k = 33, 2^29-bit tiers: every
              8th library (0, 8, ..., 56), each counted alone by the oracle
@@ -35,6 +35,8 @@ CASES = {
     "5mb_100x_k55": ("5mb_100x", 55, False),
     "1mb_30x_k21": ("1mb_30x", 21, False),
     "1mb_30x_k21_hammer": ("1mb_30x", 21, True),
+    "1mb_30x_k33": ("1mb_30x", 33, False),
+    "1mb_30x_k55": ("1mb_30x", 55, False),
     "meta_0.05_k55": ("meta_0.05", 55, False),
 }
 

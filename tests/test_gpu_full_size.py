@@ -58,9 +58,9 @@ def test_5mb_100x_matches_oracle(store_5mb, k):
         assert not diff(got, want), (diff(got, want), got, want)
 
 
-@pytest.mark.parametrize("ec", [False, True])
-def test_1mb_30x_matches_oracle(store_1mb, ec):
-    want = DIGESTS["1mb_30x_k21" + ("_hammer" if ec else "")]
-    say(f"case 1mb_30x k=21 ec={ec} reads={len(store_1mb)}")
-    for got in run_digests(store_1mb, 21, ec):
+@pytest.mark.parametrize("k,ec", [(21, False), (21, True), (33, False), (55, False)])
+def test_1mb_30x_matches_oracle(store_1mb, k, ec):
+    want = DIGESTS[f"1mb_30x_k{k}" + ("_hammer" if ec else "")]
+    say(f"case 1mb_30x k={k} ec={ec} reads={len(store_1mb)}")
+    for got in run_digests(store_1mb, k, ec):
         assert not diff(got, want), (diff(got, want), got, want)

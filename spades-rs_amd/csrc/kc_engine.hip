@@ -1049,13 +1049,13 @@ __global__ void __launch_bounds__(BLOCK) k_fs_items(const Slot<NW, uint32_t>* __
 
 // bounds[b] = the first sorted item whose partition (key bits under kmask) is >= b, for
 // b = 0 .. nparts.
-__global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint32_t nparts, uint32_t kmask,
-                            uint64_t* __restrict__ bounds) {
+__global__ void k_fs_bounds(const uint64_t* __restrict__ items, uint64_t n, uint32_t nparts, uint32_t shift,
+                            uint32_t kmask, uint64_t* __restrict__ bounds) {
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nparts; b += gridDim.x * blockDim.x) {
         uint64_t lo = 0, hi = n;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
-            if (((uint32_t)(items[mid] >> FS_KEY_SHIFT) & kmask) < b) lo = mid + 1;
+            if (((uint32_t)(items[mid] >> shift) & kmask) < b) lo = mid + 1;
             else hi = mid;
         }
         bounds[b] = lo;
@@ -1107,6 +1107,91 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict_
                 seen[e0 / 64 + w] = bits;
                 solid[e0 / 64 + w] = MULTI ? (uint64_t)sol[2 * w] | (uint64_t)sol[2 * w + 1] << 32 : 0ull;
             }
+        }
+        __syncthreads();
+    }
+}
+
+// Sorted first setters with 64-bit events (dense mode).  The setter 4*t0 + j needs up to S
+// bits (S from 4 x the group's largest library's windows; every setter is below 2^S - 1),
+// so an item is e << S | setter with the entry e in the top 64 - S bits.  Slot i of the
+// scanned range writes items 3i .. 3i+2; an empty slot writes the group's last entry with
+// setter 2^S - 1, which no real setter reaches, so the minimum of an entry that only such
+// items reach is read back as "no setter".  Partitions are 8192 entries (their u64 minima
+// fill 64 KiB of LDS).  The caller cuts the table's slots into chunks, each sorted and
+// folded into fs in turn (MERGE: the partition's minima start from fs).
+constexpr uint32_t FS64_PART_LOG2 = 13;
+constexpr uint32_t FS64_PART = 1u << FS64_PART_LOG2;
+
+template <int NW>
+__global__ void __launch_bounds__(BLOCK) k_fs_items64(const Slot<NW, uint64_t>* __restrict__ tab, uint64_t s_begin,
+                                                      uint64_t s_end, const uint64_t* __restrict__ lib_b0,
+                                                      uint32_t n_libs, const LibDesc* __restrict__ libs,
+                                                      LibOut* __restrict__ out, const uint8_t* __restrict__ fs_base,
+                                                      uint32_t S, uint64_t last_entry, uint64_t* __restrict__ items) {
+    __shared__ unsigned long long s_acc[LIB_LOCAL];
+    LibAccum acc{s_acc};
+    acc.reset();
+    __syncthreads();
+    scan_slots<NW, uint64_t>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t lib0) {
+            const Slot<NW, uint64_t> s = tab[i];
+            uint64_t* o = items + 3 * (i - s_begin);
+            if (s.k[0] == EMPTY_KEY) {
+                o[0] = o[1] = o[2] = last_entry << S | ((1ull << S) - 1);
+                return;
+            }
+            const LibDesc& L = libs[lib];
+            const Kv v = unpack_key<NW>(s.k);
+            uint64_t p[3];
+            kv_positions<kw_of_nw(NW)>(v, L.k, L.M, p);
+            const uint64_t e0 = (uint64_t)(reinterpret_cast<const uint8_t*>(L.fs) - fs_base) / sizeof(uint64_t);
+            const uint64_t setter = s.t0 * 4;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) o[j] = (e0 + p[j]) << S | (setter + j);
+            acc.add(lib, lib0, 1, &out[0].n_distinct, LIBOUT_STRIDE);
+        },
+        [&](uint32_t lib0) {
+            __syncthreads();
+            acc.flush(lib0, n_libs, &out[0].n_distinct, LIBOUT_STRIDE);
+            __syncthreads();
+            acc.reset();
+            __syncthreads();
+        });
+}
+
+template <bool MERGE>
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_min64(const uint64_t* __restrict__ items,
+                                                       const uint64_t* __restrict__ bounds, uint32_t nparts,
+                                                       uint32_t S, uint64_t* __restrict__ fs, uint64_t fs_n) {
+    __shared__ unsigned long long m[FS64_PART];
+    constexpr uint32_t U = 4;  // items in flight per thread
+    const uint64_t smask = (1ull << S) - 1;
+    for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
+        const uint64_t e0 = (uint64_t)part * FS64_PART;
+        const uint32_t ne = (uint32_t)min<uint64_t>(FS64_PART, fs_n - e0);
+        for (uint32_t e = threadIdx.x; e < FS64_PART; e += FS_BLOCK) m[e] = MERGE && e < ne ? fs[e0 + e] : ~0ull;
+        __syncthreads();
+        const uint64_t b = bounds[part], end = bounds[part + 1];
+        for (uint64_t i0 = b; i0 < end; i0 += (uint64_t)FS_BLOCK * U) {
+            uint64_t it[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t i = i0 + u * FS_BLOCK + threadIdx.x;
+                it[u] = i < end ? items[i] : ~0ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t i = i0 + u * FS_BLOCK + threadIdx.x;
+                if (i >= end) continue;
+                atomicMin(&m[(uint32_t)(it[u] >> S) & (FS64_PART - 1)], (unsigned long long)(it[u] & smask));
+            }
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < ne; e += FS_BLOCK) {
+            const uint64_t v = m[e];
+            fs[e0 + e] = v == smask ? ~0ull : v;  // an empty slot's item is no setter
         }
         __syncthreads();
     }
@@ -2558,6 +2643,22 @@ struct Run {
         return sizeof(Ev) == 4 && c->tune.setter_path == 0 && fs_n + FS_PART <= (1ull << 32);
     }
 
+    // With 64-bit events: the bits S a setter 4*t0 + j of the group needs, when an item
+    // e << S | setter and its partition key fit 64 bits; 0 when the group takes the atomics
+    // (the setting, a graph capture or its planning run, whose sizes must not be read back).
+    template <typename Ev>
+    uint32_t sorted_setters64(const std::pair<uint32_t, uint32_t>& g, uint64_t fs_n) const {
+        if (sizeof(Ev) != 8 || c->tune.setter_path != 0 || c->capturing || c->graph_planning) return 0;
+        uint64_t wmax = 1;
+        for (uint32_t l = g.first; l < g.second; ++l) wmax = std::max<uint64_t>(wmax, c->libs[l].info.n_occ);
+        uint32_t S = 2;
+        while (S < 64 && (1ull << S) < 4 * wmax) ++S;
+        const uint32_t nparts = (uint32_t)std::min<uint64_t>((fs_n + FS64_PART - 1) / FS64_PART, 1ull << 31);
+        uint32_t key_bits = 1;
+        while ((1ull << key_bits) < nparts) ++key_bits;
+        return S + FS64_PART_LOG2 + key_bits <= 64 ? S : 0;
+    }
+
     // First setters -> seen tier -> solid tier, one group of libraries at a time.
     // Dense mode fills each group's first-setter array and derives the seen tier from it.
     // Sparse mode (every group's k-mers touch under 1/32 of its entries, by the distinct
@@ -2639,7 +2740,7 @@ struct Run {
                                                           FS_KEY_SHIFT + key_bits, st)));
                 }
                 k_fs_bounds<<<grid_for((uint64_t)nparts + 1), BLOCK, 0, st>>>(
-                    c->d_fsi2.as<uint64_t>(), n_items, nparts, (uint32_t)((1ull << key_bits) - 1),
+                    c->d_fsi2.as<uint64_t>(), n_items, nparts, FS_KEY_SHIFT, (uint32_t)((1ull << key_bits) - 1),
                     c->d_fsb.as<uint64_t>());
                 KC_TRY(hipGetLastError());
                 auto fsmin = multi ? k_fs_min<true> : k_fs_min<false>;
@@ -2651,6 +2752,54 @@ struct Run {
                     TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
                                            L(), c->d_libs.as<LibDesc>(), multi)));
+            } else if (const uint32_t S = sorted_setters64<Ev>(g, fs_n)) {
+                // items e << S | setter, compacted; the slots in chunks whose items and their
+                // sort fit the device, each chunk folded into fs (the first writes every entry)
+                const uint32_t nparts = (uint32_t)((fs_n + FS64_PART - 1) / FS64_PART);
+                uint32_t key_bits = 1;
+                while ((1ull << key_bits) < nparts) ++key_bits;
+                const uint32_t shift = S + FS64_PART_LOG2;
+                // chunks of at most BIG_SCRATCH bytes of items and their sort
+                const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(s_end - s_begin, BIG_SCRATCH / 48));
+                KC_TRY(c->d_fsi.ensure(per * 24));
+                KC_TRY(c->d_fsi2.ensure(per * 24));
+                KC_TRY(c->d_fsb.ensure(((uint64_t)nparts + 1) * 8));
+                for (uint64_t c0 = s_begin;; c0 += per) {
+                    const uint64_t c1 = std::min(s_end, c0 + per);
+                    const uint64_t n = 3 * (c1 - c0);
+                    if (c1 > c0)
+                        TIMED(KID_FS_ITEMS, (k_fs_items64<NW><<<grid_for(c1 - c0, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                                reinterpret_cast<const Slot<NW, uint64_t>*>(c->d_tab.p), c0, c1,
+                                                c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                                c->d_out.as<LibOut>(), c->d_fs.as<uint8_t>(), S, fs_n - 1,
+                                                c->d_fsi.as<uint64_t>())));
+                    if (n) {
+                        size_t tb = 0;
+                        KC_TRY(rocprim::radix_sort_keys(nullptr, tb, c->d_fsi.as<uint64_t>(), c->d_fsi2.as<uint64_t>(),
+                                                        (size_t)n, shift, shift + key_bits, st));
+                        KC_TRY(c->d_temp.ensure(tb));
+                        tb = c->d_temp.bytes;
+                        TIMED(KID_FS_SORT,
+                              KC_TRY(rocprim::radix_sort_keys(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
+                                                              c->d_fsi2.as<uint64_t>(), (size_t)n, shift,
+                                                              shift + key_bits, st)));
+                    }
+                    k_fs_bounds<<<grid_for((uint64_t)nparts + 1), BLOCK, 0, st>>>(
+                        c->d_fsi2.as<uint64_t>(), n, nparts, shift, (uint32_t)((1ull << key_bits) - 1),
+                        c->d_fsb.as<uint64_t>());
+                    KC_TRY(hipGetLastError());
+                    auto fsmin = c0 == s_begin ? k_fs_min64<false> : k_fs_min64<true>;
+                    TIMED(KID_FS_MIN, (fsmin<<<nparts, FS_BLOCK, 0, st>>>(c->d_fsi2.as<uint64_t>(),
+                                                                         c->d_fsb.as<uint64_t>(), nparts, S,
+                                                                         c->d_fs.as<uint64_t>(), fs_n)));
+                    if (c1 >= s_end) break;
+                }
+                TIMED(KID_SEEN, (k_seen_from_fs<Ev><<<grid_for(w_end - w_begin), BLOCK, 0, st>>>(
+                                    w_begin, w_end, c->d_lib_w0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>())));
+                if (s_end > s_begin)
+                    TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
+                                           c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
+                                           L(), c->d_libs.as<LibDesc>(), false)));
             } else {
                 TIMED(KID_FS_CLEAR,
                       (k_fill<Ev><<<grid_for(fs_n), BLOCK, 0, st>>>(c->d_fs.as<Ev>(), fs_n, (Ev)~(Ev)0)));

@@ -180,6 +180,31 @@ def test_spectrum_ragged_and_edge_reads(oracle):
         assert_parity(oracle, st, k, 1 << 16, ec=(k in (21, 65)))
 
 
+@pytest.mark.parametrize("k,run", [(33, 4), (55, 12), (55, 300), (63, 80)])
+def test_count_table_order_with_shared_prefixes(oracle, k, run):
+    """Count-table entries that share their top 64 bits: k-mers A^32 + a varied tail (forward
+    strand canonical), each read twice so that it is Bloom-solid.  Runs of equal prefixes
+    up to 64 entries are ordered after the prefix sort; a longer run (300 at k = 55, 80 at
+    k = 63) makes the two-word sort redo the table.  Keys, counts and mask match the oracle."""
+    rng = np.random.default_rng(k * 1000 + run)
+    st = PackedReads()
+    tail = k - 32
+    bases = np.frombuffer(b"ACGT", dtype=np.uint8)
+    seen = set()
+    while len(seen) < run:
+        # the forward strand is canonical: its 32 leading A's beat the reverse complement,
+        # which has at most `tail` < 32 of them before the complement of the A run
+        t = bytes(bases[rng.integers(0, 4, tail)])
+        if t in seen:
+            continue
+        seen.add(t)
+        for _ in range(2):
+            st.add_read(b"A" * 32 + t)
+    for r in range(300):  # and unrelated reads around them
+        st.add_read(bytes(bases[rng.integers(0, 4, 150)]))
+    assert_parity(oracle, st, k, 1 << 20)
+
+
 def test_spectrum_empty_and_all_short(oracle):
     empty = PackedReads()
     info, seen, solid, keys, cnt, mask, _ = gpu_spectrum(empty, 21, 4096)

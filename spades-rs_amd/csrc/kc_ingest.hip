@@ -297,7 +297,9 @@ __global__ void k_fa_lines(const uint8_t* __restrict__ b, uint64_t n, const unsi
 
 // Sixteen lanes per sequence line (four lines a wave: FASTA lines are mostly 60-80 bytes):
 // its bases to out[off[i] ..].  Record r (the r-th header, rec[i] = headers before line i)
-// starts at its header's offset.
+// starts at its header's offset.  Lines longer than FA_LONG_LINE (unwrapped records:
+// assemblies, long reads) are left to k_fa_copy_long, a workgroup a line.
+constexpr uint32_t FA_LONG_LINE = 4096;
 __global__ __launch_bounds__(BLOCK) void k_fa_copy(const uint8_t* __restrict__ b, uint64_t n,
                                                    const unsigned long long* __restrict__ nl, uint64_t m,
                                                    uint64_t first, uint64_t nlines, const uint32_t* __restrict__ hdr,
@@ -313,10 +315,29 @@ __global__ __launch_bounds__(BLOCK) void k_fa_copy(const uint8_t* __restrict__ b
             if (sub == 0) seq_s[rec[i]] = off[i];
             continue;
         }
+        if (len[i] > FA_LONG_LINE) continue;
         uint64_t s, e;
         line_span(nl, m, n, first + i, s, e);
         const uint64_t o = off[i];
         for (uint64_t q = sub; q < len[i]; q += LANES) out[o + q] = b[s + q];
+    }
+}
+
+// The sequence lines longer than FA_LONG_LINE, one workgroup each (a workgroup passes over
+// the short lines of its stride at one scalar test a line).
+__global__ __launch_bounds__(BLOCK) void k_fa_copy_long(const uint8_t* __restrict__ b, uint64_t n,
+                                                        const unsigned long long* __restrict__ nl, uint64_t m,
+                                                        uint64_t first, uint64_t nlines,
+                                                        const uint32_t* __restrict__ hdr,
+                                                        const uint32_t* __restrict__ len,
+                                                        const unsigned long long* __restrict__ off,
+                                                        uint8_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x; i < nlines; i += gridDim.x) {
+        if (hdr[i] || len[i] <= FA_LONG_LINE) continue;
+        uint64_t s, e;
+        line_span(nl, m, n, first + i, s, e);
+        const uint64_t o = off[i];
+        for (uint64_t q = threadIdx.x; q < len[i]; q += BLOCK) out[o + q] = b[s + q];
     }
 }
 
@@ -698,6 +719,10 @@ int index_fasta(kc_ingest* c, FileIndex& F, const char* name, uint64_t m, uint64
                                                          F.fa_rec.as<unsigned long long>(),
                                                          F.fa_off.as<unsigned long long>(), F.fa_seq.as<uint8_t>(),
                                                          F.seq_s.as<uint64_t>());
+    I_TRY(hipGetLastError());
+    k_fa_copy_long<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nlines, 4096)), BLOCK, 0, st>>>(
+        b, n, F.nl.as<unsigned long long>(), m, first, nlines, F.fa_hdr.as<uint32_t>(), F.fa_len.as<uint32_t>(),
+        F.fa_off.as<unsigned long long>(), F.fa_seq.as<uint8_t>());
     I_TRY(hipGetLastError());
     k_fa_lens<<<grid_for(R), BLOCK, 0, st>>>(F.seq_s.as<uint64_t>(), R, total, F.seq_l.as<uint32_t>(),
                                              F.err.as<uint32_t>());

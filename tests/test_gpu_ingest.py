@@ -147,6 +147,10 @@ def test_fasta_matches_oracle(oracle, seed, width, crlf, blank, lead):
     check(oracle, a)
     check(oracle, b, fastq(r1[:-1], crlf))
     check(oracle, b">only\n")  # one record without bases: none
+    if width == 0:  # unwrapped records of several kilobases (one workgroup a line)
+        longs = [bytes(np.frombuffer(b"ACGTN", dtype=np.uint8)[rng.integers(0, 5, int(L))])
+                 for L in (4096, 4097, 9000, 65536 + 33)]
+        check(oracle, fasta(longs + random_reads(rng, 50), width, crlf, blank, lead))
     check(oracle, b">a\n>b\n" + b"ACGT" * 8 + b"\n")  # an empty record, then one
     with pytest.raises(KcError) as e:  # no qualities to keep
         from kc.ingest import FastqIngest

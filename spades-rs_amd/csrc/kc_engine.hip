@@ -1252,6 +1252,41 @@ __global__ void __launch_bounds__(BLOCK) k_promote(const Slot<NW, Ev>* __restric
         [](uint32_t) {});
 }
 
+// The promotion pass of the sorted first setters (items with the counted-twice mark), from
+// the items themselves: the three items of slot i, 3i .. 3i+2 of the unsorted array, hold
+// its entries and setters, so no key is read or hashed.  A k-mer counted once promotes iff
+// none of its items is its entry's first setter; a marked item's k-mer was promoted by the
+// partition pass, an empty slot's items are all ones.
+template <int NW>
+__global__ void __launch_bounds__(BLOCK) k_promote_items(const Slot<NW, uint32_t>* __restrict__ tab,
+                                                         uint64_t s_begin, uint64_t s_end,
+                                                         const uint64_t* __restrict__ lib_b0, uint32_t n_libs,
+                                                         const LibDesc* __restrict__ libs,
+                                                         const uint8_t* __restrict__ fs_base,
+                                                         const uint64_t* __restrict__ items) {
+    const uint32_t* fs = reinterpret_cast<const uint32_t*>(fs_base);
+    scan_slots<NW, uint32_t>(
+        tab, s_begin, s_end, lib_b0, n_libs,
+        [&](uint64_t i, uint32_t lib, uint32_t) {
+            const uint64_t* it = items + 3 * (i - s_begin);
+            const uint64_t a = it[0];
+            if (a >> 63) return;  // empty slot (all ones) or counted twice
+            const uint64_t b = it[1], c = it[2];
+            bool promote = fs[(a >> 32) & 0x7FFFFFFFu] != (uint32_t)a;
+            if (promote) promote = fs[(b >> 32) & 0x7FFFFFFFu] != (uint32_t)b;
+            if (promote) promote = fs[(c >> 32) & 0x7FFFFFFFu] != (uint32_t)c;
+            if (promote) {
+                const LibDesc& L = libs[lib];
+                const uint64_t e0 = (uint64_t)(reinterpret_cast<const uint8_t*>(L.fs) - fs_base) / sizeof(uint32_t);
+                for (const uint64_t x : {a, b, c}) {
+                    const uint64_t p = ((x >> 32) & 0x7FFFFFFFu) - e0;
+                    atomicOr(reinterpret_cast<unsigned long long*>(&L.solid[p >> 6]), 1ull << (p & 63));
+                }
+            }
+        },
+        [](uint32_t) {});
+}
+
 // Gather the count table: occupied slots whose k-mer is Bloom-solid (or every occupied
 // slot when the table was built from Bloom-solid windows only), as (hi, lo, count) sort
 // entries.  Each block reserves its output range with one atomic per chunk.
@@ -2748,7 +2783,12 @@ struct Run {
                                       c->d_fsi2.as<uint64_t>(), c->d_fsb.as<uint64_t>(), nparts,
                                       c->d_fs.as<uint32_t>(), fs_n, c->d_seen.as<uint64_t>() + w_begin,
                                       c->d_solid.as<uint64_t>() + w_begin)));
-                if (s_end > s_begin)
+                if (s_end > s_begin && multi && n_items)
+                    TIMED(KID_PROMOTE, (k_promote_items<NW><<<gs, BLOCK, 0, st>>>(
+                                           c->d_tab.as<Slot<NW, uint32_t>>(), s_begin, s_end,
+                                           c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
+                                           c->d_fs.as<uint8_t>(), c->d_fsi.as<uint64_t>())));
+                else if (s_end > s_begin)
                     TIMED(KID_PROMOTE, (k_promote<NW, Ev><<<gs, BLOCK, 0, st>>>(
                                            c->d_tab.as<Slot<NW, Ev>>(), s_begin, s_end, c->d_lib_b0.as<uint64_t>(),
                                            L(), c->d_libs.as<LibDesc>(), multi)));

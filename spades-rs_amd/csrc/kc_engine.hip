@@ -596,6 +596,12 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // nothing and sets ctr[2] (lcnt still totals the library's distinct k-mers).  ctr[1] is set
 // when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
 // insert instead).
+// The bin sort's onesweep passes take 10 bits each, so the 15-20 bits of (library, bin)
+// keys sort in two passes instead of three.
+using BinSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 12>, 10,
+                                        rocprim::block_radix_rank_algorithm::match>>;
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
 constexpr uint32_t BIN3_MAP = 2048;      // windows per chunk (the window -> record map)
 // Transient buffers above this size are released after use instead of kept for reuse.
@@ -2363,12 +2369,12 @@ struct Run {
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
         const uint32_t key_bits = lib_bits + bits;
-        KC_TRY(rocprim::radix_sort_pairs(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
+        KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
                                          c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
                                          st));
         KC_TRY(c->d_temp.ensure(tb));
         tb = c->d_temp.bytes;
-        TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
+        TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
                                                               c->d_bk2.as<uint32_t>(), c->d_br.as<uint64_t>(),
                                                               c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
                                                               st)));

@@ -1308,7 +1308,7 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
                                                    unsigned long long* __restrict__ counter,
                                                    const uint32_t* __restrict__ psum,
                                                    const uint8_t* __restrict__ pflag,
-                                                   WideEntry* __restrict__ out_wide) {
+                                                   WideEntry* __restrict__ out_wide, bool twice_solid = true) {
     __shared__ uint32_t s_scan[BLOCK / 64];
     __shared__ unsigned long long s_base;
     constexpr uint32_t SPB = slots_per_bucket<NW, Ev>();
@@ -1329,8 +1329,9 @@ __global__ void __launch_bounds__(BLOCK) k_compact(const Slot<NW, Ev>* __restric
                 // a slot claimed only by a prior holds no occurrence
                 const Kv v = unpack_key<NW>(sl.k);
                 // a k-mer inserted twice found its three bits set the second time and
-                // promoted (bloom.rs:95-103), so it is solid without a look
-                if (sl.cnt >= 2) {
+                // promoted (bloom.rs:95-103), so it is solid without a look (in the table of
+                // the reads the filter was built from, not in a recount of corrected reads)
+                if (twice_solid && sl.cnt >= 2) {
                     base = true;
                 } else if (single) {
                     const LibDesc& L = libs[lib0];
@@ -2291,13 +2292,17 @@ struct Run {
     static constexpr int BIN_FALLBACK = 1;
     // planned (a graph capture): the bins, record count and per-library sizes are those
     // the planning run recorded (g_bin_*), nothing is read back, allocated or copied from
-    // the host, and k_plan_guard flags a replay whose sizes differ.
+    // the host, and k_plan_guard flags a replay whose sizes differ.  recount: Stage 3's
+    // count of the reads Hammer corrected (src = the corrected copy): every window is
+    // counted (the compaction then keeps the Bloom-solid k-mers), and the libraries'
+    // distinct counts and the capture plan, which size Stage 2's bins, are left as they are.
     template <int KW, typename Ev>
-    int binned_count(bool planned = false) {
+    int binned_count(bool planned = false, const uint8_t* src = nullptr, bool recount = false) {
         constexpr int NW = nw_of(KW);
         constexpr uint64_t SPB = slots_per_bucket<NW, Ev>();
         using S = Slot<NW, Ev>;
         hipStream_t st = c->st;
+        const uint8_t* data = src ? src : c->d_data.as<uint8_t>();
         const uint64_t na = c->n_active;
         const uint32_t k = c->o.k, nL = L();
         const uint32_t gw = grid_for(na, BLOCK / 64);
@@ -2327,7 +2332,7 @@ struct Run {
             KC_TRY(c->d_skc.ensure(na * 4 + 4));
             KC_TRY(c->d_sko.ensure(na * 8 + 8));
             KC_TRY(c->d_wbin.ensure(c->n_occ * 8 + 8));
-            TIMED(KID_SK, (k_sk_wave<<<gw, BLOCK, 0, st>>>(c->d_data.as<uint8_t>(), c->d_act_off.as<uint64_t>(),
+            TIMED(KID_SK, (k_sk_wave<<<gw, BLOCK, 0, st>>>(data, c->d_act_off.as<uint64_t>(),
                                                           c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
                                                           c->d_act_ev.as<uint64_t>(), na, k,
                                                           c->tune.minimizer_len && c->tune.minimizer_len <= k
@@ -2409,7 +2414,7 @@ struct Run {
             k_fill<unsigned long long><<<grid_for(nL), BLOCK, 0, st>>>(c->d_bcnt.as<unsigned long long>(), nL, 0ull);
             k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 3, 0ull);
             TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
-                                     c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), c->d_data.as<uint8_t>(),
+                                     c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), data,
                                      c->d_act_oe.as<ulonglong2>(), k,
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
@@ -2437,7 +2442,7 @@ struct Run {
             if (rec_bytes > BIG_SCRATCH && !c->fits(tot * STAGE3_BYTES_PER_ENTRY))
                 for (DevBuf* b : {&c->d_bk, &c->d_bk2, &c->d_br, &c->d_br2}) b->release();
         }
-        if (!planned) {  // what a capture of this run replays
+        if (!planned && !recount) {  // what a capture of this run replays
             c->g_bin_bits = bits;
             c->g_bin_nrec = nrec;
             c->g_bin_cap = cap;
@@ -2450,8 +2455,10 @@ struct Run {
         uint64_t b0 = 0;
         for (uint32_t l = 0; l < nL; ++l) {
             HostLib& H = c->libs[l];
-            H.est = lcnt[l];  // exact: the first-setter mode decision and the next run's bins read it
-            H.est_k = k;
+            if (!recount) {
+                H.est = lcnt[l];  // exact: the first-setter mode decision and the next run's bins read it
+                H.est_k = k;
+            }
             H.bucket0 = b0;
             H.nbuckets = (lcnt[l] + SPB - 1) / SPB;
             b0 += H.nbuckets;
@@ -3006,6 +3013,7 @@ struct Run {
         int rc = clear_out();
         if (rc) return rc;
         const bool ec = c->o.error_correct != 0;
+        bool recount_binned = false;  // Hammer's recount counted every window (binned)
         c->lay = make_layout(c->o.k, L());
         if (c->lay.hi_bits > 64)
             return fail(KC_EUNSUPPORTED, "too many libraries for this k in one context (need 2k - 63 + log2(libraries) <= 64)");
@@ -3021,29 +3029,47 @@ struct Run {
                                       c->d_rlib.as<uint32_t>(), c->d_blooms.as<LibBloom>(), c->o.k,
                                       &c->d_out.as<LibOut>()[0].corrected, LIBOUT_STRIDE, ctr + HAMMER_QUERIES, nullptr)));
             }
-            if (c->last_binned) {  // the recount hashes: regions sized for the first count's keys
-                rc = hashed_layout<NW, uint32_t>(0);
-                if (rc) return rc;
-                c->last_binned = false;
+            // The recount: by default the binned count over the corrected reads (every
+            // window; the compaction keeps the Bloom-solid k-mers), else the table insert of
+            // the Bloom-solid windows (assemble.rs:188-205), as a graph capture needs.
+            if constexpr (KW <= 2 && sizeof(Ev) == 4) {
+                if (use_binned() && !c->capturing && !c->graph_planning) {
+                    rc = binned_count<KW, uint32_t>(false, c->d_corr.as<uint8_t>(), true);
+                    if (rc == KC_OK) {
+                        recount_binned = true;
+                        c->last_binned = true;
+                    } else if (rc != BIN_FALLBACK) {
+                        return rc;
+                    }
+                }
             }
-            const uint64_t SPB = slots_per_bucket<NW, uint32_t>();
-            const uint64_t n_slots = c->total_buckets * SPB;
-            TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
-                                 c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
-            if (c->n_tiles) {
-                rc = launch_insert<KW, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+            if (!recount_binned) {
+                if (c->last_binned) {  // the recount hashes: regions sized for the first count's keys
+                    rc = hashed_layout<NW, uint32_t>(0);
+                    if (rc) return rc;
+                    c->last_binned = false;
+                }
+                const uint64_t SPB = slots_per_bucket<NW, uint32_t>();
+                const uint64_t n_slots = c->total_buckets * SPB;
+                TIMED(KID_CLEAR, (k_table_clear<NW, uint32_t><<<grid_for(n_slots), BLOCK, 0, st>>>(
+                                     c->d_tab.as<Slot<NW, uint32_t>>(), n_slots)));
+                if (c->n_tiles) {
+                    rc = launch_insert<KW, uint32_t, false, true>(c->d_corr.as<uint8_t>(), KID_RECOUNT);
+                    if (rc) return rc;
+                }
+                rc = read_out();
                 if (rc) return rc;
+                for (uint32_t l = 0; l < L(); ++l)
+                    if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
+                        return fail(KC_ENOMEM, "device hash table overflow (recount)");
             }
-            rc = read_out();
-            if (rc) return rc;
-            for (uint32_t l = 0; l < L(); ++l)
-                if (c->h_out[l].overflow)  // the recount table holds a subset of the first table's keys
-                    return fail(KC_ENOMEM, "device hash table overflow (recount)");
         }
         bool any_prior = false;
         for (const HostLib& H : c->libs) any_prior = any_prior || H.prior_windows(c->o.k) > 0;
-        if (!ec && c->last_binned && any_prior) {
-            rc = rehash<NW, Ev>();  // the priors find or claim their k-mers by hash
+        if (c->last_binned && any_prior) {
+            // the priors find or claim their k-mers by hash (a binned recount's table has
+            // 32-bit events, Ev here)
+            rc = ec ? rehash<NW, uint32_t>() : rehash<NW, Ev>();
             if (rc) return rc;
         }
         const uint64_t slots = c->total_buckets * (ec ? slots_per_bucket<NW, uint32_t>() : slots_per_bucket<NW, Ev>());
@@ -3069,7 +3095,12 @@ struct Run {
             KC_TRY(c->d_wide.ensure(cap * sizeof(WideEntry)));
             KC_TRY(c->d_wide2.ensure(cap * sizeof(WideEntry)));
         }
-        if (ec) {
+        if (ec && recount_binned) {  // every window was counted: the full Bloom-solid test
+            TIMED(KID_COMPACT, (k_compact<NW, uint32_t, true><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
+                                   c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
+                                   c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),
+                                   c->d_cnt.as<uint32_t>(), ctr + 4, psum, pflag, c->d_wide.as<WideEntry>(), false)));
+        } else if (ec) {
             TIMED(KID_COMPACT, (k_compact<NW, uint32_t, false><<<grid_for(slots, BLOCK * SCAN_ITEMS), BLOCK, 0, st>>>(
                                    c->d_tab.as<Slot<NW, uint32_t>>(), slots, c->d_lib_b0.as<uint64_t>(), L(),
                                    c->d_libs.as<LibDesc>(), c->lay, c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(),

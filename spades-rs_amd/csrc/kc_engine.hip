@@ -220,7 +220,7 @@ __global__ void k_nwin(const uint32_t* __restrict__ lengths, uint64_t n, uint32_
 }
 
 // For every active read: its first global window, byte offset and library, and the first
-// active read of every tile whose first window it holds.
+// active read of every tile whose first window it holds.  act_ev[n_active] is the total.
 __global__ void k_active(const uint32_t* __restrict__ active, uint64_t n_active, const uint64_t* __restrict__ prefix,
                          const uint64_t* __restrict__ nwin, const uint64_t* __restrict__ offsets,
                          const uint32_t* __restrict__ read_lib, uint64_t* __restrict__ act_ev,
@@ -231,6 +231,7 @@ __global__ void k_active(const uint32_t* __restrict__ active, uint64_t n_active,
         const uint32_t r = active[a];
         const uint64_t s = prefix[r], e = s + nwin[r];
         act_ev[a] = s;
+        if (a + 1 == n_active) act_ev[n_active] = e;  // so that act_ev[a + 1] - act_ev[a] = windows of a
         act_off[a] = offsets[r];
         act_lib[a] = read_lib[r];
         for (uint64_t b = (s + TILE_WINDOWS - 1) / TILE_WINDOWS; b * TILE_WINDOWS < e; ++b) tile_first[b] = (uint32_t)a;
@@ -444,8 +445,6 @@ __device__ __forceinline__ uint64_t sk_scratch(uint32_t bin, uint32_t first, uin
 
 __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ data,
                                                    const uint64_t* __restrict__ act_off,
-                                                   const uint32_t* __restrict__ active,
-                                                   const uint64_t* __restrict__ nwin,
                                                    const uint64_t* __restrict__ act_ev, uint64_t n_active, uint32_t k,
                                                    uint32_t m, uint32_t bits, uint32_t* __restrict__ cnt,
                                                    uint64_t* __restrict__ scratch) {
@@ -456,24 +455,68 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     while (2 * p <= w) p <<= 1;
     const uint64_t mmask = (1ull << (2 * m)) - 1;
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(data);
-    for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
-        const uint64_t bit0 = act_off[a] * 8;
-        const uint32_t nw = (uint32_t)nwin[active[a]];
-        uint64_t* out = scratch + act_ev[a];
+    // A read's packed dwords sit one per lane (lane l: dword q0 + l, up to the last dword its
+    // m-mers read), so an m-mer's two words come by two shuffles; 64 dwords cover about 1000
+    // m-mers, and a longer read reloads them when a chunk passes the end.  The loop is
+    // software-pipelined over the wave's reads: the next read's dwords and the one after's
+    // (offset, first window, end window) are loaded while the current read is cut.
+    struct Meta {
+        uint64_t off, ev, ev1;
+    };
+    auto load_meta = [&](uint64_t a) -> Meta {
+        Meta r{0, 0, 0};
+        if (a < n_active) r = Meta{act_off[a], act_ev[a], act_ev[a + 1]};
+        return r;
+    };
+    // [q0, q_end): the dwords of the read's m-mers (each reads dwords q and q + 1, the last
+    // possibly in the 64-byte tail pad after the data)
+    auto dword_span = [&](const Meta& M, uint64_t& q0, uint64_t& q_end) {
+        const uint64_t b0 = M.off * 8;
+        const uint32_t nm = (uint32_t)(M.ev1 - M.ev) + w - 1;
+        q0 = b0 >> 5;
+        q_end = M.ev1 > M.ev ? ((b0 + 2ull * (nm - 1)) >> 5) + 2 : q0;
+    };
+    auto load_words = [&](uint64_t q0, uint64_t q_end) -> uint32_t {
+        return q0 + lane < q_end ? dw[q0 + lane] : 0u;
+    };
+    uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    Meta cur = load_meta(a);
+    uint64_t nq0, nqe;
+    dword_span(cur, nq0, nqe);
+    uint32_t nW = load_words(nq0, nqe);
+    Meta nxt = load_meta(a + waves);
+    for (; a < n_active; a += waves) {
+        const uint64_t bit0 = cur.off * 8;
+        const uint32_t nw = (uint32_t)(cur.ev1 - cur.ev);
+        uint64_t* out = scratch + cur.ev;
+        uint32_t W = nW;
+        uint64_t q0 = nq0;
+        const uint64_t q_end = nqe;
+        dword_span(nxt, nq0, nqe);
+        nW = load_words(nq0, nqe);
+        cur = nxt;
+        nxt = load_meta(a + 2 * waves);
         const uint32_t nm = nw + w - 1;  // m-mers of the read
         auto key_at = [&](uint32_t i) -> uint32_t {
-            if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
             const uint64_t bp = bit0 + 2ull * i;
-            const uint64_t q = bp >> 5;
+            const uint32_t l0 = (uint32_t)((bp >> 5) - q0);
+            const uint32_t x0 = __shfl(W, l0 & 63), x1 = __shfl(W, (l0 + 1) & 63);
             const uint32_t sh = (uint32_t)(bp & 31);
-            const uint64_t win = ((uint64_t)__builtin_bswap32(dw[q]) << 32) | __builtin_bswap32(dw[q + 1]);
+            const uint64_t win = ((uint64_t)__builtin_bswap32(x0) << 32) | __builtin_bswap32(x1);
             const uint64_t f = (win >> (64 - sh - 2 * m)) & mmask;
             const uint64_t r = rev_groups(~f) >> (64 - 2 * m);
-            return mmer_key(f < r ? f : r);
+            const uint32_t key = mmer_key(f < r ? f : r);
+            return i < nm ? key : 0xFFFFFFFFu;  // past the read: no valid window reaches it
         };
         uint32_t open = 0, carry_bin = 0, total = 0;
         uint32_t K = key_at(lane);  // keys of positions c0 + lane
         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
+            // the chunk's m-mers c0 + 64 .. read dwords up to the last one's second word
+            const uint64_t q_hi = ((bit0 + 2ull * min(c0 + 127, nm - 1)) >> 5) + 1;
+            if (q_hi >= q0 + 64) {  // a long read: the dwords from this chunk on
+                q0 = (bit0 + 2ull * c0) >> 5;
+                W = load_words(q0, q_end);
+            }
             const bool valid = c0 + lane < nw;
             // A = positions c0 + lane, B = positions c0 + 64 + lane (the windows read B up to
             // lane w - 2; B is the next chunk's A)
@@ -530,23 +573,26 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
 }
 
 // The records of each read from scratch to their packed place sk_off[a] ..: bin key
-// (library << bits | bin), and a | first window << 32 | windows << 56.  One wave per read.
+// (library << bits | bin), and a | first window << 32 | windows << 56.  Sixteen lanes per
+// read (a read holds about 5-25 records), so a wave follows four reads' load chains at once.
+constexpr uint32_t SK_COPY_LANES = 16;
 __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ act_ev,
                                                    const uint32_t* __restrict__ act_lib, uint64_t n_active,
                                                    uint32_t bits, const uint32_t* __restrict__ cnt,
                                                    const uint64_t* __restrict__ sk_off,
                                                    const uint64_t* __restrict__ scratch, uint32_t* __restrict__ keys,
                                                    uint64_t* __restrict__ recs, uint64_t cap) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
-    for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); a < n_active; a += waves) {
+    const uint32_t sub = threadIdx.x % SK_COPY_LANES;
+    const uint64_t groups = (uint64_t)gridDim.x * (BLOCK / SK_COPY_LANES);
+    for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / SK_COPY_LANES) + threadIdx.x / SK_COPY_LANES; a < n_active;
+         a += groups) {
         const uint32_t n = cnt[a];
         const uint64_t* src = scratch + act_ev[a];
         const uint64_t o = sk_off[a];
         const uint32_t lkey = act_lib[a] << bits;
         // (cap: the record buffers' room; a replayed plan that no longer fits writes
         // nothing past it, and its guard sends the run back to planning)
-        for (uint32_t t = lane; t < n && o + t < cap; t += 64) {
+        for (uint32_t t = sub; t < n && o + t < cap; t += SK_COPY_LANES) {
             const uint64_t r = src[t];
             keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFFu);
             recs[o + t] = a | ((r >> 24) & 0xFFFFFFull) << 32 | (r >> 48) << 56;
@@ -2333,7 +2379,6 @@ struct Run {
             KC_TRY(c->d_sko.ensure(na * 8 + 8));
             KC_TRY(c->d_wbin.ensure(c->n_occ * 8 + 8));
             TIMED(KID_SK, (k_sk_wave<<<gw, BLOCK, 0, st>>>(data, c->d_act_off.as<uint64_t>(),
-                                                          c->d_active.as<uint32_t>(), c->d_nwin.as<uint64_t>(),
                                                           c->d_act_ev.as<uint64_t>(), na, k,
                                                           c->tune.minimizer_len && c->tune.minimizer_len <= k
                                                               ? c->tune.minimizer_len

@@ -327,12 +327,13 @@ typedef struct kc_tune {
                                     atomics per window.  Other values give KC_EINVAL */
     uint32_t setter_path;        /* Bloom first setters.  0 (default) = sorted: three
                                     (entry, setter) items per distinct k-mer radix-sorted by
-                                    16384-entry partition, each partition's minima taken in
-                                    LDS and written in order; taken in the dense mode with
-                                    32-bit events and first-setter groups under 2^32 entries,
-                                    outside graph capture, and otherwise the atomics below.  1 = three device-scope
-                                    atomicMin per distinct k-mer */
-    uint32_t kmers_per_bin;      /* binned count: distinct k-mers a bin is sized for (0: 320;
+                                    partition (16384 entries with 32-bit events, 8192 with
+                                    64-bit ones), each partition's minima taken in LDS and
+                                    written in order; taken in the dense mode outside graph
+                                    capture (with 32-bit events also inside it), and otherwise
+                                    the atomics below.  1 = three device-scope atomicMin per
+                                    distinct k-mer */
+    uint32_t kmers_per_bin;      /* binned count: distinct k-mers a bin is sized for (0: 352;
                                     else 64..4096) */
     uint32_t minimizer_len;      /* binned count: m of the canonical m-mers whose minimum bins a
                                     window (0: the default for k; else 1..16, at most k) */

@@ -202,6 +202,8 @@ constexpr size_t LIBOUT_STRIDE = sizeof(LibOut) / sizeof(unsigned long long);
 constexpr uint32_t HAMMER_QUERIES = 20;
 // Counter slot a replayed graph sets when its data-dependent sizes leave the plan.
 constexpr uint32_t PLAN_GUARD = 22;
+// Counter slot holding the binned count's queue of bins (the next bin to take).
+constexpr uint32_t BIN_QUEUE = 23;
 static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
@@ -393,10 +395,9 @@ __global__ void __launch_bounds__(BLOCK) k_insert(const uint8_t* __restrict__ da
     }
 }
 
-// One 64-bit key word of an LDS slot: EMPTY -> v exactly once.  `claimed` is set when this
-// call made the change.
-__device__ __forceinline__ bool lds_claim_word(uint64_t* w, uint64_t v, bool& claimed) {
-    uint64_t cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// One 64-bit key word of an LDS slot: EMPTY -> v exactly once, given the word as loaded
+// (`cur`; a non-EMPTY word is final).  `claimed` is set when this call made the change.
+__device__ __forceinline__ bool lds_claim_loaded(uint64_t* w, uint64_t cur, uint64_t v, bool& claimed) {
     if (cur == EMPTY_KEY) {
         cur = atomicCAS(reinterpret_cast<unsigned long long*>(w), (unsigned long long)EMPTY_KEY,
                         (unsigned long long)v);
@@ -710,7 +711,8 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                                                      const uint64_t* __restrict__ cap0,
                                                      const uint64_t* __restrict__ capl,
                                                      unsigned long long* __restrict__ lcnt,
-                                                     unsigned long long* __restrict__ ctr) {
+                                                     unsigned long long* __restrict__ ctr,
+                                                     unsigned int* __restrict__ queue) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
     __shared__ uint64_t s_k0[BIN3_CAP];
@@ -726,6 +728,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     __shared__ uint32_t s_fill, s_over, s_np, s_nr, s_nwin;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_bin[2];
     static_assert(BIN3_CAP <= 65536, "slot numbers are 16-bit");
     static_assert(BIN3_RCHUNK <= 256 && BIN3_MAP >= SK_MAX_WINDOWS, "map entries are 8-bit, a record fits a chunk");
     // The table starts empty and every pass leaves it empty: its claimed slots are listed
@@ -740,9 +743,60 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
         }
     };
     clear_table();
-    for (uint32_t bin = blockIdx.x; bin < nbins; bin += gridDim.x) {
-        const uint64_t rb = bounds[bin], re = bounds[bin + 1];
-        if (rb == re) continue;
+    // Workgroups stay resident and take bins from a queue (so that uneven bins balance), and
+    // the loads that start a bin (its record range, the first chunk's records and their
+    // (offset, first event) pairs) are issued during the previous bin: the next bin's range
+    // at the start of a bin, its records after the first chunk is staged, their pairs after
+    // the first chunk's windows.  The record range is loaded by lanes 0 and 1 of each wave
+    // as vector loads and read back with readlane.
+    const uint32_t tid = threadIdx.x;
+    auto load_range = [&](uint32_t b) -> uint64_t { return b < nbins ? bounds[b + (tid & 1)] : 0; };
+    auto lane_u64 = [](uint64_t v, int l) -> uint64_t {
+        return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l) |
+               (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
+    };
+    if (tid == 0) {
+        const uint32_t b = atomicAdd(queue, 2u);
+        s_bin[0] = b;
+        s_bin[1] = b + 1;
+    }
+    __syncthreads();
+    uint32_t bin = s_bin[0], nbin = s_bin[1];
+    uint64_t c_rb, c_re, c_rec = 0;  // the current bin's range and first-chunk record of this thread
+    ulonglong2 c_oe = make_ulonglong2(0, 0);
+    {
+        const uint64_t v = load_range(bin);
+        c_rb = lane_u64(v, 0);
+        c_re = lane_u64(v, 1);
+        if (tid < min<uint64_t>(BIN3_RCHUNK, c_re - c_rb)) {
+            c_rec = recs[c_rb + tid];
+            c_oe = act_oe[(uint32_t)c_rec];
+        }
+    }
+    uint64_t n_v = load_range(nbin);
+    while (bin < nbins) {
+        __syncthreads();  // every thread has read s_bin
+        uint32_t q_next = 0;  // the bin after nbin, published at the end of this one
+        if (tid == 0) q_next = atomicAdd(queue, 1u);
+        const uint64_t rb = c_rb, re = c_re;
+        uint64_t n_rb = 0, n_re = 0, n_rec = 0;
+        ulonglong2 n_oe = make_ulonglong2(0, 0);
+        bool pf_rec = false, pf_oe = false;  // (uniform)
+        auto issue_next_records = [&]() {
+            if (pf_rec) return;
+            pf_rec = true;
+            n_rb = lane_u64(n_v, 0);
+            n_re = lane_u64(n_v, 1);
+            if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) n_rec = recs[n_rb + tid];
+        };
+        auto issue_next_pairs = [&]() {
+            if (pf_oe) return;
+            issue_next_records();
+            pf_oe = true;
+            if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) n_oe = act_oe[(uint32_t)n_rec];
+        };
+        if (rb == re) goto next_bin;
+        {
         const uint32_t lib = bin >> bits;
         const uint64_t ev0 = libs[lib].ev0;
         __syncthreads();
@@ -769,10 +823,11 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     const uint32_t i = threadIdx.x, lane = i & 63, wave = i >> 6;
                     uint32_t n0 = 0;
                     if (i < nr) {
-                        const uint64_t rec = recs[c0 + i];
-                        const uint32_t a = (uint32_t)rec, first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+                        // the bin's first chunk was loaded during the previous bin
+                        const uint64_t rec = c0 == rb ? c_rec : recs[c0 + i];
+                        const uint32_t first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
                         n0 = (uint32_t)(rec >> 56);
-                        const ulonglong2 oe = act_oe[a];
+                        const ulonglong2 oe = c0 == rb ? c_oe : act_oe[(uint32_t)rec];
                         s_pos[i] = oe.x * 4 + first;
                         s_ev[i] = (Ev)(oe.y + first - ev0);
                     }
@@ -808,6 +863,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 __syncthreads();
                 const uint32_t nwin = s_nwin;
                 c0 += s_nr;
+                issue_next_records();
                 // the record holding window x, and its window in that record
                 auto locate = [&](uint32_t x, uint32_t& rec, uint32_t& dj) {
                     rec = s_map[x];
@@ -822,7 +878,10 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     raw_fetch<KW>(data, s_pos[rec] + dj, raw);
                 }
                 for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
-                    if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                    // an overflowed pass stops early; the flag's load completes under the
+                    // key's arithmetic instead of stalling the iteration
+                    const uint32_t over =
+                        __hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     const RawWindow<KW> cur = raw;
                     const uint32_t lo = rec, cdj = dj;
                     if (x + blockDim.x < nwin) {
@@ -832,13 +891,20 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     uint64_t key[NW];
                     pack_key<NW>(canonical<KW>(raw_extract<KW>(cur, k), k), key);
                     const uint32_t h = bin_slot_hash<NW>(key);
+                    if (over) break;
                     if (parts > 1 && ((h >> 20) & (parts - 1)) != part) continue;
                     const Ev t = (Ev)(s_ev[lo] + cdj);
                     uint32_t slot = h & (BIN3_CAP - 1);
                     bool done = false;
                     for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
+                        // both key words are loaded at once: an occupied slot's words are
+                        // final once not EMPTY, and an EMPTY one is settled by the CAS
+                        uint64_t w0 = __hip_atomic_load(&s_k0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        uint64_t w1 = NW == 2 ? __hip_atomic_load(&s_k1[slot], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
+                                              : 0;
                         bool claimed = false;
-                        bool hit = lds_claim_word(&s_k0[slot], key[0], claimed);
+                        bool hit = lds_claim_loaded(&s_k0[slot], w0, key[0], claimed);
                         if (claimed) {
                             const uint32_t f = atomicAdd(&s_fill, 1u);
                             if (f < BIN3_FILL) s_list[f] = (uint16_t)slot;
@@ -846,13 +912,12 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                         }
                         if (hit && NW == 2) {
                             bool c2 = false;
-                            hit = lds_claim_word(&s_k1[slot], key[NW - 1], c2);
+                            hit = lds_claim_loaded(&s_k1[slot], w1, key[NW - 1], c2);
                         }
                         if (hit) {
+                            // neither returns a value, so the window waits on neither
                             atomicAdd(&s_cnt[slot], 1u);
-                            // t0 only falls: skip the atomic when it cannot lower it
-                            if (__hip_atomic_load(&s_t0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > t)
-                                atomicMin(&s_t0[slot], t);
+                            atomicMin(&s_t0[slot], t);
                             done = true;
                             break;
                         }
@@ -860,6 +925,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     }
                     if (!done) atomicOr(&s_over, 1u);
                 }
+                issue_next_pairs();
             }
             __syncthreads();
             if (s_over) {
@@ -908,6 +974,18 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
             }
             __syncthreads();
         }
+        }
+    next_bin:
+        issue_next_pairs();  // (an empty bin, or one whose passes all ended early)
+        if (tid == 0) s_bin[0] = q_next;
+        __syncthreads();
+        bin = nbin;
+        nbin = s_bin[0];
+        c_rb = n_rb;
+        c_re = n_re;
+        c_rec = n_rec;
+        c_oe = n_oe;
+        n_v = load_range(nbin);
     }
 }
 
@@ -2436,12 +2514,17 @@ struct Run {
                                                  c->d_act_oe.as<ulonglong2>());
         // count: one library writes straight into the table pool, several into staging
         const bool direct = nL == 1;
-        // workgroups: one per bin, so that the dispatcher balances the uneven bins (measured
-        // faster than any CU-multiple grid looping over bins); kc_tune.blocks_per_cu overrides
-        const uint64_t g_max = c->tune.blocks_per_cu ? (uint64_t)c->n_cu * c->tune.blocks_per_cu : nbins;
-        // (a launch holds at most 2^32 - 1 work-items: many libraries with many bins each
-        // loop over their bins instead)
+        // workgroups: as many as the device holds at once (they take bins from a queue);
+        // kc_tune.blocks_per_cu overrides
+        static const int resident = [] {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_bin_count<KW, Ev>, BLOCK, 0) != hipSuccess || n < 1)
+                n = 1;
+            return n;
+        }();
+        const uint64_t g_max = (uint64_t)c->n_cu * (c->tune.blocks_per_cu ? c->tune.blocks_per_cu : (uint32_t)resident);
         const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({nbins, g_max, (uint64_t)1 << 23}));
+        unsigned int* queue = reinterpret_cast<unsigned int*>(c->d_counters.as<unsigned long long>() + BIN_QUEUE);
         if (planned) {
             cap = c->g_bin_cap;
             lcnt = c->g_bin_lcnt;
@@ -2458,12 +2541,14 @@ struct Run {
             }
             k_fill<unsigned long long><<<grid_for(nL), BLOCK, 0, st>>>(c->d_bcnt.as<unsigned long long>(), nL, 0ull);
             k_fill<unsigned long long><<<1, 64, 0, st>>>(ctr, 3, 0ull);
+            k_fill<unsigned int><<<1, 64, 0, st>>>(queue, 1, 0u);
             TIMED(KID_BIN_COUNT, (k_bin_count<KW, Ev><<<g, BLOCK, 0, st>>>(
                                      c->d_br2.as<uint64_t>(), c->d_bbounds.as<uint64_t>(), data,
                                      c->d_act_oe.as<ulonglong2>(), k,
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
-                                     c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr)));
+                                     c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr,
+                                     queue)));
             if (planned) {
                 k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
                     c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
@@ -4378,7 +4463,7 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
     enum { F_LOAD, F_WPT, F_BPC, F_KPB, F_M, F_SETTER };
     std::vector<Dim> dims;
     if (binned) {
-        dims.push_back({F_KPB, {192, 256, 320, 448, 640}});
+        dims.push_back({F_KPB, {192, 256, 352, 448, 640}});
         // minimizer lengths around the default for this k (0 = the default itself)
         std::vector<double> ms{0};
         for (uint32_t m : {11u, 13u, 15u, 16u})

@@ -644,12 +644,28 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
 // when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
 // insert instead).
 // The bin sort's onesweep passes take 10 bits each, so the 15-20 bits of (library, bin)
-// keys sort in two passes instead of three.
+// keys sort in two passes instead of three (and 16 items a thread: 1.60 -> 1.50 ms at
+// k = 21 against 12).  The first-setter items (64-bit keys, 15-bit
+// partition numbers) sort fastest with 1024-thread blocks of 8 items (5 Mb / 100x, k = 55:
+// 3.71 -> 3.01 ms against rocPRIM's 512 x 12 for gfx950; 6, 10, 12 or 16 items were slower).
 using BinSortConfig = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 12>, 10,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+using FsSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 8>, 8,
                                         rocprim::block_radix_rank_algorithm::match>>;
 constexpr uint32_t BIN3_RCHUNK = BLOCK;  // records staged per chunk (one per thread)
+// A staged record: stream base number (2-bit position) of its first window, the event of
+// its first window (library-relative), and its first window in the chunk (before the
+// chunk's scan: its number of windows).  One LDS read gives a window all three.
+template <typename Ev>
+struct StagedRec {
+    uint64_t pos;
+    uint32_t woff;
+    Ev ev;
+};
 constexpr uint32_t BIN3_MAP = 2048;      // windows per chunk (the window -> record map)
 // Transient buffers above this size are released after use instead of kept for reuse.
 constexpr uint64_t BIG_SCRATCH = 8ull << 30;
@@ -702,7 +718,9 @@ __global__ void k_pair_oe(const uint64_t* __restrict__ act_off, const uint64_t* 
 }
 
 template <int KW, typename Ev>
-__global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict__ recs,
+// Waves a SIMD holds: LDS allows 5 workgroups of 4 waves a CU at k <= 63 (NW = 2) and 6 at
+// k <= 32; registers are held to what that many waves leave.
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW == 1 ? 6 : 5, KW == 1 ? 6 : 5))) k_bin_count(const uint64_t* __restrict__ recs,
                                                      const uint64_t* __restrict__ bounds,
                                                      const uint8_t* __restrict__ data,
                                                      const ulonglong2* __restrict__ act_oe, uint32_t k,
@@ -712,16 +730,15 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                                                      const uint64_t* __restrict__ capl,
                                                      unsigned long long* __restrict__ lcnt,
                                                      unsigned long long* __restrict__ ctr,
-                                                     unsigned int* __restrict__ queue) {
+                                                     unsigned int* __restrict__ queue,
+                                                     Slot<nw_of(KW), Ev>* __restrict__ kept) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
     __shared__ uint64_t s_k0[BIN3_CAP];
     __shared__ uint64_t s_k1[NW == 2 ? BIN3_CAP : 1];
     __shared__ uint32_t s_cnt[BIN3_CAP];
     __shared__ Ev s_t0[BIN3_CAP];
-    __shared__ uint64_t s_pos[BIN3_RCHUNK];  // staged record: stream base number of its first window,
-    __shared__ Ev s_ev[BIN3_RCHUNK];         // event of its first window (library-relative),
-    __shared__ uint32_t s_woff[BIN3_RCHUNK];  // and its first window in the chunk
+    __shared__ StagedRec<Ev> s_rec[BIN3_RCHUNK];  // the chunk's staged records
     __shared__ uint8_t s_map[BIN3_MAP];       // the staged record of every window of the chunk
     __shared__ uint16_t s_list[BIN3_FILL];    // the pass's claimed slots, in claim order
     __shared__ uint32_t s_wsum[BLOCK / 64], s_wfit[BLOCK / 64];
@@ -762,15 +779,22 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
     }
     __syncthreads();
     uint32_t bin = s_bin[0], nbin = s_bin[1];
-    uint64_t c_rb, c_re, c_rec = 0;  // the current bin's range and first-chunk record of this thread
-    ulonglong2 c_oe = make_ulonglong2(0, 0);
+    // A bin's first chunk of records is staged in LDS before the bin starts: pos and ev as
+    // the staging below writes them, and each record's windows in woff.
+    auto prestage = [&](uint64_t rec, ulonglong2 oe, uint64_t ev0) {
+        const uint32_t first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+        s_rec[tid].pos = oe.x * 4 + first;
+        s_rec[tid].ev = (Ev)(oe.y + first - ev0);
+        s_rec[tid].woff = (uint32_t)(rec >> 56);
+    };
+    uint64_t c_rb, c_re;  // the current bin's records
     {
         const uint64_t v = load_range(bin);
         c_rb = lane_u64(v, 0);
         c_re = lane_u64(v, 1);
         if (tid < min<uint64_t>(BIN3_RCHUNK, c_re - c_rb)) {
-            c_rec = recs[c_rb + tid];
-            c_oe = act_oe[(uint32_t)c_rec];
+            const uint64_t rec = recs[c_rb + tid];
+            prestage(rec, act_oe[(uint32_t)rec], libs[bin >> bits].ev0);
         }
     }
     uint64_t n_v = load_range(nbin);
@@ -781,6 +805,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
         const uint64_t rb = c_rb, re = c_re;
         uint64_t n_rb = 0, n_re = 0, n_rec = 0;
         ulonglong2 n_oe = make_ulonglong2(0, 0);
+        const uint64_t n_ev0 = nbin < nbins ? libs[nbin >> bits].ev0 : 0;
         bool pf_rec = false, pf_oe = false;  // (uniform)
         auto issue_next_records = [&]() {
             if (pf_rec) return;
@@ -806,30 +831,39 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
             s_pass[1] = 0;
         }
         __syncthreads();
-        while (s_np) {
-            const uint32_t parts = s_pass[2 * (s_np - 1)], part = s_pass[2 * (s_np - 1) + 1];
+        bool one_chunk = false;  // (uniform) the bin's records fit one staged chunk
+        for (uint32_t pass_no = 0; s_np; ++pass_no) {
+            uint32_t parts = s_pass[2 * (s_np - 1)];
+            const uint32_t part = s_pass[2 * (s_np - 1) + 1];
             __syncthreads();
             if (threadIdx.x == 0) {
                 s_fill = 0;
                 s_over = 0;
                 --s_np;
             }
-            for (uint64_t c0 = rb; c0 < re;) {
+            bool gave_up = false;  // (uniform) the bin needs more passes than allowed
+            for (uint64_t c0 = rb; c0 < re && !gave_up;) {
                 const uint32_t nr = (uint32_t)min<uint64_t>(BIN3_RCHUNK, re - c0);
                 __syncthreads();  // the previous chunk's staging reads are done
-                if (s_over) break;
-                {  // stage one record per thread; scan of the window counts (waves, then across);
-                   // the chunk takes the leading records whose windows fit the map
+                // the first chunk of the first pass is already in LDS; a bin of one chunk keeps
+                // it staged for its later passes
+                const bool first = c0 == rb;
+                if (!(first && pass_no > 0 && one_chunk)) {
+                    // stage one record per thread; scan of the window counts (waves, then
+                    // across); the chunk takes the leading records whose windows fit the map
                     const uint32_t i = threadIdx.x, lane = i & 63, wave = i >> 6;
                     uint32_t n0 = 0;
                     if (i < nr) {
-                        // the bin's first chunk was loaded during the previous bin
-                        const uint64_t rec = c0 == rb ? c_rec : recs[c0 + i];
-                        const uint32_t first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
-                        n0 = (uint32_t)(rec >> 56);
-                        const ulonglong2 oe = c0 == rb ? c_oe : act_oe[(uint32_t)rec];
-                        s_pos[i] = oe.x * 4 + first;
-                        s_ev[i] = (Ev)(oe.y + first - ev0);
+                        if (first && pass_no == 0) {
+                            n0 = s_rec[i].woff;
+                        } else {
+                            const uint64_t rec = recs[c0 + i];
+                            const uint32_t fw = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+                            n0 = (uint32_t)(rec >> 56);
+                            const ulonglong2 oe = act_oe[(uint32_t)rec];
+                            s_rec[i].pos = oe.x * 4 + fw;
+                            s_rec[i].ev = (Ev)(oe.y + fw - ev0);
+                        }
                     }
                     uint32_t inc = n0;
 #pragma unroll
@@ -846,7 +880,7 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                     const bool fits = i < nr && end <= BIN3_MAP;  // a prefix of the records
                     const uint64_t fit = __ballot(fits);
                     if (lane == 0) s_wfit[wave] = (uint32_t)__popcll(fit);
-                    if (i < nr) s_woff[i] = end - n0;
+                    if (i < nr) s_rec[i].woff = end - n0;
                     __syncthreads();
                     uint32_t nfit = 0;
 #pragma unroll
@@ -863,89 +897,140 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
                 __syncthreads();
                 const uint32_t nwin = s_nwin;
                 c0 += s_nr;
+                if (first && pass_no == 0) one_chunk = c0 >= re;
                 issue_next_records();
-                // the record holding window x, and its window in that record
-                auto locate = [&](uint32_t x, uint32_t& rec, uint32_t& dj) {
-                    rec = s_map[x];
-                    dj = x - s_woff[rec];
+                // window x: its stream base number and its event (one read of its record)
+                auto locate = [&](uint32_t x, uint64_t& pos, Ev& ev) {
+                    const StagedRec<Ev> r = s_rec[s_map[x]];
+                    pos = r.pos + (x - r.woff);
+                    ev = (Ev)(r.ev + (x - r.woff));
                 };
-                // The packed words of the next window are fetched one iteration ahead, so
-                // their load is in flight while this window's LDS work runs.
-                RawWindow<KW> raw;
-                uint32_t rec = 0, dj = 0;
-                if (threadIdx.x < nwin) {
-                    locate(threadIdx.x, rec, dj);
-                    raw_fetch<KW>(data, s_pos[rec] + dj, raw);
-                }
-                for (uint32_t x = threadIdx.x; x < nwin; x += blockDim.x) {
-                    // an overflowed pass stops early; the flag's load completes under the
-                    // key's arithmetic instead of stalling the iteration
-                    const uint32_t over =
-                        __hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const RawWindow<KW> cur = raw;
-                    const uint32_t lo = rec, cdj = dj;
-                    if (x + blockDim.x < nwin) {
-                        locate(x + blockDim.x, rec, dj);
-                        raw_fetch<KW>(data, s_pos[rec] + dj, raw);
+                // Each thread takes windows x = tid, tid + 256, ...; a full table stops every
+                // thread at its first window not counted, and counting resumes there.
+                uint32_t x = threadIdx.x;
+                for (;;) {
+                    // The packed words of the next window are fetched one iteration ahead, so
+                    // their load is in flight while this window's LDS work runs.
+                    RawWindow<KW> raw;
+                    uint64_t pos = 0;
+                    Ev ev = 0;
+                    if (x < nwin) {
+                        locate(x, pos, ev);
+                        raw_fetch<KW>(data, pos, raw);
                     }
-                    uint64_t key[NW];
-                    pack_key<NW>(canonical<KW>(raw_extract<KW>(cur, k), k), key);
-                    const uint32_t h = bin_slot_hash<NW>(key);
-                    if (over) break;
-                    if (parts > 1 && ((h >> 20) & (parts - 1)) != part) continue;
-                    const Ev t = (Ev)(s_ev[lo] + cdj);
-                    uint32_t slot = h & (BIN3_CAP - 1);
-                    bool done = false;
-                    for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
-                        // both key words are loaded at once: an occupied slot's words are
-                        // final once not EMPTY, and an EMPTY one is settled by the CAS
-                        uint64_t w0 = __hip_atomic_load(&s_k0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        uint64_t w1 = NW == 2 ? __hip_atomic_load(&s_k1[slot], __ATOMIC_RELAXED,
-                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
-                                              : 0;
-                        bool claimed = false;
-                        bool hit = lds_claim_loaded(&s_k0[slot], w0, key[0], claimed);
-                        if (claimed) {
-                            const uint32_t f = atomicAdd(&s_fill, 1u);
-                            if (f < BIN3_FILL) s_list[f] = (uint16_t)slot;
-                            else atomicOr(&s_over, 1u);
+                    for (; x < nwin; x += blockDim.x) {
+                        // the full-table flag's load completes under the key's arithmetic
+                        // instead of stalling the iteration
+                        const uint32_t over =
+                            __hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const RawWindow<KW> cur = raw;
+                        const Ev t = ev;
+                        if (x + blockDim.x < nwin) {
+                            locate(x + blockDim.x, pos, ev);
+                            raw_fetch<KW>(data, pos, raw);
                         }
-                        if (hit && NW == 2) {
-                            bool c2 = false;
-                            hit = lds_claim_loaded(&s_k1[slot], w1, key[NW - 1], c2);
+                        uint64_t key[NW];
+                        pack_key<NW>(canonical<KW>(raw_extract<KW>(cur, k), k), key);
+                        const uint32_t h = bin_slot_hash<NW>(key);
+                        if (over) break;  // window x is not counted
+                        if (parts > 1 && ((h >> 20) & (parts - 1)) != part) continue;
+                        uint32_t slot = h & (BIN3_CAP - 1);
+                        bool done = false;
+                        for (uint32_t probe = 0; probe < BIN3_CAP; ++probe) {
+                            // both key words are loaded at once: an occupied slot's words are
+                            // final once not EMPTY, and an EMPTY one is settled by the CAS
+                            uint64_t w0 =
+                                __hip_atomic_load(&s_k0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            uint64_t w1 = NW == 2 ? __hip_atomic_load(&s_k1[slot], __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_WORKGROUP)
+                                                  : 0;
+                            bool claimed = false;
+                            bool hit = lds_claim_loaded(&s_k0[slot], w0, key[0], claimed);
+                            if (claimed) {
+                                const uint32_t f = atomicAdd(&s_fill, 1u);
+                                if (f < BIN3_FILL) s_list[f] = (uint16_t)slot;
+                                else atomicOr(&s_over, 1u);
+                            }
+                            if (hit && NW == 2) {
+                                bool c2 = false;
+                                hit = lds_claim_loaded(&s_k1[slot], w1, key[NW - 1], c2);
+                            }
+                            if (hit) {
+                                // neither returns a value, so the window waits on neither
+                                atomicAdd(&s_cnt[slot], 1u);
+                                atomicMin(&s_t0[slot], t);
+                                done = true;
+                                break;
+                            }
+                            slot = (slot + 1) & (BIN3_CAP - 1);
                         }
-                        if (hit) {
-                            // neither returns a value, so the window waits on neither
-                            atomicAdd(&s_cnt[slot], 1u);
-                            atomicMin(&s_t0[slot], t);
-                            done = true;
+                        if (!done) {  // no free slot: window x is not counted
+                            atomicOr(&s_over, 1u);
                             break;
                         }
-                        slot = (slot + 1) & (BIN3_CAP - 1);
                     }
-                    if (!done) atomicOr(&s_over, 1u);
+                    issue_next_pairs();
+                    __syncthreads();
+                    if (!s_over) break;
+                    // The table is full.  The keys of this pass whose next key-hash bit is 0
+                    // stay (with their counts and first events) and go on counting from where
+                    // each thread stopped; those with the bit set get a pass of their own.
+                    if (2 * parts > BIN3_MAX_SPLIT || s_np + 1 > 32) {
+                        __syncthreads();
+                        clear_table();
+                        if (threadIdx.x == 0) {
+                            atomicOr(reinterpret_cast<unsigned int*>(&ctr[1]), 1u);
+                            s_np = 0;
+                        }
+                        gave_up = true;
+                        __syncthreads();
+                        break;
+                    }
+                    Slot<NW, Ev>* __restrict__ keep = kept + (size_t)blockIdx.x * BIN3_CAP;
+                    for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
+                        Slot<NW, Ev> e;
+                        e.k[0] = s_k0[sl];
+                        if (NW == 2) e.k[NW - 1] = s_k1[sl];
+                        e.t0 = s_t0[sl];
+                        e.cnt = s_cnt[sl];
+                        if (e.k[0] != EMPTY_KEY && ((bin_slot_hash<NW>(e.k) >> 20) & parts)) e.k[0] = EMPTY_KEY;
+                        keep[sl] = e;
+                    }
+                    __syncthreads();
+                    clear_table();
+                    if (threadIdx.x == 0) {
+                        s_fill = 0;
+                        s_over = 0;
+                        s_pass[2 * s_np] = 2 * parts;
+                        s_pass[2 * s_np + 1] = part + parts;
+                        ++s_np;
+                    }
+                    parts *= 2;
+                    __syncthreads();
+                    // put the kept keys back (distinct keys: a slot is taken by the CAS of its first word)
+                    for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
+                        const Slot<NW, Ev> e = keep[sl];
+                        if (e.k[0] == EMPTY_KEY) continue;
+                        uint32_t slot = bin_slot_hash<NW>(e.k) & (BIN3_CAP - 1);
+                        // (at most BIN3_CAP keys were kept, so a free slot is always found)
+                        for (uint32_t probe = 1; probe < BIN3_CAP &&
+                                                 atomicCAS(reinterpret_cast<unsigned long long*>(&s_k0[slot]),
+                                                           (unsigned long long)EMPTY_KEY,
+                                                           (unsigned long long)e.k[0]) != EMPTY_KEY;
+                             ++probe)
+                            slot = (slot + 1) & (BIN3_CAP - 1);
+                        if (NW == 2) s_k1[slot] = e.k[NW - 1];
+                        s_cnt[slot] = e.cnt;
+                        s_t0[slot] = e.t0;
+                        const uint32_t f = atomicAdd(&s_fill, 1u);
+                        if (f < BIN3_FILL) s_list[f] = (uint16_t)slot;
+                        else atomicOr(&s_over, 1u);
+                    }
+                    __syncthreads();
                 }
-                issue_next_pairs();
             }
             __syncthreads();
-            if (s_over) {
-                clear_table();
-                // split this pass in two by the next key-hash bit
-                if (threadIdx.x == 0) {
-                    if (2 * parts > BIN3_MAX_SPLIT || s_np + 2 > 32) {
-                        atomicOr(reinterpret_cast<unsigned int*>(&ctr[1]), 1u);
-                        s_np = 0;
-                    } else {
-                        s_pass[2 * s_np] = 2 * parts;
-                        s_pass[2 * s_np + 1] = part;
-                        s_pass[2 * s_np + 2] = 2 * parts;
-                        s_pass[2 * s_np + 3] = part + parts;
-                        s_np += 2;
-                    }
-                }
-                __syncthreads();
-                continue;
-            }
+            if (gave_up) continue;
             if (threadIdx.x == 0) {
                 s_base = atomicAdd(&lcnt[lib], (unsigned long long)s_fill);
                 if (s_base + s_fill > capl[lib]) {
@@ -977,14 +1062,14 @@ __global__ void __launch_bounds__(BLOCK) k_bin_count(const uint64_t* __restrict_
         }
     next_bin:
         issue_next_pairs();  // (an empty bin, or one whose passes all ended early)
+        // (every read of the staging arrays is behind the last barrier)
+        if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) prestage(n_rec, n_oe, n_ev0);
         if (tid == 0) s_bin[0] = q_next;
         __syncthreads();
         bin = nbin;
         nbin = s_bin[0];
         c_rb = n_rb;
         c_re = n_re;
-        c_rec = n_rec;
-        c_oe = n_oe;
         n_v = load_range(nbin);
     }
 }
@@ -2074,6 +2159,7 @@ struct kc_ctx {
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
+    DevBuf d_kept;  // binned count: a full LDS table's kept keys, one table's room per workgroup
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
@@ -2535,6 +2621,7 @@ struct Run {
             else KC_TRY(c->d_pack.ensure(std::max<uint64_t>(cap0[nL], 1) * sizeof(S)));
             KC_TRY(c->d_bcap.ensure(nL * 16 + 16));
             KC_TRY(c->d_bcnt.ensure(nL * 8 + 8));
+            KC_TRY(c->d_kept.ensure((uint64_t)g * BIN3_CAP * sizeof(S)));
             if (!planned) {  // a replay finds the plan's room in d_bcap
                 KC_TRY(hipMemcpyAsync(c->d_bcap.p, cap0.data(), nL * 8, hipMemcpyHostToDevice, st));
                 KC_TRY(hipMemcpyAsync(c->d_bcap.as<uint64_t>() + nL, cap.data(), nL * 8, hipMemcpyHostToDevice, st));
@@ -2548,7 +2635,7 @@ struct Run {
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr,
-                                     queue)));
+                                     queue, c->d_kept.as<S>())));
             if (planned) {
                 k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
                     c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
@@ -2908,12 +2995,13 @@ struct Run {
                                             c->d_lib_b0.as<uint64_t>(), L(), c->d_libs.as<LibDesc>(),
                                             c->d_out.as<LibOut>(), c->d_fs.as<uint8_t>(), c->d_fsi.as<uint64_t>())));
                     size_t tb = 0;
-                    KC_TRY(rocprim::radix_sort_keys(nullptr, tb, c->d_fsi.as<uint64_t>(), c->d_fsi2.as<uint64_t>(),
-                                                    (size_t)n_items, FS_KEY_SHIFT, FS_KEY_SHIFT + key_bits, st));
+                    KC_TRY(rocprim::radix_sort_keys<FsSortConfig>(nullptr, tb, c->d_fsi.as<uint64_t>(),
+                                                                  c->d_fsi2.as<uint64_t>(), (size_t)n_items,
+                                                                  FS_KEY_SHIFT, FS_KEY_SHIFT + key_bits, st));
                     KC_TRY(c->d_temp.ensure(tb));
                     tb = c->d_temp.bytes;
                     TIMED(KID_FS_SORT,
-                          KC_TRY(rocprim::radix_sort_keys(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
+                          KC_TRY(rocprim::radix_sort_keys<FsSortConfig>(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
                                                           c->d_fsi2.as<uint64_t>(), (size_t)n_items, FS_KEY_SHIFT,
                                                           FS_KEY_SHIFT + key_bits, st)));
                 }
@@ -2958,12 +3046,13 @@ struct Run {
                                                 c->d_fsi.as<uint64_t>())));
                     if (n) {
                         size_t tb = 0;
-                        KC_TRY(rocprim::radix_sort_keys(nullptr, tb, c->d_fsi.as<uint64_t>(), c->d_fsi2.as<uint64_t>(),
-                                                        (size_t)n, shift, shift + key_bits, st));
+                        KC_TRY(rocprim::radix_sort_keys<FsSortConfig>(nullptr, tb, c->d_fsi.as<uint64_t>(),
+                                                                      c->d_fsi2.as<uint64_t>(), (size_t)n, shift,
+                                                                      shift + key_bits, st));
                         KC_TRY(c->d_temp.ensure(tb));
                         tb = c->d_temp.bytes;
                         TIMED(KID_FS_SORT,
-                              KC_TRY(rocprim::radix_sort_keys(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
+                              KC_TRY(rocprim::radix_sort_keys<FsSortConfig>(c->d_temp.p, tb, c->d_fsi.as<uint64_t>(),
                                                               c->d_fsi2.as<uint64_t>(), (size_t)n, shift,
                                                               shift + key_bits, st)));
                     }
@@ -3516,7 +3605,7 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
                       &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
-                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt})
+                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_kept})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);

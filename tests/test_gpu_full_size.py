@@ -64,3 +64,23 @@ def test_1mb_30x_matches_oracle(store_1mb, k, ec):
     say(f"case 1mb_30x k={k} ec={ec} reads={len(store_1mb)}")
     for got in run_digests(store_1mb, k, ec):
         assert not diff(got, want), (diff(got, want), got, want)
+
+
+@pytest.mark.parametrize("k", [21, 55])
+def test_5mb_100x_with_full_bin_tables_matches_oracle(store_5mb, k):
+    """Bins sized for 4096 distinct k-mers (2,600-3,000 on average here, against the LDS
+    table's 768): nearly every bin fills its table, keeps half of its keys by the next
+    key-hash bit and hands the other half to a pass of its own, often several times."""
+    want = DIGESTS[f"5mb_100x_k{k}"]
+    say(f"case 5mb_100x k={k} kmers_per_bin=4096")
+    for got in run_digests(store_5mb, k, False, tune={"kmers_per_bin": 4096}):
+        assert not diff(got, want), (diff(got, want), got, want)
+
+
+def test_1mb_30x_hammer_with_full_bin_tables_matches_oracle(store_1mb):
+    """The same for Hammer's recount over the corrected reads, which also takes the
+    binned count."""
+    want = DIGESTS["1mb_30x_k21_hammer"]
+    say("case 1mb_30x k=21 ec=True kmers_per_bin=4096")
+    for got in run_digests(store_1mb, 21, True, tune={"kmers_per_bin": 4096}):
+        assert not diff(got, want), (diff(got, want), got, want)

@@ -204,6 +204,8 @@ constexpr uint32_t HAMMER_QUERIES = 20;
 constexpr uint32_t PLAN_GUARD = 22;
 // Counter slot holding the binned count's queue of bins (the next bin to take).
 constexpr uint32_t BIN_QUEUE = 23;
+// Counter slot receiving the number of bins that hold records.
+constexpr uint32_t BIN_LIST_N = 24;
 static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
@@ -631,6 +633,29 @@ __global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint
     }
 }
 
+// The binned count's work items: library l's 2^bits bins in groups of 2^gsh[l]
+// consecutive bins (gmeta = gsh[0 .. n_libs), then the first item of each library,
+// gbase[0 .. n_libs]), item i = lo | hi << 32 for its bins [lo, hi), flagged when it holds
+// records.
+__global__ void k_bin_items(const uint64_t* __restrict__ bounds, uint32_t n_libs, uint32_t bits,
+                            const uint32_t* __restrict__ gmeta, uint64_t n_items, uint64_t* __restrict__ items,
+                            uint8_t* __restrict__ flags) {
+    const uint32_t* gbase = gmeta + n_libs;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_items;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = n_libs - 1;  // the last library with gbase <= i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (gbase[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t sh = gmeta[lo];
+        const uint32_t b0 = (lo << bits) | ((uint32_t)(i - gbase[lo]) << sh), b1 = b0 + (1u << sh);
+        items[i] = b0 | (uint64_t)b1 << 32;
+        flags[i] = bounds[b1] > bounds[b0];
+    }
+}
+
 // One (library, bin) per workgroup pass (see above).  The bin's records are staged in LDS in chunks
 // with the prefix sum of their window counts, and the chunk's windows are dealt to the
 // threads in order: neighbouring lanes take neighbouring windows of one record (distinct
@@ -731,6 +756,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                                                      unsigned long long* __restrict__ lcnt,
                                                      unsigned long long* __restrict__ ctr,
                                                      unsigned int* __restrict__ queue,
+                                                     const uint64_t* __restrict__ blist,
+                                                     const unsigned int* __restrict__ n_blist,
                                                      Slot<nw_of(KW), Ev>* __restrict__ kept) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
@@ -745,7 +772,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
     __shared__ uint32_t s_fill, s_over, s_np, s_nr, s_nwin;
     __shared__ uint32_t s_pass[2 * 32];  // pending passes: (parts, part) pairs
     __shared__ unsigned long long s_base;
-    __shared__ uint32_t s_bin[2];
+    __shared__ uint64_t s_bin[2];
     static_assert(BIN3_CAP <= 65536, "slot numbers are 16-bit");
     static_assert(BIN3_RCHUNK <= 256 && BIN3_MAP >= SK_MAX_WINDOWS, "map entries are 8-bit, a record fits a chunk");
     // The table starts empty and every pass leaves it empty: its claimed slots are listed
@@ -760,25 +787,33 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
         }
     };
     clear_table();
-    // Workgroups stay resident and take bins from a queue (so that uneven bins balance), and
+    // Workgroups stay resident and take work items from a queue over the list
+    // blist[0 .. *n_blist) (so that uneven items balance): an item is a range of bins
+    // [lo, hi) of one library, lo | hi << 32, that holds records (a library with far fewer
+    // k-mers than the largest counts several of its bins at once).  And
     // the loads that start a bin (its record range, the first chunk's records and their
     // (offset, first event) pairs) are issued during the previous bin: the next bin's range
     // at the start of a bin, its records after the first chunk is staged, their pairs after
     // the first chunk's windows.  The record range is loaded by lanes 0 and 1 of each wave
     // as vector loads and read back with readlane.
     const uint32_t tid = threadIdx.x;
-    auto load_range = [&](uint32_t b) -> uint64_t { return b < nbins ? bounds[b + (tid & 1)] : 0; };
+    const uint64_t none = nbins | (uint64_t)nbins << 32;
+    auto load_range = [&](uint64_t it) -> uint64_t {
+        return (uint32_t)it < nbins ? bounds[(tid & 1) ? (uint32_t)(it >> 32) : (uint32_t)it] : 0;
+    };
     auto lane_u64 = [](uint64_t v, int l) -> uint64_t {
         return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l) |
                (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
     };
+    const uint32_t nl = *n_blist;
+    auto listed = [&](uint32_t q) -> uint64_t { return q < nl ? blist[q] : none; };
     if (tid == 0) {
-        const uint32_t b = atomicAdd(queue, 2u);
-        s_bin[0] = b;
-        s_bin[1] = b + 1;
+        const uint32_t q = atomicAdd(queue, 2u);
+        s_bin[0] = listed(q);
+        s_bin[1] = listed(q + 1);
     }
     __syncthreads();
-    uint32_t bin = s_bin[0], nbin = s_bin[1];
+    uint64_t item = s_bin[0], nitem = s_bin[1];
     // A bin's first chunk of records is staged in LDS before the bin starts: pos and ev as
     // the staging below writes them, and each record's windows in woff.
     auto prestage = [&](uint64_t rec, ulonglong2 oe, uint64_t ev0) {
@@ -789,23 +824,24 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
     };
     uint64_t c_rb, c_re;  // the current bin's records
     {
-        const uint64_t v = load_range(bin);
+        const uint64_t v = load_range(item);
         c_rb = lane_u64(v, 0);
         c_re = lane_u64(v, 1);
         if (tid < min<uint64_t>(BIN3_RCHUNK, c_re - c_rb)) {
             const uint64_t rec = recs[c_rb + tid];
-            prestage(rec, act_oe[(uint32_t)rec], libs[bin >> bits].ev0);
+            prestage(rec, act_oe[(uint32_t)rec], libs[(uint32_t)item >> bits].ev0);
         }
     }
-    uint64_t n_v = load_range(nbin);
-    while (bin < nbins) {
+    uint64_t n_v = load_range(nitem);
+    while ((uint32_t)item < nbins) {
         __syncthreads();  // every thread has read s_bin
-        uint32_t q_next = 0;  // the bin after nbin, published at the end of this one
+        uint32_t q_next = 0;  // the item after nitem: its list index, then the item
+        uint64_t nn_item = none;
         if (tid == 0) q_next = atomicAdd(queue, 1u);
         const uint64_t rb = c_rb, re = c_re;
         uint64_t n_rb = 0, n_re = 0, n_rec = 0;
         ulonglong2 n_oe = make_ulonglong2(0, 0);
-        const uint64_t n_ev0 = nbin < nbins ? libs[nbin >> bits].ev0 : 0;
+        const uint64_t n_ev0 = (uint32_t)nitem < nbins ? libs[(uint32_t)nitem >> bits].ev0 : 0;
         bool pf_rec = false, pf_oe = false;  // (uniform)
         auto issue_next_records = [&]() {
             if (pf_rec) return;
@@ -818,11 +854,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
             if (pf_oe) return;
             issue_next_records();
             pf_oe = true;
+            if (tid == 0) nn_item = listed(q_next);
             if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) n_oe = act_oe[(uint32_t)n_rec];
         };
         if (rb == re) goto next_bin;
         {
-        const uint32_t lib = bin >> bits;
+        const uint32_t lib = (uint32_t)item >> bits;
         const uint64_t ev0 = libs[lib].ev0;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1064,13 +1101,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
         issue_next_pairs();  // (an empty bin, or one whose passes all ended early)
         // (every read of the staging arrays is behind the last barrier)
         if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) prestage(n_rec, n_oe, n_ev0);
-        if (tid == 0) s_bin[0] = q_next;
+        if (tid == 0) s_bin[0] = nn_item;
         __syncthreads();
-        bin = nbin;
-        nbin = s_bin[0];
+        item = nitem;
+        nitem = s_bin[0];
         c_rb = n_rb;
         c_re = n_re;
-        n_v = load_range(nbin);
+        n_v = load_range(nitem);
     }
 }
 
@@ -2160,6 +2197,7 @@ struct kc_ctx {
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
     DevBuf d_kept;  // binned count: a full LDS table's kept keys, one table's room per workgroup
+    DevBuf d_bflag, d_bitems, d_blist, d_bgmeta;  // binned count: its work items (ranges of bins)
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
@@ -2211,7 +2249,8 @@ struct kc_ctx {
     bool g_binned = false;        // the last Stage 2 took the binned count
     uint32_t g_memset_nodes = 0;  // memset nodes the last capture recorded (replaced by fills)
     uint32_t g_bin_bits = 0;
-    uint64_t g_bin_nrec = 0;
+    uint64_t g_bin_nrec = 0, g_bin_items = 0;
+    std::vector<uint32_t> h_bgmeta;  // binned count: the work items' group sizes, as uploaded
     std::vector<uint64_t> g_bin_cap, g_bin_lcnt;
     DevBuf d_glcnt;
     uint32_t g_np = 0;              // prior sequences / windows uploaded by the last count
@@ -2595,6 +2634,47 @@ struct Run {
         k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec, nbins,
                                                                      c->d_bbounds.as<uint64_t>());
         KC_TRY(hipGetLastError());
+        // Work items: the bins are sized for the largest library, so a library with far
+        // fewer distinct k-mers counts 2^gsh of its bins at once (about as many k-mers as a
+        // bin of the largest); only items that hold records are listed.
+        uint64_t n_items = 0;
+        if (planned) {
+            n_items = c->g_bin_items;
+        } else {
+            const uint32_t target = c->tune.kmers_per_bin ? c->tune.kmers_per_bin : BIN_TARGET;
+            std::vector<uint32_t>& gmeta = c->h_bgmeta;  // (kept until the copy is done)
+            gmeta.assign(2 * nL + 1, 0);
+            for (uint32_t l = 0; l < nL; ++l) {
+                const HostLib& H = c->libs[l];
+                const double e = (double)(H.est_k == k && H.est ? H.est : H.info.n_occ / 3);
+                uint32_t sh = 0;
+                while (sh < bits && e / (double)(1ull << bits) * (double)(2ull << sh) <= target) ++sh;
+                gmeta[l] = sh;
+                gmeta[nL + l] = (uint32_t)n_items;
+                n_items += (1ull << bits) >> sh;
+            }
+            gmeta[2 * nL] = (uint32_t)n_items;
+            KC_TRY(c->d_bgmeta.ensure(gmeta.size() * 4));
+            KC_TRY(hipMemcpyAsync(c->d_bgmeta.p, gmeta.data(), gmeta.size() * 4, hipMemcpyHostToDevice, st));
+            if (!recount) c->g_bin_items = n_items;
+        }
+        KC_TRY(c->d_bflag.ensure(n_items + 1));
+        KC_TRY(c->d_bitems.ensure((n_items + 1) * 8));
+        KC_TRY(c->d_blist.ensure((n_items + 1) * 8));
+        k_bin_items<<<grid_for(n_items), BLOCK, 0, st>>>(c->d_bbounds.as<uint64_t>(), nL, bits,
+                                                         c->d_bgmeta.as<uint32_t>(), n_items,
+                                                         c->d_bitems.as<uint64_t>(), c->d_bflag.as<uint8_t>());
+        KC_TRY(hipGetLastError());
+        unsigned int* n_blist = reinterpret_cast<unsigned int*>(c->d_counters.as<unsigned long long>() + BIN_LIST_N);
+        {
+            size_t tbl = 0;
+            KC_TRY(rocprim::select(nullptr, tbl, c->d_bitems.as<uint64_t>(), c->d_bflag.as<uint8_t>(),
+                                   c->d_blist.as<uint64_t>(), n_blist, (size_t)n_items, st));
+            KC_TRY(c->d_temp.ensure(tbl));
+            tbl = c->d_temp.bytes;
+            KC_TRY(rocprim::select(c->d_temp.p, tbl, c->d_bitems.as<uint64_t>(), c->d_bflag.as<uint8_t>(),
+                                   c->d_blist.as<uint64_t>(), n_blist, (size_t)n_items, st));
+        }
         KC_TRY(c->d_act_oe.ensure(na * 16 + 16));
         k_pair_oe<<<grid_for(na), BLOCK, 0, st>>>(c->d_act_off.as<uint64_t>(), c->d_act_ev.as<uint64_t>(), na,
                                                  c->d_act_oe.as<ulonglong2>());
@@ -2635,7 +2715,7 @@ struct Run {
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr,
-                                     queue, c->d_kept.as<S>())));
+                                     queue, c->d_blist.as<uint64_t>(), n_blist, c->d_kept.as<S>())));
             if (planned) {
                 k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
                     c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
@@ -3605,7 +3685,8 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
                       &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
-                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_kept})
+                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_kept, &ctx->d_bflag,
+                      &ctx->d_bitems, &ctx->d_blist, &ctx->d_bgmeta})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);

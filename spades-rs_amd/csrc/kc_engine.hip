@@ -1325,14 +1325,52 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict_
                                                      uint64_t* __restrict__ seen, uint64_t* __restrict__ solid) {
     __shared__ uint32_t m[FS_PART];
     __shared__ uint32_t sol[MULTI ? FS_PART / 32 : 1];
+    __shared__ unsigned long long sw[FS_PART / 64];  // a sparse partition's seen words
     constexpr uint32_t U = 4;  // items in flight per thread
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
+        const uint64_t b = bounds[part], end = bounds[part + 1];
+        const uint64_t e0 = (uint64_t)part * FS_PART;
+        const uint32_t nw = (uint32_t)(min<uint64_t>(FS_PART, fs_n - e0) / 64);
+        if (end == b) {  // no item: no entry is seen
+            for (uint32_t w = threadIdx.x; w < nw; w += FS_BLOCK) {
+                seen[e0 / 64 + w] = 0;
+                solid[e0 / 64 + w] = 0;
+            }
+            continue;
+        }
+        if (end - b < FS_PART / 8) {
+            // A sparse partition (small libraries on large tiers) touches only its items'
+            // entries and builds its seen (and counted-twice solid) words from the items:
+            // the promotion pass reads fs back only at its items' entries.
+            for (uint64_t i = b + threadIdx.x; i < end; i += FS_BLOCK) m[(uint32_t)(items[i] >> 32) & (FS_PART - 1)] = ~0u;
+            for (uint32_t w = threadIdx.x; w < FS_PART / 64; w += FS_BLOCK) sw[w] = 0;
+            if (MULTI)
+                for (uint32_t e = threadIdx.x; e < FS_PART / 32; e += FS_BLOCK) sol[e] = 0;
+            __syncthreads();
+            for (uint64_t i = b + threadIdx.x; i < end; i += FS_BLOCK) {
+                const uint64_t it = items[i];
+                const uint32_t e = (uint32_t)(it >> 32) & (FS_PART - 1);
+                atomicMin(&m[e], (uint32_t)it);
+                atomicOr(&sw[e >> 6], 1ull << (e & 63));
+                if (MULTI && (it >> 63)) atomicOr(&sol[e >> 5], 1u << (e & 31));
+            }
+            __syncthreads();
+            for (uint64_t i = b + threadIdx.x; i < end; i += FS_BLOCK) {
+                const uint32_t e = (uint32_t)(items[i] >> 32) & (FS_PART - 1);
+                fs[e0 + e] = m[e];  // (items of one entry write the same minimum)
+            }
+            for (uint32_t w = threadIdx.x; w < nw; w += FS_BLOCK) {
+                seen[e0 / 64 + w] = sw[w];
+                solid[e0 / 64 + w] = MULTI ? (uint64_t)sol[2 * w] | (uint64_t)sol[2 * w + 1] << 32 : 0ull;
+            }
+            __syncthreads();
+            continue;
+        }
         for (uint32_t e = threadIdx.x; e < FS_PART; e += FS_BLOCK) m[e] = ~0u;
         if (MULTI)
             for (uint32_t e = threadIdx.x; e < FS_PART / 32; e += FS_BLOCK) sol[e] = 0;
         __syncthreads();
-        const uint64_t b = bounds[part], end = bounds[part + 1];
         for (uint64_t i0 = b; i0 < end; i0 += (uint64_t)FS_BLOCK * U) {
             uint64_t it[U];
 #pragma unroll
@@ -1349,8 +1387,6 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_min(const uint64_t* __restrict_
             }
         }
         __syncthreads();
-        const uint64_t e0 = (uint64_t)part * FS_PART;
-        const uint32_t nw = (uint32_t)(min<uint64_t>(FS_PART, fs_n - e0) / 64);
         for (uint32_t w = wave; w < nw; w += FS_BLOCK / 64) {
             const uint32_t v = m[w * 64 + lane];
             fs[e0 + w * 64 + lane] = v;
@@ -2843,7 +2879,11 @@ struct Run {
         KC_TRY(c->d_seen.ensure(w0 * 8));
         KC_TRY(c->d_solid.ensure(w0 * 8));
         // first-setter groups: consecutive libraries whose fs arrays fit the budget together
-        const uint64_t budget = 64ull << 30;
+        // (with 32-bit events and sorted first setters, at most 2^31 entries, so that an
+        // item's entry leaves its top bit to the counted-twice mark: four libraries on
+        // 2^29-bit tiers)
+        const uint64_t budget = sizeof(Ev) == 4 && c->tune.setter_path == 0 ? (1ull << 31) * sizeof(Ev)
+                                                                             : 64ull << 30;
         c->groups.clear();
         uint64_t used = 0, maxg = 0;
         uint32_t gb = 0;
@@ -3975,6 +4015,16 @@ static hipError_t replace_memset_nodes(hipGraph_t g, uint32_t* replaced) {
 }
 
 static int graph_plan_and_capture(kc_ctx* ctx, kc_info* info) {
+    // The binned count's bins are sized from each library's distinct k-mers at this k; a
+    // context that has not counted them yet runs once to learn them, so that the plan the
+    // graph replays is the one an eager run at this size would take.
+    bool exact = true;
+    for (const HostLib& H : ctx->libs) exact = exact && H.est_k == ctx->o.k && H.est;
+    const bool binned = ctx->o.k <= 63 && (ctx->tune.insert_path == 0 || ctx->tune.insert_path == 3);
+    if (binned && !exact) {
+        const int rc0 = kc_run(ctx, info);
+        if (rc0) return rc0;
+    }
     ctx->graph_planning = true;
     int rc = kc_run(ctx, info);
     ctx->graph_planning = false;

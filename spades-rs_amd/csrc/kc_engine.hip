@@ -458,11 +458,11 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     while (2 * p <= w) p <<= 1;
     const uint64_t mmask = (1ull << (2 * m)) - 1;
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(data);
-    // A read's packed dwords sit one per lane (lane l: dword q0 + l, up to the last dword its
-    // m-mers read), so an m-mer's two words come by two shuffles; 64 dwords cover about 1000
-    // m-mers, and a longer read reloads them when a chunk passes the end.  The loop is
-    // software-pipelined over the wave's reads: the next read's dwords and the one after's
-    // (offset, first window, end window) are loaded while the current read is cut.
+    // The loop is software-pipelined over the wave's reads: the (offset, first window, end
+    // window) of the read after the next are loaded while the current read is cut, so a
+    // read's first packed words wait on one load, not on a chain of three.  (Holding a
+    // read's packed dwords one per lane and reading an m-mer's words by shuffles was 8%
+    // slower than these cached loads.)
     struct Meta {
         uint64_t off, ev, ev1;
     };
@@ -471,55 +471,29 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
         if (a < n_active) r = Meta{act_off[a], act_ev[a], act_ev[a + 1]};
         return r;
     };
-    // [q0, q_end): the dwords of the read's m-mers (each reads dwords q and q + 1, the last
-    // possibly in the 64-byte tail pad after the data)
-    auto dword_span = [&](const Meta& M, uint64_t& q0, uint64_t& q_end) {
-        const uint64_t b0 = M.off * 8;
-        const uint32_t nm = (uint32_t)(M.ev1 - M.ev) + w - 1;
-        q0 = b0 >> 5;
-        q_end = M.ev1 > M.ev ? ((b0 + 2ull * (nm - 1)) >> 5) + 2 : q0;
-    };
-    auto load_words = [&](uint64_t q0, uint64_t q_end) -> uint32_t {
-        return q0 + lane < q_end ? dw[q0 + lane] : 0u;
-    };
     uint64_t a = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
     Meta cur = load_meta(a);
-    uint64_t nq0, nqe;
-    dword_span(cur, nq0, nqe);
-    uint32_t nW = load_words(nq0, nqe);
     Meta nxt = load_meta(a + waves);
     for (; a < n_active; a += waves) {
         const uint64_t bit0 = cur.off * 8;
         const uint32_t nw = (uint32_t)(cur.ev1 - cur.ev);
         uint64_t* out = scratch + cur.ev;
-        uint32_t W = nW;
-        uint64_t q0 = nq0;
-        const uint64_t q_end = nqe;
-        dword_span(nxt, nq0, nqe);
-        nW = load_words(nq0, nqe);
         cur = nxt;
         nxt = load_meta(a + 2 * waves);
         const uint32_t nm = nw + w - 1;  // m-mers of the read
         auto key_at = [&](uint32_t i) -> uint32_t {
+            if (i >= nm) return 0xFFFFFFFFu;  // past the read: no valid window reaches it
             const uint64_t bp = bit0 + 2ull * i;
-            const uint32_t l0 = (uint32_t)((bp >> 5) - q0);
-            const uint32_t x0 = __shfl(W, l0 & 63), x1 = __shfl(W, (l0 + 1) & 63);
+            const uint64_t q = bp >> 5;
             const uint32_t sh = (uint32_t)(bp & 31);
-            const uint64_t win = ((uint64_t)__builtin_bswap32(x0) << 32) | __builtin_bswap32(x1);
+            const uint64_t win = ((uint64_t)__builtin_bswap32(dw[q]) << 32) | __builtin_bswap32(dw[q + 1]);
             const uint64_t f = (win >> (64 - sh - 2 * m)) & mmask;
             const uint64_t r = rev_groups(~f) >> (64 - 2 * m);
-            const uint32_t key = mmer_key(f < r ? f : r);
-            return i < nm ? key : 0xFFFFFFFFu;  // past the read: no valid window reaches it
+            return mmer_key(f < r ? f : r);
         };
         uint32_t open = 0, carry_bin = 0, total = 0;
         uint32_t K = key_at(lane);  // keys of positions c0 + lane
         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
-            // the chunk's m-mers c0 + 64 .. read dwords up to the last one's second word
-            const uint64_t q_hi = ((bit0 + 2ull * min(c0 + 127, nm - 1)) >> 5) + 1;
-            if (q_hi >= q0 + 64) {  // a long read: the dwords from this chunk on
-                q0 = (bit0 + 2ull * c0) >> 5;
-                W = load_words(q0, q_end);
-            }
             const bool valid = c0 + lane < nw;
             // A = positions c0 + lane, B = positions c0 + 64 + lane (the windows read B up to
             // lane w - 2; B is the next chunk's A)

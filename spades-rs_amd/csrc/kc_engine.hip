@@ -206,6 +206,8 @@ constexpr uint32_t PLAN_GUARD = 22;
 constexpr uint32_t BIN_QUEUE = 23;
 // Counter slot receiving the number of bins that hold records.
 constexpr uint32_t BIN_LIST_N = 24;
+// Counter slot of the count-table sort's runs: listed long runs, and a run too long.
+constexpr uint32_t TIE_CTR = 25;
 static_assert(sizeof(LibOut) % sizeof(unsigned long long) == 0, "LibOut layout");
 
 // ---------------------------------------------------------------------------
@@ -1887,6 +1889,107 @@ __global__ void k_gather(const T* __restrict__ src, const uint32_t* __restrict__
         dst[i] = src[idx[i]];
 }
 
+// Count-table order of two-word entries, most significant part first.  An entry's key is
+// hi * 2^lo_bits + lo (lo_bits + hi_bits > 64 bits); its top 64 bits are
+// hi << (64 - hi_bits) | lo >> r, r = lo_bits + hi_bits - 64, and the r bits below them are
+// lo's low bits.  The entries are sorted by their top 64 bits (one 64-bit radix sort), then
+// every run of equal top bits is put in order by its low bits.
+__global__ void k_msd_keys(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo, uint64_t n,
+                           uint32_t hi_bits, uint32_t r, uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        key[i] = hi[i] << (64 - hi_bits) | lo[i] >> r;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+constexpr uint32_t TIE_SHORT = 16;    // runs a thread orders by insertion
+constexpr uint32_t TIE_LONG = 1024;   // runs a workgroup orders in LDS; longer: the two-pass sort
+constexpr uint32_t TIE_LONG_CAP = 1u << 16;
+
+// Runs of equal top bits (keys sorted, idx the permutation): the run's head orders a short
+// run by insertion on the low bits lo[idx] & low_mask (distinct entries: no two equal),
+// lists a long one (start, length) for k_tie_long, and flags a run it cannot order.
+__global__ void k_tie_short(const uint64_t* __restrict__ key, uint32_t* __restrict__ idx,
+                            const uint64_t* __restrict__ lo, uint64_t n, uint64_t low_mask,
+                            uint2* __restrict__ longs, unsigned int* __restrict__ ctr) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i > 0 && key[i] == key[i - 1]) continue;
+        if (i + 1 >= n || key[i + 1] != key[i]) continue;  // a run of one
+        uint64_t j = i + 2;
+        while (j < n && j - i <= TIE_LONG && key[j] == key[i]) ++j;
+        const uint64_t len = j - i;
+        if (len > TIE_LONG) {
+            atomicOr(&ctr[1], 1u);  // too long: the caller sorts by both words instead
+            continue;
+        }
+        if (len > TIE_SHORT) {
+            const uint32_t at = atomicAdd(&ctr[0], 1u);
+            if (at < TIE_LONG_CAP) longs[at] = make_uint2((uint32_t)i, (uint32_t)len);
+            else atomicOr(&ctr[1], 1u);
+            continue;
+        }
+        uint64_t lv[TIE_SHORT];
+        uint32_t iv[TIE_SHORT];
+        for (uint32_t a = 0; a < len; ++a) {
+            const uint32_t x = idx[i + a];
+            const uint64_t v = lo[x] & low_mask;
+            uint32_t b = a;
+            for (; b > 0 && lv[b - 1] > v; --b) {
+                lv[b] = lv[b - 1];
+                iv[b] = iv[b - 1];
+            }
+            lv[b] = v;
+            iv[b] = x;
+        }
+        for (uint32_t a = 0; a < len; ++a) idx[i + a] = iv[a];
+    }
+}
+
+// One listed run per workgroup pass: a bitonic sort of (low bits, index) pairs in LDS.
+__global__ void __launch_bounds__(512) k_tie_long(uint32_t* __restrict__ idx, const uint64_t* __restrict__ lo,
+                                                  uint64_t low_mask, const uint2* __restrict__ longs,
+                                                  const unsigned int* __restrict__ ctr) {
+    __shared__ uint64_t sv[TIE_LONG];
+    __shared__ uint32_t si[TIE_LONG];
+    const uint32_t nl = min(ctr[0], TIE_LONG_CAP);
+    for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
+        const uint2 run = longs[r];
+        uint32_t np = 1;
+        while (np < run.y) np <<= 1;
+        for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) {
+            if (t < run.y) {
+                const uint32_t x = idx[run.x + t];
+                sv[t] = lo[x] & low_mask;
+                si[t] = x;
+            } else {
+                sv[t] = ~0ull;  // pads sort last (low bits are below 2^63)
+                si[t] = 0;
+            }
+        }
+        __syncthreads();
+        for (uint32_t size = 2; size <= np; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) {
+                    const uint32_t u = t ^ stride;
+                    if (u > t) {
+                        const bool up = (t & size) == 0;
+                        if ((sv[t] > sv[u]) == up) {
+                            const uint64_t tv = sv[t];
+                            sv[t] = sv[u];
+                            sv[u] = tv;
+                            const uint32_t ti = si[t];
+                            si[t] = si[u];
+                            si[u] = ti;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t t = threadIdx.x; t < run.y; t += blockDim.x) idx[run.x + t] = si[t];
+        __syncthreads();
+    }
+}
+
 // Per-library histogram of min(count, HIST_BINS - 1) over the sorted entries, for
 // determine_solid_kmer_cutoff (assemble.rs:692-735).  Entries are grouped by library, so
 // a block keeps the histogram of its first entry's library in LDS.
@@ -2208,6 +2311,7 @@ struct kc_ctx {
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
     DevBuf d_kept;  // binned count: a full LDS table's kept keys, one table's room per workgroup
     DevBuf d_bflag, d_bitems, d_blist, d_bgmeta;  // binned count: its work items (ranges of bins)
+    DevBuf d_ties;  // count-table sort: the long runs of equal top bits
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
     bool last_binned = false;     // the last Stage 2 left a packed (unhashed) table
     bool graph_planning = false;  // the eager run that plans a graph capture
@@ -3219,6 +3323,53 @@ struct Run {
             std::swap(c->d_cnt, c->d_cnt2);
             return KC_OK;
         }
+        // Top 64 bits first, then the runs of equal top bits by their low bits (one 64-bit
+        // sort instead of two), when no run is too long to order that way; a graph capture
+        // (no read-back of that verdict) and such runs take the two-word sort below, and so
+        // do tables over 2^27 entries (on the 1.38e9-entry metagenome table, with its many
+        // long runs of error k-mers, the first pass cost more than it saved: 261 -> 504 ms).
+        if (!c->capturing && !c->graph_planning && n <= (1ull << 27)) {
+            const uint32_t r = lay.lo_bits + lay.hi_bits - 64;
+            const uint64_t low_mask = (1ull << r) - 1;
+            KC_TRY(c->d_idx.ensure(n * 4));
+            KC_TRY(c->d_idx2.ensure(n * 4));
+            KC_TRY(c->d_ties.ensure((uint64_t)TIE_LONG_CAP * 8));
+            k_msd_keys<<<grid_for(n), BLOCK, 0, st>>>(c->d_hi.as<uint64_t>(), c->d_lo.as<uint64_t>(), n, lay.hi_bits,
+                                                       r, c->d_lo2.as<uint64_t>(), c->d_idx.as<uint32_t>());
+            KC_TRY(hipGetLastError());
+            size_t need = 0;
+            KC_TRY(rocprim::radix_sort_pairs(nullptr, need, c->d_lo2.as<uint64_t>(), c->d_hi2.as<uint64_t>(),
+                                             c->d_idx.as<uint32_t>(), c->d_idx2.as<uint32_t>(), (size_t)n, 0u, 64u,
+                                             st));
+            KC_TRY(c->d_temp.ensure(need));
+            tb = c->d_temp.bytes;
+            TIMED(KID_SORT, KC_TRY(rocprim::radix_sort_pairs(c->d_temp.p, tb, c->d_lo2.as<uint64_t>(),
+                                                              c->d_hi2.as<uint64_t>(), c->d_idx.as<uint32_t>(),
+                                                              c->d_idx2.as<uint32_t>(), (size_t)n, 0u, 64u, st)));
+            unsigned int* tctr = reinterpret_cast<unsigned int*>(c->d_counters.as<unsigned long long>() + TIE_CTR);
+            k_fill<unsigned int><<<1, 64, 0, st>>>(tctr, 2, 0u);
+            TIMED(KID_SORT, (k_tie_short<<<grid_for(n), BLOCK, 0, st>>>(
+                                c->d_hi2.as<uint64_t>(), c->d_idx2.as<uint32_t>(), c->d_lo.as<uint64_t>(), n, low_mask,
+                                c->d_ties.as<uint2>(), tctr)));
+            TIMED(KID_SORT, (k_tie_long<<<(uint32_t)c->n_cu * 2, 512, 0, st>>>(
+                                c->d_idx2.as<uint32_t>(), c->d_lo.as<uint64_t>(), low_mask, c->d_ties.as<uint2>(),
+                                tctr)));
+            KC_TRY(hipMemcpyAsync(c->h_stage, tctr, 8, hipMemcpyDeviceToHost, st));
+            KC_TRY(hipStreamSynchronize(st));
+            const uint32_t too_long = (uint32_t)(c->h_stage[0] >> 32);
+            if (!too_long) {
+                TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(
+                                    c->d_hi.as<uint64_t>(), c->d_idx2.as<uint32_t>(), c->d_lo2.as<uint64_t>(), n)));
+                TIMED(KID_SORT, (k_gather<uint64_t><<<grid_for(n), BLOCK, 0, st>>>(
+                                    c->d_lo.as<uint64_t>(), c->d_idx2.as<uint32_t>(), c->d_hi2.as<uint64_t>(), n)));
+                TIMED(KID_SORT, (k_gather<uint32_t><<<grid_for(n), BLOCK, 0, st>>>(
+                                    c->d_cnt.as<uint32_t>(), c->d_idx2.as<uint32_t>(), c->d_cnt2.as<uint32_t>(), n)));
+                std::swap(c->d_hi, c->d_lo2);
+                std::swap(c->d_lo, c->d_hi2);
+                std::swap(c->d_cnt, c->d_cnt2);
+                return KC_OK;
+            }
+        }
         // LSD over the two words; hi, lo, cnt stay intact until the final gathers.
         //   1. (lo, iota) sorted by lo        -> perm p1 in d_idx2 (sorted lo discarded)
         //   2. h1 = hi[p1]                    -> d_hi2
@@ -3700,7 +3851,7 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
                       &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
                       &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_kept, &ctx->d_bflag,
-                      &ctx->d_bitems, &ctx->d_blist, &ctx->d_bgmeta})
+                      &ctx->d_bitems, &ctx->d_blist, &ctx->d_bgmeta, &ctx->d_ties})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);

@@ -180,12 +180,13 @@ def test_spectrum_ragged_and_edge_reads(oracle):
         assert_parity(oracle, st, k, 1 << 16, ec=(k in (21, 65)))
 
 
-@pytest.mark.parametrize("k,run", [(33, 4), (55, 12), (55, 300), (63, 80)])
+@pytest.mark.parametrize("k,run", [(33, 4), (55, 12), (55, 300), (63, 80), (55, 1500)])
 def test_count_table_order_with_shared_prefixes(oracle, k, run):
     """Count-table entries that share their top 64 bits: k-mers A^32 + a varied tail (forward
-    strand canonical), each read twice so that it is Bloom-solid.  Runs of equal prefixes
-    up to 64 entries are ordered after the prefix sort; a longer run (300 at k = 55, 80 at
-    k = 63) makes the two-word sort redo the table.  Keys, counts and mask match the oracle."""
+    strand canonical), each read twice so that it is Bloom-solid.  After the sort by the top
+    64 bits, a run of equal tops up to 16 entries is ordered by one thread (4, 12), up to 1024
+    by a workgroup in LDS (300 at k = 55, 80 at k = 63), and a longer one (1500) makes the
+    two-word sort redo the table.  Keys, counts and mask match the oracle."""
     rng = np.random.default_rng(k * 1000 + run)
     st = PackedReads()
     tail = k - 32

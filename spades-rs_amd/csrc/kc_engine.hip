@@ -422,11 +422,11 @@ __device__ __forceinline__ bool lds_claim_loaded(uint64_t* w, uint64_t cur, uint
 // k-mer and its reverse complement, so all occurrences of a canonical k-mer share a bin.
 // A read is cut into super-k-mers, runs of consecutive windows in one bin, each an 8-byte
 // record (active read, first window, window count); the records are radix-sorted by bin.
-// One workgroup then counts a bin exactly in an LDS table (count and first event of
-// every distinct k-mer) and writes the distinct k-mers densely to the table pool with
-// plain stores: no global atomic per window, one per flushed pass.  A bin with too many
-// distinct k-mers for the LDS table is split by key-hash bits into passes (2, 4, ...)
-// that each count a disjoint subset.  The table then holds exactly the entries k_insert
+// A workgroup then counts a bin (or a range of a small library's bins) exactly in an LDS
+// table (count and first event of every distinct k-mer) and writes the distinct k-mers
+// densely to the table pool with plain stores: no global atomic per window, one per
+// flushed pass.  A bin with too many distinct k-mers for the LDS table is split by key-hash
+// bits into passes (2, 4, ...) that each count a disjoint subset.  The table then holds exactly the entries k_insert
 // leaves (every distinct key with its count and first event), packed at the start of the
 // library's region with no hash placement, which is all the Stage 2 / 3 scans need.
 
@@ -632,7 +632,8 @@ __global__ void k_bin_items(const uint64_t* __restrict__ bounds, uint32_t n_libs
     }
 }
 
-// One (library, bin) per workgroup pass (see above).  The bin's records are staged in LDS in chunks
+// One work item (a range of bins of one library) per workgroup pass (see above and in the
+// kernel).  The item's records are staged in LDS in chunks
 // with the prefix sum of their window counts, and the chunk's windows are dealt to the
 // threads in order: neighbouring lanes take neighbouring windows of one record (distinct
 // k-mers), rather than the same window of reads from the same locus, which would send
@@ -642,8 +643,8 @@ __global__ void k_bin_items(const uint64_t* __restrict__ bounds, uint32_t n_libs
 // entries go to out[cap0[l] ..],
 // reserved per pass with one atomicAdd on lcnt[l]; a pass that would pass capl[l] writes
 // nothing and sets ctr[2] (lcnt still totals the library's distinct k-mers).  ctr[1] is set
-// when a bin needs more than BIN3_MAX_SPLIT passes (the caller then counts with the table
-// insert instead).
+// when an item needs more than BIN3_MAX_SPLIT passes (the caller then counts with the
+// table insert instead).
 // The bin sort's onesweep passes take 10 bits each, so the 15-20 bits of (library, bin)
 // keys sort in two passes instead of three (and 16 items a thread: 1.60 -> 1.50 ms at
 // k = 21 against 12).  The first-setter items (64-bit keys, 15-bit

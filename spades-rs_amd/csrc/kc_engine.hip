@@ -734,8 +734,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                                                      unsigned long long* __restrict__ ctr,
                                                      unsigned int* __restrict__ queue,
                                                      const uint64_t* __restrict__ blist,
-                                                     const unsigned int* __restrict__ n_blist,
-                                                     Slot<nw_of(KW), Ev>* __restrict__ kept) {
+                                                     const unsigned int* __restrict__ n_blist) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
     __shared__ uint64_t s_k0[BIN3_CAP];
@@ -1000,15 +999,19 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                         __syncthreads();
                         break;
                     }
-                    Slot<NW, Ev>* __restrict__ keep = kept + (size_t)blockIdx.x * BIN3_CAP;
-                    for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
-                        Slot<NW, Ev> e;
-                        e.k[0] = s_k0[sl];
-                        if (NW == 2) e.k[NW - 1] = s_k1[sl];
-                        e.t0 = s_t0[sl];
-                        e.cnt = s_cnt[sl];
-                        if (e.k[0] != EMPTY_KEY && ((bin_slot_hash<NW>(e.k) >> 20) & parts)) e.k[0] = EMPTY_KEY;
-                        keep[sl] = e;
+                    // each thread holds its BIN3_CAP / BLOCK slots in registers across the clear
+                    constexpr uint32_t PER = BIN3_CAP / BLOCK;
+                    static_assert(BIN3_CAP % BLOCK == 0, "slots a thread keeps");
+                    Slot<NW, Ev> keep[PER];
+#pragma unroll
+                    for (uint32_t u = 0; u < PER; ++u) {
+                        const uint32_t sl = threadIdx.x + u * BLOCK;
+                        keep[u].k[0] = s_k0[sl];
+                        if (NW == 2) keep[u].k[NW - 1] = s_k1[sl];
+                        keep[u].t0 = s_t0[sl];
+                        keep[u].cnt = s_cnt[sl];
+                        if (keep[u].k[0] != EMPTY_KEY && ((bin_slot_hash<NW>(keep[u].k) >> 20) & parts))
+                            keep[u].k[0] = EMPTY_KEY;
                     }
                     __syncthreads();
                     clear_table();
@@ -1022,8 +1025,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                     parts *= 2;
                     __syncthreads();
                     // put the kept keys back (distinct keys: a slot is taken by the CAS of its first word)
-                    for (uint32_t sl = threadIdx.x; sl < BIN3_CAP; sl += blockDim.x) {
-                        const Slot<NW, Ev> e = keep[sl];
+#pragma unroll
+                    for (uint32_t u = 0; u < PER; ++u) {
+                        const Slot<NW, Ev> e = keep[u];
                         if (e.k[0] == EMPTY_KEY) continue;
                         uint32_t slot = bin_slot_hash<NW>(e.k) & (BIN3_CAP - 1);
                         // (at most BIN3_CAP keys were kept, so a free slot is always found)
@@ -2310,7 +2314,6 @@ struct kc_ctx {
     // minimizer-binned Stage 2 count: super-k-mer bin keys / records (sorted copies), bin
     // bounds, per-read counts and offsets
     DevBuf d_bk, d_bk2, d_br, d_br2, d_bbounds, d_skc, d_sko, d_pack, d_wbin, d_bcap, d_bcnt;  // d_wbin: cut records
-    DevBuf d_kept;  // binned count: a full LDS table's kept keys, one table's room per workgroup
     DevBuf d_bflag, d_bitems, d_blist, d_bgmeta;  // binned count: its work items (ranges of bins)
     DevBuf d_ties;  // count-table sort: the long runs of equal top bits
     DevBuf d_fsi, d_fsi2, d_fsb;  // sorted first setters: items, their sort, partition bounds
@@ -2816,7 +2819,6 @@ struct Run {
             else KC_TRY(c->d_pack.ensure(std::max<uint64_t>(cap0[nL], 1) * sizeof(S)));
             KC_TRY(c->d_bcap.ensure(nL * 16 + 16));
             KC_TRY(c->d_bcnt.ensure(nL * 8 + 8));
-            KC_TRY(c->d_kept.ensure((uint64_t)g * BIN3_CAP * sizeof(S)));
             if (!planned) {  // a replay finds the plan's room in d_bcap
                 KC_TRY(hipMemcpyAsync(c->d_bcap.p, cap0.data(), nL * 8, hipMemcpyHostToDevice, st));
                 KC_TRY(hipMemcpyAsync(c->d_bcap.as<uint64_t>() + nL, cap.data(), nL * 8, hipMemcpyHostToDevice, st));
@@ -2830,7 +2832,7 @@ struct Run {
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr,
-                                     queue, c->d_blist.as<uint64_t>(), n_blist, c->d_kept.as<S>())));
+                                     queue, c->d_blist.as<uint64_t>(), n_blist)));
             if (planned) {
                 k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
                     c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
@@ -3851,7 +3853,7 @@ void kc_destroy(kc_ctx* ctx) {
     for (DevBuf& b : ctx->d_gscratch) b.release();
     for (DevBuf* b : {&ctx->d_fsi, &ctx->d_fsi2, &ctx->d_fsb, &ctx->d_bk, &ctx->d_bk2, &ctx->d_br, &ctx->d_br2,
                       &ctx->d_bbounds, &ctx->d_skc, &ctx->d_sko,
-                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_kept, &ctx->d_bflag,
+                      &ctx->d_pack, &ctx->d_wbin, &ctx->d_bcap, &ctx->d_bcnt, &ctx->d_bflag,
                       &ctx->d_bitems, &ctx->d_blist, &ctx->d_bgmeta, &ctx->d_ties})
         b->release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);

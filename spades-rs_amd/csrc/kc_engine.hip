@@ -1030,8 +1030,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                         const Slot<NW, Ev> e = keep[u];
                         if (e.k[0] == EMPTY_KEY) continue;
                         uint32_t slot = bin_slot_hash<NW>(e.k) & (BIN3_CAP - 1);
-                        // (at most BIN3_CAP keys were kept, so a free slot is always found)
-                        for (uint32_t probe = 1; probe < BIN3_CAP &&
+                        // (at most BIN3_CAP keys were kept, so one of the BIN3_CAP slots tried
+                        // is free)
+                        for (uint32_t probe = 0; probe < BIN3_CAP &&
                                                  atomicCAS(reinterpret_cast<unsigned long long*>(&s_k0[slot]),
                                                            (unsigned long long)EMPTY_KEY,
                                                            (unsigned long long)e.k[0]) != EMPTY_KEY;

@@ -336,7 +336,7 @@ typedef struct kc_tune {
     uint32_t kmers_per_bin;      /* binned count: distinct k-mers a bin is sized for (0: 352;
                                     else 64..4096) */
     uint32_t minimizer_len;      /* binned count: m of the canonical m-mers whose minimum bins a
-                                    window (0: the default for k; else 1..16, at most k) */
+                                    window (0: the default for k; else 1..32, at most k) */
     double ms;                   /* kc_autotune: Stage 2 + 3 device time of the chosen setting */
 } kc_tune;
 

@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
     const uint32_t w = k - m + 1;
     uint32_t p = 1;  // largest power of two <= w
     while (2 * p <= w) p <<= 1;
-    const uint64_t mmask = (1ull << (2 * m)) - 1;
+    const uint64_t mmask = m >= 32 ? ~0ull : (1ull << (2 * m)) - 1;
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(data);
     // The loop is software-pipelined over the wave's reads: the (offset, first window, end
     // window) of the read after the next are loaded while the current read is cut, so a
@@ -489,7 +489,13 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
             const uint64_t q = bp >> 5;
             const uint32_t sh = (uint32_t)(bp & 31);
             const uint64_t win = ((uint64_t)__builtin_bswap32(dw[q]) << 32) | __builtin_bswap32(dw[q + 1]);
-            const uint64_t f = (win >> (64 - sh - 2 * m)) & mmask;
+            uint64_t f;
+            if (m <= 17) {  // sh + 2m <= 64: the m-mer lies in two dwords
+                f = (win >> (64 - sh - 2 * m)) & mmask;
+            } else {  // a third dword (inside the read, or the buffer's padding past it)
+                const uint64_t x = (win << sh) | ((uint64_t)__builtin_bswap32(dw[q + 2]) << sh >> 32);
+                f = x >> (64 - 2 * m);
+            }
             const uint64_t r = rev_groups(~f) >> (64 - 2 * m);
             return mmer_key(f < r ? f : r);
         };
@@ -4760,8 +4766,8 @@ int kc_tune_set(kc_ctx* ctx, const kc_tune* tune) {
     if (tune->setter_path > 1) return ctx->fail(KC_EINVAL, "setter_path must be 0 or 1");
     if (tune->kmers_per_bin && (tune->kmers_per_bin < 64 || tune->kmers_per_bin > 4096))
         return ctx->fail(KC_EINVAL, "kmers_per_bin must be 0 or in 64..4096");
-    if (tune->minimizer_len > 16 || (tune->minimizer_len && tune->minimizer_len > ctx->o.k))
-        return ctx->fail(KC_EINVAL, "minimizer_len must be 0 or in 1..16 and at most k");
+    if (tune->minimizer_len > 32 || (tune->minimizer_len && tune->minimizer_len > ctx->o.k))
+        return ctx->fail(KC_EINVAL, "minimizer_len must be 0 or in 1..32 and at most k");
     ctx->tune = *tune;
     ctx->built = ctx->counted = ctx->graph_ok = false;  // the table layout depends on the load factor
     return KC_OK;
@@ -4815,7 +4821,7 @@ int kc_autotune(kc_ctx* ctx, uint32_t reps, kc_tune* best) {
         dims.push_back({F_KPB, {192, 256, 352, 448, 640}});
         // minimizer lengths around the default for this k (0 = the default itself)
         std::vector<double> ms{0};
-        for (uint32_t m : {11u, 13u, 15u, 16u})
+        for (uint32_t m : {11u, 13u, 16u, 20u, 24u, 28u, 32u})
             if (m <= ctx->o.k && m != minimizer_len(ctx->o.k)) ms.push_back(m);
         dims.push_back({F_M, ms});
         dims.push_back({F_BPC, {0, 4, 8, 16}});

@@ -20,15 +20,21 @@ constexpr uint32_t BIN3_MAX_SPLIT = 1u << 12;  // passes per bin before giving u
 constexpr uint32_t SK_MAX_WINDOWS = 255;       // windows per record
 constexpr uint32_t SK_MAX_LEN = 1u << 24;      // read length the record can address
 
-// Minimizer length: the whole k-mer up to 11, then 11 (k <= 32) or 16.  A longer m for the
-// long k-mers shortens the super-k-mers and moves more of an error's k-mers out of the
-// genomic super-k-mer's bin, so fewer bins pass the LDS table's fill (5 Mb / 100x at k =
-// 55, 2^19 bins: the windows counted, split passes included, fall from 1.39 to 1.22 times
-// the windows, and the bin count from 8.4 to 7.0 ms; k = 33: 5.26 to 4.80 ms).
-__host__ __device__ constexpr uint32_t minimizer_len(uint32_t k) { return k <= 11 ? k : k <= 32 ? 11 : 16; }
+// Minimizer length: the whole k-mer up to 11, then 11 (k <= 32), then k - 17 (windows of
+// 18 m-mers) between 16 and 32.  A longer m for the long k-mers shortens the super-k-mers
+// and moves more of an error's k-mers out of the genomic super-k-mer's bin, so fewer bins
+// pass the LDS table's fill and are counted in split passes.  Measured on 5 Mb / 100x
+// (binned count, 2^18 bins): k = 55 at m = 16, 24, 28 and 32 took 5.95, 4.86, 4.46 and 4.16
+// ms (the bin sort 0.44 -> 0.62 ms for the shorter super-k-mers); k = 63 at 16 and 32, 6.53
+// and 4.47 ms; k = 45 at 16, 21 and 26, 5.01, 4.45 and 4.13 ms; k = 33 at 12, 14, 16, 18 and
+// 20, 5.10, 4.25, 4.11, 4.00 and 4.03 ms with the cut and bin sort slower from 18 on.  An
+// m-mer key takes 64 bits, so m stops at 32.
+__host__ __device__ constexpr uint32_t minimizer_len(uint32_t k) {
+    return k <= 11 ? k : k <= 32 ? 11 : k - 17 < 16 ? 16 : k - 17 > 32 ? 32 : k - 17;
+}
 
 // MurmurHash3's 32-bit finalizer (fmix32): a bijection of 32-bit values, two 32-bit
-// multiplies (a canonical m-mer, m <= 16, fits 32 bits).
+// multiplies.
 __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x85ebca6bu;
@@ -63,9 +69,11 @@ struct BaseReader {
     }
 };
 
-// The 32-bit order key of a canonical m-mer (its mix; distinct m-mers get distinct keys).
+// The 32-bit order key of a canonical m-mer: its mix, a bijection for m <= 16 (distinct
+// m-mers get distinct keys); for m > 16 the bits above the low 32 are folded in first (a
+// key shared by two m-mers only ties their order, which any key function may).
 __host__ __device__ __forceinline__ uint32_t mmer_key(uint64_t canonical_mmer) {
-    return mix32((uint32_t)canonical_mmer);
+    return mix32((uint32_t)canonical_mmer ^ (uint32_t)(canonical_mmer >> 32) * 0x9E3779B1u);
 }
 
 // The bin of a window from its minimizer's key (the smallest key of its m-mers).  The
@@ -95,7 +103,7 @@ template <class F>
 __host__ __device__ __forceinline__ void superkmers(const uint8_t* __restrict__ data, uint64_t off, uint32_t len, uint32_t k,
                                            uint32_t bits, F&& emit) {
     const uint32_t m = minimizer_len(k), w = k - m + 1, nwin = len - k + 1;
-    const uint64_t mask = (1ull << (2 * m)) - 1;
+    const uint64_t mask = m >= 32 ? ~0ull : (1ull << (2 * m)) - 1;
     const uint32_t top = 2 * (m - 1);
     BaseReader base(data, off);
     uint64_t f = 0, r = 0, minh = ~0ull;  // minh: a key, or ~0 before the first

@@ -307,7 +307,9 @@ class SpectrumEngine:
         Bloom first setters by sorting (entry, setter) items by partition and taking minima
         in LDS where it applies (dense mode, 32-bit events), else device atomics; 1 device
         atomics.  kmers_per_bin: distinct k-mers a bin of the binned count is sized for (0:
-        352).  Every setting gives the same results."""
+        352).  minimizer_len: m of the binned count's minimizers (0: the default for k, 11 up
+        to k = 32, then k - 17 between 16 and 32; else 1..32, at most k).  Every setting
+        gives the same results."""
         t = _abi.kc_tune(windows_per_thread=windows_per_thread, blocks_per_cu=blocks_per_cu,
                          table_load=float(table_load), insert_path=insert_path, setter_path=setter_path,
                          kmers_per_bin=kmers_per_bin, minimizer_len=minimizer_len)

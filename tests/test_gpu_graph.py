@@ -143,7 +143,7 @@ def test_autotune_on_the_binned_path(oracle):
     say("case autotune binned")
     best = eng.autotune(reps=1)
     assert best["ms"] > 0 and best["kmers_per_bin"] in (0, 192, 256, 352, 448, 640)
-    assert best["minimizer_len"] in (0, 11, 13, 15, 16)
+    assert best["minimizer_len"] in (0, 11, 13, 16, 20, 24, 28, 32)
     assert best["blocks_per_cu"] in (0, 4, 8, 16) and best["setter_path"] in (0, 1)
     assert eng.tune() == best
     eng.run()

@@ -560,13 +560,26 @@ __global__ void __launch_bounds__(BLOCK) k_sk_wave(const uint8_t* __restrict__ d
 // The records of each read from scratch to their packed place sk_off[a] ..: bin key
 // (library << bits | bin), and a | first window << 32 | windows << 56.  Sixteen lanes per
 // read (a read holds about 5-25 records), so a wave follows four reads' load chains at once.
+// With pk != 0 (the pk key bits, the read index, the first window and the window count fit
+// 64 bits: every read has at most 256 windows) a record is one word, a << (pk + 16) |
+// first window << (pk + 8) | windows << pk | key, sorted on its low pk bits as a key alone,
+// and keys is unused.
 constexpr uint32_t SK_COPY_LANES = 16;
+__host__ __device__ __forceinline__ uint32_t rec_read(uint64_t rec, uint32_t pk) {
+    return pk ? (uint32_t)(rec >> (pk + 16)) : (uint32_t)rec;
+}
+__host__ __device__ __forceinline__ uint32_t rec_first(uint64_t rec, uint32_t pk) {
+    return pk ? (uint32_t)(rec >> (pk + 8)) & 0xFFu : (uint32_t)(rec >> 32) & 0xFFFFFFu;
+}
+__host__ __device__ __forceinline__ uint32_t rec_windows(uint64_t rec, uint32_t pk) {
+    return pk ? (uint32_t)(rec >> pk) & 0xFFu : (uint32_t)(rec >> 56);
+}
 __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ act_ev,
                                                    const uint32_t* __restrict__ act_lib, uint64_t n_active,
                                                    uint32_t bits, const uint32_t* __restrict__ cnt,
                                                    const uint64_t* __restrict__ sk_off,
                                                    const uint64_t* __restrict__ scratch, uint32_t* __restrict__ keys,
-                                                   uint64_t* __restrict__ recs, uint64_t cap) {
+                                                   uint64_t* __restrict__ recs, uint64_t cap, uint32_t pk) {
     const uint32_t sub = threadIdx.x % SK_COPY_LANES;
     const uint64_t groups = (uint64_t)gridDim.x * (BLOCK / SK_COPY_LANES);
     for (uint64_t a = (uint64_t)blockIdx.x * (BLOCK / SK_COPY_LANES) + threadIdx.x / SK_COPY_LANES; a < n_active;
@@ -579,8 +592,13 @@ __global__ void __launch_bounds__(BLOCK) k_sk_copy(const uint64_t* __restrict__ 
         // nothing past it, and its guard sends the run back to planning)
         for (uint32_t t = sub; t < n && o + t < cap; t += SK_COPY_LANES) {
             const uint64_t r = src[t];
-            keys[o + t] = lkey | (uint32_t)(r & 0xFFFFFFu);
-            recs[o + t] = a | ((r >> 24) & 0xFFFFFFull) << 32 | (r >> 48) << 56;
+            const uint32_t key = lkey | (uint32_t)(r & 0xFFFFFFu);
+            if (pk) {
+                recs[o + t] = a << (pk + 16) | ((r >> 24) & 0xFFull) << (pk + 8) | (r >> 48) << pk | key;
+            } else {
+                keys[o + t] = key;
+                recs[o + t] = a | ((r >> 24) & 0xFFFFFFull) << 32 | (r >> 48) << 56;
+            }
         }
     }
 }
@@ -601,14 +619,16 @@ __global__ void k_plan_guard(const uint64_t* __restrict__ sko, const uint32_t* _
     }
 }
 
-// bounds[b] = first sorted record of bin b (b <= nbins).
-__global__ void k_bin_bounds(const uint32_t* __restrict__ keys, uint64_t n, uint32_t nbins,
-                             uint64_t* __restrict__ bounds) {
+// bounds[b] = first sorted record of bin b (b <= nbins): keys from the key array, or with
+// pk != 0 from the low pk bits of the one-word records.
+__global__ void k_bin_bounds(const uint32_t* __restrict__ keys, const uint64_t* __restrict__ recs, uint32_t pk,
+                             uint64_t n, uint32_t nbins, uint64_t* __restrict__ bounds) {
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nbins; b += gridDim.x * blockDim.x) {
         uint64_t lo = 0, hi = n;
         while (lo < hi) {
             const uint64_t mid = (lo + hi) >> 1;
-            if (keys[mid] < b) lo = mid + 1;
+            const uint32_t key = pk ? (uint32_t)(recs[mid] & ((1ull << pk) - 1)) : keys[mid];
+            if (key < b) lo = mid + 1;
             else hi = mid;
         }
         bounds[b] = lo;
@@ -740,7 +760,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                                                      unsigned long long* __restrict__ ctr,
                                                      unsigned int* __restrict__ queue,
                                                      const uint64_t* __restrict__ blist,
-                                                     const unsigned int* __restrict__ n_blist) {
+                                                     const unsigned int* __restrict__ n_blist, uint32_t pk) {
     constexpr int NW = nw_of(KW);
     static_assert(NW <= 2, "k <= 63");
     __shared__ uint64_t s_k0[BIN3_CAP];
@@ -799,10 +819,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
     // A bin's first chunk of records is staged in LDS before the bin starts: pos and ev as
     // the staging below writes them, and each record's windows in woff.
     auto prestage = [&](uint64_t rec, ulonglong2 oe, uint64_t ev0) {
-        const uint32_t first = (uint32_t)(rec >> 32) & 0xFFFFFFu;
+        const uint32_t first = rec_first(rec, pk);
         s_rec[tid].pos = oe.x * 4 + first;
         s_rec[tid].ev = (Ev)(oe.y + first - ev0);
-        s_rec[tid].woff = (uint32_t)(rec >> 56);
+        s_rec[tid].woff = rec_windows(rec, pk);
     };
     uint64_t c_rb, c_re;  // the current bin's records
     {
@@ -811,7 +831,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
         c_re = lane_u64(v, 1);
         if (tid < min<uint64_t>(BIN3_RCHUNK, c_re - c_rb)) {
             const uint64_t rec = recs[c_rb + tid];
-            prestage(rec, act_oe[(uint32_t)rec], libs[(uint32_t)item >> bits].ev0);
+            prestage(rec, act_oe[rec_read(rec, pk)], libs[(uint32_t)item >> bits].ev0);
         }
     }
     uint64_t n_v = load_range(nitem);
@@ -837,7 +857,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
             issue_next_records();
             pf_oe = true;
             if (tid == 0) nn_item = listed(q_next);
-            if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) n_oe = act_oe[(uint32_t)n_rec];
+            if (tid < min<uint64_t>(BIN3_RCHUNK, n_re - n_rb)) n_oe = act_oe[rec_read(n_rec, pk)];
         };
         if (rb == re) goto next_bin;
         {
@@ -877,9 +897,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KW =
                             n0 = s_rec[i].woff;
                         } else {
                             const uint64_t rec = recs[c0 + i];
-                            const uint32_t fw = (uint32_t)(rec >> 32) & 0xFFFFFFu;
-                            n0 = (uint32_t)(rec >> 56);
-                            const ulonglong2 oe = act_oe[(uint32_t)rec];
+                            const uint32_t fw = rec_first(rec, pk);
+                            n0 = rec_windows(rec, pk);
+                            const ulonglong2 oe = act_oe[rec_read(rec, pk)];
                             s_rec[i].pos = oe.x * 4 + fw;
                             s_rec[i].ev = (Ev)(oe.y + fw - ev0);
                         }
@@ -2700,6 +2720,17 @@ struct Run {
         if (lib_bits + bits > 31) return BIN_FALLBACK;
         const uint32_t nbins = nL << bits;
         unsigned long long* ctr = c->d_counters.as<unsigned long long>() + 12;
+        const uint32_t key_bits = lib_bits + bits;
+        // one-word records (sorted as keys alone: 8 bytes a record through the sort instead
+        // of 12) when the key, the read index and 8-bit first window and window count fit
+        uint32_t pk = 0;
+        {
+            uint32_t abits = 1, maxw = 0;
+            while ((1ull << abits) < na) ++abits;
+            for (uint32_t l = 0; l < nL; ++l)
+                if (c->libs[l].max_len >= k) maxw = std::max(maxw, c->libs[l].max_len - k + 1);
+            if (maxw <= 256 && key_bits + 16 + abits <= 64) pk = key_bits;
+        }
         uint64_t nrec = 0;
         {
             // records per read, their offsets, then the records in read order
@@ -2729,14 +2760,16 @@ struct Run {
                 KC_TRY(hipStreamSynchronize(st));
                 nrec = c->h_stage[0] + (uint32_t)c->h_stage[1];
             }
-            KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
-            KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
+            if (!pk) {
+                KC_TRY(c->d_bk.ensure(nrec * 4 + 4));
+                KC_TRY(c->d_bk2.ensure(nrec * 4 + 4));
+            }
             KC_TRY(c->d_br.ensure(nrec * 8 + 8));
             KC_TRY(c->d_br2.ensure(nrec * 8 + 8));
             TIMED(KID_SK, (k_sk_copy<<<gw, BLOCK, 0, st>>>(c->d_act_ev.as<uint64_t>(), c->d_act_lib.as<uint32_t>(), na,
                                                           bits, c->d_skc.as<uint32_t>(), c->d_sko.as<uint64_t>(),
                                                           c->d_wbin.as<uint64_t>(), c->d_bk.as<uint32_t>(),
-                                                          c->d_br.as<uint64_t>(), nrec)));
+                                                          c->d_br.as<uint64_t>(), nrec, pk)));
             // the cut's per-window scratch is kept for the next run unless the device is
             // short of room for what follows (metagenome-scale inputs)
             if (!planned && !c->graph_planning && c->d_wbin.bytes > BIG_SCRATCH && !c->fits(c->d_wbin.bytes)) {
@@ -2746,18 +2779,26 @@ struct Run {
         }
         KC_TRY(c->d_bbounds.ensure(((uint64_t)nbins + 1) * 8));
         size_t tb = 0;
-        const uint32_t key_bits = lib_bits + bits;
-        KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
-                                         c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
-                                         st));
-        KC_TRY(c->d_temp.ensure(tb));
-        tb = c->d_temp.bytes;
-        TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(c->d_temp.p, tb, c->d_bk.as<uint32_t>(),
-                                                              c->d_bk2.as<uint32_t>(), c->d_br.as<uint64_t>(),
-                                                              c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits,
-                                                              st)));
-        k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), nrec, nbins,
-                                                                     c->d_bbounds.as<uint64_t>());
+        if (pk) {
+            KC_TRY(rocprim::radix_sort_keys<BinSortConfig>(nullptr, tb, c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(),
+                                                           (size_t)nrec, 0, pk, st));
+            KC_TRY(c->d_temp.ensure(tb));
+            tb = c->d_temp.bytes;
+            TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_keys<BinSortConfig>(c->d_temp.p, tb, c->d_br.as<uint64_t>(),
+                                                                               c->d_br2.as<uint64_t>(), (size_t)nrec,
+                                                                               0, pk, st)));
+        } else {
+            KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(nullptr, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
+                                                            c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(),
+                                                            (size_t)nrec, 0, key_bits, st));
+            KC_TRY(c->d_temp.ensure(tb));
+            tb = c->d_temp.bytes;
+            TIMED(KID_BIN_SORT, KC_TRY(rocprim::radix_sort_pairs<BinSortConfig>(
+                                    c->d_temp.p, tb, c->d_bk.as<uint32_t>(), c->d_bk2.as<uint32_t>(),
+                                    c->d_br.as<uint64_t>(), c->d_br2.as<uint64_t>(), (size_t)nrec, 0, key_bits, st)));
+        }
+        k_bin_bounds<<<grid_for((uint64_t)nbins + 1), BLOCK, 0, st>>>(c->d_bk2.as<uint32_t>(), c->d_br2.as<uint64_t>(),
+                                                                     pk, nrec, nbins, c->d_bbounds.as<uint64_t>());
         KC_TRY(hipGetLastError());
         // Work items: the bins are sized for the largest library, so a library with far
         // fewer distinct k-mers counts 2^gsh of its bins at once (about as many k-mers as a
@@ -2839,7 +2880,7 @@ struct Run {
                                      c->d_libs.as<LibDesc>(), bits, nbins,
                                      direct ? c->d_tab.as<S>() : c->d_pack.as<S>(), c->d_bcap.as<uint64_t>(),
                                      c->d_bcap.as<uint64_t>() + nL, c->d_bcnt.as<unsigned long long>(), ctr,
-                                     queue, c->d_blist.as<uint64_t>(), n_blist)));
+                                     queue, c->d_blist.as<uint64_t>(), n_blist, pk)));
             if (planned) {
                 k_plan_guard<<<grid_for((uint64_t)nL + 1), BLOCK, 0, st>>>(
                     c->d_sko.as<uint64_t>(), c->d_skc.as<uint32_t>(), na, nrec, c->d_bcnt.as<unsigned long long>(),
